@@ -1,0 +1,1554 @@
+/*
+ * CPU oracle for the TPLS per-timestep stencil core -- TEST INFRASTRUCTURE.
+ * See tpls_oracle.h for the contract, the floating-point rules and how each
+ * routine is pinned.  Citations are file:line into Q-Leap-Networks/tpls.
+ *
+ * OpenMP is used only where the reference loop nest is free of races and
+ * order-independent, so results do not depend on the thread count; every
+ * reduction sums fixed per-plane partials in plane order.
+ */
+#include "tpls_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Fortran-shaped array views                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double *b;
+    long lx, ly, lz, nx, nxy;
+} F3;
+typedef struct {
+    int *b;
+    long lx, ly, lz, nx, nxy;
+} I3;
+
+static F3 f3(const double *b, long lx, long ux, long ly, long uy, long lz)
+{
+    F3 a;
+    a.b = (double *)b;
+    a.lx = lx; a.ly = ly; a.lz = lz;
+    a.nx = ux - lx + 1;
+    a.nxy = a.nx * (uy - ly + 1);
+    return a;
+}
+static I3 i3(const int *b, long lx, long ux, long ly, long uy, long lz)
+{
+    I3 a;
+    a.b = (int *)b;
+    a.lx = lx; a.ly = ly; a.lz = lz;
+    a.nx = ux - lx + 1;
+    a.nxy = a.nx * (uy - ly + 1);
+    return a;
+}
+#define A3(a, i, j, k) \
+    ((a).b[((long)(i) - (a).lx) + ((long)(j) - (a).ly) * (a).nx + ((long)(k) - (a).lz) * (a).nxy])
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+/* Single-precision literals widened to double, as the Fortran does:
+ * weno.f90:184 (epslon = 1.e-7), levelset.f90:268 / momentum_allflux.f90:489 /
+ * main_ns_hybrid.F90:181 (pi).  Both pi literals round to the same float. */
+#define EPSLON ((double)1.e-7f)
+#define PI_SP ((double)3.1415926535f)
+
+/* ------------------------------------------------------------------------ */
+/* Halo exchanges of a one-rank, y-periodic communicator (mpi.f90:110-662).   */
+/* N-S swap only: the E-W neighbours are MPI_PROC_NULL in x.                  */
+/* ------------------------------------------------------------------------ */
+/* width-w halo rows of an array whose own halo width is hw, rows swapped for
+ * i in [ilo, ihi]:  row sy-d <- row ey-d+1 ,  row ey+d <- row sy+d-1 */
+static void xchg_rows(F3 a, int ilo, int ihi, int sy, int ey, int d, int kz0, int kz1)
+{
+    for (int k = kz0; k <= kz1; ++k)
+        for (int i = ilo; i <= ihi; ++i) {
+            double lo = A3(a, i, ey - d + 1, k);
+            double hi = A3(a, i, sy + d - 1, k);
+            A3(a, i, sy - d, k) = lo;
+            A3(a, i, ey + d, k) = hi;
+        }
+}
+/* exchange2d, mpi.f90:110-170: rows sy-1/ey+1 for i in sx-1..ex+1 */
+static void exchange2d(double *u, int sx, int ex, int sy, int ey, int sz, int ez)
+{
+    F3 a = f3(u, sx - 1, ex + 1, sy - 1, ey + 1, sz);
+    xchg_rows(a, sx - 1, ex + 1, sy, ey, 1, sz, ez);
+}
+/* exchange2d_aug1 / aug2, mpi.f90:215-369 on (sx-2:ex+2, sy-2:ey+2) */
+static void exchange2d_aug12(double *u, int sx, int ex, int sy, int ey, int sz, int ez)
+{
+    F3 a = f3(u, sx - 2, ex + 2, sy - 2, ey + 2, sz);
+    xchg_rows(a, sx - 1, ex + 1, sy, ey, 1, sz, ez);
+    xchg_rows(a, sx - 2, ex + 2, sy, ey, 2, sz, ez);
+}
+/* exchange2d_augaug1/2/3, mpi.f90:415-662 on (sx-3:ex+3, sy-3:ey+3) */
+static void exchange2d_augaug123(double *u, int sx, int ex, int sy, int ey, int sz, int ez)
+{
+    F3 a = f3(u, sx - 3, ex + 3, sy - 3, ey + 3, sz);
+    xchg_rows(a, sx - 1, ex + 1, sy, ey, 1, sz, ez);
+    xchg_rows(a, sx - 2, ex + 2, sy, ey, 2, sz, ez);
+    xchg_rows(a, sx - 2, ex + 2, sy, ey, 3, sz, ez);
+}
+
+/* ------------------------------------------------------------------------ */
+/* WENO5 (weno.f90)                                                           */
+/* ------------------------------------------------------------------------ */
+double orc_weno5(double v1, double v2, double v3, double v4, double v5)
+{
+    double t1, t2, t3, s1, s2, s3, a1, a2, a3, w1, w2, w3;
+    t1 = v1 - 2.0 * v2 + v3;
+    t2 = v1 - 4.0 * v2 + 3.0 * v3;
+    s1 = 13.0 / 12.0 * t1 * t1 + 0.25 * t2 * t2;
+    t1 = v2 - 2.0 * v3 + v4;
+    t2 = v2 - v4;
+    s2 = 13.0 / 12.0 * t1 * t1 + 0.25 * t2 * t2;
+    t1 = v3 - 2.0 * v4 + v5;
+    t2 = 3.0 * v3 - 4.0 * v4 + v5;
+    s3 = 13.0 / 12.0 * t1 * t1 + 0.25 * t2 * t2;
+    a1 = 0.1 / ((EPSLON + s1) * (EPSLON + s1));
+    a2 = 0.6 / ((EPSLON + s2) * (EPSLON + s2));
+    a3 = 0.3 / ((EPSLON + s3) * (EPSLON + s3));
+    w1 = a1 / (a1 + a2 + a3);
+    w2 = a2 / (a1 + a2 + a3);
+    w3 = a3 / (a1 + a2 + a3);
+    t1 = v1 / 3.0 - 7.0 * v2 / 6.0 + 11.0 * v3 / 6.0;
+    t2 = -(v2 / 6.0) + 5.0 * v3 / 6.0 + v4 / 3.0;
+    t3 = v3 / 3.0 + 5.0 * v4 / 6.0 - v5 / 6.0;
+    return w1 * t1 + w2 * t2 + w3 * t3;
+}
+
+void orc_weno_appox_convection(const double *phi_, const double *u_, const double *v_,
+                               const double *w_, int sx, int ex, int sy, int ey, int l,
+                               int n, double *dfx_, double *dfy_, double *dfz_)
+{
+    F3 phi = f3(phi_, sx - 3, ex + 3, sy - 3, ey + 3, 0);
+    F3 u = f3(u_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v = f3(v_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w = f3(w_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfx = f3(dfx_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfy = f3(dfy_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfz = f3(dfz_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+
+    /* x faces: weno.f90:39-82 */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= n - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                if (i >= 3 && i <= l - 3) {
+                    double v1, v2, v3, v4, v5;
+                    if (A3(u, i, j, k) >= 0.0) {
+                        v1 = A3(phi, i - 2, j + 1, k + 1);
+                        v2 = A3(phi, i - 1, j + 1, k + 1);
+                        v3 = A3(phi, i, j + 1, k + 1);
+                        v4 = A3(phi, i + 1, j + 1, k + 1);
+                        v5 = A3(phi, i + 2, j + 1, k + 1);
+                    } else {
+                        v1 = A3(phi, i + 3, j + 1, k + 1);
+                        v2 = A3(phi, i + 2, j + 1, k + 1);
+                        v3 = A3(phi, i + 1, j + 1, k + 1);
+                        v4 = A3(phi, i, j + 1, k + 1);
+                        v5 = A3(phi, i - 1, j + 1, k + 1);
+                    }
+                    A3(dfx, i, j, k) = orc_weno5(v1, v2, v3, v4, v5);
+                } else {
+                    A3(dfx, i, j, k) = A3(u, i, j, k) >= 0.0 ? A3(phi, i, j + 1, k + 1)
+                                                             : A3(phi, i + 1, j + 1, k + 1);
+                }
+            }
+    /* inlet face: weno.f90:86-100 */
+    if (sx == 1)
+        for (int k = 0; k <= n - 2; ++k)
+            for (int j = sy; j <= ey; ++j)
+                A3(dfx, 0, j, k) = A3(u, 0, j, k) >= 0.0 ? A3(phi, 0, j + 1, k + 1)
+                                                         : A3(phi, 1, j + 1, k + 1);
+    /* y faces: weno.f90:104-133 */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= n - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double v1, v2, v3, v4, v5;
+                if (A3(v, i, j, k) >= 0.0) {
+                    v1 = A3(phi, i, j - 2, k + 1);
+                    v2 = A3(phi, i, j - 1, k + 1);
+                    v3 = A3(phi, i, j, k + 1);
+                    v4 = A3(phi, i, j + 1, k + 1);
+                    v5 = A3(phi, i, j + 2, k + 1);
+                } else {
+                    v1 = A3(phi, i, j + 3, k + 1);
+                    v2 = A3(phi, i, j + 2, k + 1);
+                    v3 = A3(phi, i, j + 1, k + 1);
+                    v4 = A3(phi, i, j, k + 1);
+                    v5 = A3(phi, i, j - 1, k + 1);
+                }
+                A3(dfy, i, j, k) = orc_weno5(v1, v2, v3, v4, v5);
+            }
+    /* z faces: weno.f90:135-167 */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= n - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                if (k >= 3 && k <= n - 3) {
+                    double v1, v2, v3, v4, v5;
+                    if (A3(w, i, j, k) >= 0.0) {
+                        v1 = A3(phi, i, j + 1, k - 2);
+                        v2 = A3(phi, i, j + 1, k - 1);
+                        v3 = A3(phi, i, j + 1, k);
+                        v4 = A3(phi, i, j + 1, k + 1);
+                        v5 = A3(phi, i, j + 1, k + 2);
+                    } else {
+                        v1 = A3(phi, i, j + 1, k + 3);
+                        v2 = A3(phi, i, j + 1, k + 2);
+                        v3 = A3(phi, i, j + 1, k + 1);
+                        v4 = A3(phi, i, j + 1, k);
+                        v5 = A3(phi, i, j + 1, k - 1);
+                    }
+                    A3(dfz, i, j, k) = orc_weno5(v1, v2, v3, v4, v5);
+                } else {
+                    A3(dfz, i, j, k) = (A3(phi, i, j + 1, k) + A3(phi, i, j + 1, k + 1)) / 2.0;
+                }
+            }
+}
+
+/* advect_phi.f90:19-119 */
+void orc_do_advect_phi(double *phi2_, double *u2_, double *v2_, double *w2_, double *conv_,
+                       int sx, int ex, int sy, int ey, int maxl, int maxn, int ex_max,
+                       double dx, double dy, double dz)
+{
+    long naug = (long)(ex - sx + 7) * (ey - sy + 7) * (maxn + 1);
+    long nuv = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn - 1);
+    long nw = (long)(ex - sx + 3) * (ey - sy + 3) * maxn;
+    double *aug_ = calloc(naug, sizeof(double));
+    double *dfx_ = calloc(nuv, sizeof(double));
+    double *dfy_ = calloc(nuv, sizeof(double));
+    double *dfz_ = calloc(nw, sizeof(double));
+    F3 aug = f3(aug_, sx - 3, ex + 3, sy - 3, ey + 3, 0);
+    F3 phi2 = f3(phi2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 u2 = f3(u2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v2 = f3(v2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w2 = f3(w2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfx = f3(dfx_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfy = f3(dfy_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dfz = f3(dfz_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 conv = f3(conv_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+
+    /* advect_phi.f90:58-67 */
+    for (int k = 0; k <= maxn; ++k)
+        for (int j = sy; j <= ey; ++j) {
+            for (int i = sx; i <= ex; ++i) A3(aug, i, j, k) = A3(phi2, i, j, k);
+            if (sx == 1) A3(aug, 0, j, k) = A3(phi2, 0, j, k);
+            if (ex == ex_max) A3(aug, ex_max + 1, j, k) = A3(phi2, ex_max + 1, j, k);
+        }
+    exchange2d_augaug123(aug_, sx, ex, sy, ey, 0, maxn);
+    orc_weno_appox_convection(aug_, u2_, v2_, w2_, sx, ex, sy, ey, maxl, maxn, dfx_, dfy_, dfz_);
+    exchange2d(dfx_, sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(dfy_, sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(dfz_, sx, ex, sy, ey, 0, maxn - 1);
+    exchange2d(u2_, sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(v2_, sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(w2_, sx, ex, sy, ey, 0, maxn - 1);
+    /* advect_phi.f90:324-335 */
+    memset(conv_, 0, sizeof(double) * (size_t)(ex - sx + 3) * (ey - sy + 3) * (maxn + 1));
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(conv, i, j, k) =
+                    (A3(u2, i, j - 1, k - 1) * A3(dfx, i, j - 1, k - 1) -
+                     A3(u2, i - 1, j - 1, k - 1) * A3(dfx, i - 1, j - 1, k - 1)) / dx +
+                    (A3(v2, i, j, k - 1) * A3(dfy, i, j, k - 1) -
+                     A3(v2, i, j - 1, k - 1) * A3(dfy, i, j - 1, k - 1)) / dy +
+                    (A3(w2, i, j - 1, k) * A3(dfz, i, j - 1, k) -
+                     A3(w2, i, j - 1, k - 1) * A3(dfz, i, j - 1, k - 1)) / dz;
+    free(aug_); free(dfx_); free(dfy_); free(dfz_);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Level set (levelset.f90, two_phase_levelset.f90)                           */
+/* ------------------------------------------------------------------------ */
+void orc_get_levelset_status(const double *phi_, double *D_, int *ist_, double *sgn_, int sx,
+                             int ex, int sy, int ey, int n, double h)
+{
+    F3 phi = f3(phi_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 D = f3(D_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 sgn = f3(sgn_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    I3 ist = i3(ist_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (n + 1);
+    memset(D_, 0, sizeof(double) * tot);
+    memset(sgn_, 0, sizeof(double) * tot);
+    memset(ist_, 0, sizeof(int) * tot);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= n - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double p = A3(phi, i, j, k);
+                A3(sgn, i, j, k) = p < 0.0 ? -1.0 : (p > 0.0 ? 1.0 : 0.0);
+                double a1 = p * A3(phi, i + 1, j, k);
+                double a2 = p * A3(phi, i - 1, j, k);
+                double a3 = p * A3(phi, i, j + 1, k);
+                double a4 = p * A3(phi, i, j - 1, k);
+                double a5 = p * A3(phi, i, j, k + 1);
+                double a6 = p * A3(phi, i, j, k - 1);
+                double mn = dmin(dmin(dmin(dmin(dmin(a1, a2), a3), a4), a5), a6);
+                A3(ist, i, j, k) = mn <= 0.0 ? 1 : 0;
+                if (A3(ist, i, j, k) == 1) {
+                    double gx = A3(phi, i + 1, j, k) - A3(phi, i - 1, j, k);
+                    double gy = A3(phi, i, j + 1, k) - A3(phi, i, j - 1, k);
+                    double gz = A3(phi, i, j, k + 1) - A3(phi, i, j, k - 1);
+                    double b1 = sqrt(gx * gx + gy * gy + gz * gz) / 2.0;
+                    double b2 = fabs(A3(phi, i + 1, j, k) - p);
+                    double b3 = fabs(A3(phi, i - 1, j, k) - p);
+                    double b4 = fabs(A3(phi, i, j + 1, k) - p);
+                    double b5 = fabs(A3(phi, i, j - 1, k) - p);
+                    double b6 = fabs(A3(phi, i, j, k + 1) - p);
+                    double b7 = fabs(A3(phi, i, j, k - 1) - p);
+                    double mx = dmax(dmax(dmax(dmax(dmax(dmax(dmax(b1, b2), b3), b4), b5), b6), b7), h);
+                    A3(D, i, j, k) = h * p / mx;
+                }
+            }
+}
+
+void orc_do_levelset_iteration(const double *phi_, double *pr_, const double *D_,
+                               const int *ist_, const double *sgn_, int sx, int ex, int sy,
+                               int ey, int n, int ex_max, double h, double height,
+                               double *error_val)
+{
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (n + 1);
+    double *pu_ = calloc(tot, sizeof(double));
+    F3 phi = f3(phi_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 pr = f3(pr_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 pu = f3(pu_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 D = f3(D_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 sgn = f3(sgn_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    I3 ist = i3(ist_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    double dtau = 0.3 * h;
+    /* levelset.f90:125-166 (dc is per-cell; the reference shares it across
+     * threads, which is a race -- sequential semantics are used here) */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= n - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double dc;
+                double c = A3(pr, i, j, k);
+                if (A3(ist, i, j, k) == 1) {
+                    dc = A3(sgn, i, j, k) * fabs(c) - A3(D, i, j, k);
+                } else {
+                    dc = 0.0;
+                    if (A3(phi, i, j, k) > 0.0) {
+                        double aP = dmax(c - A3(pr, i - 1, j, k), 0.0);
+                        double bM = dmin(A3(pr, i + 1, j, k) - c, 0.0);
+                        double cP = dmax(c - A3(pr, i, j - 1, k), 0.0);
+                        double dM = dmin(A3(pr, i, j + 1, k) - c, 0.0);
+                        double eP = dmax(c - A3(pr, i, j, k - 1), 0.0);
+                        double fM = dmin(A3(pr, i, j, k + 1) - c, 0.0);
+                        dc = A3(sgn, i, j, k) *
+                             (sqrt(dmax(aP * aP, bM * bM) + dmax(cP * cP, dM * dM) +
+                                   dmax(eP * eP, fM * fM)) - h);
+                    } else if (A3(phi, i, j, k) < 0.0) {
+                        double bP = dmax(A3(pr, i + 1, j, k) - c, 0.0);
+                        double aM = dmin(c - A3(pr, i - 1, j, k), 0.0);
+                        double dP = dmax(A3(pr, i, j + 1, k) - c, 0.0);
+                        double cM = dmin(c - A3(pr, i, j - 1, k), 0.0);
+                        double fP = dmax(A3(pr, i, j, k + 1) - c, 0.0);
+                        double eM = dmin(c - A3(pr, i, j, k - 1), 0.0);
+                        dc = A3(sgn, i, j, k) *
+                             (sqrt(dmax(bP * bP, aM * aM) + dmax(fP * fP, eM * eM) +
+                                   dmax(dP * dP, cM * cM)) - h);
+                    }
+                }
+                A3(pu, i, j, k) = c - dtau * dc / h;
+            }
+    /* levelset.f90:168-216 */
+    double dz = h;
+    if (sx == 1)
+        for (int k = 1; k <= n - 1; ++k) {
+            double z = k * dz - 0.5 * dz;
+            for (int j = sy; j <= ey; ++j) A3(pu, 0, j, k) = z - height;
+        }
+    if (ex == ex_max)
+        for (int k = 1; k <= n - 1; ++k)
+            for (int j = sy; j <= ey; ++j) A3(pu, ex_max + 1, j, k) = A3(pu, ex_max, j, k);
+    int i0 = sx == 1 ? 0 : sx, i1 = ex == ex_max ? ex_max + 1 : ex;
+    for (int j = sy; j <= ey; ++j)
+        for (int i = i0; i <= i1; ++i) {
+            A3(pu, i, j, 0) = 2.0 * A3(pu, i, j, 1) - A3(pu, i, j, 2);
+            A3(pu, i, j, n) = 2.0 * A3(pu, i, j, n - 1) - A3(pu, i, j, n - 2);
+        }
+    /* levelset.f90:218-233: the sum is reported (the reference's
+     * reduction(MAX:...) clause makes it thread-count dependent) */
+    double err = 0.0;
+    for (int k = 0; k <= n; ++k) {
+        double part = 0.0;
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double d = A3(pr, i, j, k) - A3(pu, i, j, k);
+                part += d * d;
+            }
+        err += part;
+    }
+    *error_val = err;
+    memcpy(pr_, pu_, sizeof(double) * tot);
+    free(pu_);
+}
+
+/* two_phase_levelset.f90:20-118 */
+void orc_do_tpls(double *phi2_, double *phi3_, double *c0_, double *c1_, double *c2_, int sx,
+                 int ex, int sy, int ey, int maxn, int ex_max, int max_it, double height,
+                 double *err2, double dx, double dz, double dt)
+{
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn + 1);
+    double *rhs_ = calloc(tot, sizeof(double));
+    double *D_ = calloc(tot, sizeof(double));
+    double *sg_ = calloc(tot, sizeof(double));
+    double *pr_ = calloc(tot, sizeof(double));
+    int *ist_ = calloc(tot, sizeof(int));
+    F3 phi2 = f3(phi2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 phi3 = f3(phi3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 rhs = f3(rhs_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    for (long q = 0; q < tot; ++q)
+        rhs_[q] = (23.0 / 12.0) * c2_[q] - (4.0 / 3.0) * c1_[q] + (5.0 / 12.0) * c0_[q];
+    exchange2d(rhs_, sx, ex, sy, ey, 0, maxn);
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(phi3, i, j, k) = A3(phi2, i, j, k) - dt * A3(rhs, i, j, k);
+    if (sx == 1)
+        for (int k = 1; k <= maxn - 1; ++k) {
+            double z_val = k * dz - 0.5 * dz;
+            for (int j = sy; j <= ey; ++j) A3(phi3, 0, j, k) = z_val - height;
+        }
+    if (ex == ex_max)
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j) A3(phi3, ex_max + 1, j, k) = A3(phi3, ex_max, j, k);
+    int i0 = sx == 1 ? 0 : sx, i1 = ex == ex_max ? ex_max + 1 : ex;
+    for (int j = sy; j <= ey; ++j)
+        for (int i = i0; i <= i1; ++i) {
+            A3(phi3, i, j, 0) = 2.0 * A3(phi3, i, j, 1) - A3(phi3, i, j, 2);
+            A3(phi3, i, j, maxn) = 2.0 * A3(phi3, i, j, maxn - 1) - A3(phi3, i, j, maxn - 2);
+        }
+    exchange2d(phi3_, sx, ex, sy, ey, 0, maxn);
+    orc_get_levelset_status(phi3_, D_, ist_, sg_, sx, ex, sy, ey, maxn, dx);
+    memcpy(pr_, phi3_, sizeof(double) * tot);
+    for (int it = 1; it <= max_it; ++it) {
+        exchange2d(pr_, sx, ex, sy, ey, 0, maxn);
+        orc_do_levelset_iteration(phi3_, pr_, D_, ist_, sg_, sx, ex, sy, ey, maxn, ex_max, dx,
+                                  height, err2);
+    }
+    memcpy(phi3_, pr_, sizeof(double) * tot);
+    free(rhs_); free(D_); free(sg_); free(pr_); free(ist_);
+}
+
+/* levelset.f90:241-414 */
+void orc_get_csf(double *fx_, double *fy_, double *fz_, const double *fi_, int sx, int ex,
+                 int sy, int ey, int n, int ex_max, double dx, double dy, double dz,
+                 double scap, double sw)
+{
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (n + 1);
+    double *cv_ = calloc(tot, sizeof(double));
+    F3 fi = f3(fi_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 fx = f3(fx_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 fy = f3(fy_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 fz = f3(fz_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 cv = f3(cv_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    /* The reference writes dx**2.d0, nrm_val**0.5d0 and nrm_val**1.5d0
+     * (levelset.f90:288-317); Fortran compilers evaluate the first two as
+     * dx*dx and sqrt(nrm) and the last through pow. */
+    double max_kappa = 1.0 / dmin(dmin(dx, dy), dz);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= n - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double c = A3(fi, i, j, k);
+                double d2x = (A3(fi, i + 1, j, k) - 2.0 * c + A3(fi, i - 1, j, k)) / (dx * dx);
+                double d2y = (A3(fi, i, j + 1, k) - 2.0 * c + A3(fi, i, j - 1, k)) / (dy * dy);
+                double d2z = (A3(fi, i, j, k + 1) - 2.0 * c + A3(fi, i, j, k - 1)) / (dz * dz);
+                double gx = (A3(fi, i + 1, j, k) - A3(fi, i - 1, j, k)) / (2.0 * dx);
+                double gy = (A3(fi, i, j + 1, k) - A3(fi, i, j - 1, k)) / (2.0 * dy);
+                double gz = (A3(fi, i, j, k + 1) - A3(fi, i, j, k - 1)) / (2.0 * dz);
+                double gxu = (A3(fi, i + 1, j, k + 1) - A3(fi, i - 1, j, k + 1)) / (2.0 * dx);
+                double gxd = (A3(fi, i + 1, j, k - 1) - A3(fi, i - 1, j, k - 1)) / (2.0 * dx);
+                double dxz = (gxu - gxd) / (2.0 * dz);
+                double gyu = (A3(fi, i, j + 1, k + 1) - A3(fi, i, j - 1, k + 1)) / (2.0 * dy);
+                double gyd = (A3(fi, i, j + 1, k - 1) - A3(fi, i, j - 1, k - 1)) / (2.0 * dy);
+                double dyz = (gyu - gyd) / (2.0 * dz);
+                double gxo = (A3(fi, i + 1, j + 1, k) - A3(fi, i - 1, j + 1, k)) / (2.0 * dx);
+                double gxi = (A3(fi, i + 1, j - 1, k) - A3(fi, i - 1, j - 1, k)) / (2.0 * dx);
+                double dxy = (gxo - gxi) / (2.0 * dy);
+                double nrm = gx * gx + gy * gy + gz * gz;
+                double kap;
+                if (nrm < 1e-10) {
+                    kap = 0.0;
+                } else {
+                    double t1 = (d2x + d2y + d2z) / sqrt(nrm);
+                    double t2 = gx * gx * d2x + gy * gy * d2y + gz * gz * d2z;
+                    double t3 = gx * gy * dxy + gx * gz * dxz + gy * gz * dyz;
+                    t2 = t2 / pow(nrm, 1.5);
+                    t3 = 2.0 * t3 / pow(nrm, 1.5);
+                    kap = -(t1 - t2 - t3);
+                }
+                if (kap > max_kappa) kap = max_kappa;
+                A3(cv, i, j, k) = kap;
+            }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= n - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double gx = (A3(fi, i + 1, j, k) - A3(fi, i - 1, j, k)) / (2.0 * dx);
+                double gy = (A3(fi, i, j + 1, k) - A3(fi, i, j - 1, k)) / (2.0 * dy);
+                double gz = (A3(fi, i, j, k + 1) - A3(fi, i, j, k - 1)) / (2.0 * dz);
+                double pv = A3(fi, i, j, k);
+                double dirac;
+                if (pv < -sw) dirac = 0.0;
+                else if (pv > sw) dirac = 0.0;
+                else dirac = (1.0 / (2.0 * sw)) + (1.0 / (2.0 * sw)) * cos(PI_SP * pv / sw);
+                A3(fx, i, j, k) = dirac * scap * A3(cv, i, j, k) * gx;
+                A3(fy, i, j, k) = dirac * scap * A3(cv, i, j, k) * gy;
+                A3(fz, i, j, k) = dirac * scap * A3(cv, i, j, k) * gz;
+            }
+    /* levelset.f90:365-374 z extrapolation, then 378-412 x extrapolation */
+    for (int j = sy; j <= ey; ++j)
+        for (int i = sx; i <= ex; ++i) {
+            A3(fx, i, j, 0) = 2.0 * A3(fx, i, j, 1) - A3(fx, i, j, 2);
+            A3(fx, i, j, n) = 2.0 * A3(fx, i, j, n - 1) - A3(fx, i, j, n - 2);
+            A3(fy, i, j, 0) = 2.0 * A3(fy, i, j, 1) - A3(fy, i, j, 2);
+            A3(fy, i, j, n) = 2.0 * A3(fy, i, j, n - 1) - A3(fy, i, j, n - 2);
+            A3(fz, i, j, 0) = 2.0 * A3(fz, i, j, 1) - A3(fz, i, j, 2);
+            A3(fz, i, j, n) = 2.0 * A3(fz, i, j, n - 1) - A3(fz, i, j, n - 2);
+        }
+    F3 *fs[3] = {&fx, &fy, &fz};
+    for (int c = 0; c < 3; ++c) {
+        F3 f = *fs[c];
+        for (int k = 0; k <= n; ++k)
+            for (int j = sy; j <= ey; ++j) {
+                if (sx == 1) A3(f, 0, j, k) = 2.0 * A3(f, 1, j, k) - A3(f, 2, j, k);
+                if (ex == ex_max)
+                    A3(f, ex_max + 1, j, k) = 2.0 * A3(f, ex_max, j, k) - A3(f, ex_max - 1, j, k);
+            }
+    }
+    free(cv_);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Momentum (momentum_allflux.f90)                                            */
+/* ------------------------------------------------------------------------ */
+/* smoothed Heaviside viscosity, momentum_allflux.f90:499-512 and
+ * main_ns_hybrid.F90:667-680 (same expression) */
+static double visc_of_phi(double pv, double sw, double Re, double mu_minus, double mu_plus)
+{
+    double H;
+    if (pv < -sw) H = 0.0;
+    else if (pv > sw) H = 1.0;
+    else H = 0.5 + (pv / (2.0 * sw)) + (1.0 / (2.0 * PI_SP)) * sin(PI_SP * pv / sw);
+    double temp = mu_plus * H + mu_minus * (1.0 - H);
+    return temp / Re;
+}
+
+void orc_get_viscosity(double *visc_, const double *phi_, int ex, int ey, int sx, int sy,
+                       int maxn, double sw, double Re, double mu_minus, double mu_plus)
+{
+    F3 visc = f3(visc_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 phi = f3(phi_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    for (int k = 0; k <= maxn; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(visc, i, j, k) = visc_of_phi(A3(phi, i, j, k), sw, Re, mu_minus, mu_plus);
+}
+
+void orc_get_diffusion_all(double *du_, double *dv_, double *dw_, const double *vs_,
+                           const double *u_, const double *v_, const double *w_, int sx, int sy,
+                           int ex, int ey, int maxn, int ex_max, double dx, double dy, double dz)
+{
+    F3 du = f3(du_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dv = f3(dv_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 dw = f3(dw_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    F3 u2 = f3(u_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v2 = f3(v_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w2 = f3(w_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    long nuv = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn - 1);
+    long nw = (long)(ex - sx + 3) * (ey - sy + 3) * maxn;
+    int ex_loc = ex == ex_max ? ex - 1 : ex;
+    memset(du_, 0, sizeof(double) * nuv);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex_loc; ++i) {
+                double um = k == 0 ? -2.0 * A3(u2, i, j, 0) + A3(u2, i, j, 1) / 3.0 : A3(u2, i, j, k - 1);
+                double up = k == maxn - 2 ? -2.0 * A3(u2, i, j, maxn - 2) + A3(u2, i, j, maxn - 3) / 3.0
+                                          : A3(u2, i, j, k + 1);
+                double mxp = A3(mu, i + 1, j + 1, k + 1);
+                double mxm = A3(mu, i, j + 1, k + 1);
+                double myp = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 2, k + 1) + A3(mu, i + 1, j + 2, k + 1)) / 4.0;
+                double mym = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i + 1, j, k + 1)) / 4.0;
+                double mzp = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 2) + A3(mu, i + 1, j + 1, k + 2)) / 4.0;
+                double mzm = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k) + A3(mu, i + 1, j + 1, k)) / 4.0;
+                double c = A3(u2, i, j, k);
+                double xE = mxp * ((A3(u2, i + 1, j, k) - c) / dx);
+                double xW = mxm * ((c - A3(u2, i - 1, j, k)) / dx);
+                double yN = myp * ((A3(u2, i, j + 1, k) - c) / dy);
+                double yS = mym * ((c - A3(u2, i, j - 1, k)) / dy);
+                double zO = mzp * ((up - c) / dz);
+                double zI = mzm * ((c - um) / dz);
+                A3(du, i, j, k) = ((xE - xW) / dx) + ((yN - yS) / dy) + ((zO - zI) / dz);
+            }
+    memset(dv_, 0, sizeof(double) * nuv);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double vm = k == 0 ? -2.0 * A3(v2, i, j, 0) + A3(v2, i, j, 1) / 3.0 : A3(v2, i, j, k - 1);
+                double vp = k == maxn - 2 ? -2.0 * A3(v2, i, j, maxn - 2) + A3(v2, i, j, maxn - 3) / 3.0
+                                          : A3(v2, i, j, k + 1);
+                double vmx = A3(v2, i - 1, j, k), vpx = A3(v2, i + 1, j, k);
+                double mxp = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i + 1, j, k + 1)) / 4.0;
+                double mxm = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i - 1, j + 1, k + 1) + A3(mu, i - 1, j, k + 1)) / 4.0;
+                double myp = A3(mu, i, j + 1, k + 1);
+                double mym = A3(mu, i, j, k + 1);
+                double mzp = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i, j + 1, k + 2) + A3(mu, i, j, k + 2)) / 4.0;
+                double mzm = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i, j + 1, k) + A3(mu, i, j, k)) / 4.0;
+                double c = A3(v2, i, j, k);
+                double xE = mxp * ((vpx - c) / dx);
+                double xW = mxm * ((c - vmx) / dx);
+                double yN = myp * ((A3(v2, i, j + 1, k) - c) / dy);
+                double yS = mym * ((c - A3(v2, i, j - 1, k)) / dy);
+                double zO = mzp * ((vp - c) / dz);
+                double zI = mzm * ((c - vm) / dz);
+                A3(dv, i, j, k) = ((xE - xW) / dx) + ((yN - yS) / dy) + ((zO - zI) / dz);
+            }
+    memset(dw_, 0, sizeof(double) * nw);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double wmx = A3(w2, i - 1, j, k), wpx = A3(w2, i + 1, j, k);
+                double mxp = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k) + A3(mu, i + 1, j + 1, k + 1)) / 4.0;
+                double mxm = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i - 1, j + 1, k) + A3(mu, i - 1, j + 1, k + 1)) / 4.0;
+                double myp = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 2, k) + A3(mu, i, j + 2, k + 1)) / 4.0;
+                double mym = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k) + A3(mu, i, j, k + 1)) / 4.0;
+                double mzp = A3(mu, i, j + 1, k + 1);
+                double mzm = A3(mu, i, j + 1, k);
+                double c = A3(w2, i, j, k);
+                double xE = mxp * ((wpx - c) / dx);
+                double xW = mxm * ((c - wmx) / dx);
+                double yN = myp * ((A3(w2, i, j + 1, k) - c) / dy);
+                double yS = mym * ((c - A3(w2, i, j - 1, k)) / dy);
+                double zO = mzp * ((A3(w2, i, j, k + 1) - c) / dz);
+                double zI = mzm * ((c - A3(w2, i, j, k - 1)) / dz);
+                A3(dw, i, j, k) = ((xE - xW) / dx) + ((yN - yS) / dy) + ((zO - zI) / dz);
+            }
+}
+
+void orc_get_conv_all(double *cu_, double *cv_, double *cw_, const double *vs_, const double *u_,
+                      const double *v_, const double *w_, int sx, int sy, int ex, int ey,
+                      int maxn, int ex_max, double dx, double dy, double dz)
+{
+    F3 cu = f3(cu_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 cv = f3(cv_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 cw = f3(cw_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    F3 u2 = f3(u_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v2 = f3(v_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w2 = f3(w_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    long nuv = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn - 1);
+    long nw = (long)(ex - sx + 3) * (ey - sy + 3) * maxn;
+    int ex_loc = ex == ex_max ? ex - 1 : ex;
+    /* x direction: momentum_allflux.f90:260-341 */
+    memset(cu_, 0, sizeof(double) * nuv);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex_loc; ++i) {
+                double tv = (A3(v2, i, j, k) + A3(v2, i + 1, j, k) + A3(v2, i, j + 1, k) + A3(v2, i + 1, j + 1, k)) / 4.0;
+                double tw = (A3(w2, i, j, k) + A3(w2, i + 1, j, k) + A3(w2, i, j, k + 1) + A3(w2, i + 1, j, k + 1)) / 4.0;
+                double dudx = 0.5 * (A3(u2, i + 1, j, k) - A3(u2, i - 1, j, k)) / dx;
+                double Xc = dudx * A3(u2, i, j, k);
+                double dudy = 0.5 * (A3(u2, i, j + 1, k) - A3(u2, i, j - 1, k)) / dy;
+                double Yc = dudy * tv;
+                double dudz;
+                if (k == 0) dudz = (A3(u2, i, j, k) + A3(u2, i, j, k + 1)) / (2.0 * dz);
+                else if (k == maxn - 2) dudz = -(A3(u2, i, j, k) + A3(u2, i, j, k - 1)) / (2.0 * dz);
+                else dudz = 0.5 * (A3(u2, i, j, k + 1) - A3(u2, i, j, k - 1)) / dz;
+                double Zc = dudz * tw;
+                double c1 = Xc + Yc + Zc;
+                double mxp = A3(mu, i + 1, j + 1, k + 1);
+                double mxm = A3(mu, i, j + 1, k + 1);
+                double myp = (A3(mu, i + 1, j + 2, k + 1) + A3(mu, i, j + 2, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 1)) / 4.0;
+                double mym = (A3(mu, i + 1, j, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 1)) / 4.0;
+                double mzp = (A3(mu, i + 1, j + 1, k + 2) + A3(mu, i, j + 1, k + 2) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 1)) / 4.0;
+                double mzm = (A3(mu, i + 1, j + 1, k) + A3(mu, i, j + 1, k) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 1)) / 4.0;
+                double aE = mxp * (A3(u2, i + 1, j, k) - A3(u2, i, j, k)) / dx;
+                double aW = mxm * (A3(u2, i, j, k) - A3(u2, i - 1, j, k)) / dx;
+                double bN = myp * (A3(v2, i + 1, j + 1, k) - A3(v2, i, j + 1, k)) / dx;
+                double bS = mym * (A3(v2, i + 1, j, k) - A3(v2, i, j, k)) / dx;
+                double cO = mzp * (A3(w2, i + 1, j, k + 1) - A3(w2, i, j, k + 1)) / dx;
+                double cI = mzm * (A3(w2, i + 1, j, k) - A3(w2, i, j, k)) / dx;
+                double c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+                A3(cu, i, j, k) = c1 - c2;
+            }
+    /* y direction: momentum_allflux.f90:345-406 */
+    memset(cv_, 0, sizeof(double) * nuv);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double tu = (A3(u2, i - 1, j - 1, k) + A3(u2, i - 1, j, k) + A3(u2, i, j - 1, k) + A3(u2, i, j, k)) / 4.0;
+                double tw = (A3(w2, i, j - 1, k) + A3(w2, i, j, k) + A3(w2, i, j - 1, k + 1) + A3(w2, i, j, k + 1)) / 4.0;
+                double dvdy = 0.5 * (A3(v2, i, j + 1, k) - A3(v2, i, j - 1, k)) / dy;
+                double Yc = dvdy * A3(v2, i, j, k);
+                double dvdx = 0.5 * (A3(v2, i + 1, j, k) - A3(v2, i - 1, j, k)) / dx;
+                double Xc = dvdx * tu;
+                double dvdz;
+                if (k == 0) dvdz = (A3(v2, i, j, k) + A3(v2, i, j, k + 1)) / (2.0 * dz);
+                else if (k == maxn - 2) dvdz = -(A3(v2, i, j, k) + A3(v2, i, j, k - 1)) / (2.0 * dz);
+                else dvdz = 0.5 * (A3(v2, i, j, k + 1) - A3(v2, i, j, k - 1)) / dz;
+                double Zc = dvdz * tw;
+                double c1 = Xc + Yc + Zc;
+                double mxp = (A3(mu, i + 1, j + 1, k + 1) + A3(mu, i + 1, j, k + 1) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1)) / 4.0;
+                double mxm = (A3(mu, i - 1, j + 1, k + 1) + A3(mu, i - 1, j, k + 1) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1)) / 4.0;
+                double myp = A3(mu, i, j + 1, k + 1);
+                double mym = A3(mu, i, j, k + 1);
+                double mzp = (A3(mu, i, j + 1, k + 2) + A3(mu, i, j, k + 2) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1)) / 4.0;
+                double mzm = (A3(mu, i, j + 1, k) + A3(mu, i, j, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1)) / 4.0;
+                double aE = mxp * (A3(u2, i, j, k) - A3(u2, i, j - 1, k)) / dy;
+                double aW = mxm * (A3(u2, i - 1, j, k) - A3(u2, i - 1, j - 1, k)) / dy;
+                double bN = myp * (A3(v2, i, j + 1, k) - A3(v2, i, j, k)) / dy;
+                double bS = mym * (A3(v2, i, j, k) - A3(v2, i, j - 1, k)) / dy;
+                double cO = mzp * (A3(w2, i, j, k + 1) - A3(w2, i, j - 1, k + 1)) / dy;
+                double cI = mzm * (A3(w2, i, j, k) - A3(w2, i, j - 1, k)) / dy;
+                double c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+                A3(cv, i, j, k) = c1 - c2;
+            }
+    /* z direction: momentum_allflux.f90:410-467 */
+    memset(cw_, 0, sizeof(double) * nw);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double tu = (A3(u2, i - 1, j, k - 1) + A3(u2, i - 1, j, k) + A3(u2, i, j, k - 1) + A3(u2, i, j, k)) / 4.0;
+                double tv = (A3(v2, i, j, k - 1) + A3(v2, i, j, k) + A3(v2, i, j + 1, k - 1) + A3(v2, i, j + 1, k)) / 4.0;
+                double dwdz = 0.5 * (A3(w2, i, j, k + 1) - A3(w2, i, j, k - 1)) / dz;
+                double Zc = dwdz * A3(w2, i, j, k);
+                double dwdx = 0.5 * (A3(w2, i + 1, j, k) - A3(w2, i - 1, j, k)) / dx;
+                double Xc = dwdx * tu;
+                double dwdy = 0.5 * (A3(w2, i, j + 1, k) - A3(w2, i, j - 1, k)) / dy;
+                double Yc = dwdy * tv;
+                double c1 = Xc + Yc + Zc;
+                double mxp = (A3(mu, i + 1, j + 1, k + 1) + A3(mu, i + 1, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 1, k)) / 4.0;
+                double mxm = (A3(mu, i - 1, j + 1, k + 1) + A3(mu, i - 1, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 1, k)) / 4.0;
+                double myp = (A3(mu, i, j + 2, k + 1) + A3(mu, i, j + 2, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 1, k)) / 4.0;
+                double mym = (A3(mu, i, j, k + 1) + A3(mu, i, j, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 1, k)) / 4.0;
+                double mzp = A3(mu, i, j + 1, k + 1);
+                double mzm = A3(mu, i, j + 1, k);
+                double aE = mxp * (A3(u2, i, j, k) - A3(u2, i, j, k - 1)) / dz;
+                double aW = mxm * (A3(u2, i - 1, j, k) - A3(u2, i - 1, j, k - 1)) / dz;
+                double bN = myp * (A3(v2, i, j + 1, k) - A3(v2, i, j + 1, k - 1)) / dz;
+                double bS = mym * (A3(v2, i, j, k) - A3(v2, i, j, k - 1)) / dz;
+                double cO = mzp * (A3(w2, i, j, k + 1) - A3(w2, i, j, k)) / dz;
+                double cI = mzm * (A3(w2, i, j, k) - A3(w2, i, j, k - 1)) / dz;
+                double c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+                A3(cw, i, j, k) = c1 - c2;
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Helmholtz sweeps (jacobi_iteration_allflux.f90, sor_iteration_allflux.f90) */
+/* ------------------------------------------------------------------------ */
+/* Face viscosities of the three staggered grids; the index patterns are those
+ * of jacobi_iteration_allflux.f90:75-82 (u), 153-160 (v), 231-238 (w), which
+ * sor_iteration_allflux.f90 repeats. */
+typedef struct { double xp, xm, yp, ym, zp, zm; } MuF;
+static MuF mu_u(F3 mu, int i, int j, int k)
+{
+    MuF m;
+    m.xp = A3(mu, i + 1, j + 1, k + 1);
+    m.xm = A3(mu, i, j + 1, k + 1);
+    m.yp = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 2, k + 1) + A3(mu, i + 1, j + 2, k + 1)) / 4.0;
+    m.ym = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i + 1, j, k + 1)) / 4.0;
+    m.zp = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k + 2) + A3(mu, i + 1, j + 1, k + 2)) / 4.0;
+    m.zm = (A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i, j + 1, k) + A3(mu, i + 1, j + 1, k)) / 4.0;
+    return m;
+}
+static MuF mu_v(F3 mu, int i, int j, int k)
+{
+    MuF m;
+    m.xp = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i + 1, j + 1, k + 1) + A3(mu, i + 1, j, k + 1)) / 4.0;
+    m.xm = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i - 1, j + 1, k + 1) + A3(mu, i - 1, j, k + 1)) / 4.0;
+    m.yp = A3(mu, i, j + 1, k + 1);
+    m.ym = A3(mu, i, j, k + 1);
+    m.zp = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i, j + 1, k + 2) + A3(mu, i, j, k + 2)) / 4.0;
+    m.zm = (A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k + 1) + A3(mu, i, j + 1, k) + A3(mu, i, j, k)) / 4.0;
+    return m;
+}
+static MuF mu_w(F3 mu, int i, int j, int k)
+{
+    MuF m;
+    m.xp = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i + 1, j + 1, k) + A3(mu, i + 1, j + 1, k + 1)) / 4.0;
+    m.xm = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i - 1, j + 1, k) + A3(mu, i - 1, j + 1, k + 1)) / 4.0;
+    m.yp = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j + 2, k) + A3(mu, i, j + 2, k + 1)) / 4.0;
+    m.ym = (A3(mu, i, j + 1, k) + A3(mu, i, j + 1, k + 1) + A3(mu, i, j, k) + A3(mu, i, j, k + 1)) / 4.0;
+    m.zp = A3(mu, i, j + 1, k + 1);
+    m.zm = A3(mu, i, j + 1, k);
+    return m;
+}
+/* (1/diag)*(...) of jacobi_iteration_allflux.f90:84-90: the six neighbour
+ * values are passed in the reference's order x+, x-, y+, y-, z+, z- */
+static double helm(MuF m, double ax, double ay, double az, double xp, double xm, double yp,
+                   double ym, double zp, double zm, double rhs)
+{
+    double diag = 1.0 + (ax / 2.0) * (m.xp + m.xm) + (ay / 2.0) * (m.yp + m.ym) + (az / 2.0) * (m.zp + m.zm);
+    return (1.0 / diag) * ((ax / 2.0) * (m.xp * xp + m.xm * xm) + (ay / 2.0) * (m.yp * yp + m.ym * ym) +
+                           (az / 2.0) * (m.zp * zp + m.zm * zm) + rhs);
+}
+
+void orc_do_jacobi_u(double *u3_, const double *uo_, const double *R_, const double *vs_,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max)
+{
+    F3 u3 = f3(u3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 uo = f3(uo_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+    int ex_loc = ex == ex_max ? ex - 1 : ex;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex_loc; ++i) {
+                double um = k == 0 ? -2.0 * A3(uo, i, j, 0) + A3(uo, i, j, 1) / 3.0 : A3(uo, i, j, k - 1);
+                double up = k == maxn - 2 ? -2.0 * A3(uo, i, j, maxn - 2) + A3(uo, i, j, maxn - 3) / 3.0
+                                          : A3(uo, i, j, k + 1);
+                A3(u3, i, j, k) = helm(mu_u(mu, i, j, k), ax, ay, az, A3(uo, i + 1, j, k),
+                                       A3(uo, i - 1, j, k), A3(uo, i, j + 1, k), A3(uo, i, j - 1, k),
+                                       up, um, A3(R, i, j, k));
+            }
+}
+
+void orc_do_jacobi_v(double *v3_, const double *vo_, const double *R_, const double *vs_,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max)
+{
+    F3 v3 = f3(v3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 vo = f3(vo_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double vm = k == 0 ? -2.0 * A3(vo, i, j, 0) + A3(vo, i, j, 1) / 3.0 : A3(vo, i, j, k - 1);
+                double vp = k == maxn - 2 ? -2.0 * A3(vo, i, j, maxn - 2) + A3(vo, i, j, maxn - 3) / 3.0
+                                          : A3(vo, i, j, k + 1);
+                A3(v3, i, j, k) = helm(mu_v(mu, i, j, k), ax, ay, az, A3(vo, i + 1, j, k),
+                                       A3(vo, i - 1, j, k), A3(vo, i, j + 1, k), A3(vo, i, j - 1, k),
+                                       vp, vm, A3(R, i, j, k));
+            }
+    /* jacobi_iteration_allflux.f90:178-189 */
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j) {
+            if (sx == 1) A3(v3, 0, j, k) = A3(v3, 1, j, k);
+            if (ex == ex_max) A3(v3, ex_max + 1, j, k) = A3(v3, ex_max, j, k);
+        }
+}
+
+void orc_do_jacobi_w(double *w3_, const double *wo_, const double *R_, const double *vs_,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max)
+{
+    F3 w3 = f3(w3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 wo = f3(wo_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(w3, i, j, k) = helm(mu_w(mu, i, j, k), ax, ay, az, A3(wo, i + 1, j, k),
+                                       A3(wo, i - 1, j, k), A3(wo, i, j + 1, k), A3(wo, i, j - 1, k),
+                                       A3(wo, i, j, k + 1), A3(wo, i, j, k - 1), A3(R, i, j, k));
+    /* jacobi_iteration_allflux.f90:257-279 */
+    for (int j = sy; j <= ey; ++j)
+        for (int i = sx; i <= ex; ++i) {
+            A3(w3, i, j, 0) = 0.0;
+            A3(w3, i, j, maxn - 1) = 0.0;
+        }
+    if (sx == 1) {
+        for (int k = 1; k <= maxn - 2; ++k)
+            for (int j = sy; j <= ey; ++j) A3(w3, 0, j, k) = A3(w3, 1, j, k);
+        for (int j = sy; j <= ey; ++j) {
+            A3(w3, 0, j, 0) = 0.0;
+            A3(w3, 0, j, maxn - 1) = 0.0;
+        }
+    }
+    if (ex == ex_max)
+        for (int k = 0; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j) A3(w3, ex_max + 1, j, k) = A3(w3, ex_max, j, k);
+}
+
+double orc_get_difference(const double *a_, const double *b_, int ex, int ey, int ez, int sx,
+                          int sy, int sz)
+{
+    F3 a = f3(a_, sx - 1, ex + 1, sy - 1, ey + 1, sz);
+    F3 b = f3(b_, sx - 1, ex + 1, sy - 1, ey + 1, sz);
+    double diff = 0.0;
+    for (int k = sz; k <= ez; ++k) {
+        double part = 0.0;
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double d = A3(a, i, j, k) - A3(b, i, j, k);
+                part += d * d;
+            }
+        diff += part;
+    }
+    return diff;
+}
+
+/* Red-black over-relaxation.  Even iteration_time: 3-D colouring
+ * (i+j+k) = sx+flg (mod 2), all cells of a colour independent.  Odd: 2-D
+ * colouring (i+j) = sx+flg (mod 2) with k ascending, i.e. Gauss-Seidel up each
+ * column (sor_iteration_allflux.f90:100-146 with the k race resolved). */
+#define RELAX 1.2
+void orc_do_sor_u(double *u3_, const double *R_, const double *vs_, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max,
+                  int flg, int iteration_time)
+{
+    F3 u3 = f3(u3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+    int ex_loc = ex == ex_max ? ex - 1 : ex;
+    int even = (iteration_time % 2) == 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int j = sy; j <= ey; ++j)
+        for (int k = 0; k <= maxn - 2; ++k) {
+            int i0 = (even ? ((k + j + flg) % 2) : ((j + flg) % 2)) + sx;
+            for (int i = i0; i <= ex_loc; i += 2) {
+                double um = k == 0 ? -2.0 * A3(u3, i, j, 0) + A3(u3, i, j, 1) / 3.0 : A3(u3, i, j, k - 1);
+                double up = k == maxn - 2 ? -2.0 * A3(u3, i, j, maxn - 2) + A3(u3, i, j, maxn - 3) / 3.0
+                                          : A3(u3, i, j, k + 1);
+                double res = helm(mu_u(mu, i, j, k), ax, ay, az, A3(u3, i + 1, j, k), A3(u3, i - 1, j, k),
+                                  A3(u3, i, j + 1, k), A3(u3, i, j - 1, k), up, um, A3(R, i, j, k));
+                A3(u3, i, j, k) = A3(u3, i, j, k) + RELAX * (res - A3(u3, i, j, k));
+            }
+        }
+}
+
+void orc_do_sor_v(double *v3_, const double *R_, const double *vs_, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int flg,
+                  int iteration_time)
+{
+    F3 v3 = f3(v3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+    int even = (iteration_time % 2) == 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int j = sy; j <= ey; ++j)
+        for (int k = 0; k <= maxn - 2; ++k) {
+            int i0 = (even ? ((k + j + flg) % 2) : ((j + flg) % 2)) + sx;
+            for (int i = i0; i <= ex; i += 2) {
+                double vm = k == 0 ? -2.0 * A3(v3, i, j, 0) + A3(v3, i, j, 1) / 3.0 : A3(v3, i, j, k - 1);
+                double vp = k == maxn - 2 ? -2.0 * A3(v3, i, j, maxn - 2) + A3(v3, i, j, maxn - 3) / 3.0
+                                          : A3(v3, i, j, k + 1);
+                double res = helm(mu_v(mu, i, j, k), ax, ay, az, A3(v3, i + 1, j, k), A3(v3, i - 1, j, k),
+                                  A3(v3, i, j + 1, k), A3(v3, i, j - 1, k), vp, vm, A3(R, i, j, k));
+                A3(v3, i, j, k) = A3(v3, i, j, k) + RELAX * (res - A3(v3, i, j, k));
+            }
+        }
+}
+
+void orc_do_sor_w(double *w3_, const double *R_, const double *vs_, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int flg,
+                  int iteration_time)
+{
+    F3 w3 = f3(w3_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 R = f3(R_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 mu = f3(vs_, sx - 2, ex + 2, sy - 2, ey + 2, 0);
+    double ax = dt / (dx * dx), ay = dt / (dy * dy), az = dt / (dz * dz);
+    int even = (iteration_time % 2) == 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int j = sy; j <= ey; ++j)
+        for (int k = 1; k <= maxn - 2; ++k) {
+            int i0 = (even ? ((k + j + flg) % 2) : ((j + flg) % 2)) + sx;
+            for (int i = i0; i <= ex; i += 2) {
+                double res = helm(mu_w(mu, i, j, k), ax, ay, az, A3(w3, i + 1, j, k), A3(w3, i - 1, j, k),
+                                  A3(w3, i, j + 1, k), A3(w3, i, j - 1, k), A3(w3, i, j, k + 1),
+                                  A3(w3, i, j, k - 1), A3(R, i, j, k));
+                A3(w3, i, j, k) = A3(w3, i, j, k) + RELAX * (res - A3(w3, i, j, k));
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pressure (pressure.f90)                                                    */
+/* ------------------------------------------------------------------------ */
+void orc_get_source_pres(double *rhs_, const double *u_, const double *v_, const double *w_,
+                         int ex, int ey, int sx, int sy, int maxn, double dx, double dy,
+                         double dz, double dt)
+{
+    F3 rhs = f3(rhs_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 u3 = f3(u_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v3 = f3(v_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w3 = f3(w_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn + 1);
+    memset(rhs_, 0, sizeof(double) * tot);
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(rhs, i, j, k) = ((A3(u3, i, j - 1, k - 1) - A3(u3, i - 1, j - 1, k - 1)) / dx) +
+                                   ((A3(v3, i, j, k - 1) - A3(v3, i, j - 1, k - 1)) / dy) +
+                                   ((A3(w3, i, j - 1, k) - A3(w3, i, j - 1, k - 1)) / dz);
+    for (long q = 0; q < tot; ++q) rhs_[q] = rhs_[q] / dt;
+}
+
+void orc_get_uvw_pres(double *u_, double *v_, double *w_, const double *p_, int ex, int ey,
+                      int sx, int sy, int maxn, int ex_max, double dx, double dy, double dz,
+                      double dt, const double *u_inlet)
+{
+    F3 u3 = f3(u_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 v3 = f3(v_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 w3 = f3(w_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 p = f3(p_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    int i0 = sx == 1 ? sx - 1 : sx;
+    int i1 = ex == ex_max ? ex + 1 : ex;
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                if (i == 0) A3(u3, i, j, k) = u_inlet[k];
+                else if (i == ex_max + 1) A3(u3, i, j, k) = 0.0;
+                else A3(u3, i, j, k) = A3(u3, i, j, k) - (dt / dx) * (A3(p, i + 1, j + 1, k + 1) - A3(p, i, j + 1, k + 1));
+            }
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = i0; i <= i1; ++i)
+                A3(v3, i, j, k) = A3(v3, i, j, k) - (dt / dy) * (A3(p, i, j + 1, k + 1) - A3(p, i, j, k + 1));
+    for (int k = 0; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = i0; i <= i1; ++i)
+                A3(w3, i, j, k) = A3(w3, i, j, k) - (dt / dz) * (A3(p, i, j + 1, k + 1) - A3(p, i, j + 1, k));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pressure Poisson solve                                                     */
+/* ------------------------------------------------------------------------ */
+/* The operator of pressure_solver.F90:296-441 on the DMDA
+ * (maxm-1) x (maxl+1) x (maxn+1) has two-term Neumann ghost rows in TPLS x
+ * (PETSc j) and z, and the 7-point Laplacian elsewhere.  Eliminating the ghost
+ * rows (p(0) = p(1) + dy*b_in, p(maxl) = p(maxl-1), p(k=0) = p(1),
+ * p(k=maxn) = p(maxn-1)) leaves a symmetric operator on the interior cells
+ * (1..maxl-1, 1..maxm-1 periodic, 1..maxn-1) whose null space is the
+ * constant.  The inlet flux moves to the right-hand side.  Coefficients are
+ * the reference's: TPLS-x neighbours 1/dy^2 (PETSc j direction), TPLS-y
+ * neighbours 1/dx^2 (PETSc i), z 1/dz^2, diagonal 2*(1/dx^2+1/dy^2+1/dz^2).
+ *
+ * Compact unknown layout: index (i-1) + nx*((j-1) + ny*(k-1)). */
+typedef struct {
+    int nx, ny, nz;
+    double cx, cy, cz, diag;
+} PoissonOp;
+
+static void op_apply(const PoissonOp *o, const double *x, double *y)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) {
+            int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+            for (int i = 0; i < nx; ++i) {
+                long c = i + (long)nx * (j + (long)ny * k);
+                double xc = x[c];
+                /* ghost substitution: a Neumann neighbour equals the centre */
+                double xm = i > 0 ? x[c - 1] : xc;
+                double xp = i < nx - 1 ? x[c + 1] : xc;
+                double ym = x[i + (long)nx * (jm + (long)ny * k)];
+                double yp = x[i + (long)nx * (jp + (long)ny * k)];
+                double zm = k > 0 ? x[c - (long)nx * ny] : xc;
+                double zp = k < nz - 1 ? x[c + (long)nx * ny] : xc;
+                y[c] = o->diag * xc - o->cz * zm - o->cx * xm - o->cy * ym - o->cy * yp -
+                       o->cx * xp - o->cz * zp;
+            }
+        }
+}
+
+/* Deterministic sums: per-plane sequential partials, planes added in order. */
+static void sums2(const double *a, const double *b, long plane, int nz, double *s_ab,
+                  double *s_a)
+{
+    double *pa = malloc(sizeof(double) * nz), *pab = malloc(sizeof(double) * nz);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < nz; ++k) {
+        double t = 0.0, u = 0.0;
+        const double *ak = a + (long)k * plane, *bk = b + (long)k * plane;
+        for (long q = 0; q < plane; ++q) {
+            t += ak[q] * bk[q];
+            u += ak[q];
+        }
+        pab[k] = t;
+        pa[k] = u;
+    }
+    double t = 0.0, u = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        t += pab[k];
+        u += pa[k];
+    }
+    *s_ab = t;
+    if (s_a) *s_a = u;
+    free(pa);
+    free(pab);
+}
+
+/* PETSc KSPConvergedReason codes used by pressure_solver.F90:101-142 */
+enum { R_DIV_INDEF_MAT = -10, R_DIV_NAN = -9, R_DIV_INDEF_PC = -8, R_DIV_DTOL = -4,
+       R_DIV_ITS = -3, R_CONV_RTOL = 2, R_CONV_ATOL = 3 };
+
+int orc_pressure_solve(double *pres_, const double *RHS_p_, const double *u3_,
+                       const double *u_inlet, int maxl, int maxm, int maxn, double dx,
+                       double dy, double dz, double dt, double rtol, double abstol,
+                       double dtol, int maxits, int *its_out, double *rnorm_out)
+{
+    int sx = 1, ex = maxl - 1, sy = 1, ey = maxm - 1;
+    F3 pres = f3(pres_, 0, maxl, 0, maxm, 0);
+    F3 rp = f3(RHS_p_, 0, maxl, 0, maxm, 0);
+    F3 u3 = f3(u3_, 0, maxl, 0, maxm, 0);
+    PoissonOp o;
+    o.nx = maxl - 1; o.ny = maxm - 1; o.nz = maxn - 1;
+    /* pressure_solver.F90:394-421 */
+    o.cz = 1 / (dz * dz);
+    o.cx = 1 / (dy * dy);
+    o.cy = 1 / (dx * dx);
+    o.diag = 2 * (1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz));
+    long N = (long)o.nx * o.ny * o.nz, plane = (long)o.nx * o.ny;
+    double *b = malloc(sizeof(double) * N), *x = malloc(sizeof(double) * N);
+    double *r = malloc(sizeof(double) * N), *p = malloc(sizeof(double) * N);
+    double *w = malloc(sizeof(double) * N);
+    /* b = -RHS_p (pressure_solver.F90:272-278) with the inlet Neumann flux of
+     * pressure_solver.F90:280-286 folded in: (1/dy^2)*dy*b_in = b_in/dy */
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                long c = (i - 1) + (long)o.nx * ((j - 1) + (long)o.ny * (k - 1));
+                double v = -A3(rp, i, j, k);
+                if (i == 1) {
+                    double b_in = (dx / dt) * (u_inlet[k - 1] - A3(u3, 0, j - 1, k - 1)) / dy;
+                    v = v + b_in / dy;
+                }
+                b[c] = v;
+                x[c] = A3(pres, i, j, k);
+            }
+    /* remove the constant null-space component from the right-hand side */
+    double sb, dummy;
+    sums2(b, b, plane, o.nz, &dummy, &sb);
+    double mb = sb / (double)N;
+    for (long q = 0; q < N; ++q) b[q] -= mb;
+    double bb;
+    sums2(b, b, plane, o.nz, &bb, NULL);
+    double bnorm = sqrt(bb);
+    /* r = b - A x */
+    op_apply(&o, x, w);
+    for (long q = 0; q < N; ++q) r[q] = b[q] - w[q];
+    /* z = r - mean(r); beta = z.r = |z|^2 = sum r^2 - (sum r)^2 / N */
+    double rr, sr;
+    sums2(r, r, plane, o.nz, &rr, &sr);
+    double m = sr / (double)N;
+    double beta = rr - sr * m;
+    double dp = sqrt(beta > 0 ? beta : 0);
+    double rnorm0 = bnorm > 0 ? bnorm : dp;
+    double ttol = rtol * rnorm0 > abstol ? rtol * rnorm0 : abstol;
+    int reason = 0, its = 0;
+    if (isnan(dp)) reason = R_DIV_NAN;
+    else if (dp <= ttol) reason = dp < abstol ? R_CONV_ATOL : R_CONV_RTOL;
+    double betaold = 0.0;
+    int i = 0;
+    while (!reason) {
+        its = i + 1;
+        if (beta == 0.0) { reason = R_CONV_ATOL; break; }
+        if (beta < 0.0) { reason = R_DIV_INDEF_PC; break; }
+        if (i == 0) {
+            for (long q = 0; q < N; ++q) p[q] = r[q] - m;
+        } else {
+            double bt = beta / betaold;
+            for (long q = 0; q < N; ++q) p[q] = (r[q] - m) + bt * p[q];
+        }
+        betaold = beta;
+        op_apply(&o, p, w);
+        double dpi;
+        sums2(p, w, plane, o.nz, &dpi, NULL);
+        if (!(dpi > 0.0)) { reason = isnan(dpi) ? R_DIV_NAN : R_DIV_INDEF_MAT; break; }
+        double a = beta / dpi;
+        for (long q = 0; q < N; ++q) {
+            x[q] = x[q] + a * p[q];
+            r[q] = r[q] - a * w[q];
+        }
+        sums2(r, r, plane, o.nz, &rr, &sr);
+        m = sr / (double)N;
+        beta = rr - sr * m;
+        dp = sqrt(beta > 0 ? beta : 0);
+        if (isnan(dp) || isnan(beta)) reason = R_DIV_NAN;
+        else if (dp <= ttol) reason = dp < abstol ? R_CONV_ATOL : R_CONV_RTOL;
+        else if (dp >= dtol * rnorm0) reason = R_DIV_DTOL;
+        if (reason) break;
+        ++i;
+        if (i >= maxits) { reason = R_DIV_ITS; break; }
+    }
+    *its_out = its;
+    *rnorm_out = dp;
+    /* copy-out with ghost planes (pressure_solver.F90:444-486): interior,
+     * then the eliminated ghost rows, then periodic y halos */
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j) {
+            for (int ii = sx; ii <= ex; ++ii)
+                A3(pres, ii, j, k) = x[(ii - 1) + (long)o.nx * ((j - 1) + (long)o.ny * (k - 1))];
+            double b_in = (dx / dt) * (u_inlet[k - 1] - A3(u3, 0, j - 1, k - 1)) / dy;
+            A3(pres, 0, j, k) = A3(pres, 1, j, k) + dy * b_in;
+            A3(pres, maxl, j, k) = A3(pres, maxl - 1, j, k);
+        }
+    for (int j = sy; j <= ey; ++j)
+        for (int ii = 0; ii <= maxl; ++ii) {
+            A3(pres, ii, j, 0) = A3(pres, ii, j, 1);
+            A3(pres, ii, j, maxn) = A3(pres, ii, j, maxn - 1);
+        }
+    exchange2d(pres_, sx, ex, sy, ey, 0, maxn);
+    free(b); free(x); free(r); free(p); free(w);
+    return reason;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Whole-step state and driver (main_ns_hybrid.F90)                           */
+/* ------------------------------------------------------------------------ */
+struct orc_state {
+    int maxl, maxm, maxn;
+    orc_params prm;
+    double *f[ORC_NFIELDS];
+    long cnt[ORC_NFIELDS];
+    double *u3_old, *v3_old, *w3_old;
+};
+
+static int field_grid(int id)
+{
+    /* 0 = uv grid (0:maxn-2), 1 = w grid (0:maxn-1), 2 = p grid (0:maxn),
+     * 3 = aug2 p grid, 4 = 1-D inlet profile */
+    switch (id) {
+    case ORC_W2: case ORC_W3: case ORC_CONV0_W: case ORC_CONV1_W: case ORC_CONV2_W:
+    case ORC_CSF_W0: case ORC_CSF_W1: case ORC_CSF_W2: case ORC_DIFF_W: case ORC_RHS_W:
+        return 1;
+    case ORC_PHI2: case ORC_PHI3: case ORC_PRES: case ORC_PRES_OLD: case ORC_CONV0_PHI:
+    case ORC_CONV1_PHI: case ORC_CONV2_PHI: case ORC_FX_CSF: case ORC_FY_CSF: case ORC_FZ_CSF:
+    case ORC_VISC2: case ORC_RHS_P:
+        return 2;
+    case ORC_VISC2_AUG: return 3;
+    case ORC_U_INLET: return 4;
+    default: return 0;
+    }
+}
+
+orc_state *orc_state_create(int maxl, int maxm, int maxn, const orc_params *prm)
+{
+    orc_state *s = calloc(1, sizeof(orc_state));
+    s->maxl = maxl; s->maxm = maxm; s->maxn = maxn;
+    s->prm = *prm;
+    long nxy = (long)(maxl + 1) * (maxm + 1);
+    for (int id = 0; id < ORC_NFIELDS; ++id) {
+        long c;
+        switch (field_grid(id)) {
+        case 0: c = nxy * (maxn - 1); break;
+        case 1: c = nxy * maxn; break;
+        case 2: c = nxy * (maxn + 1); break;
+        case 3: c = (long)(maxl + 3) * (maxm + 3) * (maxn + 1); break;
+        default: c = maxn - 1; break;
+        }
+        s->cnt[id] = c;
+        s->f[id] = calloc(c, sizeof(double));
+    }
+    s->u3_old = calloc(s->cnt[ORC_U3], sizeof(double));
+    s->v3_old = calloc(s->cnt[ORC_V3], sizeof(double));
+    s->w3_old = calloc(s->cnt[ORC_W3], sizeof(double));
+    return s;
+}
+
+void orc_state_destroy(orc_state *s)
+{
+    if (!s) return;
+    for (int id = 0; id < ORC_NFIELDS; ++id) free(s->f[id]);
+    free(s->u3_old); free(s->v3_old); free(s->w3_old);
+    free(s);
+}
+
+double *orc_state_field(orc_state *s, int id, long *count)
+{
+    if (id < 0 || id >= ORC_NFIELDS) return NULL;
+    if (count) *count = s->cnt[id];
+    return s->f[id];
+}
+
+/* main_ns_hybrid.F90:412-549: u2,v2,w2,phi2,pres from the initial files,
+ * every other array zero (viscosity arrays are recomputed before first use),
+ * u_inlet = u(0,0,:). */
+void orc_state_load(orc_state *s, const double *u, const double *v, const double *w,
+                    const double *p, const double *phi)
+{
+    for (int id = 0; id < ORC_NFIELDS; ++id) memset(s->f[id], 0, sizeof(double) * s->cnt[id]);
+    memset(s->u3_old, 0, sizeof(double) * s->cnt[ORC_U3]);
+    memset(s->v3_old, 0, sizeof(double) * s->cnt[ORC_V3]);
+    memset(s->w3_old, 0, sizeof(double) * s->cnt[ORC_W3]);
+    memcpy(s->f[ORC_U2], u, sizeof(double) * s->cnt[ORC_U2]);
+    memcpy(s->f[ORC_V2], v, sizeof(double) * s->cnt[ORC_V2]);
+    memcpy(s->f[ORC_W2], w, sizeof(double) * s->cnt[ORC_W2]);
+    memcpy(s->f[ORC_PRES], p, sizeof(double) * s->cnt[ORC_PRES]);
+    memcpy(s->f[ORC_PHI2], phi, sizeof(double) * s->cnt[ORC_PHI2]);
+    F3 u2 = f3(s->f[ORC_U2], 0, s->maxl, 0, s->maxm, 0);
+    for (int k = 0; k <= s->maxn - 2; ++k) s->f[ORC_U_INLET][k] = A3(u2, 0, 0, k);
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void rot3(double **a0, double **a1, double **a2)
+{
+    /* conv0 <- conv1 <- conv2 (main_ns_hybrid.F90:2480-2497); the old conv0
+     * buffer becomes the next conv2, which every step rewrites in full */
+    double *t = *a0;
+    *a0 = *a1;
+    *a1 = *a2;
+    *a2 = t;
+}
+
+/* u3 inlet copy and outlet pressure extrapolation, main_ns_hybrid.F90:764-776
+ * (repeated at :787-799); the outlet reads pres with an unshifted j */
+static void u_inout_bc(double *u3_, F3 pres, int maxl, int maxm, int maxn, double dt, double dx)
+{
+    F3 U = f3(u3_, 0, maxl, 0, maxm, 0);
+    int ex_max = maxl - 1;
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = 1; j <= maxm - 1; ++j) {
+            A3(U, 0, j, k) = A3(U, 1, j, k);
+            A3(U, ex_max, j, k) =
+                A3(U, ex_max - 1, j, k) +
+                (dt / dx) * (A3(pres, ex_max + 1, j, k + 1) + A3(pres, ex_max - 1, j, k + 1) -
+                             2.0 * A3(pres, ex_max, j, k + 1));
+            A3(U, ex_max + 1, j, k) = 0.0;
+        }
+}
+
+static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
+{
+    const int maxl = s->maxl, maxm = s->maxm, maxn = s->maxn;
+    const int sx = 1, ex = maxl - 1, sy = 1, ey = maxm - 1, ex_max = maxl - 1;
+    const orc_params *P = &s->prm;
+    const double dx = P->dx, dy = P->dy, dz = P->dz, dt = P->dt;
+    double **f = s->f;
+    double t0 = now_s();
+    F3 phi2 = f3(f[ORC_PHI2], 0, maxl, 0, maxm, 0);
+    F3 pres = f3(f[ORC_PRES], 0, maxl, 0, maxm, 0);
+    long n_uv = s->cnt[ORC_U2], n_w = s->cnt[ORC_W2];
+
+    /* main:585 */
+    orc_do_advect_phi(f[ORC_PHI2], f[ORC_U2], f[ORC_V2], f[ORC_W2], f[ORC_CONV2_PHI], sx, ex, sy,
+                      ey, maxl, maxn, ex_max, dx, dy, dz);
+    /* main:596-599 */
+    double err2 = 0.0;
+    orc_do_tpls(f[ORC_PHI2], f[ORC_PHI3], f[ORC_CONV0_PHI], f[ORC_CONV1_PHI], f[ORC_CONV2_PHI], sx,
+                ex, sy, ey, maxn, ex_max, P->levelset_its, P->height, &err2, dx, dz, dt);
+    dg->err_phi = err2;
+    /* main:618-627 */
+    exchange2d(f[ORC_PHI3], sx, ex, sy, ey, 0, maxn);
+    orc_get_csf(f[ORC_FX_CSF], f[ORC_FY_CSF], f[ORC_FZ_CSF], f[ORC_PHI3], sx, ex, sy, ey, maxn,
+                ex_max, dx, dy, dz, P->scap, P->smooth_width);
+    exchange2d(f[ORC_FX_CSF], sx, ex, sy, ey, 0, maxn);
+    exchange2d(f[ORC_FY_CSF], sx, ex, sy, ey, 0, maxn);
+    exchange2d(f[ORC_FZ_CSF], sx, ex, sy, ey, 0, maxn);
+    /* main:629-641 */
+    {
+        F3 fx = f3(f[ORC_FX_CSF], 0, maxl, 0, maxm, 0), fy = f3(f[ORC_FY_CSF], 0, maxl, 0, maxm, 0);
+        F3 fz = f3(f[ORC_FZ_CSF], 0, maxl, 0, maxm, 0);
+        F3 cu = f3(f[ORC_CSF_U2], 0, maxl, 0, maxm, 0), cv = f3(f[ORC_CSF_V2], 0, maxl, 0, maxm, 0);
+        F3 cw = f3(f[ORC_CSF_W2], 0, maxl, 0, maxm, 0);
+        for (int k = 0; k <= maxn - 2; ++k)
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i) {
+                    A3(cu, i, j, k) = (A3(fx, i + 1, j + 1, k + 1) + A3(fx, i, j + 1, k + 1)) / 2.0;
+                    A3(cv, i, j, k) = (A3(fy, i, j + 1, k + 1) + A3(fy, i, j, k + 1)) / 2.0;
+                }
+        for (int k = 0; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i)
+                    A3(cw, i, j, k) = (A3(fz, i, j + 1, k + 1) + A3(fz, i, j + 1, k)) / 2.0;
+    }
+    /* main:646-710 */
+    orc_get_viscosity(f[ORC_VISC2], f[ORC_PHI2], ex, ey, sx, sy, maxn, P->smooth_width, P->Re,
+                      P->mu_minus, P->mu_plus);
+    {
+        F3 v2 = f3(f[ORC_VISC2], 0, maxl, 0, maxm, 0);
+        F3 va = f3(f[ORC_VISC2_AUG], -1, maxl + 1, -1, maxm + 1, 0);
+        for (int k = 0; k <= maxn; ++k)
+            for (int j = sy; j <= ey; ++j) {
+                for (int i = sx; i <= ex; ++i) A3(va, i, j, k) = A3(v2, i, j, k);
+                double m0 = visc_of_phi(A3(phi2, 0, j, k), P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
+                A3(va, 0, j, k) = m0;
+                A3(v2, 0, j, k) = m0;
+                double m1 = visc_of_phi(A3(phi2, ex_max + 1, j, k), P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
+                A3(va, ex_max + 1, j, k) = m1;
+                A3(v2, ex_max + 1, j, k) = m1;
+            }
+        exchange2d_aug12(f[ORC_VISC2_AUG], sx, ex, sy, ey, 0, maxn);
+        exchange2d(f[ORC_VISC2], sx, ex, sy, ey, 0, maxn);
+    }
+    /* main:724-735 */
+    exchange2d(f[ORC_U2], sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(f[ORC_V2], sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(f[ORC_W2], sx, ex, sy, ey, 0, maxn - 1);
+    orc_get_conv_all(f[ORC_CONV2_U], f[ORC_CONV2_V], f[ORC_CONV2_W], f[ORC_VISC2_AUG], f[ORC_U2],
+                     f[ORC_V2], f[ORC_W2], sx, sy, ex, ey, maxn, ex_max, dx, dy, dz);
+    orc_get_diffusion_all(f[ORC_DIFF_U], f[ORC_DIFF_V], f[ORC_DIFF_W], f[ORC_VISC2_AUG], f[ORC_U2],
+                          f[ORC_V2], f[ORC_W2], sx, sy, ex, ey, maxn, ex_max, dx, dy, dz);
+    /* main:743-757 (AB3 + CN right-hand sides) */
+    {
+        int cid[3][7] = {{ORC_RHS_U, ORC_U2, ORC_CONV2_U, ORC_CONV1_U, ORC_CONV0_U, ORC_DIFF_U, ORC_CSF_U2},
+                         {ORC_RHS_V, ORC_V2, ORC_CONV2_V, ORC_CONV1_V, ORC_CONV0_V, ORC_DIFF_V, ORC_CSF_V2},
+                         {ORC_RHS_W, ORC_W2, ORC_CONV2_W, ORC_CONV1_W, ORC_CONV0_W, ORC_DIFF_W, ORC_CSF_W2}};
+        for (int c = 0; c < 3; ++c) {
+            double *R = f[cid[c][0]], *x2 = f[cid[c][1]], *c2 = f[cid[c][2]], *c1 = f[cid[c][3]];
+            double *c0 = f[cid[c][4]], *df = f[cid[c][5]], *s2 = f[cid[c][6]];
+            double *s1 = f[cid[c][6] - 1], *s0 = f[cid[c][6] - 2];
+            long cnt = c == 2 ? n_w : n_uv;
+            for (long q = 0; q < cnt; ++q)
+                R[q] = x2[q] - dt * ((23.0 / 12.0) * c2[q] - (4.0 / 3.0) * c1[q] + (5.0 / 12.0) * c0[q]) +
+                       (dt / 2.0) * df[q] +
+                       dt * ((23.0 / 12.0) * s2[q] - (4.0 / 3.0) * s1[q] + (5.0 / 12.0) * s0[q]);
+        }
+    }
+    /* main:758-808: u */
+    {
+        double *u3 = f[ORC_U3];
+        memset(u3, 0, sizeof(double) * n_uv);
+        for (int it = 1; it <= P->mom_u / 2; ++it) {
+            memcpy(s->u3_old, u3, sizeof(double) * n_uv);
+            exchange2d(s->u3_old, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_jacobi_u(u3, s->u3_old, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx,
+                            sy, maxn, ex_max);
+            u_inout_bc(u3, pres, maxl, maxm, maxn, dt, dx);
+            exchange2d(u3, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_sor_u(u3, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn,
+                         ex_max, 0, it);
+            exchange2d(u3, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_sor_u(u3, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn,
+                         ex_max, 1, it);
+            u_inout_bc(u3, pres, maxl, maxm, maxn, dt, dx);
+        }
+        dg->err_u = orc_get_difference(u3, s->u3_old, ex, ey, maxn - 2, sx, sy, 0);
+    }
+    /* main:811 */
+    exchange2d(f[ORC_PRES], sx, ex, sy, ey, 0, maxn);
+    /* main:813-845: v */
+    {
+        double *v3 = f[ORC_V3];
+        F3 V = f3(v3, 0, maxl, 0, maxm, 0);
+        memset(v3, 0, sizeof(double) * n_uv);
+        for (int it = 1; it <= P->mom_v / 2; ++it) {
+            memcpy(s->v3_old, v3, sizeof(double) * n_uv);
+            exchange2d(s->v3_old, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_jacobi_v(v3, s->v3_old, f[ORC_RHS_V], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx,
+                            sy, maxn, ex_max);
+            exchange2d(v3, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_sor_v(v3, f[ORC_RHS_V], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 0, it);
+            exchange2d(v3, sx, ex, sy, ey, 0, maxn - 2);
+            orc_do_sor_v(v3, f[ORC_RHS_V], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 1, it);
+            for (int k = 0; k <= maxn - 2; ++k)
+                for (int j = sy; j <= ey; ++j) {
+                    A3(V, 0, j, k) = A3(V, 1, j, k);
+                    A3(V, ex_max + 1, j, k) = A3(V, ex_max, j, k);
+                }
+        }
+        dg->err_v = orc_get_difference(v3, s->v3_old, ex, ey, maxn - 2, sx, sy, 0);
+    }
+    /* main:847-887: w */
+    {
+        double *w3 = f[ORC_W3];
+        F3 W = f3(w3, 0, maxl, 0, maxm, 0);
+        memset(w3, 0, sizeof(double) * n_w);
+        for (int it = 1; it <= P->mom_w / 2; ++it) {
+            memcpy(s->w3_old, w3, sizeof(double) * n_w);
+            exchange2d(s->w3_old, sx, ex, sy, ey, 0, maxn - 1);
+            orc_do_jacobi_w(w3, s->w3_old, f[ORC_RHS_W], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx,
+                            sy, maxn, ex_max);
+            exchange2d(w3, sx, ex, sy, ey, 0, maxn - 1);
+            orc_do_sor_w(w3, f[ORC_RHS_W], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 0, it);
+            exchange2d(w3, sx, ex, sy, ey, 0, maxn - 1);
+            orc_do_sor_w(w3, f[ORC_RHS_W], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 1, it);
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i) {
+                    A3(W, i, j, 0) = 0.0;
+                    A3(W, i, j, maxn - 1) = 0.0;
+                }
+            for (int j = sy; j <= ey; ++j) {
+                for (int k = 1; k <= maxn - 2; ++k) A3(W, 0, j, k) = A3(W, 1, j, k);
+                A3(W, 0, j, 0) = 0.0;
+                A3(W, 0, j, maxn - 1) = 0.0;
+                for (int k = 0; k <= maxn - 1; ++k) A3(W, ex_max + 1, j, k) = A3(W, ex_max, j, k);
+            }
+        }
+        dg->err_w = orc_get_difference(w3, s->w3_old, ex, ey, maxn - 1, sx, sy, 0);
+    }
+    /* main:889-894 */
+    exchange2d(f[ORC_U3], sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(f[ORC_V3], sx, ex, sy, ey, 0, maxn - 2);
+    exchange2d(f[ORC_W3], sx, ex, sy, ey, 0, maxn - 1);
+    orc_get_source_pres(f[ORC_RHS_P], f[ORC_U3], f[ORC_V3], f[ORC_W3], ex, ey, sx, sy, maxn, dx, dy,
+                        dz, dt);
+    exchange2d(f[ORC_RHS_P], sx, ex, sy, ey, 0, maxn);
+    /* main:896-913 */
+    memcpy(f[ORC_PRES], f[ORC_PRES_OLD], sizeof(double) * s->cnt[ORC_PRES]);
+    double t1 = now_s();
+    dg->ksp_reason = orc_pressure_solve(f[ORC_PRES], f[ORC_RHS_P], f[ORC_U3], f[ORC_U_INLET], maxl,
+                                        maxm, maxn, dx, dy, dz, dt, P->ksp_rtol, P->ksp_abstol,
+                                        P->ksp_dtol, P->ksp_maxits, &dg->ksp_its, &dg->ksp_rnorm);
+    double t2 = now_s();
+    /* main:918 */
+    orc_get_uvw_pres(f[ORC_U3], f[ORC_V3], f[ORC_W3], f[ORC_PRES], ex, ey, sx, sy, maxn, ex_max,
+                     dx, dy, dz, dt, f[ORC_U_INLET]);
+    /* main:2480-2509 */
+    rot3(&f[ORC_CONV0_U], &f[ORC_CONV1_U], &f[ORC_CONV2_U]);
+    rot3(&f[ORC_CONV0_V], &f[ORC_CONV1_V], &f[ORC_CONV2_V]);
+    rot3(&f[ORC_CONV0_W], &f[ORC_CONV1_W], &f[ORC_CONV2_W]);
+    rot3(&f[ORC_CSF_U0], &f[ORC_CSF_U1], &f[ORC_CSF_U2]);
+    rot3(&f[ORC_CSF_V0], &f[ORC_CSF_V1], &f[ORC_CSF_V2]);
+    rot3(&f[ORC_CSF_W0], &f[ORC_CSF_W1], &f[ORC_CSF_W2]);
+    rot3(&f[ORC_CONV0_PHI], &f[ORC_CONV1_PHI], &f[ORC_CONV2_PHI]);
+    memcpy(f[ORC_PRES_OLD], f[ORC_PRES], sizeof(double) * s->cnt[ORC_PRES]);
+    memcpy(f[ORC_U2], f[ORC_U3], sizeof(double) * n_uv);
+    memcpy(f[ORC_V2], f[ORC_V3], sizeof(double) * n_uv);
+    memcpy(f[ORC_W2], f[ORC_W3], sizeof(double) * n_w);
+    memcpy(f[ORC_PHI2], f[ORC_PHI3], sizeof(double) * s->cnt[ORC_PHI2]);
+    double t3 = now_s();
+    if (t_nonp) *t_nonp = (t1 - t0) + (t3 - t2);
+    if (t_p) *t_p = t2 - t1;
+}
+
+void orc_step(orc_state *s, int nsteps, orc_diag *diag)
+{
+    orc_diag d;
+    memset(&d, 0, sizeof d);
+    for (int n = 0; n < nsteps; ++n) step_impl(s, &d, NULL, NULL);
+    if (diag) *diag = d;
+}
+
+void orc_step_timed(orc_state *s, double *t_nonp, double *t_p, int *its)
+{
+    orc_diag d;
+    memset(&d, 0, sizeof d);
+    step_impl(s, &d, t_nonp, t_p);
+    if (its) *its = d.ksp_its;
+}
