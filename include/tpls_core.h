@@ -1,0 +1,204 @@
+/*
+ * tpls_core.h -- C ABI of the MI355X-native TPLS stencil core.
+ *
+ * Two layers:
+ *
+ *  1. Per-routine drop-in entry points.  One entry per reference procedure on
+ *     the per-timestep path, with the reference's argument order and meaning
+ *     (Fortran arrays passed as pointers to their first element, column-major,
+ *     with the reference's index bounds; integers and doubles by value).
+ *     Host arrays go in and out; the computation runs on the GPU.  Each
+ *     comment cites the reference interface it replaces.  The MPI-only
+ *     arguments of the reference (neighbours, communicator, stride types) are
+ *     dropped: the whole domain lives on one GPU, so the entry points require
+ *     the single-rank decomposition sx = 1, ex = ex_max = maxl-1, sy = 1,
+ *     ey = maxm-1 (and return TPLS_ERR_DECOMP otherwise).
+ *
+ *  2. A device-resident case: all state arrays of main_ns_hybrid.F90 stay in
+ *     HBM and tpls_step() advances whole timesteps (main_ns_hybrid.F90:583-2509
+ *     without I/O) with the same kernels.
+ *
+ * Every function returns TPLS_OK (0) or a negative TPLS_ERR_* code;
+ * tpls_last_error() describes the last failure of the calling thread.
+ * Calls on one case must not overlap; different cases may be driven from
+ * different threads.
+ */
+#ifndef TPLS_CORE_H
+#define TPLS_CORE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TPLS_OK = 0,
+    TPLS_ERR_HIP = -1,       /* a HIP runtime call failed */
+    TPLS_ERR_ARG = -2,       /* bad argument (null pointer, size, field id) */
+    TPLS_ERR_DECOMP = -3,    /* not the single-rank decomposition */
+    TPLS_ERR_NOMEM = -4,     /* device allocation failed */
+    TPLS_ERR_NOGPU = -5      /* no usable gfx950 device */
+};
+
+/* Storage precision of a case. */
+enum { TPLS_FP64 = 0, TPLS_FP32 = 1 };
+
+/* Case parameters: the option values the core reads (tpls_config.opt:29-34,
+ * initial_config.dat keys written by create_initial_conditions_program.F90:135-169),
+ * plus the Krylov tolerances PETSc would take from its runtime options
+ * (configure_mpi_petsc.F90:158, 177-182). */
+typedef struct {
+    double dx, dy, dz, dt, Re, mu_plus, mu_minus, height, smooth_width, scap;
+    int mom_u, mom_v, mom_w, levelset_its;
+    double ksp_rtol, ksp_abstol, ksp_dtol;
+    int ksp_maxits;
+    int precision;    /* TPLS_FP64 or TPLS_FP32 */
+    int diagnostics;  /* 1: compute the per-step residual prints of
+                         main_ns_hybrid.F90:603-605, 806-808, 843-845, 885-887 */
+} tpls_params;
+
+/* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
+typedef struct {
+    double err_phi, err_u, err_v, err_w;
+    int ksp_its, ksp_reason; /* PETSc KSPConvergedReason codes */
+    double ksp_rnorm;
+} tpls_diag;
+
+/* Field identifiers of a case (main_ns_hybrid.F90:416-549 arrays). */
+enum {
+    TPLS_F_U2 = 0, TPLS_F_V2, TPLS_F_W2, TPLS_F_U3, TPLS_F_V3, TPLS_F_W3,
+    TPLS_F_PHI2, TPLS_F_PHI3, TPLS_F_PRES, TPLS_F_PRES_OLD,
+    TPLS_F_CONV0_U, TPLS_F_CONV1_U, TPLS_F_CONV2_U, TPLS_F_CONV0_V, TPLS_F_CONV1_V, TPLS_F_CONV2_V,
+    TPLS_F_CONV0_W, TPLS_F_CONV1_W, TPLS_F_CONV2_W,
+    TPLS_F_CSF_U0, TPLS_F_CSF_U1, TPLS_F_CSF_U2, TPLS_F_CSF_V0, TPLS_F_CSF_V1, TPLS_F_CSF_V2,
+    TPLS_F_CSF_W0, TPLS_F_CSF_W1, TPLS_F_CSF_W2,
+    TPLS_F_CONV0_PHI, TPLS_F_CONV1_PHI, TPLS_F_CONV2_PHI,
+    TPLS_F_FX_CSF, TPLS_F_FY_CSF, TPLS_F_FZ_CSF,
+    TPLS_F_VISC2, TPLS_F_VISC2_AUG, TPLS_F_RHS_P,
+    TPLS_F_DIFF_U, TPLS_F_DIFF_V, TPLS_F_DIFF_W, TPLS_F_RHS_U, TPLS_F_RHS_V, TPLS_F_RHS_W,
+    TPLS_F_U_INLET,
+    TPLS_NFIELDS
+};
+
+typedef struct tpls_case tpls_case;
+
+const char *tpls_last_error(void);
+
+/* ---- device-resident case ---- */
+
+/* Device bytes a case of this size and precision allocates. */
+size_t tpls_case_bytes(int maxl, int maxm, int maxn, int precision);
+/* Allocate a case on the current device.  Replaces the array allocation of
+ * main_ns_hybrid.F90:416-549 and the DMDA/KSP set-up of
+ * configure_mpi_petsc.F90:61-212. */
+int tpls_case_create(const tpls_params *prm, int maxl, int maxm, int maxn, tpls_case **out);
+int tpls_case_destroy(tpls_case *c);
+/* Initial state exactly as main_ns_hybrid.F90:412-549 builds it from the five
+ * initial_*.dat arrays (u, v, w, pressure, phi in their Fortran shapes):
+ * every other array zero, pres_old = 0, u_inlet = u(0,0,:). */
+int tpls_case_load(tpls_case *c, const double *u, const double *v, const double *w,
+                   const double *pres, const double *phi);
+/* Copy a field to / from a host array of its Fortran shape (halo rows of the
+ * periodic y direction are returned as the periodic wrap). */
+int tpls_case_get_field(tpls_case *c, int field, double *host);
+int tpls_case_set_field(tpls_case *c, int field, const double *host);
+/* Advance nsteps timesteps (main_ns_hybrid.F90:583-918, 2480-2509). */
+int tpls_step(tpls_case *c, int nsteps);
+/* Diagnostics of the last step. */
+int tpls_case_diag(tpls_case *c, tpls_diag *out);
+/* Wait for all work of the case. */
+int tpls_case_sync(tpls_case *c);
+/* Kernel timing: while enabled, every launch of the CG update kernel (cg_x)
+ * is bracketed by HIP events on the case's stream; tpls_case_kernel_time
+ * returns the summed milliseconds and launch count since enabling. */
+int tpls_case_time_kernels(tpls_case *c, int enable);
+int tpls_case_kernel_time(tpls_case *c, double *ms, long *launches);
+
+/* ---- per-routine drop-in entry points (host arrays) ---- */
+
+/* weno.f90:18  WENOAppoxConvection(phi,u,v,w,sx,ex,sy,ey,l,n,dfx,dfy,dfz) */
+int tpls_wenoappoxconvection(const double *phi, const double *u, const double *v, const double *w,
+                             int sx, int ex, int sy, int ey, int l, int n, double *dfx, double *dfy,
+                             double *dfz);
+/* advect_phi.f90:19  do_advect_phi (MPI arguments dropped) */
+int tpls_do_advect_phi(double *phi2, double *u2, double *v2, double *w2, double *conv2_phi, int sx,
+                       int ex, int sy, int ey, int maxl, int maxn, int ex_max, double dx, double dy,
+                       double dz);
+/* two_phase_levelset.f90:20  do_tpls (MPI arguments dropped) */
+int tpls_do_tpls(double *phi2, double *phi3, double *conv0_phi, double *conv1_phi,
+                 double *conv2_phi, int sx, int ex, int sy, int ey, int maxn, int ex_max,
+                 int max_iteration_levelset, double height, double *err2, double dx, double dz,
+                 double dt);
+/* levelset.f90:17  get_levelset_status(phi,D_mx,istatus,sign_mx,sx,ex,sy,ey,n,h) */
+int tpls_get_levelset_status(const double *phi, double *D_mx, int *istatus, double *sign_mx, int sx,
+                             int ex, int sy, int ey, int n, double h);
+/* levelset.f90:97  do_levelset_iteration(...,error_val) */
+int tpls_do_levelset_iteration(const double *phi, double *phi_reinit, const double *D_mx,
+                               const int *istatus, const double *sign_mx, int sx, int ex, int sy,
+                               int ey, int n, int ex_max, double h, double height,
+                               double *error_val);
+/* levelset.f90:241  get_csf(fx,fy,fz,fi,sx,ex,sy,ey,n,ex_max,dx,dy,dz,scap,smooth_width) */
+int tpls_get_csf(double *fx, double *fy, double *fz, const double *fi, int sx, int ex, int sy,
+                 int ey, int n, int ex_max, double dx, double dy, double dz, double scap,
+                 double smooth_width);
+/* momentum_allflux.f90:22  get_diffusion_all */
+int tpls_get_diffusion_all(double *diff_u, double *diff_v, double *diff_w, const double *viscosity,
+                           const double *u2, const double *v2, const double *w2, int sx, int sy,
+                           int ex, int ey, int maxn, int ex_max, double dx, double dy, double dz);
+/* momentum_allflux.f90:224  get_conv_all */
+int tpls_get_conv_all(double *conv_u2, double *conv_v2, double *conv_w2, const double *viscosity,
+                      const double *u2, const double *v2, const double *w2, int sx, int sy, int ex,
+                      int ey, int maxn, int ex_max, double dx, double dy, double dz);
+/* momentum_allflux.f90:474  get_viscosity */
+int tpls_get_viscosity(double *viscosity, const double *phi, int ex, int ey, int sx, int sy,
+                       int maxn, double smooth_width, double Re, double mu_minus, double mu_plus);
+/* jacobi_iteration_allflux.f90:21 / 102 / 195  do_jacobi_u / _v / _w */
+int tpls_do_jacobi_u(double *u3, const double *u3_old, const double *RHS, const double *viscosity,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max);
+int tpls_do_jacobi_v(double *v3, const double *v3_old, const double *RHS, const double *viscosity,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max);
+int tpls_do_jacobi_w(double *w3, const double *w3_old, const double *RHS, const double *viscosity,
+                     double dx, double dy, double dz, double dt, int ex, int ey, int sx, int sy,
+                     int maxn, int ex_max);
+/* jacobi_iteration_allflux.f90:285  get_difference(ua,ub,ex,ey,ez,sx,sy,sz,diff) */
+int tpls_get_difference(const double *ua, const double *ub, int ex, int ey, int ez, int sx, int sy,
+                        int sz, double *diff);
+/* sor_iteration_allflux.f90:19 / 152 / 280  do_sor_u / _v / _w (only do_sor_u
+ * takes ex_max, as in the reference) */
+int tpls_do_sor_u(double *u3, const double *RHS, const double *viscosity, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max,
+                  int flg, int iteration_time);
+int tpls_do_sor_v(double *v3, const double *RHS, const double *viscosity, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int flg,
+                  int iteration_time);
+int tpls_do_sor_w(double *w3, const double *RHS, const double *viscosity, double dx, double dy,
+                  double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int flg,
+                  int iteration_time);
+/* pressure.f90:16  get_source_pres(rhs,u3,v3,w3,ex,ey,sx,sy,maxn,dx,dy,dz,dt) */
+int tpls_get_source_pres(double *rhs, const double *u3, const double *v3, const double *w3, int ex,
+                         int ey, int sx, int sy, int maxn, double dx, double dy, double dz,
+                         double dt);
+/* pressure.f90:59  get_uvw_pres(u3,v3,w3,p,ex,ey,sx,sy,maxn,ex_max,dx,dy,dz,dt,u_inlet) */
+int tpls_get_uvw_pres(double *u3, double *v3, double *w3, const double *p, int ex, int ey, int sx,
+                      int sy, int maxn, int ex_max, double dx, double dy, double dz, double dt,
+                      const double *u_inlet);
+/* The pressure solve main_ns_hybrid.F90:896-913 runs through PETSc:
+ * KSPSolve with the callbacks copy_pressure_in_to_petsc (initial guess = pres),
+ * compute_rhs (RHS_p, u3(0,:,:), u_inlet) and compute_matrix
+ * (pressure_solver.F90:205-441), then KSPGetIterationNumber,
+ * KSPGetResidualNorm, KSPGetConvergedReason and copy_pressure_out_of_petsc
+ * (pressure_solver.F90:88-145, 444-486).  This entry point is that whole
+ * sequence: pres holds the initial guess on entry and the solution with its
+ * ghost planes on return. */
+int tpls_pressure_solve(double *pres, const double *RHS_p, const double *u3,
+                        const double *u_inlet, int maxl, int maxm, int maxn, double dx, double dy,
+                        double dz, double dt, double rtol, double abstol, double dtol, int maxits,
+                        int *its, double *rnorm, int *reason);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

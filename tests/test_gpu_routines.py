@@ -1,0 +1,208 @@
+"""Per-routine drop-in entry points (HIP, through the C ABI) against the CPU
+oracle on identical seeded inputs.
+
+Tolerance: FP64 results agree with the oracle to 1e-12 in the normwise
+relative sense max|gpu - oracle| / max|oracle|; integer outputs (istatus)
+must be identical.  The oracle itself is pinned bit-for-bit to the
+reference's Fortran (tests/test_oracle_vs_ref.py).
+"""
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from tests.compare import relerr
+from tests.fields import level_set, rand, shapes, wrap_y
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+MAXL, MAXM, MAXN = 37, 17, 29
+SX, EX, SY, EY, EXMAX = 1, MAXL - 1, 1, MAXM - 1, MAXL - 1
+DX = 1.0 / (MAXN - 1)
+DT = 2e-4
+
+
+@pytest.fixture(scope="module")
+def L():
+    from tpls_amd import _lib
+    return _lib
+
+
+def report(name, **errs):
+    print(f"[{name}] " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+
+
+def test_weno_faces(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(20)
+    phi = level_set(r, MAXL, MAXM, MAXN, DX, noise=0.01, hw=3)
+    u, v, w = rand(r, S["uv"]), rand(r, S["uv"]), rand(r, S["w"])
+    outs_o = [np.zeros(S["uv"], order="F"), np.zeros(S["uv"], order="F"), np.zeros(S["w"], order="F")]
+    outs_g = [a.copy(order="F") for a in outs_o]
+    B.oracle("weno_appox_convection", phi, u, v, w, SX, EX, SY, EY, MAXL, MAXN, *outs_o)
+    L.call("wenoappoxconvection", phi, u, v, w, SX, EX, SY, EY, MAXL, MAXN, *outs_g)
+    errs = {n: relerr(g[:, 1:MAXM, :], o[:, 1:MAXM, :]) for n, g, o in zip(("dfx", "dfy", "dfz"), outs_g, outs_o)}
+    report("weno", **errs)
+    assert max(errs.values()) < TOL
+
+
+def test_advect_phi(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(21)
+    phi = level_set(r, MAXL, MAXM, MAXN, DX, noise=0.01)
+    u, v, w = rand(r, S["uv"]), rand(r, S["uv"]), rand(r, S["w"])
+    co, cg = np.zeros(S["p"], order="F"), np.ones(S["p"], order="F")
+    B.oracle("do_advect_phi", phi, u.copy(order="F"), v.copy(order="F"), w.copy(order="F"), co,
+             SX, EX, SY, EY, MAXL, MAXN, EXMAX, DX, DX, DX)
+    L.call("do_advect_phi", phi, u, v, w, cg, SX, EX, SY, EY, MAXL, MAXN, EXMAX, DX, DX, DX)
+    e = relerr(cg, co)
+    report("advect", conv2_phi=e)
+    assert e < TOL
+
+
+def test_levelset_chain(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(22)
+    phi2 = level_set(r, MAXL, MAXM, MAXN, DX, noise=0.005)
+    cs = [rand(r, S["p"], scale=0.1) for _ in range(3)]
+    p3o, p3g = np.zeros(S["p"], order="F"), np.zeros(S["p"], order="F")
+    eo = B.oracle("do_tpls", phi2, p3o, *cs, SX, EX, SY, EY, MAXN, EXMAX, 10, 0.5, DX, DX, DT)[0]
+    eg = L.call("do_tpls", phi2, p3g, *cs, SX, EX, SY, EY, MAXN, EXMAX, 10, 0.5, DX, DX, DT)[0]
+    e = relerr(p3g[:, 1:MAXM], p3o[:, 1:MAXM])
+    report("do_tpls", phi3=e, err2=abs(eg - eo) / abs(eo))
+    assert e < TOL
+    assert eg == pytest.approx(eo, rel=1e-9)
+
+    # status + one iteration on their own; main_ns_hybrid.F90 exchanges phi3
+    # before these calls (two_phase_levelset.f90:100-101)
+    wrap_y(p3o, 1)
+    D, ist, sg = np.zeros(S["p"], order="F"), np.zeros(S["p"], order="F", dtype=np.int32), np.zeros(S["p"], order="F")
+    Dg, istg, sgg = D.copy(order="F"), ist.copy(order="F"), sg.copy(order="F")
+    B.oracle("get_levelset_status", p3o, D, ist, sg, SX, EX, SY, EY, MAXN, DX)
+    L.call("get_levelset_status", p3o, Dg, istg, sgg, SX, EX, SY, EY, MAXN, DX)
+    assert np.array_equal(ist, istg)
+    assert np.array_equal(sg, sgg)
+    assert relerr(Dg, D) < TOL
+    pr = wrap_y(np.asfortranarray(p3o + 0.01 * r.standard_normal(p3o.shape)), 1)
+    pro, prg = pr.copy(order="F"), pr.copy(order="F")
+    e1 = B.oracle("do_levelset_iteration", p3o, pro, D, ist, sg, SX, EX, SY, EY, MAXN, EXMAX, DX, 0.5)[0]
+    e2 = L.call("do_levelset_iteration", p3o, prg, D, ist, sg, SX, EX, SY, EY, MAXN, EXMAX, DX, 0.5)[0]
+    assert relerr(prg[:, 1:MAXM], pro[:, 1:MAXM]) < TOL
+    assert e2 == pytest.approx(e1, rel=1e-9)
+
+
+def test_csf(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(23)
+    phi = level_set(r, MAXL, MAXM, MAXN, DX, amp=0.08, noise=0.001)
+    f0 = rand(r, S["p"])
+    o = [f0.copy(order="F") for _ in range(3)]
+    g = [f0.copy(order="F") for _ in range(3)]
+    B.oracle("get_csf", *o, phi, SX, EX, SY, EY, MAXN, EXMAX, DX, DX, DX, 1.0, 1.5 * DX)
+    L.call("get_csf", *g, phi, SX, EX, SY, EY, MAXN, EXMAX, DX, DX, DX, 1.0, 1.5 * DX)
+    errs = {n: relerr(a[:, 1:MAXM], b[:, 1:MAXM]) for n, a, b in zip(("fx", "fy", "fz"), g, o)}
+    report("csf", **errs)
+    assert max(errs.values()) < TOL
+
+
+def test_viscosity(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(24)
+    phi = level_set(r, MAXL, MAXM, MAXN, DX, noise=0.01)
+    m0 = rand(r, S["p"])
+    mo, mg = m0.copy(order="F"), m0.copy(order="F")
+    B.oracle("get_viscosity", mo, phi, EX, EY, SX, SY, MAXN, 1.5 * DX, 500.0, 1.0, 10.0)
+    L.call("get_viscosity", mg, phi, EX, EY, SX, SY, MAXN, 1.5 * DX, 500.0, 1.0, 10.0)
+    e = relerr(mg, mo)
+    report("viscosity", mu=e)
+    assert e < TOL
+
+
+@pytest.mark.parametrize("name", ["get_conv_all", "get_diffusion_all"])
+def test_momentum(L, name):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(25)
+    mu = wrap_y(np.asfortranarray(0.01 + 0.01 * r.random(S["aug"])), 2)
+    u, v, w = rand(r, S["uv"]), rand(r, S["uv"]), rand(r, S["w"])
+    o = [np.zeros(S["uv"], order="F"), np.zeros(S["uv"], order="F"), np.zeros(S["w"], order="F")]
+    g = [np.full(S["uv"], 7.0, order="F"), np.full(S["uv"], 7.0, order="F"), np.full(S["w"], 7.0, order="F")]
+    B.oracle(name, *o, mu, u, v, w, SX, SY, EX, EY, MAXN, EXMAX, DX, DX, DX)
+    L.call(name, *g, mu, u, v, w, SX, SY, EX, EY, MAXN, EXMAX, DX, DX, DX)
+    errs = {c: relerr(a, b) for c, a, b in zip("uvw", g, o)}
+    report(name, **errs)
+    assert max(errs.values()) < TOL
+
+
+@pytest.mark.parametrize("comp", ["u", "v", "w"])
+def test_jacobi_and_sor(L, comp):
+    S = shapes(MAXL, MAXM, MAXN)
+    shp = S["w"] if comp == "w" else S["uv"]
+    r = np.random.default_rng(26)
+    mu = wrap_y(np.asfortranarray(0.01 + 0.01 * r.random(S["aug"])), 2)
+    R = rand(r, shp)
+    x0 = rand(r, shp)
+    xo = rand(r, shp)
+    a, b = x0.copy(order="F"), x0.copy(order="F")
+    B.oracle("do_jacobi_" + comp, a, xo, R, mu, DX, DX, DX, DT, EX, EY, SX, SY, MAXN, EXMAX)
+    L.call("do_jacobi_" + comp, b, xo, R, mu, DX, DX, DX, DT, EX, EY, SX, SY, MAXN, EXMAX)
+    ej = relerr(b[:, 1:MAXM], a[:, 1:MAXM])
+    es = []
+    for it in (1, 2, 3):
+        for flg in (0, 1):
+            wrap_y(a, 1)
+            wrap_y(b, 1)
+            extra = [EXMAX] if comp == "u" else []
+            args = [DX, DX, DX, DT, EX, EY, SX, SY, MAXN, *extra, flg, it]
+            B.oracle("do_sor_" + comp, a, R, mu, *args)
+            L.call("do_sor_" + comp, b, R, mu, *args)
+            es.append(relerr(b[:, 1:MAXM], a[:, 1:MAXM]))
+    report("helmholtz_" + comp, jacobi=ej, sor=max(es))
+    assert ej < TOL and max(es) < TOL
+
+
+def test_pressure_pieces(L):
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(27)
+    u, v, w = rand(r, S["uv"]), rand(r, S["uv"]), rand(r, S["w"])
+    ro, rg = np.zeros(S["p"], order="F"), np.ones(S["p"], order="F")
+    B.oracle("get_source_pres", ro, u, v, w, EX, EY, SX, SY, MAXN, DX, DX, DX, DT)
+    L.call("get_source_pres", rg, u, v, w, EX, EY, SX, SY, MAXN, DX, DX, DX, DT)
+    e1 = relerr(rg, ro)
+    p = rand(r, S["p"])
+    inl = np.ascontiguousarray(r.standard_normal(MAXN - 1))
+    uo, vo, wo = u.copy(order="F"), v.copy(order="F"), w.copy(order="F")
+    B.oracle("get_uvw_pres", uo, vo, wo, p, EX, EY, SX, SY, MAXN, EXMAX, DX, DX, DX, DT, inl)
+    L.call("get_uvw_pres", u, v, w, p, EX, EY, SX, SY, MAXN, EXMAX, DX, DX, DX, DT, inl)
+    e2 = max(relerr(u, uo), relerr(v, vo), relerr(w, wo))
+    report("pressure", source=e1, projection=e2)
+    assert e1 < TOL and e2 < TOL
+
+
+def test_pressure_solve(L):
+    """KSPSolve replacement: same CG, same stopping rule; the iteration
+    counts agree to within one and the solutions to the solver tolerance."""
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(28)
+    u3 = rand(r, S["uv"], scale=0.01)
+    v3, w3 = rand(r, S["uv"], scale=0.01), rand(r, S["w"], scale=0.01)
+    w3[:, :, 0] = 0.0
+    w3[:, :, -1] = 0.0
+    rhs = np.zeros(S["p"], order="F")
+    B.oracle("get_source_pres", rhs, u3, v3, w3, EX, EY, SX, SY, MAXN, DX, DX, DX, DT)
+    inl = np.ascontiguousarray(0.5 + 0.01 * r.standard_normal(MAXN - 1))
+    p0 = np.zeros(S["p"], order="F")
+    po, pg = p0.copy(order="F"), p0.copy(order="F")
+    import ctypes as C
+    D_ = C.POINTER(C.c_double)
+    its_o, rn_o = C.c_int(), C.c_double()
+    reason_o = B.oracle_lib().lib.orc_pressure_solve(
+        po.ctypes.data_as(D_), rhs.ctypes.data_as(D_), u3.ctypes.data_as(D_), inl.ctypes.data_as(D_),
+        MAXL, MAXM, MAXN, C.c_double(DX), C.c_double(DX), C.c_double(DX), C.c_double(DT),
+        C.c_double(1e-10), C.c_double(1e-50), C.c_double(1e5), 10000, C.byref(its_o), C.byref(rn_o))
+    its_g, rn_g, reason_g = L.pressure_solve(pg, rhs, u3, inl, MAXL, MAXM, MAXN, DX, DX, DX, DT,
+                                             1e-10, 1e-50, 1e5, 10000)
+    e = relerr(pg[:, 1:MAXM], po[:, 1:MAXM])
+    report("pressure_solve", its_oracle=its_o.value, its_gpu=its_g, p=e)
+    assert reason_o == reason_g == 2
+    assert abs(its_o.value - its_g) <= 1
+    assert e < 1e-8

@@ -1,0 +1,105 @@
+"""A device-resident TPLS case: all state arrays of main_ns_hybrid.F90 live in
+HBM and ``step()`` advances whole timesteps on the GPU."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .config import CaseParams
+
+
+def field_shape(name: str, maxl: int, maxm: int, maxn: int):
+    """Fortran extents of a case field (main_ns_hybrid.F90:416-549)."""
+    if name == "u_inlet":
+        return (maxn - 1,)
+    if name == "visc2_aug":
+        return (maxl + 3, maxm + 3, maxn + 1)
+    if name in ("w2", "w3") or name.endswith("_w") or name.startswith("csf_w"):
+        return (maxl + 1, maxm + 1, maxn)
+    if name in ("phi2", "phi3", "pres", "pres_old", "conv0_phi", "conv1_phi", "conv2_phi",
+                "fx_csf", "fy_csf", "fz_csf", "visc2", "rhs_p"):
+        return (maxl + 1, maxm + 1, maxn + 1)
+    return (maxl + 1, maxm + 1, maxn - 1)
+
+
+def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True) -> _lib.Params:
+    prm = _lib.Params()
+    for k, v in p.solver_dict().items():
+        setattr(prm, k, v)
+    prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
+    prm.diagnostics = 1 if diagnostics else 0
+    return prm
+
+
+class Case:
+    def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True):
+        params.check()
+        self.params = params
+        self.precision = precision
+        self.shape = (params.maxl, params.maxm, params.maxn)
+        self._prm = make_params(params, precision, diagnostics)
+        h = C.c_void_p()
+        L = _lib.lib()
+        _lib.check(L.tpls_case_create(C.byref(self._prm), *self.shape, C.byref(h)), "tpls_case_create")
+        self._h = h
+
+    @staticmethod
+    def device_bytes(params: CaseParams, precision: str = "fp64") -> int:
+        return int(_lib.lib().tpls_case_bytes(params.maxl, params.maxm, params.maxn,
+                                              {"fp64": 0, "fp32": 1}[precision]))
+
+    def load(self, u, v, w, pres, phi) -> None:
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (u, v, w, pres, phi)]
+        _lib.check(_lib.lib().tpls_case_load(self._h, *[a.ctypes.data_as(_lib._D) for a in arrs]),
+                   "tpls_case_load")
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.zeros(field_shape(name, *self.shape), order="F")
+        _lib.check(_lib.lib().tpls_case_get_field(self._h, _lib.FIELD_ID[name],
+                                                  out.ctypes.data_as(_lib._D)), f"get {name}")
+        return out
+
+    def set_field(self, name: str, arr) -> None:
+        a = np.asfortranarray(arr, dtype=np.float64)
+        if a.shape != field_shape(name, *self.shape):
+            raise ValueError(f"{name}: shape {a.shape} != {field_shape(name, *self.shape)}")
+        _lib.check(_lib.lib().tpls_case_set_field(self._h, _lib.FIELD_ID[name],
+                                                  a.ctypes.data_as(_lib._D)), f"set {name}")
+
+    def step(self, n: int = 1) -> None:
+        _lib.check(_lib.lib().tpls_step(self._h, int(n)), "tpls_step")
+
+    def diag(self) -> dict:
+        d = _lib.Diag()
+        _lib.check(_lib.lib().tpls_case_diag(self._h, C.byref(d)), "tpls_case_diag")
+        return {f: getattr(d, f) for f, _ in _lib.Diag._fields_}
+
+    def sync(self) -> None:
+        _lib.check(_lib.lib().tpls_case_sync(self._h), "tpls_case_sync")
+
+    def time_kernels(self, enable: bool) -> None:
+        _lib.check(_lib.lib().tpls_case_time_kernels(self._h, 1 if enable else 0), "time_kernels")
+
+    def kernel_time(self):
+        ms, n = C.c_double(0), C.c_long(0)
+        _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().tpls_case_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1220 @@
+// Host side of the MI355X-native TPLS stencil core: the device-resident case
+// (the timestep of main_ns_hybrid.F90:583-918, 2480-2509 as a sequence of
+// kernel launches on one HIP stream) and the C ABI of include/tpls_core.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tpls_core.h"
+#include "tpls_kernels.hpp"
+
+using namespace tpls;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(TPLS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Fortran bounds of a field (main_ns_hybrid.F90:416-549).
+struct Shape {
+    int lx, ux, ly, uy, lz, uz;
+    long nx() const { return ux - lx + 1; }
+    long ny() const { return uy - ly + 1; }
+    long nz() const { return uz - lz + 1; }
+    long count() const { return nx() * ny() * nz(); }
+};
+
+enum Grid3 { G_UV, G_W, G_P, G_AUG, G_INLET };
+
+int grid_of(int f)
+{
+    switch (f) {
+    case TPLS_F_W2: case TPLS_F_W3: case TPLS_F_CONV0_W: case TPLS_F_CONV1_W: case TPLS_F_CONV2_W:
+    case TPLS_F_CSF_W0: case TPLS_F_CSF_W1: case TPLS_F_CSF_W2: case TPLS_F_DIFF_W: case TPLS_F_RHS_W:
+        return G_W;
+    case TPLS_F_PHI2: case TPLS_F_PHI3: case TPLS_F_PRES: case TPLS_F_PRES_OLD: case TPLS_F_CONV0_PHI:
+    case TPLS_F_CONV1_PHI: case TPLS_F_CONV2_PHI: case TPLS_F_FX_CSF: case TPLS_F_FY_CSF: case TPLS_F_FZ_CSF:
+    case TPLS_F_VISC2: case TPLS_F_RHS_P:
+        return G_P;
+    case TPLS_F_VISC2_AUG: return G_AUG;
+    case TPLS_F_U_INLET: return G_INLET;
+    default: return G_UV;
+    }
+}
+
+Shape shape_of(int grid, int maxl, int maxm, int maxn)
+{
+    switch (grid) {
+    case G_UV: return {0, maxl, 0, maxm, 0, maxn - 2};
+    case G_W: return {0, maxl, 0, maxm, 0, maxn - 1};
+    case G_P: return {0, maxl, 0, maxm, 0, maxn};
+    case G_AUG: return {-1, maxl + 1, -1, maxm + 1, 0, maxn};
+    default: return {0, maxn - 2, 0, 0, 0, 0};
+    }
+}
+
+// Device allocation of a Fortran-shaped field with a padded x pitch.
+struct DevArray {
+    void* ptr = nullptr;
+    Shape sh{};
+    long pitch = 0;  // elements per (j,k) row
+    int esz = 8;
+    long sy() const { return pitch; }
+    long sz() const { return pitch * sh.ny(); }
+    size_t bytes() const { return size_t(pitch) * sh.ny() * sh.nz() * esz; }
+    template <typename T>
+    FieldView<T> view() const
+    {
+        FieldView<T> v;
+        v.sy = sy();
+        v.sz = sz();
+        v.p = static_cast<T*>(ptr) - sh.lx - long(sh.ly) * v.sy - long(sh.lz) * v.sz;
+        return v;
+    }
+};
+
+long pitch_for(long nx, int esz) { return (nx * esz + 127) / 128 * 128 / esz; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The case
+// ---------------------------------------------------------------------------
+struct tpls_case {
+    int maxl, maxm, maxn, prec;
+    tpls_params prm;
+    Grid g;
+    Phys ph;
+    hipStream_t s = nullptr;
+    int esz;
+    std::vector<DevArray*> owned;
+
+    // state; x[c][0..2] are the three buffers of component c (u,v,w):
+    // x2 (the state) and the two Helmholtz iterates
+    DevArray* x[3][3];
+    int i2[3];
+    int i3[3];          // index of the current x3 (after the last step)
+    DevArray* conv[3][3];  // [comp][0,1,2]
+    DevArray* csf[3][3];
+    DevArray* cphi[3];
+    DevArray* rhs[3];
+    DevArray *phi2, *phi3, *pra, *prb, *D, *sg, *ist;
+    DevArray *fx, *fy, *fz, *mu;
+    DevArray* pres;   // storage precision: outlet BC and projection
+    DevArray* pold;   // double: CG unknown, pres_old
+    DevArray *cg_r, *cg_p0, *cg_p1;
+    DevArray* inlet;
+    double* red_part = nullptr;
+    double* red_out = nullptr;
+    unsigned* red_cnt = nullptr;
+    CgState* cg = nullptr;
+    CgState* h_cg = nullptr;   // pinned host mirror
+    double* h_out = nullptr;   // pinned host mirror of red_out
+    tpls_diag diag{};
+    long steps_done = 0;
+    int last_its = 0;
+    // kernel timing of cg_x
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double t_ms = 0.0;
+    long t_n = 0;
+};
+
+namespace {
+
+int alloc_field(tpls_case* c, DevArray** out, const Shape& sh, int esz, bool zero = true)
+{
+    DevArray* a = new DevArray;
+    a->sh = sh;
+    a->esz = esz;
+    a->pitch = pitch_for(sh.nx(), esz);
+    hipError_t e = hipMalloc(&a->ptr, a->bytes());
+    if (e != hipSuccess) {
+        delete a;
+        return fail(TPLS_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    c->owned.push_back(a);
+    if (zero) HIPCHK(hipMemsetAsync(a->ptr, 0, a->bytes(), c->s));
+    *out = a;
+    return TPLS_OK;
+}
+
+Phys phys_of(const tpls_params& p)
+{
+    return Phys{p.dx, p.dy, p.dz, p.dt, p.Re, p.mu_plus, p.mu_minus, p.height, p.smooth_width, p.scap};
+}
+
+// host <-> device copies of a field, converting to/from the storage type
+int put_array(tpls_case* c, DevArray* a, const double* host)
+{
+    const Shape& sh = a->sh;
+    const size_t row = sh.nx();
+    if (a->esz == 8) {
+        HIPCHK(hipMemcpy2DAsync(a->ptr, a->pitch * 8, host, row * 8, row * 8, sh.ny() * sh.nz(), hipMemcpyHostToDevice,
+                                c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    } else {
+        std::vector<float> tmp(sh.count());
+        for (long q = 0; q < sh.count(); ++q) tmp[q] = float(host[q]);
+        HIPCHK(hipMemcpy2DAsync(a->ptr, a->pitch * 4, tmp.data(), row * 4, row * 4, sh.ny() * sh.nz(),
+                                hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
+    return TPLS_OK;
+}
+
+int get_array(tpls_case* c, const DevArray* a, double* host)
+{
+    const Shape& sh = a->sh;
+    const size_t row = sh.nx();
+    if (a->esz == 8) {
+        HIPCHK(hipMemcpy2DAsync(host, row * 8, a->ptr, a->pitch * 8, row * 8, sh.ny() * sh.nz(), hipMemcpyDeviceToHost,
+                                c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    } else {
+        std::vector<float> tmp(sh.count());
+        HIPCHK(hipMemcpy2DAsync(tmp.data(), row * 4, a->ptr, a->pitch * 4, row * 4, sh.ny() * sh.nz(),
+                                hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (long q = 0; q < sh.count(); ++q) host[q] = double(tmp[q]);
+    }
+    return TPLS_OK;
+}
+
+// periodic y wrap of halo rows (what exchange2d* leaves in them)
+void wrap_rows(double* h, const Shape& sh, int maxm)
+{
+    const long nx = sh.nx(), ny = sh.ny();
+    const int P = maxm - 1;
+    for (long k = 0; k < sh.nz(); ++k)
+        for (int j = sh.ly; j <= sh.uy; ++j) {
+            if (j >= 1 && j <= P) continue;
+            int src = j < 1 ? j + P : j - P;
+            std::memcpy(h + (k * ny + (j - sh.ly)) * nx, h + (k * ny + (src - sh.ly)) * nx, nx * sizeof(double));
+        }
+}
+
+template <typename T>
+int step_impl(tpls_case* c);
+
+DevArray* field_ptr(tpls_case* c, int f)
+{
+    switch (f) {
+    case TPLS_F_U2: return c->x[0][c->i2[0]];
+    case TPLS_F_V2: return c->x[1][c->i2[1]];
+    case TPLS_F_W2: return c->x[2][c->i2[2]];
+    case TPLS_F_U3: return c->x[0][c->i3[0]];
+    case TPLS_F_V3: return c->x[1][c->i3[1]];
+    case TPLS_F_W3: return c->x[2][c->i3[2]];
+    case TPLS_F_PHI2: return c->phi2;
+    case TPLS_F_PHI3: return c->phi3;
+    case TPLS_F_PRES: return c->pres;
+    case TPLS_F_PRES_OLD: return c->pold;
+    case TPLS_F_CONV0_PHI: return c->cphi[0];
+    case TPLS_F_CONV1_PHI: return c->cphi[1];
+    case TPLS_F_CONV2_PHI: return c->cphi[2];
+    case TPLS_F_FX_CSF: return c->fx;
+    case TPLS_F_FY_CSF: return c->fy;
+    case TPLS_F_FZ_CSF: return c->fz;
+    case TPLS_F_VISC2: return c->mu;
+    case TPLS_F_RHS_U: return c->rhs[0];
+    case TPLS_F_RHS_V: return c->rhs[1];
+    case TPLS_F_RHS_W: return c->rhs[2];
+    case TPLS_F_U_INLET: return c->inlet;
+    default: break;
+    }
+    if (f >= TPLS_F_CONV0_U && f <= TPLS_F_CONV2_W) {
+        int q = f - TPLS_F_CONV0_U;
+        return c->conv[q / 3][q % 3];
+    }
+    if (f >= TPLS_F_CSF_U0 && f <= TPLS_F_CSF_W2) {
+        int q = f - TPLS_F_CSF_U0;
+        return c->csf[q / 3][q % 3];
+    }
+    return nullptr;  // VISC2_AUG, RHS_P, DIFF_*: not kept by the case
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* tpls_last_error(void) { return g_err.c_str(); }
+
+size_t tpls_case_bytes(int maxl, int maxm, int maxn, int precision)
+{
+    const int esz = precision == TPLS_FP32 ? 4 : 8;
+    const size_t rows = size_t(maxm) + 1;
+    const size_t pt = pitch_for(maxl + 1, esz), pd = pitch_for(maxl + 1, 8);
+    const size_t uv = pt * rows * (maxn - 1), w = pt * rows * maxn, p = pt * rows * (maxn + 1);
+    size_t n = 0;
+    n += (6 * uv + 3 * w) * esz;             // u,v,w x 3 buffers
+    n += (6 * uv + 3 * w) * esz * 2;         // conv + csf histories
+    n += (2 * uv + w) * esz;                 // rhs
+    n += p * esz * (3 + 4 + 3 + 4 + 1);      // cphi, phi2/3/ra/rb, D/sg/ist, fx/fy/fz, mu
+    n += p * esz;                            // pres
+    n += pd * rows * (maxn + 1) * 8 * 4;     // pold + CG vectors
+    return n + (1 << 20);
+}
+
+int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_case** out)
+{
+    if (!prm || !out) return fail(TPLS_ERR_ARG, "null argument");
+    if (maxl < 6 || maxm < 5 || maxn < 6) return fail(TPLS_ERR_ARG, "grid too small");
+    if ((maxm - 1) % 2) return fail(TPLS_ERR_ARG, "maxm-1 must be even (red-black colouring across the periodic seam)");
+    if (prm->mom_u < 2 || prm->mom_v < 2 || prm->mom_w < 2) return fail(TPLS_ERR_ARG, "mom_* must be >= 2");
+    if (prm->precision != TPLS_FP64 && prm->precision != TPLS_FP32) return fail(TPLS_ERR_ARG, "bad precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TPLS_ERR_NOGPU, "no HIP device");
+    tpls_case* c = new tpls_case;
+    c->maxl = maxl;
+    c->maxm = maxm;
+    c->maxn = maxn;
+    c->prm = *prm;
+    c->prec = prm->precision;
+    c->esz = c->prec == TPLS_FP32 ? 4 : 8;
+    c->g = Grid{maxl, maxm, maxn, maxm - 1};
+    c->ph = phys_of(*prm);
+    int rc = TPLS_OK;
+    auto bail = [&](int r) {
+        tpls_case_destroy(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "stream"));
+    const Shape suv = shape_of(G_UV, maxl, maxm, maxn), sw = shape_of(G_W, maxl, maxm, maxn);
+    const Shape sp = shape_of(G_P, maxl, maxm, maxn), si = shape_of(G_INLET, maxl, maxm, maxn);
+    const int e = c->esz;
+    for (int q = 0; q < 3; ++q) {
+        const Shape& s = q == 2 ? sw : suv;
+        for (int b = 0; b < 3; ++b) {
+            if ((rc = alloc_field(c, &c->x[q][b], s, e))) return bail(rc);
+            if ((rc = alloc_field(c, &c->conv[q][b], s, e))) return bail(rc);
+            if ((rc = alloc_field(c, &c->csf[q][b], s, e))) return bail(rc);
+        }
+        if ((rc = alloc_field(c, &c->rhs[q], s, e))) return bail(rc);
+        c->i2[q] = 0;
+        c->i3[q] = 1;
+    }
+    for (int b = 0; b < 3; ++b)
+        if ((rc = alloc_field(c, &c->cphi[b], sp, e))) return bail(rc);
+    DevArray** pg[] = {&c->phi2, &c->phi3, &c->pra, &c->prb, &c->D, &c->sg, &c->fx, &c->fy, &c->fz, &c->mu, &c->pres};
+    for (DevArray** a : pg)
+        if ((rc = alloc_field(c, a, sp, e))) return bail(rc);
+    if ((rc = alloc_field(c, &c->ist, sp, 4))) return bail(rc);
+    DevArray** pd[] = {&c->pold, &c->cg_r, &c->cg_p0, &c->cg_p1};
+    for (DevArray** a : pd)
+        if ((rc = alloc_field(c, a, sp, 8))) return bail(rc);
+    if ((rc = alloc_field(c, &c->inlet, si, e))) return bail(rc);
+    const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);
+    if (hipMalloc(&c->red_part, sizeof(double) * 3 * nb) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "partials"));
+    if (hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "outs"));
+    if (hipMalloc(&c->red_cnt, sizeof(unsigned) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "counter"));
+    if (hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "cg state"));
+    if (hipHostMalloc(&c->h_cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
+    if (hipHostMalloc(&c->h_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
+    if (hipMemsetAsync(c->red_cnt, 0, sizeof(unsigned) * 8, c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "memset"));
+    if (hipMemsetAsync(c->red_out, 0, sizeof(double) * 8, c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "memset"));
+    if (hipStreamSynchronize(c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "sync"));
+    *out = c;
+    return TPLS_OK;
+}
+
+int tpls_case_destroy(tpls_case* c)
+{
+    if (!c) return TPLS_OK;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (DevArray* a : c->owned) {
+        hipFree(a->ptr);
+        delete a;
+    }
+    hipFree(c->red_part);
+    hipFree(c->red_out);
+    hipFree(c->red_cnt);
+    hipFree(c->cg);
+    if (c->h_cg) hipHostFree(c->h_cg);
+    if (c->h_out) hipHostFree(c->h_out);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return TPLS_OK;
+}
+
+int tpls_case_load(tpls_case* c, const double* u, const double* v, const double* w, const double* pres,
+                   const double* phi)
+{
+    if (!c || !u || !v || !w || !pres || !phi) return fail(TPLS_ERR_ARG, "null argument");
+    for (DevArray* a : c->owned) HIPCHK(hipMemsetAsync(a->ptr, 0, a->bytes(), c->s));
+    for (int q = 0; q < 3; ++q) {
+        c->i2[q] = 0;
+        c->i3[q] = 1;
+    }
+    int rc;
+    if ((rc = put_array(c, c->x[0][0], u))) return rc;
+    if ((rc = put_array(c, c->x[1][0], v))) return rc;
+    if ((rc = put_array(c, c->x[2][0], w))) return rc;
+    if ((rc = put_array(c, c->pres, pres))) return rc;
+    if ((rc = put_array(c, c->phi2, phi))) return rc;
+    // u_inlet = u(0,0,:), main_ns_hybrid.F90:412
+    const long nx = c->maxl + 1, ny = c->maxm + 1;
+    std::vector<double> inl(c->maxn - 1);
+    for (int k = 0; k <= c->maxn - 2; ++k) inl[k] = u[k * nx * ny];
+    if ((rc = put_array(c, c->inlet, inl.data()))) return rc;
+    c->steps_done = 0;
+    c->last_its = 0;
+    std::memset(&c->diag, 0, sizeof c->diag);
+    return TPLS_OK;
+}
+
+int tpls_case_get_field(tpls_case* c, int f, double* host)
+{
+    if (!c || !host || f < 0 || f >= TPLS_NFIELDS) return fail(TPLS_ERR_ARG, "bad argument");
+    DevArray* a = field_ptr(c, f);
+    const Shape sh = shape_of(grid_of(f), c->maxl, c->maxm, c->maxn);
+    if (f == TPLS_F_VISC2_AUG) {
+        // the case keeps one viscosity array on the p grid; the augmented
+        // copy of main_ns_hybrid.F90:650-718 holds the same values
+        const Shape sp = shape_of(G_P, c->maxl, c->maxm, c->maxn);
+        std::vector<double> t(sp.count());
+        int rc = get_array(c, c->mu, t.data());
+        if (rc) return rc;
+        std::fill(host, host + sh.count(), 0.0);
+        const int P = c->maxm - 1;
+        for (long k = 0; k < sh.nz(); ++k)
+            for (int j = -1; j <= c->maxm + 1; ++j) {
+                int sj = j < 1 ? j + P : (j > P ? j - P : j);
+                for (int i = 0; i <= c->maxl; ++i)
+                    host[(i + 1) + (j + 1) * sh.nx() + k * sh.nx() * sh.ny()] = t[i + sj * sp.nx() + k * sp.nx() * sp.ny()];
+            }
+        return TPLS_OK;
+    }
+    if (f == TPLS_F_PRES_OLD && c->steps_done > 0) a = c->pres;  // pres_old = pres after every step
+    if (!a) return fail(TPLS_ERR_ARG, "field not kept by the device case");
+    int rc = get_array(c, a, host);
+    if (rc) return rc;
+    if (grid_of(f) != G_INLET) wrap_rows(host, sh, c->maxm);
+    return TPLS_OK;
+}
+
+int tpls_case_set_field(tpls_case* c, int f, const double* host)
+{
+    if (!c || !host || f < 0 || f >= TPLS_NFIELDS) return fail(TPLS_ERR_ARG, "bad argument");
+    DevArray* a = field_ptr(c, f);
+    if (!a) return fail(TPLS_ERR_ARG, "field not kept by the device case");
+    if (f == TPLS_F_PRES_OLD) {
+        // the CG unknown is kept in double precision
+        return put_array(c, c->pold, host);
+    }
+    return put_array(c, a, host);
+}
+
+int tpls_case_diag(tpls_case* c, tpls_diag* out)
+{
+    if (!c || !out) return fail(TPLS_ERR_ARG, "null argument");
+    *out = c->diag;
+    return TPLS_OK;
+}
+
+int tpls_case_sync(tpls_case* c)
+{
+    if (!c) return fail(TPLS_ERR_ARG, "null case");
+    HIPCHK(hipStreamSynchronize(c->s));
+    return TPLS_OK;
+}
+
+int tpls_case_time_kernels(tpls_case* c, int enable)
+{
+    if (!c) return fail(TPLS_ERR_ARG, "null case");
+    c->timing = enable != 0;
+    c->t_ms = 0.0;
+    c->t_n = 0;
+    c->ev_used = 0;
+    return TPLS_OK;
+}
+
+int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
+{
+    if (!c || !ms || !launches) return fail(TPLS_ERR_ARG, "null argument");
+    *ms = c->t_ms;
+    *launches = c->t_n;
+    return TPLS_OK;
+}
+
+int tpls_step(tpls_case* c, int nsteps)
+{
+    if (!c || nsteps < 0) return fail(TPLS_ERR_ARG, "bad argument");
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = c->prec == TPLS_FP32 ? step_impl<float>(c) : step_impl<double>(c);
+        if (rc) return rc;
+    }
+    return TPLS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One timestep
+// ---------------------------------------------------------------------------
+namespace {
+
+Reduce red_of(tpls_case* c, int slot) { return Reduce{c->red_part, c->red_out + slot, c->red_cnt}; }
+
+#define LCHK(x)                                                                              \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) return fail(TPLS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipEvent_t next_event(tpls_case* c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+int harvest_events(tpls_case* c)
+{
+    for (size_t q = 0; q + 1 < c->ev_used; q += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[q], c->ev_pool[q + 1]));
+        c->t_ms += ms;
+        c->t_n += 1;
+    }
+    c->ev_used = 0;
+    return TPLS_OK;
+}
+
+// The pressure solve: KSPSolve replacement (main_ns_hybrid.F90:896-913).
+template <typename T>
+int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p,
+                   int use_rhs, const T* inlet, FieldView<double> x)
+{
+    const Grid& g = c->g;
+    CgState init{};
+    init.rtol = c->prm.ksp_rtol;
+    init.abstol = c->prm.ksp_abstol;
+    init.dtol = c->prm.ksp_dtol;
+    init.maxits = c->prm.ksp_maxits;
+    *c->h_cg = init;
+    LCHK(hipMemcpyAsync(c->cg, c->h_cg, sizeof(CgState), hipMemcpyHostToDevice, c->s));
+    FieldView<double> r = c->cg_r->view<double>();
+    LCHK(launch_cg_setup_b<T>(c->s, g, c->ph, u3, v3, w3, inlet, rhs_p, use_rhs, r, red_of(c, 7), c->cg));
+    LCHK(launch_cg_setup_r<T>(c->s, g, c->ph, x, r, red_of(c, 7), c->cg));
+    DevArray* pb[2] = {c->cg_p0, c->cg_p1};
+    int cur = 0;
+    long launched = 0;
+    long chunk = std::max(8, c->last_its - 2);
+    for (;;) {
+        chunk = std::min<long>(chunk, std::max<long>(1, c->prm.ksp_maxits - launched));
+        for (long q = 0; q < chunk; ++q) {
+            FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
+            LCHK(launch_cg_p(c->s, g, c->ph, r, po, pn, red_of(c, 7), c->cg));
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (c->timing) {
+                e0 = next_event(c);
+                LCHK(hipEventRecord(e0, c->s));
+            }
+            LCHK(launch_cg_x(c->s, g, c->ph, pn, x, r, red_of(c, 7), c->cg));
+            if (c->timing) {
+                e1 = next_event(c);
+                LCHK(hipEventRecord(e1, c->s));
+            }
+            cur ^= 1;
+        }
+        launched += chunk;
+        LCHK(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgState), hipMemcpyDeviceToHost, c->s));
+        LCHK(hipStreamSynchronize(c->s));
+        if (c->timing) {
+            // drop pairs recorded after convergence (those launches returned at once)
+            long live = std::max<long>(0, c->h_cg->its - (launched - chunk));
+            if (live < chunk) c->ev_used -= size_t(2 * (chunk - live));
+            int rc = harvest_events(c);
+            if (rc) return rc;
+        }
+        if (c->h_cg->done || launched >= c->prm.ksp_maxits) break;
+        chunk = 16;
+    }
+    c->diag.ksp_its = c->h_cg->its;
+    c->diag.ksp_reason = c->h_cg->reason;
+    c->diag.ksp_rnorm = c->h_cg->dp;
+    c->last_its = c->h_cg->its;
+    return TPLS_OK;
+}
+
+template <typename T>
+int step_impl(tpls_case* c)
+{
+    const Grid& g = c->g;
+    const Phys& ph = c->ph;
+    hipStream_t s = c->s;
+    const bool dg = c->prm.diagnostics != 0;
+    auto V = [](DevArray* a) { return a->view<T>(); };
+
+    // ---- do_advect_phi (main:585) ----
+    LCHK(launch_advect_phi<T>(s, g, ph, V(c->phi2), V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
+                              V(c->cphi[2])));
+    // ---- do_tpls (main:596) ----
+    LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
+    LCHK(launch_phi_bc<T>(s, g, ph.dz, ph.height, V(c->phi3), 1));
+    LCHK(launch_levelset_status<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg)));
+    {
+        DevArray* src = c->phi3;
+        DevArray* bufs[2] = {c->pra, c->prb};
+        const int nit = c->prm.levelset_its;
+        for (int it = 1; it <= nit; ++it) {
+            DevArray* dst = bufs[(it - 1) & 1];
+            LCHK(launch_levelset_iter<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg), V(src), V(dst)));
+            // levelset.f90:168-216 (h = dx is passed as dz there)
+            LCHK(launch_phi_bc<T>(s, g, ph.dx, ph.height, V(dst), 1));
+            if (dg && it == nit) LCHK(launch_levelset_err<T>(s, g, V(src), V(dst), red_of(c, 0)));
+            src = dst;
+        }
+        if (nit > 0) {
+            // phi3 <- phi_reinit: the final iterate becomes phi3
+            DevArray* old = c->phi3;
+            c->phi3 = src;
+            if (src == c->pra) c->pra = old;
+            else c->prb = old;
+        }
+    }
+    // ---- CSF (main:618-627) ----
+    LCHK(launch_csf<T>(s, g, ph, V(c->phi3), V(c->fx), V(c->fy), V(c->fz)));
+    LCHK(launch_csf_bc<T>(s, g, V(c->fx), V(c->fy), V(c->fz)));
+    // ---- viscosity incl. inlet/outlet planes (main:646-710) ----
+    LCHK(launch_viscosity<T>(s, g, ph, V(c->phi2), V(c->mu), 0, g.maxl));
+    // ---- conv, diffusion, CSF faces, AB3/CN right-hand sides (main:629-641, 732-757) ----
+    DevArray* fcs[3] = {c->fx, c->fy, c->fz};
+    for (int q = 0; q < 3; ++q)
+        LCHK(launch_momentum<T>(s, g, ph, q, 4, V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
+                                V(c->mu), V(c->conv[q][2]), V(c->rhs[q]), V(fcs[q]), V(c->conv[q][0]),
+                                V(c->conv[q][1]), V(c->csf[q][0]), V(c->csf[q][1]), V(c->csf[q][2]), V(c->rhs[q])));
+    // ---- Helmholtz outer loops (main:758-887) ----
+    const int mom[3] = {c->prm.mom_u, c->prm.mom_v, c->prm.mom_w};
+    int fin[3];
+    for (int q = 0; q < 3; ++q) {
+        int others[2], n = 0;
+        for (int b = 0; b < 3; ++b)
+            if (b != c->i2[q]) others[n++] = b;
+        int srcb = others[0];
+        int dstb = others[1];
+        for (int it = 1; it <= mom[q] / 2; ++it) {
+            FieldView<T> xs = V(c->x[q][srcb]), xd = V(c->x[q][dstb]);
+            LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xd, it == 1));
+            LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
+            LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), 0, it));
+            LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), 1, it));
+            LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
+            std::swap(srcb, dstb);
+        }
+        fin[q] = srcb;  // the last destination
+        if (dg) {
+            // get_difference(x3, x3_old); at it == 1 x3_old is zero
+            const int nz = q == 2 ? g.maxn : g.maxn - 1;
+            if (mom[q] / 2 >= 2)
+                LCHK(launch_difference<T>(s, g, nz, V(c->x[q][srcb]), V(c->x[q][dstb]), red_of(c, 1 + q)));
+        }
+    }
+    // ---- divergence + pressure solve + projection (main:889-918) ----
+    FieldView<T> u3 = V(c->x[0][fin[0]]), v3 = V(c->x[1][fin[1]]), w3 = V(c->x[2][fin[2]]);
+    const T* inlet = static_cast<const T*>(c->inlet->ptr);
+    int rc = pressure_solve<T>(c, u3, v3, w3, c->cg_r->view<double>(), 0, inlet, c->pold->view<double>());
+    if (rc) return rc;
+    LCHK(launch_pres_ghosts<T>(s, g, ph, c->pold->view<double>(), u3, inlet, V(c->pres)));
+    LCHK(launch_uvw_pres<T>(s, g, ph, V(c->pres), inlet, u3, v3, w3));
+    // ---- history rotation (main:2480-2509) ----
+    for (int q = 0; q < 3; ++q) {
+        DevArray* t = c->conv[q][0];
+        c->conv[q][0] = c->conv[q][1];
+        c->conv[q][1] = c->conv[q][2];
+        c->conv[q][2] = t;
+        t = c->csf[q][0];
+        c->csf[q][0] = c->csf[q][1];
+        c->csf[q][1] = c->csf[q][2];
+        c->csf[q][2] = t;
+        c->i3[q] = fin[q];
+        c->i2[q] = fin[q];  // u2 <- u3
+    }
+    {
+        DevArray* t = c->cphi[0];
+        c->cphi[0] = c->cphi[1];
+        c->cphi[1] = c->cphi[2];
+        c->cphi[2] = t;
+        std::swap(c->phi2, c->phi3);  // phi2 <- phi3
+    }
+    if (dg) {
+        LCHK(hipMemcpyAsync(c->h_out, c->red_out, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
+        LCHK(hipStreamSynchronize(s));
+        c->diag.err_phi = c->h_out[0];
+        c->diag.err_u = c->h_out[1];
+        c->diag.err_v = c->h_out[2];
+        c->diag.err_w = c->h_out[3];
+    }
+    c->steps_done += 1;
+    return TPLS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Per-routine drop-in entry points: host arrays in the reference's shapes are
+// staged through per-thread device buffers, the same kernels run, results are
+// copied back.  Arrays a routine only partly writes are uploaded first so
+// that the elements the reference leaves untouched keep the caller's values.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DropCtx {
+    hipStream_t s = nullptr;
+    std::vector<void*> buf;
+    std::vector<size_t> cap;
+    double* red_part = nullptr;
+    double* red_out = nullptr;
+    unsigned* red_cnt = nullptr;
+    double* h_out = nullptr;
+    // cached device case for tpls_pressure_solve
+    tpls_case* pcase = nullptr;
+    int pl = 0, pm = 0, pn = 0;
+    ~DropCtx()
+    {
+        if (pcase) tpls_case_destroy(pcase);
+        for (void* p : buf) hipFree(p);
+        hipFree(red_part);
+        hipFree(red_out);
+        hipFree(red_cnt);
+        if (h_out) hipHostFree(h_out);
+        if (s) hipStreamDestroy(s);
+    }
+};
+
+thread_local DropCtx g_drop;
+
+int drop_init()
+{
+    DropCtx& d = g_drop;
+    if (d.s) return TPLS_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TPLS_ERR_NOGPU, "no HIP device");
+    HIPCHK(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&d.red_part, sizeof(double) * 3 * REDUCE_BLOCKS));
+    HIPCHK(hipMalloc(&d.red_out, sizeof(double) * 8));
+    HIPCHK(hipMalloc(&d.red_cnt, sizeof(unsigned) * 8));
+    HIPCHK(hipHostMalloc(&d.h_out, sizeof(double) * 8));
+    HIPCHK(hipMemsetAsync(d.red_cnt, 0, sizeof(unsigned) * 8, d.s));
+    return TPLS_OK;
+}
+
+// A staged array: contiguous device copy of a host Fortran array.
+struct Stg {
+    DevArray a;
+    void* host = nullptr;
+    bool out = false;
+    int esz = 8;
+};
+
+// mode: 0 = upload, 1 = zero-fill (whole array written by the routine)
+int stage(int slot, const void* host, const Shape& sh, int esz, int mode, bool out, Stg& st)
+{
+    DropCtx& d = g_drop;
+    if ((int)d.buf.size() <= slot) {
+        d.buf.resize(slot + 1, nullptr);
+        d.cap.resize(slot + 1, 0);
+    }
+    const size_t bytes = size_t(sh.count()) * esz;
+    if (d.cap[slot] < bytes) {
+        if (d.buf[slot]) HIPCHK(hipFree(d.buf[slot]));
+        d.buf[slot] = nullptr;
+        d.cap[slot] = 0;
+        if (hipMalloc(&d.buf[slot], bytes) != hipSuccess) return fail(TPLS_ERR_NOMEM, "staging allocation");
+        d.cap[slot] = bytes;
+    }
+    st.a.ptr = d.buf[slot];
+    st.a.sh = sh;
+    st.a.pitch = sh.nx();
+    st.a.esz = esz;
+    st.host = const_cast<void*>(host);
+    st.out = out;
+    st.esz = esz;
+    if (mode == 0)
+        HIPCHK(hipMemcpyAsync(st.a.ptr, host, bytes, hipMemcpyHostToDevice, d.s));
+    else
+        HIPCHK(hipMemsetAsync(st.a.ptr, 0, bytes, d.s));
+    return TPLS_OK;
+}
+
+int unstage(std::initializer_list<Stg*> outs)
+{
+    DropCtx& d = g_drop;
+    for (Stg* st : outs)
+        HIPCHK(hipMemcpyAsync(st->host, st->a.ptr, size_t(st->a.sh.count()) * st->esz, hipMemcpyDeviceToHost, d.s));
+    HIPCHK(hipStreamSynchronize(d.s));
+    return TPLS_OK;
+}
+
+int check_decomp(int sx, int ex, int sy, int ey, int ex_max)
+{
+    if (sx != 1 || sy != 1 || ex != ex_max)
+        return fail(TPLS_ERR_DECOMP, "single-rank decomposition required: sx = sy = 1, ex = ex_max = maxl-1");
+    if (ey < 4 || ex < 4 || (ey % 2) != 0)
+        return fail(TPLS_ERR_DECOMP, "ey = maxm-1 must be even and >= 4; ex >= 4");
+    return TPLS_OK;
+}
+
+Phys phys_d(double dx, double dy, double dz, double dt)
+{
+    Phys p{};
+    p.dx = dx;
+    p.dy = dy;
+    p.dz = dz;
+    p.dt = dt;
+    return p;
+}
+
+#define RC(x)                 \
+    do {                      \
+        int r_ = (x);         \
+        if (r_) return r_;    \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int tpls_wenoappoxconvection(const double* phi, const double* u, const double* v, const double* w, int sx, int ex,
+                             int sy, int ey, int l, int n, double* dfx, double* dfy, double* dfz)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, l - 1));
+    if (!phi || !u || !v || !w || !dfx || !dfy || !dfz) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = l, maxm = ey + 1, maxn = n;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    Stg a_phi, a_u, a_v, a_w, a_x, a_y, a_z;
+    RC(stage(0, phi, {-2, maxl + 2, -2, maxm + 2, 0, maxn}, 8, 0, false, a_phi));
+    RC(stage(1, u, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, a_u));
+    RC(stage(2, v, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, a_v));
+    RC(stage(3, w, shape_of(G_W, maxl, maxm, maxn), 8, 0, false, a_w));
+    RC(stage(4, dfx, shape_of(G_UV, maxl, maxm, maxn), 8, 0, true, a_x));
+    RC(stage(5, dfy, shape_of(G_UV, maxl, maxm, maxn), 8, 0, true, a_y));
+    RC(stage(6, dfz, shape_of(G_W, maxl, maxm, maxn), 8, 0, true, a_z));
+    HIPCHK(launch_weno_faces<double>(g_drop.s, g, a_phi.a.view<double>(), a_u.a.view<double>(), a_v.a.view<double>(),
+                                     a_w.a.view<double>(), a_x.a.view<double>(), a_y.a.view<double>(),
+                                     a_z.a.view<double>()));
+    return unstage({&a_x, &a_y, &a_z});
+}
+
+int tpls_do_advect_phi(double* phi2, double* u2, double* v2, double* w2, double* conv2_phi, int sx, int ex, int sy,
+                       int ey, int maxl, int maxn, int ex_max, double dx, double dy, double dz)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (ex_max != maxl - 1) return fail(TPLS_ERR_DECOMP, "ex_max must be maxl-1");
+    if (!phi2 || !u2 || !v2 || !w2 || !conv2_phi) return fail(TPLS_ERR_ARG, "null array");
+    const int maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    Stg a_phi, a_u, a_v, a_w, a_c;
+    RC(stage(0, phi2, shape_of(G_P, maxl, maxm, maxn), 8, 0, false, a_phi));
+    RC(stage(1, u2, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, a_u));
+    RC(stage(2, v2, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, a_v));
+    RC(stage(3, w2, shape_of(G_W, maxl, maxm, maxn), 8, 0, false, a_w));
+    RC(stage(4, conv2_phi, shape_of(G_P, maxl, maxm, maxn), 8, 1, true, a_c));
+    HIPCHK(launch_advect_phi<double>(g_drop.s, g, phys_d(dx, dy, dz, 0), a_phi.a.view<double>(), a_u.a.view<double>(),
+                                     a_v.a.view<double>(), a_w.a.view<double>(), a_c.a.view<double>()));
+    RC(unstage({&a_c}));
+    // advect_phi.f90:317-322 refresh the y halos of u2, v2, w2 in place
+    wrap_rows(u2, shape_of(G_UV, maxl, maxm, maxn), maxm);
+    wrap_rows(v2, shape_of(G_UV, maxl, maxm, maxn), maxm);
+    wrap_rows(w2, shape_of(G_W, maxl, maxm, maxn), maxm);
+    return TPLS_OK;
+}
+
+int tpls_do_tpls(double* phi2, double* phi3, double* c0, double* c1, double* c2, int sx, int ex, int sy, int ey,
+                 int maxn, int ex_max, int max_it, double height, double* err2, double dx, double dz, double dt)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!phi2 || !phi3 || !c0 || !c1 || !c2 || !err2) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex_max + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, maxn);
+    Stg a2, a3, b0, b1, b2, aD, aS, aI, ra, rb;
+    RC(stage(0, phi2, sp, 8, 0, false, a2));
+    RC(stage(1, phi3, sp, 8, 0, true, a3));
+    RC(stage(2, c0, sp, 8, 0, false, b0));
+    RC(stage(3, c1, sp, 8, 0, false, b1));
+    RC(stage(4, c2, sp, 8, 0, false, b2));
+    RC(stage(5, nullptr, sp, 8, 1, false, aD));
+    RC(stage(6, nullptr, sp, 8, 1, false, aS));
+    RC(stage(7, nullptr, sp, 4, 1, false, aI));
+    RC(stage(8, nullptr, sp, 8, 1, false, ra));
+    RC(stage(9, nullptr, sp, 8, 1, false, rb));
+    hipStream_t s = g_drop.s;
+    Phys ph = phys_d(dx, dx, dz, dt);
+    ph.height = height;
+    HIPCHK(launch_tpls_update<double>(s, g, ph, a2.a.view<double>(), b0.a.view<double>(), b1.a.view<double>(),
+                                      b2.a.view<double>(), a3.a.view<double>()));
+    HIPCHK(launch_phi_bc<double>(s, g, dz, height, a3.a.view<double>(), 1));
+    HIPCHK(launch_levelset_status<double>(s, g, dx, a3.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
+                                          aS.a.view<double>()));
+    Stg* src = &a3;
+    Stg* bufs[2] = {&ra, &rb};
+    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    for (int it = 1; it <= max_it; ++it) {
+        Stg* dst = bufs[(it - 1) & 1];
+        HIPCHK(launch_levelset_iter<double>(s, g, dx, a3.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
+                                            aS.a.view<double>(), src->a.view<double>(), dst->a.view<double>()));
+        HIPCHK(launch_phi_bc<double>(s, g, dx, height, dst->a.view<double>(), 1));
+        if (it == max_it) HIPCHK(launch_levelset_err<double>(s, g, src->a.view<double>(), dst->a.view<double>(), red));
+        src = dst;
+    }
+    if (max_it > 0) {
+        src->host = phi3;
+        src->out = true;
+        RC(unstage({src}));
+        HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *err2 = g_drop.h_out[0];
+    } else {
+        RC(unstage({&a3}));
+    }
+    wrap_rows(phi3, sp, maxm);
+    return TPLS_OK;
+}
+
+int tpls_get_levelset_status(const double* phi, double* D, int* ist, double* sg, int sx, int ex, int sy, int ey,
+                             int n, double h)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex));
+    if (!phi || !D || !ist || !sg) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, n, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, n);
+    Stg a, aD, aI, aS;
+    RC(stage(0, phi, sp, 8, 0, false, a));
+    RC(stage(1, D, sp, 8, 1, true, aD));
+    RC(stage(2, ist, sp, 4, 1, true, aI));
+    RC(stage(3, sg, sp, 8, 1, true, aS));
+    HIPCHK(launch_levelset_status<double>(g_drop.s, g, h, a.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
+                                          aS.a.view<double>()));
+    return unstage({&aD, &aI, &aS});
+}
+
+int tpls_do_levelset_iteration(const double* phi, double* pr, const double* D, const int* ist, const double* sg,
+                               int sx, int ex, int sy, int ey, int n, int ex_max, double h, double height,
+                               double* error_val)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!phi || !pr || !D || !ist || !sg || !error_val) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, n, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, n);
+    Stg a, r0, r1, aD, aI, aS;
+    RC(stage(0, phi, sp, 8, 0, false, a));
+    RC(stage(1, pr, sp, 8, 0, false, r0));
+    RC(stage(2, pr, sp, 8, 0, true, r1));
+    RC(stage(3, D, sp, 8, 0, false, aD));
+    RC(stage(4, ist, sp, 4, 0, false, aI));
+    RC(stage(5, sg, sp, 8, 0, false, aS));
+    hipStream_t s = g_drop.s;
+    HIPCHK(launch_levelset_iter<double>(s, g, h, a.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
+                                        aS.a.view<double>(), r0.a.view<double>(), r1.a.view<double>()));
+    HIPCHK(launch_phi_bc<double>(s, g, h, height, r1.a.view<double>(), 1));
+    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    HIPCHK(launch_levelset_err<double>(s, g, r0.a.view<double>(), r1.a.view<double>(), red));
+    RC(unstage({&r1}));
+    HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *error_val = g_drop.h_out[0];
+    return TPLS_OK;
+}
+
+int tpls_get_csf(double* fx, double* fy, double* fz, const double* fi, int sx, int ex, int sy, int ey, int n,
+                 int ex_max, double dx, double dy, double dz, double scap, double sw)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!fx || !fy || !fz || !fi) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, n, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, n);
+    Stg a, bx, by, bz;
+    RC(stage(0, fi, sp, 8, 0, false, a));
+    RC(stage(1, fx, sp, 8, 0, true, bx));
+    RC(stage(2, fy, sp, 8, 0, true, by));
+    RC(stage(3, fz, sp, 8, 0, true, bz));
+    Phys ph = phys_d(dx, dy, dz, 0);
+    ph.scap = scap;
+    ph.sw = sw;
+    HIPCHK(launch_csf<double>(g_drop.s, g, ph, a.a.view<double>(), bx.a.view<double>(), by.a.view<double>(),
+                              bz.a.view<double>()));
+    HIPCHK(launch_csf_bc<double>(g_drop.s, g, bx.a.view<double>(), by.a.view<double>(), bz.a.view<double>()));
+    return unstage({&bx, &by, &bz});
+}
+
+static int momentum_dropin(int mode, double* o_u, double* o_v, double* o_w, const double* visc, const double* u2,
+                           const double* v2, const double* w2, int sx, int sy, int ex, int ey, int maxn, int ex_max,
+                           double dx, double dy, double dz)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!o_u || !o_v || !o_w || !visc || !u2 || !v2 || !w2) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape suv = shape_of(G_UV, maxl, maxm, maxn), sw = shape_of(G_W, maxl, maxm, maxn);
+    Stg au, av, aw, amu, ou, ov, ow;
+    RC(stage(0, u2, suv, 8, 0, false, au));
+    RC(stage(1, v2, suv, 8, 0, false, av));
+    RC(stage(2, w2, sw, 8, 0, false, aw));
+    RC(stage(3, visc, shape_of(G_AUG, maxl, maxm, maxn), 8, 0, false, amu));
+    // the reference zeroes the whole output array first
+    RC(stage(4, o_u, suv, 8, 1, true, ou));
+    RC(stage(5, o_v, suv, 8, 1, true, ov));
+    RC(stage(6, o_w, sw, 8, 1, true, ow));
+    Phys ph = phys_d(dx, dy, dz, 0);
+    Stg* outs[3] = {&ou, &ov, &ow};
+    for (int q = 0; q < 3; ++q) {
+        FieldView<double> o = outs[q]->a.view<double>();
+        HIPCHK(launch_momentum<double>(g_drop.s, g, ph, q, mode, au.a.view<double>(), av.a.view<double>(),
+                                       aw.a.view<double>(), amu.a.view<double>(), o, o, o, o, o, o, o, o, o));
+    }
+    return unstage({&ou, &ov, &ow});
+}
+
+int tpls_get_diffusion_all(double* du, double* dv, double* dw, const double* visc, const double* u2, const double* v2,
+                           const double* w2, int sx, int sy, int ex, int ey, int maxn, int ex_max, double dx,
+                           double dy, double dz)
+{
+    return momentum_dropin(2, du, dv, dw, visc, u2, v2, w2, sx, sy, ex, ey, maxn, ex_max, dx, dy, dz);
+}
+
+int tpls_get_conv_all(double* cu, double* cv, double* cw, const double* visc, const double* u2, const double* v2,
+                      const double* w2, int sx, int sy, int ex, int ey, int maxn, int ex_max, double dx, double dy,
+                      double dz)
+{
+    return momentum_dropin(1, cu, cv, cw, visc, u2, v2, w2, sx, sy, ex, ey, maxn, ex_max, dx, dy, dz);
+}
+
+int tpls_get_viscosity(double* visc, const double* phi, int ex, int ey, int sx, int sy, int maxn, double sw, double Re,
+                       double mu_minus, double mu_plus)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex));
+    if (!visc || !phi) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, maxn);
+    Stg a, m;
+    RC(stage(0, phi, sp, 8, 0, false, a));
+    RC(stage(1, visc, sp, 8, 0, true, m));
+    Phys ph{};
+    ph.sw = sw;
+    ph.Re = Re;
+    ph.mu_minus = mu_minus;
+    ph.mu_plus = mu_plus;
+    HIPCHK(launch_viscosity<double>(g_drop.s, g, ph, a.a.view<double>(), m.a.view<double>(), 1, ex));
+    return unstage({&m});
+}
+
+static int jacobi_dropin(int comp, double* x, const double* xo, const double* R, const double* visc, double dx,
+                         double dy, double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!x || !xo || !R || !visc) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape sh = shape_of(comp == 2 ? G_W : G_UV, maxl, maxm, maxn);
+    Stg a, ao, ar, am;
+    RC(stage(0, x, sh, 8, 0, true, a));
+    RC(stage(1, xo, sh, 8, 0, false, ao));
+    RC(stage(2, R, sh, 8, 0, false, ar));
+    RC(stage(3, visc, shape_of(G_AUG, maxl, maxm, maxn), 8, 0, false, am));
+    Phys ph = phys_d(dx, dy, dz, dt);
+    HIPCHK(launch_jacobi<double>(g_drop.s, g, ph, comp, ao.a.view<double>(), ar.a.view<double>(), am.a.view<double>(),
+                                 a.a.view<double>(), false));
+    // do_jacobi_v / do_jacobi_w close with their x and wall conditions
+    // (jacobi_iteration_allflux.f90:178-189, 257-279); do_jacobi_u does not
+    if (comp != 0)
+        HIPCHK(launch_helmholtz_bc<double>(g_drop.s, g, ph, comp, a.a.view<double>(), a.a.view<double>()));
+    return unstage({&a});
+}
+
+int tpls_do_jacobi_u(double* u3, const double* uo, const double* R, const double* visc, double dx, double dy,
+                     double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max)
+{
+    return jacobi_dropin(0, u3, uo, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max);
+}
+int tpls_do_jacobi_v(double* v3, const double* vo, const double* R, const double* visc, double dx, double dy,
+                     double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max)
+{
+    return jacobi_dropin(1, v3, vo, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max);
+}
+int tpls_do_jacobi_w(double* w3, const double* wo, const double* R, const double* visc, double dx, double dy,
+                     double dz, double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max)
+{
+    return jacobi_dropin(2, w3, wo, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max);
+}
+
+int tpls_get_difference(const double* ua, const double* ub, int ex, int ey, int ez, int sx, int sy, int sz,
+                        double* diff)
+{
+    RC(drop_init());
+    if (!ua || !ub || !diff) return fail(TPLS_ERR_ARG, "null array");
+    if (sx != 1 || sy != 1 || sz != 0) return fail(TPLS_ERR_DECOMP, "single-rank decomposition required");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, ez + 2, maxm - 1};
+    const Shape sh{0, maxl, 0, maxm, 0, ez};
+    Stg a, b;
+    RC(stage(0, ua, sh, 8, 0, false, a));
+    RC(stage(1, ub, sh, 8, 0, false, b));
+    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    HIPCHK(launch_difference<double>(g_drop.s, g, ez + 1, a.a.view<double>(), b.a.view<double>(), red));
+    HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, g_drop.s));
+    HIPCHK(hipStreamSynchronize(g_drop.s));
+    *diff = g_drop.h_out[0];
+    return TPLS_OK;
+}
+
+static int sor_dropin(int comp, double* x, const double* R, const double* visc, double dx, double dy, double dz,
+                      double dt, int ex, int ey, int sx, int sy, int maxn, int ex_max, int flg, int it)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!x || !R || !visc) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape sh = shape_of(comp == 2 ? G_W : G_UV, maxl, maxm, maxn);
+    Stg a, ar, am;
+    RC(stage(0, x, sh, 8, 0, true, a));
+    RC(stage(1, R, sh, 8, 0, false, ar));
+    RC(stage(2, visc, shape_of(G_AUG, maxl, maxm, maxn), 8, 0, false, am));
+    HIPCHK(launch_sor<double>(g_drop.s, g, phys_d(dx, dy, dz, dt), comp, a.a.view<double>(), ar.a.view<double>(),
+                              am.a.view<double>(), flg, it));
+    return unstage({&a});
+}
+
+int tpls_do_sor_u(double* u3, const double* R, const double* visc, double dx, double dy, double dz, double dt, int ex,
+                  int ey, int sx, int sy, int maxn, int ex_max, int flg, int it)
+{
+    return sor_dropin(0, u3, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max, flg, it);
+}
+int tpls_do_sor_v(double* v3, const double* R, const double* visc, double dx, double dy, double dz, double dt, int ex,
+                  int ey, int sx, int sy, int maxn, int flg, int it)
+{
+    return sor_dropin(1, v3, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex, flg, it);
+}
+int tpls_do_sor_w(double* w3, const double* R, const double* visc, double dx, double dy, double dz, double dt, int ex,
+                  int ey, int sx, int sy, int maxn, int flg, int it)
+{
+    return sor_dropin(2, w3, R, visc, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex, flg, it);
+}
+
+int tpls_get_source_pres(double* rhs, const double* u3, const double* v3, const double* w3, int ex, int ey, int sx,
+                         int sy, int maxn, double dx, double dy, double dz, double dt)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex));
+    if (!rhs || !u3 || !v3 || !w3) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    Stg r, au, av, aw;
+    RC(stage(0, rhs, shape_of(G_P, maxl, maxm, maxn), 8, 1, true, r));
+    RC(stage(1, u3, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, au));
+    RC(stage(2, v3, shape_of(G_UV, maxl, maxm, maxn), 8, 0, false, av));
+    RC(stage(3, w3, shape_of(G_W, maxl, maxm, maxn), 8, 0, false, aw));
+    HIPCHK(launch_source_pres<double>(g_drop.s, g, phys_d(dx, dy, dz, dt), au.a.view<double>(), av.a.view<double>(),
+                                      aw.a.view<double>(), r.a.view<double>()));
+    return unstage({&r});
+}
+
+int tpls_get_uvw_pres(double* u3, double* v3, double* w3, const double* p, int ex, int ey, int sx, int sy, int maxn,
+                      int ex_max, double dx, double dy, double dz, double dt, const double* u_inlet)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, ex_max));
+    if (!u3 || !v3 || !w3 || !p || !u_inlet) return fail(TPLS_ERR_ARG, "null array");
+    const int maxl = ex + 1, maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    Stg au, av, aw, ap, ai;
+    RC(stage(0, u3, shape_of(G_UV, maxl, maxm, maxn), 8, 0, true, au));
+    RC(stage(1, v3, shape_of(G_UV, maxl, maxm, maxn), 8, 0, true, av));
+    RC(stage(2, w3, shape_of(G_W, maxl, maxm, maxn), 8, 0, true, aw));
+    RC(stage(3, p, shape_of(G_P, maxl, maxm, maxn), 8, 0, false, ap));
+    RC(stage(4, u_inlet, shape_of(G_INLET, maxl, maxm, maxn), 8, 0, false, ai));
+    HIPCHK(launch_uvw_pres<double>(g_drop.s, g, phys_d(dx, dy, dz, dt), ap.a.view<double>(),
+                                   static_cast<const double*>(ai.a.ptr), au.a.view<double>(), av.a.view<double>(),
+                                   aw.a.view<double>()));
+    return unstage({&au, &av, &aw});
+}
+
+int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, const double* u_inlet, int maxl, int maxm,
+                        int maxn, double dx, double dy, double dz, double dt, double rtol, double abstol, double dtol,
+                        int maxits, int* its, double* rnorm, int* reason)
+{
+    RC(drop_init());
+    if (!pres || !RHS_p || !u3 || !u_inlet || !its || !rnorm || !reason) return fail(TPLS_ERR_ARG, "null argument");
+    DropCtx& d = g_drop;
+    if (!d.pcase || d.pl != maxl || d.pm != maxm || d.pn != maxn) {
+        if (d.pcase) tpls_case_destroy(d.pcase);
+        d.pcase = nullptr;
+        tpls_params prm{};
+        prm.dx = dx;
+        prm.dy = dy;
+        prm.dz = dz;
+        prm.dt = dt;
+        prm.Re = 1;
+        prm.mom_u = prm.mom_v = prm.mom_w = 2;
+        RC(tpls_case_create(&prm, maxl, maxm, maxn, &d.pcase));
+        d.pl = maxl;
+        d.pm = maxm;
+        d.pn = maxn;
+    }
+    tpls_case* c = d.pcase;
+    c->ph = phys_of(c->prm);
+    c->ph.dx = dx;
+    c->ph.dy = dy;
+    c->ph.dz = dz;
+    c->ph.dt = dt;
+    c->prm.ksp_rtol = rtol;
+    c->prm.ksp_abstol = abstol;
+    c->prm.ksp_dtol = dtol;
+    c->prm.ksp_maxits = maxits;
+    // the case's rhs[0..2] hold u3 (uv grid) and RHS_p (p grid, double)
+    RC(put_array(c, c->x[0][0], u3));
+    RC(put_array(c, c->pold, pres));
+    RC(put_array(c, c->cg_p0, RHS_p));
+    std::vector<double> inl(u_inlet, u_inlet + (maxn - 1));
+    RC(put_array(c, c->inlet, inl.data()));
+    FieldView<double> u = c->x[0][0]->view<double>();
+    RC(pressure_solve<double>(c, u, u, u, c->cg_p0->view<double>(), 1, static_cast<const double*>(c->inlet->ptr),
+                              c->pold->view<double>()));
+    HIPCHK(launch_pres_ghosts<double>(c->s, c->g, c->ph, c->pold->view<double>(), u,
+                                      static_cast<const double*>(c->inlet->ptr), c->pres->view<double>()));
+    RC(get_array(c, c->pres, pres));
+    wrap_rows(pres, shape_of(G_P, maxl, maxm, maxn), maxm);
+    *its = c->diag.ksp_its;
+    *rnorm = c->diag.ksp_rnorm;
+    *reason = c->diag.ksp_reason;
+    return TPLS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+// Level-set update, Russo-Smereka reinitialisation and CSF force:
+// two_phase_levelset.f90:20-118 and levelset.f90:17-414.
+#include "tpls_kernels.hpp"
+#include "reduce.hpp"
+
+namespace tpls {
+
+// phi3 = phi2 - dt * AB3(conv), two_phase_levelset.f90:56-70, p interior.
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_tpls_update(Grid g, T dt, FieldView<T> phi2, FieldView<T> c0,
+                                                        FieldView<T> c1, FieldView<T> c2, FieldView<T> phi3)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.maxl - 1 || j > g.P) return;
+    T rhs = T(23.0 / 12.0) * c2(i, j, k) - T(4.0 / 3.0) * c1(i, j, k) + T(5.0 / 12.0) * c0(i, j, k);
+    phi3(i, j, k) = phi2(i, j, k) - dt * rhs;
+}
+
+// Boundary planes of a level-set field, one thread per (i, j) column,
+// i in 0..maxl: inlet phi = z - height and outlet copy on k = 1..n-1
+// (two_phase_levelset.f90:72-82, levelset.f90:170-191), then linear
+// extrapolation to k = 0 and k = n (two_phase_levelset.f90:94-95,
+// levelset.f90:193-216).
+template <typename T>
+__global__ void k_phi_bc(Grid g, T dz, T height, FieldView<T> phi, int inlet_outlet)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    const int n = g.maxn;
+    if (i > g.maxl) return;
+    if (inlet_outlet && i == 0) {
+        for (int k = 1; k <= n - 1; ++k) {
+            T z = T(k) * dz - T(0.5) * dz;
+            phi(0, j, k) = z - height;
+        }
+    } else if (inlet_outlet && i == g.maxl) {
+        for (int k = 1; k <= n - 1; ++k) phi(i, j, k) = phi(i - 1, j, k);
+    }
+    phi(i, j, 0) = T(2) * phi(i, j, 1) - phi(i, j, 2);
+    phi(i, j, n) = T(2) * phi(i, j, n - 1) - phi(i, j, n - 2);
+}
+
+// levelset.f90:44-91: sign, interface indicator and subcell distance.
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_levelset_status(Grid g, T h, FieldView<T> phi, FieldView<T> D,
+                                                            FieldView<int> ist, FieldView<T> sg)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.maxl - 1 || j > g.P) return;
+    const int jp = wrapj(j + 1, g.P), jm = wrapj(j - 1, g.P);
+    T c = phi(i, j, k);
+    T xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, jp, k), ym = phi(i, jm, k);
+    T zp = phi(i, j, k + 1), zm = phi(i, j, k - 1);
+    sg(i, j, k) = c < T(0) ? T(-1) : (c > T(0) ? T(1) : T(0));
+    T mn = fmin(fmin(fmin(fmin(fmin(c * xp, c * xm), c * yp), c * ym), c * zp), c * zm);
+    int s = mn <= T(0) ? 1 : 0;
+    ist(i, j, k) = s;
+    if (s) {
+        T gx = xp - xm, gy = yp - ym, gz = zp - zm;
+        T a1 = sqrt(gx * gx + gy * gy + gz * gz) / T(2);
+        T mx = fmax(fmax(fmax(fmax(fmax(fmax(fmax(a1, fabs(xp - c)), fabs(xm - c)), fabs(yp - c)), fabs(ym - c)),
+                             fabs(zp - c)),
+                        fabs(zm - c)),
+                   h);
+        D(i, j, k) = h * c / mx;
+    }
+}
+
+// levelset.f90:125-166: one pseudo-time step, dtau = 0.3 h.
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_levelset_iter(Grid g, T h, FieldView<T> phi, FieldView<T> D,
+                                                          FieldView<int> ist, FieldView<T> sg,
+                                                          FieldView<T> pr, FieldView<T> pn)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.maxl - 1 || j > g.P) return;
+    const T dtau = T(0.3) * h;
+    T c = pr(i, j, k);
+    T dc;
+    if (ist(i, j, k) == 1) {
+        dc = sg(i, j, k) * fabs(c) - D(i, j, k);
+    } else {
+        dc = T(0);
+        const T p0 = phi(i, j, k);
+        if (p0 != T(0)) {
+            const int jp = wrapj(j + 1, g.P), jm = wrapj(j - 1, g.P);
+            T dxm = c - pr(i - 1, j, k), dxp = pr(i + 1, j, k) - c;
+            T dym = c - pr(i, jm, k), dyp = pr(i, jp, k) - c;
+            T dzm = c - pr(i, j, k - 1), dzp = pr(i, j, k + 1) - c;
+            T gx, gy, gz;
+            if (p0 > T(0)) {
+                T aP = fmax(dxm, T(0)), bM = fmin(dxp, T(0));
+                T cP = fmax(dym, T(0)), dM = fmin(dyp, T(0));
+                T eP = fmax(dzm, T(0)), fM = fmin(dzp, T(0));
+                gx = fmax(aP * aP, bM * bM);
+                gy = fmax(cP * cP, dM * dM);
+                gz = fmax(eP * eP, fM * fM);
+                dc = sg(i, j, k) * (sqrt(gx + gy + gz) - h);
+            } else {
+                T bP = fmax(dxp, T(0)), aM = fmin(dxm, T(0));
+                T dP = fmax(dyp, T(0)), cM = fmin(dym, T(0));
+                T fP = fmax(dzp, T(0)), eM = fmin(dzm, T(0));
+                gx = fmax(bP * bP, aM * aM);
+                gz = fmax(fP * fP, eM * eM);
+                gy = fmax(dP * dP, cM * cM);
+                dc = sg(i, j, k) * (sqrt(gx + gz + gy) - h);  // levelset.f90:154 sums x, z, y
+            }
+        }
+    }
+    pn(i, j, k) = c - dtau * dc / h;
+}
+
+// Diagnostic of levelset.f90:218-228: sum over i in 1..maxl-1, all j, k in
+// 0..n of (pr - pn)^2 (the reference's reduction(MAX:) clause is inert for a
+// single thread, leaving the sum).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, FieldView<T> b, Reduce red)
+{
+    const int nx = g.maxl - 1;
+    const long n = (long)nx * g.P * nz;
+    double acc = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        int i = int(q % nx) + 1;
+        long r = q / nx;
+        int j = int(r % g.P) + 1;
+        int k = int(r / g.P);
+        double d = double(a(i, j, k)) - double(b(i, j, k));
+        acc += d * d;
+    }
+    block_reduce_finish(acc, red);
+}
+
+// levelset.f90:270-363: curvature (clipped above at 1/min(dx,dy,dz)) and the
+// CSF force delta(phi) * scap * kappa * grad(phi), p interior.
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_csf(Grid g, T dx, T dy, T dz, T scap, T sw, FieldView<T> fi,
+                                                FieldView<T> fx, FieldView<T> fy, FieldView<T> fz)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.maxl - 1 || j > g.P) return;
+    const int jp = wrapj(j + 1, g.P), jm = wrapj(j - 1, g.P);
+    T c = fi(i, j, k);
+    T xp = fi(i + 1, j, k), xm = fi(i - 1, j, k), yp = fi(i, jp, k), ym = fi(i, jm, k);
+    T zp = fi(i, j, k + 1), zm = fi(i, j, k - 1);
+    T d2x = (xp - T(2) * c + xm) / (dx * dx);
+    T d2y = (yp - T(2) * c + ym) / (dy * dy);
+    T d2z = (zp - T(2) * c + zm) / (dz * dz);
+    T gx = (xp - xm) / (T(2) * dx);
+    T gy = (yp - ym) / (T(2) * dy);
+    T gz = (zp - zm) / (T(2) * dz);
+    T dxz = ((fi(i + 1, j, k + 1) - fi(i - 1, j, k + 1)) / (T(2) * dx) -
+             (fi(i + 1, j, k - 1) - fi(i - 1, j, k - 1)) / (T(2) * dx)) /
+            (T(2) * dz);
+    T dyz = ((fi(i, jp, k + 1) - fi(i, jm, k + 1)) / (T(2) * dy) - (fi(i, jp, k - 1) - fi(i, jm, k - 1)) / (T(2) * dy)) /
+            (T(2) * dz);
+    T dxy = ((fi(i + 1, jp, k) - fi(i - 1, jp, k)) / (T(2) * dx) - (fi(i + 1, jm, k) - fi(i - 1, jm, k)) / (T(2) * dx)) /
+            (T(2) * dy);
+    T nrm = gx * gx + gy * gy + gz * gz;
+    T kap;
+    if (nrm < T(1e-10)) {
+        kap = T(0);
+    } else {
+        T rn = sqrt(nrm);
+        T t1 = (d2x + d2y + d2z) / rn;
+        T t2 = gx * gx * d2x + gy * gy * d2y + gz * gz * d2z;
+        T t3 = gx * gy * dxy + gx * gz * dxz + gy * gz * dyz;
+        T n15 = nrm * rn;
+        t2 = t2 / n15;
+        t3 = T(2) * t3 / n15;
+        kap = -(t1 - t2 - t3);
+    }
+    const T max_kappa = T(1) / fmin(fmin(dx, dy), dz);
+    if (kap > max_kappa) kap = max_kappa;
+    T dirac;
+    if (c < -sw || c > sw)
+        dirac = T(0);
+    else
+        dirac = (T(1) / (T(2) * sw)) + (T(1) / (T(2) * sw)) * cos(T(TPLS_PI_SP) * c / sw);
+    T base = dirac * scap * kap;
+    fx(i, j, k) = base * gx;
+    fy(i, j, k) = base * gy;
+    fz(i, j, k) = base * gz;
+}
+
+// levelset.f90:365-412: z extrapolation on the interior columns, then x
+// extrapolation of whole inlet/outlet columns (which the column threads at
+// i = 0 and i = maxl rebuild from their neighbours' extrapolated values).
+template <typename T>
+__device__ __forceinline__ T zext(const FieldView<T>& f, int i, int j, int k, int n)
+{
+    if (k == 0) return T(2) * f(i, j, 1) - f(i, j, 2);
+    if (k == n) return T(2) * f(i, j, n - 1) - f(i, j, n - 2);
+    return f(i, j, k);
+}
+
+template <typename T>
+__device__ __forceinline__ void csf_col_bc(const FieldView<T>& f, int i, int j, int maxl, int n)
+{
+    if (i >= 1 && i <= maxl - 1) {
+        T a = T(2) * f(i, j, 1) - f(i, j, 2);
+        T b = T(2) * f(i, j, n - 1) - f(i, j, n - 2);
+        f(i, j, 0) = a;
+        f(i, j, n) = b;
+    } else if (i == 0) {
+        for (int k = 0; k <= n; ++k) f(0, j, k) = T(2) * zext(f, 1, j, k, n) - zext(f, 2, j, k, n);
+    } else {
+        for (int k = 0; k <= n; ++k) f(maxl, j, k) = T(2) * zext(f, maxl - 1, j, k, n) - zext(f, maxl - 2, j, k, n);
+    }
+}
+
+template <typename T>
+__global__ void k_csf_bc(Grid g, FieldView<T> fx, FieldView<T> fy, FieldView<T> fz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    if (i > g.maxl) return;
+    csf_col_bc(fx, i, j, g.maxl, g.maxn);
+    csf_col_bc(fy, i, j, g.maxl, g.maxn);
+    csf_col_bc(fz, i, j, g.maxl, g.maxn);
+}
+
+static inline dim3 grid3(const Grid& g, int nzb) { return dim3((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, nzb); }
+
+template <typename T>
+hipError_t launch_tpls_update(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi2, FieldView<T> c0,
+                              FieldView<T> c1, FieldView<T> c2, FieldView<T> phi3)
+{
+    hipLaunchKernelGGL(k_tpls_update<T>, grid3(g, g.maxn - 1), dim3(BX, BY), 0, s, g, T(ph.dt), phi2, c0, c1, c2,
+                       phi3);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_phi_bc(hipStream_t s, const Grid& g, double dz, double height, FieldView<T> phi, int inlet_outlet)
+{
+    hipLaunchKernelGGL(k_phi_bc<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, T(dz), T(height), phi,
+                       inlet_outlet);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_levelset_status(hipStream_t s, const Grid& g, double h, FieldView<T> phi, FieldView<T> D,
+                                  FieldView<int> ist, FieldView<T> sign)
+{
+    hipLaunchKernelGGL(k_levelset_status<T>, grid3(g, g.maxn - 1), dim3(BX, BY), 0, s, g, T(h), phi, D, ist, sign);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_levelset_iter(hipStream_t s, const Grid& g, double h, FieldView<T> phi, FieldView<T> D,
+                                FieldView<int> ist, FieldView<T> sign, FieldView<T> pr, FieldView<T> pr_new)
+{
+    hipLaunchKernelGGL(k_levelset_iter<T>, grid3(g, g.maxn - 1), dim3(BX, BY), 0, s, g, T(h), phi, D, ist, sign,
+                       pr, pr_new);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_levelset_err(hipStream_t s, const Grid& g, FieldView<T> a, FieldView<T> b, Reduce red)
+{
+    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, g.maxn + 1, a, b, red);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b, Reduce red)
+{
+    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, nz, a, b, red);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_csf(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, FieldView<T> fx,
+                      FieldView<T> fy, FieldView<T> fz)
+{
+    hipLaunchKernelGGL(k_csf<T>, grid3(g, g.maxn - 1), dim3(BX, BY), 0, s, g, T(ph.dx), T(ph.dy), T(ph.dz),
+                       T(ph.scap), T(ph.sw), phi, fx, fy, fz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_csf_bc(hipStream_t s, const Grid& g, FieldView<T> fx, FieldView<T> fy, FieldView<T> fz)
+{
+    hipLaunchKernelGGL(k_csf_bc<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, fx, fy, fz);
+    return hipGetLastError();
+}
+
+#define TPLS_INST_LS(T)                                                                                              \
+    template hipError_t launch_tpls_update<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, FieldView<T>,     \
+                                              FieldView<T>, FieldView<T>, FieldView<T>);                             \
+    template hipError_t launch_phi_bc<T>(hipStream_t, const Grid&, double, double, FieldView<T>, int);               \
+    template hipError_t launch_levelset_status<T>(hipStream_t, const Grid&, double, FieldView<T>, FieldView<T>,      \
+                                                  FieldView<int>, FieldView<T>);                                     \
+    template hipError_t launch_levelset_iter<T>(hipStream_t, const Grid&, double, FieldView<T>, FieldView<T>,        \
+                                                FieldView<int>, FieldView<T>, FieldView<T>, FieldView<T>);           \
+    template hipError_t launch_levelset_err<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, Reduce);        \
+    template hipError_t launch_difference<T>(hipStream_t, const Grid&, int, FieldView<T>, FieldView<T>, Reduce);     \
+    template hipError_t launch_csf<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, FieldView<T>,             \
+                                      FieldView<T>, FieldView<T>);                                                   \
+    template hipError_t launch_csf_bc<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, FieldView<T>);
+
+TPLS_INST_LS(double)
+TPLS_INST_LS(float)
+
+}  // namespace tpls

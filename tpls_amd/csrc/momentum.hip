@@ -1,0 +1,246 @@
+// Momentum flux assembly: momentum_allflux.f90:22-521 and the right-hand
+// sides of main_ns_hybrid.F90:629-757.
+//
+// The fused kernel (mode bit 4) produces, per face cell, the convective term
+// conv2, the diffusive term (kept in registers), the CSF face average csf2 and
+// the AB3 + Crank-Nicolson right-hand side in one pass.
+#include "tpls_kernels.hpp"
+
+namespace tpls {
+
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_viscosity(Grid g, T sw, T Re, T mum, T mup, FieldView<T> phi,
+                                                      FieldView<T> mu, int ilo, int ihi)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + ilo;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    if (i > ihi || j > g.P) return;
+    mu(i, j, k) = visc_of_phi<T>(phi(i, j, k), sw, Re, mum, mup);
+}
+
+// ---- convective terms, momentum_allflux.f90:281-337 / 350-402 / 415-463 ----
+template <typename C, typename T>
+__device__ __forceinline__ C conv_u(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
+                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, C dx, C dy, C dz)
+{
+    const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
+    const int j1 = jp, j2 = wrapj(j + 2, P), j0 = j;
+    C tv = (C(v2(i, j, k)) + C(v2(i + 1, j, k)) + C(v2(i, jp, k)) + C(v2(i + 1, jp, k))) / C(4);
+    C tw = (C(w2(i, j, k)) + C(w2(i + 1, j, k)) + C(w2(i, j, k + 1)) + C(w2(i + 1, j, k + 1))) / C(4);
+    C uc = u2(i, j, k), uxp = u2(i + 1, j, k), uxm = u2(i - 1, j, k);
+    C dudx = C(0.5) * (uxp - uxm) / dx;
+    C Xc = dudx * uc;
+    C dudy = C(0.5) * (C(u2(i, jp, k)) - C(u2(i, jm, k))) / dy;
+    C Yc = dudy * tv;
+    C dudz;
+    if (k == 0)
+        dudz = (uc + C(u2(i, j, k + 1))) / (C(2) * dz);
+    else if (k == maxn - 2)
+        dudz = -(uc + C(u2(i, j, k - 1))) / (C(2) * dz);
+    else
+        dudz = C(0.5) * (C(u2(i, j, k + 1)) - C(u2(i, j, k - 1))) / dz;
+    C Zc = dudz * tw;
+    C c1 = Xc + Yc + Zc;
+    C mxp = mu(i + 1, j1, k + 1), mxm = mu(i, j1, k + 1);
+    C myp = (C(mu(i + 1, j2, k + 1)) + C(mu(i, j2, k + 1)) + mxp + mxm) / C(4);
+    C mym = (C(mu(i + 1, j0, k + 1)) + C(mu(i, j0, k + 1)) + mxp + mxm) / C(4);
+    C mzp = (C(mu(i + 1, j1, k + 2)) + C(mu(i, j1, k + 2)) + mxp + mxm) / C(4);
+    C mzm = (C(mu(i + 1, j1, k)) + C(mu(i, j1, k)) + mxp + mxm) / C(4);
+    C aE = mxp * (uxp - uc) / dx;
+    C aW = mxm * (uc - uxm) / dx;
+    C bN = myp * (C(v2(i + 1, jp, k)) - C(v2(i, jp, k))) / dx;
+    C bS = mym * (C(v2(i + 1, j, k)) - C(v2(i, j, k))) / dx;
+    C cO = mzp * (C(w2(i + 1, j, k + 1)) - C(w2(i, j, k + 1))) / dx;
+    C cI = mzm * (C(w2(i + 1, j, k)) - C(w2(i, j, k))) / dx;
+    C c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+    return c1 - c2;
+}
+
+template <typename C, typename T>
+__device__ __forceinline__ C conv_v(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
+                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, C dx, C dy, C dz)
+{
+    const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
+    C tu = (C(u2(i - 1, jm, k)) + C(u2(i - 1, j, k)) + C(u2(i, jm, k)) + C(u2(i, j, k))) / C(4);
+    C tw = (C(w2(i, jm, k)) + C(w2(i, j, k)) + C(w2(i, jm, k + 1)) + C(w2(i, j, k + 1))) / C(4);
+    C vc = v2(i, j, k), vyp = v2(i, jp, k), vym = v2(i, jm, k);
+    C dvdy = C(0.5) * (vyp - vym) / dy;
+    C Yc = dvdy * vc;
+    C dvdx = C(0.5) * (C(v2(i + 1, j, k)) - C(v2(i - 1, j, k))) / dx;
+    C Xc = dvdx * tu;
+    C dvdz;
+    if (k == 0)
+        dvdz = (vc + C(v2(i, j, k + 1))) / (C(2) * dz);
+    else if (k == maxn - 2)
+        dvdz = -(vc + C(v2(i, j, k - 1))) / (C(2) * dz);
+    else
+        dvdz = C(0.5) * (C(v2(i, j, k + 1)) - C(v2(i, j, k - 1))) / dz;
+    C Zc = dvdz * tw;
+    C c1 = Xc + Yc + Zc;
+    C a = mu(i, jp, k + 1), b = mu(i, j, k + 1);
+    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, j, k + 1)) + a + b) / C(4);
+    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, j, k + 1)) + a + b) / C(4);
+    C myp = a, mym = b;
+    C mzp = (C(mu(i, jp, k + 2)) + C(mu(i, j, k + 2)) + a + b) / C(4);
+    C mzm = (C(mu(i, jp, k)) + C(mu(i, j, k)) + a + b) / C(4);
+    C aE = mxp * (C(u2(i, j, k)) - C(u2(i, jm, k))) / dy;
+    C aW = mxm * (C(u2(i - 1, j, k)) - C(u2(i - 1, jm, k))) / dy;
+    C bN = myp * (vyp - vc) / dy;
+    C bS = mym * (vc - vym) / dy;
+    C cO = mzp * (C(w2(i, j, k + 1)) - C(w2(i, jm, k + 1))) / dy;
+    C cI = mzm * (C(w2(i, j, k)) - C(w2(i, jm, k))) / dy;
+    C c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+    return c1 - c2;
+}
+
+template <typename C, typename T>
+__device__ __forceinline__ C conv_w(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
+                                    const FieldView<T>& mu, int i, int j, int k, int P, C dx, C dy, C dz)
+{
+    const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P), j2 = wrapj(j + 2, P);
+    C tu = (C(u2(i - 1, j, k - 1)) + C(u2(i - 1, j, k)) + C(u2(i, j, k - 1)) + C(u2(i, j, k))) / C(4);
+    C tv = (C(v2(i, j, k - 1)) + C(v2(i, j, k)) + C(v2(i, jp, k - 1)) + C(v2(i, jp, k))) / C(4);
+    C wc = w2(i, j, k), wzp = w2(i, j, k + 1), wzm = w2(i, j, k - 1);
+    C dwdz = C(0.5) * (wzp - wzm) / dz;
+    C Zc = dwdz * wc;
+    C dwdx = C(0.5) * (C(w2(i + 1, j, k)) - C(w2(i - 1, j, k))) / dx;
+    C Xc = dwdx * tu;
+    C dwdy = C(0.5) * (C(w2(i, jp, k)) - C(w2(i, jm, k))) / dy;
+    C Yc = dwdy * tv;
+    C c1 = Xc + Yc + Zc;
+    C a = mu(i, jp, k + 1), b = mu(i, jp, k);
+    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, jp, k)) + a + b) / C(4);
+    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, jp, k)) + a + b) / C(4);
+    C myp = (C(mu(i, j2, k + 1)) + C(mu(i, j2, k)) + a + b) / C(4);
+    C mym = (C(mu(i, j, k + 1)) + C(mu(i, j, k)) + a + b) / C(4);
+    C mzp = a, mzm = b;
+    C aE = mxp * (C(u2(i, j, k)) - C(u2(i, j, k - 1))) / dz;
+    C aW = mxm * (C(u2(i - 1, j, k)) - C(u2(i - 1, j, k - 1))) / dz;
+    C bN = myp * (C(v2(i, jp, k)) - C(v2(i, jp, k - 1))) / dz;
+    C bS = mym * (C(v2(i, j, k)) - C(v2(i, j, k - 1))) / dz;
+    C cO = mzp * (wzp - wc) / dz;
+    C cI = mzm * (wc - wzm) / dz;
+    C c2 = ((aE - aW) / dx) + ((bN - bS) / dy) + ((cO - cI) / dz);
+    return c1 - c2;
+}
+
+// ---- diffusive terms, momentum_allflux.f90:70-110 / 121-164 / 175-206 ----
+template <typename C, typename T>
+__device__ __forceinline__ C diff_uv(const FieldView<T>& x2, const MuFaces<C>& m, int i, int j, int k, int maxn,
+                                     int P, C dx, C dy, C dz)
+{
+    C c = x2(i, j, k);
+    C xm = (k == 0) ? C(-2) * c + C(x2(i, j, 1)) / C(3) : C(x2(i, j, k - 1));
+    C xp = (k == maxn - 2) ? C(-2) * c + C(x2(i, j, maxn - 3)) / C(3) : C(x2(i, j, k + 1));
+    C xE = m.xp * ((C(x2(i + 1, j, k)) - c) / dx);
+    C xW = m.xm * ((c - C(x2(i - 1, j, k))) / dx);
+    C yN = m.yp * ((C(x2(i, wrapj(j + 1, P), k)) - c) / dy);
+    C yS = m.ym * ((c - C(x2(i, wrapj(j - 1, P), k))) / dy);
+    C zO = m.zp * ((xp - c) / dz);
+    C zI = m.zm * ((c - xm) / dz);
+    return ((xE - xW) / dx) + ((yN - yS) / dy) + ((zO - zI) / dz);
+}
+
+template <typename C, typename T>
+__device__ __forceinline__ C diff_w(const FieldView<T>& w2, const MuFaces<C>& m, int i, int j, int k, int P, C dx,
+                                    C dy, C dz)
+{
+    C c = w2(i, j, k);
+    C xE = m.xp * ((C(w2(i + 1, j, k)) - c) / dx);
+    C xW = m.xm * ((c - C(w2(i - 1, j, k))) / dx);
+    C yN = m.yp * ((C(w2(i, wrapj(j + 1, P), k)) - c) / dy);
+    C yS = m.ym * ((c - C(w2(i, wrapj(j - 1, P), k))) / dy);
+    C zO = m.zp * ((C(w2(i, j, k + 1)) - c) / dz);
+    C zI = m.zm * ((c - C(w2(i, j, k - 1))) / dz);
+    return ((xE - xW) / dx) + ((yN - yS) / dy) + ((zO - zI) / dz);
+}
+
+// One thread per face cell (i in 1..maxl-1, j in 1..P, k over the component's
+// full z range).  comp 0/1/2 = u/v/w.
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_momentum(Grid g, int comp, int mode, T dx, T dy, T dz, T dt,
+                                                     FieldView<T> u2, FieldView<T> v2, FieldView<T> w2,
+                                                     FieldView<T> mu, FieldView<T> conv2, FieldView<T> diff,
+                                                     FieldView<T> fc, FieldView<T> conv0, FieldView<T> conv1,
+                                                     FieldView<T> csf0, FieldView<T> csf1, FieldView<T> csf2,
+                                                     FieldView<T> rhs)
+{
+    using C = T;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z;
+    const int P = g.P, maxn = g.maxn, maxl = g.maxl;
+    if (i > maxl - 1 || j > P) return;
+    bool in_range;
+    if (comp == 0) in_range = (i <= maxl - 2);
+    else if (comp == 1) in_range = true;
+    else in_range = (k >= 1 && k <= maxn - 2);
+    C cv = C(0), df = C(0);
+    if (in_range) {
+        if (comp == 0) {
+            if (mode & 1 || mode & 4) cv = conv_u<C>(u2, v2, w2, mu, i, j, k, maxn, P, dx, dy, dz);
+            if (mode & 2 || mode & 4) df = diff_uv<C>(u2, mu_faces_u<C>(mu, i, j, k, P), i, j, k, maxn, P, dx, dy, dz);
+        } else if (comp == 1) {
+            if (mode & 1 || mode & 4) cv = conv_v<C>(u2, v2, w2, mu, i, j, k, maxn, P, dx, dy, dz);
+            if (mode & 2 || mode & 4) df = diff_uv<C>(v2, mu_faces_v<C>(mu, i, j, k, P), i, j, k, maxn, P, dx, dy, dz);
+        } else {
+            if (mode & 1 || mode & 4) cv = conv_w<C>(u2, v2, w2, mu, i, j, k, P, dx, dy, dz);
+            if (mode & 2 || mode & 4) df = diff_w<C>(w2, mu_faces_w<C>(mu, i, j, k, P), i, j, k, P, dx, dy, dz);
+        }
+    }
+    if (mode & 1) conv2(i, j, k) = cv;
+    if (mode & 2) diff(i, j, k) = df;
+    if (mode & 4) {
+        // CSF face average, main_ns_hybrid.F90:629-641
+        const int jp = wrapj(j + 1, P);
+        C s2;
+        if (comp == 0) s2 = (C(fc(i + 1, jp, k + 1)) + C(fc(i, jp, k + 1))) / C(2);
+        else if (comp == 1) s2 = (C(fc(i, jp, k + 1)) + C(fc(i, j, k + 1))) / C(2);
+        else s2 = (C(fc(i, jp, k + 1)) + C(fc(i, jp, k))) / C(2);
+        csf2(i, j, k) = s2;
+        conv2(i, j, k) = cv;
+        // AB3 + Crank-Nicolson, main_ns_hybrid.F90:743-751
+        const FieldView<T>& x2 = comp == 0 ? u2 : (comp == 1 ? v2 : w2);
+        C r = C(x2(i, j, k)) -
+              dt * ((C(23.0 / 12.0)) * cv - (C(4.0 / 3.0)) * C(conv1(i, j, k)) + (C(5.0 / 12.0)) * C(conv0(i, j, k))) +
+              (dt / C(2)) * df +
+              dt * ((C(23.0 / 12.0)) * s2 - (C(4.0 / 3.0)) * C(csf1(i, j, k)) + (C(5.0 / 12.0)) * C(csf0(i, j, k)));
+        rhs(i, j, k) = r;
+    }
+}
+
+template <typename T>
+hipError_t launch_viscosity(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, FieldView<T> mu, int ilo,
+                            int ihi)
+{
+    dim3 grd((ihi - ilo + 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn + 1);
+    hipLaunchKernelGGL(k_viscosity<T>, grd, dim3(BX, BY), 0, s, g, T(ph.sw), T(ph.Re), T(ph.mu_minus), T(ph.mu_plus),
+                       phi, mu, ilo, ihi);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int comp, int mode, FieldView<T> u2,
+                           FieldView<T> v2, FieldView<T> w2, FieldView<T> mu, FieldView<T> conv2, FieldView<T> diff,
+                           FieldView<T> f_csf, FieldView<T> conv0, FieldView<T> conv1, FieldView<T> csf0,
+                           FieldView<T> csf1, FieldView<T> csf2, FieldView<T> rhs)
+{
+    const int nz = comp == 2 ? g.maxn : g.maxn - 1;
+    dim3 grd((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, nz);
+    hipLaunchKernelGGL(k_momentum<T>, grd, dim3(BX, BY), 0, s, g, comp, mode, T(ph.dx), T(ph.dy), T(ph.dz), T(ph.dt),
+                       u2, v2, w2, mu, conv2, diff, f_csf, conv0, conv1, csf0, csf1, csf2, rhs);
+    return hipGetLastError();
+}
+
+#define TPLS_INST_MOM(T)                                                                                          \
+    template hipError_t launch_viscosity<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, FieldView<T>,   \
+                                            int, int);                                                          \
+    template hipError_t launch_momentum<T>(hipStream_t, const Grid&, const Phys&, int, int, FieldView<T>,        \
+                                           FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, \
+                                           FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, \
+                                           FieldView<T>, FieldView<T>);
+TPLS_INST_MOM(double)
+TPLS_INST_MOM(float)
+
+}  // namespace tpls

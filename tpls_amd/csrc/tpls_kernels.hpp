@@ -1,0 +1,133 @@
+// Kernel launchers of the TPLS stencil core (gfx950).  Each launcher enqueues
+// on the given stream and returns the launch status.  Index conventions are
+// the reference's (see tpls_dev.hpp); every launcher assumes the single-rank
+// domain sx = 1, ex = maxl-1, sy = 1, ey = maxm-1.
+#pragma once
+
+#include "tpls_dev.hpp"
+
+namespace tpls {
+
+struct Phys {
+    double dx, dy, dz, dt, Re, mu_plus, mu_minus, height, sw, scap;
+};
+
+// Block-partial reductions: each launch writes one partial per block into
+// `part` and the last block to finish sums them in block order into `out`
+// (deterministic for a fixed launch shape).  `counter` must be 0 on entry
+// and is reset to 0 by that last block.
+struct Reduce {
+    double* part;
+    double* out;
+    unsigned* counter;
+};
+
+// ---- advection (weno.f90, advect_phi.f90) ----
+template <typename T>
+hipError_t launch_advect_phi(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi2,
+                             FieldView<T> u2, FieldView<T> v2, FieldView<T> w2, FieldView<T> conv2_phi);
+template <typename T>
+hipError_t launch_weno_faces(hipStream_t s, const Grid& g, FieldView<T> phi, FieldView<T> u,
+                             FieldView<T> v, FieldView<T> w, FieldView<T> dfx, FieldView<T> dfy,
+                             FieldView<T> dfz);
+
+// ---- level set (two_phase_levelset.f90, levelset.f90) ----
+template <typename T>
+hipError_t launch_tpls_update(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi2,
+                              FieldView<T> c0, FieldView<T> c1, FieldView<T> c2, FieldView<T> phi3);
+template <typename T>
+hipError_t launch_phi_bc(hipStream_t s, const Grid& g, double dz, double height, FieldView<T> phi,
+                         int inlet_outlet);
+template <typename T>
+hipError_t launch_levelset_status(hipStream_t s, const Grid& g, double h, FieldView<T> phi,
+                                  FieldView<T> D, FieldView<int> ist, FieldView<T> sign);
+template <typename T>
+hipError_t launch_levelset_iter(hipStream_t s, const Grid& g, double h, FieldView<T> phi,
+                                FieldView<T> D, FieldView<int> ist, FieldView<T> sign,
+                                FieldView<T> pr, FieldView<T> pr_new);
+template <typename T>
+hipError_t launch_levelset_err(hipStream_t s, const Grid& g, FieldView<T> a, FieldView<T> b,
+                               Reduce red);
+template <typename T>
+hipError_t launch_csf(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi,
+                      FieldView<T> fx, FieldView<T> fy, FieldView<T> fz);
+template <typename T>
+hipError_t launch_csf_bc(hipStream_t s, const Grid& g, FieldView<T> fx, FieldView<T> fy,
+                         FieldView<T> fz);
+
+// ---- momentum (momentum_allflux.f90, main_ns_hybrid.F90:629-757) ----
+template <typename T>
+hipError_t launch_viscosity(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi,
+                            FieldView<T> mu, int ilo, int ihi);
+// mode bit 1: write conv, bit 2: write diff, bit 4: fused AB3/CN right-hand
+// side (needs conv0/1, csf faces and histories)
+template <typename T>
+hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int comp, int mode,
+                           FieldView<T> u2, FieldView<T> v2, FieldView<T> w2, FieldView<T> mu,
+                           FieldView<T> conv2, FieldView<T> diff, FieldView<T> f_csf,
+                           FieldView<T> conv0, FieldView<T> conv1, FieldView<T> csf0,
+                           FieldView<T> csf1, FieldView<T> csf2, FieldView<T> rhs);
+
+// ---- Helmholtz sweeps (jacobi_iteration_allflux.f90, sor_iteration_allflux.f90) ----
+template <typename T>
+hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,
+                         FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, bool xold_is_zero);
+template <typename T>
+hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> x,
+                      FieldView<T> rhs, FieldView<T> mu, int flg, int iteration_time);
+// outer-loop boundary conditions (main_ns_hybrid.F90:764-776, 826-836,
+// 864-879; the same statements close do_jacobi_v / do_jacobi_w)
+template <typename T>
+hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int comp,
+                               FieldView<T> x, FieldView<T> pres);
+template <typename T>
+hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b,
+                             Reduce red);
+
+// ---- pressure (pressure.f90, pressure_solver.F90) ----
+template <typename T>
+hipError_t launch_source_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3,
+                              FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p);
+template <typename T>
+hipError_t launch_uvw_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> p,
+                           const T* u_inlet, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3);
+
+// Conjugate-gradient state on the device (PETSc 3.6 KSPCG with pc none and a
+// constant null space, restated on the reduced symmetric system).
+struct CgState {
+    double beta, betaold, alpha, m, bnorm, rnorm0, ttol, dp, dpi;
+    double rtol, abstol, dtol, mb;
+    int its, i, reason, done, maxits, pad;
+};
+// All CG vectors use the p-grid layout and touch interior cells only.
+template <typename T>
+hipError_t launch_cg_setup_b(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3,
+                             FieldView<T> v3, FieldView<T> w3, const T* u_inlet,
+                             FieldView<double> rhs_p, int use_rhs, FieldView<double> b, Reduce red,
+                             CgState* st);
+template <typename T>
+hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x,
+                             FieldView<double> r, Reduce red2, CgState* st);
+hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> r,
+                       FieldView<double> p_old, FieldView<double> p_new, Reduce red, CgState* st);
+hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> p,
+                       FieldView<double> x, FieldView<double> r, Reduce red2, CgState* st);
+template <typename T>
+hipError_t launch_pres_ghosts(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x,
+                              FieldView<T> u3, const T* u_inlet, FieldView<T> pres);
+
+// ---- utilities ----
+template <typename T>
+hipError_t launch_fill(hipStream_t s, T* p, long n, T v);
+template <typename T>
+hipError_t launch_copy(hipStream_t s, T* dst, const T* src, long n);
+
+// Launch-shape helpers shared by the .hip files.
+constexpr int BX = 64;  // one wavefront along x
+constexpr int BY = 4;
+constexpr int REDUCE_BLOCKS = 1024;  // grid of the grid-stride reduction kernels
+
+// Number of blocks of the CG kernels (= partials per reduction).
+long cg_num_blocks(const Grid& g);
+
+}  // namespace tpls
