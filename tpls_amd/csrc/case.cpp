@@ -122,7 +122,6 @@ struct tpls_case {
     DevArray* inlet;
     double* red_part = nullptr;
     double* red_out = nullptr;
-    unsigned* red_cnt = nullptr;
     CgState* cg = nullptr;
     CgState* h_cg = nullptr;   // pinned host mirror
     double* h_out = nullptr;   // pinned host mirror of red_out
@@ -325,11 +324,9 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);
     if (hipMalloc(&c->red_part, sizeof(double) * 3 * nb) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "partials"));
     if (hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "outs"));
-    if (hipMalloc(&c->red_cnt, sizeof(unsigned) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "counter"));
     if (hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "cg state"));
     if (hipHostMalloc(&c->h_cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
     if (hipHostMalloc(&c->h_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
-    if (hipMemsetAsync(c->red_cnt, 0, sizeof(unsigned) * 8, c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "memset"));
     if (hipMemsetAsync(c->red_out, 0, sizeof(double) * 8, c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "memset"));
     if (hipStreamSynchronize(c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "sync"));
     *out = c;
@@ -346,7 +343,6 @@ int tpls_case_destroy(tpls_case* c)
     }
     hipFree(c->red_part);
     hipFree(c->red_out);
-    hipFree(c->red_cnt);
     hipFree(c->cg);
     if (c->h_cg) hipHostFree(c->h_cg);
     if (c->h_out) hipHostFree(c->h_out);
@@ -473,7 +469,7 @@ int tpls_step(tpls_case* c, int nsteps)
 // ---------------------------------------------------------------------------
 namespace {
 
-Reduce red_of(tpls_case* c, int slot) { return Reduce{c->red_part, c->red_out + slot, c->red_cnt}; }
+Reduce red_of(tpls_case* c, int slot) { return Reduce{c->red_part, c->red_out + slot}; }
 
 #define LCHK(x)                                                                              \
     do {                                                                                     \
@@ -688,7 +684,6 @@ struct DropCtx {
     std::vector<size_t> cap;
     double* red_part = nullptr;
     double* red_out = nullptr;
-    unsigned* red_cnt = nullptr;
     double* h_out = nullptr;
     // cached device case for tpls_pressure_solve
     tpls_case* pcase = nullptr;
@@ -699,7 +694,6 @@ struct DropCtx {
         for (void* p : buf) hipFree(p);
         hipFree(red_part);
         hipFree(red_out);
-        hipFree(red_cnt);
         if (h_out) hipHostFree(h_out);
         if (s) hipStreamDestroy(s);
     }
@@ -716,9 +710,7 @@ int drop_init()
     HIPCHK(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&d.red_part, sizeof(double) * 3 * REDUCE_BLOCKS));
     HIPCHK(hipMalloc(&d.red_out, sizeof(double) * 8));
-    HIPCHK(hipMalloc(&d.red_cnt, sizeof(unsigned) * 8));
     HIPCHK(hipHostMalloc(&d.h_out, sizeof(double) * 8));
-    HIPCHK(hipMemsetAsync(d.red_cnt, 0, sizeof(unsigned) * 8, d.s));
     return TPLS_OK;
 }
 
@@ -875,7 +867,7 @@ int tpls_do_tpls(double* phi2, double* phi3, double* c0, double* c1, double* c2,
                                           aS.a.view<double>()));
     Stg* src = &a3;
     Stg* bufs[2] = {&ra, &rb};
-    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    Reduce red{g_drop.red_part, g_drop.red_out};
     for (int it = 1; it <= max_it; ++it) {
         Stg* dst = bufs[(it - 1) & 1];
         HIPCHK(launch_levelset_iter<double>(s, g, dx, a3.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
@@ -938,7 +930,7 @@ int tpls_do_levelset_iteration(const double* phi, double* pr, const double* D, c
     HIPCHK(launch_levelset_iter<double>(s, g, h, a.a.view<double>(), aD.a.view<double>(), aI.a.view<int>(),
                                         aS.a.view<double>(), r0.a.view<double>(), r1.a.view<double>()));
     HIPCHK(launch_phi_bc<double>(s, g, h, height, r1.a.view<double>(), 1));
-    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    Reduce red{g_drop.red_part, g_drop.red_out};
     HIPCHK(launch_levelset_err<double>(s, g, r0.a.view<double>(), r1.a.view<double>(), red));
     RC(unstage({&r1}));
     HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1086,7 +1078,7 @@ int tpls_get_difference(const double* ua, const double* ub, int ex, int ey, int 
     Stg a, b;
     RC(stage(0, ua, sh, 8, 0, false, a));
     RC(stage(1, ub, sh, 8, 0, false, b));
-    Reduce red{g_drop.red_part, g_drop.red_out, g_drop.red_cnt};
+    Reduce red{g_drop.red_part, g_drop.red_out};
     HIPCHK(launch_difference<double>(g_drop.s, g, ez + 1, a.a.view<double>(), b.a.view<double>(), red));
     HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, g_drop.s));
     HIPCHK(hipStreamSynchronize(g_drop.s));

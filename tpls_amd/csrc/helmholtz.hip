@@ -131,19 +131,25 @@ __global__ void k_bc_uv(Grid g, int comp, T dtdx, FieldView<T> x, FieldView<T> p
     }
 }
 
+// Two independent segments: threads [0, P*(maxn-2)) copy the inlet/outlet
+// columns for k in 1..maxn-2; threads [P*(maxn-2), P*(maxn-2) + (maxl+1)*P)
+// zero both walls of column (i, j).
 template <typename T>
 __global__ void k_bc_w(Grid g, FieldView<T> w)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1;
-    const int L = g.maxl, n = g.maxn;
-    if (i > L) return;
-    if (i == 0 || i == L) {
-        const int src = i == 0 ? 1 : L - 1;
-        for (int k = 1; k <= n - 2; ++k) w(i, j, k) = w(src, j, k);
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int L = g.maxl, n = g.maxn, P = g.P;
+    const long n1 = long(P) * (n - 2), n2 = long(L + 1) * P;
+    if (q < n1) {
+        const int k = int(q % (n - 2)) + 1, j = int(q / (n - 2)) + 1;
+        w(0, j, k) = w(1, j, k);
+        w(L, j, k) = w(L - 1, j, k);
+    } else if (q < n1 + n2) {
+        const long r = q - n1;
+        const int i = int(r % (L + 1)), j = int(r / (L + 1)) + 1;
+        w(i, j, 0) = T(0);
+        w(i, j, n - 1) = T(0);
     }
-    w(i, j, 0) = T(0);
-    w(i, j, n - 1) = T(0);
 }
 
 static inline int nz_of(int comp, const Grid& g) { return comp == 2 ? g.maxn - 2 : g.maxn - 1; }
@@ -182,7 +188,8 @@ hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int
                                FieldView<T> pres)
 {
     if (comp == 2) {
-        hipLaunchKernelGGL(k_bc_w<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, x);
+        const long n = long(g.P) * (g.maxn - 2) + long(g.maxl + 1) * g.P;
+        hipLaunchKernelGGL(k_bc_w<T>, dim3((n + 255) / 256), dim3(256), 0, s, g, x);
     } else {
         hipLaunchKernelGGL(k_bc_uv<T>, dim3((g.maxn - 1 + 63) / 64, g.P), dim3(64), 0, s, g, comp, T(ph.dt / ph.dx), x,
                            pres);

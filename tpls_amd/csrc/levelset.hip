@@ -1,7 +1,6 @@
 // Level-set update, Russo-Smereka reinitialisation and CSF force:
 // two_phase_levelset.f90:20-118 and levelset.f90:17-414.
 #include "tpls_kernels.hpp"
-#include "reduce.hpp"
 
 namespace tpls {
 
@@ -18,26 +17,29 @@ __global__ void __launch_bounds__(BX* BY) k_tpls_update(Grid g, T dt, FieldView<
     phi3(i, j, k) = phi2(i, j, k) - dt * rhs;
 }
 
-// Boundary planes of a level-set field, one thread per (i, j) column,
-// i in 0..maxl: inlet phi = z - height and outlet copy on k = 1..n-1
-// (two_phase_levelset.f90:72-82, levelset.f90:170-191), then linear
-// extrapolation to k = 0 and k = n (two_phase_levelset.f90:94-95,
+// Boundary planes of a level-set field.  Pass 1, one thread per (j, k),
+// k in 1..n-1: inlet phi = z - height and outlet copy (two_phase_levelset.f90:
+// 72-82, levelset.f90:170-191).  Pass 2, one thread per (i, j), i in 0..maxl:
+// linear extrapolation to k = 0 and k = n (two_phase_levelset.f90:94-95,
 // levelset.f90:193-216).
 template <typename T>
-__global__ void k_phi_bc(Grid g, T dz, T height, FieldView<T> phi, int inlet_outlet)
+__global__ void k_phi_inout(Grid g, T dz, T height, FieldView<T> phi)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (k > g.maxn - 1) return;
+    T z = T(k) * dz - T(0.5) * dz;
+    phi(0, j, k) = z - height;
+    phi(g.maxl, j, k) = phi(g.maxl - 1, j, k);
+}
+
+template <typename T>
+__global__ void k_phi_zext(Grid g, FieldView<T> phi)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y + 1;
     const int n = g.maxn;
     if (i > g.maxl) return;
-    if (inlet_outlet && i == 0) {
-        for (int k = 1; k <= n - 1; ++k) {
-            T z = T(k) * dz - T(0.5) * dz;
-            phi(0, j, k) = z - height;
-        }
-    } else if (inlet_outlet && i == g.maxl) {
-        for (int k = 1; k <= n - 1; ++k) phi(i, j, k) = phi(i - 1, j, k);
-    }
     phi(i, j, 0) = T(2) * phi(i, j, 1) - phi(i, j, 2);
     phi(i, j, n) = T(2) * phi(i, j, n - 1) - phi(i, j, n - 2);
 }
@@ -116,26 +118,6 @@ __global__ void __launch_bounds__(BX* BY) k_levelset_iter(Grid g, T h, FieldView
     pn(i, j, k) = c - dtau * dc / h;
 }
 
-// Diagnostic of levelset.f90:218-228: sum over i in 1..maxl-1, all j, k in
-// 0..n of (pr - pn)^2 (the reference's reduction(MAX:) clause is inert for a
-// single thread, leaving the sum).
-template <typename T>
-__global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, FieldView<T> b, Reduce red)
-{
-    const int nx = g.maxl - 1;
-    const long n = (long)nx * g.P * nz;
-    double acc = 0.0;
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
-        int i = int(q % nx) + 1;
-        long r = q / nx;
-        int j = int(r % g.P) + 1;
-        int k = int(r / g.P);
-        double d = double(a(i, j, k)) - double(b(i, j, k));
-        acc += d * d;
-    }
-    block_reduce_finish(acc, red);
-}
-
 // levelset.f90:270-363: curvature (clipped above at 1/min(dx,dy,dz)) and the
 // CSF force delta(phi) * scap * kappa * grad(phi), p interior.
 template <typename T>
@@ -201,30 +183,36 @@ __device__ __forceinline__ T zext(const FieldView<T>& f, int i, int j, int k, in
     return f(i, j, k);
 }
 
-template <typename T>
-__device__ __forceinline__ void csf_col_bc(const FieldView<T>& f, int i, int j, int maxl, int n)
-{
-    if (i >= 1 && i <= maxl - 1) {
-        T a = T(2) * f(i, j, 1) - f(i, j, 2);
-        T b = T(2) * f(i, j, n - 1) - f(i, j, n - 2);
-        f(i, j, 0) = a;
-        f(i, j, n) = b;
-    } else if (i == 0) {
-        for (int k = 0; k <= n; ++k) f(0, j, k) = T(2) * zext(f, 1, j, k, n) - zext(f, 2, j, k, n);
-    } else {
-        for (int k = 0; k <= n; ++k) f(maxl, j, k) = T(2) * zext(f, maxl - 1, j, k, n) - zext(f, maxl - 2, j, k, n);
-    }
-}
-
+// One kernel, two independent index segments: threads [0, (maxl-1)*P) do
+// the z extrapolation of interior column (i, j); threads
+// [(maxl-1)*P, (maxl-1)*P + (maxn+1)*P) do the x extrapolation of (j, k) at
+// both ends, rebuilding the extrapolated k = 0, n values of columns 1, 2,
+// maxl-2, maxl-1 locally.
 template <typename T>
 __global__ void k_csf_bc(Grid g, FieldView<T> fx, FieldView<T> fy, FieldView<T> fz)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1;
-    if (i > g.maxl) return;
-    csf_col_bc(fx, i, j, g.maxl, g.maxn);
-    csf_col_bc(fy, i, j, g.maxl, g.maxn);
-    csf_col_bc(fz, i, j, g.maxl, g.maxn);
+    const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int L = g.maxl, n = g.maxn, P = g.P;
+    const long n1 = long(L - 1) * P, n2 = long(n + 1) * P;
+    FieldView<T> fs[3] = {fx, fy, fz};
+    if (q < n1) {
+        const int i = int(q % (L - 1)) + 1, j = int(q / (L - 1)) + 1;
+        for (int c = 0; c < 3; ++c) {
+            const FieldView<T>& f = fs[c];
+            T a = T(2) * f(i, j, 1) - f(i, j, 2);
+            T b = T(2) * f(i, j, n - 1) - f(i, j, n - 2);
+            f(i, j, 0) = a;
+            f(i, j, n) = b;
+        }
+    } else if (q < n1 + n2) {
+        const long r = q - n1;
+        const int k = int(r % (n + 1)), j = int(r / (n + 1)) + 1;
+        for (int c = 0; c < 3; ++c) {
+            const FieldView<T>& f = fs[c];
+            f(0, j, k) = T(2) * zext(f, 1, j, k, n) - zext(f, 2, j, k, n);
+            f(L, j, k) = T(2) * zext(f, L - 1, j, k, n) - zext(f, L - 2, j, k, n);
+        }
+    }
 }
 
 static inline dim3 grid3(const Grid& g, int nzb) { return dim3((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, nzb); }
@@ -241,8 +229,9 @@ hipError_t launch_tpls_update(hipStream_t s, const Grid& g, const Phys& ph, Fiel
 template <typename T>
 hipError_t launch_phi_bc(hipStream_t s, const Grid& g, double dz, double height, FieldView<T> phi, int inlet_outlet)
 {
-    hipLaunchKernelGGL(k_phi_bc<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, T(dz), T(height), phi,
-                       inlet_outlet);
+    if (inlet_outlet)
+        hipLaunchKernelGGL(k_phi_inout<T>, dim3((g.maxn - 1 + 63) / 64, g.P), dim3(64), 0, s, g, T(dz), T(height), phi);
+    hipLaunchKernelGGL(k_phi_zext<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, phi);
     return hipGetLastError();
 }
 
@@ -264,20 +253,6 @@ hipError_t launch_levelset_iter(hipStream_t s, const Grid& g, double h, FieldVie
 }
 
 template <typename T>
-hipError_t launch_levelset_err(hipStream_t s, const Grid& g, FieldView<T> a, FieldView<T> b, Reduce red)
-{
-    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, g.maxn + 1, a, b, red);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b, Reduce red)
-{
-    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, nz, a, b, red);
-    return hipGetLastError();
-}
-
-template <typename T>
 hipError_t launch_csf(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, FieldView<T> fx,
                       FieldView<T> fy, FieldView<T> fz)
 {
@@ -289,7 +264,8 @@ hipError_t launch_csf(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T>
 template <typename T>
 hipError_t launch_csf_bc(hipStream_t s, const Grid& g, FieldView<T> fx, FieldView<T> fy, FieldView<T> fz)
 {
-    hipLaunchKernelGGL(k_csf_bc<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, fx, fy, fz);
+    const long n = long(g.maxl - 1) * g.P + long(g.maxn + 1) * g.P;
+    hipLaunchKernelGGL(k_csf_bc<T>, dim3((n + 255) / 256), dim3(256), 0, s, g, fx, fy, fz);
     return hipGetLastError();
 }
 
@@ -301,8 +277,6 @@ hipError_t launch_csf_bc(hipStream_t s, const Grid& g, FieldView<T> fx, FieldVie
                                                   FieldView<int>, FieldView<T>);                                     \
     template hipError_t launch_levelset_iter<T>(hipStream_t, const Grid&, double, FieldView<T>, FieldView<T>,        \
                                                 FieldView<int>, FieldView<T>, FieldView<T>, FieldView<T>);           \
-    template hipError_t launch_levelset_err<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, Reduce);        \
-    template hipError_t launch_difference<T>(hipStream_t, const Grid&, int, FieldView<T>, FieldView<T>, Reduce);     \
     template hipError_t launch_csf<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, FieldView<T>,             \
                                       FieldView<T>, FieldView<T>);                                                   \
     template hipError_t launch_csf_bc<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, FieldView<T>);

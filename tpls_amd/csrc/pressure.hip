@@ -13,27 +13,43 @@
 // |z| <= max(rtol*|b|, abstol) (KSPConvergedDefault with a nonzero initial
 // guess); divergence if |z| >= dtol*|b|.
 //
-// Each iteration is two kernels, each a single pass with one fused grid
-// reduction:
-//   cg_p: p <- z + beta/beta_old * p (double-buffered, recomputed at the
-//         stencil neighbours), w = A p kept in registers, partial p.w;
-//         the last block forms alpha.
-//   cg_x: x <- x + alpha p, r <- r - alpha A p (A p recomputed from p),
-//         partial sums of r and r^2; the last block forms mean(r), beta, |z|
-//         and runs the convergence test, raising st->done.
-// Every CG kernel returns at once when st->done is set, so the host can
-// enqueue iterations ahead of the convergence test.  HBM traffic per
-// iteration: r, p_old read and p written (24 B) + p, x, r read and x, r
-// written (40 B) = 64 B per unknown.
+// An iteration is two streaming kernels, each followed by a one-block kernel
+// that finishes its reduction and the scalar recurrences:
+//   cg_p:  p <- z + beta/beta_old * p (double-buffered; each plane of the new
+//          p is shared through LDS for the stencil), w = A p kept in
+//          registers, partials of p.w;
+//   cg_pf: alpha = beta / p.w;
+//   cg_x:  x <- x + alpha p, r <- r - alpha A p (A p recomputed from p),
+//          partials of sum r and sum r^2;
+//   cg_xf: mean(r), beta = |z|^2, |z| and the convergence test, which raises
+//          st->done.
+// Every CG kernel returns at once when st->done is set, so the host enqueues
+// iterations ahead of the convergence test.  HBM traffic per iteration:
+// r, p_old read and p written (24 B) + p, x, r read and x, r written (40 B)
+// = 64 B per unknown.
+//
+// The streaming kernels walk tiles of 64 x 4 columns x 4 levels (one
+// wavefront spans a 512-byte row segment; the 4-level march reuses the z
+// neighbours in registers), grid-stride over at most CG_BLOCKS blocks.
 #include "reduce.hpp"
 #include "tpls_kernels.hpp"
 
 namespace tpls {
 
+constexpr int CG_KC = 4;
+constexpr int CG_BLOCKS = 4096;
+
 // ---- get_source_pres (pressure.f90:38-50): rhs_p = div(u3)/dt ----
+// Divergence of the provisional velocity (pressure.f90:42-43), evaluated in
+// double precision for every storage type.
 template <typename T>
 __device__ __forceinline__ double div_at(const FieldView<T>& u3, const FieldView<T>& v3, const FieldView<T>& w3, int i,
-                                         int j, int jm, int k, double dx, double dy, double dz);
+                                         int j, int jm, int k, double dx, double dy, double dz)
+{
+    return ((double(u3(i, jm, k - 1)) - double(u3(i - 1, jm, k - 1))) / dx) +
+           ((double(v3(i, j, k - 1)) - double(v3(i, jm, k - 1))) / dy) +
+           ((double(w3(i, jm, k)) - double(w3(i, jm, k - 1))) / dz);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(BX* BY) k_source_pres(Grid g, double dx, double dy, double dz, double dt,
@@ -95,107 +111,83 @@ __device__ __forceinline__ double apply_op(const Op& o, double c, double xm, dou
     return o.diag * c - o.cz * zm - o.cx * xm - o.cy * ym - o.cy * yp - o.cx * xp - o.cz * zp;
 }
 
-// Launch shape of the CG kernels: 64 x 4 columns per block, each thread
-// marching kc consecutive levels.
-struct CgShape {
-    dim3 grid;
-    int kc;
+struct Tiles {
+    int nbx, nby, nbz, ntiles, nblocks;
 };
 
-static CgShape cg_shape(const Grid& g)
+static Tiles cg_tiles(const Grid& g)
 {
-    const int nx = g.maxl - 1, nz = g.maxn - 1;
-    const int bxy = ((nx + BX - 1) / BX) * ((g.P + BY - 1) / BY);
-    int nch = (2048 + bxy - 1) / bxy;
-    if (nch > nz) nch = nz;
-    if (nch < 1) nch = 1;
-    const int kc = (nz + nch - 1) / nch;
-    nch = (nz + kc - 1) / kc;
-    CgShape s;
-    s.grid = dim3((nx + BX - 1) / BX, (g.P + BY - 1) / BY, nch);
-    s.kc = kc;
-    return s;
+    Tiles t;
+    t.nbx = (g.maxl - 1 + BX - 1) / BX;
+    t.nby = (g.P + BY - 1) / BY;
+    t.nbz = (g.maxn - 1 + CG_KC - 1) / CG_KC;
+    t.ntiles = t.nbx * t.nby * t.nbz;
+    t.nblocks = t.ntiles < CG_BLOCKS ? t.ntiles : CG_BLOCKS;
+    return t;
 }
 
-long cg_num_blocks(const Grid& g)
-{
-    CgShape sh = cg_shape(g);
-    return long(sh.grid.x) * sh.grid.y * sh.grid.z;
-}
+long cg_num_blocks(const Grid& g) { return cg_tiles(g).nblocks; }
 
-__device__ __forceinline__ int tid256() { return threadIdx.x + BX * threadIdx.y; }
-__device__ __forceinline__ int bid3() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-__device__ __forceinline__ int nblk3() { return gridDim.x * gridDim.y * gridDim.z; }
-
-// Reduction of up to 3 values over the grid; returns true in the last block
-// with the totals in thread 0's tot[].
-__device__ __forceinline__ bool grid_sums(int nval, double* v, const Reduce& red, double* tot)
+// Decode tile t into the first interior cell (i, j, k0) of this thread.
+__device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int& j, int& k0)
 {
-    __shared__ double sm[8];
-    double s[3];
-    for (int q = 0; q < nval; ++q) s[q] = block_sum256(v[q], sm);
-    if (!publish_partials(red, nval, s, nblk3(), bid3())) return false;
-    for (int q = 0; q < nval; ++q) tot[q] = sum_partials(red, q, nblk3(), sm);
-    return true;
-}
-
-// Divergence of the provisional velocity, get_source_pres (pressure.f90:42-43),
-// evaluated in double precision for every storage type.
-template <typename T>
-__device__ __forceinline__ double div_at(const FieldView<T>& u3, const FieldView<T>& v3, const FieldView<T>& w3, int i,
-                                         int j, int jm, int k, double dx, double dy, double dz)
-{
-    return ((double(u3(i, jm, k - 1)) - double(u3(i - 1, jm, k - 1))) / dx) +
-           ((double(v3(i, j, k - 1)) - double(v3(i, jm, k - 1))) / dy) +
-           ((double(w3(i, jm, k)) - double(w3(i, jm, k - 1))) / dz);
+    const int bx = t % tl.nbx;
+    const int r = t / tl.nbx;
+    const int by = r % tl.nby;
+    const int bz = r / tl.nby;
+    i = bx * BX + threadIdx.x + 1;
+    j = by * BY + threadIdx.y + 1;
+    k0 = bz * CG_KC + 1;
 }
 
 // b = -RHS_p with the inlet Neumann flux folded in (pressure_solver.F90:272-286),
-// written into the r buffer; the last block stores mean(b).  RHS_p is either
-// read (use_rhs) or formed on the fly as div(u3)/dt.
+// written into the r buffer; partials of sum b.  RHS_p is either read
+// (use_rhs) or formed on the fly as div(u3)/dt.
 template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, double dx, double dy, double dz, double dt, int kc,
+__global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, Tiles tl, double dx, double dy, double dz, double dt,
                                                        FieldView<T> u3, FieldView<T> v3, FieldView<T> w3,
                                                        const T* u_inlet, FieldView<double> rhs_p, int use_rhs,
-                                                       FieldView<double> b, Reduce red, CgState* st)
+                                                       FieldView<double> b, double* part)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
-    const int nz = g.maxn - 1;
-    double acc = 0.0;
-    if (i <= g.maxl - 1 && j <= g.P) {
+    const int nx = g.maxl - 1, nz = g.maxn - 1;
+    double acc[1] = {0.0};
+    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+        int i, j, k0;
+        tile_origin(tl, t, i, j, k0);
+        if (i > nx || j > g.P) continue;
         const int jm = wrapj(j - 1, g.P);
-        for (int k = k0; k < k0 + kc && k <= nz; ++k) {
+        for (int k = k0; k < k0 + CG_KC && k <= nz; ++k) {
             double val = use_rhs ? -rhs_p(i, j, k) : -(div_at(u3, v3, w3, i, j, jm, k, dx, dy, dz) / dt);
             if (i == 1) {
                 double b_in = (dx / dt) * (double(u_inlet[k - 1]) - double(u3(0, jm, k - 1))) / dy;
                 val = val + b_in / dy;
             }
             b(i, j, k) = val;
-            acc += val;
+            acc[0] += val;
         }
     }
-    double tot[1];
-    if (grid_sums(1, &acc, red, tot) && tid256() == 0) {
-        const double N = double(g.maxl - 1) * g.P * (g.maxn - 1);
-        st->mb = tot[0] / N;
-        *red.counter = 0u;
-    }
+    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
 }
 
-// b <- b - mean(b); r = b - A x; sums |b|^2, sum r, sum r^2; initial test.
-__global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Grid g, Op o, int kc, FieldView<double> x, FieldView<double> r,
-                                                       Reduce red, CgState* st)
+__global__ void __launch_bounds__(256) k_cg_setup_bf(const double* part, int nb, double N, CgState* st)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
+    double tot[1];
+    finish_sums<1>(part, nb, tot);
+    if (tid256() == 0) st->mb = tot[0] / N;
+}
+
+// b <- b - mean(b); r = b - A x; partials of |b|^2, sum r, sum r^2.
+__global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Op o, Tiles tl, FieldView<double> x, FieldView<double> r,
+                                                       double* part, const CgState* st)
+{
     const double mb = st->mb;
     double acc[3] = {0.0, 0.0, 0.0};
-    if (i <= o.nx && j <= o.ny) {
+    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+        int i, j, k0;
+        tile_origin(tl, t, i, j, k0);
+        if (i > o.nx || j > o.ny) continue;
         const int jm = wrapj(j - 1, o.ny), jp = wrapj(j + 1, o.ny);
-        for (int k = k0; k < k0 + kc && k <= o.nz; ++k) {
+        for (int k = k0; k < k0 + CG_KC && k <= o.nz; ++k) {
             double c = x(i, j, k);
             double ax = apply_op(o, c, i > 1 ? x(i - 1, j, k) : c, i < o.nx ? x(i + 1, j, k) : c, x(i, jm, k),
                                  x(i, jp, k), k > 1 ? x(i, j, k - 1) : c, k < o.nz ? x(i, j, k + 1) : c);
@@ -207,99 +199,136 @@ __global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Grid g, Op o, int kc, Fie
             acc[2] += rr * rr;
         }
     }
+    block_partials<3>(acc, part, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb, double N, CgState* st)
+{
     double tot[3];
-    if (grid_sums(3, acc, red, tot) && tid256() == 0) {
-        const double N = double(o.nx) * o.ny * o.nz;
-        double bnorm = sqrt(tot[0]);
-        double m = tot[1] / N;
-        double beta = tot[2] - tot[1] * m;
-        double dp = sqrt(beta > 0.0 ? beta : 0.0);
-        double rnorm0 = bnorm > 0.0 ? bnorm : dp;
-        double ttol = st->rtol * rnorm0 > st->abstol ? st->rtol * rnorm0 : st->abstol;
-        st->bnorm = bnorm;
-        st->m = m;
-        st->beta = beta;
-        st->betaold = 0.0;
-        st->dp = dp;
-        st->rnorm0 = rnorm0;
-        st->ttol = ttol;
-        st->its = 0;
-        st->i = 0;
-        st->reason = 0;
-        st->done = 0;
-        if (isnan(dp)) {
-            st->reason = -9;
-            st->done = 1;
-        } else if (dp <= ttol) {
-            st->reason = dp < st->abstol ? 3 : 2;
-            st->done = 1;
-        }
-        *red.counter = 0u;
+    finish_sums<3>(part, nb, tot);
+    if (tid256() != 0) return;
+    double bnorm = sqrt(tot[0]);
+    double m = tot[1] / N;
+    double beta = tot[2] - tot[1] * m;
+    double dp = sqrt(beta > 0.0 ? beta : 0.0);
+    double rnorm0 = bnorm > 0.0 ? bnorm : dp;
+    double ttol = st->rtol * rnorm0 > st->abstol ? st->rtol * rnorm0 : st->abstol;
+    st->bnorm = bnorm;
+    st->m = m;
+    st->beta = beta;
+    st->betaold = 0.0;
+    st->dp = dp;
+    st->rnorm0 = rnorm0;
+    st->ttol = ttol;
+    st->its = 0;
+    st->i = 0;
+    st->reason = 0;
+    st->done = 0;
+    if (isnan(dp)) {
+        st->reason = -9;
+        st->done = 1;
+    } else if (dp <= ttol) {
+        st->reason = dp < st->abstol ? 3 : 2;
+        st->done = 1;
     }
 }
 
-// p <- z + (beta/beta_old) p  (p <- z on the first iteration), w = A p, p.w
-__global__ void __launch_bounds__(BX* BY) k_cg_p(Grid g, Op o, int kc, FieldView<double> r, FieldView<double> po,
-                                                 FieldView<double> pn, Reduce red, CgState* st)
+// p <- z + (beta/beta_old) p  (p <- z on the first iteration), w = A p, p.w.
+// Each plane of the new p over the block's 64 x 4 columns is formed once and
+// shared through LDS; the x/y neighbours outside the block are formed by the
+// edge lanes / rows.  FIRST is the first iteration (p = z exactly).
+template <bool FIRST>
+__device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const FieldView<double>& r,
+                                           const FieldView<double>& po, const FieldView<double>& pn, double m,
+                                           double bt, double& acc)
 {
-    if (st->done) return;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
-    const double m = st->m;
-    const bool first = st->i == 0;
-    const double bt = first ? 0.0 : st->beta / st->betaold;
-    double acc = 0.0;
-    if (i <= o.nx && j <= o.ny) {
+    __shared__ double sp[BY][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    auto P = [&](int ii, int jj, int kk) -> double {
+        double z = r(ii, jj, kk) - m;
+        return FIRST ? z : z + bt * po(ii, jj, kk);
+    };
+    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+        int i, j, k0;
+        tile_origin(tl, t, i, j, k0);
+        const bool act = i <= o.nx && j <= o.ny;
         const int jm = wrapj(j - 1, o.ny), jp = wrapj(j + 1, o.ny);
-        auto P = [&](int ii, int jj, int kk) -> double {
-            double z = r(ii, jj, kk) - m;
-            return first ? z : z + bt * po(ii, jj, kk);
-        };
-        const int k1 = min(k0 + kc - 1, o.nz);
-        double pc = P(i, j, k0);
-        double pm = k0 > 1 ? P(i, j, k0 - 1) : pc;
-        for (int k = k0; k <= k1; ++k) {
-            double pp = k < o.nz ? P(i, j, k + 1) : pc;
-            double w = apply_op(o, pc, i > 1 ? P(i - 1, j, k) : pc, i < o.nx ? P(i + 1, j, k) : pc, P(i, jm, k),
-                                P(i, jp, k), k > 1 ? pm : pc, pp);
-            pn(i, j, k) = pc;
-            acc += pc * w;
+        double pc = 0.0, pm = 0.0;
+        if (act) {
+            pc = P(i, j, k0);
+            pm = k0 > 1 ? P(i, j, k0 - 1) : pc;
+        }
+        for (int q = 0; q < CG_KC; ++q) {
+            const int k = k0 + q;
+            if (k > o.nz) break;  // uniform over the block
+            double pp = 0.0;
+            if (act) pp = k < o.nz ? P(i, j, k + 1) : pc;
+            __syncthreads();
+            sp[ty][tx] = pc;
+            __syncthreads();
+            if (act) {
+                double xm = i == 1 ? pc : (tx > 0 ? sp[ty][tx - 1] : P(i - 1, j, k));
+                double xp = i == o.nx ? pc : (tx < BX - 1 ? sp[ty][tx + 1] : P(i + 1, j, k));
+                double ym = ty > 0 ? sp[ty - 1][tx] : P(i, jm, k);
+                double yp = (ty < BY - 1 && j < o.ny) ? sp[ty + 1][tx] : P(i, jp, k);
+                double w = apply_op(o, pc, xm, xp, ym, yp, k > 1 ? pm : pc, pp);
+                pn(i, j, k) = pc;
+                acc += pc * w;
+            }
             pm = pc;
             pc = pp;
         }
     }
+}
+
+__global__ void __launch_bounds__(BX* BY) k_cg_p(Op o, Tiles tl, FieldView<double> r, FieldView<double> po,
+                                                 FieldView<double> pn, double* part, const CgState* st)
+{
+    if (st->done) return;
+    const double m = st->m;
+    double acc[1] = {0.0};
+    if (st->i == 0)
+        cg_p_tiles<true>(o, tl, r, po, pn, m, 0.0, acc[0]);
+    else
+        cg_p_tiles<false>(o, tl, r, po, pn, m, st->beta / st->betaold, acc[0]);
+    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_cg_pf(const double* part, int nb, CgState* st)
+{
+    if (st->done) return;
     double tot[1];
-    if (grid_sums(1, &acc, red, tot) && tid256() == 0) {
-        double dpi = tot[0];
-        st->dpi = dpi;
-        if (!(dpi > 0.0)) {
-            st->reason = isnan(dpi) ? -9 : -10;
-            st->its = st->i + 1;
-            st->done = 1;
-        } else {
-            st->alpha = st->beta / dpi;
-        }
-        *red.counter = 0u;
+    finish_sums<1>(part, nb, tot);
+    if (tid256() != 0) return;
+    double dpi = tot[0];
+    st->dpi = dpi;
+    if (!(dpi > 0.0)) {
+        st->reason = isnan(dpi) ? -9 : -10;
+        st->its = st->i + 1;
+        st->done = 1;
+    } else {
+        st->alpha = st->beta / dpi;
     }
 }
 
-// x <- x + alpha p, r <- r - alpha A p, sum r, sum r^2, convergence test.
-__global__ void __launch_bounds__(BX* BY) k_cg_x(Grid g, Op o, int kc, FieldView<double> p, FieldView<double> x,
-                                                 FieldView<double> r, Reduce red, CgState* st)
+// x <- x + alpha p, r <- r - alpha A p, partials of sum r, sum r^2.
+__global__ void __launch_bounds__(BX* BY) k_cg_x(Op o, Tiles tl, FieldView<double> p, FieldView<double> x,
+                                                 FieldView<double> r, double* part, const CgState* st)
 {
     if (st->done) return;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
     const double a = st->alpha;
     double acc[2] = {0.0, 0.0};
-    if (i <= o.nx && j <= o.ny) {
+    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+        int i, j, k0;
+        tile_origin(tl, t, i, j, k0);
+        if (i > o.nx || j > o.ny) continue;
         const int jm = wrapj(j - 1, o.ny), jp = wrapj(j + 1, o.ny);
-        const int k1 = min(k0 + kc - 1, o.nz);
         double pc = p(i, j, k0);
         double pm = k0 > 1 ? p(i, j, k0 - 1) : pc;
-        for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < CG_KC; ++q) {
+            const int k = k0 + q;
+            if (k > o.nz) break;
             double pp = k < o.nz ? p(i, j, k + 1) : pc;
             double w = apply_op(o, pc, i > 1 ? p(i - 1, j, k) : pc, i < o.nx ? p(i + 1, j, k) : pc, p(i, jm, k),
                                 p(i, jp, k), k > 1 ? pm : pc, pp);
@@ -312,64 +341,94 @@ __global__ void __launch_bounds__(BX* BY) k_cg_x(Grid g, Op o, int kc, FieldView
             pc = pp;
         }
     }
+    block_partials<2>(acc, part, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_cg_xf(const double* part, int nb, double N, CgState* st)
+{
+    if (st->done) return;
     double tot[2];
-    if (grid_sums(2, acc, red, tot) && tid256() == 0) {
-        const double N = double(o.nx) * o.ny * o.nz;
-        double m = tot[0] / N;
-        double beta = tot[1] - tot[0] * m;
-        double dp = sqrt(beta > 0.0 ? beta : 0.0);
-        st->its = st->i + 1;
-        st->m = m;
-        st->betaold = st->beta;
-        st->beta = beta;
-        st->dp = dp;
-        int reason = 0;
-        if (isnan(dp) || isnan(beta))
-            reason = -9;
-        else if (dp <= st->ttol)
-            reason = dp < st->abstol ? 3 : 2;
-        else if (dp >= st->dtol * st->rnorm0)
-            reason = -4;
-        if (!reason) {
-            st->i = st->i + 1;
-            if (st->i >= st->maxits) reason = -3;
-        }
-        if (reason) {
-            st->reason = reason;
-            st->done = 1;
-        }
-        *red.counter = 0u;
+    finish_sums<2>(part, nb, tot);
+    if (tid256() != 0) return;
+    double m = tot[0] / N;
+    double beta = tot[1] - tot[0] * m;
+    double dp = sqrt(beta > 0.0 ? beta : 0.0);
+    st->its = st->i + 1;
+    st->m = m;
+    st->betaold = st->beta;
+    st->beta = beta;
+    st->dp = dp;
+    int reason = 0;
+    if (isnan(dp) || isnan(beta))
+        reason = -9;
+    else if (dp <= st->ttol)
+        reason = dp < st->abstol ? 3 : 2;
+    else if (dp >= st->dtol * st->rnorm0)
+        reason = -4;
+    if (!reason) {
+        st->i = st->i + 1;
+        if (st->i >= st->maxits) reason = -3;
+    }
+    if (reason) {
+        st->reason = reason;
+        st->done = 1;
     }
 }
 
 // Ghost planes of the solution (copy_pressure_out_of_petsc,
-// pressure_solver.F90:444-486, with the eliminated Neumann rows).  One thread
-// per (i, j) column, i in 0..maxl; pres may alias x.
+// pressure_solver.F90:444-486, with the eliminated Neumann rows), written
+// with the interior into pres.  One thread per cell, i in 0..maxl,
+// k in 0..maxn; the z ghost planes copy k = 1 and k = maxn-1.
 template <typename T>
-__global__ void k_pres_ghosts(Grid g, T dx, T dy, T dt, FieldView<double> x, FieldView<T> u3, const T* u_inlet,
-                              FieldView<T> pres)
+__global__ void __launch_bounds__(BX* BY) k_pres_out(Grid g, double dx, double dy, double dt, FieldView<double> x,
+                                                     FieldView<T> u3, const T* u_inlet, FieldView<T> pres)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z;
     const int L = g.maxl, n = g.maxn;
-    if (i > L) return;
-    const int jm = wrapj(j - 1, g.P);
-    for (int k = 1; k <= n - 1; ++k) {
-        double v;
-        if (i == 0) {
-            double b_in = (double(dx) / double(dt)) * (double(u_inlet[k - 1]) - double(u3(0, jm, k - 1))) / double(dy);
-            v = x(1, j, k) + double(dy) * b_in;
-        } else if (i == L) {
-            v = x(L - 1, j, k);
-        } else {
-            v = x(i, j, k);
-        }
-        pres(i, j, k) = T(v);
-        if (k == 1) pres(i, j, 0) = T(v);
-        if (k == n - 1) pres(i, j, n) = T(v);
+    if (i > L || j > g.P) return;
+    const int ks = k == 0 ? 1 : (k == n ? n - 1 : k);
+    double v;
+    if (i == 0) {
+        const int jm = wrapj(j - 1, g.P);
+        double b_in = (dx / dt) * (double(u_inlet[ks - 1]) - double(u3(0, jm, ks - 1))) / dy;
+        v = x(1, j, ks) + dy * b_in;
+    } else if (i == L) {
+        v = x(L - 1, j, ks);
+    } else {
+        v = x(i, j, ks);
     }
+    pres(i, j, k) = T(v);
 }
 
+// Sum of squared differences over i in 1..maxl-1, j in 1..P, k in 0..nz-1
+// (levelset.f90:218-228 and get_difference, jacobi_iteration_allflux.f90:297-308).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, FieldView<T> b, double* part)
+{
+    const int nx = g.maxl - 1;
+    const long n = (long)nx * g.P * nz;
+    double acc[1] = {0.0};
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        int i = int(q % nx) + 1;
+        long rr = q / nx;
+        int j = int(rr % g.P) + 1;
+        int k = int(rr / g.P);
+        double d = double(a(i, j, k)) - double(b(i, j, k));
+        acc[0] += d * d;
+    }
+    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_final1(const double* part, int nb, double* out)
+{
+    double tot[1];
+    finish_sums<1>(part, nb, tot);
+    if (tid256() == 0) out[0] = tot[0];
+}
+
+// ---- launchers ----
 template <typename T>
 hipError_t launch_source_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3, FieldView<T> v3,
                               FieldView<T> w3, FieldView<double> rhs_p)
@@ -389,14 +448,17 @@ hipError_t launch_uvw_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldVi
     return hipGetLastError();
 }
 
+static double n_unknowns(const Grid& g) { return double(g.maxl - 1) * g.P * (g.maxn - 1); }
+
 template <typename T>
 hipError_t launch_cg_setup_b(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3, FieldView<T> v3,
                              FieldView<T> w3, const T* u_inlet, FieldView<double> rhs_p, int use_rhs,
                              FieldView<double> b, Reduce red, CgState* st)
 {
-    CgShape sh = cg_shape(g);
-    hipLaunchKernelGGL(k_cg_setup_b<T>, sh.grid, dim3(BX, BY), 0, s, g, ph.dx, ph.dy, ph.dz, ph.dt, sh.kc, u3, v3, w3,
-                       u_inlet, rhs_p, use_rhs, b, red, st);
+    Tiles tl = cg_tiles(g);
+    hipLaunchKernelGGL(k_cg_setup_b<T>, dim3(tl.nblocks), dim3(BX, BY), 0, s, g, tl, ph.dx, ph.dy, ph.dz, ph.dt, u3, v3,
+                       w3, u_inlet, rhs_p, use_rhs, b, red.part);
+    hipLaunchKernelGGL(k_cg_setup_bf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, n_unknowns(g), st);
     return hipGetLastError();
 }
 
@@ -404,24 +466,27 @@ template <typename T>
 hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x, FieldView<double> r,
                              Reduce red, CgState* st)
 {
-    CgShape sh = cg_shape(g);
-    hipLaunchKernelGGL(k_cg_setup_r, sh.grid, dim3(BX, BY), 0, s, g, make_op(g, ph), sh.kc, x, r, red, st);
+    Tiles tl = cg_tiles(g);
+    hipLaunchKernelGGL(k_cg_setup_r, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, x, r, red.part, st);
+    hipLaunchKernelGGL(k_cg_setup_rf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, n_unknowns(g), st);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> r, FieldView<double> p_old,
                        FieldView<double> p_new, Reduce red, CgState* st)
 {
-    CgShape sh = cg_shape(g);
-    hipLaunchKernelGGL(k_cg_p, sh.grid, dim3(BX, BY), 0, s, g, make_op(g, ph), sh.kc, r, p_old, p_new, red, st);
+    Tiles tl = cg_tiles(g);
+    hipLaunchKernelGGL(k_cg_p, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, r, p_old, p_new, red.part, st);
+    hipLaunchKernelGGL(k_cg_pf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, st);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> p, FieldView<double> x,
                        FieldView<double> r, Reduce red, CgState* st)
 {
-    CgShape sh = cg_shape(g);
-    hipLaunchKernelGGL(k_cg_x, sh.grid, dim3(BX, BY), 0, s, g, make_op(g, ph), sh.kc, p, x, r, red, st);
+    Tiles tl = cg_tiles(g);
+    hipLaunchKernelGGL(k_cg_x, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, p, x, r, red.part, st);
+    hipLaunchKernelGGL(k_cg_xf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, n_unknowns(g), st);
     return hipGetLastError();
 }
 
@@ -429,8 +494,24 @@ template <typename T>
 hipError_t launch_pres_ghosts(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x, FieldView<T> u3,
                               const T* u_inlet, FieldView<T> pres)
 {
-    hipLaunchKernelGGL(k_pres_ghosts<T>, dim3((g.maxl + 1 + 63) / 64, g.P), dim3(64), 0, s, g, T(ph.dx), T(ph.dy),
-                       T(ph.dt), x, u3, u_inlet, pres);
+    dim3 grd((g.maxl + 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn + 1);
+    hipLaunchKernelGGL(k_pres_out<T>, grd, dim3(BX, BY), 0, s, g, ph.dx, ph.dy, ph.dt, x, u3, u_inlet, pres);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_levelset_err(hipStream_t s, const Grid& g, FieldView<T> a, FieldView<T> b, Reduce red)
+{
+    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, g.maxn + 1, a, b, red.part);
+    hipLaunchKernelGGL(k_final1, dim3(1), dim3(256), 0, s, red.part, REDUCE_BLOCKS, red.out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b, Reduce red)
+{
+    hipLaunchKernelGGL(k_sqdiff<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, nz, a, b, red.part);
+    hipLaunchKernelGGL(k_final1, dim3(1), dim3(256), 0, s, red.part, REDUCE_BLOCKS, red.out);
     return hipGetLastError();
 }
 
@@ -477,6 +558,8 @@ hipError_t launch_copy(hipStream_t s, T* dst, const T* src, long n)
                                              FieldView<double>, Reduce, CgState*);                                  \
     template hipError_t launch_pres_ghosts<T>(hipStream_t, const Grid&, const Phys&, FieldView<double>,             \
                                               FieldView<T>, const T*, FieldView<T>);                                \
+    template hipError_t launch_levelset_err<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, Reduce);       \
+    template hipError_t launch_difference<T>(hipStream_t, const Grid&, int, FieldView<T>, FieldView<T>, Reduce);    \
     template hipError_t launch_fill<T>(hipStream_t, T*, long, T);                                                   \
     template hipError_t launch_copy<T>(hipStream_t, T*, const T*, long);
 TPLS_INST_PR(double)

@@ -1,10 +1,11 @@
-// Deterministic block reductions for 256-thread blocks (4 wavefronts of 64).
+// Deterministic grid reductions without fences or atomics.
 //
-// Each block reduces its values in registers (wavefront shuffles, then LDS
-// across the 4 waves) and writes one partial.  The last block to arrive (an
-// agent-scope atomic counter, fenced on both sides so that partials written
-// on other XCDs are visible) sums all partials in block order.  The result
-// therefore depends only on the launch shape, never on scheduling.
+// A producing kernel reduces its values per block (wavefront shuffles, then
+// LDS across the block's 4 wavefronts) and writes one partial per block and
+// value: part[q * nblocks + block].  A one-block finishing kernel then sums
+// the partials in a fixed order.  On gfx950 a device-scope release fence per
+// block (the classic "last block finishes" pattern) writes back the L2 and
+// costs tens of nanoseconds per block, far more than one extra launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,80 +21,51 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Sum over a 256-thread block; the result is valid in thread 0.
-__device__ __forceinline__ double block_sum256(double v, double* sm)
-{
-    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    v = wave_sum(v);
-    __syncthreads();
-    if ((tid & 63) == 0) sm[tid >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (tid == 0) r = (sm[0] + sm[1]) + (sm[2] + sm[3]);
-    return r;
-}
+__device__ __forceinline__ int tid256() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
 
-// Two sums at once (halves the synchronisations of fused CG kernels).
-__device__ __forceinline__ void block_sum256x2(double& a, double& b, double* sm)
+// Sum of nv values over a 256-thread block, written as partials of block bid.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part, int nblocks, int bid)
 {
-    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    a = wave_sum(a);
-    b = wave_sum(b);
-    __syncthreads();
+    __shared__ double sm[4 * NV];
+    const int tid = tid256();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
     if ((tid & 63) == 0) {
-        sm[tid >> 6] = a;
-        sm[4 + (tid >> 6)] = b;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sm[4 * q + (tid >> 6)] = v[q];
     }
     __syncthreads();
     if (tid == 0) {
-        a = (sm[0] + sm[1]) + (sm[2] + sm[3]);
-        b = (sm[4] + sm[5]) + (sm[6] + sm[7]);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) part[(long)q * nblocks + bid] = (sm[4 * q] + sm[4 * q + 1]) + (sm[4 * q + 2] + sm[4 * q + 3]);
     }
 }
 
-// Publish the block partial(s); returns true in every thread of the last
-// block, after which red.part holds all partials.
-__device__ __forceinline__ bool publish_partials(const Reduce& red, int nval, const double* vals, int nblocks,
-                                                 int bid)
+// In a one-block (256-thread) finishing kernel: totals of nv partial rows,
+// valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void finish_sums(const double* part, int nblocks, double (&tot)[NV])
 {
-    __shared__ int last;
-    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    if (tid == 0) {
-        for (int q = 0; q < nval; ++q) red.part[q * nblocks + bid] = vals[q];
-        __threadfence();
-        unsigned prev = atomicAdd(red.counter, 1u);
-        last = (prev == unsigned(nblocks - 1));
+    __shared__ double sm[4 * NV];
+    const int tid = tid256();
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = 0.0;
+    for (int b = tid; b < nblocks; b += 256) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) t[q] += part[(long)q * nblocks + b];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = wave_sum(t[q]);
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sm[4 * q + (tid >> 6)] = t[q];
     }
     __syncthreads();
-    if (last) __threadfence();
-    return last != 0;
-}
-
-// Sum nblocks partials (in block order per thread stripe, then the fixed
-// block tree).  Valid in thread 0.  Call only from the last block.
-__device__ __forceinline__ double sum_partials(const Reduce& red, int which, int nblocks, double* sm)
-{
-    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    double t = 0.0;
-    const volatile double* p = red.part + (long)which * nblocks;
-    for (int q = tid; q < nblocks; q += 256) t += p[q];
-    return block_sum256(t, sm);
-}
-
-// Single-value reduction finishing into *red.out (used by diagnostics).
-__device__ __forceinline__ void block_reduce_finish(double v, const Reduce& red)
-{
-    __shared__ double sm[8];
-    const int nblocks = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    double s = block_sum256(v, sm);
-    if (publish_partials(red, 1, &s, nblocks, bid)) {
-        double t = sum_partials(red, 0, nblocks, sm);
-        const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-        if (tid == 0) {
-            *red.out = t;
-            *red.counter = 0u;
-        }
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] = (sm[4 * q] + sm[4 * q + 1]) + (sm[4 * q + 2] + sm[4 * q + 3]);
     }
 }
 

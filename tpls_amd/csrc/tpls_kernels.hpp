@@ -12,14 +12,12 @@ struct Phys {
     double dx, dy, dz, dt, Re, mu_plus, mu_minus, height, sw, scap;
 };
 
-// Block-partial reductions: each launch writes one partial per block into
-// `part` and the last block to finish sums them in block order into `out`
-// (deterministic for a fixed launch shape).  `counter` must be 0 on entry
-// and is reset to 0 by that last block.
+// Grid reductions: the producing kernel writes per-block partials into
+// `part`; a one-block finishing kernel sums them in a fixed order into `out`
+// (deterministic for a fixed launch shape).
 struct Reduce {
     double* part;
     double* out;
-    unsigned* counter;
 };
 
 // ---- advection (weno.f90, advect_phi.f90) ----
