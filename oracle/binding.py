@@ -118,6 +118,8 @@ def oracle_lib() -> _Lib:
         L.orc_step_timed.argtypes = [C.c_void_p, _D, _D, _I]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_pressure_solve.restype = C.c_int
+        L.orc_mg_apply.restype = C.c_int
+        L.orc_mg_apply.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int, _D, _D]
     return _oracle
 
 
@@ -150,6 +152,31 @@ def ref(name, *args):
     return outs
 
 
+def pressure_solve(pres, rhs_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt, rtol=1e-10,
+                   abstol=1e-50, dtol=1e5, maxits=10000, pc=0):
+    """orc_pressure_solve on Fortran-ordered arrays (pres updated in place);
+    returns (reason, its, rnorm)."""
+    L = oracle_lib().lib
+    its, rn = C.c_int(0), C.c_double(0.0)
+    ptr = lambda a: a.ctypes.data_as(_D)
+    reason = L.orc_pressure_solve(ptr(pres), ptr(rhs_p), ptr(u3), ptr(u_inlet), int(maxl), int(maxm),
+                                  int(maxn), C.c_double(dx), C.c_double(dy), C.c_double(dz),
+                                  C.c_double(dt), C.c_double(rtol), C.c_double(abstol),
+                                  C.c_double(dtol), int(maxits), int(pc), C.byref(its), C.byref(rn))
+    return reason, its.value, rn.value
+
+
+def mg_apply(r, nx, ny, nz, cx, cy, cz, diag, nu=1):
+    """z = M r, one symmetric V-cycle of the multigrid preconditioner on an
+    nx x ny x nz reduced operator (r, z flat, x fastest); returns (z, levels)."""
+    L = oracle_lib().lib
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    nl = L.orc_mg_apply(nx, ny, nz, cx, cy, cz, diag, int(nu), r.ctypes.data_as(_D),
+                        z.ctypes.data_as(_D))
+    return z, nl
+
+
 def set_threads(n: int) -> None:
     oracle_lib().lib.orc_set_threads(int(n))
 
@@ -163,7 +190,7 @@ class OrcParams(C.Structure):
                  "smooth_width", "scap")] + \
                [(n, C.c_int) for n in ("mom_u", "mom_v", "mom_w", "levelset_its")] + \
                [(n, C.c_double) for n in ("ksp_rtol", "ksp_abstol", "ksp_dtol")] + \
-               [("ksp_maxits", C.c_int)]
+               [("ksp_maxits", C.c_int), ("ksp_pc", C.c_int)]
 
 
 class OrcDiag(C.Structure):
@@ -207,7 +234,7 @@ class OracleState:
         self.shape = (maxl, maxm, maxn)
         p = OrcParams()
         for f in fields_of(OrcParams):
-            setattr(p, f, params[f])
+            setattr(p, f, params.get(f, 0) if f == "ksp_pc" else params[f])
         self._p = p
         self.h = self.L.orc_state_create(maxl, maxm, maxn, C.byref(p))
 

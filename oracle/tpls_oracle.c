@@ -1107,6 +1107,259 @@ static void sums2(const double *a, const double *b, long plane, int nz, double *
     free(pab);
 }
 
+/* ---- geometric multigrid preconditioner (symmetric V-cycle) ------------
+ * Levels: the reduced operator on nx x ny x nz cells, coarsened by 2 in each
+ * direction while nx and nz are even, ny is a multiple of 4 (red-black
+ * colouring across the periodic y seam needs an even ny on every level) and
+ * all three extents are >= 4; coefficients scale by 1/4 per level
+ * (rediscretisation).  Neumann in x and z (a boundary
+ * neighbour is the cell itself), periodic in y.
+ *   smoothing: red-black Gauss-Seidel, colour (i+j+k) mod 2 (red = even);
+ *              pre-smoothing red then black from a zero guess, post-smoothing
+ *              black then red (the adjoint, so the V-cycle is symmetric);
+ *   restriction: R = P^T / 8 (full weighting), the transpose of
+ *   prolongation: trilinear cell-centred interpolation, weights 3/4 and 1/4
+ *              per direction, reflected at Neumann boundaries, wrapped in y;
+ *   coarsest: mean-free right-hand side, 20 + nmax^2/2 symmetric
+ *              Gauss-Seidel sweeps (red, black, black, red) from zero, nmax
+ *              being its largest extent; mean-free result.
+ * The same definition is implemented on the GPU (tpls_amd/csrc/multigrid.hip). */
+#define MG_MAXLEV 16
+typedef struct {
+    int nx, ny, nz;
+    double cx, cy, cz, diag;
+    double *x, *b, *t;
+} MgLev;
+typedef struct {
+    int nlev, nu;
+    MgLev L[MG_MAXLEV];
+} Mg;
+
+#define MIX(o, i, j, k) ((i) - 1 + (long)(o)->nx * (((j) - 1) + (long)(o)->ny * ((k) - 1)))
+
+static double mg_deff(const MgLev *o, int i, int k)
+{
+    double d = o->diag;
+    if (k == 1) d = d - o->cz;
+    if (k == o->nz) d = d - o->cz;
+    if (i == 1) d = d - o->cx;
+    if (i == o->nx) d = d - o->cx;
+    return d;
+}
+
+/* sum of c * neighbour over the interior neighbours of (i,j,k) */
+static double mg_nsum(const MgLev *o, const double *x, int i, int j, int k)
+{
+    int jm = j == 1 ? o->ny : j - 1, jp = j == o->ny ? 1 : j + 1;
+    double zm = k > 1 ? x[MIX(o, i, j, k - 1)] : 0.0;
+    double zp = k < o->nz ? x[MIX(o, i, j, k + 1)] : 0.0;
+    double xm = i > 1 ? x[MIX(o, i - 1, j, k)] : 0.0;
+    double xp = i < o->nx ? x[MIX(o, i + 1, j, k)] : 0.0;
+    double ym = x[MIX(o, i, jm, k)], yp = x[MIX(o, i, jp, k)];
+    return o->cz * (zm + zp) + o->cx * (xm + xp) + o->cy * (ym + yp);
+}
+
+static void mg_smooth(const MgLev *o, double *x, const double *b, int colour)
+{
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= o->nz; ++k)
+        for (int j = 1; j <= o->ny; ++j)
+            for (int i = 1 + ((colour + j + k + 1) & 1); i <= o->nx; i += 2)
+                x[MIX(o, i, j, k)] = (b[MIX(o, i, j, k)] + mg_nsum(o, x, i, j, k)) / mg_deff(o, i, k);
+}
+
+static void mg_resid(const MgLev *o, const double *x, const double *b, double *t)
+{
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= o->nz; ++k)
+        for (int j = 1; j <= o->ny; ++j)
+            for (int i = 1; i <= o->nx; ++i)
+                t[MIX(o, i, j, k)] = b[MIX(o, i, j, k)] -
+                                     (mg_deff(o, i, k) * x[MIX(o, i, j, k)] - mg_nsum(o, x, i, j, k));
+}
+
+/* 1-D trilinear pieces: fine cell f (1-based) takes 3/4 of its parent
+ * (f+1)/2 and 1/4 of the neighbour toward f (parent-1 for odd f, parent+1
+ * for even f), the neighbour reflected to the parent at a Neumann end. */
+static int mg_nbr(int f, int nc, int periodic)
+{
+    int I = (f + 1) / 2;
+    int n = (f & 1) ? I - 1 : I + 1;
+    if (n < 1) n = periodic ? nc : I;
+    if (n > nc) n = periodic ? 1 : I;
+    return n;
+}
+
+/* x_f += P e_c (z first, then y, then x) */
+static void mg_prolong(const MgLev *f, const MgLev *c, double *xf, const double *ec)
+{
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= f->nz; ++k)
+        for (int j = 1; j <= f->ny; ++j)
+            for (int i = 1; i <= f->nx; ++i) {
+                int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
+                int In = mg_nbr(i, c->nx, 0), Jn = mg_nbr(j, c->ny, 1), Kn = mg_nbr(k, c->nz, 0);
+                double a00 = 0.75 * ec[MIX(c, I, J, K)] + 0.25 * ec[MIX(c, I, J, Kn)];
+                double a01 = 0.75 * ec[MIX(c, I, Jn, K)] + 0.25 * ec[MIX(c, I, Jn, Kn)];
+                double a10 = 0.75 * ec[MIX(c, In, J, K)] + 0.25 * ec[MIX(c, In, J, Kn)];
+                double a11 = 0.75 * ec[MIX(c, In, Jn, K)] + 0.25 * ec[MIX(c, In, Jn, Kn)];
+                double b0 = 0.75 * a00 + 0.25 * a01;
+                double b1 = 0.75 * a10 + 0.25 * a11;
+                xf[MIX(f, i, j, k)] += 0.75 * b0 + 0.25 * b1;
+            }
+}
+
+/* 1-D transpose weights: the fine cells feeding coarse cell I and their
+ * weights (fine 2I-2: 1/4, 2I-1: 3/4, 2I: 3/4, 2I+1: 1/4, with the
+ * reflected or wrapped end contributions added where they land). */
+static int mg_rweights(int I, int nc, int nf, int periodic, int *fi, double *w)
+{
+    int n = 0;
+    int cand[4] = {2 * I - 2, 2 * I - 1, 2 * I, 2 * I + 1};
+    for (int q = 0; q < 4; ++q) {
+        int f = cand[q];
+        if (f < 1) f = periodic ? f + nf : 0;
+        if (f > nf) f = periodic ? f - nf : 0;
+        if (f == 0) continue;
+        /* weight of coarse I in fine f's interpolation */
+        int par = (f + 1) / 2, nb = mg_nbr(f, nc, periodic);
+        double wt = (par == I ? 0.75 : 0.0) + (nb == I ? 0.25 : 0.0);
+        if (wt == 0.0) continue;
+        fi[n] = f;
+        w[n] = wt;
+        ++n;
+    }
+    return n;
+}
+
+/* b_c = P^T t_f / 8 */
+static void mg_restrict(const MgLev *f, const MgLev *c, const double *tf, double *bc)
+{
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int K = 1; K <= c->nz; ++K)
+        for (int J = 1; J <= c->ny; ++J)
+            for (int I = 1; I <= c->nx; ++I) {
+                int fi[4], fj[4], fk[4];
+                double wi[4], wj[4], wk[4];
+                int ni = mg_rweights(I, c->nx, f->nx, 0, fi, wi);
+                int nj = mg_rweights(J, c->ny, f->ny, 1, fj, wj);
+                int nk = mg_rweights(K, c->nz, f->nz, 0, fk, wk);
+                double s = 0.0;
+                for (int a = 0; a < nk; ++a) {
+                    double sj = 0.0;
+                    for (int bq = 0; bq < nj; ++bq) {
+                        double si = 0.0;
+                        for (int q = 0; q < ni; ++q) si += wi[q] * tf[MIX(f, fi[q], fj[bq], fk[a])];
+                        sj += wj[bq] * si;
+                    }
+                    s += wk[a] * sj;
+                }
+                bc[MIX(c, I, J, K)] = 0.125 * s;
+            }
+}
+
+static void mg_coarse(const MgLev *o, double *x, double *b)
+{
+    long n = (long)o->nx * o->ny * o->nz;
+    double sb = 0.0;
+    for (long q = 0; q < n; ++q) sb += b[q];
+    double mb = sb / (double)n;
+    for (long q = 0; q < n; ++q) {
+        b[q] -= mb;
+        x[q] = 0.0;
+    }
+    int nmax = o->nx > o->ny ? o->nx : o->ny;
+    if (o->nz > nmax) nmax = o->nz;
+    const int sweeps = 20 + nmax * nmax / 2;
+    for (int s = 0; s < sweeps; ++s) {
+        mg_smooth(o, x, b, 0);
+        mg_smooth(o, x, b, 1);
+        mg_smooth(o, x, b, 1);
+        mg_smooth(o, x, b, 0);
+    }
+    double sx = 0.0;
+    for (long q = 0; q < n; ++q) sx += x[q];
+    double mx = sx / (double)n;
+    for (long q = 0; q < n; ++q) x[q] -= mx;
+}
+
+static void mg_vcycle(Mg *mg, int l)
+{
+    MgLev *o = &mg->L[l];
+    if (l == mg->nlev - 1) {
+        mg_coarse(o, o->x, o->b);
+        return;
+    }
+    long n = (long)o->nx * o->ny * o->nz;
+    memset(o->x, 0, sizeof(double) * n);
+    for (int s = 0; s < mg->nu; ++s) {
+        mg_smooth(o, o->x, o->b, 0);
+        mg_smooth(o, o->x, o->b, 1);
+    }
+    mg_resid(o, o->x, o->b, o->t);
+    mg_restrict(o, &mg->L[l + 1], o->t, mg->L[l + 1].b);
+    mg_vcycle(mg, l + 1);
+    mg_prolong(o, &mg->L[l + 1], o->x, mg->L[l + 1].x);
+    for (int s = 0; s < mg->nu; ++s) {
+        mg_smooth(o, o->x, o->b, 1);
+        mg_smooth(o, o->x, o->b, 0);
+    }
+}
+
+static void mg_build(Mg *mg, int nx, int ny, int nz, double cx, double cy, double cz, double diag, int nu)
+{
+    memset(mg, 0, sizeof *mg);
+    mg->nu = nu;
+    int l = 0;
+    for (;;) {
+        MgLev *o = &mg->L[l];
+        o->nx = nx; o->ny = ny; o->nz = nz;
+        o->cx = cx; o->cy = cy; o->cz = cz; o->diag = diag;
+        long n = (long)nx * ny * nz;
+        o->x = calloc(n, sizeof(double));
+        o->b = calloc(n, sizeof(double));
+        o->t = calloc(n, sizeof(double));
+        ++l;
+        if (l == MG_MAXLEV || nx % 2 || ny % 4 || nz % 2 || nx < 4 || ny < 4 || nz < 4) break;
+        nx /= 2; ny /= 2; nz /= 2;
+        cx *= 0.25; cy *= 0.25; cz *= 0.25; diag *= 0.25;
+    }
+    mg->nlev = l;
+}
+
+static void mg_free(Mg *mg)
+{
+    for (int l = 0; l < mg->nlev; ++l) {
+        free(mg->L[l].x);
+        free(mg->L[l].b);
+        free(mg->L[l].t);
+    }
+}
+
+/* z = M r on the finest level (then the caller removes the mean) */
+static void mg_apply(Mg *mg, const double *r, double *z)
+{
+    MgLev *o = &mg->L[0];
+    long n = (long)o->nx * o->ny * o->nz;
+    memcpy(o->b, r, sizeof(double) * n);
+    mg_vcycle(mg, 0);
+    memcpy(z, o->x, sizeof(double) * n);
+}
+
+/* Test hook: z = M r for an nx x ny x nz reduced operator with the given
+ * coefficients and nu pre- and post-smoothing sweeps (no mean removal);
+ * returns the number of levels. */
+int orc_mg_apply(int nx, int ny, int nz, double cx, double cy, double cz, double diag, int nu,
+                 const double *r, double *z)
+{
+    Mg mg;
+    mg_build(&mg, nx, ny, nz, cx, cy, cz, diag, nu < 1 ? 1 : nu);
+    mg_apply(&mg, r, z);
+    int nl = mg.nlev;
+    mg_free(&mg);
+    return nl;
+}
+
 /* PETSc KSPConvergedReason codes used by pressure_solver.F90:101-142 */
 enum { R_DIV_INDEF_MAT = -10, R_DIV_NAN = -9, R_DIV_INDEF_PC = -8, R_DIV_DTOL = -4,
        R_DIV_ITS = -3, R_CONV_RTOL = 2, R_CONV_ATOL = 3 };
@@ -1114,7 +1367,7 @@ enum { R_DIV_INDEF_MAT = -10, R_DIV_NAN = -9, R_DIV_INDEF_PC = -8, R_DIV_DTOL = 
 int orc_pressure_solve(double *pres_, const double *RHS_p_, const double *u3_,
                        const double *u_inlet, int maxl, int maxm, int maxn, double dx,
                        double dy, double dz, double dt, double rtol, double abstol,
-                       double dtol, int maxits, int *its_out, double *rnorm_out)
+                       double dtol, int maxits, int pc, int *its_out, double *rnorm_out)
 {
     int sx = 1, ex = maxl - 1, sy = 1, ey = maxm - 1;
     F3 pres = f3(pres_, 0, maxl, 0, maxm, 0);
@@ -1156,28 +1409,47 @@ int orc_pressure_solve(double *pres_, const double *RHS_p_, const double *u3_,
     /* r = b - A x */
     op_apply(&o, x, w);
     for (long q = 0; q < N; ++q) r[q] = b[q] - w[q];
-    /* z = r - mean(r); beta = z.r = |z|^2 = sum r^2 - (sum r)^2 / N */
-    double rr, sr;
+    /* Stopping test on the null-space-projected residual |r - mean(r)|
+     * (KSP_NORM_UNPRECONDITIONED; with pc none it equals PETSc's default
+     * preconditioned norm, so the reference's test is unchanged).
+     * pc none: z = r - mean(r), beta = z.r = sum r^2 - (sum r)^2 / N.
+     * pc mg:   z = M r - mean(M r), beta = z.r = sum zr - mean(z) sum r. */
+    Mg mg;
+    double *zb = NULL;
+    if (pc >= 1) {
+        mg_build(&mg, o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, pc);
+        zb = malloc(sizeof(double) * N);
+    }
+    double rr, sr, szr, sz, mz = 0.0;
     sums2(r, r, plane, o.nz, &rr, &sr);
     double m = sr / (double)N;
+    double dp = sqrt(rr - sr * m > 0 ? rr - sr * m : 0);
     double beta = rr - sr * m;
-    double dp = sqrt(beta > 0 ? beta : 0);
     double rnorm0 = bnorm > 0 ? bnorm : dp;
     double ttol = rtol * rnorm0 > abstol ? rtol * rnorm0 : abstol;
     int reason = 0, its = 0;
     if (isnan(dp)) reason = R_DIV_NAN;
     else if (dp <= ttol) reason = dp < abstol ? R_CONV_ATOL : R_CONV_RTOL;
+    /* the preconditioner is applied only to a residual that failed the test */
+    if (pc >= 1 && !reason) {
+        mg_apply(&mg, r, zb);
+        sums2(zb, r, plane, o.nz, &szr, &sz);
+        mz = sz / (double)N;
+        beta = szr - mz * sr;
+    }
     double betaold = 0.0;
     int i = 0;
     while (!reason) {
         its = i + 1;
         if (beta == 0.0) { reason = R_CONV_ATOL; break; }
         if (beta < 0.0) { reason = R_DIV_INDEF_PC; break; }
+        const double *zz = pc >= 1 ? zb : r;
+        const double zm = pc >= 1 ? mz : m;
         if (i == 0) {
-            for (long q = 0; q < N; ++q) p[q] = r[q] - m;
+            for (long q = 0; q < N; ++q) p[q] = zz[q] - zm;
         } else {
             double bt = beta / betaold;
-            for (long q = 0; q < N; ++q) p[q] = (r[q] - m) + bt * p[q];
+            for (long q = 0; q < N; ++q) p[q] = (zz[q] - zm) + bt * p[q];
         }
         betaold = beta;
         op_apply(&o, p, w);
@@ -1199,6 +1471,16 @@ int orc_pressure_solve(double *pres_, const double *RHS_p_, const double *u3_,
         if (reason) break;
         ++i;
         if (i >= maxits) { reason = R_DIV_ITS; break; }
+        if (pc >= 1) {
+            mg_apply(&mg, r, zb);
+            sums2(zb, r, plane, o.nz, &szr, &sz);
+            mz = sz / (double)N;
+            beta = szr - mz * sr;
+        }
+    }
+    if (pc >= 1) {
+        mg_free(&mg);
+        free(zb);
     }
     *its_out = its;
     *rnorm_out = dp;
@@ -1514,7 +1796,8 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
     double t1 = now_s();
     dg->ksp_reason = orc_pressure_solve(f[ORC_PRES], f[ORC_RHS_P], f[ORC_U3], f[ORC_U_INLET], maxl,
                                         maxm, maxn, dx, dy, dz, dt, P->ksp_rtol, P->ksp_abstol,
-                                        P->ksp_dtol, P->ksp_maxits, &dg->ksp_its, &dg->ksp_rnorm);
+                                        P->ksp_dtol, P->ksp_maxits, P->ksp_pc, &dg->ksp_its,
+                                        &dg->ksp_rnorm);
     double t2 = now_s();
     /* main:918 */
     orc_get_uvw_pres(f[ORC_U3], f[ORC_V3], f[ORC_W3], f[ORC_PRES], ex, ey, sx, sy, maxn, ex_max,

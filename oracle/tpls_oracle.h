@@ -19,8 +19,8 @@
  * fast-math, so each expression rounds exactly as written, which is how
  * gfortran/flang evaluate the reference on x86-64 (no FMA contraction).
  *
- * Parity pinning (see tests/test_oracle_vs_ref.py and
- * tests/test_oracle_invariants.py): every routine that exists in a
+ * Parity pinning (see tests/test_oracle_vs_ref.py, tests/test_oracle_step.py
+ * and tests/test_oracle_pressure.py): every routine that exists in a
  * dependency-free reference module is checked bit-for-bit against the
  * reference Fortran compiled by oracle/Makefile into oracle/_ref/.  The step
  * driver's main-program pieces (CSF face averages, inlet/outlet viscosity,
@@ -28,8 +28,10 @@
  * by invariants and by composing the reference modules in the tests.  The
  * pressure solve restates PETSc 3.6 KSPCG (pc none, constant null space) on
  * the symmetric reduced system: PETSc is not available here, so its parity
- * with PETSc itself is UNPINNED; it is pinned against a scipy direct solve
- * of the same discrete system on small grids.
+ * with PETSc itself (iterates, iteration counts) is UNPINNED.  Its solution is
+ * pinned against a scipy direct solve of the full DMDA system that
+ * compute_matrix / compute_rhs assemble, ghost rows included, and the
+ * multigrid preconditioner by symmetry, definiteness and contraction checks.
  */
 #ifndef TPLS_ORACLE_H
 #define TPLS_ORACLE_H
@@ -120,7 +122,15 @@ void orc_get_uvw_pres(double *u3, double *v3, double *w3, const double *p, int e
 int orc_pressure_solve(double *pres, const double *RHS_p, const double *u3,
                        const double *u_inlet, int maxl, int maxm, int maxn, double dx,
                        double dy, double dz, double dt, double rtol, double abstol,
-                       double dtol, int maxits, int *its, double *rnorm);
+                       double dtol, int maxits, int pc, int *its, double *rnorm);
+/* Preconditioner of the solve: pc = 0 none (PETSc -pc_type none), pc = n >= 1
+ * the geometric multigrid V(n,n) cycle defined in tpls_oracle.c.  The stopping
+ * test is the null-space-projected residual norm |r - mean(r)| <= rtol |b| for
+ * both (PETSc's KSP_NORM_UNPRECONDITIONED; identical to its default norm when
+ * pc = 0).  orc_mg_apply: z = M r for an nx x ny x nz reduced operator, flat
+ * arrays with x fastest; returns the number of levels. */
+int orc_mg_apply(int nx, int ny, int nz, double cx, double cy, double cz, double diag, int nu,
+                 const double *r, double *z);
 
 /* ---- whole-step driver: main_ns_hybrid.F90:583-918, 2480-2509 ---- */
 typedef struct {
@@ -128,6 +138,7 @@ typedef struct {
     int mom_u, mom_v, mom_w, levelset_its;
     double ksp_rtol, ksp_abstol, ksp_dtol;
     int ksp_maxits;
+    int ksp_pc;  /* 0 none, n >= 1 multigrid V(n,n) */
 } orc_params;
 
 typedef struct {

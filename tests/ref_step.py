@@ -214,7 +214,7 @@ class RefStep:
                               self.u_inlet.ctypes.data_as(D_), L, M, N, C.c_double(dx), C.c_double(dy),
                               C.c_double(dz), C.c_double(dt), C.c_double(P["ksp_rtol"]),
                               C.c_double(P["ksp_abstol"]), C.c_double(P["ksp_dtol"]),
-                              P["ksp_maxits"], C.byref(its), C.byref(rn))
+                              P["ksp_maxits"], P.get("ksp_pc", 0), C.byref(its), C.byref(rn))
         self.ksp_its = its.value
         B.ref("get_uvw_pres", u3, v3, w3, pres, ex, ey, sx, sy, N, exm, dx, dy, dz, dt, self.u_inlet)
         # ---- rotation (main:2480-2509) ----

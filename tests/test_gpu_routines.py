@@ -198,7 +198,7 @@ def test_pressure_solve(L):
     reason_o = B.oracle_lib().lib.orc_pressure_solve(
         po.ctypes.data_as(D_), rhs.ctypes.data_as(D_), u3.ctypes.data_as(D_), inl.ctypes.data_as(D_),
         MAXL, MAXM, MAXN, C.c_double(DX), C.c_double(DX), C.c_double(DX), C.c_double(DT),
-        C.c_double(1e-10), C.c_double(1e-50), C.c_double(1e5), 10000, C.byref(its_o), C.byref(rn_o))
+        C.c_double(1e-10), C.c_double(1e-50), C.c_double(1e5), 10000, 0, C.byref(its_o), C.byref(rn_o))
     its_g, rn_g, reason_g = L.pressure_solve(pg, rhs, u3, inl, MAXL, MAXM, MAXN, DX, DX, DX, DT,
                                              1e-10, 1e-50, 1e5, 10000)
     e = relerr(pg[:, 1:MAXM], po[:, 1:MAXM])

@@ -33,6 +33,8 @@ class CaseParams:
     ksp_abstol: float = 1e-50
     ksp_dtol: float = 1e5
     ksp_maxits: int = 10000
+    ksp_pc: str = "none"           # "none" (plain CG) or "mg" (multigrid-preconditioned CG)
+    ksp_mg_nu: int = 2             # pre- and post-smoothing sweeps of the V-cycle
     noise: float = 1e-3            # amplitude of the seeded uniform perturbation
     seed: int = 0
 
@@ -79,7 +81,12 @@ class CaseParams:
                     mom_u=self.mom_u, mom_v=self.mom_v, mom_w=self.mom_w,
                     levelset_its=self.levelset_its, ksp_rtol=self.ksp_rtol,
                     ksp_abstol=self.ksp_abstol, ksp_dtol=self.ksp_dtol,
-                    ksp_maxits=self.ksp_maxits)
+                    ksp_maxits=self.ksp_maxits, ksp_pc=self.pc_code())
+
+    def pc_code(self) -> int:
+        """Preconditioner code of the solver interface: 0 none, n >= 1 the
+        multigrid V(n,n) cycle."""
+        return {"none": 0, "mg": self.ksp_mg_nu}[self.ksp_pc]
 
 
 # Plain method, FP64, single case, CPU-runnable: 128x32x64, Re 100, viscosity
