@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", default="512x128x256")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--pc", default="none", choices=["none", "mg"],
+                    help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
+    ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -77,7 +80,7 @@ def main():
     from tpls_amd.initial import make_initial
 
     nx, ny, nz = (int(v) for v in a.grid.split("x"))
-    p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz)
+    p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
     init = make_initial(p)
     case = Case(p, precision=a.precision)
     case.load(*init)
@@ -128,6 +131,7 @@ def main():
         "config": {"workload": f"plain {a.precision} single channel case {nx}x{ny}x{nz} cells, "
                                f"Re {p.Re:g}, viscosity ratio {p.mu_plus / p.mu_minus:g}, dt {p.dt:g}, "
                                f"CG rtol {p.ksp_rtol:g}",
+                   "solver": "cg" if a.pc == "none" else f"cg + multigrid V({a.mg_nu},{a.mg_nu})",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
         "roofline": {"kernel": "cg_x", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

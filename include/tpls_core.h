@@ -56,6 +56,13 @@ typedef struct {
     int precision;    /* TPLS_FP64 or TPLS_FP32 */
     int diagnostics;  /* 1: compute the per-step residual prints of
                          main_ns_hybrid.F90:603-605, 806-808, 843-845, 885-887 */
+    int ksp_pc;       /* preconditioner of the pressure CG, the choice the
+                         reference leaves to PETSc's options database
+                         (KSPSetFromOptions, configure_mpi_petsc.F90:158):
+                         0 none (-pc_type none); n >= 1 geometric multigrid,
+                         one symmetric V(n,n) cycle per iteration (the class
+                         of -pc_type mg).  The stopping test is the
+                         unpreconditioned residual norm in both cases. */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */

@@ -26,7 +26,8 @@ class Params(C.Structure):
                  "smooth_width", "scap")] + \
                [(n, C.c_int) for n in ("mom_u", "mom_v", "mom_w", "levelset_its")] + \
                [(n, C.c_double) for n in ("ksp_rtol", "ksp_abstol", "ksp_dtol")] + \
-               [("ksp_maxits", C.c_int), ("precision", C.c_int), ("diagnostics", C.c_int)]
+               [("ksp_maxits", C.c_int), ("precision", C.c_int), ("diagnostics", C.c_int),
+                ("ksp_pc", C.c_int)]
 
 
 class Diag(C.Structure):
@@ -152,3 +153,16 @@ def pressure_solve(pres, rhs_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt,
                                     rtol, abstol, dtol, maxits, C.byref(its), C.byref(rnorm),
                                     C.byref(reason)), "tpls_pressure_solve")
     return its.value, rnorm.value, reason.value
+
+
+def test_mg_apply(r, nx, ny, nz, cx, cy, cz, diag, nu):
+    """Test hook: z = M r, one V-cycle of the pressure solve's multigrid
+    preconditioner on the GPU (r flat, x fastest); returns (z, levels)."""
+    f = lib().tpls_test_mg_apply
+    f.restype = C.c_int
+    f.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int, C.c_void_p, C.c_void_p]
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    nl = f(nx, ny, nz, cx, cy, cz, diag, nu, r.ctypes.data, z.ctypes.data)
+    check(min(nl, 0), "tpls_test_mg_apply")
+    return z, nl

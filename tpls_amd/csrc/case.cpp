@@ -119,6 +119,12 @@ struct tpls_case {
     DevArray* pres;   // storage precision: outlet BC and projection
     DevArray* pold;   // double: CG unknown, pres_old
     DevArray *cg_r, *cg_p0, *cg_p1;
+    // multigrid preconditioner (ksp_pc >= 1): z and t on the pressure grid,
+    // levels 1.. in their own arrays
+    DevArray *cg_z = nullptr, *cg_t = nullptr;
+    MgHierarchy mg{};
+    MgLevel mg_coef[MG_MAXLEV];
+    bool use_mg = false;
     DevArray* inlet;
     double* red_part = nullptr;
     double* red_out = nullptr;
@@ -270,6 +276,7 @@ size_t tpls_case_bytes(int maxl, int maxm, int maxn, int precision)
     n += p * esz * (3 + 4 + 3 + 4 + 1);      // cphi, phi2/3/ra/rb, D/sg/ist, fx/fy/fz, mu
     n += p * esz;                            // pres
     n += pd * rows * (maxn + 1) * 8 * 4;     // pold + CG vectors
+    n += pd * rows * (maxn + 1) * 8 * 3;     // multigrid z, t and coarse levels (< 1/7 x 3), if enabled
     return n + (1 << 20);
 }
 
@@ -280,6 +287,16 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if ((maxm - 1) % 2) return fail(TPLS_ERR_ARG, "maxm-1 must be even (red-black colouring across the periodic seam)");
     if (prm->mom_u < 2 || prm->mom_v < 2 || prm->mom_w < 2) return fail(TPLS_ERR_ARG, "mom_* must be >= 2");
     if (prm->precision != TPLS_FP64 && prm->precision != TPLS_FP32) return fail(TPLS_ERR_ARG, "bad precision");
+    if (prm->ksp_pc < 0 || prm->ksp_pc > 8) return fail(TPLS_ERR_ARG, "ksp_pc must be 0 (none) or 1..8 (multigrid sweeps)");
+    if (prm->ksp_pc > 0) {
+        MgLevel L[MG_MAXLEV];
+        const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
+        const MgLevel& t = L[nl - 1];
+        if (long(t.nx) * t.ny * t.nz > MG_COARSE_MAX)
+            return fail(TPLS_ERR_ARG, "multigrid: the coarsest level has " + std::to_string(long(t.nx) * t.ny * t.nz) +
+                                          " cells (at most " + std::to_string(MG_COARSE_MAX) +
+                                          "); halving stops when nx or nz is odd or ny is not a multiple of 4");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TPLS_ERR_NOGPU, "no HIP device");
     tpls_case* c = new tpls_case;
@@ -321,6 +338,27 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     for (DevArray** a : pd)
         if ((rc = alloc_field(c, a, sp, 8))) return bail(rc);
     if ((rc = alloc_field(c, &c->inlet, si, e))) return bail(rc);
+    if (prm->ksp_pc > 0) {
+        c->use_mg = true;
+        c->mg.nu = prm->ksp_pc;
+        c->mg.nlev = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, c->mg.L);
+        if ((rc = alloc_field(c, &c->cg_z, sp, 8))) return bail(rc);
+        if ((rc = alloc_field(c, &c->cg_t, sp, 8))) return bail(rc);
+        c->mg.L[0].x = c->cg_z->view<double>();
+        c->mg.L[0].b = c->cg_r->view<double>();
+        c->mg.L[0].t = c->cg_t->view<double>();
+        for (int l = 1; l < c->mg.nlev; ++l) {
+            MgLevel& o = c->mg.L[l];
+            const Shape sl{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
+            DevArray *ax, *ab, *at;
+            if ((rc = alloc_field(c, &ax, sl, 8)) || (rc = alloc_field(c, &ab, sl, 8)) ||
+                (rc = alloc_field(c, &at, sl, 8)))
+                return bail(rc);
+            o.x = ax->view<double>();
+            o.b = ab->view<double>();
+            o.t = at->view<double>();
+        }
+    }
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);
     if (hipMalloc(&c->red_part, sizeof(double) * 3 * nb) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "partials"));
     if (hipMalloc(&c->red_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "outs"));
@@ -500,6 +538,9 @@ int harvest_events(tpls_case* c)
 }
 
 // The pressure solve: KSPSolve replacement (main_ns_hybrid.F90:896-913).
+// Iterations are enqueued in chunks ahead of the convergence test (every
+// kernel of a converged solve returns at once); the host reads the device
+// state back after each chunk.
 template <typename T>
 int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p,
                    int use_rhs, const T* inlet, FieldView<double> x)
@@ -513,17 +554,35 @@ int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> 
     *c->h_cg = init;
     LCHK(hipMemcpyAsync(c->cg, c->h_cg, sizeof(CgState), hipMemcpyHostToDevice, c->s));
     FieldView<double> r = c->cg_r->view<double>();
+    const bool mg = c->use_mg;
+    if (mg) {
+        // coefficients of the reduced operator (pressure_solver.F90:394-421),
+        // a quarter per coarser level
+        const Phys& ph = c->ph;
+        mg_plan(g.maxl - 1, g.P, g.maxn - 1, 1.0 / (ph.dy * ph.dy), 1.0 / (ph.dx * ph.dx), 1.0 / (ph.dz * ph.dz),
+                2.0 * (1.0 / (ph.dx * ph.dx) + 1.0 / (ph.dy * ph.dy) + 1.0 / (ph.dz * ph.dz)), c->mg_coef);
+        for (int l = 0; l < c->mg.nlev; ++l) {
+            c->mg.L[l].cx = c->mg_coef[l].cx;
+            c->mg.L[l].cy = c->mg_coef[l].cy;
+            c->mg.L[l].cz = c->mg_coef[l].cz;
+            c->mg.L[l].diag = c->mg_coef[l].diag;
+        }
+    }
     LCHK(launch_cg_setup_b<T>(c->s, g, c->ph, u3, v3, w3, inlet, rhs_p, use_rhs, r, red_of(c, 7), c->cg));
     LCHK(launch_cg_setup_r<T>(c->s, g, c->ph, x, r, red_of(c, 7), c->cg));
+    if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
+    FieldView<double> z = mg ? c->cg_z->view<double>() : r;
     DevArray* pb[2] = {c->cg_p0, c->cg_p1};
     int cur = 0;
     long launched = 0;
-    long chunk = std::max(8, c->last_its - 2);
+    // a multigrid iteration costs tens of launches: enqueue close to the last
+    // solve's count
+    long chunk = mg ? std::max(4, c->last_its) : std::max(8, c->last_its - 2);
     for (;;) {
         chunk = std::min<long>(chunk, std::max<long>(1, c->prm.ksp_maxits - launched));
         for (long q = 0; q < chunk; ++q) {
             FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
-            LCHK(launch_cg_p(c->s, g, c->ph, r, po, pn, red_of(c, 7), c->cg));
+            LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (c->timing) {
                 e0 = next_event(c);
@@ -534,6 +593,7 @@ int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> 
                 e1 = next_event(c);
                 LCHK(hipEventRecord(e1, c->s));
             }
+            if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
             cur ^= 1;
         }
         launched += chunk;
@@ -547,7 +607,7 @@ int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> 
             if (rc) return rc;
         }
         if (c->h_cg->done || launched >= c->prm.ksp_maxits) break;
-        chunk = 16;
+        chunk = mg ? 2 : 16;
     }
     c->diag.ksp_its = c->h_cg->its;
     c->diag.ksp_reason = c->h_cg->reason;
@@ -1207,6 +1267,80 @@ int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, con
     *rnorm = c->diag.ksp_rnorm;
     *reason = c->diag.ksp_reason;
     return TPLS_OK;
+}
+
+// Test hook, not part of the public interface: z = M r, one V-cycle of the
+// pressure solve's multigrid preconditioner on an nx x ny x nz reduced
+// operator (r, z flat with x fastest, no mean removal), for comparison with
+// the oracle's orc_mg_apply.  Returns the number of levels or a status < 0.
+int tpls_test_mg_apply(int nx, int ny, int nz, double cx, double cy, double cz, double diag, int nu, const double* r,
+                       double* z)
+{
+    RC(drop_init());
+    if (!r || !z || nx < 1 || ny < 2 || nz < 1 || (ny % 2) || nu < 1) return fail(TPLS_ERR_ARG, "bad argument");
+    MgHierarchy h{};
+    h.nu = nu;
+    h.nlev = mg_plan(nx, ny, nz, cx, cy, cz, diag, h.L);
+    const MgLevel& top = h.L[h.nlev - 1];
+    if (long(top.nx) * top.ny * top.nz > MG_COARSE_MAX) return fail(TPLS_ERR_ARG, "coarsest level too large");
+    hipStream_t s = g_drop.s;
+    std::vector<void*> mem;
+    auto release = [&]() {
+        hipStreamSynchronize(s);
+        for (void* p : mem) hipFree(p);
+    };
+    DevArray lev[MG_MAXLEV][3];
+    for (int l = 0; l < h.nlev; ++l) {
+        MgLevel& o = h.L[l];
+        FieldView<double>* v[3] = {&o.x, &o.b, &o.t};
+        for (int q = 0; q < 3; ++q) {
+            DevArray& a = lev[l][q];
+            a.sh = Shape{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
+            a.esz = 8;
+            a.pitch = pitch_for(a.sh.nx(), 8);
+            if (hipMalloc(&a.ptr, a.bytes()) != hipSuccess) {
+                release();
+                return fail(TPLS_ERR_NOMEM, "hipMalloc");
+            }
+            mem.push_back(a.ptr);
+            if (hipMemsetAsync(a.ptr, 0, a.bytes(), s) != hipSuccess) {
+                release();
+                return fail(TPLS_ERR_HIP, "memset");
+            }
+            *v[q] = a.view<double>();
+        }
+    }
+    const Grid g{nx + 1, ny + 1, nz + 1, ny};
+    const long nb = std::max<long>(cg_num_blocks(g), (long)REDUCE_BLOCKS);
+    double* part = nullptr;
+    CgState* st = nullptr;
+    if (hipMalloc(&part, sizeof(double) * 3 * nb) == hipSuccess) mem.push_back(part);
+    if (hipMalloc(&st, sizeof(CgState)) == hipSuccess) mem.push_back(st);
+    if (!part || !st) {
+        release();
+        return fail(TPLS_ERR_NOMEM, "hipMalloc");
+    }
+    CgState init{};
+    const DevArray& b0 = lev[0][1];
+    const DevArray& x0 = lev[0][0];
+    const size_t row = size_t(nx) * 8;
+    int rc = TPLS_OK;
+    auto plane = [&](const DevArray& a, int k) {
+        return static_cast<char*>(a.ptr) + (size_t(1) + size_t(a.pitch) + size_t(k) * a.pitch * a.sh.ny()) * 8;
+    };
+    if (hipMemcpyAsync(st, &init, sizeof init, hipMemcpyHostToDevice, s) != hipSuccess) rc = fail(TPLS_ERR_HIP, "copy");
+    for (int k = 1; k <= nz && !rc; ++k)
+        if (hipMemcpy2DAsync(plane(b0, k), b0.pitch * 8, r + size_t(k - 1) * nx * ny, row, row, ny,
+                             hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = fail(TPLS_ERR_HIP, "copy in");
+    if (!rc && launch_mg_vcycle(s, g, h, Reduce{part, nullptr}, st) != hipSuccess) rc = fail(TPLS_ERR_HIP, "launch");
+    for (int k = 1; k <= nz && !rc; ++k)
+        if (hipMemcpy2DAsync(z + size_t(k - 1) * nx * ny, row, plane(x0, k), x0.pitch * 8, row, ny,
+                             hipMemcpyDeviceToHost, s) != hipSuccess)
+            rc = fail(TPLS_ERR_HIP, "copy out");
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TPLS_ERR_HIP, "sync");
+    release();
+    return rc ? rc : h.nlev;
 }
 
 }  // extern "C"

@@ -1,0 +1,355 @@
+// Geometric multigrid preconditioner of the pressure solve: one symmetric
+// V-cycle z = M r per conjugate-gradient iteration (the class of
+// configuration PETSc selects with -ksp_type cg -pc_type mg; the reference
+// takes its KSP and PC from the options database, configure_mpi_petsc.F90:158).
+//
+// Definition (identical to the oracle's, operation for operation):
+//   levels      the reduced operator on nx x ny x nz cells, halved in every
+//               direction while nx and nz are even, ny is a multiple of 4
+//               (the red-black colouring across the periodic y seam needs an
+//               even ny on every level) and all extents are >= 4; the
+//               coefficients scale by 1/4 per level;
+//   smoothing   red-black Gauss-Seidel, red = (i+j+k) even; nu sweeps red
+//               then black from a zero guess before the coarse correction,
+//               nu sweeps black then red after it (the adjoint order, so M is
+//               symmetric);
+//   transfer    trilinear cell-centred prolongation (3/4, 1/4 per direction,
+//               reflected at the Neumann ends, wrapped in y) and its
+//               transpose / 8 as restriction;
+//   coarsest    mean-free right-hand side, 20 + nmax^2/2 symmetric
+//               Gauss-Seidel sweeps (red, black, black, red) from zero in
+//               one block's LDS, mean-free result.
+// Within a colour every update reads only the other colour, so the sweeps are
+// order-independent and the kernels reproduce the oracle's values; FMA
+// contraction is off in this file for the same reason.
+//
+// Kernels (one thread per cell unless noted):
+//   k_mg_pre     the first pre-smoothing sweep pair from x = 0, fused: red
+//                cells take b/d, black cells form their red neighbours'
+//                values b_n/d_n on the fly;
+//   k_mg_smooth  one colour sweep;
+//   k_mg_resid   t = b - A x;
+//   k_mg_restrict  one thread per coarse cell, up to 4 x 4 x 4 fine reads;
+//   k_mg_prolong x += P e;
+//   k_mg_coarse  the coarsest solve, one block;
+//   k_mg_last    the final red sweep on the finest level, fused with the
+//                partials of z.r and sum z (grid-stride tiles, at most
+//                CG_BLOCKS partials);
+//   k_mg_zf      one block: mz = mean z, beta = z.r - mz sum r and the
+//                PETSc breakdown checks of a preconditioned CG step.
+#include "reduce.hpp"
+#include "tiles.hpp"
+#include "tpls_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace tpls {
+
+__host__ __device__ __forceinline__ double mg_deff(const MgLevel& o, int i, int k)
+{
+    double d = o.diag;
+    if (k == 1) d = d - o.cz;
+    if (k == o.nz) d = d - o.cz;
+    if (i == 1) d = d - o.cx;
+    if (i == o.nx) d = d - o.cx;
+    return d;
+}
+
+// sum of c * neighbour over the interior neighbours of (i,j,k); v(i,j,k)
+// gives the neighbour values
+template <typename V>
+__device__ __forceinline__ double mg_nsum(const MgLevel& o, const V& v, int i, int j, int k)
+{
+    const int jm = j == 1 ? o.ny : j - 1, jp = j == o.ny ? 1 : j + 1;
+    const double zm = k > 1 ? v(i, j, k - 1) : 0.0;
+    const double zp = k < o.nz ? v(i, j, k + 1) : 0.0;
+    const double xm = i > 1 ? v(i - 1, j, k) : 0.0;
+    const double xp = i < o.nx ? v(i + 1, j, k) : 0.0;
+    const double ym = v(i, jm, k), yp = v(i, jp, k);
+    return o.cz * (zm + zp) + o.cx * (xm + xp) + o.cy * (ym + yp);
+}
+
+// 1-D trilinear pieces: fine cell f (1-based) takes 3/4 of its parent
+// (f+1)/2 and 1/4 of the neighbour toward f, reflected to the parent at a
+// Neumann end or wrapped when periodic.
+__device__ __forceinline__ int mg_nbr(int f, int nc, bool periodic)
+{
+    const int I = (f + 1) / 2;
+    int n = (f & 1) ? I - 1 : I + 1;
+    if (n < 1) n = periodic ? nc : I;
+    if (n > nc) n = periodic ? 1 : I;
+    return n;
+}
+
+// Fine cells feeding coarse cell I under the transpose of the interpolation
+// and their weights (candidates 2I-2 .. 2I+1 in order).
+__device__ __forceinline__ int mg_rweights(int I, int nc, int nf, bool periodic, int* fi, double* w)
+{
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int f = 2 * I - 2 + q;
+        if (f < 1) f = periodic ? f + nf : 0;
+        if (f > nf) f = periodic ? f - nf : 0;
+        if (f == 0) continue;
+        const int par = (f + 1) / 2, nb = mg_nbr(f, nc, periodic);
+        const double wt = (par == I ? 0.75 : 0.0) + (nb == I ? 0.25 : 0.0);
+        if (wt == 0.0) continue;
+        fi[n] = f;
+        w[n] = wt;
+        ++n;
+    }
+    return n;
+}
+
+__device__ __forceinline__ bool mg_cell(const MgLevel& o, int& i, int& j, int& k)
+{
+    i = blockIdx.x * BX + threadIdx.x + 1;
+    j = blockIdx.y * BY + threadIdx.y + 1;
+    k = blockIdx.z + 1;
+    return i <= o.nx && j <= o.ny;
+}
+
+// first pre-smoothing pair (red then black) from x = 0
+__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevel o, const CgState* st)
+{
+    if (st->done) return;
+    int i, j, k;
+    if (!mg_cell(o, i, j, k)) return;
+    const double d = mg_deff(o, i, k);
+    const double bv = o.b(i, j, k);
+    if (((i + j + k) & 1) == 0) {
+        o.x(i, j, k) = (bv + 0.0) / d;
+    } else {
+        auto red = [&](int ii, int jj, int kk) { return (o.b(ii, jj, kk) + 0.0) / mg_deff(o, ii, kk); };
+        o.x(i, j, k) = (bv + mg_nsum(o, red, i, j, k)) / d;
+    }
+}
+
+__global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevel o, int colour, const CgState* st)
+{
+    if (st->done) return;
+    int i, j, k;
+    if (!mg_cell(o, i, j, k)) return;
+    if (((i + j + k) & 1) != colour) return;
+    o.x(i, j, k) = (o.b(i, j, k) + mg_nsum(o, o.x, i, j, k)) / mg_deff(o, i, k);
+}
+
+__global__ void __launch_bounds__(BX* BY) k_mg_resid(MgLevel o, const CgState* st)
+{
+    if (st->done) return;
+    int i, j, k;
+    if (!mg_cell(o, i, j, k)) return;
+    o.t(i, j, k) = o.b(i, j, k) - (mg_deff(o, i, k) * o.x(i, j, k) - mg_nsum(o, o.x, i, j, k));
+}
+
+// b_c = P^T t_f / 8
+__global__ void __launch_bounds__(BX* BY) k_mg_restrict(MgLevel f, MgLevel c, const CgState* st)
+{
+    if (st->done) return;
+    int I, J, K;
+    if (!mg_cell(c, I, J, K)) return;
+    int fi[4], fj[4], fk[4];
+    double wi[4], wj[4], wk[4];
+    const int ni = mg_rweights(I, c.nx, f.nx, false, fi, wi);
+    const int nj = mg_rweights(J, c.ny, f.ny, true, fj, wj);
+    const int nk = mg_rweights(K, c.nz, f.nz, false, fk, wk);
+    double s = 0.0;
+    for (int a = 0; a < nk; ++a) {
+        double sj = 0.0;
+        for (int bq = 0; bq < nj; ++bq) {
+            double si = 0.0;
+            for (int q = 0; q < ni; ++q) si += wi[q] * f.t(fi[q], fj[bq], fk[a]);
+            sj += wj[bq] * si;
+        }
+        s += wk[a] * sj;
+    }
+    c.b(I, J, K) = 0.125 * s;
+}
+
+// x_f += P e_c (z first, then y, then x)
+__global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevel f, MgLevel c, const CgState* st)
+{
+    if (st->done) return;
+    int i, j, k;
+    if (!mg_cell(f, i, j, k)) return;
+    const int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
+    const int In = mg_nbr(i, c.nx, false), Jn = mg_nbr(j, c.ny, true), Kn = mg_nbr(k, c.nz, false);
+    const double a00 = 0.75 * c.x(I, J, K) + 0.25 * c.x(I, J, Kn);
+    const double a01 = 0.75 * c.x(I, Jn, K) + 0.25 * c.x(I, Jn, Kn);
+    const double a10 = 0.75 * c.x(In, J, K) + 0.25 * c.x(In, J, Kn);
+    const double a11 = 0.75 * c.x(In, Jn, K) + 0.25 * c.x(In, Jn, Kn);
+    const double b0 = 0.75 * a00 + 0.25 * a01;
+    const double b1 = 0.75 * a10 + 0.25 * a11;
+    f.x(i, j, k) = f.x(i, j, k) + (0.75 * b0 + 0.25 * b1);
+}
+
+// Coarsest level: one 256-thread block; x and b live in LDS (n <= MG_COARSE_MAX).
+// The two means are summed by one thread in index order.
+__global__ void __launch_bounds__(256) k_mg_coarse(MgLevel o, const CgState* st)
+{
+    if (st->done) return;
+    __shared__ double sx[MG_COARSE_MAX], sb[MG_COARSE_MAX];
+    __shared__ double mean;
+    const int tid = threadIdx.x;
+    const int n = o.nx * o.ny * o.nz;
+    auto ijk = [&](int q, int& i, int& j, int& k) {
+        i = q % o.nx + 1;
+        const int r = q / o.nx;
+        j = r % o.ny + 1;
+        k = r / o.ny + 1;
+    };
+    for (int q = tid; q < n; q += 256) {
+        int i, j, k;
+        ijk(q, i, j, k);
+        sb[q] = o.b(i, j, k);
+        sx[q] = 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int q = 0; q < n; ++q) s += sb[q];
+        mean = s / (double)n;
+    }
+    __syncthreads();
+    const double mb = mean;
+    for (int q = tid; q < n; q += 256) sb[q] = sb[q] - mb;
+    __syncthreads();
+    const int nmax = max(o.nx, max(o.ny, o.nz));
+    const int sweeps = 20 + nmax * nmax / 2;
+    auto lds = [&](int i, int j, int k) { return sx[(i - 1) + o.nx * ((j - 1) + o.ny * (k - 1))]; };
+    for (int s = 0; s < sweeps; ++s) {
+        for (int pass = 0; pass < 4; ++pass) {
+            const int colour = (pass == 1 || pass == 2) ? 1 : 0;
+            for (int q = tid; q < n; q += 256) {
+                int i, j, k;
+                ijk(q, i, j, k);
+                if (((i + j + k) & 1) != colour) continue;
+                sx[q] = (sb[q] + mg_nsum(o, lds, i, j, k)) / mg_deff(o, i, k);
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        double s = 0.0;
+        for (int q = 0; q < n; ++q) s += sx[q];
+        mean = s / (double)n;
+    }
+    __syncthreads();
+    const double mx = mean;
+    for (int q = tid; q < n; q += 256) {
+        int i, j, k;
+        ijk(q, i, j, k);
+        o.x(i, j, k) = sx[q] - mx;
+    }
+}
+
+// Final red sweep on the finest level with the partials of z.r (r = b) and
+// sum z over all cells.  With update = 0 the sweep is skipped (a one-level
+// hierarchy, where the coarsest solve already produced z).
+__global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevel o, Tiles tl, int update, double* part, const CgState* st)
+{
+    if (st->done) return;
+    double acc[2] = {0.0, 0.0};
+    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+        int i, j, k0;
+        tile_origin(tl, t, i, j, k0);
+        if (i > o.nx || j > o.ny) continue;
+        for (int k = k0; k < k0 + CG_KC && k <= o.nz; ++k) {
+            const double bv = o.b(i, j, k);
+            double z;
+            if (update && ((i + j + k) & 1) == 0) {
+                z = (bv + mg_nsum(o, o.x, i, j, k)) / mg_deff(o, i, k);
+                o.x(i, j, k) = z;
+            } else {
+                z = o.x(i, j, k);
+            }
+            acc[0] += z * bv;
+            acc[1] += z;
+        }
+    }
+    block_partials<2>(acc, part, gridDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_mg_zf(const double* part, int nb, double N, CgState* st)
+{
+    if (st->done) return;
+    double tot[2];
+    finish_sums<2>(part, nb, tot);
+    if (tid256() != 0) return;
+    const double mz = tot[1] / N;
+    const double beta = tot[0] - mz * st->sr;
+    st->mz = mz;
+    st->beta = beta;
+    // breakdown checks at the top of the next iteration
+    if (beta == 0.0) {
+        st->reason = 3;
+        st->its = st->i + 1;
+        st->done = 1;
+    } else if (beta < 0.0) {
+        st->reason = -8;
+        st->its = st->i + 1;
+        st->done = 1;
+    }
+}
+
+// ---- host side ----
+int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag, MgLevel* L)
+{
+    int l = 0;
+    for (;;) {
+        MgLevel& o = L[l];
+        o = MgLevel{};
+        o.nx = nx;
+        o.ny = ny;
+        o.nz = nz;
+        o.cx = cx;
+        o.cy = cy;
+        o.cz = cz;
+        o.diag = diag;
+        ++l;
+        if (l == MG_MAXLEV || nx % 2 || ny % 4 || nz % 2 || nx < 4 || ny < 4 || nz < 4) break;
+        nx /= 2;
+        ny /= 2;
+        nz /= 2;
+        cx *= 0.25;
+        cy *= 0.25;
+        cz *= 0.25;
+        diag *= 0.25;
+    }
+    return l;
+}
+
+static dim3 level_grid(const MgLevel& o) { return dim3((o.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
+
+hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
+{
+    const dim3 blk(BX, BY);
+    const int top = h.nlev - 1;
+    for (int l = 0; l < top; ++l) {
+        const MgLevel& o = h.L[l];
+        hipLaunchKernelGGL(k_mg_pre, level_grid(o), blk, 0, s, o, st);
+        for (int q = 1; q < h.nu; ++q) {
+            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 1, st);
+        }
+        hipLaunchKernelGGL(k_mg_resid, level_grid(o), blk, 0, s, o, st);
+        hipLaunchKernelGGL(k_mg_restrict, level_grid(h.L[l + 1]), blk, 0, s, o, h.L[l + 1], st);
+    }
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(256), 0, s, h.L[top], st);
+    for (int l = top - 1; l >= 0; --l) {
+        const MgLevel& o = h.L[l];
+        hipLaunchKernelGGL(k_mg_prolong, level_grid(o), blk, 0, s, o, h.L[l + 1], st);
+        for (int q = 0; q < h.nu; ++q) {
+            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 1, st);
+            if (l > 0 || q < h.nu - 1) hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 0, st);
+        }
+    }
+    const Tiles tl = cg_tiles(g);
+    hipLaunchKernelGGL(k_mg_last, dim3(tl.nblocks), blk, 0, s, h.L[0], tl, top > 0 ? 1 : 0, red.part, st);
+    const double N = double(h.L[0].nx) * h.L[0].ny * h.L[0].nz;
+    hipLaunchKernelGGL(k_mg_zf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, N, st);
+    return hipGetLastError();
+}
+
+}  // namespace tpls

@@ -32,12 +32,10 @@
 // wavefront spans a 512-byte row segment; the 4-level march reuses the z
 // neighbours in registers), grid-stride over at most CG_BLOCKS blocks.
 #include "reduce.hpp"
+#include "tiles.hpp"
 #include "tpls_kernels.hpp"
 
 namespace tpls {
-
-constexpr int CG_KC = 4;
-constexpr int CG_BLOCKS = 4096;
 
 // ---- get_source_pres (pressure.f90:38-50): rhs_p = div(u3)/dt ----
 // Divergence of the provisional velocity (pressure.f90:42-43), evaluated in
@@ -111,34 +109,7 @@ __device__ __forceinline__ double apply_op(const Op& o, double c, double xm, dou
     return o.diag * c - o.cz * zm - o.cx * xm - o.cy * ym - o.cy * yp - o.cx * xp - o.cz * zp;
 }
 
-struct Tiles {
-    int nbx, nby, nbz, ntiles, nblocks;
-};
-
-static Tiles cg_tiles(const Grid& g)
-{
-    Tiles t;
-    t.nbx = (g.maxl - 1 + BX - 1) / BX;
-    t.nby = (g.P + BY - 1) / BY;
-    t.nbz = (g.maxn - 1 + CG_KC - 1) / CG_KC;
-    t.ntiles = t.nbx * t.nby * t.nbz;
-    t.nblocks = t.ntiles < CG_BLOCKS ? t.ntiles : CG_BLOCKS;
-    return t;
-}
-
 long cg_num_blocks(const Grid& g) { return cg_tiles(g).nblocks; }
-
-// Decode tile t into the first interior cell (i, j, k0) of this thread.
-__device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int& j, int& k0)
-{
-    const int bx = t % tl.nbx;
-    const int r = t / tl.nbx;
-    const int by = r % tl.nby;
-    const int bz = r / tl.nby;
-    i = bx * BX + threadIdx.x + 1;
-    j = by * BY + threadIdx.y + 1;
-    k0 = bz * CG_KC + 1;
-}
 
 // b = -RHS_p with the inlet Neumann flux folded in (pressure_solver.F90:272-286),
 // written into the r buffer; partials of sum b.  RHS_p is either read
@@ -215,6 +186,7 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
     double ttol = st->rtol * rnorm0 > st->abstol ? st->rtol * rnorm0 : st->abstol;
     st->bnorm = bnorm;
     st->m = m;
+    st->sr = tot[1];
     st->beta = beta;
     st->betaold = 0.0;
     st->dp = dp;
@@ -282,10 +254,10 @@ __device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const F
 }
 
 __global__ void __launch_bounds__(BX* BY) k_cg_p(Op o, Tiles tl, FieldView<double> r, FieldView<double> po,
-                                                 FieldView<double> pn, double* part, const CgState* st)
+                                                 FieldView<double> pn, int use_mz, double* part, const CgState* st)
 {
     if (st->done) return;
-    const double m = st->m;
+    const double m = use_mz ? st->mz : st->m;
     double acc[1] = {0.0};
     if (st->i == 0)
         cg_p_tiles<true>(o, tl, r, po, pn, m, 0.0, acc[0]);
@@ -355,6 +327,7 @@ __global__ void __launch_bounds__(256) k_cg_xf(const double* part, int nb, doubl
     double dp = sqrt(beta > 0.0 ? beta : 0.0);
     st->its = st->i + 1;
     st->m = m;
+    st->sr = tot[0];
     st->betaold = st->beta;
     st->beta = beta;
     st->dp = dp;
@@ -472,11 +445,12 @@ hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, Field
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> r, FieldView<double> p_old,
-                       FieldView<double> p_new, Reduce red, CgState* st)
+hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> z, FieldView<double> p_old,
+                       FieldView<double> p_new, int use_mz, Reduce red, CgState* st)
 {
     Tiles tl = cg_tiles(g);
-    hipLaunchKernelGGL(k_cg_p, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, r, p_old, p_new, red.part, st);
+    hipLaunchKernelGGL(k_cg_p, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, z, p_old, p_new, use_mz,
+                       red.part, st);
     hipLaunchKernelGGL(k_cg_pf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, st);
     return hipGetLastError();
 }

@@ -1,0 +1,43 @@
+// Tiling of the pressure unknowns (i in 1..maxl-1, j in 1..P, k in 1..maxn-1)
+// shared by the conjugate-gradient and multigrid kernels: tiles of
+// BX x BY columns x CG_KC levels (one wavefront spans a 512-byte row
+// segment; the level march reuses z neighbours in registers), walked
+// grid-stride by at most CG_BLOCKS blocks so that a reduction has at most
+// CG_BLOCKS partials.
+#pragma once
+
+#include "tpls_kernels.hpp"
+
+namespace tpls {
+
+constexpr int CG_KC = 4;
+constexpr int CG_BLOCKS = 4096;
+
+struct Tiles {
+    int nbx, nby, nbz, ntiles, nblocks;
+};
+
+inline Tiles cg_tiles(const Grid& g)
+{
+    Tiles t;
+    t.nbx = (g.maxl - 1 + BX - 1) / BX;
+    t.nby = (g.P + BY - 1) / BY;
+    t.nbz = (g.maxn - 1 + CG_KC - 1) / CG_KC;
+    t.ntiles = t.nbx * t.nby * t.nbz;
+    t.nblocks = t.ntiles < CG_BLOCKS ? t.ntiles : CG_BLOCKS;
+    return t;
+}
+
+// Decode tile t into the first interior cell (i, j, k0) of this thread.
+__device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int& j, int& k0)
+{
+    const int bx = t % tl.nbx;
+    const int r = t / tl.nbx;
+    const int by = r % tl.nby;
+    const int bz = r / tl.nby;
+    i = bx * BX + threadIdx.x + 1;
+    j = by * BY + threadIdx.y + 1;
+    k0 = bz * CG_KC + 1;
+}
+
+}  // namespace tpls

@@ -92,9 +92,11 @@ hipError_t launch_uvw_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldVi
 
 // Conjugate-gradient state on the device (PETSc 3.6 KSPCG with pc none and a
 // constant null space, restated on the reduced symmetric system).
+// With the multigrid preconditioner z = M r - mean(M r); sr = sum r and
+// mz = mean(M r) feed beta = z.r = sum(M r * r) - mz * sr.
 struct CgState {
     double beta, betaold, alpha, m, bnorm, rnorm0, ttol, dp, dpi;
-    double rtol, abstol, dtol, mb;
+    double rtol, abstol, dtol, mb, sr, mz;
     int its, i, reason, done, maxits, pad;
 };
 // All CG vectors use the p-grid layout and touch interior cells only.
@@ -106,13 +108,39 @@ hipError_t launch_cg_setup_b(hipStream_t s, const Grid& g, const Phys& ph, Field
 template <typename T>
 hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x,
                              FieldView<double> r, Reduce red2, CgState* st);
-hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> r,
-                       FieldView<double> p_old, FieldView<double> p_new, Reduce red, CgState* st);
+// p <- (z - mean) + beta/beta_old p with z = r (mean st->m) or, when
+// use_mz, z = M r (mean st->mz)
+hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> z,
+                       FieldView<double> p_old, FieldView<double> p_new, int use_mz, Reduce red,
+                       CgState* st);
 hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> p,
                        FieldView<double> x, FieldView<double> r, Reduce red2, CgState* st);
 template <typename T>
 hipError_t launch_pres_ghosts(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> x,
                               FieldView<T> u3, const T* u_inlet, FieldView<T> pres);
+
+// ---- multigrid preconditioner of the pressure solve ----
+// One level of the hierarchy: the reduced operator on nx x ny x nz cells
+// (Neumann in x and z, periodic in y) and its vectors, addressed with
+// Fortran indices 1..n.  Level 0 is the pressure grid itself.
+struct MgLevel {
+    int nx, ny, nz;
+    double cx, cy, cz, diag;
+    FieldView<double> x, b, t;
+};
+constexpr int MG_MAXLEV = 16;
+constexpr int MG_COARSE_MAX = 2048;  // cells of the coarsest level (one block, LDS)
+struct MgHierarchy {
+    int nlev, nu;
+    MgLevel L[MG_MAXLEV];
+};
+// Extents and coefficients of every level (vectors left unset); returns the
+// number of levels.
+int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag, MgLevel* L);
+// z = M r into L[0].x from L[0].b, then partials of z.r and sum z and the
+// scalar update of st (mz, beta); every kernel returns at once when st->done.
+// g is the level-0 grid (maxl = nx+1, P = ny, maxn = nz+1).
+hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st);
 
 // ---- utilities ----
 template <typename T>
