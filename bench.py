@@ -7,10 +7,16 @@ seeded synthetic initial conditions.  A step is one full timestep
 iterations, CG pressure solve to rtol 1e-10, projection).  The whole state is
 resident in HBM before the timed region starts.
 
+The pressure solve is the multigrid-preconditioned CG by default (--pc mg);
+a shorter run of the same case with the plain CG (PETSc's -pc_type none
+configuration) is reported beside it under "plain_cg".
+
 Prints one JSON line with the throughput, p50/p99 step latency, the roofline
-of the dominant kernel (the CG update kernel, timed with HIP events on the
-case's stream during the timed steps) and a CPU baseline (the oracle's C
-restatement on the host cores, on a bounded sample of the same workload).
+of the dominant kernel (timed with HIP events on the case's stream during the
+timed steps; its HBM traffic per launch comes from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary in profiles/ when one exists for this grid) and a CPU
+baseline (the oracle's C restatement on the host cores, on a bounded sample
+of the same workload).
 """
 from __future__ import annotations
 
@@ -33,10 +39,16 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", default="512x128x256")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--pc", default="none", choices=["none", "mg"],
+    ap.add_argument("--pc", default="mg", choices=["none", "mg"],
                     help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
     ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
+    ap.add_argument("--roofline-kernel", default="sor_odd", choices=["sor_odd", "jacobi", "sor_even", "cg_x"])
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="timed steps of the plain-CG comparison run (0 to skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 64 Mi doubles three times (the reference "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -68,8 +80,82 @@ def cpu_baseline(p, its_gpu: float, threads: int):
     }
 
 
+KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x"}
+
+
+def algorithmic_bytes(p, kernel: str) -> float:
+    """Algorithmic HBM bytes per launch: each distinct array read or written
+    once over the interior.  Jacobi: x_old, RHS, viscosity read, x written,
+    32 B per cell; SOR half sweep: x, RHS and viscosity read over the colour's
+    half plus the other colour's x, half of x written, 24 B per cell; CG
+    update: p, x, r read, x, r written, 40 B per unknown.  The Helmholtz
+    launches cycle over u, v, w, so their cell count is the mean of the three
+    component grids."""
+    L, P, N = p.maxl, p.maxm - 1, p.maxn
+    if kernel == "cg_x":
+        return 40.0 * (L - 1) * P * (N - 1)
+    cells = ((L - 2) * P * (N - 1) + (L - 1) * P * (N - 1) + (L - 1) * P * (N - 2)) / 3.0
+    return (32.0 if kernel == "jacobi" else 24.0) * cells
+
+
+def measured_traffic(kernel: str, nx: int, ny: int, nz: int, precision: str):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary of this grid, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{nx}x{ny}x{nz}_{precision}.md")
+    if not os.path.exists(path):
+        return None
+    name = KERNEL_NAMES[kernel]
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == name:
+            try:
+                return int(float(cells[-1]))
+            except ValueError:
+                return None
+    return None
+
+
+def plain_cg_run(p, a):
+    """The same case with the plain CG: a shorter timed run."""
+    import numpy as np
+    from dataclasses import replace
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    q = replace(p, ksp_pc="none")
+    case = Case(q, precision=a.precision)
+    case.load(*make_initial(q))
+    case.step(1)
+    case.sync()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(a.plain_steps):
+        case.step(1)
+        case.sync()
+        its.append(case.diag()["ksp_its"])
+    el = time.perf_counter() - t0
+    case.close()
+    return {"ms_per_step": round(el / a.plain_steps * 1e3, 3),
+            "value": round(q.cells * a.plain_steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
+            "steps": a.plain_steps, "warmup": 1, "cg_its_mean": round(float(np.mean(its)), 1)}
+
+
+def copy_probe():
+    import torch
+    n = 64 << 20
+    src = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe": {"launches": 3, "bytes_read_per_launch": n * 8,
+                                     "bytes_written_per_launch": n * 8}}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs one case per GPU on a single GPU; "
                                    "--gpus must be 1 (see DESIGN.md for how the path shards)"}))
@@ -88,7 +174,7 @@ def main():
     for _ in range(a.warmup):
         case.step(1)
     case.sync()
-    case.time_kernels(True)
+    case.time_kernels(a.roofline_kernel)
     lat = []
     its = []
     t0 = time.perf_counter()
@@ -101,18 +187,21 @@ def main():
     case.sync()
     elapsed = time.perf_counter() - t0
     ms, n = case.kernel_time()
-    case.time_kernels(False)
+    case.time_kernels(None)
     d = case.diag()
     case.close()
+
+    plain = None
+    if a.plain_steps > 0 and a.pc != "none":
+        plain = plain_cg_run(p, a)
 
     cells = p.cells
     value = cells * a.steps / elapsed * 1e-6
     lat_ms = np.array(lat) * 1e3
-    # cg_x per launch: reads p, x, r and writes x, r -> 40 B per unknown
-    n_unknowns = (p.maxl - 1) * (p.maxm - 1) * (p.maxn - 1)
-    kbytes = 40.0 * n_unknowns
+    kbytes = algorithmic_bytes(p, a.roofline_kernel)
     avg_ms = ms / max(n, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
+    traffic = measured_traffic(a.roofline_kernel, nx, ny, nz, a.precision)
     out = {
         "metric": "Mcell-steps/s batched-FP32 vs plain-FP64 at stated error; p50/p99 step latency",
         "value": round(value, 4),
@@ -133,11 +222,13 @@ def main():
                                f"CG rtol {p.ksp_rtol:g}",
                    "solver": "cg" if a.pc == "none" else f"cg + multigrid V({a.mg_nu},{a.mg_nu})",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
-        "roofline": {"kernel": "cg_x", "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": KERNEL_NAMES[a.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
+                     "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
                      "algorithmic_bytes_per_launch": int(kbytes)},
     }
+    if plain:
+        out["plain_cg"] = plain
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)

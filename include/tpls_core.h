@@ -116,10 +116,19 @@ int tpls_step(tpls_case *c, int nsteps);
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Wait for all work of the case. */
 int tpls_case_sync(tpls_case *c);
-/* Kernel timing: while enabled, every launch of the CG update kernel (cg_x)
- * is bracketed by HIP events on the case's stream; tpls_case_kernel_time
- * returns the summed milliseconds and launch count since enabling. */
-int tpls_case_time_kernels(tpls_case *c, int enable);
+/* Kernel timing: while a kernel class is selected, every launch of that
+ * class is bracketed by HIP events on the case's stream (the stream it runs
+ * on); tpls_case_kernel_time waits for the case and returns the summed
+ * milliseconds and launch count since the selection.  Launches of the CG
+ * update that return at once after convergence are not counted. */
+enum {
+    TPLS_TIME_OFF = 0,
+    TPLS_TIME_CG_X = 1,      /* CG update x += alpha p, r -= alpha A p */
+    TPLS_TIME_JACOBI = 2,    /* Jacobi sweep (do_jacobi_u/v/w) */
+    TPLS_TIME_SOR_ODD = 3,   /* SOR half sweep, odd iteration_time (k-ascending columns) */
+    TPLS_TIME_SOR_EVEN = 4   /* SOR half sweep, even iteration_time (3-D colouring) */
+};
+int tpls_case_time_kernels(tpls_case *c, int kernel_class);
 int tpls_case_kernel_time(tpls_case *c, double *ms, long *launches);
 
 /* ---- per-routine drop-in entry points (host arrays) ---- */

@@ -79,8 +79,13 @@ class Case:
     def sync(self) -> None:
         _lib.check(_lib.lib().tpls_case_sync(self._h), "tpls_case_sync")
 
-    def time_kernels(self, enable: bool) -> None:
-        _lib.check(_lib.lib().tpls_case_time_kernels(self._h, 1 if enable else 0), "time_kernels")
+    TIMED = {None: 0, False: 0, "cg_x": 1, True: 1, "jacobi": 2, "sor_odd": 3, "sor_even": 4}
+
+    def time_kernels(self, kernel=True) -> None:
+        """Bracket every launch of one kernel class with HIP events on the
+        case's stream: "cg_x", "jacobi", "sor_odd", "sor_even" (True = "cg_x";
+        None or False stops timing)."""
+        _lib.check(_lib.lib().tpls_case_time_kernels(self._h, self.TIMED[kernel]), "time_kernels")
 
     def kernel_time(self):
         ms, n = C.c_double(0), C.c_long(0)

@@ -134,8 +134,8 @@ struct tpls_case {
     tpls_diag diag{};
     long steps_done = 0;
     int last_its = 0;
-    // kernel timing of cg_x
-    bool timing = false;
+    // kernel timing: the selected TPLS_TIME_* class
+    int timed = TPLS_TIME_OFF;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     double t_ms = 0.0;
@@ -218,6 +218,7 @@ void wrap_rows(double* h, const Shape& sh, int maxm)
 
 template <typename T>
 int step_impl(tpls_case* c);
+int harvest_events(tpls_case* c);
 
 DevArray* field_ptr(tpls_case* c, int f)
 {
@@ -472,10 +473,12 @@ int tpls_case_sync(tpls_case* c)
     return TPLS_OK;
 }
 
-int tpls_case_time_kernels(tpls_case* c, int enable)
+int tpls_case_time_kernels(tpls_case* c, int kernel_class)
 {
     if (!c) return fail(TPLS_ERR_ARG, "null case");
-    c->timing = enable != 0;
+    if (kernel_class < TPLS_TIME_OFF || kernel_class > TPLS_TIME_SOR_EVEN) return fail(TPLS_ERR_ARG, "bad kernel class");
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->timed = kernel_class;
     c->t_ms = 0.0;
     c->t_n = 0;
     c->ev_used = 0;
@@ -485,6 +488,9 @@ int tpls_case_time_kernels(tpls_case* c, int enable)
 int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
 {
     if (!c || !ms || !launches) return fail(TPLS_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    int rc = harvest_events(c);
+    if (rc) return rc;
     *ms = c->t_ms;
     *launches = c->t_n;
     return TPLS_OK;
@@ -583,26 +589,22 @@ int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> 
         for (long q = 0; q < chunk; ++q) {
             FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
             LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (c->timing) {
-                e0 = next_event(c);
-                LCHK(hipEventRecord(e0, c->s));
-            }
+            const bool tx = c->timed == TPLS_TIME_CG_X;
+            if (tx) LCHK(hipEventRecord(next_event(c), c->s));
             LCHK(launch_cg_x(c->s, g, c->ph, pn, x, r, red_of(c, 7), c->cg));
-            if (c->timing) {
-                e1 = next_event(c);
-                LCHK(hipEventRecord(e1, c->s));
-            }
+            if (tx) LCHK(hipEventRecord(next_event(c), c->s));
             if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
             cur ^= 1;
         }
         launched += chunk;
         LCHK(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgState), hipMemcpyDeviceToHost, c->s));
         LCHK(hipStreamSynchronize(c->s));
-        if (c->timing) {
+        if (c->timed == TPLS_TIME_CG_X) {
             // drop pairs recorded after convergence (those launches returned at once)
             long live = std::max<long>(0, c->h_cg->its - (launched - chunk));
             if (live < chunk) c->ev_used -= size_t(2 * (chunk - live));
+        }
+        if (c->timed) {
             int rc = harvest_events(c);
             if (rc) return rc;
         }
@@ -674,10 +676,17 @@ int step_impl(tpls_case* c)
         int dstb = others[1];
         for (int it = 1; it <= mom[q] / 2; ++it) {
             FieldView<T> xs = V(c->x[q][srcb]), xd = V(c->x[q][dstb]);
+            const bool tj = c->timed == TPLS_TIME_JACOBI;
+            const bool ts = c->timed == (it % 2 ? TPLS_TIME_SOR_ODD : TPLS_TIME_SOR_EVEN);
+            if (tj) LCHK(hipEventRecord(next_event(c), s));
             LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xd, it == 1));
+            if (tj) LCHK(hipEventRecord(next_event(c), s));
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
-            LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), 0, it));
-            LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), 1, it));
+            for (int flg = 0; flg < 2; ++flg) {
+                if (ts) LCHK(hipEventRecord(next_event(c), s));
+                LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), flg, it));
+                if (ts) LCHK(hipEventRecord(next_event(c), s));
+            }
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
             std::swap(srcb, dstb);
         }

@@ -10,21 +10,72 @@
 
 namespace tpls {
 
-template <typename C, typename T>
-__device__ __forceinline__ MuFaces<C> mu_faces(int comp, const FieldView<T>& mu, int i, int j, int k, int P)
+// Raw viscosity values behind the six face viscosities of one cell: a, b
+// and the two extra values of each of the four averaged faces, in the
+// averaging order of jacobi_iteration_allflux.f90:75-82 (u), 153-160 (v),
+// 231-238 (w).  Loading them apart from the averaging lets the column sweep
+// issue the loads of later levels before it needs them.
+template <typename C>
+struct MuRaw {
+    C a, b, q[4][2];
+};
+
+template <int COMP, typename C, typename T>
+__device__ __forceinline__ MuRaw<C> mu_raw(const FieldView<T>& mu, int i, int j, int k, int P)
 {
-    if (comp == 0) return mu_faces_u<C>(mu, i, j, k, P);
-    if (comp == 1) return mu_faces_v<C>(mu, i, j, k, P);
-    return mu_faces_w<C>(mu, i, j, k, P);
+    const int j1 = wrapj(j + 1, P), j0 = wrapj(j, P), j2 = wrapj(j + 2, P);
+    MuRaw<C> r;
+    if (COMP == 0) {
+        r.a = mu(i, j1, k + 1);
+        r.b = mu(i + 1, j1, k + 1);
+        r.q[0][0] = mu(i, j2, k + 1), r.q[0][1] = mu(i + 1, j2, k + 1);   // yp
+        r.q[1][0] = mu(i, j0, k + 1), r.q[1][1] = mu(i + 1, j0, k + 1);   // ym
+        r.q[2][0] = mu(i, j1, k + 2), r.q[2][1] = mu(i + 1, j1, k + 2);   // zp
+        r.q[3][0] = mu(i, j1, k), r.q[3][1] = mu(i + 1, j1, k);           // zm
+    } else if (COMP == 1) {
+        r.a = mu(i, j1, k + 1);
+        r.b = mu(i, j0, k + 1);
+        r.q[0][0] = mu(i + 1, j1, k + 1), r.q[0][1] = mu(i + 1, j0, k + 1);  // xp
+        r.q[1][0] = mu(i - 1, j1, k + 1), r.q[1][1] = mu(i - 1, j0, k + 1);  // xm
+        r.q[2][0] = mu(i, j1, k + 2), r.q[2][1] = mu(i, j0, k + 2);          // zp
+        r.q[3][0] = mu(i, j1, k), r.q[3][1] = mu(i, j0, k);                  // zm
+    } else {
+        r.a = mu(i, j1, k);
+        r.b = mu(i, j1, k + 1);
+        r.q[0][0] = mu(i + 1, j1, k), r.q[0][1] = mu(i + 1, j1, k + 1);  // xp
+        r.q[1][0] = mu(i - 1, j1, k), r.q[1][1] = mu(i - 1, j1, k + 1);  // xm
+        r.q[2][0] = mu(i, j2, k), r.q[2][1] = mu(i, j2, k + 1);          // yp
+        r.q[3][0] = mu(i, j0, k), r.q[3][1] = mu(i, j0, k + 1);          // ym
+    }
+    return r;
 }
 
-__device__ __forceinline__ int imax_of(int comp, int maxl) { return comp == 0 ? maxl - 2 : maxl - 1; }
+// The face viscosities from the raw values, with mu_faces_u/v/w's arithmetic.
+template <int COMP, typename C>
+__device__ __forceinline__ MuFaces<C> mu_cook(const MuRaw<C>& r)
+{
+    auto avg = [&](int f) { return (r.a + r.b + r.q[f][0] + r.q[f][1]) / C(4); };
+    MuFaces<C> m;
+    if (COMP == 0) {
+        m.xp = r.b, m.xm = r.a, m.yp = avg(0), m.ym = avg(1), m.zp = avg(2), m.zm = avg(3);
+    } else if (COMP == 1) {
+        m.xp = avg(0), m.xm = avg(1), m.yp = r.a, m.ym = r.b, m.zp = avg(2), m.zm = avg(3);
+    } else {
+        m.xp = avg(0), m.xm = avg(1), m.yp = avg(2), m.ym = avg(3), m.zp = r.b, m.zm = r.a;
+    }
+    return m;
+}
+
+template <int COMP>
+__device__ __forceinline__ int imax_of(int maxl) { return COMP == 0 ? maxl - 2 : maxl - 1; }
+template <int COMP>
+__device__ __forceinline__ int kmin_of() { return COMP == 2 ? 1 : 0; }
 
 // Wall-ghost neighbours in z for u and v (jacobi_iteration_allflux.f90:63-73).
-template <typename C, typename T>
-__device__ __forceinline__ void z_nbrs(int comp, const FieldView<T>& x, int i, int j, int k, int maxn, C& zp, C& zm)
+template <int COMP, typename C, typename T>
+__device__ __forceinline__ void z_nbrs(const FieldView<T>& x, int i, int j, int k, int maxn, C& zp, C& zm)
 {
-    if (comp == 2) {
+    if (COMP == 2) {
         zp = x(i, j, k + 1);
         zm = x(i, j, k - 1);
         return;
@@ -33,41 +84,64 @@ __device__ __forceinline__ void z_nbrs(int comp, const FieldView<T>& x, int i, i
     zp = (k == maxn - 2) ? C(-2) * C(x(i, j, maxn - 2)) + C(x(i, j, maxn - 3)) / C(3) : C(x(i, j, k + 1));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_jacobi(Grid g, int comp, T ax2, T ay2, T az2, FieldView<T> xo,
+// Blocks of a 1-D launch are dealt to the 8 XCDs round-robin; map them so
+// that each XCD walks a contiguous range of tiles (neighbouring planes then
+// share that XCD's L2).  A bijection on [0, nblk).
+__device__ __forceinline__ int xcd_tile(int b, int nblk)
+{
+    const int per = nblk >> 3;
+    return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+}
+
+// One cell per thread over tiles of 64 x 4 columns, one level per tile.
+template <int COMP>
+__device__ __forceinline__ bool cell_of(const Grid& g, int nbx, int nby, int& i, int& j, int& k, int ihalf)
+{
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+    i = bx * BX + threadIdx.x;
+    j = by * BY + threadIdx.y + 1;
+    k = bz + kmin_of<COMP>();
+    (void)ihalf;
+    return j <= g.P;
+}
+
+template <int COMP, typename T>
+__global__ void __launch_bounds__(BX* BY) k_jacobi(Grid g, int nbx, int nby, T ax2, T ay2, T az2, FieldView<T> xo,
                                                    FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, int xo_zero)
 {
     using C = T;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + (comp == 2 ? 1 : 0);
-    if (i > imax_of(comp, g.maxl) || j > g.P) return;
-    MuFaces<C> m = mu_faces<C>(comp, mu, i, j, k, g.P);
+    int i, j, k;
+    if (!cell_of<COMP>(g, nbx, nby, i, j, k, 0)) return;
+    i += 1;
+    if (i > imax_of<COMP>(g.maxl)) return;
+    MuFaces<C> m = mu_cook<COMP, C>(mu_raw<COMP, C>(mu, i, j, k, g.P));
     C xp = 0, xm = 0, yp = 0, ym = 0, zp = 0, zm = 0;
     if (!xo_zero) {
         xp = xo(i + 1, j, k);
         xm = xo(i - 1, j, k);
         yp = xo(i, wrapj(j + 1, g.P), k);
         ym = xo(i, wrapj(j - 1, g.P), k);
-        z_nbrs<C>(comp, xo, i, j, k, g.maxn, zp, zm);
+        z_nbrs<COMP, C>(xo, i, j, k, g.maxn, zp, zm);
     }
     x(i, j, k) = helmholtz<C>(m, ax2, ay2, az2, xp, xm, yp, ym, zp, zm, C(rhs(i, j, k)));
 }
 
 // Even iteration_time: 3-D colouring (i+j+k) = 1+flg (mod 2); every cell of a
-// colour reads only the other colour, so the sweep is fully parallel.
-template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_sor3d(Grid g, int comp, int flg, T ax2, T ay2, T az2, FieldView<T> x,
-                                                  FieldView<T> rhs, FieldView<T> mu)
+// colour reads only the other colour, so the sweep is fully parallel.  One
+// thread per cell of the colour.
+template <int COMP, typename T>
+__global__ void __launch_bounds__(BX* BY) k_sor3d(Grid g, int nbx, int nby, int flg, T ax2, T ay2, T az2,
+                                                  FieldView<T> x, FieldView<T> rhs, FieldView<T> mu)
 {
     using C = T;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + (comp == 2 ? 1 : 0);
-    const int i = ((k + j + flg) & 1) + 1 + 2 * (blockIdx.x * BX + threadIdx.x);
-    if (i > imax_of(comp, g.maxl) || j > g.P) return;
-    MuFaces<C> m = mu_faces<C>(comp, mu, i, j, k, g.P);
+    int t, j, k;
+    if (!cell_of<COMP>(g, nbx, nby, t, j, k, 1)) return;
+    const int i = ((k + j + flg) & 1) + 1 + 2 * t;
+    if (i > imax_of<COMP>(g.maxl)) return;
+    MuFaces<C> m = mu_cook<COMP, C>(mu_raw<COMP, C>(mu, i, j, k, g.P));
     C zp, zm;
-    z_nbrs<C>(comp, x, i, j, k, g.maxn, zp, zm);
+    z_nbrs<COMP, C>(x, i, j, k, g.maxn, zp, zm);
     C c = x(i, j, k);
     C res = helmholtz<C>(m, ax2, ay2, az2, C(x(i + 1, j, k)), C(x(i - 1, j, k)), C(x(i, wrapj(j + 1, g.P), k)),
                          C(x(i, wrapj(j - 1, g.P), k)), zp, zm, C(rhs(i, j, k)));
@@ -76,37 +150,72 @@ __global__ void __launch_bounds__(BX* BY) k_sor3d(Grid g, int comp, int flg, T a
 
 // Odd iteration_time: 2-D colouring (i+j) = 1+flg (mod 2) with k ascending,
 // i.e. Gauss-Seidel up each column (sor_iteration_allflux.f90:100-146 run on
-// one thread).  One thread per column of the colour.
-template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_sor2d(Grid g, int comp, int flg, T ax2, T ay2, T az2, FieldView<T> x,
+// one thread).  One thread per column of the colour.  The only dependence
+// between levels is the updated x(k-1); every other input of level k (the
+// other colour's neighbours, the old x(k+1), RHS, viscosity) is loaded SOR_D
+// levels ahead into a register ring, so the march is not serialised on
+// memory latency.
+constexpr int SOR_D = 4;
+
+template <typename C>
+struct SorLevel {
+    MuRaw<C> mu;
+    C xp, xm, yp, ym, rhs, self, up;
+};
+
+template <int COMP, typename C, typename T>
+__device__ __forceinline__ void sor_load(SorLevel<C>& L, const FieldView<T>& x, const FieldView<T>& rhs,
+                                         const FieldView<T>& mu, int i, int j, int jp, int jm, int k, int k1, int P)
+{
+    L.mu = mu_raw<COMP, C>(mu, i, j, k, P);
+    L.xp = x(i + 1, j, k);
+    L.xm = x(i - 1, j, k);
+    L.yp = x(i, jp, k);
+    L.ym = x(i, jm, k);
+    L.rhs = rhs(i, j, k);
+    L.self = x(i, j, k);
+    // w reads its wall value x(maxn-1) above the top level; u and v have no
+    // level above k1 = maxn-2
+    L.up = (k < k1 || COMP == 2) ? C(x(i, j, k + 1)) : C(0);
+}
+
+template <int COMP, typename T>
+__global__ void __launch_bounds__(BX* BY) k_sor2d(Grid g, int flg, T ax2, T ay2, T az2, FieldView<T> x,
                                                   FieldView<T> rhs, FieldView<T> mu)
 {
     using C = T;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     const int i = ((j + flg) & 1) + 1 + 2 * (blockIdx.x * BX + threadIdx.x);
-    if (i > imax_of(comp, g.maxl) || j > g.P) return;
+    if (i > imax_of<COMP>(g.maxl) || j > g.P) return;
     const int P = g.P, maxn = g.maxn;
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
-    const int k0 = comp == 2 ? 1 : 0, k1 = maxn - 2;
-    C prev = 0;        // x(i,j,k-1) after its update
-    C cur = x(i, j, k0);
-    for (int k = k0; k <= k1; ++k) {
-        MuFaces<C> m = mu_faces<C>(comp, mu, i, j, k, P);
-        C nxt = (k < k1 || comp == 2) ? C(x(i, j, k + 1)) : C(0);
-        C zp, zm;
-        if (comp == 2) {
-            zp = nxt;
-            zm = (k == k0) ? C(x(i, j, k - 1)) : prev;
-        } else {
-            zm = (k == 0) ? C(-2) * cur + nxt / C(3) : prev;
-            zp = (k == maxn - 2) ? C(-2) * cur + prev / C(3) : nxt;
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2;
+    SorLevel<C> ring[SOR_D];
+#pragma unroll
+    for (int d = 0; d < SOR_D; ++d)
+        if (k0 + d <= k1) sor_load<COMP, C>(ring[d], x, rhs, mu, i, j, jp, jm, k0 + d, k1, P);
+    C prev = COMP == 2 ? C(x(i, j, k0 - 1)) : C(0);  // x(i,j,k-1) after its update
+    for (int kb = k0; kb <= k1; kb += SOR_D) {
+#pragma unroll
+        for (int d = 0; d < SOR_D; ++d) {
+            const int k = kb + d;
+            if (k > k1) break;
+            const SorLevel<C>& L = ring[d];
+            const C cur = L.self, nxt = L.up;
+            C zp, zm;
+            if (COMP == 2) {
+                zp = nxt;
+                zm = prev;
+            } else {
+                zm = (k == 0) ? C(-2) * cur + nxt / C(3) : prev;
+                zp = (k == maxn - 2) ? C(-2) * cur + prev / C(3) : nxt;
+            }
+            const C res = helmholtz<C>(mu_cook<COMP, C>(L.mu), ax2, ay2, az2, L.xp, L.xm, L.yp, L.ym, zp, zm, L.rhs);
+            const C upd = cur + C(1.2) * (res - cur);
+            x(i, j, k) = upd;
+            prev = upd;
+            if (k + SOR_D <= k1) sor_load<COMP, C>(ring[d], x, rhs, mu, i, j, jp, jm, k + SOR_D, k1, P);
         }
-        C res = helmholtz<C>(m, ax2, ay2, az2, C(x(i + 1, j, k)), C(x(i - 1, j, k)), C(x(i, jp, k)), C(x(i, jm, k)),
-                             zp, zm, C(rhs(i, j, k)));
-        C upd = cur + C(1.2) * (res - cur);
-        x(i, j, k) = upd;
-        prev = upd;
-        cur = nxt;
     }
 }
 
@@ -159,9 +268,16 @@ hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp,
                          FieldView<T> mu, FieldView<T> x, bool xold_is_zero)
 {
     const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
-    dim3 grd((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, nz_of(comp, g));
-    hipLaunchKernelGGL(k_jacobi<T>, grd, dim3(BX, BY), 0, s, g, comp, T(ax / 2.0), T(ay / 2.0), T(az / 2.0), xold,
-                       rhs, mu, x, xold_is_zero ? 1 : 0);
+    const int nbx = (g.maxl - 1 + BX - 1) / BX, nby = (g.P + BY - 1) / BY;
+    const dim3 grd(nbx * nby * nz_of(comp, g));
+    const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
+    const int z = xold_is_zero ? 1 : 0;
+    if (comp == 0)
+        hipLaunchKernelGGL((k_jacobi<0, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
+    else if (comp == 1)
+        hipLaunchKernelGGL((k_jacobi<1, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
+    else
+        hipLaunchKernelGGL((k_jacobi<2, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
     return hipGetLastError();
 }
 
@@ -170,15 +286,25 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
                       FieldView<T> mu, int flg, int iteration_time)
 {
     const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
+    const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
     const int half = (g.maxl - 1 + 1) / 2;
+    const int nbx = (half + BX - 1) / BX, nby = (g.P + BY - 1) / BY;
     if (iteration_time % 2 == 0) {
-        dim3 grd((half + BX - 1) / BX, (g.P + BY - 1) / BY, nz_of(comp, g));
-        hipLaunchKernelGGL(k_sor3d<T>, grd, dim3(BX, BY), 0, s, g, comp, flg, T(ax / 2.0), T(ay / 2.0), T(az / 2.0), x,
-                           rhs, mu);
+        const dim3 grd(nbx * nby * nz_of(comp, g));
+        if (comp == 0)
+            hipLaunchKernelGGL((k_sor3d<0, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
+        else if (comp == 1)
+            hipLaunchKernelGGL((k_sor3d<1, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
+        else
+            hipLaunchKernelGGL((k_sor3d<2, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
     } else {
-        dim3 grd((half + BX - 1) / BX, (g.P + BY - 1) / BY, 1);
-        hipLaunchKernelGGL(k_sor2d<T>, grd, dim3(BX, BY), 0, s, g, comp, flg, T(ax / 2.0), T(ay / 2.0), T(az / 2.0), x,
-                           rhs, mu);
+        const dim3 grd(nbx, nby, 1);
+        if (comp == 0)
+            hipLaunchKernelGGL((k_sor2d<0, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
+        else if (comp == 1)
+            hipLaunchKernelGGL((k_sor2d<1, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
+        else
+            hipLaunchKernelGGL((k_sor2d<2, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
     }
     return hipGetLastError();
 }
