@@ -1,8 +1,13 @@
-"""Build the HIP library ``tpls_amd/libtpls_hip.so`` for gfx950.
+"""Build the HIP library ``tpls_amd/libtpls_hip.so`` for gfx950 and the
+Fortran host modules.
 
 Every source under ``tpls_amd/csrc`` is compiled with hipcc in parallel and
 linked into one shared library exporting the C ABI of
-``include/tpls_core.h``.  Usage: ``python -m tpls_amd.build``.
+``include/tpls_core.h``.  ``tpls_amd/fortran/tpls_modules.f90`` (the modules
+with the reference's names that main_ns_hybrid.F90 uses) is compiled with
+amdflang into ``tpls_amd/fortran/libtpls_fortran.a`` plus its ``.mod`` files,
+and the test driver ``tests/fortran/drive_modules`` is linked against both.
+Usage: ``python -m tpls_amd.build``.
 """
 from __future__ import annotations
 
@@ -16,6 +21,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtpls_hip.so")
+FDIR = os.path.join(HERE, "fortran")
+FMOD = os.path.join(FDIR, "mod")
+FLIB = os.path.join(FDIR, "libtpls_fortran.a")
+DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drive_modules.f90")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_modules")
+FC = os.environ.get("TPLS_FC", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -53,7 +64,35 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    build_fortran(verbose)
     return LIB
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_fortran(verbose: bool = True) -> None:
+    """The Fortran host modules and the test driver (skipped without amdflang)."""
+    if not os.path.exists(FC):
+        if verbose:
+            print(f"{FC} not found: Fortran host modules not built")
+        return
+    os.makedirs(FMOD, exist_ok=True)
+    obj = os.path.join(FMOD, "tpls_modules.o")
+    _run([FC, "-O2", "-fPIC", "-module-dir", FMOD, "-c", os.path.join(FDIR, "tpls_modules.f90"), "-o", obj],
+         "amdflang tpls_modules.f90")
+    if os.path.exists(FLIB):
+        os.remove(FLIB)
+    _run(["ar", "rcs", FLIB, obj], "ar")
+    dobj = os.path.join(FMOD, "drive_modules.o")
+    _run([FC, "-O2", "-module-dir", FMOD, "-I", FMOD, "-c", DRIVER_SRC, "-o", dobj], "amdflang drive_modules.f90")
+    _run([FC, "-o", DRIVER, dobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
+         "link drive_modules")
+    if verbose:
+        print(f"built {FLIB} and {DRIVER}")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,524 @@
+! Fortran host side of the MI355X-native TPLS stencil core.
+!
+! These modules carry the reference's module and procedure names and dummy
+! argument lists (weno.f90:18, advect_phi.f90:19, two_phase_levelset.f90:20,
+! levelset.f90:17/97/241, momentum_allflux.f90:22/224/474,
+! jacobi_iteration_allflux.f90:21/102/195/285, sor_iteration_allflux.f90:19/
+! 152/280, pressure.f90:16/59), so main_ns_hybrid.F90 compiles against them
+! unchanged and its calls run the gfx950 kernels of libtpls_hip.so through
+! the C ABI of include/tpls_core.h (module tpls_core_c below).
+!
+! The MPI arguments of do_advect_phi and do_tpls (neighbours, communicator,
+! stride datatypes) are accepted and ignored: the whole domain lives on one
+! GPU and the halo exchanges are in-kernel periodic wraps.  A non-zero status
+! from the C ABI stops the program with the library's message, as the
+! reference's abort() does for a failing MPI or PETSc call.
+!
+! The pressure solve main_ns_hybrid.F90:896-913 goes through PETSc objects
+! (KSPSolve and the callbacks of pressure_solver.F90); module tpls_pressure
+! offers the whole sequence as one call.
+
+module tpls_core_c
+  use iso_c_binding
+  implicit none
+
+  interface
+    function tpls_last_error() bind(C, name="tpls_last_error")
+      import :: c_ptr
+      type(c_ptr) :: tpls_last_error
+    end function
+
+    integer(c_int) function c_weno(phi, u, v, w, sx, ex, sy, ey, l, n, dfx, dfy, dfz) &
+        bind(C, name="tpls_wenoappoxconvection")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: phi(*), u(*), v(*), w(*)
+      integer(c_int), value :: sx, ex, sy, ey, l, n
+      real(c_double), intent(inout) :: dfx(*), dfy(*), dfz(*)
+    end function
+
+    integer(c_int) function c_advect_phi(phi2, u2, v2, w2, conv2_phi, sx, ex, sy, ey, maxl, maxn, &
+        ex_max, dx, dy, dz) bind(C, name="tpls_do_advect_phi")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: phi2(*), u2(*), v2(*), w2(*), conv2_phi(*)
+      integer(c_int), value :: sx, ex, sy, ey, maxl, maxn, ex_max
+      real(c_double), value :: dx, dy, dz
+    end function
+
+    integer(c_int) function c_tpls(phi2, phi3, c0, c1, c2, sx, ex, sy, ey, maxn, ex_max, its, &
+        height, err2, dx, dz, dt) bind(C, name="tpls_do_tpls")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: phi2(*), phi3(*), c0(*), c1(*), c2(*)
+      integer(c_int), value :: sx, ex, sy, ey, maxn, ex_max, its
+      real(c_double), value :: height, dx, dz, dt
+      real(c_double), intent(out) :: err2
+    end function
+
+    integer(c_int) function c_status(phi, d, ist, sg, sx, ex, sy, ey, n, h) &
+        bind(C, name="tpls_get_levelset_status")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: phi(*)
+      real(c_double), intent(inout) :: d(*), sg(*)
+      integer(c_int), intent(inout) :: ist(*)
+      integer(c_int), value :: sx, ex, sy, ey, n
+      real(c_double), value :: h
+    end function
+
+    integer(c_int) function c_reinit(phi, pr, d, ist, sg, sx, ex, sy, ey, n, ex_max, h, height, err) &
+        bind(C, name="tpls_do_levelset_iteration")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: phi(*), d(*), sg(*)
+      real(c_double), intent(inout) :: pr(*)
+      integer(c_int), intent(in) :: ist(*)
+      integer(c_int), value :: sx, ex, sy, ey, n, ex_max
+      real(c_double), value :: h, height
+      real(c_double), intent(out) :: err
+    end function
+
+    integer(c_int) function c_csf(fx, fy, fz, fi, sx, ex, sy, ey, n, ex_max, dx, dy, dz, scap, sw) &
+        bind(C, name="tpls_get_csf")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: fx(*), fy(*), fz(*)
+      real(c_double), intent(in) :: fi(*)
+      integer(c_int), value :: sx, ex, sy, ey, n, ex_max
+      real(c_double), value :: dx, dy, dz, scap, sw
+    end function
+
+    integer(c_int) function c_diffusion(du, dv, dw, mu, u2, v2, w2, sx, sy, ex, ey, maxn, ex_max, &
+        dx, dy, dz) bind(C, name="tpls_get_diffusion_all")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: du(*), dv(*), dw(*)
+      real(c_double), intent(in) :: mu(*), u2(*), v2(*), w2(*)
+      integer(c_int), value :: sx, sy, ex, ey, maxn, ex_max
+      real(c_double), value :: dx, dy, dz
+    end function
+
+    integer(c_int) function c_conv(cu, cv, cw, mu, u2, v2, w2, sx, sy, ex, ey, maxn, ex_max, &
+        dx, dy, dz) bind(C, name="tpls_get_conv_all")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: cu(*), cv(*), cw(*)
+      real(c_double), intent(in) :: mu(*), u2(*), v2(*), w2(*)
+      integer(c_int), value :: sx, sy, ex, ey, maxn, ex_max
+      real(c_double), value :: dx, dy, dz
+    end function
+
+    integer(c_int) function c_viscosity(mu, phi, ex, ey, sx, sy, maxn, sw, re, mu_minus, mu_plus) &
+        bind(C, name="tpls_get_viscosity")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: mu(*)
+      real(c_double), intent(in) :: phi(*)
+      integer(c_int), value :: ex, ey, sx, sy, maxn
+      real(c_double), value :: sw, re, mu_minus, mu_plus
+    end function
+
+    integer(c_int) function c_jacobi_u(x, xo, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max) &
+        bind(C, name="tpls_do_jacobi_u")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: xo(*), rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, ex_max
+    end function
+    integer(c_int) function c_jacobi_v(x, xo, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max) &
+        bind(C, name="tpls_do_jacobi_v")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: xo(*), rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, ex_max
+    end function
+    integer(c_int) function c_jacobi_w(x, xo, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max) &
+        bind(C, name="tpls_do_jacobi_w")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: xo(*), rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, ex_max
+    end function
+
+    integer(c_int) function c_difference(ua, ub, ex, ey, ez, sx, sy, sz, diff) &
+        bind(C, name="tpls_get_difference")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: ua(*), ub(*)
+      integer(c_int), value :: ex, ey, ez, sx, sy, sz
+      real(c_double), intent(out) :: diff
+    end function
+
+    integer(c_int) function c_sor_u(x, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max, flg, it) &
+        bind(C, name="tpls_do_sor_u")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, ex_max, flg, it
+    end function
+    integer(c_int) function c_sor_v(x, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, it) &
+        bind(C, name="tpls_do_sor_v")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, flg, it
+    end function
+    integer(c_int) function c_sor_w(x, rhs, mu, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, it) &
+        bind(C, name="tpls_do_sor_w")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), intent(in) :: rhs(*), mu(*)
+      real(c_double), value :: dx, dy, dz, dt
+      integer(c_int), value :: ex, ey, sx, sy, maxn, flg, it
+    end function
+
+    integer(c_int) function c_source_pres(rhs, u3, v3, w3, ex, ey, sx, sy, maxn, dx, dy, dz, dt) &
+        bind(C, name="tpls_get_source_pres")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: rhs(*)
+      real(c_double), intent(in) :: u3(*), v3(*), w3(*)
+      integer(c_int), value :: ex, ey, sx, sy, maxn
+      real(c_double), value :: dx, dy, dz, dt
+    end function
+
+    integer(c_int) function c_uvw_pres(u3, v3, w3, p, ex, ey, sx, sy, maxn, ex_max, dx, dy, dz, dt, &
+        u_inlet) bind(C, name="tpls_get_uvw_pres")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: u3(*), v3(*), w3(*)
+      real(c_double), intent(in) :: p(*), u_inlet(*)
+      integer(c_int), value :: ex, ey, sx, sy, maxn, ex_max
+      real(c_double), value :: dx, dy, dz, dt
+    end function
+
+    integer(c_int) function c_pressure_solve(pres, rhs_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, &
+        dt, rtol, abstol, dtol, maxits, its, rnorm, reason) bind(C, name="tpls_pressure_solve")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: pres(*)
+      real(c_double), intent(in) :: rhs_p(*), u3(*), u_inlet(*)
+      integer(c_int), value :: maxl, maxm, maxn, maxits
+      real(c_double), value :: dx, dy, dz, dt, rtol, abstol, dtol
+      integer(c_int), intent(out) :: its, reason
+      real(c_double), intent(out) :: rnorm
+    end function
+  end interface
+
+contains
+
+  ! Stop with the library's message when a call fails.
+  subroutine tpls_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    if (rc == 0) return
+    p = tpls_last_error()
+    n = 0
+    if (c_associated(p)) then
+      call c_f_pointer(p, s, [4096])
+      do while (n < 4096)
+        if (s(n + 1) == c_null_char) exit
+        n = n + 1
+      end do
+      write(0, '(a, a, i0, a, 4096a1)') where, ': tpls status ', rc, ': ', s(1:n)
+    else
+      write(0, '(a, a, i0)') where, ': tpls status ', rc
+    end if
+    error stop 1
+  end subroutine tpls_check
+end module tpls_core_c
+
+
+module weno
+  use tpls_core_c
+  implicit none
+contains
+  subroutine WENOAppoxConvection(phi, u, v, w, sx, ex, sy, ey, l, n, dfx, dfy, dfz)
+    integer, intent(in) :: sx, ex, sy, ey, l, n
+    double precision, intent(in) :: phi(sx-3:ex+3, sy-3:ey+3, 0:n)
+    double precision, intent(in) :: u(sx-1:ex+1, sy-1:ey+1, 0:n-2)
+    double precision, intent(in) :: v(sx-1:ex+1, sy-1:ey+1, 0:n-2)
+    double precision, intent(in) :: w(sx-1:ex+1, sy-1:ey+1, 0:n-1)
+    double precision, intent(out) :: dfx(sx-1:ex+1, sy-1:ey+1, 0:n-2)
+    double precision, intent(out) :: dfy(sx-1:ex+1, sy-1:ey+1, 0:n-2)
+    double precision, intent(out) :: dfz(sx-1:ex+1, sy-1:ey+1, 0:n-1)
+    call tpls_check(c_weno(phi, u, v, w, sx, ex, sy, ey, l, n, dfx, dfy, dfz), 'WENOAppoxConvection')
+  end subroutine WENOAppoxConvection
+end module weno
+
+
+module advect_phi
+  use tpls_core_c
+  implicit none
+contains
+  subroutine do_advect_phi(phi2, u2, v2, w2, conv2_phi, sx, ex, sy, ey, sz_p, ez_p, sz_uv, sz_w, &
+      ez_uv, ez_w, maxl, maxn, ex_max, neighbours, comm2d_quasiperiodic, dx, dy, dz, &
+      stride_p_augaug1_xz, stride_p_augaug1_yz, stride_p_augaug2_xz, stride_p_augaug2_yz, &
+      stride_p_augaug3_xz, stride_p_augaug3_yz, stride_uv_xz, stride_uv_yz, stride_w_xz, stride_w_yz)
+    integer, intent(in) :: sx, ex, sy, ey, sz_p, ez_p, sz_uv, sz_w, ez_uv, ez_w
+    integer, intent(in) :: maxl, maxn, ex_max
+    integer, intent(in) :: neighbours(6)
+    integer, intent(in) :: comm2d_quasiperiodic
+    integer, intent(in) :: stride_p_augaug1_xz, stride_p_augaug1_yz
+    integer, intent(in) :: stride_p_augaug2_xz, stride_p_augaug2_yz
+    integer, intent(in) :: stride_p_augaug3_xz, stride_p_augaug3_yz
+    integer, intent(in) :: stride_uv_xz, stride_uv_yz, stride_w_xz, stride_w_yz
+    double precision, intent(in) :: dx, dy, dz
+    double precision, intent(inout) :: phi2(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: u2(sx-1:ex+1, sy-1:ey+1, sz_uv:ez_uv)
+    double precision, intent(inout) :: v2(sx-1:ex+1, sy-1:ey+1, sz_uv:ez_uv)
+    double precision, intent(inout) :: w2(sx-1:ex+1, sy-1:ey+1, sz_w:ez_w)
+    double precision, intent(out) :: conv2_phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    if (sz_p /= 0 .or. ez_p /= maxn .or. sz_uv /= 0 .or. ez_uv /= maxn - 2 .or. sz_w /= 0 &
+        .or. ez_w /= maxn - 1) call tpls_check(-2_c_int, 'do_advect_phi: z bounds')
+    call tpls_check(c_advect_phi(phi2, u2, v2, w2, conv2_phi, sx, ex, sy, ey, maxl, maxn, ex_max, &
+        dx, dy, dz), 'do_advect_phi')
+  end subroutine do_advect_phi
+end module advect_phi
+
+
+module levelset
+  use tpls_core_c
+  implicit none
+contains
+  subroutine get_levelset_status(phi, D_mx, istatus, sign_mx, sx, ex, sy, ey, n, h)
+    integer, intent(in) :: sx, ex, sy, ey, n
+    double precision, intent(in) :: h
+    double precision, intent(in) :: phi(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: sign_mx(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: D_mx(sx-1:ex+1, sy-1:ey+1, 0:n)
+    integer, intent(out) :: istatus(sx-1:ex+1, sy-1:ey+1, 0:n)
+    call tpls_check(c_status(phi, D_mx, istatus, sign_mx, sx, ex, sy, ey, n, h), 'get_levelset_status')
+  end subroutine get_levelset_status
+
+  subroutine do_levelset_iteration(phi, phi_reinit, D_mx, istatus, sign_mx, sx, ex, sy, ey, n, &
+      ex_max, h, height, error_val)
+    integer, intent(in) :: sx, ex, sy, ey, n, ex_max
+    integer, intent(in) :: istatus(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(in) :: h, height
+    double precision, intent(in) :: phi(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(in) :: sign_mx(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(in) :: D_mx(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(inout) :: phi_reinit(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: error_val
+    call tpls_check(c_reinit(phi, phi_reinit, D_mx, istatus, sign_mx, sx, ex, sy, ey, n, ex_max, h, &
+        height, error_val), 'do_levelset_iteration')
+  end subroutine do_levelset_iteration
+
+  subroutine get_csf(fx_csf, fy_csf, fz_csf, fi, sx, ex, sy, ey, n, ex_max, dx, dy, dz, scap, &
+      smooth_width)
+    integer, intent(in) :: sx, ex, sy, ey, n, ex_max
+    double precision, intent(in) :: dx, dy, dz, scap, smooth_width
+    double precision, intent(in) :: fi(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: fx_csf(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: fy_csf(sx-1:ex+1, sy-1:ey+1, 0:n)
+    double precision, intent(out) :: fz_csf(sx-1:ex+1, sy-1:ey+1, 0:n)
+    call tpls_check(c_csf(fx_csf, fy_csf, fz_csf, fi, sx, ex, sy, ey, n, ex_max, dx, dy, dz, scap, &
+        smooth_width), 'get_csf')
+  end subroutine get_csf
+end module levelset
+
+
+module two_phase_levelset
+  use tpls_core_c
+  implicit none
+contains
+  subroutine do_tpls(phi2, phi3, conv0_phi, conv1_phi, conv2_phi, sx, ex, sy, ey, sz_p, ez_p, maxn, &
+      ex_max, stride_p_xz, stride_p_yz, max_iteration_levelset, neighbours, comm2d_quasiperiodic, &
+      height, err2, dx, dz, dt)
+    integer, intent(in) :: sx, ex, sy, ey, sz_p, ez_p
+    integer, intent(in) :: maxn, ex_max
+    integer, intent(in) :: stride_p_xz, stride_p_yz
+    integer, intent(in) :: max_iteration_levelset
+    integer, intent(in) :: neighbours(6)
+    integer, intent(in) :: comm2d_quasiperiodic
+    double precision, intent(in) :: dx, dz, dt
+    double precision, intent(in) :: height
+    double precision, intent(inout) :: err2
+    double precision, intent(inout) :: phi2(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: conv0_phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: conv1_phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: conv2_phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: phi3(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    if (sz_p /= 0 .or. ez_p /= maxn) call tpls_check(-2_c_int, 'do_tpls: z bounds')
+    call tpls_check(c_tpls(phi2, phi3, conv0_phi, conv1_phi, conv2_phi, sx, ex, sy, ey, maxn, ex_max, &
+        max_iteration_levelset, height, err2, dx, dz, dt), 'do_tpls')
+  end subroutine do_tpls
+end module two_phase_levelset
+
+
+module momentum_allflux
+  use tpls_core_c
+  implicit none
+contains
+  subroutine get_diffusion_all(diff_u, diff_v, diff_w, viscosity, u2, v2, w2, sx, sy, ex, ey, maxn, &
+      ex_max, dx, dy, dz)
+    integer, intent(in) :: sx, sy, ex, ey, maxn, ex_max
+    double precision, intent(in) :: u2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: diff_u(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: v2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: diff_v(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: w2(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(inout) :: diff_w(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz
+    call tpls_check(c_diffusion(diff_u, diff_v, diff_w, viscosity, u2, v2, w2, sx, sy, ex, ey, maxn, &
+        ex_max, dx, dy, dz), 'get_diffusion_all')
+  end subroutine get_diffusion_all
+
+  subroutine get_conv_all(conv_u2, conv_v2, conv_w2, viscosity_aug, u2, v2, w2, sx, sy, ex, ey, &
+      maxn, ex_max, dx, dy, dz)
+    integer, intent(in) :: sx, ex, sy, ey, ex_max, maxn
+    double precision, intent(in) :: dx, dy, dz
+    double precision, intent(inout) :: conv_u2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: u2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: conv_v2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: v2(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: conv_w2(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: w2(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: viscosity_aug(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    call tpls_check(c_conv(conv_u2, conv_v2, conv_w2, viscosity_aug, u2, v2, w2, sx, sy, ex, ey, maxn, &
+        ex_max, dx, dy, dz), 'get_conv_all')
+  end subroutine get_conv_all
+
+  subroutine get_viscosity(viscosity, phi, ex, ey, sx, sy, maxn, smooth_width, Re, mu_minus, mu_plus)
+    integer, intent(in) :: sx, ex, sy, ey, maxn
+    double precision, intent(inout) :: viscosity(sx-1:ex+1, sy-1:ey+1, 0:maxn)
+    double precision, intent(in) :: phi(sx-1:ex+1, sy-1:ey+1, 0:maxn)
+    double precision, intent(in) :: smooth_width, Re, mu_minus, mu_plus
+    call tpls_check(c_viscosity(viscosity, phi, ex, ey, sx, sy, maxn, smooth_width, Re, mu_minus, &
+        mu_plus), 'get_viscosity')
+  end subroutine get_viscosity
+end module momentum_allflux
+
+
+module jacobi_iteration_allflux
+  use tpls_core_c
+  implicit none
+contains
+  subroutine do_jacobi_u(u3, u3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max)
+    integer, intent(in) :: sx, sy, ex, ey, maxn, ex_max
+    double precision, intent(inout) :: u3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: u3_old(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_jacobi_u(u3, u3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, &
+        ex_max), 'do_jacobi_u')
+  end subroutine do_jacobi_u
+
+  subroutine do_jacobi_v(v3, v3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max)
+    integer, intent(in) :: sx, sy, ex, ey, maxn, ex_max
+    double precision, intent(inout) :: v3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: v3_old(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_jacobi_v(v3, v3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, &
+        ex_max), 'do_jacobi_v')
+  end subroutine do_jacobi_v
+
+  subroutine do_jacobi_w(w3, w3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max)
+    integer, intent(in) :: sx, sy, ex, ey, maxn, ex_max
+    double precision, intent(inout) :: w3(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: w3_old(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_jacobi_w(w3, w3_old, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, &
+        ex_max), 'do_jacobi_w')
+  end subroutine do_jacobi_w
+
+  subroutine get_difference(ua, ub, ex, ey, ez, sx, sy, sz, diff)
+    integer, intent(in) :: ex, ey, ez, sx, sy, sz
+    double precision, dimension(sx-1:ex+1, sy-1:ey+1, sz:ez), intent(in) :: ua, ub
+    double precision, intent(out) :: diff
+    call tpls_check(c_difference(ua, ub, ex, ey, ez, sx, sy, sz, diff), 'get_difference')
+  end subroutine get_difference
+end module jacobi_iteration_allflux
+
+
+module sor_iteration_allflux
+  use tpls_core_c
+  implicit none
+contains
+  subroutine do_sor_u(u3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max, flg, &
+      iteration_time)
+    integer, intent(in) :: sx, sy, ex, ey, flg, maxn, iteration_time, ex_max
+    double precision, intent(inout) :: u3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_sor_u(u3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, ex_max, flg, &
+        iteration_time), 'do_sor_u')
+  end subroutine do_sor_u
+
+  subroutine do_sor_v(v3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, iteration_time)
+    integer, intent(in) :: sx, sy, ex, ey, flg, maxn, iteration_time
+    double precision, intent(inout) :: v3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_sor_v(v3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, &
+        iteration_time), 'do_sor_v')
+  end subroutine do_sor_v
+
+  subroutine do_sor_w(w3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, iteration_time)
+    integer, intent(in) :: sx, sy, ex, ey, flg, maxn, iteration_time
+    double precision, intent(inout) :: w3(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: RHS(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: viscosity(sx-2:ex+2, sy-2:ey+2, 0:maxn)
+    double precision, intent(in) :: dx, dy, dz, dt
+    call tpls_check(c_sor_w(w3, RHS, viscosity, dx, dy, dz, dt, ex, ey, sx, sy, maxn, flg, &
+        iteration_time), 'do_sor_w')
+  end subroutine do_sor_w
+end module sor_iteration_allflux
+
+
+module pressure
+  use tpls_core_c
+  implicit none
+contains
+  subroutine get_source_pres(rhs, u3, v3, w3, ex, ey, sx, sy, maxn, dx, dy, dz, dt)
+    integer, intent(in) :: ex, ey, sx, sy, maxn
+    double precision, intent(in) :: dx, dy, dz, dt
+    double precision, intent(inout) :: rhs(sx-1:ex+1, sy-1:ey+1, 0:maxn)
+    double precision, intent(in) :: u3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: v3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(in) :: w3(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    call tpls_check(c_source_pres(rhs, u3, v3, w3, ex, ey, sx, sy, maxn, dx, dy, dz, dt), &
+        'get_source_pres')
+  end subroutine get_source_pres
+
+  subroutine get_uvw_pres(u3, v3, w3, p, ex, ey, sx, sy, maxn, ex_max, dx, dy, dz, dt, u_inlet)
+    integer, intent(in) :: ex, ey, sx, sy, maxn, ex_max
+    double precision, intent(in) :: dx, dy, dz, dt
+    double precision, intent(in) :: p(sx-1:ex+1, sy-1:ey+1, 0:maxn)
+    double precision, intent(inout) :: u3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: v3(sx-1:ex+1, sy-1:ey+1, 0:maxn-2)
+    double precision, intent(inout) :: w3(sx-1:ex+1, sy-1:ey+1, 0:maxn-1)
+    double precision, intent(in) :: u_inlet(0:maxn-2)
+    call tpls_check(c_uvw_pres(u3, v3, w3, p, ex, ey, sx, sy, maxn, ex_max, dx, dy, dz, dt, u_inlet), &
+        'get_uvw_pres')
+  end subroutine get_uvw_pres
+end module pressure
+
+
+! The KSPSolve sequence of main_ns_hybrid.F90:896-913 (initial guess from
+! pres, compute_rhs, compute_matrix, KSPSolve, iteration count, residual norm,
+! converged reason, copy_pressure_out_of_petsc) as one call.  pres holds the
+! initial guess on entry and the solution with its ghost planes on return.
+module tpls_pressure
+  use tpls_core_c
+  implicit none
+contains
+  subroutine tpls_ksp_solve(pres, RHS_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt, rtol, abstol, &
+      dtol, maxits, its, rnorm, reason)
+    integer, intent(in) :: maxl, maxm, maxn, maxits
+    double precision, intent(in) :: dx, dy, dz, dt, rtol, abstol, dtol
+    double precision, intent(inout) :: pres(0:maxl, 0:maxm, 0:maxn)
+    double precision, intent(in) :: RHS_p(0:maxl, 0:maxm, 0:maxn)
+    double precision, intent(in) :: u3(0:maxl, 0:maxm, 0:maxn-2)
+    double precision, intent(in) :: u_inlet(0:maxn-2)
+    integer, intent(out) :: its, reason
+    double precision, intent(out) :: rnorm
+    call tpls_check(c_pressure_solve(pres, RHS_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt, rtol, &
+        abstol, dtol, maxits, its, rnorm, reason), 'tpls_ksp_solve')
+  end subroutine tpls_ksp_solve
+end module tpls_pressure
