@@ -43,6 +43,9 @@ def parse():
                     help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
     ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
     ap.add_argument("--roofline-kernel", default="sor_odd", choices=["sor_odd", "jacobi", "sor_even", "cg_x"])
+    ap.add_argument("--fp32-compare", type=int, default=1,
+                    help="also run the case with FP32 storage over the same steps and report its "
+                         "throughput and error against this run (0 to skip)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="timed steps of the plain-CG comparison run (0 to skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -83,7 +86,7 @@ def cpu_baseline(p, its_gpu: float, threads: int):
 KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x"}
 
 
-def algorithmic_bytes(p, kernel: str) -> float:
+def algorithmic_bytes(p, kernel: str, precision: str = "fp64") -> float:
     """Algorithmic HBM bytes per launch: each distinct array read or written
     once over the interior.  Jacobi: x_old, RHS, viscosity read, x written,
     32 B per cell; SOR half sweep: x, RHS and viscosity read over the colour's
@@ -93,9 +96,10 @@ def algorithmic_bytes(p, kernel: str) -> float:
     component grids."""
     L, P, N = p.maxl, p.maxm - 1, p.maxn
     if kernel == "cg_x":
-        return 40.0 * (L - 1) * P * (N - 1)
+        return 40.0 * (L - 1) * P * (N - 1)          # FP64 in both precisions
     cells = ((L - 2) * P * (N - 1) + (L - 1) * P * (N - 1) + (L - 1) * P * (N - 2)) / 3.0
-    return (32.0 if kernel == "jacobi" else 24.0) * cells
+    per = 32.0 if kernel == "jacobi" else 24.0
+    return per * cells * (0.5 if precision == "fp32" else 1.0)
 
 
 def measured_traffic(kernel: str, nx: int, ny: int, nz: int, precision: str):
@@ -151,6 +155,44 @@ def copy_probe():
                                      "bytes_written_per_launch": n * 8}}), flush=True)
 
 
+COMPARED = ("phi2", "u2", "v2", "w2", "pres")
+
+
+def fp32_run(p, a, ref, ref_volume):
+    """The same case with FP32 storage over the same warm-up and timed steps:
+    throughput and the error of the final state against the FP64 run (max and
+    L2 relative; pressure modulo its mean; relative liquid-volume difference)."""
+    import numpy as np
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    case = Case(p, precision="fp32")
+    case.load(*make_initial(p))
+    for _ in range(a.warmup):
+        case.step(1)
+    case.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        case.step(1)
+        case.sync()
+    el = time.perf_counter() - t0
+    P = p.maxm - 1
+    err = {}
+    for f in COMPARED:
+        g, r = case.field(f)[:, 1:P + 1], ref[f][:, 1:P + 1]
+        if f == "pres":
+            g = g[1:p.maxl, :, 1:p.maxn] - g[1:p.maxl, :, 1:p.maxn].mean()
+            r = r[1:p.maxl, :, 1:p.maxn] - r[1:p.maxl, :, 1:p.maxn].mean()
+        d = g - r
+        err[f] = {"max_rel": float(np.abs(d).max() / np.abs(r).max()),
+                  "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(r))}
+    vol = case.volume()
+    case.close()
+    return {"value": round(p.cells * a.steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
+            "ms_per_step": round(el / a.steps * 1e3, 3), "steps": a.steps, "warmup": a.warmup,
+            "error_vs_fp64_after_steps": a.warmup + a.steps, "error": err,
+            "volume_rel_diff": abs(vol - ref_volume) / ref_volume}
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -189,6 +231,9 @@ def main():
     ms, n = case.kernel_time()
     case.time_kernels(None)
     d = case.diag()
+    keep = a.fp32_compare and a.precision == "fp64"
+    ref = {f: case.field(f) for f in COMPARED} if keep else None
+    ref_volume = case.volume() if keep else None
     case.close()
 
     plain = None
@@ -198,7 +243,7 @@ def main():
     cells = p.cells
     value = cells * a.steps / elapsed * 1e-6
     lat_ms = np.array(lat) * 1e3
-    kbytes = algorithmic_bytes(p, a.roofline_kernel)
+    kbytes = algorithmic_bytes(p, a.roofline_kernel, a.precision)
     avg_ms = ms / max(n, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
     traffic = measured_traffic(a.roofline_kernel, nx, ny, nz, a.precision)
@@ -229,6 +274,8 @@ def main():
     }
     if plain:
         out["plain_cg"] = plain
+    if ref is not None:
+        out["fp32"] = fp32_run(p, a, ref, ref_volume)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)

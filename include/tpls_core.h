@@ -114,6 +114,12 @@ int tpls_case_set_field(tpls_case *c, int field, const double *host);
 int tpls_step(tpls_case *c, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
+/* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
+ * accumulated in FP64, with the smoothed Heaviside of get_viscosity
+ * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this
+ * one measures the volume drift of a run against another (e.g. FP32 storage
+ * against FP64). */
+int tpls_case_volume(tpls_case *c, double *volume);
 /* Wait for all work of the case. */
 int tpls_case_sync(tpls_case *c);
 /* Kernel timing: while a kernel class is selected, every launch of that

@@ -92,6 +92,12 @@ class Case:
         _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
+    def volume(self) -> float:
+        """Liquid volume: sum of H_eps(phi2) dx dy dz over the interior, FP64."""
+        v = C.c_double(0)
+        _lib.check(_lib.lib().tpls_case_volume(self._h, C.byref(v)), "volume")
+        return v.value
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.lib().tpls_case_destroy(self._h)

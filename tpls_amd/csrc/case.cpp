@@ -496,6 +496,20 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
     return TPLS_OK;
 }
 
+int tpls_case_volume(tpls_case* c, double* volume)
+{
+    if (!c || !volume) return fail(TPLS_ERR_ARG, "null argument");
+    const Reduce red = Reduce{c->red_part, c->red_out + 6};
+    if (c->prec == TPLS_FP32)
+        HIPCHK(launch_volume<float>(c->s, c->g, c->ph, c->phi2->view<float>(), red));
+    else
+        HIPCHK(launch_volume<double>(c->s, c->g, c->ph, c->phi2->view<double>(), red));
+    HIPCHK(hipMemcpyAsync(c->h_out + 6, c->red_out + 6, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    *volume = c->h_out[6];
+    return TPLS_OK;
+}
+
 int tpls_step(tpls_case* c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(TPLS_ERR_ARG, "bad argument");

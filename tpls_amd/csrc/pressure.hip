@@ -401,6 +401,33 @@ __global__ void __launch_bounds__(256) k_final1(const double* part, int nb, doub
     if (tid256() == 0) out[0] = tot[0];
 }
 
+// Liquid volume V = sum over interior cells of H_eps(phi) dx dy dz in FP64,
+// with the smoothed Heaviside of the viscosity (momentum_allflux.f90:499-512).
+// The reference has no mass diagnostic; this one measures volume drift.
+template <typename T>
+__global__ void __launch_bounds__(256) k_volume(Grid g, double sw, double dv, FieldView<T> phi, double* part)
+{
+    const int nx = g.maxl - 1, nz = g.maxn - 1;
+    const long n = (long)nx * g.P * nz;
+    double acc[1] = {0.0};
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const int i = int(q % nx) + 1;
+        const long rr = q / nx;
+        const int j = int(rr % g.P) + 1;
+        const int k = int(rr / g.P) + 1;
+        const double pv = double(phi(i, j, k));
+        double H;
+        if (pv < -sw)
+            H = 0.0;
+        else if (pv > sw)
+            H = 1.0;
+        else
+            H = 0.5 + pv / (2.0 * sw) + (1.0 / (2.0 * TPLS_PI_SP)) * sin(TPLS_PI_SP * pv / sw);
+        acc[0] += H * dv;
+    }
+    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+}
+
 // ---- launchers ----
 template <typename T>
 hipError_t launch_source_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3, FieldView<T> v3,
@@ -489,6 +516,15 @@ hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> 
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_volume(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, Reduce red)
+{
+    hipLaunchKernelGGL(k_volume<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, ph.sw, ph.dx * ph.dy * ph.dz, phi,
+                       red.part);
+    hipLaunchKernelGGL(k_final1, dim3(1), dim3(256), 0, s, red.part, REDUCE_BLOCKS, red.out);
+    return hipGetLastError();
+}
+
 // ---- utilities ----
 template <typename T>
 __global__ void k_fill(T* p, long n, T v)
@@ -534,6 +570,7 @@ hipError_t launch_copy(hipStream_t s, T* dst, const T* src, long n)
                                               FieldView<T>, const T*, FieldView<T>);                                \
     template hipError_t launch_levelset_err<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, Reduce);       \
     template hipError_t launch_difference<T>(hipStream_t, const Grid&, int, FieldView<T>, FieldView<T>, Reduce);    \
+    template hipError_t launch_volume<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, Reduce);              \
     template hipError_t launch_fill<T>(hipStream_t, T*, long, T);                                                   \
     template hipError_t launch_copy<T>(hipStream_t, T*, const T*, long);
 TPLS_INST_PR(double)
