@@ -43,9 +43,19 @@ def parse():
                     help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
     ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
     ap.add_argument("--roofline-kernel", default="sor_odd", choices=["sor_odd", "jacobi", "sor_even", "cg_x"])
+    ap.add_argument("--roofline-steps", type=int, default=2, help="steps over which the kernel is timed")
     ap.add_argument("--fp32-compare", type=int, default=1,
                     help="also run the case with FP32 storage over the same steps and report its "
                          "throughput and error against this run (0 to skip)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay each step's launches from HIP graphs (identical results)")
+    ap.add_argument("--batch-cases", type=int, default=64,
+                    help="cases of the batched FP32 run reported beside the single case (0 to skip)")
+    ap.add_argument("--batch-steps", type=int, default=4, help="timed batched steps")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
+    ap.add_argument("--batch-warmup", type=int, default=8,
+                    help="untimed batched steps (the step's buffer roles cycle with period 6, so 7 steps "
+                         "capture every graph)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="timed steps of the plain-CG comparison run (0 to skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -126,7 +136,7 @@ def plain_cg_run(p, a):
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
     q = replace(p, ksp_pc="none")
-    case = Case(q, precision=a.precision)
+    case = Case(q, precision=a.precision, graphs=bool(a.graphs))
     case.load(*make_initial(q))
     case.step(1)
     case.sync()
@@ -165,7 +175,7 @@ def fp32_run(p, a, ref, ref_volume):
     import numpy as np
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
-    case = Case(p, precision="fp32")
+    case = Case(p, precision="fp32", graphs=bool(a.graphs))
     case.load(*make_initial(p))
     for _ in range(a.warmup):
         case.step(1)
@@ -193,6 +203,44 @@ def fp32_run(p, a, ref, ref_volume):
             "volume_rel_diff": abs(vol - ref_volume) / ref_volume}
 
 
+def batched_run(a):
+    """The batched workload: n cases drawn by seed from the batched
+    configuration (grids 64x16x32 .. 256x64x128, uneven sizes), FP32 storage
+    with the FP64 pressure solve, advanced together with tpls_batch_step.
+    Reports the aggregate Mcell-steps/s over all cases and the batch-step
+    latency (every case advances one step per batch step)."""
+    import numpy as np
+    from tpls_amd.batch import batch_configs
+    from tpls_amd.case import Case, batch_step
+    from tpls_amd.initial import make_initial
+    cfgs = batch_configs(a.batch_cases, seed=0)
+    cases = []
+    for p in cfgs:
+        c = Case(p, precision="fp32", diagnostics=False, graphs=bool(a.graphs))
+        c.load(*make_initial(p))
+        cases.append(c)
+    batch_step(cases, a.batch_warmup)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.batch_steps):
+        s0 = time.perf_counter()
+        batch_step(cases, 1)
+        lat.append(time.perf_counter() - s0)
+    el = time.perf_counter() - t0
+    its = [c.diag()["ksp_its"] for c in cases]
+    for c in cases:
+        c.close()
+    cells = sum(p.cells for p in cfgs)
+    lat_ms = np.array(lat) * 1e3
+    return {"value": round(cells * a.batch_steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
+            "cases": len(cfgs), "cells": int(cells), "precision": "fp32 (pressure CG in fp64)",
+            "steps": a.batch_steps, "warmup": a.batch_warmup, "ms_per_batch_step": round(el / a.batch_steps * 1e3, 3),
+            "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "cg_its_mean": round(float(np.mean(its)), 1),
+            "grids": "seeded draw from 64x16x32, 128x32x64, 192x48x96, 256x64x128"}
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -206,17 +254,19 @@ def main():
     from tpls_amd.case import Case
     from tpls_amd.config import CHANNEL_512x128x256, scaled
     from tpls_amd.initial import make_initial
+    if a.batch_only:
+        print(json.dumps({"batched_fp32": batched_run(a)}), flush=True)
+        return
 
     nx, ny, nz = (int(v) for v in a.grid.split("x"))
     p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
     init = make_initial(p)
-    case = Case(p, precision=a.precision)
+    case = Case(p, precision=a.precision, graphs=bool(a.graphs))
     case.load(*init)
     del init
     for _ in range(a.warmup):
         case.step(1)
     case.sync()
-    case.time_kernels(a.roofline_kernel)
     lat = []
     its = []
     t0 = time.perf_counter()
@@ -228,12 +278,16 @@ def main():
         its.append(case.diag()["ksp_its"])
     case.sync()
     elapsed = time.perf_counter() - t0
-    ms, n = case.kernel_time()
-    case.time_kernels(None)
     d = case.diag()
     keep = a.fp32_compare and a.precision == "fp64"
     ref = {f: case.field(f) for f in COMPARED} if keep else None
     ref_volume = case.volume() if keep else None
+    # the dominant kernel, timed with HIP events on the case's stream over
+    # further steps (launched eagerly: events bracket each launch)
+    case.time_kernels(a.roofline_kernel)
+    case.step(a.roofline_steps)
+    ms, n = case.kernel_time()
+    case.time_kernels(None)
     case.close()
 
     plain = None
@@ -266,16 +320,20 @@ def main():
                                f"Re {p.Re:g}, viscosity ratio {p.mu_plus / p.mu_minus:g}, dt {p.dt:g}, "
                                f"CG rtol {p.ksp_rtol:g}",
                    "solver": "cg" if a.pc == "none" else f"cg + multigrid V({a.mg_nu},{a.mg_nu})",
+                   "launch": "hip graphs" if a.graphs else "eager",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
         "roofline": {"kernel": KERNEL_NAMES[a.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
+                     "timed_over_steps": a.roofline_steps,
                      "algorithmic_bytes_per_launch": int(kbytes)},
     }
     if plain:
         out["plain_cg"] = plain
     if ref is not None:
         out["fp32"] = fp32_run(p, a, ref, ref_volume)
+    if a.batch_cases > 0:
+        out["batched_fp32"] = batched_run(a)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)

@@ -63,6 +63,9 @@ typedef struct {
                          one symmetric V(n,n) cycle per iteration (the class
                          of -pc_type mg).  The stopping test is the
                          unpreconditioned residual norm in both cases. */
+    int graphs;       /* 1: replay each step's launches from HIP graphs
+                         (captured on first use per buffer-role pattern);
+                         identical results, far fewer host launches */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -112,6 +115,13 @@ int tpls_case_get_field(tpls_case *c, int field, double *host);
 int tpls_case_set_field(tpls_case *c, int field, const double *host);
 /* Advance nsteps timesteps (main_ns_hybrid.F90:583-918, 2480-2509). */
 int tpls_step(tpls_case *c, int nsteps);
+/* Advance several cases (of any sizes and precisions) by nsteps timesteps
+ * together: every case runs on its own stream and each phase of the step is
+ * enqueued for all cases before the host waits on any, so the GPU overlaps
+ * them.  Each case computes exactly what tpls_step would compute for it
+ * alone.  A batched counterpart of main_ns_hybrid.F90:583-918 for
+ * independent cases; the reference runs one case per job. */
+int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,

@@ -24,22 +24,26 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
     return (maxl + 1, maxm + 1, maxn - 1)
 
 
-def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True) -> _lib.Params:
+def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
+                graphs: bool = False) -> _lib.Params:
     prm = _lib.Params()
     for k, v in p.solver_dict().items():
         setattr(prm, k, v)
     prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
     prm.diagnostics = 1 if diagnostics else 0
+    prm.graphs = 1 if graphs else 0
     return prm
 
 
 class Case:
-    def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True):
+    def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
+                 graphs: bool = False):
+        """graphs: replay each step's launches from HIP graphs (same results)."""
         params.check()
         self.params = params
         self.precision = precision
         self.shape = (params.maxl, params.maxm, params.maxn)
-        self._prm = make_params(params, precision, diagnostics)
+        self._prm = make_params(params, precision, diagnostics, graphs)
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.tpls_case_create(C.byref(self._prm), *self.shape, C.byref(h)), "tpls_case_create")
@@ -114,3 +118,11 @@ class Case:
             self.close()
         except Exception:
             pass
+
+
+def batch_step(cases, nsteps: int = 1) -> None:
+    """Advance several cases together (tpls_batch_step): each case runs on its
+    own stream and the GPU overlaps them; every case computes what
+    Case.step would compute for it alone."""
+    arr = (C.c_void_p * len(cases))(*[c._h for c in cases])
+    _lib.check(_lib.lib().tpls_batch_step(arr, len(cases), int(nsteps)), "tpls_batch_step")

@@ -134,6 +134,23 @@ struct tpls_case {
     tpls_diag diag{};
     long steps_done = 0;
     int last_its = 0;
+    // pressure solve in flight (the step is split into begin / poll / end so
+    // that several cases can be in flight on their own streams)
+    int fin[3] = {0, 0, 0};     // Helmholtz result buffer per component
+    long ps_launched = 0;       // CG iterations enqueued
+    long ps_chunk = 0;          // size of the chunk in flight
+    int ps_cur = 0;             // p double-buffer index
+    hipEvent_t ps_ev = nullptr; // completes when the chunk's state read-back lands
+    bool diag_pending = false;  // residual diagnostics copied back, not yet parsed
+    // HIP graphs (prm.graphs): the launches of a step up to the first CG
+    // chunk, keyed by the buffer roles of the step, and one CG iteration per
+    // p-buffer parity
+    struct PreGraph {
+        std::vector<const void*> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<PreGraph> pre_graphs;
+    hipGraphExec_t it_graph[2] = {nullptr, nullptr};
     // kernel timing: the selected TPLS_TIME_* class
     int timed = TPLS_TIME_OFF;
     std::vector<hipEvent_t> ev_pool;
@@ -219,6 +236,7 @@ void wrap_rows(double* h, const Shape& sh, int maxm)
 template <typename T>
 int step_impl(tpls_case* c);
 int harvest_events(tpls_case* c);
+int finish_diag(tpls_case* c);
 
 DevArray* field_ptr(tpls_case* c, int f)
 {
@@ -366,6 +384,8 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "cg state"));
     if (hipHostMalloc(&c->h_cg, sizeof(CgState)) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
     if (hipHostMalloc(&c->h_out, sizeof(double) * 8) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "pinned"));
+    if (hipEventCreateWithFlags(&c->ps_ev, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(TPLS_ERR_HIP, "event"));
     if (hipMemsetAsync(c->red_out, 0, sizeof(double) * 8, c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "memset"));
     if (hipStreamSynchronize(c->s) != hipSuccess) return bail(fail(TPLS_ERR_HIP, "sync"));
     *out = c;
@@ -386,6 +406,10 @@ int tpls_case_destroy(tpls_case* c)
     if (c->h_cg) hipHostFree(c->h_cg);
     if (c->h_out) hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->ps_ev) hipEventDestroy(c->ps_ev);
+    for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);
+    for (hipGraphExec_t e : c->it_graph)
+        if (e) hipGraphExecDestroy(e);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return TPLS_OK;
@@ -462,6 +486,8 @@ int tpls_case_set_field(tpls_case* c, int f, const double* host)
 int tpls_case_diag(tpls_case* c, tpls_diag* out)
 {
     if (!c || !out) return fail(TPLS_ERR_ARG, "null argument");
+    int rc = finish_diag(c);
+    if (rc) return rc;
     *out = c->diag;
     return TPLS_OK;
 }
@@ -535,6 +561,12 @@ Reduce red_of(tpls_case* c, int slot) { return Reduce{c->red_part, c->red_out + 
         if (e_ != hipSuccess) return fail(TPLS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define RC(x)                 \
+    do {                      \
+        int r_ = (x);         \
+        if (r_) return r_;    \
+    } while (0)
+
 hipEvent_t next_event(tpls_case* c)
 {
     if (c->ev_used == c->ev_pool.size()) {
@@ -559,81 +591,191 @@ int harvest_events(tpls_case* c)
 
 // The pressure solve: KSPSolve replacement (main_ns_hybrid.F90:896-913).
 // Iterations are enqueued in chunks ahead of the convergence test (every
-// kernel of a converged solve returns at once); the host reads the device
-// state back after each chunk.
-template <typename T>
-int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p,
-                   int use_rhs, const T* inlet, FieldView<double> x)
+// kernel of a converged solve returns at once); after each chunk the device
+// state is copied back to pinned memory and ps_ev recorded.  ps_start
+// enqueues the set-up and the first chunk, ps_poll (once ps_ev has completed)
+// either finishes the solve or enqueues the next chunk.  The unknown is pold,
+// the residual cg_r.
+// Record the launches of fn on the case's stream into an executable graph.
+template <typename F>
+int capture(tpls_case* c, hipGraphExec_t* out, F&& fn)
+{
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    int rc = fn();
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->s, &gr);
+    if (!rc && e != hipSuccess) rc = fail(TPLS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (!rc) {
+        e = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(TPLS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (gr) hipGraphDestroy(gr);
+    return rc;
+}
+
+int ps_enqueue_chunk(tpls_case* c)
 {
     const Grid& g = c->g;
-    CgState init{};
-    init.rtol = c->prm.ksp_rtol;
-    init.abstol = c->prm.ksp_abstol;
-    init.dtol = c->prm.ksp_dtol;
-    init.maxits = c->prm.ksp_maxits;
-    *c->h_cg = init;
-    LCHK(hipMemcpyAsync(c->cg, c->h_cg, sizeof(CgState), hipMemcpyHostToDevice, c->s));
-    FieldView<double> r = c->cg_r->view<double>();
     const bool mg = c->use_mg;
-    if (mg) {
-        // coefficients of the reduced operator (pressure_solver.F90:394-421),
-        // a quarter per coarser level
-        const Phys& ph = c->ph;
-        mg_plan(g.maxl - 1, g.P, g.maxn - 1, 1.0 / (ph.dy * ph.dy), 1.0 / (ph.dx * ph.dx), 1.0 / (ph.dz * ph.dz),
-                2.0 * (1.0 / (ph.dx * ph.dx) + 1.0 / (ph.dy * ph.dy) + 1.0 / (ph.dz * ph.dz)), c->mg_coef);
-        for (int l = 0; l < c->mg.nlev; ++l) {
-            c->mg.L[l].cx = c->mg_coef[l].cx;
-            c->mg.L[l].cy = c->mg_coef[l].cy;
-            c->mg.L[l].cz = c->mg_coef[l].cz;
-            c->mg.L[l].diag = c->mg_coef[l].diag;
-        }
-    }
-    LCHK(launch_cg_setup_b<T>(c->s, g, c->ph, u3, v3, w3, inlet, rhs_p, use_rhs, r, red_of(c, 7), c->cg));
-    LCHK(launch_cg_setup_r<T>(c->s, g, c->ph, x, r, red_of(c, 7), c->cg));
-    if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
+    FieldView<double> x = c->pold->view<double>(), r = c->cg_r->view<double>();
     FieldView<double> z = mg ? c->cg_z->view<double>() : r;
     DevArray* pb[2] = {c->cg_p0, c->cg_p1};
-    int cur = 0;
-    long launched = 0;
-    // a multigrid iteration costs tens of launches: enqueue close to the last
-    // solve's count
-    long chunk = mg ? std::max(4, c->last_its) : std::max(8, c->last_its - 2);
-    for (;;) {
-        chunk = std::min<long>(chunk, std::max<long>(1, c->prm.ksp_maxits - launched));
-        for (long q = 0; q < chunk; ++q) {
-            FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
-            LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
-            const bool tx = c->timed == TPLS_TIME_CG_X;
-            if (tx) LCHK(hipEventRecord(next_event(c), c->s));
-            LCHK(launch_cg_x(c->s, g, c->ph, pn, x, r, red_of(c, 7), c->cg));
-            if (tx) LCHK(hipEventRecord(next_event(c), c->s));
-            if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
-            cur ^= 1;
+    long chunk = std::min<long>(c->ps_chunk, std::max<long>(1, c->prm.ksp_maxits - c->ps_launched));
+    c->ps_chunk = chunk;
+    auto iteration = [&](int cur) -> int {
+        FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
+        LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
+        const bool tx = c->timed == TPLS_TIME_CG_X;
+        if (tx) LCHK(hipEventRecord(next_event(c), c->s));
+        LCHK(launch_cg_x(c->s, g, c->ph, pn, x, r, red_of(c, 7), c->cg));
+        if (tx) LCHK(hipEventRecord(next_event(c), c->s));
+        if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
+        return TPLS_OK;
+    };
+    const bool graphs = c->prm.graphs && !c->timed;
+    for (long q = 0; q < chunk; ++q) {
+        if (graphs) {
+            hipGraphExec_t& ex = c->it_graph[c->ps_cur];
+            if (!ex) {
+                const int cur = c->ps_cur;
+                RC(capture(c, &ex, [&]() { return iteration(cur); }));
+            }
+            LCHK(hipGraphLaunch(ex, c->s));
+        } else {
+            RC(iteration(c->ps_cur));
         }
-        launched += chunk;
-        LCHK(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgState), hipMemcpyDeviceToHost, c->s));
-        LCHK(hipStreamSynchronize(c->s));
-        if (c->timed == TPLS_TIME_CG_X) {
-            // drop pairs recorded after convergence (those launches returned at once)
-            long live = std::max<long>(0, c->h_cg->its - (launched - chunk));
-            if (live < chunk) c->ev_used -= size_t(2 * (chunk - live));
-        }
-        if (c->timed) {
-            int rc = harvest_events(c);
-            if (rc) return rc;
-        }
-        if (c->h_cg->done || launched >= c->prm.ksp_maxits) break;
-        chunk = mg ? 2 : 16;
+        c->ps_cur ^= 1;
     }
-    c->diag.ksp_its = c->h_cg->its;
-    c->diag.ksp_reason = c->h_cg->reason;
-    c->diag.ksp_rnorm = c->h_cg->dp;
-    c->last_its = c->h_cg->its;
+    c->ps_launched += chunk;
+    LCHK(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgState), hipMemcpyDeviceToHost, c->s));
+    LCHK(hipEventRecord(c->ps_ev, c->s));
     return TPLS_OK;
 }
 
+// Coefficients of the reduced operator (pressure_solver.F90:394-421) on every
+// multigrid level, a quarter per coarser level.
+void mg_coefficients(tpls_case* c)
+{
+    const Phys& ph = c->ph;
+    const Grid& g = c->g;
+    mg_plan(g.maxl - 1, g.P, g.maxn - 1, 1.0 / (ph.dy * ph.dy), 1.0 / (ph.dx * ph.dx), 1.0 / (ph.dz * ph.dz),
+            2.0 * (1.0 / (ph.dx * ph.dx) + 1.0 / (ph.dy * ph.dy) + 1.0 / (ph.dz * ph.dz)), c->mg_coef);
+    for (int l = 0; l < c->mg.nlev; ++l) {
+        c->mg.L[l].cx = c->mg_coef[l].cx;
+        c->mg.L[l].cy = c->mg_coef[l].cy;
+        c->mg.L[l].cz = c->mg_coef[l].cz;
+        c->mg.L[l].diag = c->mg_coef[l].diag;
+    }
+}
+
+// Launches of the solve's set-up: device state, right-hand side, residual
+// and the first preconditioned residual.  No host state changes (capturable).
 template <typename T>
-int step_impl(tpls_case* c)
+int ps_setup(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p, int use_rhs,
+             const T* inlet)
+{
+    const Grid& g = c->g;
+    FieldView<double> r = c->cg_r->view<double>();
+    LCHK(launch_cg_init(c->s, c->cg, c->prm.ksp_rtol, c->prm.ksp_abstol, c->prm.ksp_dtol, c->prm.ksp_maxits));
+    LCHK(launch_cg_setup_b<T>(c->s, g, c->ph, u3, v3, w3, inlet, rhs_p, use_rhs, r, red_of(c, 7), c->cg));
+    LCHK(launch_cg_setup_r<T>(c->s, g, c->ph, c->pold->view<double>(), r, red_of(c, 7), c->cg));
+    if (c->use_mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
+    return TPLS_OK;
+}
+
+void ps_reset(tpls_case* c)
+{
+    c->ps_cur = 0;
+    c->ps_launched = 0;
+    // a multigrid iteration costs tens of launches: enqueue close to the last
+    // solve's count
+    c->ps_chunk = c->use_mg ? std::max(4, c->last_its) : std::max(8, c->last_its - 2);
+}
+
+template <typename T>
+int ps_start(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p, int use_rhs,
+             const T* inlet)
+{
+    if (c->use_mg) mg_coefficients(c);
+    RC(ps_setup<T>(c, u3, v3, w3, rhs_p, use_rhs, inlet));
+    ps_reset(c);
+    return ps_enqueue_chunk(c);
+}
+
+// After ps_ev has completed: finish the solve (*finished = true) or enqueue
+// the next chunk.
+int ps_poll(tpls_case* c, bool* finished)
+{
+    *finished = false;
+    if (c->timed == TPLS_TIME_CG_X) {
+        // drop pairs recorded after convergence (those launches returned at once)
+        long live = std::max<long>(0, c->h_cg->its - (c->ps_launched - c->ps_chunk));
+        if (live < c->ps_chunk) c->ev_used -= size_t(2 * (c->ps_chunk - live));
+    }
+    if (c->timed) {
+        int rc = harvest_events(c);
+        if (rc) return rc;
+    }
+    if (c->h_cg->done || c->ps_launched >= c->prm.ksp_maxits) {
+        c->diag.ksp_its = c->h_cg->its;
+        c->diag.ksp_reason = c->h_cg->reason;
+        c->diag.ksp_rnorm = c->h_cg->dp;
+        c->last_its = c->h_cg->its;
+        *finished = true;
+        return TPLS_OK;
+    }
+    c->ps_chunk = c->use_mg ? 2 : 16;
+    return ps_enqueue_chunk(c);
+}
+
+template <typename T>
+int pressure_solve(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, FieldView<double> rhs_p,
+                   int use_rhs, const T* inlet)
+{
+    int rc = ps_start<T>(c, u3, v3, w3, rhs_p, use_rhs, inlet);
+    for (bool fin = false; !rc && !fin;) {
+        LCHK(hipEventSynchronize(c->ps_ev));
+        rc = ps_poll(c, &fin);
+    }
+    return rc;
+}
+
+// Buffer roles of one step, a pure function of the case state: the
+// reinitialisation ping-pong leaves its final iterate as phi3 and the old
+// phi3 in the freed slot (two_phase_levelset.f90:100-116); the Helmholtz
+// iterations alternate between the two buffers that are not u2 and end in
+// fin (main_ns_hybrid.F90:758-887).
+struct StepPlan {
+    DevArray *phi3, *pra, *prb;
+    int fin[3];
+};
+
+StepPlan plan_step(const tpls_case* c)
+{
+    StepPlan p{c->phi3, c->pra, c->prb, {0, 0, 0}};
+    const int nit = c->prm.levelset_its;
+    if (nit > 0) {
+        DevArray* last = ((nit - 1) & 1) ? c->prb : c->pra;
+        p.phi3 = last;
+        if (last == c->pra)
+            p.pra = c->phi3;
+        else
+            p.prb = c->phi3;
+    }
+    const int mom[3] = {c->prm.mom_u, c->prm.mom_v, c->prm.mom_w};
+    for (int q = 0; q < 3; ++q) {
+        int others[2], n = 0;
+        for (int b = 0; b < 3; ++b)
+            if (b != c->i2[q]) others[n++] = b;
+        p.fin[q] = ((mom[q] / 2) & 1) ? others[1] : others[0];
+    }
+    return p;
+}
+
+// Every launch of a step up to the pressure solve's set-up (main:585-893),
+// in the buffer roles of pl; changes no host state, so it can be captured.
+template <typename T>
+int enqueue_pre(tpls_case* c, const StepPlan& pl)
 {
     const Grid& g = c->g;
     const Phys& ph = c->ph;
@@ -660,16 +802,9 @@ int step_impl(tpls_case* c)
             if (dg && it == nit) LCHK(launch_levelset_err<T>(s, g, V(src), V(dst), red_of(c, 0)));
             src = dst;
         }
-        if (nit > 0) {
-            // phi3 <- phi_reinit: the final iterate becomes phi3
-            DevArray* old = c->phi3;
-            c->phi3 = src;
-            if (src == c->pra) c->pra = old;
-            else c->prb = old;
-        }
     }
-    // ---- CSF (main:618-627) ----
-    LCHK(launch_csf<T>(s, g, ph, V(c->phi3), V(c->fx), V(c->fy), V(c->fz)));
+    // ---- CSF from the reinitialised phi3 (main:618-627) ----
+    LCHK(launch_csf<T>(s, g, ph, V(pl.phi3), V(c->fx), V(c->fy), V(c->fz)));
     LCHK(launch_csf_bc<T>(s, g, V(c->fx), V(c->fy), V(c->fz)));
     // ---- viscosity incl. inlet/outlet planes (main:646-710) ----
     LCHK(launch_viscosity<T>(s, g, ph, V(c->phi2), V(c->mu), 0, g.maxl));
@@ -681,7 +816,6 @@ int step_impl(tpls_case* c)
                                 V(c->conv[q][1]), V(c->csf[q][0]), V(c->csf[q][1]), V(c->csf[q][2]), V(c->rhs[q])));
     // ---- Helmholtz outer loops (main:758-887) ----
     const int mom[3] = {c->prm.mom_u, c->prm.mom_v, c->prm.mom_w};
-    int fin[3];
     for (int q = 0; q < 3; ++q) {
         int others[2], n = 0;
         for (int b = 0; b < 3; ++b)
@@ -704,7 +838,6 @@ int step_impl(tpls_case* c)
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
             std::swap(srcb, dstb);
         }
-        fin[q] = srcb;  // the last destination
         if (dg) {
             // get_difference(x3, x3_old); at it == 1 x3_old is zero
             const int nz = q == 2 ? g.maxn : g.maxn - 1;
@@ -712,11 +845,65 @@ int step_impl(tpls_case* c)
                 LCHK(launch_difference<T>(s, g, nz, V(c->x[q][srcb]), V(c->x[q][dstb]), red_of(c, 1 + q)));
         }
     }
-    // ---- divergence + pressure solve + projection (main:889-918) ----
+    // ---- divergence and the pressure solve's set-up (main:889-899) ----
+    const int* fin = pl.fin;
+    return ps_setup<T>(c, V(c->x[0][fin[0]]), V(c->x[1][fin[1]]), V(c->x[2][fin[2]]), c->cg_r->view<double>(), 0,
+                       static_cast<const T*>(c->inlet->ptr));
+}
+
+// The step up to the first CG chunk: eager launches, or the replay of a
+// graph captured for the same buffer roles.
+template <typename T>
+int step_begin(tpls_case* c)
+{
+    const StepPlan pl = plan_step(c);
+    if (c->use_mg) mg_coefficients(c);
+    if (c->prm.graphs && !c->timed) {
+        std::vector<const void*> key = {c->phi2, c->phi3, c->pra, c->prb, c->cphi[0], c->cphi[1], c->cphi[2]};
+        for (int q = 0; q < 3; ++q) {
+            key.push_back(c->x[q][c->i2[q]]);
+            for (int b = 0; b < 3; ++b) {
+                key.push_back(c->x[q][b]);
+                key.push_back(c->conv[q][b]);
+                key.push_back(c->csf[q][b]);
+            }
+        }
+        hipGraphExec_t ex = nullptr;
+        for (const auto& pg : c->pre_graphs)
+            if (pg.key == key) ex = pg.exec;
+        if (!ex) {
+            if (c->pre_graphs.size() >= 24) {  // bounded cache; the roles cycle with a short period
+                for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);
+                c->pre_graphs.clear();
+            }
+            RC(capture(c, &ex, [&]() { return enqueue_pre<T>(c, pl); }));
+            c->pre_graphs.push_back({key, ex});
+        }
+        LCHK(hipGraphLaunch(ex, c->s));
+    } else {
+        RC(enqueue_pre<T>(c, pl));
+    }
+    c->phi3 = pl.phi3;
+    c->pra = pl.pra;
+    c->prb = pl.prb;
+    for (int q = 0; q < 3; ++q) c->fin[q] = pl.fin[q];
+    ps_reset(c);
+    return ps_enqueue_chunk(c);
+}
+
+// The rest of the step once the pressure solve has finished.
+template <typename T>
+int step_end(tpls_case* c)
+{
+    const Grid& g = c->g;
+    const Phys& ph = c->ph;
+    hipStream_t s = c->s;
+    const bool dg = c->prm.diagnostics != 0;
+    auto V = [](DevArray* a) { return a->view<T>(); };
+    const int* fin = c->fin;
     FieldView<T> u3 = V(c->x[0][fin[0]]), v3 = V(c->x[1][fin[1]]), w3 = V(c->x[2][fin[2]]);
     const T* inlet = static_cast<const T*>(c->inlet->ptr);
-    int rc = pressure_solve<T>(c, u3, v3, w3, c->cg_r->view<double>(), 0, inlet, c->pold->view<double>());
-    if (rc) return rc;
+    // ---- copy-out and projection (main:914-918) ----
     LCHK(launch_pres_ghosts<T>(s, g, ph, c->pold->view<double>(), u3, inlet, V(c->pres)));
     LCHK(launch_uvw_pres<T>(s, g, ph, V(c->pres), inlet, u3, v3, w3));
     // ---- history rotation (main:2480-2509) ----
@@ -741,14 +928,36 @@ int step_impl(tpls_case* c)
     }
     if (dg) {
         LCHK(hipMemcpyAsync(c->h_out, c->red_out, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
-        LCHK(hipStreamSynchronize(s));
-        c->diag.err_phi = c->h_out[0];
-        c->diag.err_u = c->h_out[1];
-        c->diag.err_v = c->h_out[2];
-        c->diag.err_w = c->h_out[3];
+        c->diag_pending = true;
     }
     c->steps_done += 1;
     return TPLS_OK;
+}
+
+// Parse the residual diagnostics once their copy has landed.
+int finish_diag(tpls_case* c)
+{
+    if (!c->diag_pending) return TPLS_OK;
+    LCHK(hipStreamSynchronize(c->s));
+    c->diag.err_phi = c->h_out[0];
+    c->diag.err_u = c->h_out[1];
+    c->diag.err_v = c->h_out[2];
+    c->diag.err_w = c->h_out[3];
+    c->diag_pending = false;
+    return TPLS_OK;
+}
+
+template <typename T>
+int step_impl(tpls_case* c)
+{
+    int rc = step_begin<T>(c);
+    for (bool fin = false; !rc && !fin;) {
+        LCHK(hipEventSynchronize(c->ps_ev));
+        rc = ps_poll(c, &fin);
+    }
+    if (!rc) rc = step_end<T>(c);
+    if (!rc) rc = finish_diag(c);
+    return rc;
 }
 
 }  // namespace
@@ -863,11 +1072,6 @@ Phys phys_d(double dx, double dy, double dz, double dt)
     return p;
 }
 
-#define RC(x)                 \
-    do {                      \
-        int r_ = (x);         \
-        if (r_) return r_;    \
-    } while (0)
 
 }  // namespace
 
@@ -1280,8 +1484,7 @@ int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, con
     std::vector<double> inl(u_inlet, u_inlet + (maxn - 1));
     RC(put_array(c, c->inlet, inl.data()));
     FieldView<double> u = c->x[0][0]->view<double>();
-    RC(pressure_solve<double>(c, u, u, u, c->cg_p0->view<double>(), 1, static_cast<const double*>(c->inlet->ptr),
-                              c->pold->view<double>()));
+    RC(pressure_solve<double>(c, u, u, u, c->cg_p0->view<double>(), 1, static_cast<const double*>(c->inlet->ptr)));
     HIPCHK(launch_pres_ghosts<double>(c->s, c->g, c->ph, c->pold->view<double>(), u,
                                       static_cast<const double*>(c->inlet->ptr), c->pres->view<double>()));
     RC(get_array(c, c->pres, pres));
@@ -1289,6 +1492,44 @@ int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, con
     *its = c->diag.ksp_its;
     *rnorm = c->diag.ksp_rnorm;
     *reason = c->diag.ksp_reason;
+    return TPLS_OK;
+}
+
+// Several cases advanced together: each case runs on its own stream, and
+// every phase of the step is enqueued for all cases before the host waits on
+// any of them, so the GPU overlaps the cases' kernels.  Each case computes
+// exactly what tpls_step computes for it alone.
+int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
+{
+    if (!cases || ncases < 0 || nsteps < 0) return fail(TPLS_ERR_ARG, "bad argument");
+    for (int q = 0; q < ncases; ++q) {
+        if (!cases[q]) return fail(TPLS_ERR_ARG, "null case");
+        for (int r = 0; r < q; ++r)
+            if (cases[r] == cases[q]) return fail(TPLS_ERR_ARG, "a case appears twice in the batch");
+    }
+    std::vector<char> open(ncases);
+    for (int n = 0; n < nsteps; ++n) {
+        for (int q = 0; q < ncases; ++q) {
+            tpls_case* c = cases[q];
+            RC(c->prec == TPLS_FP32 ? step_begin<float>(c) : step_begin<double>(c));
+            open[q] = 1;
+        }
+        for (int left = ncases; left > 0;) {
+            for (int q = 0; q < ncases; ++q) {
+                if (!open[q]) continue;
+                tpls_case* c = cases[q];
+                HIPCHK(hipEventSynchronize(c->ps_ev));
+                bool fin = false;
+                RC(ps_poll(c, &fin));
+                if (fin) {
+                    RC(c->prec == TPLS_FP32 ? step_end<float>(c) : step_end<double>(c));
+                    open[q] = 0;
+                    --left;
+                }
+            }
+        }
+    }
+    for (int q = 0; q < ncases; ++q) RC(finish_diag(cases[q]));
     return TPLS_OK;
 }
 

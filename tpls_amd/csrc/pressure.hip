@@ -140,6 +140,19 @@ __global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, Tiles tl, double 
     block_partials<1>(acc, part, gridDim.x, blockIdx.x);
 }
 
+// Reset the device state of a solve (a kernel rather than a host copy so that
+// the set-up can be captured in a graph).
+__global__ void k_cg_init(CgState* st, double rtol, double abstol, double dtol, int maxits)
+{
+    if (threadIdx.x != 0) return;
+    CgState s{};
+    s.rtol = rtol;
+    s.abstol = abstol;
+    s.dtol = dtol;
+    s.maxits = maxits;
+    *st = s;
+}
+
 __global__ void __launch_bounds__(256) k_cg_setup_bf(const double* part, int nb, double N, CgState* st)
 {
     double tot[1];
@@ -449,6 +462,12 @@ hipError_t launch_uvw_pres(hipStream_t s, const Grid& g, const Phys& ph, FieldVi
 }
 
 static double n_unknowns(const Grid& g) { return double(g.maxl - 1) * g.P * (g.maxn - 1); }
+
+hipError_t launch_cg_init(hipStream_t s, CgState* st, double rtol, double abstol, double dtol, int maxits)
+{
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(64), 0, s, st, rtol, abstol, dtol, maxits);
+    return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_cg_setup_b(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3, FieldView<T> v3,

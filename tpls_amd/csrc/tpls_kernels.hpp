@@ -103,6 +103,7 @@ struct CgState {
     int its, i, reason, done, maxits, pad;
 };
 // All CG vectors use the p-grid layout and touch interior cells only.
+hipError_t launch_cg_init(hipStream_t s, CgState* st, double rtol, double abstol, double dtol, int maxits);
 template <typename T>
 hipError_t launch_cg_setup_b(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> u3,
                              FieldView<T> v3, FieldView<T> w3, const T* u_inlet,
