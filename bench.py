@@ -28,6 +28,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The batched and serving runs keep one stream per case; more hardware queues
+# let more of them run side by side (HIP's default is 4).  Set before the HIP
+# runtime starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK_GBS = 8000.0
 
@@ -53,6 +57,13 @@ def parse():
                     help="cases of the batched FP32 run reported beside the single case (0 to skip)")
     ap.add_argument("--batch-steps", type=int, default=4, help="timed batched steps")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the serving run reported beside the single case (0 to skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0,
+                    help="Poisson arrival rate of that run, requests/s")
+    ap.add_argument("--serve-steps", default="20,200",
+                    help="step range per request of that run (the full workload, python -m "
+                         "tpls_amd.serve, asks 50-2000 over 10 minutes)")
     ap.add_argument("--batch-warmup", type=int, default=8,
                     help="untimed batched steps (the step's buffer roles cycle with period 6, so 7 steps "
                          "capture every graph)")
@@ -241,6 +252,24 @@ def batched_run(a):
             "grids": "seeded draw from 64x16x32, 128x32x64, 192x48x96, 256x64x128"}
 
 
+def serving_run(a):
+    """Continuous-batching serving: Poisson arrivals of the batched cases plus
+    5% at 512x128x256 over a short window, 200 GB HBM budget, FP32 cases;
+    stopped at three times the window.  The default rate keeps the offered
+    load near the GPU's batched capacity, so latencies are those of a loaded
+    but stable server (the full workload, 20 requests/s of 50-2000 steps,
+    offers about 100 times that capacity)."""
+    from tpls_amd.serve import Server, poisson_requests
+    lo, hi = (int(v) for v in a.serve_steps.split(","))
+    reqs = poisson_requests(a.serve_rate, a.serve_seconds, seed=1, steps=(lo, hi))
+    offered = sum(r.params.cells * r.steps for r in reqs) / a.serve_seconds * 1e-6
+    out = Server(200e9, graphs=bool(a.graphs)).run(reqs, horizon=a.serve_seconds, drain=True,
+                                                   max_wall=3 * a.serve_seconds)
+    out.update({"arrival_rate_per_s": a.serve_rate, "window_s": a.serve_seconds, "steps_per_request": [lo, hi],
+                "offered_mcell_steps_per_s": round(offered, 2), "precision": "fp32 (pressure CG in fp64)"})
+    return out
+
+
 def main():
     a = parse()
     if a.copy_probe:
@@ -334,6 +363,8 @@ def main():
         out["fp32"] = fp32_run(p, a, ref, ref_volume)
     if a.batch_cases > 0:
         out["batched_fp32"] = batched_run(a)
+    if a.serve_seconds > 0:
+        out["serving_fp32"] = serving_run(a)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)

@@ -1,0 +1,198 @@
+"""Serving: a continuous-batching request loop over device-resident cases.
+
+Requests (a channel case plus a number of steps) arrive over time.  Each pass
+of the loop:
+
+1. moves the requests whose arrival time has passed into a FIFO queue;
+2. admits queued requests in order while their device bytes
+   (tpls_case_bytes) fit in the HBM budget; an admitted request's case is
+   created and loaded on the GPU and stays resident until it finishes;
+3. advances every admitted case by one step with one tpls_batch_step, so
+   requests that arrived while others were running join the next batch;
+4. retires the cases that have done their steps: the result is read back and
+   the case's memory goes back to the budget.
+
+Every case computes exactly what it computes alone (the batch only overlaps
+independent cases), which tests/test_gpu_serve.py checks.  Reported: per-step
+latency (the batch step a case advanced in), per-request latency (arrival to
+retirement), queueing delay and aggregate Mcell-steps/s.
+
+Usage: python -m tpls_amd.serve --minutes 10 (the full serving workload:
+Poisson arrivals at 20/s of the batched-case distribution plus 5% at
+512x128x256, 50-2000 steps, 200 GB budget).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .batch import batch_configs, stable_dt
+from .config import CHANNEL_512x128x256, CaseParams
+
+
+@dataclass
+class Request:
+    rid: int
+    params: CaseParams
+    steps: int
+    arrival: float                 # seconds after the start of the run
+    precision: str = "fp32"
+    keep: tuple = ()               # fields to return when finished
+    # filled in by the server
+    admitted: float = -1.0
+    finished: float = -1.0
+    step_ms: list = field(default_factory=list)
+    result: dict = field(default_factory=dict)
+    its: list = field(default_factory=list)
+
+
+class Server:
+    def __init__(self, hbm_budget: float, graphs: bool = True, max_batch: int = 0):
+        self.budget = float(hbm_budget)
+        self.graphs = graphs
+        self.max_batch = max_batch
+        self.used = 0.0
+        self.peak_used = 0.0
+        self.active = []           # (request, case, remaining, bytes)
+        self.done = []
+        self.batch_sizes = []
+        self.step_samples = []     # (batch-step ms, cases that advanced in it)
+        self.cell_steps = 0
+
+    def _bytes(self, r: Request) -> int:
+        from .case import Case
+        return Case.device_bytes(r.params, r.precision)
+
+    def _admit(self, r: Request, now: float) -> None:
+        from .case import Case
+        from .initial import make_initial
+        nb = self._bytes(r)
+        c = Case(r.params, precision=r.precision, diagnostics=False, graphs=self.graphs)
+        c.load(*make_initial(r.params))
+        self.active.append([r, c, r.steps, nb])
+        self.used += nb
+        self.peak_used = max(self.peak_used, self.used)
+        r.admitted = now
+
+    def _retire(self, entry, now: float) -> None:
+        r, c, _, nb = entry
+        r.result = {f: c.field(f) for f in r.keep}
+        r.finished = now
+        c.close()
+        self.used -= nb
+        self.done.append(r)
+
+    def run(self, requests: list[Request], horizon: float = float("inf"), drain: bool = True,
+            max_wall: float = float("inf")) -> dict:
+        """Serve the requests (sorted by arrival).  Arrivals stop at horizon;
+        with drain the loop runs until every admitted request has finished,
+        otherwise it stops at horizon; it never runs past max_wall seconds."""
+        from .case import batch_step
+        pending = sorted(requests, key=lambda r: r.arrival)
+        queue = []
+        t0 = time.perf_counter()
+        nxt = 0
+        while True:
+            now = time.perf_counter() - t0
+            while nxt < len(pending) and pending[nxt].arrival <= now and pending[nxt].arrival <= horizon:
+                queue.append(pending[nxt])
+                nxt += 1
+            while queue and (not self.max_batch or len(self.active) < self.max_batch):
+                if self.used + self._bytes(queue[0]) > self.budget:
+                    if not self.active:
+                        raise ValueError(f"request {queue[0].rid} does not fit the HBM budget on its own")
+                    break
+                self._admit(queue.pop(0), now)
+            if not self.active:
+                more = nxt < len(pending) and pending[nxt].arrival <= horizon
+                if not more or (now > horizon and not drain):
+                    break
+                time.sleep(max(0.0, min(pending[nxt].arrival - now, 0.05)))
+                continue
+            if (now > horizon and not drain) or now > max_wall:
+                break
+            cases = [e[1] for e in self.active]
+            s0 = time.perf_counter()
+            batch_step(cases, 1)
+            ms = (time.perf_counter() - s0) * 1e3
+            self.batch_sizes.append(len(cases))
+            self.step_samples.append((ms, len(cases)))
+            now = time.perf_counter() - t0
+            keep = []
+            for e in self.active:
+                r = e[0]
+                r.step_ms.append(ms)
+                r.its.append(e[1].diag()["ksp_its"])
+                self.cell_steps += r.params.cells
+                e[2] -= 1
+                if e[2] == 0:
+                    self._retire(e, now)
+                else:
+                    keep.append(e)
+            self.active = keep
+        wall = time.perf_counter() - t0
+        for e in self.active:          # stopped at the horizon: release the rest
+            e[1].close()
+        return self.report(wall, len(requests), nxt, len(queue))
+
+    def report(self, wall: float, submitted: int, arrived: int, queued: int) -> dict:
+        # per-step latency over every case-step (a batch step counts once per
+        # case that advanced in it)
+        step = (np.repeat([m for m, _ in self.step_samples], [n for _, n in self.step_samples])
+                if self.step_samples else np.array([0.0]))
+        req = np.array([(r.finished - r.arrival) * 1e3 for r in self.done]) if self.done else np.array([0.0])
+        wait = np.array([(r.admitted - r.arrival) * 1e3 for r in self.done]) if self.done else np.array([0.0])
+        pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+        return {"value": round(self.cell_steps / wall * 1e-6, 4), "unit": "Mcell-steps/s",
+                "wall_s": round(wall, 3), "requests_submitted": submitted, "requests_arrived": arrived,
+                "requests_finished": len(self.done), "requests_unfinished": len(self.active) + queued,
+                "step_ms_p50": pct(step, 50), "step_ms_p99": pct(step, 99),
+                "request_ms_p50": pct(req, 50), "request_ms_p99": pct(req, 99),
+                "queue_ms_p50": pct(wait, 50), "queue_ms_p99": pct(wait, 99),
+                "batch_size_mean": round(float(np.mean(self.batch_sizes)), 2) if self.batch_sizes else 0,
+                "batch_size_max": int(max(self.batch_sizes)) if self.batch_sizes else 0,
+                "hbm_budget_gb": round(self.budget / 1e9, 3), "hbm_peak_gb": round(self.peak_used / 1e9, 3)}
+
+
+def poisson_requests(rate: float, duration: float, seed: int = 0, big_fraction: float = 0.05,
+                     steps=(50, 2000), grids=None) -> list[Request]:
+    """Poisson arrivals over [0, duration): cases from the batched-case
+    distribution, plus big_fraction at 512x128x256; steps uniform in the range."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    kw = {"grids": grids} if grids else {}
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            break
+        if rng.random() < big_fraction:
+            p = replace(CHANNEL_512x128x256, ksp_pc="mg", seed=int(rng.integers(1 << 30)))
+            p = replace(p, dt=min(p.dt, stable_dt(p.nz, p.Re, p.mu_plus / p.mu_minus)))
+        else:
+            p = batch_configs(1, seed=int(rng.integers(1 << 30)), **kw)[0]
+        out.append(Request(rid=len(out), params=p, steps=int(rng.integers(steps[0], steps[1] + 1)), arrival=t))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--minutes", type=float, default=10.0)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--min-steps", type=int, default=50)
+    ap.add_argument("--max-steps", type=int, default=2000)
+    ap.add_argument("--no-drain", action="store_true", help="stop at the end of the arrival window")
+    a = ap.parse_args()
+    reqs = poisson_requests(a.rate, a.minutes * 60, a.seed, steps=(a.min_steps, a.max_steps))
+    srv = Server(a.budget_gb * 1e9)
+    out = srv.run(reqs, horizon=a.minutes * 60, drain=not a.no_drain)
+    print(json.dumps({"serving": out}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
