@@ -53,6 +53,8 @@ def parse():
                          "throughput and error against this run (0 to skip)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay each step's launches from HIP graphs (identical results)")
+    ap.add_argument("--tiles", default="auto",
+                    help="launch shapes of the Helmholtz sweeps: auto (the autotuner's table) or default")
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases of the batched FP32 run reported beside the single case (0 to skip)")
     ap.add_argument("--batch-steps", type=int, default=4, help="timed batched steps")
@@ -147,7 +149,7 @@ def plain_cg_run(p, a):
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
     q = replace(p, ksp_pc="none")
-    case = Case(q, precision=a.precision, graphs=bool(a.graphs))
+    case = Case(q, precision=a.precision, graphs=bool(a.graphs), tiles=tiles_of(a))
     case.load(*make_initial(q))
     case.step(1)
     case.sync()
@@ -179,6 +181,10 @@ def copy_probe():
 COMPARED = ("phi2", "u2", "v2", "w2", "pres")
 
 
+def tiles_of(a):
+    return "auto" if a.tiles == "auto" else None
+
+
 def fp32_run(p, a, ref, ref_volume):
     """The same case with FP32 storage over the same warm-up and timed steps:
     throughput and the error of the final state against the FP64 run (max and
@@ -186,7 +192,7 @@ def fp32_run(p, a, ref, ref_volume):
     import numpy as np
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
-    case = Case(p, precision="fp32", graphs=bool(a.graphs))
+    case = Case(p, precision="fp32", graphs=bool(a.graphs), tiles=tiles_of(a))
     case.load(*make_initial(p))
     for _ in range(a.warmup):
         case.step(1)
@@ -227,7 +233,7 @@ def batched_run(a):
     cfgs = batch_configs(a.batch_cases, seed=0)
     cases = []
     for p in cfgs:
-        c = Case(p, precision="fp32", diagnostics=False, graphs=bool(a.graphs))
+        c = Case(p, precision="fp32", diagnostics=False, graphs=bool(a.graphs), tiles=tiles_of(a))
         c.load(*make_initial(p))
         cases.append(c)
     batch_step(cases, a.batch_warmup)
@@ -290,7 +296,7 @@ def main():
     nx, ny, nz = (int(v) for v in a.grid.split("x"))
     p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
     init = make_initial(p)
-    case = Case(p, precision=a.precision, graphs=bool(a.graphs))
+    case = Case(p, precision=a.precision, graphs=bool(a.graphs), tiles=tiles_of(a))
     case.load(*init)
     del init
     for _ in range(a.warmup):
@@ -350,6 +356,7 @@ def main():
                                f"CG rtol {p.ksp_rtol:g}",
                    "solver": "cg" if a.pc == "none" else f"cg + multigrid V({a.mg_nu},{a.mg_nu})",
                    "launch": "hip graphs" if a.graphs else "eager",
+                   "tiles": "autotuned" if a.tiles == "auto" else "default",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
         "roofline": {"kernel": KERNEL_NAMES[a.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

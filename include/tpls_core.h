@@ -66,6 +66,11 @@ typedef struct {
     int graphs;       /* 1: replay each step's launches from HIP graphs
                          (captured on first use per buffer-role pattern);
                          identical results, far fewer host launches */
+    int tile_rows;    /* launch shape of the Helmholtz sweeps (0 = default):
+                         rows per 64-wide tile of the Jacobi and even-parity
+                         SOR sweeps, 1/2/4/8 */
+    int tile_rows_odd;  /* the same for the odd-parity (column) SOR sweep */
+    int prefetch_odd;   /* levels the column sweep loads ahead, 4 or 8 */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -124,6 +129,10 @@ int tpls_step(tpls_case *c, int nsteps);
 int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
+/* Change the launch shapes of the Helmholtz sweeps (the tile_rows,
+ * tile_rows_odd and prefetch_odd parameters) of an existing case; used by the
+ * autotuner.  Results do not depend on them. */
+int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
  * accumulated in FP64, with the smoothed Heaviside of get_viscosity
  * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this

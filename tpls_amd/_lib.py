@@ -27,7 +27,8 @@ class Params(C.Structure):
                [(n, C.c_int) for n in ("mom_u", "mom_v", "mom_w", "levelset_its")] + \
                [(n, C.c_double) for n in ("ksp_rtol", "ksp_abstol", "ksp_dtol")] + \
                [("ksp_maxits", C.c_int), ("precision", C.c_int), ("diagnostics", C.c_int),
-                ("ksp_pc", C.c_int), ("graphs", C.c_int)]
+                ("ksp_pc", C.c_int), ("graphs", C.c_int), ("tile_rows", C.c_int),
+                ("tile_rows_odd", C.c_int), ("prefetch_odd", C.c_int)]
 
 
 class Diag(C.Structure):

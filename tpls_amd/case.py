@@ -25,25 +25,34 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
 
 
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                graphs: bool = False) -> _lib.Params:
+                graphs: bool = False, tiles=None) -> _lib.Params:
     prm = _lib.Params()
     for k, v in p.solver_dict().items():
         setattr(prm, k, v)
     prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
     prm.diagnostics = 1 if diagnostics else 0
     prm.graphs = 1 if graphs else 0
+    if tiles:
+        prm.tile_rows, prm.tile_rows_odd, prm.prefetch_odd = (int(v) for v in tiles)
     return prm
 
 
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                 graphs: bool = False):
-        """graphs: replay each step's launches from HIP graphs (same results)."""
+                 graphs: bool = False, tiles=None):
+        """graphs: replay each step's launches from HIP graphs (same results).
+        tiles: (rows, rows_odd, prefetch_odd) launch shapes of the Helmholtz
+        sweeps, "auto" for the tuned shapes of this grid (tpls_amd.autotune),
+        or None for the defaults; every choice computes the same values."""
+        if tiles == "auto":
+            from .autotune import tuned
+            tiles = tuned(params, precision)
         params.check()
         self.params = params
         self.precision = precision
         self.shape = (params.maxl, params.maxm, params.maxn)
-        self._prm = make_params(params, precision, diagnostics, graphs)
+        self.tiles = tiles
+        self._prm = make_params(params, precision, diagnostics, graphs, tiles)
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.tpls_case_create(C.byref(self._prm), *self.shape, C.byref(h)), "tpls_case_create")
@@ -95,6 +104,10 @@ class Case:
         ms, n = C.c_double(0), C.c_long(0)
         _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int) -> None:
+        _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd), "set_tiling")
+        self.tiles = (rows, rows_odd, prefetch_odd)
 
     def volume(self) -> float:
         """Liquid volume: sum of H_eps(phi2) dx dy dz over the interior, FP64."""

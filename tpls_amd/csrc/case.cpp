@@ -124,6 +124,7 @@ struct tpls_case {
     DevArray *cg_z = nullptr, *cg_t = nullptr;
     MgHierarchy mg{};
     MgLevel mg_coef[MG_MAXLEV];
+    Tiling tiling;              // launch shapes of the Helmholtz sweeps
     bool use_mg = false;
     DevArray* inlet;
     double* red_part = nullptr;
@@ -307,6 +308,11 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->mom_u < 2 || prm->mom_v < 2 || prm->mom_w < 2) return fail(TPLS_ERR_ARG, "mom_* must be >= 2");
     if (prm->precision != TPLS_FP64 && prm->precision != TPLS_FP32) return fail(TPLS_ERR_ARG, "bad precision");
     if (prm->ksp_pc < 0 || prm->ksp_pc > 8) return fail(TPLS_ERR_ARG, "ksp_pc must be 0 (none) or 1..8 (multigrid sweeps)");
+    auto rows_ok = [](int r) { return r == 0 || r == 1 || r == 2 || r == 4 || r == 8; };
+    if (!rows_ok(prm->tile_rows) || !rows_ok(prm->tile_rows_odd))
+        return fail(TPLS_ERR_ARG, "tile rows must be 0 (default), 1, 2, 4 or 8");
+    if (prm->prefetch_odd != 0 && prm->prefetch_odd != 4 && prm->prefetch_odd != 8)
+        return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
         const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
@@ -327,6 +333,9 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     c->esz = c->prec == TPLS_FP32 ? 4 : 8;
     c->g = Grid{maxl, maxm, maxn, maxm - 1};
     c->ph = phys_of(*prm);
+    if (prm->tile_rows) c->tiling.rows = prm->tile_rows;
+    if (prm->tile_rows_odd) c->tiling.rows_odd = prm->tile_rows_odd;
+    if (prm->prefetch_odd) c->tiling.depth_odd = prm->prefetch_odd;
     int rc = TPLS_OK;
     auto bail = [&](int r) {
         tpls_case_destroy(c);
@@ -519,6 +528,21 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
     if (rc) return rc;
     *ms = c->t_ms;
     *launches = c->t_n;
+    return TPLS_OK;
+}
+
+int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd)
+{
+    if (!c) return fail(TPLS_ERR_ARG, "null case");
+    auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
+    if (!rows_ok(rows) || !rows_ok(rows_odd) || (prefetch_odd != 4 && prefetch_odd != 8))
+        return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8 and prefetch_odd 4 or 8");
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->tiling.rows = rows;
+    c->tiling.rows_odd = rows_odd;
+    c->tiling.depth_odd = prefetch_odd;
+    for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);  // captured with the old shapes
+    c->pre_graphs.clear();
     return TPLS_OK;
 }
 
@@ -827,12 +851,12 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
             const bool tj = c->timed == TPLS_TIME_JACOBI;
             const bool ts = c->timed == (it % 2 ? TPLS_TIME_SOR_ODD : TPLS_TIME_SOR_EVEN);
             if (tj) LCHK(hipEventRecord(next_event(c), s));
-            LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xd, it == 1));
+            LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xd, it == 1, c->tiling));
             if (tj) LCHK(hipEventRecord(next_event(c), s));
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
             for (int flg = 0; flg < 2; ++flg) {
                 if (ts) LCHK(hipEventRecord(next_event(c), s));
-                LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), flg, it));
+                LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), flg, it, c->tiling));
                 if (ts) LCHK(hipEventRecord(next_event(c), s));
             }
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));

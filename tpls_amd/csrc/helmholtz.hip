@@ -6,6 +6,8 @@
 // maxl-1 is a boundary condition), k in 0..maxn-2 with the wall ghost
 // -2 x0 + x1/3; v on i in 1..maxl-1, same k; w on i in 1..maxl-1,
 // k in 1..maxn-2 between the walls w = 0.
+#include <type_traits>
+
 #include "tpls_kernels.hpp"
 
 namespace tpls {
@@ -93,26 +95,25 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk)
     return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
-// One cell per thread over tiles of 64 x 4 columns, one level per tile.
-template <int COMP>
-__device__ __forceinline__ bool cell_of(const Grid& g, int nbx, int nby, int& i, int& j, int& k, int ihalf)
+// One cell per thread over tiles of 64 x RY columns, one level per tile.
+template <int COMP, int RY>
+__device__ __forceinline__ bool cell_of(const Grid& g, int nbx, int nby, int& i, int& j, int& k)
 {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
     i = bx * BX + threadIdx.x;
-    j = by * BY + threadIdx.y + 1;
+    j = by * RY + threadIdx.y + 1;
     k = bz + kmin_of<COMP>();
-    (void)ihalf;
     return j <= g.P;
 }
 
-template <int COMP, typename T>
-__global__ void __launch_bounds__(BX* BY) k_jacobi(Grid g, int nbx, int nby, T ax2, T ay2, T az2, FieldView<T> xo,
+template <int COMP, int RY, typename T>
+__global__ void __launch_bounds__(BX* RY) k_jacobi(Grid g, int nbx, int nby, T ax2, T ay2, T az2, FieldView<T> xo,
                                                    FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, int xo_zero)
 {
     using C = T;
     int i, j, k;
-    if (!cell_of<COMP>(g, nbx, nby, i, j, k, 0)) return;
+    if (!cell_of<COMP, RY>(g, nbx, nby, i, j, k)) return;
     i += 1;
     if (i > imax_of<COMP>(g.maxl)) return;
     MuFaces<C> m = mu_cook<COMP, C>(mu_raw<COMP, C>(mu, i, j, k, g.P));
@@ -130,13 +131,13 @@ __global__ void __launch_bounds__(BX* BY) k_jacobi(Grid g, int nbx, int nby, T a
 // Even iteration_time: 3-D colouring (i+j+k) = 1+flg (mod 2); every cell of a
 // colour reads only the other colour, so the sweep is fully parallel.  One
 // thread per cell of the colour.
-template <int COMP, typename T>
-__global__ void __launch_bounds__(BX* BY) k_sor3d(Grid g, int nbx, int nby, int flg, T ax2, T ay2, T az2,
+template <int COMP, int RY, typename T>
+__global__ void __launch_bounds__(BX* RY) k_sor3d(Grid g, int nbx, int nby, int flg, T ax2, T ay2, T az2,
                                                   FieldView<T> x, FieldView<T> rhs, FieldView<T> mu)
 {
     using C = T;
     int t, j, k;
-    if (!cell_of<COMP>(g, nbx, nby, t, j, k, 1)) return;
+    if (!cell_of<COMP, RY>(g, nbx, nby, t, j, k)) return;
     const int i = ((k + j + flg) & 1) + 1 + 2 * t;
     if (i > imax_of<COMP>(g.maxl)) return;
     MuFaces<C> m = mu_cook<COMP, C>(mu_raw<COMP, C>(mu, i, j, k, g.P));
@@ -179,12 +180,13 @@ __device__ __forceinline__ void sor_load(SorLevel<C>& L, const FieldView<T>& x, 
     L.up = (k < k1 || COMP == 2) ? C(x(i, j, k + 1)) : C(0);
 }
 
-template <int COMP, typename T>
-__global__ void __launch_bounds__(BX* BY) k_sor2d(Grid g, int flg, T ax2, T ay2, T az2, FieldView<T> x,
+template <int COMP, int RY, int D, typename T>
+__global__ void __launch_bounds__(BX* RY) k_sor2d(Grid g, int flg, T ax2, T ay2, T az2, FieldView<T> x,
                                                   FieldView<T> rhs, FieldView<T> mu)
 {
     using C = T;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    constexpr int SOR_D = D;
+    const int j = blockIdx.y * RY + threadIdx.y + 1;
     const int i = ((j + flg) & 1) + 1 + 2 * (blockIdx.x * BX + threadIdx.x);
     if (i > imax_of<COMP>(g.maxl) || j > g.P) return;
     const int P = g.P, maxn = g.maxn;
@@ -263,50 +265,75 @@ __global__ void k_bc_w(Grid g, FieldView<T> w)
 
 static inline int nz_of(int comp, const Grid& g) { return comp == 2 ? g.maxn - 2 : g.maxn - 1; }
 
+// Launch shapes selectable at run time (tile rows 1, 2, 4, 8; odd-sweep
+// prefetch depth 4 or 8): the autotuner picks among them per grid.
+template <int COMP, typename T, typename F>
+static hipError_t with_rows(int rows, F&& f)
+{
+    switch (rows) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    default: return f(std::integral_constant<int, 4>{});
+    }
+}
+
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold, FieldView<T> rhs,
-                         FieldView<T> mu, FieldView<T> x, bool xold_is_zero)
+                         FieldView<T> mu, FieldView<T> x, bool xold_is_zero, const Tiling& tl)
 {
     const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
-    const int nbx = (g.maxl - 1 + BX - 1) / BX, nby = (g.P + BY - 1) / BY;
-    const dim3 grd(nbx * nby * nz_of(comp, g));
     const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
     const int z = xold_is_zero ? 1 : 0;
-    if (comp == 0)
-        hipLaunchKernelGGL((k_jacobi<0, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
-    else if (comp == 1)
-        hipLaunchKernelGGL((k_jacobi<1, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
-    else
-        hipLaunchKernelGGL((k_jacobi<2, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs, mu, x, z);
-    return hipGetLastError();
+    auto go = [&](auto comp_c) {
+        constexpr int CP = decltype(comp_c)::value;
+        return with_rows<CP, T>(tl.rows, [&](auto ry) {
+            constexpr int RY = decltype(ry)::value;
+            const int nbx = (g.maxl - 1 + BX - 1) / BX, nby = (g.P + RY - 1) / RY;
+            const dim3 grd(nbx * nby * nz_of(CP, g));
+            hipLaunchKernelGGL((k_jacobi<CP, RY, T>), grd, dim3(BX, RY), 0, s, g, nbx, nby, a2, b2, c2, xold, rhs,
+                               mu, x, z);
+            return hipGetLastError();
+        });
+    };
+    if (comp == 0) return go(std::integral_constant<int, 0>{});
+    if (comp == 1) return go(std::integral_constant<int, 1>{});
+    return go(std::integral_constant<int, 2>{});
 }
 
 template <typename T>
 hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> x, FieldView<T> rhs,
-                      FieldView<T> mu, int flg, int iteration_time)
+                      FieldView<T> mu, int flg, int iteration_time, const Tiling& tl)
 {
     const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
     const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
     const int half = (g.maxl - 1 + 1) / 2;
-    const int nbx = (half + BX - 1) / BX, nby = (g.P + BY - 1) / BY;
-    if (iteration_time % 2 == 0) {
-        const dim3 grd(nbx * nby * nz_of(comp, g));
-        if (comp == 0)
-            hipLaunchKernelGGL((k_sor3d<0, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
-        else if (comp == 1)
-            hipLaunchKernelGGL((k_sor3d<1, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
-        else
-            hipLaunchKernelGGL((k_sor3d<2, T>), grd, dim3(BX, BY), 0, s, g, nbx, nby, flg, a2, b2, c2, x, rhs, mu);
-    } else {
-        const dim3 grd(nbx, nby, 1);
-        if (comp == 0)
-            hipLaunchKernelGGL((k_sor2d<0, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
-        else if (comp == 1)
-            hipLaunchKernelGGL((k_sor2d<1, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
-        else
-            hipLaunchKernelGGL((k_sor2d<2, T>), grd, dim3(BX, BY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
-    }
-    return hipGetLastError();
+    const int nbx = (half + BX - 1) / BX;
+    auto go = [&](auto comp_c) {
+        constexpr int CP = decltype(comp_c)::value;
+        if (iteration_time % 2 == 0) {
+            return with_rows<CP, T>(tl.rows, [&](auto ry) {
+                constexpr int RY = decltype(ry)::value;
+                const int nby = (g.P + RY - 1) / RY;
+                const dim3 grd(nbx * nby * nz_of(CP, g));
+                hipLaunchKernelGGL((k_sor3d<CP, RY, T>), grd, dim3(BX, RY), 0, s, g, nbx, nby, flg, a2, b2, c2, x,
+                                   rhs, mu);
+                return hipGetLastError();
+            });
+        }
+        return with_rows<CP, T>(tl.rows_odd, [&](auto ry) {
+            constexpr int RY = decltype(ry)::value;
+            const dim3 grd(nbx, (g.P + RY - 1) / RY, 1);
+            if (tl.depth_odd == 8)
+                hipLaunchKernelGGL((k_sor2d<CP, RY, 8, T>), grd, dim3(BX, RY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
+            else
+                hipLaunchKernelGGL((k_sor2d<CP, RY, 4, T>), grd, dim3(BX, RY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
+            return hipGetLastError();
+        });
+    };
+    if (comp == 0) return go(std::integral_constant<int, 0>{});
+    if (comp == 1) return go(std::integral_constant<int, 1>{});
+    return go(std::integral_constant<int, 2>{});
 }
 
 template <typename T>
@@ -325,9 +352,9 @@ hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int
 
 #define TPLS_INST_HH(T)                                                                                              \
     template hipError_t launch_jacobi<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,     \
-                                         FieldView<T>, FieldView<T>, bool);                                          \
+                                         FieldView<T>, FieldView<T>, bool, const Tiling&);                           \
     template hipError_t launch_sor<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,        \
-                                      FieldView<T>, int, int);                                                       \
+                                      FieldView<T>, int, int, const Tiling&);                                        \
     template hipError_t launch_helmholtz_bc<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>);
 TPLS_INST_HH(double)
 TPLS_INST_HH(float)

@@ -67,12 +67,20 @@ hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int com
                            FieldView<T> csf1, FieldView<T> csf2, FieldView<T> rhs);
 
 // ---- Helmholtz sweeps (jacobi_iteration_allflux.f90, sor_iteration_allflux.f90) ----
+// Launch shapes of the sweeps: rows per 64-wide tile of the Jacobi and
+// even-parity SOR sweeps and of the odd-parity (column) SOR sweep, and the
+// odd sweep's prefetch depth.  Every choice computes the same values.
+struct Tiling {
+    int rows = 4, rows_odd = 4, depth_odd = 4;
+};
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,
-                         FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, bool xold_is_zero);
+                         FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, bool xold_is_zero,
+                         const Tiling& tl = Tiling{});
 template <typename T>
 hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> x,
-                      FieldView<T> rhs, FieldView<T> mu, int flg, int iteration_time);
+                      FieldView<T> rhs, FieldView<T> mu, int flg, int iteration_time,
+                      const Tiling& tl = Tiling{});
 // outer-loop boundary conditions (main_ns_hybrid.F90:764-776, 826-836,
 // 864-879; the same statements close do_jacobi_v / do_jacobi_w)
 template <typename T>

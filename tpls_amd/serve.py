@@ -51,8 +51,9 @@ class Request:
 
 
 class Server:
-    def __init__(self, hbm_budget: float, graphs: bool = True, max_batch: int = 0):
+    def __init__(self, hbm_budget: float, graphs: bool = True, max_batch: int = 0, tiles="auto"):
         self.budget = float(hbm_budget)
+        self.tiles = tiles
         self.graphs = graphs
         self.max_batch = max_batch
         self.used = 0.0
@@ -71,7 +72,7 @@ class Server:
         from .case import Case
         from .initial import make_initial
         nb = self._bytes(r)
-        c = Case(r.params, precision=r.precision, diagnostics=False, graphs=self.graphs)
+        c = Case(r.params, precision=r.precision, diagnostics=False, graphs=self.graphs, tiles=self.tiles)
         c.load(*make_initial(r.params))
         self.active.append([r, c, r.steps, nb])
         self.used += nb
