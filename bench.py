@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 # The batched and serving runs keep one stream per case; more hardware queues
 # let more of them run side by side (HIP's default is 4).  Set before the HIP
 # runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 

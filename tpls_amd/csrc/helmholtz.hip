@@ -8,6 +8,7 @@
 // k in 1..maxn-2 between the walls w = 0.
 #include <type_traits>
 
+#include "tiles.hpp"
 #include "tpls_kernels.hpp"
 
 namespace tpls {
@@ -84,15 +85,6 @@ __device__ __forceinline__ void z_nbrs(const FieldView<T>& x, int i, int j, int 
     }
     zm = (k == 0) ? C(-2) * C(x(i, j, 0)) + C(x(i, j, 1)) / C(3) : C(x(i, j, k - 1));
     zp = (k == maxn - 2) ? C(-2) * C(x(i, j, maxn - 2)) + C(x(i, j, maxn - 3)) / C(3) : C(x(i, j, k + 1));
-}
-
-// Blocks of a 1-D launch are dealt to the 8 XCDs round-robin; map them so
-// that each XCD walks a contiguous range of tiles (neighbouring planes then
-// share that XCD's L2).  A bijection on [0, nblk).
-__device__ __forceinline__ int xcd_tile(int b, int nblk)
-{
-    const int per = nblk >> 3;
-    return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
 // One cell per thread over tiles of 64 x RY columns, one level per tile.
@@ -186,8 +178,13 @@ __global__ void __launch_bounds__(BX* RY) k_sor2d(Grid g, int flg, T ax2, T ay2,
 {
     using C = T;
     constexpr int SOR_D = D;
-    const int j = blockIdx.y * RY + threadIdx.y + 1;
-    const int i = ((j + flg) & 1) + 1 + 2 * (blockIdx.x * BX + threadIdx.x);
+    // 1-D grid of nbx x nby tiles, XCD-aware: each XCD marches a contiguous
+    // band of rows, so the neighbour rows and viscosity rows a tile shares
+    // with the next stay in one L2
+    const int nbx = (g.maxl / 2 + BX - 1) / BX;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = (t / nbx) * RY + threadIdx.y + 1;
+    const int i = ((j + flg) & 1) + 1 + 2 * ((t % nbx) * BX + threadIdx.x);
     if (i > imax_of<COMP>(g.maxl) || j > g.P) return;
     const int P = g.P, maxn = g.maxn;
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
@@ -323,7 +320,7 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
         }
         return with_rows<CP, T>(tl.rows_odd, [&](auto ry) {
             constexpr int RY = decltype(ry)::value;
-            const dim3 grd(nbx, (g.P + RY - 1) / RY, 1);
+            const dim3 grd(nbx * ((g.P + RY - 1) / RY));
             if (tl.depth_odd == 8)
                 hipLaunchKernelGGL((k_sor2d<CP, RY, 8, T>), grd, dim3(BX, RY), 0, s, g, flg, a2, b2, c2, x, rhs, mu);
             else

@@ -28,6 +28,15 @@ inline Tiles cg_tiles(const Grid& g)
     return t;
 }
 
+// Blocks of a 1-D launch are dealt to the 8 XCDs round-robin; this order
+// gives each XCD a contiguous range of tiles, so tiles that share
+// neighbour rows and planes run under one L2.  A bijection on [0, nblk).
+__device__ __forceinline__ int xcd_tile(int b, int nblk)
+{
+    const int per = nblk >> 3;
+    return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+}
+
 // Decode tile t into the first interior cell (i, j, k0) of this thread.
 __device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int& j, int& k0)
 {
