@@ -53,6 +53,9 @@ def parse():
                          "throughput and error against this run (0 to skip)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay each step's launches from HIP graphs (identical results)")
+    ap.add_argument("--sor-fused", type=int, default=0,
+                    help="1: each SOR iteration as one fused out-of-place pass instead of the two in-place "
+                         "colour sweeps (identical results)")
     ap.add_argument("--tiles", default="auto",
                     help="launch shapes of the Helmholtz sweeps: auto (the autotuner's table) or default")
     ap.add_argument("--batch-cases", type=int, default=64,
@@ -107,31 +110,39 @@ def cpu_baseline(p, its_gpu: float, threads: int):
 
 
 KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x"}
+FUSED_NAMES = {"sor_odd": "k_rb2d", "sor_even": "k_rb3d"}
 
 
-def algorithmic_bytes(p, kernel: str, precision: str = "fp64") -> float:
+def kernel_name(kernel: str, split: bool) -> str:
+    """The kernel a timed class launches: the fused SOR pass unless the two
+    in-place sweeps were asked for."""
+    return KERNEL_NAMES[kernel] if split or kernel not in FUSED_NAMES else FUSED_NAMES[kernel]
+
+
+def algorithmic_bytes(p, kernel: str, precision: str = "fp64", split: bool = False) -> float:
     """Algorithmic HBM bytes per launch: each distinct array read or written
     once over the interior.  Jacobi: x_old, RHS, viscosity read, x written,
-    32 B per cell; SOR half sweep: x, RHS and viscosity read over the colour's
-    half plus the other colour's x, half of x written, 24 B per cell; CG
-    update: p, x, r read, x, r written, 40 B per unknown.  The Helmholtz
-    launches cycle over u, v, w, so their cell count is the mean of the three
-    component grids."""
+    32 B per cell; fused SOR pass (both colours): x, RHS, viscosity read, x
+    written, 32 B per cell; one in-place SOR colour sweep: x, RHS and
+    viscosity read over the colour's half plus the other colour's x, half of
+    x written, 24 B per cell; CG update: p, x, r read, x, r written, 40 B per
+    unknown.  The Helmholtz launches cycle over u, v, w, so their cell count
+    is the mean of the three component grids."""
     L, P, N = p.maxl, p.maxm - 1, p.maxn
     if kernel == "cg_x":
         return 40.0 * (L - 1) * P * (N - 1)          # FP64 in both precisions
     cells = ((L - 2) * P * (N - 1) + (L - 1) * P * (N - 1) + (L - 1) * P * (N - 2)) / 3.0
-    per = 32.0 if kernel == "jacobi" else 24.0
+    per = 32.0 if kernel == "jacobi" or not split else 24.0
     return per * cells * (0.5 if precision == "fp32" else 1.0)
 
 
-def measured_traffic(kernel: str, nx: int, ny: int, nz: int, precision: str):
+def measured_traffic(kernel: str, nx: int, ny: int, nz: int, precision: str, split: bool = False):
     """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
     summary of this grid, or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{nx}x{ny}x{nz}_{precision}.md")
     if not os.path.exists(path):
         return None
-    name = KERNEL_NAMES[kernel]
+    name = kernel_name(kernel, split)
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if cells and cells[0] == name:
@@ -149,7 +160,7 @@ def plain_cg_run(p, a):
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
     q = replace(p, ksp_pc="none")
-    case = Case(q, precision=a.precision, graphs=bool(a.graphs), tiles=tiles_of(a))
+    case = Case(q, precision=a.precision, **case_opts(a))
     case.load(*make_initial(q))
     case.step(1)
     case.sync()
@@ -185,6 +196,11 @@ def tiles_of(a):
     return "auto" if a.tiles == "auto" else None
 
 
+def case_opts(a) -> dict:
+    """Launch options of every case the bench creates (none changes a value)."""
+    return {"graphs": bool(a.graphs), "tiles": tiles_of(a), "sor_fused": bool(a.sor_fused)}
+
+
 def fp32_run(p, a, ref, ref_volume):
     """The same case with FP32 storage over the same warm-up and timed steps:
     throughput and the error of the final state against the FP64 run (max and
@@ -192,7 +208,7 @@ def fp32_run(p, a, ref, ref_volume):
     import numpy as np
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
-    case = Case(p, precision="fp32", graphs=bool(a.graphs), tiles=tiles_of(a))
+    case = Case(p, precision="fp32", **case_opts(a))
     case.load(*make_initial(p))
     for _ in range(a.warmup):
         case.step(1)
@@ -233,7 +249,7 @@ def batched_run(a):
     cfgs = batch_configs(a.batch_cases, seed=0)
     cases = []
     for p in cfgs:
-        c = Case(p, precision="fp32", diagnostics=False, graphs=bool(a.graphs), tiles=tiles_of(a))
+        c = Case(p, precision="fp32", diagnostics=False, **case_opts(a))
         c.load(*make_initial(p))
         cases.append(c)
     batch_step(cases, a.batch_warmup)
@@ -296,7 +312,7 @@ def main():
     nx, ny, nz = (int(v) for v in a.grid.split("x"))
     p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
     init = make_initial(p)
-    case = Case(p, precision=a.precision, graphs=bool(a.graphs), tiles=tiles_of(a))
+    case = Case(p, precision=a.precision, **case_opts(a))
     case.load(*init)
     del init
     for _ in range(a.warmup):
@@ -332,10 +348,11 @@ def main():
     cells = p.cells
     value = cells * a.steps / elapsed * 1e-6
     lat_ms = np.array(lat) * 1e3
-    kbytes = algorithmic_bytes(p, a.roofline_kernel, a.precision)
+    split = not a.sor_fused
+    kbytes = algorithmic_bytes(p, a.roofline_kernel, a.precision, split)
     avg_ms = ms / max(n, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
-    traffic = measured_traffic(a.roofline_kernel, nx, ny, nz, a.precision)
+    traffic = measured_traffic(a.roofline_kernel, nx, ny, nz, a.precision, split)
     out = {
         "metric": "Mcell-steps/s batched-FP32 vs plain-FP64 at stated error; p50/p99 step latency",
         "value": round(value, 4),
@@ -358,7 +375,7 @@ def main():
                    "launch": "hip graphs" if a.graphs else "eager",
                    "tiles": "autotuned" if a.tiles == "auto" else "default",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
-        "roofline": {"kernel": KERNEL_NAMES[a.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": kernel_name(a.roofline_kernel, split), "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
                      "timed_over_steps": a.roofline_steps,

@@ -71,6 +71,13 @@ typedef struct {
                          SOR sweeps, 1/2/4/8 */
     int tile_rows_odd;  /* the same for the odd-parity (column) SOR sweep */
     int prefetch_odd;   /* levels the column sweep loads ahead, 4 or 8 */
+    int sor_fused;      /* 0 (default): each SOR iteration as the
+                           reference's two in-place colour sweeps; 1: both
+                           sweeps in one out-of-place pass over HBM, with
+                           identical results (every cell sees the same
+                           inputs) */
+    int tile_rows_rb;     /* rows per tile of that pass, even parity: 2/4/8 */
+    int tile_rows_rb_odd; /* the same, odd parity (0 = default for both) */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -130,9 +137,10 @@ int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Change the launch shapes of the Helmholtz sweeps (the tile_rows,
- * tile_rows_odd and prefetch_odd parameters) of an existing case; used by the
- * autotuner.  Results do not depend on them. */
-int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd);
+ * tile_rows_odd, prefetch_odd, tile_rows_rb and tile_rows_rb_odd parameters)
+ * of an existing case; used by the autotuner.  Results do not depend on
+ * them. */
+int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
  * accumulated in FP64, with the smoothed Heaviside of get_viscosity
  * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this

@@ -1,6 +1,10 @@
-"""The Helmholtz sweeps' launch shapes (tile rows, column-sweep prefetch
-depth) change no computed value: every shape the autotuner may pick gives the
-default shape's fields bit for bit."""
+"""The Helmholtz sweeps' launch shapes change no computed value.  Every shape
+the autotuner may pick -- tile rows, the column sweep's prefetch depth, and
+the fused SOR pass (both colour sweeps of an iteration in one out-of-place
+kernel) with its tile rows -- gives the fields of the reference's schedule
+(two in-place colour sweeps, default shapes) bit for bit.  The grids cover
+partial x tiles, y periods that the fused tile rows do not divide, and more
+than one k chunk of the even-parity pass."""
 import numpy as np
 import pytest
 
@@ -10,25 +14,47 @@ from tpls_amd.config import CaseParams
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("u2", "v2", "w2", "phi2", "pres")
+GRIDS = {
+    "72x12x20": CaseParams(nx=72, ny=12, nz=20, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=21, mom_u=6,
+                           mom_v=6, mom_w=6),
+    "130x8x38": CaseParams(nx=130, ny=8, nz=38, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=22, mom_u=8,
+                           mom_v=8, mom_w=8),
+    "64x16x24": CaseParams(nx=64, ny=16, nz=24, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=23),
+}
+
+
+def run(p, init, precision, tiles, split):
+    from tpls_amd.case import Case
+    with Case(p, precision=precision, tiles=tiles, sor_fused=not split) as c:
+        c.load(*init)
+        c.step(3)
+        return {f: c.field(f) for f in FIELDS}
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_every_launch_shape_gives_the_same_fields(precision):
-    from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
-    p = CaseParams(nx=72, ny=12, nz=20, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=21, mom_u=6, mom_v=6,
-                   mom_w=6)
+    p = GRIDS["72x12x20"]
     init = make_initial(p)
-    ref = None
-    shapes = [(r, 4, 4) for r in ROWS] + [(4, ro, d) for ro, d in ODD]
-    for t in shapes:
-        with Case(p, precision=precision, tiles=t) as c:
-            c.load(*init)
-            c.step(3)
-            got = {f: c.field(f) for f in FIELDS}
-        if ref is None:
-            ref = got
-            continue
+    ref = run(p, init, precision, None, True)
+    shapes = [((r, 4, 4), True) for r in ROWS] + [((4, ro, d), True) for ro, d in ODD]
+    shapes += [((r, 4, 4, rb, rbo), False) for r in (1, 4) for rb in (2, 4, 8) for rbo in (2, 4, 8)]
+    for t, split in shapes:
+        got = run(p, init, precision, t, split)
         for f in FIELDS:
-            assert np.array_equal(got[f], ref[f]), (t, f)
+            assert np.array_equal(got[f], ref[f]), (t, split, f)
     print(f"[tiling] {precision}: {len(shapes)} launch shapes identical", flush=True)
+
+
+@pytest.mark.parametrize("grid", ["130x8x38", "64x16x24"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_sor_pass_matches_the_two_sweeps(grid, precision):
+    from tpls_amd.initial import make_initial
+    p = GRIDS[grid]
+    init = make_initial(p)
+    ref = run(p, init, precision, None, True)
+    for rb in (2, 4, 8):
+        got = run(p, init, precision, (4, 2, 8, rb, rb), False)
+        for f in FIELDS:
+            assert np.array_equal(got[f], ref[f]), (grid, rb, f)
+    print(f"[sor pass] {grid} {precision}: fused == split for tile rows 2/4/8", flush=True)

@@ -28,7 +28,8 @@ class Params(C.Structure):
                [(n, C.c_double) for n in ("ksp_rtol", "ksp_abstol", "ksp_dtol")] + \
                [("ksp_maxits", C.c_int), ("precision", C.c_int), ("diagnostics", C.c_int),
                 ("ksp_pc", C.c_int), ("graphs", C.c_int), ("tile_rows", C.c_int),
-                ("tile_rows_odd", C.c_int), ("prefetch_odd", C.c_int)]
+                ("tile_rows_odd", C.c_int), ("prefetch_odd", C.c_int), ("sor_fused", C.c_int),
+                ("tile_rows_rb", C.c_int), ("tile_rows_rb_odd", C.c_int)]
 
 
 class Diag(C.Structure):

@@ -1,17 +1,20 @@
 """Autotuner of the Helmholtz sweeps' launch shapes.
 
-The sweeps have three run-time launch parameters that do not change any
-computed value: the rows per 64-wide tile of the Jacobi and even-parity SOR
-sweeps, the rows per tile of the odd-parity (column) SOR sweep, and how many
-levels that column sweep loads ahead.  tune() times every candidate on the
-actual grid and precision with the library's HIP-event kernel timing and
-returns the fastest; tuned() looks the answer up in tuning.json (results of
-earlier tuning runs on the GPU, keyed by grid and precision) and tunes on a
-miss.
+The sweeps have run-time launch parameters that do not change any computed
+value: the rows per 64-wide tile of the Jacobi and even-parity SOR sweeps,
+the rows per tile of the odd-parity (column) SOR sweep and how many levels
+that column sweep loads ahead; with the fused SOR pass (both colour sweeps
+of an iteration in one kernel, sor_fused) instead, the rows per tile of the
+Jacobi sweep and of the fused pass for each parity.  tune()
+times every candidate on the actual grid and precision with the library's
+HIP-event kernel timing and returns the fastest; tuned() looks the answer
+up in tuning.json (results of earlier tuning runs on the GPU, keyed by grid
+and precision) and tunes on a miss.
 
     python -m tpls_amd.autotune 512x128x256 64x16x32 ... [--precision fp32]
 
-prints the tuned shapes as JSON lines for tuning.json.
+prints the tuned shapes as JSON lines for tuning.json.  A shape is
+(rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd).
 """
 from __future__ import annotations
 
@@ -22,49 +25,61 @@ from dataclasses import replace
 
 ROWS = (1, 2, 4, 8)
 ODD = ((1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8))
+RB = (2, 4, 8)
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 _cache: dict = {}
 
 
-def key(p, precision: str) -> str:
-    return f"{p.nx}x{p.ny}x{p.nz}/{precision}"
+def key(p, precision: str, fused: bool = False) -> str:
+    return f"{p.nx}x{p.ny}x{p.nz}/{precision}" + ("/fused" if fused else "")
 
 
-def tune(p, precision: str = "fp64", steps: int = 1, report: dict | None = None):
-    """(rows, rows_odd, prefetch_odd) with the least kernel time on this grid."""
+def _per_launch(c, cls: str, steps: int) -> float:
+    c.time_kernels(cls)
+    c.step(steps)
+    ms, n = c.kernel_time()
+    c.time_kernels(None)
+    return ms / max(n, 1)
+
+
+def tune(p, precision: str = "fp64", steps: int = 1, report: dict | None = None, fused: bool = False):
+    """The launch shape with the least kernel time on this grid."""
     from .case import Case
     from .initial import make_initial
     q = replace(p, ksp_pc="mg") if p.nx % 2 == 0 else p
-    with Case(q, precision=precision, diagnostics=False) as c:
+    times = {}
+    split = not fused
+    with Case(q, precision=precision, diagnostics=False, sor_fused=fused) as c:
         c.load(*make_initial(q))
         c.step(1)
-        times = {}
         for r in ROWS:
             c.set_tiling(r, 4, 4)
-            tot = 0.0
-            for cls in ("jacobi", "sor_even"):
-                c.time_kernels(cls)
-                c.step(2 * steps)        # even iteration_time sweeps run every other outer iteration
-                ms, n = c.kernel_time()
-                tot += ms / max(n, 1)
-            c.time_kernels(None)
-            times[("rows", r)] = tot
+            # even iteration_time sweeps run every other outer iteration
+            times[("rows", r)] = _per_launch(c, "jacobi", 2 * steps)
+            if split:
+                times[("rows", r)] += _per_launch(c, "sor_even", 2 * steps)
         best_rows = min(ROWS, key=lambda r: times[("rows", r)])
-        for ro, d in ODD:
-            c.set_tiling(best_rows, ro, d)
-            c.time_kernels("sor_odd")
-            c.step(steps)
-            ms, n = c.kernel_time()
-            c.time_kernels(None)
-            times[("odd", ro, d)] = ms / max(n, 1)
-        best_odd = min(ODD, key=lambda t: times[("odd",) + t])
+        if split:
+            for ro, d in ODD:
+                c.set_tiling(best_rows, ro, d)
+                times[("odd", ro, d)] = _per_launch(c, "sor_odd", steps)
+            best = (best_rows,) + min(ODD, key=lambda t: times[("odd",) + t]) + (4, 4)
+        else:
+            for rb in RB:
+                c.set_tiling(best_rows, 2, 8, rb, 4)
+                times[("rb", rb)] = _per_launch(c, "sor_even", 2 * steps)
+            for rb in RB:
+                c.set_tiling(best_rows, 2, 8, 4, rb)
+                times[("rb_odd", rb)] = _per_launch(c, "sor_odd", steps)
+            best = (best_rows, 2, 8, min(RB, key=lambda r: times[("rb", r)]),
+                    min(RB, key=lambda r: times[("rb_odd", r)]))
     if report is not None:
         report.update({"/".join(map(str, k)): round(v * 1e3, 2) for k, v in times.items()})
-    return (best_rows,) + best_odd
+    return best
 
 
-def tuned(p, precision: str = "fp64"):
-    k = key(p, precision)
+def tuned(p, precision: str = "fp64", fused: bool = False):
+    k = key(p, precision, fused)
     if k in _cache:
         return _cache[k]
     if os.path.exists(TABLE):
@@ -73,7 +88,7 @@ def tuned(p, precision: str = "fp64"):
         if k in table:
             _cache[k] = tuple(table[k]["tiles"])
             return _cache[k]
-    _cache[k] = tune(p, precision)
+    _cache[k] = tune(p, precision, fused=fused)
     return _cache[k]
 
 
@@ -82,13 +97,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("grids", nargs="+")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--fused", action="store_true", help="tune the fused SOR pass instead")
     a = ap.parse_args()
     for gs in a.grids:
         nx, ny, nz = (int(v) for v in gs.split("x"))
         p = replace(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz) if nx >= 512 else CaseParams(nx=nx, ny=ny, nz=nz)
         rep: dict = {}
-        best = tune(p, a.precision, report=rep)
-        print(json.dumps({key(p, a.precision): {"tiles": list(best), "us_per_launch": rep}}), flush=True)
+        best = tune(p, a.precision, report=rep, fused=a.fused)
+        print(json.dumps({key(p, a.precision, a.fused): {"tiles": list(best), "us_per_launch": rep}}), flush=True)
 
 
 if __name__ == "__main__":

@@ -25,34 +25,39 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
 
 
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                graphs: bool = False, tiles=None) -> _lib.Params:
+                graphs: bool = False, tiles=None, sor_fused: bool = False) -> _lib.Params:
     prm = _lib.Params()
     for k, v in p.solver_dict().items():
         setattr(prm, k, v)
     prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
     prm.diagnostics = 1 if diagnostics else 0
     prm.graphs = 1 if graphs else 0
+    prm.sor_fused = 1 if sor_fused else 0
     if tiles:
-        prm.tile_rows, prm.tile_rows_odd, prm.prefetch_odd = (int(v) for v in tiles)
+        prm.tile_rows, prm.tile_rows_odd, prm.prefetch_odd = (int(v) for v in tiles[:3])
+        if len(tiles) > 3:
+            prm.tile_rows_rb, prm.tile_rows_rb_odd = (int(v) for v in tiles[3:5])
     return prm
 
 
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                 graphs: bool = False, tiles=None):
+                 graphs: bool = False, tiles=None, sor_fused: bool = False):
         """graphs: replay each step's launches from HIP graphs (same results).
-        tiles: (rows, rows_odd, prefetch_odd) launch shapes of the Helmholtz
-        sweeps, "auto" for the tuned shapes of this grid (tpls_amd.autotune),
-        or None for the defaults; every choice computes the same values."""
+        tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd]) launch
+        shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
+        grid (tpls_amd.autotune), or None for the defaults.  sor_fused: run
+        each SOR iteration as one out-of-place pass instead of the two
+        in-place colour sweeps.  Every choice computes the same values."""
         if tiles == "auto":
             from .autotune import tuned
-            tiles = tuned(params, precision)
+            tiles = tuned(params, precision, sor_fused)
         params.check()
         self.params = params
         self.precision = precision
         self.shape = (params.maxl, params.maxm, params.maxn)
         self.tiles = tiles
-        self._prm = make_params(params, precision, diagnostics, graphs, tiles)
+        self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused)
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.tpls_case_create(C.byref(self._prm), *self.shape, C.byref(h)), "tpls_case_create")
@@ -105,9 +110,10 @@ class Case:
         _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
-    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int) -> None:
-        _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd), "set_tiling")
-        self.tiles = (rows, rows_odd, prefetch_odd)
+    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int, rows_rb: int = 4, rows_rb_odd: int = 4) -> None:
+        _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd),
+                   "set_tiling")
+        self.tiles = (rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd)
 
     def volume(self) -> float:
         """Liquid volume: sum of H_eps(phi2) dx dy dz over the interior, FP64."""

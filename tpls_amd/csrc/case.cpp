@@ -313,6 +313,10 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         return fail(TPLS_ERR_ARG, "tile rows must be 0 (default), 1, 2, 4 or 8");
     if (prm->prefetch_odd != 0 && prm->prefetch_odd != 4 && prm->prefetch_odd != 8)
         return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
+    auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
+    if (prm->sor_fused != 0 && prm->sor_fused != 1) return fail(TPLS_ERR_ARG, "sor_fused must be 0 or 1");
+    if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd))
+        return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
         const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
@@ -336,6 +340,9 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->tile_rows) c->tiling.rows = prm->tile_rows;
     if (prm->tile_rows_odd) c->tiling.rows_odd = prm->tile_rows_odd;
     if (prm->prefetch_odd) c->tiling.depth_odd = prm->prefetch_odd;
+    c->tiling.fused = prm->sor_fused;
+    if (prm->tile_rows_rb) c->tiling.rb_rows = prm->tile_rows_rb;
+    if (prm->tile_rows_rb_odd) c->tiling.rb_rows_odd = prm->tile_rows_rb_odd;
     int rc = TPLS_OK;
     auto bail = [&](int r) {
         tpls_case_destroy(c);
@@ -531,16 +538,20 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
     return TPLS_OK;
 }
 
-int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd)
+int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd)
 {
     if (!c) return fail(TPLS_ERR_ARG, "null case");
     auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
-    if (!rows_ok(rows) || !rows_ok(rows_odd) || (prefetch_odd != 4 && prefetch_odd != 8))
-        return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8 and prefetch_odd 4 or 8");
+    auto rb_ok = [](int r) { return r == 2 || r == 4 || r == 8; };
+    if (!rows_ok(rows) || !rows_ok(rows_odd) || (prefetch_odd != 4 && prefetch_odd != 8) || !rb_ok(rows_rb) ||
+        !rb_ok(rows_rb_odd))
+        return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8, prefetch_odd 4 or 8 and the fused pass's rows 2, 4 or 8");
     HIPCHK(hipStreamSynchronize(c->s));
     c->tiling.rows = rows;
     c->tiling.rows_odd = rows_odd;
     c->tiling.depth_odd = prefetch_odd;
+    c->tiling.rb_rows = rows_rb;
+    c->tiling.rb_rows_odd = rows_rb_odd;
     for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);  // captured with the old shapes
     c->pre_graphs.clear();
     return TPLS_OK;
@@ -846,18 +857,29 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
             if (b != c->i2[q]) others[n++] = b;
         int srcb = others[0];
         int dstb = others[1];
+        // fused SOR pass: the Jacobi step writes the third buffer (u2's,
+        // dead once the momentum right-hand sides exist) and the pass reads
+        // it and writes x3; split: Jacobi into x3, then both sweeps in place
+        const bool fused = c->tiling.fused != 0;
         for (int it = 1; it <= mom[q] / 2; ++it) {
             FieldView<T> xs = V(c->x[q][srcb]), xd = V(c->x[q][dstb]);
+            FieldView<T> xj = fused ? V(c->x[q][c->i2[q]]) : xd;
             const bool tj = c->timed == TPLS_TIME_JACOBI;
             const bool ts = c->timed == (it % 2 ? TPLS_TIME_SOR_ODD : TPLS_TIME_SOR_EVEN);
             if (tj) LCHK(hipEventRecord(next_event(c), s));
-            LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xd, it == 1, c->tiling));
+            LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xj, it == 1, c->tiling));
             if (tj) LCHK(hipEventRecord(next_event(c), s));
-            LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
-            for (int flg = 0; flg < 2; ++flg) {
+            LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xj, V(c->pres)));
+            if (fused) {
                 if (ts) LCHK(hipEventRecord(next_event(c), s));
-                LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), flg, it, c->tiling));
+                LCHK(launch_sor_pair<T>(s, g, ph, q, xj, xd, V(c->rhs[q]), V(c->mu), it, c->tiling));
                 if (ts) LCHK(hipEventRecord(next_event(c), s));
+            } else {
+                for (int flg = 0; flg < 2; ++flg) {
+                    if (ts) LCHK(hipEventRecord(next_event(c), s));
+                    LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), flg, it, c->tiling));
+                    if (ts) LCHK(hipEventRecord(next_event(c), s));
+                }
             }
             LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
             std::swap(srcb, dstb);

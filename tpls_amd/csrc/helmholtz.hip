@@ -6,6 +6,13 @@
 // maxl-1 is a boundary condition), k in 0..maxn-2 with the wall ghost
 // -2 x0 + x1/3; v on i in 1..maxl-1, same k; w on i in 1..maxl-1,
 // k in 1..maxn-2 between the walls w = 0.
+//
+// No FMA contraction anywhere in this file, including the shared helpers
+// (helmholtz(), MuFaces) included below: every update then rounds exactly as
+// the reference's Fortran does, operation by operation, and the in-place
+// sweeps, the fused pass and the CPU restatement agree bit for bit.
+#pragma clang fp contract(off)
+
 #include <type_traits>
 
 #include "tiles.hpp"
@@ -218,6 +225,326 @@ __global__ void __launch_bounds__(BX* RY) k_sor2d(Grid g, int flg, T ax2, T ay2,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Both colour sweeps of one SOR iteration in one pass: the sweep flg = 0
+// ("red", i+j(+k) odd) and then flg = 1 ("black"), as main_ns_hybrid.F90
+// calls do_sor_* twice per iteration (:782-785, 818-821, 854-857).  Out of
+// place: xin holds x after the Jacobi step and its boundary conditions and is
+// only read; xout receives every updated cell.  Each cell is updated from the
+// same inputs, with the same arithmetic, as the two in-place sweeps give it:
+//  - a red cell reads its black neighbours before the black sweep: xin;
+//  - a black cell reads its red neighbours after the red sweep.  A block
+//    computes the red cells of its tile and of a one-cell ring around it
+//    (the ring redundantly, as the neighbouring tiles do) and hands them to
+//    the black cells through LDS, one level at a time;
+//  - the 2-D colouring (odd iteration_time) is Gauss-Seidel up each column,
+//    so both colours march k, black one level behind red;
+//  - with the 3-D colouring (even iteration_time) a black cell also reads
+//    the red cells above and below it, so black trails red by two levels;
+//    nothing is sequential in k, and the levels are split into chunks of KS
+//    (each chunk recomputing the red level on either side of it).
+// A tile is RB_TX = 62 interior columns by TY rows; with the ring it spans
+// 64 columns, 32 cells of each colour per row.  The red cells of the TY + 2
+// rows take the first (TY+2)*32 lanes and the black cells of the TY interior
+// rows the next TY*32, so every wavefront is of one colour.  Cells outside
+// the updated range (boundary columns, wall levels) pass their xin value.
+constexpr int RB_TX = 62;
+constexpr int RB_W = RB_TX + 2;
+
+template <typename C>
+struct RbLevel {
+    MuRaw<C> mu;
+    C self, rhs, xp, xm, yp, ym, zl, zh;
+};
+
+// Loads of one cell, all unconditional, so that the level loop keeps a
+// uniform control flow and the compiler waits for exactly the loads a level
+// uses (a conditional load there makes it drain every load in flight).
+// (is, ms) is the stencil position clamped into the updated range and
+// (it, mt) the cell itself clamped into the array, so every load stays
+// inside the arrays whether or not the lane has a cell there; a lane without
+// one discards what it loaded.  zl/zh: x one level below / above ms, within
+// the array's levels 0..mtop (ZL: load zl at all).
+template <int COMP, bool ZL, typename C, typename T>
+__device__ __forceinline__ void rb_load(RbLevel<C>& L, const FieldView<T>& x, const FieldView<T>& rhs,
+                                        const FieldView<T>& mu, int is, int it, int j, int jp, int jm, int ms,
+                                        int mt, int mtop, int P)
+{
+    L.mu = mu_raw<COMP, C>(mu, is, j, ms, P);
+    L.self = x(it, j, mt);
+    L.rhs = rhs(is, j, ms);
+    L.xp = x(is + 1, j, ms);
+    L.xm = x(is - 1, j, ms);
+    L.yp = x(is, jp, ms);
+    L.ym = x(is, jm, ms);
+    L.zl = ZL ? C(x(is, j, max(ms - 1, 0))) : C(0);
+    L.zh = x(is, j, min(ms + 1, mtop));
+}
+
+// z neighbours of the cell at level k with current value cur, given the
+// values below (lo) and above (hi) at the time of its update; u and v use
+// the wall ghost -2 x0 + x1/3 at both ends (jacobi_iteration_allflux.f90:63-73).
+// One division, always done, for whichever end the level is at, so that no
+// branch surrounds it (the two ends never coincide: maxn >= 6).
+template <int COMP, typename C>
+__device__ __forceinline__ void rb_z(int k, int maxn, C cur, C lo, C hi, C& zp, C& zm)
+{
+    if (COMP == 2) {
+        zp = hi;
+        zm = lo;
+        return;
+    }
+    const bool bot = k == 0;
+    const C ghost = C(-2) * cur + (bot ? hi : lo) / C(3);
+    zm = bot ? ghost : lo;
+    zp = (k == maxn - 2) ? ghost : hi;
+}
+
+template <int COMP, typename C>
+__device__ __forceinline__ C rb_update(const RbLevel<C>& L, C ax2, C ay2, C az2, C xp, C xm, C yp, C ym, C zp,
+                                       C zm)
+{
+    const C cur = L.self;
+    const C res = helmholtz<C>(mu_cook<COMP, C>(L.mu), ax2, ay2, az2, xp, xm, yp, ym, zp, zm, L.rhs);
+    return cur + C(1.2) * (res - cur);
+}
+
+// The fused passes stage each level of x and of the viscosity into LDS for
+// the whole tile and its halo (rows j0-2 .. j0+TY+1 of x, j0-1 .. j0+TY+2 of
+// mu; columns i0-2 .. i0+63): every value is loaded from HBM once per block
+// with coalesced loads, a few per lane, instead of the ~20 scattered loads
+// per cell a stencil needs, so a register ring of a few values per lane
+// keeps many levels in flight.  Columns and rows outside the arrays are
+// clamped to the nearest existing ones: only cells that are not updated read
+// them, and those discard what they compute.
+template <int TY>
+struct RbPlane {
+    static constexpr int R = TY + 4, W = RB_W + 2, N = R * W;
+};
+
+// Viscosity values of the cell in slot (r, c) at level k from the staged mu
+// planes, in mu_raw's order.  Row r + dj holds mu row j0-1+r+dj, i.e. the
+// cell's row wrapj(j+dj); column c + 1 + di holds mu column i + di.
+template <int COMP, int TY, typename C>
+__device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbPlane<TY>::W], int k, int r, int c)
+{
+    // k >= -1 (a black lane's step before its first level), so k + dk + 5 > 0
+    auto m = [&](int di, int dj, int dk) { return ms[(k + dk + 5) % 5][r + dj][c + 1 + di]; };
+    MuRaw<C> q;
+    if (COMP == 0) {
+        q.a = m(0, 1, 1);
+        q.b = m(1, 1, 1);
+        q.q[0][0] = m(0, 2, 1), q.q[0][1] = m(1, 2, 1);
+        q.q[1][0] = m(0, 0, 1), q.q[1][1] = m(1, 0, 1);
+        q.q[2][0] = m(0, 1, 2), q.q[2][1] = m(1, 1, 2);
+        q.q[3][0] = m(0, 1, 0), q.q[3][1] = m(1, 1, 0);
+    } else if (COMP == 1) {
+        q.a = m(0, 1, 1);
+        q.b = m(0, 0, 1);
+        q.q[0][0] = m(1, 1, 1), q.q[0][1] = m(1, 0, 1);
+        q.q[1][0] = m(-1, 1, 1), q.q[1][1] = m(-1, 0, 1);
+        q.q[2][0] = m(0, 1, 2), q.q[2][1] = m(0, 0, 2);
+        q.q[3][0] = m(0, 1, 0), q.q[3][1] = m(0, 0, 0);
+    } else {
+        q.a = m(0, 1, 0);
+        q.b = m(0, 1, 1);
+        q.q[0][0] = m(1, 1, 0), q.q[0][1] = m(1, 1, 1);
+        q.q[1][0] = m(-1, 1, 0), q.q[1][1] = m(-1, 1, 1);
+        q.q[2][0] = m(0, 2, 0), q.q[2][1] = m(0, 2, 1);
+        q.q[3][0] = m(0, 0, 0), q.q[3][1] = m(0, 0, 1);
+    }
+    return q;
+}
+
+// Odd iteration_time.  Step s: red lanes update level s, black lanes level
+// s-1 from the red plane of level s-1.  Red lanes march their column's red
+// cell and black lanes their column's black cell, so `prev` is the updated
+// value one level down in the lane's own column, as the in-place sweep has
+// it.  Stage t (x plane t, mu plane t+1) is loaded D steps before step t-2
+// writes it to LDS; at step s the planes x s-1..s+1 and mu s-1..s+2 are read
+// (rings of four and five).  The steps are padded to whole rounds of the
+// register ring, with nothing stored outside k0..k1.
+template <int COMP, int TY, int D, typename T>
+__global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb2d(Grid g, int nbx, T ax2, T ay2, T az2, FieldView<T> xin,
+                                                            FieldView<T> xout, FieldView<T> rhs, FieldView<T> mu)
+{
+    using C = T;
+    using PL = RbPlane<TY>;
+    constexpr int RL = (TY + 2) * 32, NT = (2 * TY + 2) * 32, NS = (PL::N + NT - 1) / NT;
+    __shared__ C xs[4][PL::R][PL::W];
+    __shared__ C ms[5][PL::R][PL::W];
+    __shared__ C plane[3][TY + 2][RB_W];  // red values of two levels and a spare for lanes with nothing to publish
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int i0 = 1 + (t % nbx) * RB_TX, j0 = 1 + (t / nbx) * TY;
+    const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
+    // one colour per wavefront (RL is a multiple of 64): a uniform value
+    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < RL;
+    const int off = red ? 0 : 1;  // the level a lane updates at step s is s - off
+    const int q = red ? threadIdx.x : threadIdx.x - RL;
+    const int r = q / 32 + off, l = q % 32;
+    const int jj = j0 - 1 + r;
+    const int j = wrapj(jj, P);
+    const int c = 2 * l + (((i0 + jj) & 1) ^ off);
+    const int i = i0 - 1 + c;
+    const bool upd = i >= 1 && i <= imax;
+    const bool inner = r >= 1 && r <= TY && c >= 1 && c <= RB_TX;
+    const int is = min(max(i, 1), imax);
+    // neighbour slots in the red plane, clamped (only black lanes inside the
+    // tile use them, and theirs are never clamped)
+    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0), cp = min(c + 1, RB_W - 1), cm = max(c - 1, 0);
+    // this thread's staged elements: offsets within a plane of x and of mu
+    long ox[NS], om[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int e = min(int(threadIdx.x) + n * NT, PL::N - 1);
+        const int er = e / PL::W, ec = e % PL::W;
+        sl[n] = e;
+        ox[n] = min(max(i0 - 2 + ec, 0), g.maxl) + long(wrapj(j0 - 2 + er, P)) * xin.sy;
+        om[n] = min(i0 - 2 + ec, g.maxl + 1) + long(wrapj(j0 - 1 + er, P)) * mu.sy;
+    }
+    auto stage_load = [&](C* vx, C* vm, int st) {
+        const long kx = min(max(st, 0), mtop) * xin.sz, km = min(max(st + 1, 0), maxn) * mu.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            vx[n] = xin.p[ox[n] + kx];
+            vm[n] = mu.p[om[n] + km];
+        }
+    };
+    auto stage_store = [&](const C* vx, const C* vm, int st) {
+        C* px = &xs[st & 3][0][0];
+        C* pm = &ms[(st + 1) % 5][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            px[sl[n]] = vx[n];
+            pm[sl[n]] = vm[n];
+        }
+    };
+    auto lev = [&](int s) { return min(max(s - off, k0), k1); };
+    // stages k0-1 .. k0+1 into LDS; stages k0+2 .. k0+1+D and the RHS of the
+    // first D steps into the register ring
+    {
+        C vx[NS], vm[NS];
+#pragma unroll
+        for (int st = -1; st <= 1; ++st) {
+            stage_load(vx, vm, k0 + st);
+            stage_store(vx, vm, k0 + st);
+        }
+    }
+    C rx[D][NS], rm_[D][NS], rh[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        stage_load(rx[d], rm_[d], k0 + 2 + d);
+        rh[d] = rhs(is, j, lev(k0 + d));
+    }
+    // start the loop with nothing in flight: loads pending from before it,
+    // in another order than the loop issues them, would make the compiler
+    // wait for every load at the top of each round of the ring
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    C prev = xs[(k0 - 1) & 3][r + 1][c + 1];  // w: the wall below level 1 (u, v start with the wall ghost)
+    const int send = k0 + (k1 + 2 - k0 + D - 1) / D * D;
+    for (int sb = k0; sb < send; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int s = sb + d, k = s - off, b = k & 1;
+            const bool on = k >= k0 && k <= k1;
+            // stage s+2 to LDS (read from step s+1 on), then reload the slot
+            stage_store(rx[d], rm_[d], s + 2);
+            stage_load(rx[d], rm_[d], s + 2 + D);
+            RbLevel<C> L;
+            L.mu = mu_raw_lds<COMP, TY, C>(ms, k, r, c);
+            L.self = xs[k & 3][r + 1][c + 1];
+            L.zh = xs[(k + 1) & 3][r + 1][c + 1];
+            L.rhs = rh[d];
+            rh[d] = rhs(is, j, lev(s + D));
+            // every lane reads both sources and picks by colour: no branch
+            const C pxp = plane[b][r][cp], pxm = plane[b][r][cm], pyp = plane[b][rp][c], pym = plane[b][rm][c];
+            const C sxp = xs[k & 3][r + 1][c + 2], sxm = xs[k & 3][r + 1][c];
+            const C syp = xs[k & 3][r + 2][c + 1], sym = xs[k & 3][r][c + 1];
+            const C xp = red ? sxp : pxp, xm = red ? sxm : pxm, yp = red ? syp : pyp, ym = red ? sym : pym;
+            C zp, zm;
+            rb_z<COMP, C>(k, maxn, L.self, prev, L.zh, zp, zm);
+            const C nv = rb_update<COMP, C>(L, ax2, ay2, az2, xp, xm, yp, ym, zp, zm);
+            const C v = upd ? nv : L.self;
+            prev = on ? v : prev;
+            plane[red && on ? b : 2][r][c] = v;
+            if (on && upd && inner) xout(i, j, k) = v;
+            __syncthreads();
+        }
+    }
+}
+
+// Even iteration_time, levels kb..ke of one chunk.  Step s: red lanes update
+// level s+1, black lanes level s-1 (reading the red planes of s-2, s-1, s).
+// The cells of a colour zigzag between the two columns of a lane's pair from
+// one level to the next.  Red levels run from kb-1 to ke+1 (where they
+// exist: the wall levels of w pass their xin value); red planes live in a
+// ring of four, so one barrier per step separates every write from the
+// reads of that plane.
+template <int COMP, int TY, int KS, int D, typename T>
+__global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb3d(Grid g, int nbx, int nby, T ax2, T ay2, T az2,
+                                                            FieldView<T> xin, FieldView<T> xout, FieldView<T> rhs,
+                                                            FieldView<T> mu)
+{
+    using C = T;
+    constexpr int RL = (TY + 2) * 32;
+    __shared__ C plane[5][TY + 2][RB_W];  // four level planes and a spare
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int i0 = 1 + bx * RB_TX, j0 = 1 + by * TY;
+    const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
+    const int kb = k0 + bz * KS, ke = min(kb + KS - 1, k1);
+    const int rlo = max(kb - 1, 0), rhi = min(ke + 1, mtop);
+    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < RL;
+    const int q = red ? threadIdx.x : threadIdx.x - RL;
+    const int r = q / 32 + (red ? 0 : 1), l = q % 32;
+    const int jj = j0 - 1 + r;
+    const int j = wrapj(jj, P), jp = wrapj(jj + 1, P), jm = wrapj(jj - 1, P);
+    const int par = (i0 + jj + (red ? 0 : 1)) & 1;
+    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0);
+    const int dl = red ? 1 : -1;  // the level a lane updates at step s is s + dl
+    // the lane's column slot at level m: red where i+j+m is odd
+    auto col = [&](int m) { return 2 * l + ((par + m) & 1); };
+    auto load = [&](RbLevel<C>& L, int m) {
+        const int ii = i0 - 1 + col(m);
+        const int is = min(max(ii, 1), imax), it = min(ii, g.maxl);
+        const int ms = min(max(m, k0), k1), mt = min(max(m, 0), mtop);
+        rb_load<COMP, true, C>(L, xin, rhs, mu, is, it, j, jp, jm, ms, mt, mtop, P);
+    };
+    const int s0 = rlo - 1, send = s0 + (ke + 2 - s0 + D - 1) / D * D;
+    RbLevel<C> ring[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(ring[d], s0 + d + dl);
+    __builtin_amdgcn_s_waitcnt(0);  // as in k_rb2d
+    for (int sb = s0; sb < send; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int s = sb + d, m = s + dl, b = m & 3;
+            const int cc = col(m), ii = i0 - 1 + cc;
+            const int cp = min(cc + 1, RB_W - 1), cm = max(cc - 1, 0);
+            const bool upd = ii >= 1 && ii <= imax && m >= k0 && m <= k1;
+            const bool mine = r >= 1 && r <= TY && cc >= 1 && cc <= RB_TX && m >= kb && m <= ke;
+            const bool on = red ? (m >= rlo && m <= rhi) : (m >= kb && m <= ke);
+            RbLevel<C>& L = ring[d];
+            const C pxp = plane[b][r][cp], pxm = plane[b][r][cm], pyp = plane[b][rp][cc], pym = plane[b][rm][cc];
+            const C plo = plane[(m - 1) & 3][r][cc], phi = plane[(m + 1) & 3][r][cc];
+            const C xp = red ? L.xp : pxp, xm = red ? L.xm : pxm, yp = red ? L.yp : pyp, ym = red ? L.ym : pym;
+            const C lo = red ? L.zl : plo, hi = red ? L.zh : phi;
+            C zp, zm;
+            rb_z<COMP, C>(m, maxn, L.self, lo, hi, zp, zm);
+            const C nv = rb_update<COMP, C>(L, ax2, ay2, az2, xp, xm, yp, ym, zp, zm);
+            const C v = upd ? nv : L.self;
+            plane[red && on ? b : 4][r][cc] = v;
+            if (on && upd && mine) xout(ii, j, m) = v;
+            load(L, s + D + dl);
+            __syncthreads();
+        }
+    }
+}
+
 // Outer-loop boundary conditions.
 // u: main_ns_hybrid.F90:764-776 (inlet copy, outlet pressure extrapolation
 //    with the unshifted j of pres, fictitious face maxl = 0);
@@ -333,6 +660,48 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
     return go(std::integral_constant<int, 2>{});
 }
 
+// Tile rows of a fused launch: the requested count where it divides the
+// period, else 2 (the period is even).
+static inline int rb_rows(int want, int P) { return (want == 4 || want == 8) && P % want == 0 ? want : 2; }
+constexpr int RB_KS = 16;  // levels per chunk of the 3-D colouring pass
+
+template <typename T>
+hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xin, FieldView<T> xout,
+                           FieldView<T> rhs, FieldView<T> mu, int iteration_time, const Tiling& tl)
+{
+    const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
+    const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
+    const int imax = comp == 0 ? g.maxl - 2 : g.maxl - 1;
+    const int nbx = (imax + RB_TX - 1) / RB_TX;
+    const bool odd = iteration_time % 2 != 0;
+    const int ty = rb_rows(odd ? tl.rb_rows_odd : tl.rb_rows, g.P);
+    auto go = [&](auto comp_c) {
+        constexpr int CP = decltype(comp_c)::value;
+        auto launch = [&](auto ty_c) {
+            constexpr int TY = decltype(ty_c)::value;
+            const int nby = g.P / TY;
+            const dim3 blk((2 * TY + 2) * 32);
+            if (odd) {
+                // ring depth: 8 stages, 4 where 8 would spill (FP64 with 8-row tiles)
+                constexpr int D = (TY == 8 && sizeof(T) == 8) ? 4 : 8;
+                hipLaunchKernelGGL((k_rb2d<CP, TY, D, T>), dim3(nbx * nby), blk, 0, s, g, nbx, a2, b2, c2, xin, xout,
+                                   rhs, mu);
+            } else {
+                const int nk = (nz_of(CP, g) + RB_KS - 1) / RB_KS;
+                hipLaunchKernelGGL((k_rb3d<CP, TY, RB_KS, 2, T>), dim3(nbx * nby * nk), blk, 0, s, g, nbx, nby, a2, b2, c2,
+                                   xin, xout, rhs, mu);
+            }
+            return hipGetLastError();
+        };
+        if (ty == 8) return launch(std::integral_constant<int, 8>{});
+        if (ty == 4) return launch(std::integral_constant<int, 4>{});
+        return launch(std::integral_constant<int, 2>{});
+    };
+    if (comp == 0) return go(std::integral_constant<int, 0>{});
+    if (comp == 1) return go(std::integral_constant<int, 1>{});
+    return go(std::integral_constant<int, 2>{});
+}
+
 template <typename T>
 hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> x,
                                FieldView<T> pres)
@@ -352,6 +721,8 @@ hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int
                                          FieldView<T>, FieldView<T>, bool, const Tiling&);                           \
     template hipError_t launch_sor<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,        \
                                       FieldView<T>, int, int, const Tiling&);                                        \
+    template hipError_t launch_sor_pair<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,   \
+                                           FieldView<T>, FieldView<T>, int, const Tiling&);                          \
     template hipError_t launch_helmholtz_bc<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>);
 TPLS_INST_HH(double)
 TPLS_INST_HH(float)

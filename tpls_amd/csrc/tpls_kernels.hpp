@@ -72,6 +72,10 @@ hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int com
 // odd sweep's prefetch depth.  Every choice computes the same values.
 struct Tiling {
     int rows = 4, rows_odd = 4, depth_odd = 4;
+    // both SOR sweeps of an iteration in one out-of-place pass (0: the two
+    // in-place sweeps), with tiles of rb_rows (even parity) / rb_rows_odd
+    // rows
+    int fused = 0, rb_rows = 4, rb_rows_odd = 8;
 };
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,
@@ -81,6 +85,13 @@ template <typename T>
 hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> x,
                       FieldView<T> rhs, FieldView<T> mu, int flg, int iteration_time,
                       const Tiling& tl = Tiling{});
+// both sweeps (flg = 0, then 1) of one SOR iteration, out of place: xin is
+// x after the Jacobi step and its boundary conditions, xout receives every
+// updated cell
+template <typename T>
+hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xin,
+                           FieldView<T> xout, FieldView<T> rhs, FieldView<T> mu, int iteration_time,
+                           const Tiling& tl = Tiling{});
 // outer-loop boundary conditions (main_ns_hybrid.F90:764-776, 826-836,
 // 864-879; the same statements close do_jacobi_v / do_jacobi_w)
 template <typename T>
