@@ -121,7 +121,7 @@ struct tpls_case {
     DevArray *cg_r, *cg_p0, *cg_p1;
     // multigrid preconditioner (ksp_pc >= 1): z and t on the pressure grid,
     // levels 1.. in their own arrays
-    DevArray *cg_z = nullptr, *cg_t = nullptr;
+    DevArray* cg_z = nullptr;
     MgHierarchy mg{};
     MgLevel mg_coef[MG_MAXLEV];
     Tiling tiling;              // launch shapes of the Helmholtz sweeps
@@ -296,7 +296,7 @@ size_t tpls_case_bytes(int maxl, int maxm, int maxn, int precision)
     n += p * esz * (3 + 4 + 3 + 4 + 1);      // cphi, phi2/3/ra/rb, D/sg/ist, fx/fy/fz, mu
     n += p * esz;                            // pres
     n += pd * rows * (maxn + 1) * 8 * 4;     // pold + CG vectors
-    n += pd * rows * (maxn + 1) * 8 * 3;     // multigrid z, t and coarse levels (< 1/7 x 3), if enabled
+    n += pd * rows * (maxn + 1) * 8 * 2;     // multigrid z and coarse levels (< 1/7 x 2), if enabled
     return n + (1 << 20);
 }
 
@@ -378,20 +378,15 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         c->mg.nu = prm->ksp_pc;
         c->mg.nlev = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, c->mg.L);
         if ((rc = alloc_field(c, &c->cg_z, sp, 8))) return bail(rc);
-        if ((rc = alloc_field(c, &c->cg_t, sp, 8))) return bail(rc);
         c->mg.L[0].x = c->cg_z->view<double>();
         c->mg.L[0].b = c->cg_r->view<double>();
-        c->mg.L[0].t = c->cg_t->view<double>();
         for (int l = 1; l < c->mg.nlev; ++l) {
             MgLevel& o = c->mg.L[l];
             const Shape sl{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
-            DevArray *ax, *ab, *at;
-            if ((rc = alloc_field(c, &ax, sl, 8)) || (rc = alloc_field(c, &ab, sl, 8)) ||
-                (rc = alloc_field(c, &at, sl, 8)))
-                return bail(rc);
+            DevArray *ax, *ab;
+            if ((rc = alloc_field(c, &ax, sl, 8)) || (rc = alloc_field(c, &ab, sl, 8))) return bail(rc);
             o.x = ax->view<double>();
             o.b = ab->view<double>();
-            o.t = at->view<double>();
         }
     }
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);
@@ -1599,11 +1594,11 @@ int tpls_test_mg_apply(int nx, int ny, int nz, double cx, double cy, double cz, 
         hipStreamSynchronize(s);
         for (void* p : mem) hipFree(p);
     };
-    DevArray lev[MG_MAXLEV][3];
+    DevArray lev[MG_MAXLEV][2];
     for (int l = 0; l < h.nlev; ++l) {
         MgLevel& o = h.L[l];
-        FieldView<double>* v[3] = {&o.x, &o.b, &o.t};
-        for (int q = 0; q < 3; ++q) {
+        FieldView<double>* v[2] = {&o.x, &o.b};
+        for (int q = 0; q < 2; ++q) {
             DevArray& a = lev[l][q];
             a.sh = Shape{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
             a.esz = 8;

@@ -26,10 +26,10 @@
 // Kernels (one thread per cell unless noted):
 //   k_mg_pre     the first pre-smoothing sweep pair from x = 0, fused: red
 //                cells take b/d, black cells form their red neighbours'
-//                values b_n/d_n on the fly;
+//                values b_n/d_n on the fly (one thread per red-black pair);
 //   k_mg_smooth  one colour sweep;
-//   k_mg_resid   t = b - A x;
-//   k_mg_restrict  one thread per coarse cell, up to 4 x 4 x 4 fine reads;
+//   k_mg_rr      the residual b - A x and its restriction to the next level,
+//                fused (the residual never leaves LDS);
 //   k_mg_prolong x += P e;
 //   k_mg_coarse  the coarsest solve, one block;
 //   k_mg_last    the final red sweep on the finest level, fused with the
@@ -81,27 +81,6 @@ __device__ __forceinline__ int mg_nbr(int f, int nc, bool periodic)
     return n;
 }
 
-// Fine cells feeding coarse cell I under the transpose of the interpolation
-// and their weights (candidates 2I-2 .. 2I+1 in order).
-__device__ __forceinline__ int mg_rweights(int I, int nc, int nf, bool periodic, int* fi, double* w)
-{
-    int n = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        int f = 2 * I - 2 + q;
-        if (f < 1) f = periodic ? f + nf : 0;
-        if (f > nf) f = periodic ? f - nf : 0;
-        if (f == 0) continue;
-        const int par = (f + 1) / 2, nb = mg_nbr(f, nc, periodic);
-        const double wt = (par == I ? 0.75 : 0.0) + (nb == I ? 0.25 : 0.0);
-        if (wt == 0.0) continue;
-        fi[n] = f;
-        w[n] = wt;
-        ++n;
-    }
-    return n;
-}
-
 __device__ __forceinline__ bool mg_cell(const MgLevel& o, int& i, int& j, int& k)
 {
     i = blockIdx.x * BX + threadIdx.x + 1;
@@ -110,20 +89,20 @@ __device__ __forceinline__ bool mg_cell(const MgLevel& o, int& i, int& j, int& k
     return i <= o.nx && j <= o.ny;
 }
 
-// first pre-smoothing pair (red then black) from x = 0
+// first pre-smoothing pair (red then black) from x = 0.  One thread per two
+// cells along x, one of each colour (nx is even on every smoothed level), so
+// no lane waits on the other colour's branch.
 __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevel o, const CgState* st)
 {
     if (st->done) return;
-    int i, j, k;
-    if (!mg_cell(o, i, j, k)) return;
-    const double d = mg_deff(o, i, k);
-    const double bv = o.b(i, j, k);
-    if (((i + j + k) & 1) == 0) {
-        o.x(i, j, k) = (bv + 0.0) / d;
-    } else {
-        auto red = [&](int ii, int jj, int kk) { return (o.b(ii, jj, kk) + 0.0) / mg_deff(o, ii, kk); };
-        o.x(i, j, k) = (bv + mg_nsum(o, red, i, j, k)) / d;
-    }
+    const int i0 = 2 * (blockIdx.x * BX + threadIdx.x) + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i0 > o.nx || j > o.ny) return;
+    const int ir = ((i0 + j + k) & 1) == 0 ? i0 : i0 + 1;
+    const int ib = ir == i0 ? i0 + 1 : i0;
+    auto red = [&](int ii, int jj, int kk) { return (o.b(ii, jj, kk) + 0.0) / mg_deff(o, ii, kk); };
+    o.x(ir, j, k) = red(ir, j, k);
+    o.x(ib, j, k) = (o.b(ib, j, k) + mg_nsum(o, red, ib, j, k)) / mg_deff(o, ib, k);
 }
 
 __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevel o, int colour, const CgState* st)
@@ -135,36 +114,141 @@ __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevel o, int colour, con
     o.x(i, j, k) = (o.b(i, j, k) + mg_nsum(o, o.x, i, j, k)) / mg_deff(o, i, k);
 }
 
-__global__ void __launch_bounds__(BX* BY) k_mg_resid(MgLevel o, const CgState* st)
+// Weights of the fine cells 2I-2+q (q = 0..3) feeding coarse cell I under the
+// transpose of the interpolation: the fine cell's parent contributes 3/4 and
+// its neighbour toward it 1/4 (mg_nbr); w[q] = 0 for a candidate outside a
+// Neumann end.  Sums over q = 0..3 skip zero weights.  A periodic candidate
+// wraps but keeps its position q.
+__device__ __forceinline__ void mg_rweights_q(int I, int nc, int nf, bool periodic, double* w)
 {
-    if (st->done) return;
-    int i, j, k;
-    if (!mg_cell(o, i, j, k)) return;
-    o.t(i, j, k) = o.b(i, j, k) - (mg_deff(o, i, k) * o.x(i, j, k) - mg_nsum(o, o.x, i, j, k));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int f = 2 * I - 2 + q;
+        if (f < 1) f = periodic ? f + nf : 0;
+        if (f > nf) f = periodic ? f - nf : 0;
+        const int par = (f + 1) / 2, nb = mg_nbr(f, nc, periodic);
+        w[q] = f == 0 ? 0.0 : (par == I ? 0.75 : 0.0) + (nb == I ? 0.25 : 0.0);
+    }
 }
 
-// b_c = P^T t_f / 8
-__global__ void __launch_bounds__(BX* BY) k_mg_restrict(MgLevel f, MgLevel c, const CgState* st)
+// Residual and restriction fused: b_c = P^T (b - A x) / 8 without writing the
+// fine residual.  A block owns RR_CX x RR_CY coarse columns and rk coarse
+// levels and marches the fine planes 2K-2 .. 2K+1 that feed them.  Each fine
+// plane's residual over the block's footprint (2 RR_CX + 2 by 2 RR_CY + 2
+// cells, y wrapped) is formed in LDS with k_mg_resid's arithmetic (x * d -
+// (cz (zm + zp) + cx (xm + xp) + cy (ym + yp)), absent neighbours 0), summed
+// along x (si, LDS) and y (sj, registers of the coarse cell's thread), and
+// every second plane closes a coarse level along z.  The sums nest and run
+// in the order of the restriction's definition, so the coarse values are the
+// oracle's.  The inputs of the next plane are loaded while the current one
+// is summed.
+constexpr int RR_CX = 32, RR_CY = 4, RR_T = 256;
+constexpr int RR_FX = 2 * RR_CX + 2, RR_FY = 2 * RR_CY + 2, RR_N = RR_FX * RR_FY;
+constexpr int RR_C = (RR_N + RR_T - 1) / RR_T;  // footprint cells per thread
+
+struct RrIn {
+    double b, c, xm, xp, ym, yp, zm, zp;
+};
+
+__global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, int nty, int rk, const CgState* st)
 {
     if (st->done) return;
-    int I, J, K;
-    if (!mg_cell(c, I, J, K)) return;
-    int fi[4], fj[4], fk[4];
-    double wi[4], wj[4], wk[4];
-    const int ni = mg_rweights(I, c.nx, f.nx, false, fi, wi);
-    const int nj = mg_rweights(J, c.ny, f.ny, true, fj, wj);
-    const int nk = mg_rweights(K, c.nz, f.nz, false, fk, wk);
-    double s = 0.0;
-    for (int a = 0; a < nk; ++a) {
-        double sj = 0.0;
-        for (int bq = 0; bq < nj; ++bq) {
-            double si = 0.0;
-            for (int q = 0; q < ni; ++q) si += wi[q] * f.t(fi[q], fj[bq], fk[a]);
-            sj += wj[bq] * si;
-        }
-        s += wk[a] * sj;
+    __shared__ double T[RR_FY][RR_FX];
+    __shared__ double SI[RR_FY][RR_CX];
+    const int tid = threadIdx.x;
+    const int bk = blockIdx.x, tx = bk % ntx, r0 = bk / ntx, ty = r0 % nty, tz = r0 / nty;
+    const int I0 = 1 + tx * RR_CX, J0 = 1 + ty * RR_CY, Kb = 1 + tz * rk, Ke = min(Kb + rk - 1, c.nz);
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;  // footprint origin (fine, unwrapped)
+    // this thread's footprint cells: (fine i, wrapped fine j, its y neighbours)
+    int ci[RR_C], cj[RR_C], cjm[RR_C], cjp[RR_C];
+    bool live[RR_C];
+#pragma unroll
+    for (int n = 0; n < RR_C; ++n) {
+        const int e = min(tid + n * RR_T, RR_N - 1);
+        const int row = e / RR_FX, col = e % RR_FX;
+        ci[n] = fi0 + col;
+        // the footprint can span more than one period on small levels
+        cj[n] = ((fj0 + row - 1) % f.ny + f.ny) % f.ny + 1;
+        cjm[n] = cj[n] == 1 ? f.ny : cj[n] - 1;
+        cjp[n] = cj[n] == f.ny ? 1 : cj[n] + 1;
+        live[n] = tid + n * RR_T < RR_N && ci[n] >= 1 && ci[n] <= f.nx;
     }
-    c.b(I, J, K) = 0.125 * s;
+    auto load = [&](RrIn* in, int fk) {
+#pragma unroll
+        for (int n = 0; n < RR_C; ++n) {
+            if (!live[n]) continue;
+            const int i = ci[n], j = cj[n];
+            in[n].b = f.b(i, j, fk);
+            in[n].c = f.x(i, j, fk);
+            in[n].zm = fk > 1 ? f.x(i, j, fk - 1) : 0.0;
+            in[n].zp = fk < f.nz ? f.x(i, j, fk + 1) : 0.0;
+            in[n].xm = i > 1 ? f.x(i - 1, j, fk) : 0.0;
+            in[n].xp = i < f.nx ? f.x(i + 1, j, fk) : 0.0;
+            in[n].ym = f.x(i, cjm[n], fk);
+            in[n].yp = f.x(i, cjp[n], fk);
+        }
+    };
+    // this thread's coarse cell in the y and z sums
+    const int cI = tid % RR_CX, cJ = tid / RR_CX;
+    const int I = I0 + cI, J = J0 + cJ;
+    const bool own = cJ < RR_CY && I <= c.nx && J <= c.ny;
+    double wj[4];
+    mg_rweights_q(J, c.ny, f.ny, true, wj);
+    const int f_lo = max(2 * Kb - 2, 1), f_hi = min(2 * Ke + 1, f.nz);
+    RrIn in[RR_C];
+    load(in, f_lo);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // sj of fine planes 2K-2 .. 2K+1
+    for (int fk = f_lo; fk <= f_hi; ++fk) {
+        // the residual of plane fk over the footprint
+#pragma unroll
+        for (int n = 0; n < RR_C; ++n) {
+            const int e = tid + n * RR_T;
+            if (e >= RR_N) continue;
+            double t = 0.0;  // outside 1..nx: weight 0
+            if (live[n]) {
+                const RrIn& v = in[n];
+                const double nsum = f.cz * (v.zm + v.zp) + f.cx * (v.xm + v.xp) + f.cy * (v.ym + v.yp);
+                t = v.b - (mg_deff(f, ci[n], fk) * v.c - nsum);
+            }
+            T[e / RR_FX][e % RR_FX] = t;
+        }
+        if (fk < f_hi) load(in, fk + 1);
+        __syncthreads();
+        for (int e = tid; e < RR_FY * RR_CX; e += RR_T) {
+            const int row = e / RR_CX, cx = e % RR_CX;
+            double wi[4];
+            mg_rweights_q(I0 + cx, c.nx, f.nx, false, wi);
+            double si = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (wi[q] != 0.0) si += wi[q] * T[row][2 * cx + q];
+            SI[row][cx] = si;
+        }
+        __syncthreads();
+        double sj = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (wj[q] != 0.0) sj += wj[q] * SI[2 * cJ + q < RR_FY ? 2 * cJ + q : 0][cI];
+        // plane fk is candidate 2K + (fk & 1) of coarse level K = fk / 2 and
+        // candidate 2K - 2 + (fk & 1) of level K + 1
+        if (fk & 1) s3 = sj; else s2 = sj;
+        const int K = fk / 2;
+        if ((fk & 1) || fk == f.nz) {
+            // plane 2K+1 done (or the last plane): close coarse level K
+            if (own && K >= Kb && K <= Ke) {
+                double wk[4];
+                mg_rweights_q(K, c.nz, f.nz, false, wk);
+                const double sv[4] = {s0, s1, s2, s3};
+                double sk = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (wk[q] != 0.0) sk += wk[q] * sv[q];
+                c.b(I, J, K) = 0.125 * sk;
+            }
+            s0 = s2;
+            s1 = s3;
+        }
+    }
 }
 
 // x_f += P e_c (z first, then y, then x)
@@ -321,6 +405,7 @@ int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag
 }
 
 static dim3 level_grid(const MgLevel& o) { return dim3((o.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
+static dim3 pair_grid(const MgLevel& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
 
 hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
 {
@@ -328,13 +413,19 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     const int top = h.nlev - 1;
     for (int l = 0; l < top; ++l) {
         const MgLevel& o = h.L[l];
-        hipLaunchKernelGGL(k_mg_pre, level_grid(o), blk, 0, s, o, st);
+        const MgLevel& oc = h.L[l + 1];
+        hipLaunchKernelGGL(k_mg_pre, pair_grid(o), blk, 0, s, o, st);
         for (int q = 1; q < h.nu; ++q) {
             hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 0, st);
             hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 1, st);
         }
-        hipLaunchKernelGGL(k_mg_resid, level_grid(o), blk, 0, s, o, st);
-        hipLaunchKernelGGL(k_mg_restrict, level_grid(h.L[l + 1]), blk, 0, s, o, h.L[l + 1], st);
+        // coarse levels per block: as many as keep >= 1024 blocks, up to 8
+        // (each block recomputes the two fine planes below its first level)
+        const int ntx = (oc.nx + RR_CX - 1) / RR_CX, nty = (oc.ny + RR_CY - 1) / RR_CY;
+        int rk = 8;
+        while (rk > 1 && long(ntx) * nty * ((oc.nz + rk - 1) / rk) < 1024) rk /= 2;
+        const int ntz = (oc.nz + rk - 1) / rk;
+        hipLaunchKernelGGL(k_mg_rr, dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
     }
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(256), 0, s, h.L[top], st);
     for (int l = top - 1; l >= 0; --l) {

@@ -149,7 +149,7 @@ hipError_t launch_pres_ghosts(hipStream_t s, const Grid& g, const Phys& ph, Fiel
 struct MgLevel {
     int nx, ny, nz;
     double cx, cy, cz, diag;
-    FieldView<double> x, b, t;
+    FieldView<double> x, b;
 };
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_COARSE_MAX = 2048;  // cells of the coarsest level (one block, LDS)
