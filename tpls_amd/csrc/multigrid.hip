@@ -268,27 +268,39 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevel f, MgLevel c, con
     f.x(i, j, k) = f.x(i, j, k) + (0.75 * b0 + 0.25 * b1);
 }
 
-// Coarsest level: one 256-thread block; x and b live in LDS (n <= MG_COARSE_MAX).
-// The two means are summed by one thread in index order.
-__global__ void __launch_bounds__(256) k_mg_coarse(MgLevel o, const CgState* st)
+// Coarsest level: one block, a single wavefront for up to 64 cells (its
+// barriers cost next to nothing), else 256 threads; x and b live in LDS
+// (n <= MG_COARSE_MAX).  Each thread's cells, their colours and diagonals are
+// worked out once, not in every pass.  The two means are summed by one
+// thread in index order.
+constexpr int MGC_T = 256, MGC_CPT = MG_COARSE_MAX / MGC_T;
+
+__global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevel o, const CgState* st)
 {
     if (st->done) return;
     __shared__ double sx[MG_COARSE_MAX], sb[MG_COARSE_MAX];
     __shared__ double mean;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int n = o.nx * o.ny * o.nz;
-    auto ijk = [&](int q, int& i, int& j, int& k) {
-        i = q % o.nx + 1;
+    int ci[MGC_CPT], cj[MGC_CPT], ck[MGC_CPT], col[MGC_CPT];
+    double cd[MGC_CPT];
+#pragma unroll
+    for (int c = 0; c < MGC_CPT; ++c) {
+        const int q = min(tid + c * nt, n - 1);
+        ci[c] = q % o.nx + 1;
         const int r = q / o.nx;
-        j = r % o.ny + 1;
-        k = r / o.ny + 1;
-    };
-    for (int q = tid; q < n; q += 256) {
-        int i, j, k;
-        ijk(q, i, j, k);
-        sb[q] = o.b(i, j, k);
-        sx[q] = 0.0;
+        cj[c] = r % o.ny + 1;
+        ck[c] = r / o.ny + 1;
+        col[c] = tid + c * nt < n ? (ci[c] + cj[c] + ck[c]) & 1 : 2;  // 2: no cell
+        cd[c] = mg_deff(o, ci[c], ck[c]);
     }
+#pragma unroll
+    for (int c = 0; c < MGC_CPT; ++c)
+        if (col[c] != 2) {
+            const int q = tid + c * nt;
+            sb[q] = o.b(ci[c], cj[c], ck[c]);
+            sx[q] = 0.0;
+        }
     __syncthreads();
     if (tid == 0) {
         double s = 0.0;
@@ -297,7 +309,7 @@ __global__ void __launch_bounds__(256) k_mg_coarse(MgLevel o, const CgState* st)
     }
     __syncthreads();
     const double mb = mean;
-    for (int q = tid; q < n; q += 256) sb[q] = sb[q] - mb;
+    for (int q = tid; q < n; q += nt) sb[q] = sb[q] - mb;
     __syncthreads();
     const int nmax = max(o.nx, max(o.ny, o.nz));
     const int sweeps = 20 + nmax * nmax / 2;
@@ -305,12 +317,12 @@ __global__ void __launch_bounds__(256) k_mg_coarse(MgLevel o, const CgState* st)
     for (int s = 0; s < sweeps; ++s) {
         for (int pass = 0; pass < 4; ++pass) {
             const int colour = (pass == 1 || pass == 2) ? 1 : 0;
-            for (int q = tid; q < n; q += 256) {
-                int i, j, k;
-                ijk(q, i, j, k);
-                if (((i + j + k) & 1) != colour) continue;
-                sx[q] = (sb[q] + mg_nsum(o, lds, i, j, k)) / mg_deff(o, i, k);
-            }
+#pragma unroll
+            for (int c = 0; c < MGC_CPT; ++c)
+                if (col[c] == colour) {
+                    const int q = tid + c * nt;
+                    sx[q] = (sb[q] + mg_nsum(o, lds, ci[c], cj[c], ck[c])) / cd[c];
+                }
             __syncthreads();
         }
     }
@@ -321,11 +333,9 @@ __global__ void __launch_bounds__(256) k_mg_coarse(MgLevel o, const CgState* st)
     }
     __syncthreads();
     const double mx = mean;
-    for (int q = tid; q < n; q += 256) {
-        int i, j, k;
-        ijk(q, i, j, k);
-        o.x(i, j, k) = sx[q] - mx;
-    }
+#pragma unroll
+    for (int c = 0; c < MGC_CPT; ++c)
+        if (col[c] != 2) o.x(ci[c], cj[c], ck[c]) = sx[tid + c * nt] - mx;
 }
 
 // Final red sweep on the finest level with the partials of z.r (r = b) and
@@ -427,7 +437,9 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
         const int ntz = (oc.nz + rk - 1) / rk;
         hipLaunchKernelGGL(k_mg_rr, dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
     }
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(256), 0, s, h.L[top], st);
+    const long nc = long(h.L[top].nx) * h.L[top].ny * h.L[top].nz;
+    const int ct = nc <= 64 ? 64 : MGC_T;
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(ct), 0, s, h.L[top], st);
     for (int l = top - 1; l >= 0; --l) {
         const MgLevel& o = h.L[l];
         hipLaunchKernelGGL(k_mg_prolong, level_grid(o), blk, 0, s, o, h.L[l + 1], st);
