@@ -78,6 +78,10 @@ typedef struct {
                            inputs) */
     int tile_rows_rb;     /* rows per tile of that pass, even parity: 2/4/8 */
     int tile_rows_rb_odd; /* the same, odd parity (0 = default for both) */
+    int tile_rows_jst;    /* Jacobi sweep with its planes staged through LDS,
+                             tiles of 2/4/8 rows (0: default, 4); -1: the
+                             unstaged sweep with tile_rows instead (same
+                             results either way) */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -137,10 +141,11 @@ int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Change the launch shapes of the Helmholtz sweeps (the tile_rows,
- * tile_rows_odd, prefetch_odd, tile_rows_rb and tile_rows_rb_odd parameters)
- * of an existing case; used by the autotuner.  Results do not depend on
- * them. */
-int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd);
+ * tile_rows_odd, prefetch_odd, tile_rows_rb, tile_rows_rb_odd and
+ * tile_rows_jst parameters) of an existing case; used by the autotuner.
+ * Results do not depend on them. */
+int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd,
+                         int rows_jst);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
  * accumulated in FP64, with the smoothed Heaviside of get_viscosity
  * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this

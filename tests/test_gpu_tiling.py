@@ -1,7 +1,8 @@
 """The Helmholtz sweeps' launch shapes change no computed value.  Every shape
-the autotuner may pick -- tile rows, the column sweep's prefetch depth, and
-the fused SOR pass (both colour sweeps of an iteration in one out-of-place
-kernel) with its tile rows -- gives the fields of the reference's schedule
+the autotuner may pick -- tile rows, the column sweep's prefetch depth, the
+fused SOR pass (both colour sweeps of an iteration in one out-of-place
+kernel) with its tile rows, the Jacobi sweep with staged planes -- gives the
+fields of the reference's schedule
 (two in-place colour sweeps, default shapes) bit for bit.  The grids cover
 partial x tiles, y periods that the fused tile rows do not divide, and more
 than one k chunk of the even-parity pass."""
@@ -37,8 +38,9 @@ def test_every_launch_shape_gives_the_same_fields(precision):
     p = GRIDS["72x12x20"]
     init = make_initial(p)
     ref = run(p, init, precision, None, True)
-    shapes = [((r, 4, 4), True) for r in ROWS] + [((4, ro, d), True) for ro, d in ODD]
+    shapes = [((r, 4, 4, 4, 8, -1), True) for r in ROWS] + [((4, ro, d), True) for ro, d in ODD]
     shapes += [((r, 4, 4, rb, rbo), False) for r in (1, 4) for rb in (2, 4, 8) for rbo in (2, 4, 8)]
+    shapes += [((4, 2, 8, 4, 8, jst), split) for jst in (2, 4, 8) for split in (True, False)]
     for t, split in shapes:
         got = run(p, init, precision, t, split)
         for f in FIELDS:
@@ -57,4 +59,7 @@ def test_fused_sor_pass_matches_the_two_sweeps(grid, precision):
         got = run(p, init, precision, (4, 2, 8, rb, rb), False)
         for f in FIELDS:
             assert np.array_equal(got[f], ref[f]), (grid, rb, f)
+        got = run(p, init, precision, (4, 2, 8, 4, 8, rb), True)
+        for f in FIELDS:
+            assert np.array_equal(got[f], ref[f]), (grid, "jst", rb, f)
     print(f"[sor pass] {grid} {precision}: fused == split for tile rows 2/4/8", flush=True)

@@ -37,6 +37,8 @@ def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True
         prm.tile_rows, prm.tile_rows_odd, prm.prefetch_odd = (int(v) for v in tiles[:3])
         if len(tiles) > 3:
             prm.tile_rows_rb, prm.tile_rows_rb_odd = (int(v) for v in tiles[3:5])
+        if len(tiles) > 5:
+            prm.tile_rows_jst = int(tiles[5])
     return prm
 
 
@@ -44,7 +46,7 @@ class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                  graphs: bool = False, tiles=None, sor_fused: bool = False):
         """graphs: replay each step's launches from HIP graphs (same results).
-        tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd]) launch
+        tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
         grid (tpls_amd.autotune), or None for the defaults.  sor_fused: run
         each SOR iteration as one out-of-place pass instead of the two
@@ -110,10 +112,11 @@ class Case:
         _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
-    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int, rows_rb: int = 4, rows_rb_odd: int = 4) -> None:
-        _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd),
-                   "set_tiling")
-        self.tiles = (rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd)
+    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int, rows_rb: int = 4, rows_rb_odd: int = 8,
+                   rows_jst: int = 4) -> None:
+        _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd,
+                                                   rows_jst), "set_tiling")
+        self.tiles = (rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd, rows_jst)
 
     def volume(self) -> float:
         """Liquid volume: sum of H_eps(phi2) dx dy dz over the interior, FP64."""

@@ -315,8 +315,9 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
     auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
     if (prm->sor_fused != 0 && prm->sor_fused != 1) return fail(TPLS_ERR_ARG, "sor_fused must be 0 or 1");
-    if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd))
-        return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8");
+    if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) || (prm->tile_rows_jst != -1 && !rb_ok(prm->tile_rows_jst)))
+        return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
+                                  "tile_rows_jst -1 (unstaged), 0 (default), 2, 4 or 8");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
         const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
@@ -343,6 +344,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     c->tiling.fused = prm->sor_fused;
     if (prm->tile_rows_rb) c->tiling.rb_rows = prm->tile_rows_rb;
     if (prm->tile_rows_rb_odd) c->tiling.rb_rows_odd = prm->tile_rows_rb_odd;
+    c->tiling.jst = prm->tile_rows_jst < 0 ? 0 : (prm->tile_rows_jst ? prm->tile_rows_jst : 4);
     int rc = TPLS_OK;
     auto bail = [&](int r) {
         tpls_case_destroy(c);
@@ -533,20 +535,23 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
     return TPLS_OK;
 }
 
-int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd)
+int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd,
+                         int rows_jst)
 {
     if (!c) return fail(TPLS_ERR_ARG, "null case");
     auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
     auto rb_ok = [](int r) { return r == 2 || r == 4 || r == 8; };
     if (!rows_ok(rows) || !rows_ok(rows_odd) || (prefetch_odd != 4 && prefetch_odd != 8) || !rb_ok(rows_rb) ||
-        !rb_ok(rows_rb_odd))
-        return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8, prefetch_odd 4 or 8 and the fused pass's rows 2, 4 or 8");
+        !rb_ok(rows_rb_odd) || (rows_jst != -1 && !rb_ok(rows_jst)))
+        return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8, prefetch_odd 4 or 8, the fused pass's rows 2, 4 "
+                                  "or 8 and the staged Jacobi's 2, 4 or 8 (-1: unstaged)");
     HIPCHK(hipStreamSynchronize(c->s));
     c->tiling.rows = rows;
     c->tiling.rows_odd = rows_odd;
     c->tiling.depth_odd = prefetch_odd;
     c->tiling.rb_rows = rows_rb;
     c->tiling.rb_rows_odd = rows_rb_odd;
+    c->tiling.jst = rows_jst < 0 ? 0 : rows_jst;
     for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);  // captured with the old shapes
     c->pre_graphs.clear();
     return TPLS_OK;

@@ -322,14 +322,12 @@ struct RbPlane {
     static constexpr int R = TY + 4, W = RB_W + 2, N = R * W;
 };
 
-// Viscosity values of the cell in slot (r, c) at level k from the staged mu
-// planes, in mu_raw's order.  Row r + dj holds mu row j0-1+r+dj, i.e. the
-// cell's row wrapj(j+dj); column c + 1 + di holds mu column i + di.
-template <int COMP, int TY, typename C>
-__device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbPlane<TY>::W], int k, int r, int c)
+// Viscosity values of one cell from staged planes, in mu_raw's order:
+// m(di, dj, dk) returns mu at the cell's column + di, its row wrapj(j + dj)
+// and level k + dk.
+template <int COMP, typename C, typename M>
+__device__ __forceinline__ MuRaw<C> mu_raw_at(const M& m)
 {
-    // k >= -1 (a black lane's step before its first level), so k + dk + 5 > 0
-    auto m = [&](int di, int dj, int dk) { return ms[(k + dk + 5) % 5][r + dj][c + 1 + di]; };
     MuRaw<C> q;
     if (COMP == 0) {
         q.a = m(0, 1, 1);
@@ -354,6 +352,17 @@ __device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbP
         q.q[3][0] = m(0, 0, 0), q.q[3][1] = m(0, 0, 1);
     }
     return q;
+}
+
+// The cell in slot (r, c) at level k from the fused pass's staged mu planes:
+// row r + dj holds mu row j0-1+r+dj, i.e. the cell's row wrapj(j+dj); column
+// c + 1 + di holds mu column i + di.
+template <int COMP, int TY, typename C>
+__device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbPlane<TY>::W], int k, int r, int c)
+{
+    // k >= -1 (a black lane's step before its first level), so k + dk + 5 > 0
+    return mu_raw_at<COMP, C>(
+        [&](int di, int dj, int dk) { return ms[(k + dk + 5) % 5][r + dj][c + 1 + di]; });
 }
 
 // Odd iteration_time.  Step s: red lanes update level s, black lanes level
@@ -545,6 +554,108 @@ __global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb3d(Grid g, int nbx, int
     }
 }
 
+// The Jacobi sweep with staged planes.  A block of 64 x TY threads owns
+// 64 x TY columns and a chunk of ks levels.  Level k reads x_old on the planes
+// k-1..k+1 and mu on k..k+2 over the tile and a one-cell halo; the block
+// loads each of those planes once, with coalesced loads (stage t: x plane t
+// and mu plane t+1), D stages ahead into registers, and writes stage k+2 to
+// LDS rings of four planes at step k.  A few loads per lane and level instead
+// of the ~20 scattered ones of k_jacobi, with k_jacobi's arithmetic.  ZERO:
+// x_old is zero (the first iteration), nothing of it is read.
+template <int COMP, int TY, int D, bool ZERO, typename T>
+__global__ void __launch_bounds__(BX* TY) k_jacobi_st(Grid g, int nbx, int nby, int ks, T ax2, T ay2, T az2,
+                                                     FieldView<T> xo, FieldView<T> rhs, FieldView<T> mu,
+                                                     FieldView<T> x)
+{
+    using C = T;
+    constexpr int R = TY + 2, W = BX + 2, N = R * W, NT = BX * TY, NS = (N + NT - 1) / NT;
+    __shared__ C xs[ZERO ? 1 : 4][R][W];
+    __shared__ C ms[4][R][W];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int i0 = 1 + bx * BX, j0 = 1 + by * TY;
+    const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
+    const int kb = k0 + bz * ks, ke = min(kb + ks - 1, k1);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int i = i0 + tx, j = j0 + ty;  // j <= P: the tile rows divide the period
+    const bool mine = i <= imax;
+    const int ic = min(i, imax);
+    // this thread's staged elements: offsets within a plane of x_old (rows
+    // j0-1.., columns i0-1..) and of mu (rows j0.., columns i0-1..), clamped
+    // into the arrays (a clamped element is read by no updated cell)
+    long ox[NS], om[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int e = min(tid + n * NT, N - 1);
+        const int a = e / W, b = e % W;
+        sl[n] = e;
+        ox[n] = min(max(i0 - 1 + b, 0), g.maxl) + long(wrapj(j0 - 1 + a, P)) * xo.sy;
+        om[n] = min(i0 - 1 + b, g.maxl + 1) + long(wrapj(j0 + a, P)) * mu.sy;
+    }
+    auto stage_load = [&](C* vx, C* vm, int st) {
+        const long kx = long(min(max(st, 0), mtop)) * xo.sz, km = long(min(max(st + 1, 0), maxn)) * mu.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            if (!ZERO) vx[n] = xo.p[ox[n] + kx];
+            vm[n] = mu.p[om[n] + km];
+        }
+    };
+    auto stage_store = [&](const C* vx, const C* vm, int st) {
+        C* px = &xs[ZERO ? 0 : (st & 3)][0][0];
+        C* pm = &ms[(st + 1) & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            if (!ZERO) px[sl[n]] = vx[n];
+            pm[sl[n]] = vm[n];
+        }
+    };
+    {
+        C vx[NS], vm[NS];
+#pragma unroll
+        for (int st = -1; st <= 1; ++st) {
+            stage_load(vx, vm, kb + st);
+            stage_store(vx, vm, kb + st);
+        }
+    }
+    C rx[D][NS], rm_[D][NS], rh[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        stage_load(rx[d], rm_[d], kb + 2 + d);
+        rh[d] = rhs(ic, j, min(kb + d, ke));
+    }
+    // start the loop with nothing in flight (see k_rb2d)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int kend = kb + (ke - kb + D) / D * D;
+    for (int sb = kb; sb < kend; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int k = sb + d;
+            stage_store(rx[d], rm_[d], k + 2);
+            stage_load(rx[d], rm_[d], k + 2 + D);
+            const C rv = rh[d];
+            rh[d] = rhs(ic, j, min(k + D, ke));
+            const MuRaw<C> mr =
+                mu_raw_at<COMP, C>([&](int di, int dj, int dk) { return ms[(k + dk) & 3][ty + dj][tx + 1 + di]; });
+            C xp = 0, xm = 0, yp = 0, ym = 0, zp = 0, zm = 0;
+            if (!ZERO) {
+                const int b = k & 3;
+                xp = xs[b][ty + 1][tx + 2];
+                xm = xs[b][ty + 1][tx];
+                yp = xs[b][ty + 2][tx + 1];
+                ym = xs[b][ty][tx + 1];
+                rb_z<COMP, C>(k, maxn, C(xs[b][ty + 1][tx + 1]), C(xs[(k - 1) & 3][ty + 1][tx + 1]),
+                              C(xs[(k + 1) & 3][ty + 1][tx + 1]), zp, zm);
+            }
+            const C v = helmholtz<C>(mu_cook<COMP, C>(mr), ax2, ay2, az2, xp, xm, yp, ym, zp, zm, rv);
+            if (mine && k <= ke) x(i, j, k) = v;
+            __syncthreads();
+        }
+    }
+}
+
 // Outer-loop boundary conditions.
 // u: main_ns_hybrid.F90:764-776 (inlet copy, outlet pressure extrapolation
 //    with the unshifted j of pres, fictitious face maxl = 0);
@@ -609,6 +720,30 @@ hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp,
     const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
     const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
     const int z = xold_is_zero ? 1 : 0;
+    if (tl.jst) {
+        // staged sweep: tile rows that divide the period, chunks of 32 levels
+        const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
+        auto go_st = [&](auto comp_c, auto ty_c, auto zero_c) {
+            constexpr int CP = decltype(comp_c)::value, TY = decltype(ty_c)::value;
+            constexpr bool ZERO = decltype(zero_c)::value;
+            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, ks = 32;
+            const int nbx = (imax + BX - 1) / BX, nby = g.P / TY, nbz = (nz_of(CP, g) + ks - 1) / ks;
+            hipLaunchKernelGGL((k_jacobi_st<CP, TY, 8, ZERO, T>), dim3(nbx * nby * nbz), dim3(BX, TY), 0, s, g, nbx,
+                               nby, ks, a2, b2, c2, xold, rhs, mu, x);
+            return hipGetLastError();
+        };
+        auto by_ty = [&](auto comp_c, auto zero_c) {
+            if (ty == 8) return go_st(comp_c, std::integral_constant<int, 8>{}, zero_c);
+            if (ty == 4) return go_st(comp_c, std::integral_constant<int, 4>{}, zero_c);
+            return go_st(comp_c, std::integral_constant<int, 2>{}, zero_c);
+        };
+        auto by_zero = [&](auto comp_c) {
+            return z ? by_ty(comp_c, std::true_type{}) : by_ty(comp_c, std::false_type{});
+        };
+        if (comp == 0) return by_zero(std::integral_constant<int, 0>{});
+        if (comp == 1) return by_zero(std::integral_constant<int, 1>{});
+        return by_zero(std::integral_constant<int, 2>{});
+    }
     auto go = [&](auto comp_c) {
         constexpr int CP = decltype(comp_c)::value;
         return with_rows<CP, T>(tl.rows, [&](auto ry) {

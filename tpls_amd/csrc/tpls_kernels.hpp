@@ -76,6 +76,9 @@ struct Tiling {
     // in-place sweeps), with tiles of rb_rows (even parity) / rb_rows_odd
     // rows
     int fused = 0, rb_rows = 4, rb_rows_odd = 8;
+    // Jacobi sweep with staged planes and tiles of jst rows (0: the unstaged
+    // sweep with `rows`)
+    int jst = 4;
 };
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,
