@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--pc", default="mg", choices=["none", "mg"],
                     help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
     ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
-    ap.add_argument("--roofline-kernel", default="sor_odd", choices=["sor_odd", "jacobi", "sor_even", "cg_x"])
+    ap.add_argument("--roofline-kernel", default="jacobi", choices=["jacobi", "sor_odd", "sor_even", "cg_x"])
     ap.add_argument("--roofline-steps", type=int, default=2, help="steps over which the kernel is timed")
     ap.add_argument("--fp32-compare", type=int, default=1,
                     help="also run the case with FP32 storage over the same steps and report its "
@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--graphs", type=int, default=1,
                     help="replay each step's launches from HIP graphs (identical results)")
     ap.add_argument("--sor-fused", type=int, default=0,
-                    help="1: each SOR iteration as one fused out-of-place pass instead of the two in-place "
-                         "colour sweeps (identical results)")
+                    help="SOR iterations run as one fused out-of-place pass instead of two in-place colour "
+                         "sweeps (identical results): 0 default (odd iteration_time), -1 none, 1 odd, 2 even, "
+                         "3 all")
     ap.add_argument("--tiles", default="auto",
                     help="launch shapes of the Helmholtz sweeps: auto (the autotuner's table) or default")
     ap.add_argument("--batch-cases", type=int, default=64,
@@ -111,12 +112,18 @@ def cpu_baseline(p, its_gpu: float, threads: int):
 
 KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x"}
 FUSED_NAMES = {"sor_odd": "k_rb2d", "sor_even": "k_rb3d"}
+STAGED_NAMES = {"jacobi": "k_jacobi_st", "sor_even": "k_sor3d_st"}
 
 
-def kernel_name(kernel: str, split: bool) -> str:
+def kernel_name(kernel: str, split: bool, staged: bool = True) -> str:
     """The kernel a timed class launches: the fused SOR pass unless the two
-    in-place sweeps were asked for."""
-    return KERNEL_NAMES[kernel] if split or kernel not in FUSED_NAMES else FUSED_NAMES[kernel]
+    in-place sweeps run; the staged Jacobi / even-parity sweep unless the
+    unstaged ones were asked for."""
+    if not split and kernel in FUSED_NAMES:
+        return FUSED_NAMES[kernel]
+    if staged and kernel in STAGED_NAMES:
+        return STAGED_NAMES[kernel]
+    return KERNEL_NAMES[kernel]
 
 
 def algorithmic_bytes(p, kernel: str, precision: str = "fp64", split: bool = False) -> float:
@@ -136,13 +143,12 @@ def algorithmic_bytes(p, kernel: str, precision: str = "fp64", split: bool = Fal
     return per * cells * (0.5 if precision == "fp32" else 1.0)
 
 
-def measured_traffic(kernel: str, nx: int, ny: int, nz: int, precision: str, split: bool = False):
-    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
-    summary of this grid, or None."""
+def measured_traffic(name: str, nx: int, ny: int, nz: int, precision: str):
+    """HBM bytes per launch of the named kernel from the committed rocprofv3
+    PMC summary of this grid, or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{nx}x{ny}x{nz}_{precision}.md")
     if not os.path.exists(path):
         return None
-    name = kernel_name(kernel, split)
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if cells and cells[0] == name:
@@ -198,7 +204,14 @@ def tiles_of(a):
 
 def case_opts(a) -> dict:
     """Launch options of every case the bench creates (none changes a value)."""
-    return {"graphs": bool(a.graphs), "tiles": tiles_of(a), "sor_fused": bool(a.sor_fused)}
+    return {"graphs": bool(a.graphs), "tiles": tiles_of(a), "sor_fused": a.sor_fused}
+
+
+def sor_split(a, kernel: str) -> bool:
+    """Whether the timed SOR class runs as two in-place sweeps (else the fused pass)."""
+    code = 1 if a.sor_fused == 0 else a.sor_fused
+    bit = {"sor_odd": 1, "sor_even": 2}.get(kernel, 0)
+    return code < 0 or not code & bit
 
 
 def fp32_run(p, a, ref, ref_volume):
@@ -313,6 +326,8 @@ def main():
     p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
     init = make_initial(p)
     case = Case(p, precision=a.precision, **case_opts(a))
+    # the Jacobi and even-parity sweeps run staged unless the shape says -1
+    staged = not (case.tiles and len(case.tiles) > 5 and int(case.tiles[5]) < 0)
     case.load(*init)
     del init
     for _ in range(a.warmup):
@@ -348,11 +363,12 @@ def main():
     cells = p.cells
     value = cells * a.steps / elapsed * 1e-6
     lat_ms = np.array(lat) * 1e3
-    split = not a.sor_fused
+    split = sor_split(a, a.roofline_kernel)
+    name = kernel_name(a.roofline_kernel, split, staged)
     kbytes = algorithmic_bytes(p, a.roofline_kernel, a.precision, split)
     avg_ms = ms / max(n, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
-    traffic = measured_traffic(a.roofline_kernel, nx, ny, nz, a.precision, split)
+    traffic = measured_traffic(name, nx, ny, nz, a.precision)
     out = {
         "metric": "Mcell-steps/s batched-FP32 vs plain-FP64 at stated error; p50/p99 step latency",
         "value": round(value, 4),
@@ -375,7 +391,7 @@ def main():
                    "launch": "hip graphs" if a.graphs else "eager",
                    "tiles": "autotuned" if a.tiles == "auto" else "default",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
-        "roofline": {"kernel": kernel_name(a.roofline_kernel, split), "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
                      "timed_over_steps": a.roofline_steps,

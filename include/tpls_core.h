@@ -71,11 +71,12 @@ typedef struct {
                          SOR sweeps, 1/2/4/8 */
     int tile_rows_odd;  /* the same for the odd-parity (column) SOR sweep */
     int prefetch_odd;   /* levels the column sweep loads ahead, 4 or 8 */
-    int sor_fused;      /* 0 (default): each SOR iteration as the
-                           reference's two in-place colour sweeps; 1: both
-                           sweeps in one out-of-place pass over HBM, with
-                           identical results (every cell sees the same
-                           inputs) */
+    int sor_fused;      /* which SOR iterations run both colour sweeps in
+                           one out-of-place pass over HBM instead of the
+                           reference's two in-place sweeps, with identical
+                           results (every cell sees the same inputs):
+                           -1 none, 1 odd iteration_time (the column
+                           sweeps), 2 even, 3 all; 0: default (1) */
     int tile_rows_rb;     /* rows per tile of that pass, even parity: 2/4/8 */
     int tile_rows_rb_odd; /* the same, odd parity (0 = default for both) */
     int tile_rows_jst;    /* Jacobi sweep with its planes staged through LDS,
@@ -141,11 +142,11 @@ int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Change the launch shapes of the Helmholtz sweeps (the tile_rows,
- * tile_rows_odd, prefetch_odd, tile_rows_rb, tile_rows_rb_odd and
- * tile_rows_jst parameters) of an existing case; used by the autotuner.
+ * tile_rows_odd, prefetch_odd, tile_rows_rb, tile_rows_rb_odd, tile_rows_jst
+ * and sor_fused parameters) of an existing case; used by the autotuner.
  * Results do not depend on them. */
 int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd,
-                         int rows_jst);
+                         int rows_jst, int sor_fused);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
  * accumulated in FP64, with the smoothed Heaviside of get_viscosity
  * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this

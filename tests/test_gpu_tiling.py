@@ -25,8 +25,10 @@ GRIDS = {
 
 
 def run(p, init, precision, tiles, split):
+    """split: True (two sweeps everywhere), False (fused everywhere) or None
+    (the default schedule)."""
     from tpls_amd.case import Case
-    with Case(p, precision=precision, tiles=tiles, sor_fused=not split) as c:
+    with Case(p, precision=precision, tiles=tiles, sor_fused=None if split is None else not split) as c:
         c.load(*init)
         c.step(3)
         return {f: c.field(f) for f in FIELDS}
@@ -40,7 +42,7 @@ def test_every_launch_shape_gives_the_same_fields(precision):
     ref = run(p, init, precision, None, True)
     shapes = [((r, 4, 4, 4, 8, -1), True) for r in ROWS] + [((4, ro, d), True) for ro, d in ODD]
     shapes += [((r, 4, 4, rb, rbo), False) for r in (1, 4) for rb in (2, 4, 8) for rbo in (2, 4, 8)]
-    shapes += [((4, 2, 8, 4, 8, jst), split) for jst in (2, 4, 8) for split in (True, False)]
+    shapes += [((4, 2, 8, 4, 8, jst), split) for jst in (2, 4, 8) for split in (True, False, None)]
     for t, split in shapes:
         got = run(p, init, precision, t, split)
         for f in FIELDS:

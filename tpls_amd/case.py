@@ -24,36 +24,50 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
     return (maxl + 1, maxm + 1, maxn - 1)
 
 
+def fused_code(sor_fused) -> int:
+    """tpls_params.sor_fused from None (the default), a bool (all or no
+    SOR iterations fused) or the code itself (-1, 1, 2, 3)."""
+    if sor_fused is None:
+        return 0
+    if isinstance(sor_fused, bool):
+        return 3 if sor_fused else -1
+    return int(sor_fused)
+
+
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                graphs: bool = False, tiles=None, sor_fused: bool = False) -> _lib.Params:
+                graphs: bool = False, tiles=None, sor_fused=None) -> _lib.Params:
     prm = _lib.Params()
     for k, v in p.solver_dict().items():
         setattr(prm, k, v)
     prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
     prm.diagnostics = 1 if diagnostics else 0
     prm.graphs = 1 if graphs else 0
-    prm.sor_fused = 1 if sor_fused else 0
+    prm.sor_fused = fused_code(sor_fused)
     if tiles:
         prm.tile_rows, prm.tile_rows_odd, prm.prefetch_odd = (int(v) for v in tiles[:3])
         if len(tiles) > 3:
             prm.tile_rows_rb, prm.tile_rows_rb_odd = (int(v) for v in tiles[3:5])
         if len(tiles) > 5:
             prm.tile_rows_jst = int(tiles[5])
+        if len(tiles) > 6 and sor_fused is None:
+            prm.sor_fused = int(tiles[6])
     return prm
 
 
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                 graphs: bool = False, tiles=None, sor_fused: bool = False):
+                 graphs: bool = False, tiles=None, sor_fused=None):
         """graphs: replay each step's launches from HIP graphs (same results).
-        tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst]]) launch
+        tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst[, sor_fused]]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
-        grid (tpls_amd.autotune), or None for the defaults.  sor_fused: run
-        each SOR iteration as one out-of-place pass instead of the two
-        in-place colour sweeps.  Every choice computes the same values."""
+        grid (tpls_amd.autotune), or None for the defaults.  sor_fused: which
+        SOR iterations run as one out-of-place pass instead of two in-place
+        colour sweeps (None: the default, odd iteration_time; True / False:
+        all / none; or the library's code).  Every choice computes the same
+        values."""
         if tiles == "auto":
             from .autotune import tuned
-            tiles = tuned(params, precision, sor_fused)
+            tiles = tuned(params, precision, fused_code(sor_fused))
         params.check()
         self.params = params
         self.precision = precision
@@ -112,11 +126,11 @@ class Case:
         _lib.check(_lib.lib().tpls_case_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
 
-    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int, rows_rb: int = 4, rows_rb_odd: int = 8,
-                   rows_jst: int = 4) -> None:
+    def set_tiling(self, rows: int, rows_odd: int, prefetch_odd: int, rows_rb: int = 8, rows_rb_odd: int = 8,
+                   rows_jst: int = 4, sor_fused: int = 0) -> None:
         _lib.check(_lib.lib().tpls_case_set_tiling(self._h, rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd,
-                                                   rows_jst), "set_tiling")
-        self.tiles = (rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd, rows_jst)
+                                                   rows_jst, sor_fused), "set_tiling")
+        self.tiles = (rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd, rows_jst, sor_fused)
 
     def volume(self) -> float:
         """Liquid volume: sum of H_eps(phi2) dx dy dz over the interior, FP64."""

@@ -314,7 +314,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->prefetch_odd != 0 && prm->prefetch_odd != 4 && prm->prefetch_odd != 8)
         return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
     auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
-    if (prm->sor_fused != 0 && prm->sor_fused != 1) return fail(TPLS_ERR_ARG, "sor_fused must be 0 or 1");
+    if (prm->sor_fused < -1 || prm->sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
     if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) || (prm->tile_rows_jst != -1 && !rb_ok(prm->tile_rows_jst)))
         return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
                                   "tile_rows_jst -1 (unstaged), 0 (default), 2, 4 or 8");
@@ -341,7 +341,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->tile_rows) c->tiling.rows = prm->tile_rows;
     if (prm->tile_rows_odd) c->tiling.rows_odd = prm->tile_rows_odd;
     if (prm->prefetch_odd) c->tiling.depth_odd = prm->prefetch_odd;
-    c->tiling.fused = prm->sor_fused;
+    c->tiling.fused = prm->sor_fused < 0 ? 0 : (prm->sor_fused ? prm->sor_fused : 1);
     if (prm->tile_rows_rb) c->tiling.rb_rows = prm->tile_rows_rb;
     if (prm->tile_rows_rb_odd) c->tiling.rb_rows_odd = prm->tile_rows_rb_odd;
     c->tiling.jst = prm->tile_rows_jst < 0 ? 0 : (prm->tile_rows_jst ? prm->tile_rows_jst : 4);
@@ -536,8 +536,9 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
 }
 
 int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd,
-                         int rows_jst)
+                         int rows_jst, int sor_fused)
 {
+    if (sor_fused < -1 || sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
     if (!c) return fail(TPLS_ERR_ARG, "null case");
     auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
     auto rb_ok = [](int r) { return r == 2 || r == 4 || r == 8; };
@@ -552,6 +553,7 @@ int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd,
     c->tiling.rb_rows = rows_rb;
     c->tiling.rb_rows_odd = rows_rb_odd;
     c->tiling.jst = rows_jst < 0 ? 0 : rows_jst;
+    c->tiling.fused = sor_fused < 0 ? 0 : (sor_fused ? sor_fused : 1);
     for (auto& pg : c->pre_graphs) hipGraphExecDestroy(pg.exec);  // captured with the old shapes
     c->pre_graphs.clear();
     return TPLS_OK;
@@ -860,8 +862,9 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
         // fused SOR pass: the Jacobi step writes the third buffer (u2's,
         // dead once the momentum right-hand sides exist) and the pass reads
         // it and writes x3; split: Jacobi into x3, then both sweeps in place
-        const bool fused = c->tiling.fused != 0;
         for (int it = 1; it <= mom[q] / 2; ++it) {
+            // bit 0: odd iteration_time fused, bit 1: even
+            const bool fused = (c->tiling.fused >> (it % 2 ? 0 : 1)) & 1;
             FieldView<T> xs = V(c->x[q][srcb]), xd = V(c->x[q][dstb]);
             FieldView<T> xj = fused ? V(c->x[q][c->i2[q]]) : xd;
             const bool tj = c->timed == TPLS_TIME_JACOBI;

@@ -354,16 +354,82 @@ __device__ __forceinline__ MuRaw<C> mu_raw_at(const M& m)
     return q;
 }
 
-// The cell in slot (r, c) at level k from the fused pass's staged mu planes:
-// row r + dj holds mu row j0-1+r+dj, i.e. the cell's row wrapj(j+dj); column
-// c + 1 + di holds mu column i + di.
-template <int COMP, int TY, typename C>
-__device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbPlane<TY>::W], int k, int r, int c)
-{
-    // k >= -1 (a black lane's step before its first level), so k + dk + 5 > 0
-    return mu_raw_at<COMP, C>(
-        [&](int di, int dj, int dk) { return ms[(k + dk + 5) % 5][r + dj][c + 1 + di]; });
-}
+// Viscosity values of one column marching up its levels, one level per call
+// of next(): the planes consecutive levels share are kept in registers, so a
+// level reads 6 (u, v) or 5 (w) of its 14 values.  m(di, dj, dk) reads mu at
+// the column + di, row wrapj(j + dj), the new level + dk.  raw() gives
+// mu_raw's values of the current level.
+template <int COMP, typename C>
+struct MuColumn {
+    C z[3][2];  // u: (i, i+1) on row j1; v: (j1, j0) on column i; planes k, k+1, k+2
+    C e[4];     // u: rows j2, j0 at plane k+1 (i, i+1); v: columns i+1, i-1 at plane k+1 (j1, j0)
+    C w[2][5];  // w: (i,j1), (i+1,j1), (i-1,j1), (i,j2), (i,j0) on planes k, k+1
+
+    template <typename M>
+    __device__ __forceinline__ void next(const M& m)
+    {
+        if (COMP == 2) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) w[0][q] = w[1][q];
+            w[1][0] = m(0, 1, 1), w[1][1] = m(1, 1, 1), w[1][2] = m(-1, 1, 1), w[1][3] = m(0, 2, 1),
+            w[1][4] = m(0, 0, 1);
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) z[0][q] = z[1][q], z[1][q] = z[2][q];
+        if (COMP == 0) {
+            z[2][0] = m(0, 1, 2), z[2][1] = m(1, 1, 2);
+            e[0] = m(0, 2, 1), e[1] = m(1, 2, 1), e[2] = m(0, 0, 1), e[3] = m(1, 0, 1);
+        } else {
+            z[2][0] = m(0, 1, 2), z[2][1] = m(0, 0, 2);
+            e[0] = m(1, 1, 1), e[1] = m(1, 0, 1), e[2] = m(-1, 1, 1), e[3] = m(-1, 0, 1);
+        }
+    }
+    // the planes below the first level's (they are shifted in by two calls
+    // of next(), so the first raw() needs no more)
+    template <typename M>
+    __device__ __forceinline__ void prime(const M& m)
+    {
+        if (COMP == 2) {
+            w[1][0] = m(0, 1, 1), w[1][1] = m(1, 1, 1), w[1][2] = m(-1, 1, 1), w[1][3] = m(0, 2, 1),
+            w[1][4] = m(0, 0, 1);
+            return;
+        }
+        if (COMP == 0) {
+            z[1][0] = m(0, 1, 1), z[1][1] = m(1, 1, 1);
+            z[2][0] = m(0, 1, 2), z[2][1] = m(1, 1, 2);
+        } else {
+            z[1][0] = m(0, 1, 1), z[1][1] = m(0, 0, 1);
+            z[2][0] = m(0, 1, 2), z[2][1] = m(0, 0, 2);
+        }
+    }
+    __device__ __forceinline__ MuRaw<C> raw() const
+    {
+        MuRaw<C> r;
+        if (COMP == 2) {
+            r.a = w[0][0], r.b = w[1][0];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) r.q[f][0] = w[0][f + 1], r.q[f][1] = w[1][f + 1];
+        } else {
+            r.a = z[1][0], r.b = z[1][1];
+            r.q[0][0] = e[0], r.q[0][1] = e[1];
+            r.q[1][0] = e[2], r.q[1][1] = e[3];
+            r.q[2][0] = z[2][0], r.q[2][1] = z[2][1];
+            r.q[3][0] = z[0][0], r.q[3][1] = z[0][1];
+        }
+        return r;
+    }
+};
+
+// Lanes of a fused-pass tile of TX x TY interior columns and its one-cell
+// ring: RPR red cells in each of the TY+2 rows, BPR black cells in each of
+// the TY interior rows; each colour's lanes rounded up to whole wavefronts.
+template <int TX, int TY>
+struct RbLanes {
+    static constexpr int RPR = (TX + 2) / 2, BPR = TX / 2;
+    static constexpr int RC = (TY + 2) * RPR, BC = TY * BPR;
+    static constexpr int RL = (RC + 63) / 64 * 64, BL = (BC + 63) / 64 * 64, NT = RL + BL;
+};
 
 // Odd iteration_time.  Step s: red lanes update level s, black lanes level
 // s-1 from the red plane of level s-1.  Red lanes march their column's red
@@ -373,42 +439,48 @@ __device__ __forceinline__ MuRaw<C> mu_raw_lds(const C (*ms)[RbPlane<TY>::R][RbP
 // writes it to LDS; at step s the planes x s-1..s+1 and mu s-1..s+2 are read
 // (rings of four and five).  The steps are padded to whole rounds of the
 // register ring, with nothing stored outside k0..k1.
-template <int COMP, int TY, int D, typename T>
-__global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb2d(Grid g, int nbx, T ax2, T ay2, T az2, FieldView<T> xin,
-                                                            FieldView<T> xout, FieldView<T> rhs, FieldView<T> mu)
+template <int COMP, int TX, int TY, int D, typename T>
+__global__ void __launch_bounds__((RbLanes<TX, TY>::NT)) k_rb2d(Grid g, int nbx, T ax2, T ay2, T az2,
+                                                              FieldView<T> xin, FieldView<T> xout, FieldView<T> rhs,
+                                                              FieldView<T> mu)
 {
     using C = T;
-    using PL = RbPlane<TY>;
-    constexpr int RL = (TY + 2) * 32, NT = (2 * TY + 2) * 32, NS = (PL::N + NT - 1) / NT;
-    __shared__ C xs[4][PL::R][PL::W];
-    __shared__ C ms[5][PL::R][PL::W];
-    __shared__ C plane[3][TY + 2][RB_W];  // red values of two levels and a spare for lanes with nothing to publish
+    using LN = RbLanes<TX, TY>;
+    constexpr int SR = TY + 4, SW = TX + 4, SN = SR * SW, WR = TX + 2;  // staged planes; red plane width
+    constexpr int NT = LN::NT, NS = (SN + NT - 1) / NT;
+    __shared__ C xs[4][SR][SW];
+    __shared__ C ms[5][SR][SW];
+    __shared__ C plane[3][TY + 2][WR];  // red values of two levels and a spare for lanes with nothing to publish
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int i0 = 1 + (t % nbx) * RB_TX, j0 = 1 + (t / nbx) * TY;
+    const int i0 = 1 + (t % nbx) * TX, j0 = 1 + (t / nbx) * TY;
     const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
     const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
     // one colour per wavefront (RL is a multiple of 64): a uniform value
-    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < RL;
+    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < LN::RL;
     const int off = red ? 0 : 1;  // the level a lane updates at step s is s - off
-    const int q = red ? threadIdx.x : threadIdx.x - RL;
-    const int r = q / 32 + off, l = q % 32;
+    // red lanes: the (TX+2)/2 red cells of each of the TY+2 rows of the tile
+    // and its ring; black lanes: the TX/2 black cells of each interior row
+    const int q = red ? threadIdx.x : threadIdx.x - LN::RL;
+    const int per = red ? LN::RPR : LN::BPR;
+    const bool lane = q < (red ? LN::RC : LN::BC);  // lanes past the last cell hold none
+    const int r = lane ? q / per + off : 1, h = lane ? q % per : 0;
     const int jj = j0 - 1 + r;
     const int j = wrapj(jj, P);
-    const int c = 2 * l + (((i0 + jj) & 1) ^ off);
+    const int c = 2 * h + off + ((i0 + jj) & 1);
     const int i = i0 - 1 + c;
-    const bool upd = i >= 1 && i <= imax;
-    const bool inner = r >= 1 && r <= TY && c >= 1 && c <= RB_TX;
+    const bool upd = lane && i >= 1 && i <= imax;
+    const bool inner = r >= 1 && r <= TY && c >= 1 && c <= TX;
     const int is = min(max(i, 1), imax);
     // neighbour slots in the red plane, clamped (only black lanes inside the
     // tile use them, and theirs are never clamped)
-    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0), cp = min(c + 1, RB_W - 1), cm = max(c - 1, 0);
+    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0), cp = min(c + 1, WR - 1), cm = max(c - 1, 0);
     // this thread's staged elements: offsets within a plane of x and of mu
     long ox[NS], om[NS];
     int sl[NS];
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
-        const int e = min(int(threadIdx.x) + n * NT, PL::N - 1);
-        const int er = e / PL::W, ec = e % PL::W;
+        const int e = min(int(threadIdx.x) + n * NT, SN - 1);
+        const int er = e / SW, ec = e % SW;
         sl[n] = e;
         ox[n] = min(max(i0 - 2 + ec, 0), g.maxl) + long(wrapj(j0 - 2 + er, P)) * xin.sy;
         om[n] = min(i0 - 2 + ec, g.maxl + 1) + long(wrapj(j0 - 1 + er, P)) * mu.sy;
@@ -453,6 +525,14 @@ __global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb2d(Grid g, int nbx, T a
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     C prev = xs[(k0 - 1) & 3][r + 1][c + 1];  // w: the wall below level 1 (u, v start with the wall ghost)
+    // the lane's viscosity column, primed for the level below its first step's
+    MuColumn<COMP, C> mc;
+    auto mu_at = [&](int k) {
+        // k >= -3 (a black lane three levels below its first), so k + dk + 5 > 0
+        return [&, k](int di, int dj, int dk) { return ms[(k + dk + 5) % 5][r + dj][c + 1 + di]; };
+    };
+    mc.prime(mu_at(k0 - off - 2));
+    mc.next(mu_at(k0 - off - 1));
     const int send = k0 + (k1 + 2 - k0 + D - 1) / D * D;
     for (int sb = k0; sb < send; sb += D) {
 #pragma unroll
@@ -463,22 +543,25 @@ __global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb2d(Grid g, int nbx, T a
             stage_store(rx[d], rm_[d], s + 2);
             stage_load(rx[d], rm_[d], s + 2 + D);
             RbLevel<C> L;
-            L.mu = mu_raw_lds<COMP, TY, C>(ms, k, r, c);
+            mc.next(mu_at(k));
+            L.mu = mc.raw();
             L.self = xs[k & 3][r + 1][c + 1];
             L.zh = xs[(k + 1) & 3][r + 1][c + 1];
             L.rhs = rh[d];
             rh[d] = rhs(is, j, lev(s + D));
-            // every lane reads both sources and picks by colour: no branch
-            const C pxp = plane[b][r][cp], pxm = plane[b][r][cm], pyp = plane[b][rp][c], pym = plane[b][rm][c];
-            const C sxp = xs[k & 3][r + 1][c + 2], sxm = xs[k & 3][r + 1][c];
-            const C syp = xs[k & 3][r + 2][c + 1], sym = xs[k & 3][r][c + 1];
-            const C xp = red ? sxp : pxp, xm = red ? sxm : pxm, yp = red ? syp : pyp, ym = red ? sym : pym;
+            C xp, xm, yp, ym;
+            if (red) {  // a wavefront-uniform branch: the other colour's old values
+                xp = xs[k & 3][r + 1][c + 2], xm = xs[k & 3][r + 1][c];
+                yp = xs[k & 3][r + 2][c + 1], ym = xs[k & 3][r][c + 1];
+            } else {  // the red plane of this level
+                xp = plane[b][r][cp], xm = plane[b][r][cm], yp = plane[b][rp][c], ym = plane[b][rm][c];
+            }
             C zp, zm;
             rb_z<COMP, C>(k, maxn, L.self, prev, L.zh, zp, zm);
             const C nv = rb_update<COMP, C>(L, ax2, ay2, az2, xp, xm, yp, ym, zp, zm);
             const C v = upd ? nv : L.self;
             prev = on ? v : prev;
-            plane[red && on ? b : 2][r][c] = v;
+            plane[red && on && lane ? b : 2][r][c] = v;
             if (on && upd && inner) xout(i, j, k) = v;
             __syncthreads();
         }
@@ -488,141 +571,219 @@ __global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb2d(Grid g, int nbx, T a
 // Even iteration_time, levels kb..ke of one chunk.  Step s: red lanes update
 // level s+1, black lanes level s-1 (reading the red planes of s-2, s-1, s).
 // The cells of a colour zigzag between the two columns of a lane's pair from
-// one level to the next.  Red levels run from kb-1 to ke+1 (where they
-// exist: the wall levels of w pass their xin value); red planes live in a
-// ring of four, so one barrier per step separates every write from the
-// reads of that plane.
-template <int COMP, int TY, int KS, int D, typename T>
-__global__ void __launch_bounds__((2 * TY + 2) * 32) k_rb3d(Grid g, int nbx, int nby, T ax2, T ay2, T az2,
-                                                            FieldView<T> xin, FieldView<T> xout, FieldView<T> rhs,
-                                                            FieldView<T> mu)
+// one level to the next.  Red levels run from kb-1 to ke+1 (where they exist:
+// the wall levels of w pass their xin value).  Planes are staged as in
+// k_rb2d: at step s the x planes s-1..s+2 and mu planes s-1..s+3 are read
+// (rings of five and six) and stage s+3 is written; red planes live in a ring
+// of four, so one barrier per step separates every write from the reads of
+// that plane.
+template <int COMP, int TX, int TY, int D, typename T>
+__global__ void __launch_bounds__((RbLanes<TX, TY>::NT)) k_rb3d(Grid g, int nbx, int nby, int ks, T ax2, T ay2,
+                                                                T az2, FieldView<T> xin, FieldView<T> xout,
+                                                                FieldView<T> rhs, FieldView<T> mu)
 {
     using C = T;
-    constexpr int RL = (TY + 2) * 32;
-    __shared__ C plane[5][TY + 2][RB_W];  // four level planes and a spare
+    using LN = RbLanes<TX, TY>;
+    constexpr int SR = TY + 4, SW = TX + 4, SN = SR * SW, WR = TX + 2;
+    constexpr int NT = LN::NT, NS = (SN + NT - 1) / NT;
+    __shared__ C xs[5][SR][SW];
+    __shared__ C ms[6][SR][SW];
+    __shared__ C plane[5][TY + 2][WR];  // red values of four levels and a spare
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
-    const int i0 = 1 + bx * RB_TX, j0 = 1 + by * TY;
+    const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
     const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
     const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
-    const int kb = k0 + bz * KS, ke = min(kb + KS - 1, k1);
+    const int kb = k0 + bz * ks, ke = min(kb + ks - 1, k1);
     const int rlo = max(kb - 1, 0), rhi = min(ke + 1, mtop);
-    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < RL;
-    const int q = red ? threadIdx.x : threadIdx.x - RL;
-    const int r = q / 32 + (red ? 0 : 1), l = q % 32;
+    const bool red = __builtin_amdgcn_readfirstlane(threadIdx.x) < LN::RL;
+    const int off = red ? 0 : 1, dl = red ? 1 : -1;  // the level a lane updates at step s is s + dl
+    const int q = red ? threadIdx.x : threadIdx.x - LN::RL;
+    const int per = red ? LN::RPR : LN::BPR;
+    const bool lane = q < (red ? LN::RC : LN::BC);
+    const int r = lane ? q / per + off : 1, h = lane ? q % per : 0;
     const int jj = j0 - 1 + r;
-    const int j = wrapj(jj, P), jp = wrapj(jj + 1, P), jm = wrapj(jj - 1, P);
-    const int par = (i0 + jj + (red ? 0 : 1)) & 1;
-    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0);
-    const int dl = red ? 1 : -1;  // the level a lane updates at step s is s + dl
+    const int j = wrapj(jj, P);
     // the lane's column slot at level m: red where i+j+m is odd
-    auto col = [&](int m) { return 2 * l + ((par + m) & 1); };
-    auto load = [&](RbLevel<C>& L, int m) {
-        const int ii = i0 - 1 + col(m);
-        const int is = min(max(ii, 1), imax), it = min(ii, g.maxl);
-        const int ms = min(max(m, k0), k1), mt = min(max(m, 0), mtop);
-        rb_load<COMP, true, C>(L, xin, rhs, mu, is, it, j, jp, jm, ms, mt, mtop, P);
+    auto col = [&](int m) { return 2 * h + off + ((i0 + jj + m) & 1); };
+    const int rp = min(r + 1, TY + 1), rm = max(r - 1, 0);
+    long ox[NS], om[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int e = min(int(threadIdx.x) + n * NT, SN - 1);
+        const int er = e / SW, ec = e % SW;
+        sl[n] = e;
+        ox[n] = min(max(i0 - 2 + ec, 0), g.maxl) + long(wrapj(j0 - 2 + er, P)) * xin.sy;
+        om[n] = min(i0 - 2 + ec, g.maxl + 1) + long(wrapj(j0 - 1 + er, P)) * mu.sy;
+    }
+    auto stage_load = [&](C* vx, C* vm, int st) {
+        const long kx = min(max(st, 0), mtop) * xin.sz, km = min(max(st + 1, 0), maxn) * mu.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            vx[n] = xin.p[ox[n] + kx];
+            vm[n] = mu.p[om[n] + km];
+        }
+    };
+    // plane p sits in slot (p + 5) % 5 of xs and (p + 6) % 6 of ms; every
+    // plane here is >= -3, so those are positive
+    auto stage_store = [&](const C* vx, const C* vm, int st) {
+        C* px = &xs[(st + 5) % 5][0][0];
+        C* pm = &ms[(st + 1 + 6) % 6][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            px[sl[n]] = vx[n];
+            pm[sl[n]] = vm[n];
+        }
+    };
+    // the lane's RHS at step s (its cell's level and column), clamped into the arrays
+    auto rhs_at = [&](int s) {
+        const int m = min(max(s + dl, k0), k1);
+        return C(rhs(min(max(i0 - 1 + col(m), 1), imax), j, m));
     };
     const int s0 = rlo - 1, send = s0 + (ke + 2 - s0 + D - 1) / D * D;
-    RbLevel<C> ring[D];
+    {
+        C vx[NS], vm[NS];
 #pragma unroll
-    for (int d = 0; d < D; ++d) load(ring[d], s0 + d + dl);
+        for (int u = -1; u <= 2; ++u) {
+            stage_load(vx, vm, s0 + u);
+            stage_store(vx, vm, s0 + u);
+        }
+    }
+    C rx[D][NS], rm_[D][NS], rh[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        stage_load(rx[d], rm_[d], s0 + 3 + d);
+        rh[d] = rhs_at(s0 + d);
+    }
     __builtin_amdgcn_s_waitcnt(0);  // as in k_rb2d
+    __syncthreads();
     for (int sb = s0; sb < send; sb += D) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int s = sb + d, m = s + dl, b = m & 3;
-            const int cc = col(m), ii = i0 - 1 + cc;
-            const int cp = min(cc + 1, RB_W - 1), cm = max(cc - 1, 0);
-            const bool upd = ii >= 1 && ii <= imax && m >= k0 && m <= k1;
-            const bool mine = r >= 1 && r <= TY && cc >= 1 && cc <= RB_TX && m >= kb && m <= ke;
+            stage_store(rx[d], rm_[d], s + 3);
+            stage_load(rx[d], rm_[d], s + 3 + D);
+            const int c = col(m), ii = i0 - 1 + c;
+            const int cp = min(c + 1, WR - 1), cm = max(c - 1, 0);
+            const bool upd = lane && ii >= 1 && ii <= imax && m >= k0 && m <= k1;
+            const bool mine = r >= 1 && r <= TY && c >= 1 && c <= TX && m >= kb && m <= ke;
             const bool on = red ? (m >= rlo && m <= rhi) : (m >= kb && m <= ke);
-            RbLevel<C>& L = ring[d];
-            const C pxp = plane[b][r][cp], pxm = plane[b][r][cm], pyp = plane[b][rp][cc], pym = plane[b][rm][cc];
-            const C plo = plane[(m - 1) & 3][r][cc], phi = plane[(m + 1) & 3][r][cc];
-            const C xp = red ? L.xp : pxp, xm = red ? L.xm : pxm, yp = red ? L.yp : pyp, ym = red ? L.ym : pym;
-            const C lo = red ? L.zl : plo, hi = red ? L.zh : phi;
+            RbLevel<C> L;
+            L.mu = mu_raw_at<COMP, C>(
+                [&](int di, int dj, int dk) { return ms[(m + dk + 6) % 6][r + dj][c + 1 + di]; });
+            L.self = xs[(m + 5) % 5][r + 1][c + 1];
+            L.rhs = rh[d];
+            rh[d] = rhs_at(s + D);
+            C xp, xm, yp, ym, lo, hi;
+            if (red) {  // the other colour's old values
+                const int pm = (m + 5) % 5;
+                xp = xs[pm][r + 1][c + 2], xm = xs[pm][r + 1][c];
+                yp = xs[pm][r + 2][c + 1], ym = xs[pm][r][c + 1];
+                lo = xs[(m + 4) % 5][r + 1][c + 1], hi = xs[(m + 6) % 5][r + 1][c + 1];
+            } else {  // the red planes of this level and the two around it
+                xp = plane[b][r][cp], xm = plane[b][r][cm], yp = plane[b][rp][c], ym = plane[b][rm][c];
+                lo = plane[(m - 1) & 3][r][c], hi = plane[(m + 1) & 3][r][c];
+            }
             C zp, zm;
             rb_z<COMP, C>(m, maxn, L.self, lo, hi, zp, zm);
             const C nv = rb_update<COMP, C>(L, ax2, ay2, az2, xp, xm, yp, ym, zp, zm);
             const C v = upd ? nv : L.self;
-            plane[red && on ? b : 4][r][cc] = v;
+            plane[red && on && lane ? b : 4][r][c] = v;
             if (on && upd && mine) xout(ii, j, m) = v;
-            load(L, s + D + dl);
             __syncthreads();
         }
     }
 }
 
+// Staged planes of a tile of 64 x TY columns: x on rows j0-1 .. j0+TY and mu
+// on rows j0 .. j0+TY+1, both on columns i0-1 .. i0+64, clamped into the
+// arrays (a clamped element is read by no updated cell).  Stage t is x plane
+// t and mu plane t+1.  NT threads share the loads, NS elements each; LDS
+// holds rings of four planes, indexed by plane & 3.
+template <typename C, int TY, int NT>
+struct Staged {
+    static constexpr int R = TY + 2, W = BX + 2, N = R * W, NS = (N + NT - 1) / NT;
+    long ox[NS], om[NS];
+    int sl[NS];
+
+    __device__ void init(const Grid& g, const FieldView<C>& xo, const FieldView<C>& mu, int i0, int j0, int tid)
+    {
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            const int e = min(tid + n * NT, N - 1);
+            const int a = e / W, b = e % W;
+            sl[n] = e;
+            ox[n] = min(max(i0 - 1 + b, 0), g.maxl) + long(wrapj(j0 - 1 + a, g.P)) * xo.sy;
+            om[n] = min(i0 - 1 + b, g.maxl + 1) + long(wrapj(j0 + a, g.P)) * mu.sy;
+        }
+    }
+    // x planes clamped to 0..mtop, mu planes to 0..maxn; no x when WX is false
+    template <bool WX>
+    __device__ void load(C* vx, C* vm, int st, const FieldView<C>& xo, const FieldView<C>& mu, int mtop,
+                         int maxn) const
+    {
+        const long kx = long(min(max(st, 0), mtop)) * xo.sz, km = long(min(max(st + 1, 0), maxn)) * mu.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            if (WX) vx[n] = xo.p[ox[n] + kx];
+            vm[n] = mu.p[om[n] + km];
+        }
+    }
+    template <bool WX>
+    __device__ void store(const C* vx, const C* vm, int st, C (*xs)[R][W], C (*ms)[R][W]) const
+    {
+        C* px = &xs[WX ? (st & 3) : 0][0][0];
+        C* pm = &ms[(st + 1) & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            if (WX) px[sl[n]] = vx[n];
+            pm[sl[n]] = vm[n];
+        }
+    }
+};
+
 // The Jacobi sweep with staged planes.  A block of 64 x TY threads owns
 // 64 x TY columns and a chunk of ks levels.  Level k reads x_old on the planes
-// k-1..k+1 and mu on k..k+2 over the tile and a one-cell halo; the block
-// loads each of those planes once, with coalesced loads (stage t: x plane t
-// and mu plane t+1), D stages ahead into registers, and writes stage k+2 to
-// LDS rings of four planes at step k.  A few loads per lane and level instead
-// of the ~20 scattered ones of k_jacobi, with k_jacobi's arithmetic.  ZERO:
-// x_old is zero (the first iteration), nothing of it is read.
+// k-1..k+1 and mu on k..k+2 over the tile and a one-cell halo: the block
+// loads each plane once, with coalesced loads, D stages ahead into
+// registers, and writes stage k+2 to LDS at step k.  A few loads per lane and
+// level instead of the ~20 scattered ones of k_jacobi, with its arithmetic.
+// ZERO: x_old is zero (the first iteration), nothing of it is read.
 template <int COMP, int TY, int D, bool ZERO, typename T>
 __global__ void __launch_bounds__(BX* TY) k_jacobi_st(Grid g, int nbx, int nby, int ks, T ax2, T ay2, T az2,
                                                      FieldView<T> xo, FieldView<T> rhs, FieldView<T> mu,
                                                      FieldView<T> x)
 {
     using C = T;
-    constexpr int R = TY + 2, W = BX + 2, N = R * W, NT = BX * TY, NS = (N + NT - 1) / NT;
-    __shared__ C xs[ZERO ? 1 : 4][R][W];
-    __shared__ C ms[4][R][W];
+    using S = Staged<C, TY, BX * TY>;
+    constexpr int NS = S::NS;
+    __shared__ C xs[ZERO ? 1 : 4][S::R][S::W];
+    __shared__ C ms[4][S::R][S::W];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
     const int i0 = 1 + bx * BX, j0 = 1 + by * TY;
-    const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
+    const int maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
     const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
     const int kb = k0 + bz * ks, ke = min(kb + ks - 1, k1);
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = i0 + tx, j = j0 + ty;  // j <= P: the tile rows divide the period
     const bool mine = i <= imax;
     const int ic = min(i, imax);
-    // this thread's staged elements: offsets within a plane of x_old (rows
-    // j0-1.., columns i0-1..) and of mu (rows j0.., columns i0-1..), clamped
-    // into the arrays (a clamped element is read by no updated cell)
-    long ox[NS], om[NS];
-    int sl[NS];
-#pragma unroll
-    for (int n = 0; n < NS; ++n) {
-        const int e = min(tid + n * NT, N - 1);
-        const int a = e / W, b = e % W;
-        sl[n] = e;
-        ox[n] = min(max(i0 - 1 + b, 0), g.maxl) + long(wrapj(j0 - 1 + a, P)) * xo.sy;
-        om[n] = min(i0 - 1 + b, g.maxl + 1) + long(wrapj(j0 + a, P)) * mu.sy;
-    }
-    auto stage_load = [&](C* vx, C* vm, int st) {
-        const long kx = long(min(max(st, 0), mtop)) * xo.sz, km = long(min(max(st + 1, 0), maxn)) * mu.sz;
-#pragma unroll
-        for (int n = 0; n < NS; ++n) {
-            if (!ZERO) vx[n] = xo.p[ox[n] + kx];
-            vm[n] = mu.p[om[n] + km];
-        }
-    };
-    auto stage_store = [&](const C* vx, const C* vm, int st) {
-        C* px = &xs[ZERO ? 0 : (st & 3)][0][0];
-        C* pm = &ms[(st + 1) & 3][0][0];
-#pragma unroll
-        for (int n = 0; n < NS; ++n) {
-            if (!ZERO) px[sl[n]] = vx[n];
-            pm[sl[n]] = vm[n];
-        }
-    };
+    S st;
+    st.init(g, xo, mu, i0, j0, ty * BX + tx);
     {
         C vx[NS], vm[NS];
 #pragma unroll
-        for (int st = -1; st <= 1; ++st) {
-            stage_load(vx, vm, kb + st);
-            stage_store(vx, vm, kb + st);
+        for (int u = -1; u <= 1; ++u) {
+            st.template load<!ZERO>(vx, vm, kb + u, xo, mu, mtop, maxn);
+            st.template store<!ZERO>(vx, vm, kb + u, xs, ms);
         }
     }
     C rx[D][NS], rm_[D][NS], rh[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        stage_load(rx[d], rm_[d], kb + 2 + d);
+        st.template load<!ZERO>(rx[d], rm_[d], kb + 2 + d, xo, mu, mtop, maxn);
         rh[d] = rhs(ic, j, min(kb + d, ke));
     }
     // start the loop with nothing in flight (see k_rb2d)
@@ -633,8 +794,8 @@ __global__ void __launch_bounds__(BX* TY) k_jacobi_st(Grid g, int nbx, int nby, 
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int k = sb + d;
-            stage_store(rx[d], rm_[d], k + 2);
-            stage_load(rx[d], rm_[d], k + 2 + D);
+            st.template store<!ZERO>(rx[d], rm_[d], k + 2, xs, ms);
+            st.template load<!ZERO>(rx[d], rm_[d], k + 2 + D, xo, mu, mtop, maxn);
             const C rv = rh[d];
             rh[d] = rhs(ic, j, min(k + D, ke));
             const MuRaw<C> mr =
@@ -651,6 +812,76 @@ __global__ void __launch_bounds__(BX* TY) k_jacobi_st(Grid g, int nbx, int nby, 
             }
             const C v = helmholtz<C>(mu_cook<COMP, C>(mr), ax2, ay2, az2, xp, xm, yp, ym, zp, zm, rv);
             if (mine && k <= ke) x(i, j, k) = v;
+            __syncthreads();
+        }
+    }
+}
+
+// The even-parity SOR colour sweep (3-D colouring, in place) with staged
+// planes: a block of 32 x TY threads owns 64 x TY columns, a thread the two
+// columns (i0+2tx, i0+2tx+1), of which it updates the one of the sweep's
+// colour at each level.  Only that colour is written, and it reads only the
+// other colour and its own old value, so loading the planes ahead is safe in
+// place.  k_sor3d's arithmetic.
+template <int COMP, int TY, int D, typename T>
+__global__ void __launch_bounds__(32 * TY) k_sor3d_st(Grid g, int nbx, int nby, int ks, int flg, T ax2, T ay2,
+                                                     T az2, FieldView<T> x, FieldView<T> rhs, FieldView<T> mu)
+{
+    using C = T;
+    using S = Staged<C, TY, 32 * TY>;
+    constexpr int NS = S::NS;
+    __shared__ C xs[4][S::R][S::W];
+    __shared__ C ms[4][S::R][S::W];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int i0 = 1 + bx * BX, j0 = 1 + by * TY;
+    const int maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
+    const int kb = k0 + bz * ks, ke = min(kb + ks - 1, k1);
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    const int j = j0 + ty;
+    // the column of this thread's pair that holds the sweep's colour at level k
+    auto col = [&](int k) { return 2 * tx + ((k + j + flg) & 1); };
+    S st;
+    st.init(g, x, mu, i0, j0, threadIdx.x);
+    {
+        C vx[NS], vm[NS];
+#pragma unroll
+        for (int u = -1; u <= 1; ++u) {
+            st.template load<true>(vx, vm, kb + u, x, mu, mtop, maxn);
+            st.template store<true>(vx, vm, kb + u, xs, ms);
+        }
+    }
+    C rx[D][NS], rm_[D][NS], rh[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        st.template load<true>(rx[d], rm_[d], kb + 2 + d, x, mu, mtop, maxn);
+        const int kd = min(kb + d, ke);
+        rh[d] = rhs(min(i0 + col(kd), imax), j, kd);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int kend = kb + (ke - kb + D) / D * D;
+    for (int sb = kb; sb < kend; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int k = sb + d;
+            st.template store<true>(rx[d], rm_[d], k + 2, xs, ms);
+            st.template load<true>(rx[d], rm_[d], k + 2 + D, x, mu, mtop, maxn);
+            const C rv = rh[d];
+            const int kn = min(k + D, ke);
+            rh[d] = rhs(min(i0 + col(kn), imax), j, kn);
+            const int c = col(k), i = i0 + c;
+            const MuRaw<C> mr =
+                mu_raw_at<COMP, C>([&](int di, int dj, int dk) { return ms[(k + dk) & 3][ty + dj][c + 1 + di]; });
+            const int b = k & 3;
+            const C cur = xs[b][ty + 1][c + 1];
+            C zp, zm;
+            rb_z<COMP, C>(k, maxn, cur, C(xs[(k - 1) & 3][ty + 1][c + 1]), C(xs[(k + 1) & 3][ty + 1][c + 1]), zp,
+                          zm);
+            const C res = helmholtz<C>(mu_cook<COMP, C>(mr), ax2, ay2, az2, C(xs[b][ty + 1][c + 2]),
+                                       C(xs[b][ty + 1][c]), C(xs[b][ty + 2][c + 1]), C(xs[b][ty][c + 1]), zp, zm, rv);
+            if (i <= imax && k <= ke) x(i, j, k) = cur + C(1.2) * (res - cur);
             __syncthreads();
         }
     }
@@ -770,6 +1001,22 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
     const int nbx = (half + BX - 1) / BX;
     auto go = [&](auto comp_c) {
         constexpr int CP = decltype(comp_c)::value;
+        if (iteration_time % 2 == 0 && tl.jst) {
+            // staged planes, as the Jacobi sweep: rows that divide the period, chunks of 32 levels
+            const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
+            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, ks = 32;
+            const int nbx3 = (imax + BX - 1) / BX, nbz = (nz_of(CP, g) + ks - 1) / ks;
+            auto go_st = [&](auto ty_c) {
+                constexpr int TY = decltype(ty_c)::value;
+                const int nby = g.P / TY;
+                hipLaunchKernelGGL((k_sor3d_st<CP, TY, 8, T>), dim3(nbx3 * nby * nbz), dim3(32 * TY), 0, s, g, nbx3,
+                                   nby, ks, flg, a2, b2, c2, x, rhs, mu);
+                return hipGetLastError();
+            };
+            if (ty == 8) return go_st(std::integral_constant<int, 8>{});
+            if (ty == 4) return go_st(std::integral_constant<int, 4>{});
+            return go_st(std::integral_constant<int, 2>{});
+        }
         if (iteration_time % 2 == 0) {
             return with_rows<CP, T>(tl.rows, [&](auto ry) {
                 constexpr int RY = decltype(ry)::value;
@@ -798,7 +1045,7 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
 // Tile rows of a fused launch: the requested count where it divides the
 // period, else 2 (the period is even).
 static inline int rb_rows(int want, int P) { return (want == 4 || want == 8) && P % want == 0 ? want : 2; }
-constexpr int RB_KS = 16;  // levels per chunk of the 3-D colouring pass
+constexpr int RB_KS = 32;  // levels per chunk of the 3-D colouring pass
 
 template <typename T>
 hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xin, FieldView<T> xout,
@@ -817,14 +1064,18 @@ hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int com
             const int nby = g.P / TY;
             const dim3 blk((2 * TY + 2) * 32);
             if (odd) {
-                // ring depth: 8 stages, 4 where 8 would spill (FP64 with 8-row tiles)
-                constexpr int D = (TY == 8 && sizeof(T) == 8) ? 4 : 8;
-                hipLaunchKernelGGL((k_rb2d<CP, TY, D, T>), dim3(nbx * nby), blk, 0, s, g, nbx, a2, b2, c2, xin, xout,
-                                   rhs, mu);
+                // tiles of 8 rows are 32 columns wide (16 x 16 tiles fill the
+                // chip at 512 x 128), the others 62 (a 64-column row with its
+                // ring)
+                constexpr int TX = TY == 8 ? 32 : 62;
+                const int nbx2 = (imax + TX - 1) / TX;
+                hipLaunchKernelGGL((k_rb2d<CP, TX, TY, 8, T>), dim3(nbx2 * nby), dim3(RbLanes<TX, TY>::NT), 0, s, g,
+                                   nbx2, a2, b2, c2, xin, xout, rhs, mu);
             } else {
-                const int nk = (nz_of(CP, g) + RB_KS - 1) / RB_KS;
-                hipLaunchKernelGGL((k_rb3d<CP, TY, RB_KS, 2, T>), dim3(nbx * nby * nk), blk, 0, s, g, nbx, nby, a2, b2, c2,
-                                   xin, xout, rhs, mu);
+                constexpr int TX = TY == 8 ? 32 : 62;
+                const int nbx3 = (imax + TX - 1) / TX, nk = (nz_of(CP, g) + RB_KS - 1) / RB_KS;
+                hipLaunchKernelGGL((k_rb3d<CP, TX, TY, 8, T>), dim3(nbx3 * nby * nk), dim3(RbLanes<TX, TY>::NT), 0, s,
+                                   g, nbx3, nby, RB_KS, a2, b2, c2, xin, xout, rhs, mu);
             }
             return hipGetLastError();
         };

@@ -72,10 +72,10 @@ hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int com
 // odd sweep's prefetch depth.  Every choice computes the same values.
 struct Tiling {
     int rows = 4, rows_odd = 4, depth_odd = 4;
-    // both SOR sweeps of an iteration in one out-of-place pass (0: the two
-    // in-place sweeps), with tiles of rb_rows (even parity) / rb_rows_odd
-    // rows
-    int fused = 0, rb_rows = 4, rb_rows_odd = 8;
+    // both SOR sweeps of an iteration in one out-of-place pass (bit 0: odd
+    // iteration_time, bit 1: even; else the two in-place sweeps), with tiles
+    // of rb_rows (even parity) / rb_rows_odd rows
+    int fused = 1, rb_rows = 8, rb_rows_odd = 8;
     // Jacobi sweep with staged planes and tiles of jst rows (0: the unstaged
     // sweep with `rows`)
     int jst = 4;
