@@ -421,16 +421,6 @@ struct MuColumn {
     }
 };
 
-// Lanes of a fused-pass tile of TX x TY interior columns and its one-cell
-// ring: RPR red cells in each of the TY+2 rows, BPR black cells in each of
-// the TY interior rows; each colour's lanes rounded up to whole wavefronts.
-template <int TX, int TY>
-struct RbLanes {
-    static constexpr int RPR = (TX + 2) / 2, BPR = TX / 2;
-    static constexpr int RC = (TY + 2) * RPR, BC = TY * BPR;
-    static constexpr int RL = (RC + 63) / 64 * 64, BL = (BC + 63) / 64 * 64, NT = RL + BL;
-};
-
 // Odd iteration_time.  Step s: red lanes update level s, black lanes level
 // s-1 from the red plane of level s-1.  Red lanes march their column's red
 // cell and black lanes their column's black cell, so `prev` is the updated

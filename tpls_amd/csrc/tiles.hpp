@@ -49,4 +49,15 @@ __device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int&
     k0 = bz * CG_KC + 1;
 }
 
+// Lanes of a fused red-black pass over a tile of TX x TY interior columns and
+// its one-cell ring: RPR cells of the first colour in each of the TY+2 rows,
+// BPR cells of the second in each of the TY interior rows; each colour's
+// lanes rounded up to whole wavefronts, so that a wavefront is of one colour.
+template <int TX, int TY>
+struct RbLanes {
+    static constexpr int RPR = (TX + 2) / 2, BPR = TX / 2;
+    static constexpr int RC = (TY + 2) * RPR, BC = TY * BPR;
+    static constexpr int RL = (RC + 63) / 64 * 64, BL = (BC + 63) / 64 * 64, NT = RL + BL;
+};
+
 }  // namespace tpls
