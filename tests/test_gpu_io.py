@@ -1,0 +1,85 @@
+"""A case read from a TPLS configuration directory on the GPU
+(tpls_case_load_config, main_ns_hybrid.F90:242-410) and fields saved from
+the device in the data-file format (tpls_case_save_field).
+
+A case configured from files must step bit-identically to the same case
+built from the arrays and parameters directly (grid spacings that F20.10
+represents exactly, so both hold the same parameters); a field saved from the
+device must read back bit-identically to the field copied from it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tpls_amd import io
+from tpls_amd.config import CaseParams
+from tpls_amd.initial import make_initial
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u2", "v2", "w2", "phi2", "pres")
+
+
+def write_case(d, p):
+    init = make_initial(p)
+    io.write_initial_conditions(d, p, init)
+    io.write_options_file(os.path.join(d, "tpls_config.opt"), p, n_timesteps=3)
+    return init
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_case_from_config_steps_like_case_from_arrays(tmp_path, precision):
+    from tpls_amd.case import Case
+    # nz = 32: dx = 1/32 and smooth_width = 1.5/32 are exact in F20.10
+    p = CaseParams(nx=48, ny=16, nz=32, Re=80.0, mu_plus=2.0, dt=1e-4, seed=6)
+    init = write_case(str(tmp_path), p)
+    print(f"case from config, 48x16x32 {precision}", flush=True)
+    with Case.from_config(tmp_path, tmp_path / "tpls_config.opt", precision=precision) as a, \
+            Case(p, precision=precision) as b:
+        assert a.run.n_timesteps == 3
+        b.load(*init)
+        for f in FIELDS:
+            assert np.array_equal(a.field(f), b.field(f)), f
+        a.step(a.run.n_timesteps)
+        b.step(3)
+        for f in FIELDS:
+            assert np.array_equal(a.field(f), b.field(f)), f
+
+
+def test_save_field_reads_back(tmp_path):
+    from tpls_amd.case import Case
+    p = CaseParams(nx=32, ny=8, nz=16, seed=3)
+    print("save_field, 32x8x16", flush=True)
+    with Case(p) as c:
+        c.load(*make_initial(p))
+        c.step(1)
+        for name, desc in (("u2", "velocity (u)"), ("w2", "velocity (w)"), ("pres", "pressure"),
+                           ("phi2", "level-set (phi)")):
+            f = tmp_path / f"{name}.dat"
+            c.save_field(name, f, desc)
+            data, lx, ly, mx, my, nx, ny, ierr = io.load_data(f)
+            want = c.field(name)
+            assert ierr == 0 and (lx, mx, nx) == (0, 0, 0)
+            assert data.shape == want.shape
+            assert np.array_equal(data.view(np.int64), want.view(np.int64)), name
+
+
+def test_load_config_errors(tmp_path):
+    from tpls_amd.case import Case
+    p = CaseParams(nx=16, ny=8, nz=16, seed=1)
+    write_case(str(tmp_path), p)
+    print("load_config errors, 16x8x16", flush=True)
+    with Case(p) as c:
+        c.load_config(tmp_path)
+        q = CaseParams(nx=16, ny=8, nz=12, seed=1)
+        other = tmp_path / "other"
+        write_case(str(other), q)
+        os.replace(other / "initial_w.dat", tmp_path / "initial_w.dat")
+        with pytest.raises(io.ConfigError) as e:
+            c.load_config(tmp_path)
+        assert e.value.ierr == 3 and "initial_w.dat data dimensions do not match" in str(e.value)
+        os.remove(tmp_path / "initial_viscosity.dat")
+        with pytest.raises(io.ConfigError) as e:
+            c.load_config(tmp_path)
+        assert e.value.ierr == 200

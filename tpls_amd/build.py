@@ -6,7 +6,9 @@ linked into one shared library exporting the C ABI of
 ``include/tpls_core.h``.  ``tpls_amd/fortran/tpls_modules.f90`` (the modules
 with the reference's names that main_ns_hybrid.F90 uses) is compiled with
 amdflang into ``tpls_amd/fortran/libtpls_fortran.a`` plus its ``.mod`` files,
-and the test driver ``tests/fortran/drive_modules`` is linked against both.
+and the test driver ``tests/fortran/drive_modules`` is linked against both;
+``tests/fortran/drive_io`` (Fortran list-directed I/O) is built for the
+data-format tests.
 Usage: ``python -m tpls_amd.build``.
 """
 from __future__ import annotations
@@ -26,6 +28,8 @@ FMOD = os.path.join(FDIR, "mod")
 FLIB = os.path.join(FDIR, "libtpls_fortran.a")
 DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drive_modules.f90")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_modules")
+DRIVE_IO_SRC = os.path.join(ROOT, "tests", "fortran", "drive_io.f90")
+DRIVE_IO = os.path.join(ROOT, "tests", "fortran", "drive_io")
 FC = os.environ.get("TPLS_FC", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -91,6 +95,8 @@ def build_fortran(verbose: bool = True) -> None:
     _run([FC, "-O2", "-module-dir", FMOD, "-I", FMOD, "-c", DRIVER_SRC, "-o", dobj], "amdflang drive_modules.f90")
     _run([FC, "-o", DRIVER, dobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
          "link drive_modules")
+    # Fortran list-directed I/O in the data-file layout (tests/test_io.py)
+    _run([FC, "-O2", "-module-dir", FMOD, "-o", DRIVE_IO, DRIVE_IO_SRC], "amdflang drive_io.f90")
     if verbose:
         print(f"built {FLIB} and {DRIVER}")
 

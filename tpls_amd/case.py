@@ -54,9 +54,14 @@ def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True
     return prm
 
 
+# The tpls_params fields a TPLS configuration sets (include/tpls_io.h).
+FILE_FIELDS = ("dx", "dy", "dz", "dt", "Re", "mu_plus", "mu_minus", "height", "smooth_width", "scap",
+               "mom_u", "mom_v", "mom_w", "levelset_its")
+
+
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                 graphs: bool = False, tiles=None, sor_fused=None):
+                 graphs: bool = False, tiles=None, sor_fused=None, file_params: _lib.Params | None = None):
         """graphs: replay each step's launches from HIP graphs (same results).
         tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst[, sor_fused]]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
@@ -64,7 +69,9 @@ class Case:
         SOR iterations run as one out-of-place pass instead of two in-place
         colour sweeps (None: the default, odd iteration_time; True / False:
         all / none; or the library's code).  Every choice computes the same
-        values."""
+        values.  file_params: the tpls_params of a configuration
+        (tpls_amd.io.read_configuration), whose file-backed fields replace
+        those params implies."""
         if tiles == "auto":
             from .autotune import tuned
             tiles = tuned(params, precision, fused_code(sor_fused))
@@ -74,10 +81,50 @@ class Case:
         self.shape = (params.maxl, params.maxm, params.maxn)
         self.tiles = tiles
         self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused)
+        if file_params is not None:
+            for k in FILE_FIELDS:
+                setattr(self._prm, k, getattr(file_params, k))
         h = C.c_void_p()
         L = _lib.lib()
         _lib.check(L.tpls_case_create(C.byref(self._prm), *self.shape, C.byref(h)), "tpls_case_create")
         self._h = h
+
+    @classmethod
+    def from_config(cls, config_dir, options_file, precision: str = "fp64", diagnostics: bool = True,
+                    graphs: bool = False, tiles=None, sor_fused=None, **solver):
+        """A case configured and initialised as main_ns_hybrid.F90:242-410
+        does it: the options file and config_dir/initial_config.dat give the
+        parameters (exactly as written there), the initial_*.dat files of
+        config_dir the state.  solver: the CaseParams fields the files do not
+        hold (ksp_pc, ksp_rtol, ...).  The run configuration is kept as
+        ``case.run``."""
+        from . import io
+        prm, run = io.read_configuration(config_dir, options_file)
+        if run.idm_dim:
+            raise _lib.TplsError("idm dim (the diffuse interface method) is not built; only idm lsm runs")
+        p = CaseParams(nx=run.maxl - 1, ny=run.maxm - 1, nz=run.maxn - 1, Re=prm.Re, mu_minus=prm.mu_minus,
+                       mu_plus=prm.mu_plus, height=prm.height, scap=prm.scap, dt=prm.dt,
+                       smooth_width_scale=prm.smooth_width * (run.maxn - 1), mom_u=prm.mom_u,
+                       mom_v=prm.mom_v, mom_w=prm.mom_w, levelset_its=prm.levelset_its, **solver)
+        case = cls(p, precision, diagnostics, graphs, tiles, sor_fused, file_params=prm)
+        case.run = run
+        case.load_config(config_dir)
+        return case
+
+    def load_config(self, config_dir) -> None:
+        """Initial state from config_dir/initial_*.dat (main_ns_hybrid.F90:380-393)."""
+        from . import io
+        ierr = C.c_int(0)
+        rc = io._io().tpls_case_load_config(self._h, io._b(config_dir), C.byref(ierr))
+        if rc == -8:
+            raise io.ConfigError(ierr.value, _lib.lib().tpls_last_error().decode(errors="replace"))
+        _lib.check(rc, "tpls_case_load_config")
+
+    def save_field(self, name: str, filename, description: str) -> None:
+        """save_data of a field (io.F90:47-73) straight from the device."""
+        from . import io
+        _lib.check(io._io().tpls_case_save_field(self._h, _lib.FIELD_ID[name], io._b(filename),
+                                                 description.encode()), f"save {name}")
 
     @staticmethod
     def device_bytes(params: CaseParams, precision: str = "fp64") -> int:

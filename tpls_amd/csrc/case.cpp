@@ -11,6 +11,8 @@
 #include <vector>
 
 #include "../../include/tpls_core.h"
+#include "../../include/tpls_io.h"
+#include "host.hpp"
 #include "tpls_kernels.hpp"
 
 using namespace tpls;
@@ -278,6 +280,17 @@ DevArray* field_ptr(tpls_case* c, int f)
 
 }  // namespace
 
+int tpls::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int tpls::case_dims(const tpls_case* c, int* maxl, int* maxm, int* maxn)
+{
+    if (!c || !maxl || !maxm || !maxn) return fail(TPLS_ERR_ARG, "null argument");
+    *maxl = c->maxl;
+    *maxm = c->maxm;
+    *maxn = c->maxn;
+    return TPLS_OK;
+}
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -494,6 +507,16 @@ int tpls_case_set_field(tpls_case* c, int f, const double* host)
         return put_array(c, c->pold, host);
     }
     return put_array(c, a, host);
+}
+
+int tpls_case_save_field(tpls_case* c, int f, const char* filename, const char* description)
+{
+    if (!c || !filename || !description || f < 0 || f >= TPLS_NFIELDS) return fail(TPLS_ERR_ARG, "bad argument");
+    const Shape sh = shape_of(grid_of(f), c->maxl, c->maxm, c->maxn);
+    std::vector<double> host(size_t(sh.count()));
+    int rc = tpls_case_get_field(c, f, host.data());
+    if (rc) return rc;
+    return tpls_save_data(filename, description, host.data(), sh.lx, sh.ux, sh.ly, sh.uy, sh.lz, sh.uz);
 }
 
 int tpls_case_diag(tpls_case* c, tpls_diag* out)

@@ -1,0 +1,355 @@
+"""The reference's input formats through the library (include/tpls_io.h).
+
+Mirrors the procedures of io.F90 (save_data, load_data,
+load_data_into_array: io.F90:47-169) and options_utils.F90 (has_option,
+get_*_option, put_*_option, load_options, save_options, append_options:
+options_utils.F90:56-374) with the reference's names, argument meaning and
+status codes.  Where the reference returns a status through ``ierr`` the
+Python function returns it; hard failures (a file that cannot be read, a
+value of the wrong type, where the reference's Fortran runtime would stop
+the program) raise ``TplsError``.
+
+Also: ``read_configuration`` (main_ns_hybrid.F90:242-311 through
+configure.F90) and ``write_initial_conditions`` (the files
+create_initial_conditions_program.F90:105-169 writes).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import TplsError, check
+
+OPTION_NOT_FOUND_ERR = 100
+OPTION_FORMAT_ERR = 101
+OPTION_INVALID_ERR = 102
+OPTION_FILE_NOT_FOUND_ERR = 103
+CONFIG_FILE_NOT_FOUND_ERR = 200
+CONFIG_DIR_NOT_FOUND_ERR = 201
+
+# option_names.F90:15-27
+INITIAL_FILES = {"u": "initial_u.dat", "v": "initial_v.dat", "w": "initial_w.dat",
+                 "pressure": "initial_pressure.dat", "phi": "initial_phi.dat",
+                 "viscosity": "initial_viscosity.dat"}
+INITIAL_CONFIG_FILE = "initial_config.dat"
+
+_D = C.POINTER(C.c_double)
+_OPT = C.c_void_p
+
+
+class RunConfig(C.Structure):
+    """tpls_run_config of include/tpls_io.h"""
+    _fields_ = [("maxl", C.c_int), ("maxm", C.c_int), ("maxn", C.c_int),
+                ("dpdl", C.c_double), ("omega_max", C.c_double), ("maxu", C.c_double),
+                ("idm_dim", C.c_int), ("Pe", C.c_double), ("epn", C.c_double),
+                ("max_iteration_dim", C.c_int), ("auto_decomp", C.c_int),
+                ("num_procs_x", C.c_int), ("num_procs_y", C.c_int), ("n_timesteps", C.c_int),
+                ("phi_dat_frequency", C.c_int), ("uvw_dat_frequency", C.c_int),
+                ("backup_frequency", C.c_int), ("backup_text_format", C.c_int),
+                ("backup_hdf5_format", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_bound = False
+
+
+def _io():
+    global _bound
+    L = _lib.lib()
+    if not _bound:
+        s, i, ip = C.c_char_p, C.c_int, C.POINTER(C.c_int)
+        L.tpls_save_data.argtypes = [s, s, _D] + [i] * 6
+        L.tpls_data_bounds.argtypes = [s, ip]
+        L.tpls_load_data.argtypes = [s, C.POINTER(_D), ip, ip]
+        L.tpls_load_data_into_array.argtypes = [s, _D] + [i] * 6 + [ip]
+        L.tpls_free.argtypes = [C.c_void_p]
+        L.tpls_free.restype = None
+        L.tpls_options_create.argtypes = [C.POINTER(_OPT)]
+        L.tpls_options_destroy.argtypes = [_OPT]
+        L.tpls_options_count.argtypes = [_OPT]
+        L.tpls_options_entry.argtypes = [_OPT, i, C.POINTER(s), C.POINTER(s)]
+        L.tpls_has_option.argtypes = [_OPT, s]
+        L.tpls_get_option.argtypes = [_OPT, s, C.c_char_p, C.c_size_t, ip]
+        L.tpls_get_integer_option.argtypes = [_OPT, s, ip, ip]
+        L.tpls_get_double_option.argtypes = [_OPT, s, _D, ip]
+        L.tpls_get_logical_option.argtypes = [_OPT, s, ip, ip]
+        L.tpls_put_option.argtypes = [_OPT, s, s]
+        L.tpls_put_integer_option.argtypes = [_OPT, s, i]
+        L.tpls_put_double_option.argtypes = [_OPT, s, C.c_double]
+        L.tpls_put_logical_option.argtypes = [_OPT, s, i]
+        L.tpls_load_options.argtypes = [s, C.POINTER(_OPT), ip]
+        L.tpls_save_options.argtypes = [s, _OPT, C.POINTER(s), i]
+        L.tpls_append_options.argtypes = [s, _OPT]
+        L.tpls_read_config.argtypes = [s, s, C.POINTER(_lib.Params), C.POINTER(RunConfig), ip]
+        L.tpls_case_load_config.argtypes = [C.c_void_p, s, ip]
+        L.tpls_case_save_field.argtypes = [C.c_void_p, i, s, s]
+        _bound = True
+    return L
+
+
+def _b(s) -> bytes:
+    return os.fsencode(s) if not isinstance(s, bytes) else s
+
+
+def _fortran_array(data, shape):
+    a = np.asarray(data, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"array shape {a.shape} does not match the bounds {shape}")
+    return np.asfortranarray(a)
+
+
+def _shape(lx, ly, mx, my, nx, ny):
+    return (max(0, ly - lx + 1), max(0, my - mx + 1), max(0, ny - nx + 1))
+
+
+# ---------------------------------------------------------------------------
+# 3-D data files (io.F90:47-169)
+
+def save_data(filename, description: str, data, lx: int, ly: int, mx: int, my: int, nx: int, ny: int) -> None:
+    """Write data(lx:ly,mx:my,nx:ny) (io.F90:47-73)."""
+    a = _fortran_array(data, _shape(lx, ly, mx, my, nx, ny))
+    check(_io().tpls_save_data(_b(filename), description.encode(), a.ctypes.data_as(_D),
+                               lx, ly, mx, my, nx, ny), "save_data")
+
+
+def data_bounds(filename) -> tuple:
+    b = (C.c_int * 6)()
+    check(_io().tpls_data_bounds(_b(filename), b), "data_bounds")
+    return tuple(b)
+
+
+def load_data(filename):
+    """io.F90:83-116: returns (data, lx, ly, mx, my, nx, ny, ierr); data is a
+    Fortran-ordered array of the recorded bounds, ierr = 1 when the file
+    holds fewer values than its bounds."""
+    L = _io()
+    p = _D()
+    b = (C.c_int * 6)()
+    ierr = C.c_int(0)
+    check(L.tpls_load_data(_b(filename), C.byref(p), b, C.byref(ierr)), "load_data")
+    try:
+        shape = _shape(*b)
+        n = int(np.prod(shape))
+        flat = np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].copy()
+    finally:
+        L.tpls_free(p)
+    return (flat.reshape(shape, order="F"), *tuple(b), ierr.value)
+
+
+def load_data_into_array(filename, data: np.ndarray, lx: int, ly: int, mx: int, my: int, nx: int,
+                         ny: int) -> int:
+    """io.F90:127-169: fills data (a Fortran-contiguous float64 array of the
+    given bounds) and returns ierr: 1, 2 or 3 when the file's l, m or n
+    bounds differ (data untouched), else 0."""
+    if not (isinstance(data, np.ndarray) and data.dtype == np.float64 and data.flags.f_contiguous):
+        raise TypeError("data must be a Fortran-contiguous float64 array")
+    if data.shape != _shape(lx, ly, mx, my, nx, ny):
+        raise ValueError(f"array shape {data.shape} does not match the bounds")
+    ierr = C.c_int(0)
+    check(_io().tpls_load_data_into_array(_b(filename), data.ctypes.data_as(_D), lx, ly, mx, my, nx, ny,
+                                          C.byref(ierr)), "load_data_into_array")
+    return ierr.value
+
+
+# ---------------------------------------------------------------------------
+# NAME VALUE options (options_utils.F90)
+
+class Options:
+    """An options table: the (options, num_options) pair of options_utils.F90."""
+
+    def __init__(self, _handle=None):
+        if _handle is None:
+            h = _OPT()
+            check(_io().tpls_options_create(C.byref(h)), "tpls_options_create")
+            _handle = h.value
+        self._h = _handle
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            _io().tpls_options_destroy(h)
+
+    @property
+    def num_options(self) -> int:
+        return _io().tpls_options_count(self._h)
+
+    def __len__(self) -> int:
+        return self.num_options
+
+    def items(self) -> list:
+        L = _io()
+        out = []
+        for i in range(self.num_options):
+            n, v = C.c_char_p(), C.c_char_p()
+            check(L.tpls_options_entry(self._h, i, C.byref(n), C.byref(v)), "tpls_options_entry")
+            out.append((n.value.decode(), v.value.decode()))
+        return out
+
+    def has_option(self, name: str) -> bool:
+        return bool(_io().tpls_has_option(self._h, name.encode()))
+
+    def get_option(self, name: str):
+        """(value, ierr); value is '' when ierr is OPTION_NOT_FOUND_ERR."""
+        buf = C.create_string_buffer(4096)
+        ierr = C.c_int(0)
+        check(_io().tpls_get_option(self._h, name.encode(), buf, len(buf), C.byref(ierr)), "get_option")
+        return (buf.value.decode() if ierr.value == 0 else ""), ierr.value
+
+    get_string_option = get_option
+
+    def _typed(self, fn, name, box, what):
+        ierr = C.c_int(0)
+        check(fn(self._h, name.encode(), C.byref(box), C.byref(ierr)), what)
+        return ierr.value
+
+    def get_integer_option(self, name: str):
+        v = C.c_int(0)
+        ierr = self._typed(_io().tpls_get_integer_option, name, v, "get_integer_option")
+        return (v.value if ierr == 0 else None), ierr
+
+    def get_double_option(self, name: str):
+        v = C.c_double(0.0)
+        ierr = self._typed(_io().tpls_get_double_option, name, v, "get_double_option")
+        return (v.value if ierr == 0 else None), ierr
+
+    def get_logical_option(self, name: str):
+        v = C.c_int(0)
+        ierr = self._typed(_io().tpls_get_logical_option, name, v, "get_logical_option")
+        return (bool(v.value) if ierr == 0 else None), ierr
+
+    def get_typed_option(self, name: str, kind: type):
+        """The get_typed_option interface: kind is bool, int, float or str."""
+        return {bool: self.get_logical_option, int: self.get_integer_option,
+                float: self.get_double_option, str: self.get_option}[kind](name)
+
+    def put_option(self, name: str, value: str) -> None:
+        check(_io().tpls_put_option(self._h, name.encode(), value.encode()), "put_option")
+
+    put_string_option = put_option
+
+    def put_integer_option(self, name: str, value: int) -> None:
+        check(_io().tpls_put_integer_option(self._h, name.encode(), int(value)), "put_integer_option")
+
+    def put_double_option(self, name: str, value: float) -> None:
+        check(_io().tpls_put_double_option(self._h, name.encode(), float(value)), "put_double_option")
+
+    def put_logical_option(self, name: str, value: bool) -> None:
+        check(_io().tpls_put_logical_option(self._h, name.encode(), int(bool(value))), "put_logical_option")
+
+    def put_typed_option(self, name: str, value) -> None:
+        """The put_typed_option interface, dispatched on the Python type."""
+        if isinstance(value, (bool, np.bool_)):
+            self.put_logical_option(name, bool(value))
+        elif isinstance(value, (int, np.integer)):
+            self.put_integer_option(name, int(value))
+        elif isinstance(value, (float, np.floating)):
+            self.put_double_option(name, float(value))
+        elif isinstance(value, str):
+            self.put_option(name, value)
+        else:
+            raise TypeError(f"no typed option for {type(value).__name__}")
+
+    def save_options(self, filename, hdr=()) -> None:
+        lines = [h.encode() for h in hdr]
+        arr = (C.c_char_p * max(1, len(lines)))(*lines)
+        check(_io().tpls_save_options(_b(filename), self._h, arr, len(lines)), "save_options")
+
+    def append_options(self, filename) -> None:
+        check(_io().tpls_append_options(_b(filename), self._h), "append_options")
+
+    @classmethod
+    def load_options(cls, filename):
+        """(options, ierr): options is None when ierr is
+        OPTION_FILE_NOT_FOUND_ERR; with OPTION_FORMAT_ERR it holds the
+        options read before the bad line."""
+        h = _OPT()
+        ierr = C.c_int(0)
+        check(_io().tpls_load_options(_b(filename), C.byref(h), C.byref(ierr)), "load_options")
+        return (cls(h.value) if h.value else None), ierr.value
+
+
+def load_options(filename):
+    return Options.load_options(filename)
+
+
+# ---------------------------------------------------------------------------
+# Run configuration and initial-condition files
+
+class ConfigError(TplsError):
+    def __init__(self, ierr: int, message: str):
+        super().__init__(message)
+        self.ierr = ierr
+
+
+def read_configuration(config_dir, options_file, params: _lib.Params | None = None):
+    """Read config_dir/initial_config.dat and options_file as
+    main_ns_hybrid.F90:242-311 does.  Returns (params, run): the tpls_params
+    with its file-backed fields set (other fields as given) and the
+    RunConfig.  A failed check raises ConfigError carrying the reference's
+    code (ierr) and message."""
+    prm = params if params is not None else _lib.Params()
+    run = RunConfig()
+    ierr = C.c_int(0)
+    rc = _io().tpls_read_config(_b(config_dir), _b(options_file), C.byref(prm), C.byref(run), C.byref(ierr))
+    if rc == -8:
+        raise ConfigError(ierr.value, _io().tpls_last_error().decode(errors="replace"))
+    check(rc, "read_configuration")
+    return prm, run
+
+
+def initial_viscosity(phi: np.ndarray, smooth_width: float, Re: float, mu_minus: float,
+                      mu_plus: float) -> np.ndarray:
+    """Smoothed-Heaviside viscosity of a level set (momentum_allflux.f90:499-512)."""
+    sw = smooth_width
+    H = np.where(phi < -sw, 0.0,
+                 np.where(phi > sw, 1.0, 0.5 + phi / (2 * sw) + np.sin(np.pi * phi / sw) / (2 * np.pi)))
+    return np.asfortranarray((mu_plus * H + mu_minus * (1.0 - H)) / Re)
+
+
+def write_initial_conditions(config_dir, p, fields, dpdl: float = -1.0, omega_max: float = 0.0) -> None:
+    """Write the initial-condition files of a case in the layout
+    create_initial_conditions_program.F90:105-169 writes: the six
+    initial_*.dat arrays and initial_config.dat.  p is a CaseParams, fields
+    the (u, v, w, pres, phi) arrays of make_initial."""
+    u, v, w, pres, phi = fields
+    maxl, maxm, maxn = p.maxl, p.maxm, p.maxn
+    os.makedirs(config_dir, exist_ok=True)
+    path = lambda f: os.path.join(config_dir, f)  # noqa: E731
+    visc = initial_viscosity(phi, p.smooth_width, p.Re, p.mu_minus, p.mu_plus)
+    save_data(path(INITIAL_FILES["u"]), "Initial velocity (u)", u, 0, maxl, 0, maxm, 0, maxn - 2)
+    save_data(path(INITIAL_FILES["v"]), "Initial velocity (v)", v, 0, maxl, 0, maxm, 0, maxn - 2)
+    save_data(path(INITIAL_FILES["w"]), "Initial velocity (w)", w, 0, maxl, 0, maxm, 0, maxn - 1)
+    save_data(path(INITIAL_FILES["pressure"]), "Initial pressure", pres, 0, maxl, 0, maxm, 0, maxn)
+    save_data(path(INITIAL_FILES["phi"]), "Initial level-set (phi)", phi, 0, maxl, 0, maxm, 0, maxn)
+    save_data(path(INITIAL_FILES["viscosity"]), "Initial viscosity", visc, 0, maxl, 0, maxm, 0, maxn)
+    o = Options()
+    for name, value in (("maxl", maxl), ("maxm", maxm), ("maxn", maxn), ("re", float(p.Re)),
+                        ("dpdl", float(dpdl)), ("mu_plus", float(p.mu_plus)),
+                        ("mu_minus", float(p.mu_minus)), ("height", float(p.height)),
+                        ("scap", float(p.scap)), ("omega_max", float(omega_max)),
+                        ("smooth_width", float(p.smooth_width)), ("dx", float(p.dx)),
+                        ("dy", float(p.dx)), ("dz", float(p.dx)), ("dt", float(p.dt)),
+                        ("idm", "lsm")):
+        o.put_typed_option(name, value)
+    o.save_options(path(INITIAL_CONFIG_FILE),
+                   ["TPLS initial conditions", "Auto-generated by tpls_amd.io"])
+
+
+def write_options_file(filename, p, n_timesteps: int = 1, maxu: float = 10.0) -> None:
+    """A tpls_config.opt for a case: the keys configure.F90 requires, one
+    process grid cell, and p's iteration counts."""
+    o = Options()
+    for name, value in (("num_procs_x", 1), ("num_procs_y", 1), ("auto_decomp", False),
+                        ("mom_u", p.mom_u), ("mom_v", p.mom_v), ("mom_w", p.mom_w),
+                        ("levelset", p.levelset_its), ("maxu", float(maxu)),
+                        ("phi_dat_frequency", 1), ("uvw_dat_frequency", 1), ("backup_frequency", 1),
+                        ("backup_text_format", True), ("backup_hdf5_format", False),
+                        ("num_timesteps", n_timesteps)):
+        o.put_typed_option(name, value)
+    o.save_options(filename, ["TPLS configuration file"])
