@@ -79,6 +79,8 @@ def test_load_config_errors(tmp_path):
         with pytest.raises(io.ConfigError) as e:
             c.load_config(tmp_path)
         assert e.value.ierr == 3 and "initial_w.dat data dimensions do not match" in str(e.value)
+        write_case(str(tmp_path), p)
+        c.load_config(tmp_path)
         os.remove(tmp_path / "initial_viscosity.dat")
         with pytest.raises(io.ConfigError) as e:
             c.load_config(tmp_path)

@@ -22,6 +22,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <string>
 #include <thread>
 #include <utility>
@@ -968,7 +970,8 @@ int tpls_case_load_config(tpls_case* c, const char* config_dir, int* ierr)
     } files[6] = {{"initial_u.dat", maxn - 2},        {"initial_v.dat", maxn - 2},
                   {"initial_w.dat", maxn - 1},        {"initial_pressure.dat", maxn},
                   {"initial_phi.dat", maxn},          {"initial_viscosity.dat", maxn}};
-    std::vector<std::vector<double>> a(6);
+    // every value is parsed or the load fails: no need to clear the buffers
+    std::unique_ptr<double[]> a[6];
     const long plane = long(maxl + 1) * (maxm + 1);
     // load_configuration_data (configure.F90:94-122) for each file
     for (int q = 0; q < 6; ++q) {
@@ -977,8 +980,9 @@ int tpls_case_load_config(tpls_case* c, const char* config_dir, int* ierr)
             *ierr = TPLS_CONFIG_FILE_NOT_FOUND_ERR;
             return set_error(TPLS_ERR_CONFIG, "Error: Configuration file not found: " + path);
         }
-        a[q].assign(size_t(plane * (files[q].nz + 1)), 0.0);
-        if ((rc = tpls_load_data_into_array(path.c_str(), a[q].data(), 0, maxl, 0, maxm, 0, files[q].nz, ierr)))
+        a[q].reset(new (std::nothrow) double[size_t(plane * (files[q].nz + 1))]);
+        if (!a[q]) return set_error(TPLS_ERR_NOMEM, "load_config: host allocation failed");
+        if ((rc = tpls_load_data_into_array(path.c_str(), a[q].get(), 0, maxl, 0, maxm, 0, files[q].nz, ierr)))
             return rc;
         if (*ierr) {
             char b[64];
@@ -987,7 +991,7 @@ int tpls_case_load_config(tpls_case* c, const char* config_dir, int* ierr)
         }
     }
     *ierr = 0;
-    return tpls_case_load(c, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data());
+    return tpls_case_load(c, a[0].get(), a[1].get(), a[2].get(), a[3].get(), a[4].get());
 }
 
 }  // extern "C"

@@ -7,8 +7,10 @@ the size of initial_pressure.dat) it times the library's save_data and
 load_data_into_array, and, given the Fortran list-directed driver
 (tests/fortran/drive_io), the same write and read done with the statements
 of io.F90:60-72 and io.F90:98-109 by the Fortran runtime on one thread.
---case also times tpls_case_load_config of the whole grid (six files into
-HBM, GPU needed).  Prints one JSON line: values/s and text MB/s of each.
+--case also times, on the GPU, tpls_case_load of the five initial arrays
+from host memory, tpls_case_load_config of the grid (six files into HBM)
+and tpls_case_save_field of the pressure (HBM to a file).  Prints one JSON
+line: seconds, values/s and text MB/s.
 """
 from __future__ import annotations
 
@@ -85,13 +87,22 @@ def main(argv=None):
             from .config import CaseParams
             from .initial import make_initial
             p = CaseParams(nx=nx, ny=ny, nz=nz)
-            io.write_initial_conditions(d, p, make_initial(p))
+            fields = make_initial(p)
+            io.write_initial_conditions(d, p, fields)
             with Case(p) as c:
+                c.load(*fields)  # first use: staging buffers
                 c.sync()
+                t0 = time.perf_counter()
+                c.load(*fields)
+                c.sync()
+                out["case_load_arrays_s"] = round(time.perf_counter() - t0, 3)
                 t0 = time.perf_counter()
                 c.load_config(d)
                 c.sync()
                 out["case_load_config_s"] = round(time.perf_counter() - t0, 3)
+                t0 = time.perf_counter()
+                c.save_field("pres", os.path.join(d, "pres.dat"), "pressure")
+                out["case_save_field_s"] = round(time.perf_counter() - t0, 3)
     print(json.dumps(out), flush=True)
 
 
