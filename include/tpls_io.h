@@ -163,6 +163,25 @@ int tpls_case_load_config(tpls_case *c, const char *config_dir, int *ierr);
  * tpls_case_get_field). */
 int tpls_case_save_field(tpls_case *c, int field, const char *filename, const char *description);
 
+/* ---- checkpoint / resume ---- */
+
+/* A full-state snapshot of a case between steps: every device array that
+ * carries state into the next step, in its storage precision (u2, v2, w2;
+ * conv0/conv1 and csf0/csf1 of each component; conv0/conv1 of phi; phi2;
+ * pres; the FP64 pres_old the pressure solve starts from; u_inlet), with a
+ * checksum per array and the step count.  A case restored from it steps
+ * bit-identically to the case that wrote it.  The reference's backup_channel
+ * (io.F90:647-716) leaves out the CSF histories, u_inlet and the step count,
+ * so a run cannot resume exactly from it.  The file is written under a
+ * temporary name and renamed into place.  Binary, host byte order. */
+int tpls_case_checkpoint(tpls_case *c, const char *filename);
+/* Restores a snapshot into a case of the same grid and precision (other
+ * parameters, dt for instance, may differ).  A checksum mismatch returns
+ * TPLS_ERR_FORMAT and leaves the case state undefined. */
+int tpls_case_restore(tpls_case *c, const char *filename);
+/* Timesteps the case has advanced (restored with a snapshot). */
+int tpls_case_steps(tpls_case *c, long *steps);
+
 #ifdef __cplusplus
 }
 #endif

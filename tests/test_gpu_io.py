@@ -85,3 +85,59 @@ def test_load_config_errors(tmp_path):
         with pytest.raises(io.ConfigError) as e:
             c.load_config(tmp_path)
         assert e.value.ierr == 200
+
+
+@pytest.mark.parametrize("precision,pc,graphs", [("fp64", "none", False), ("fp32", "mg", True)])
+def test_restored_case_steps_bit_identically(tmp_path, precision, pc, graphs):
+    """Run 3 steps, snapshot, run 4 more; a fresh case restored from the
+    snapshot and run 4 steps ends with the same bits in every field."""
+    from tpls_amd.case import Case
+    p = CaseParams(nx=40, ny=12, nz=20, Re=60.0, mu_plus=3.0, dt=1e-4, seed=9, ksp_pc=pc)
+    print(f"checkpoint/restore 40x12x20 {precision} {pc} graphs={graphs}", flush=True)
+    ck = tmp_path / "case.ckpt"
+    with Case(p, precision=precision, graphs=graphs) as a:
+        a.load(*make_initial(p))
+        a.step(3)
+        a.checkpoint(ck)
+        a.step(4)
+        want = {f: a.field(f) for f in FIELDS + ("conv0_u", "csf_w1", "conv1_phi", "pres_old")}
+        dw = a.diag()
+        assert a.steps == 7
+    with Case(p, precision=precision, graphs=graphs) as b:
+        b.restore(ck)
+        assert b.steps == 3
+        b.step(4)
+        assert b.steps == 7
+        for f, w in want.items():
+            assert np.array_equal(b.field(f).view(np.int64), w.view(np.int64)), f
+        db = b.diag()
+        assert db["ksp_its"] == dw["ksp_its"] and db["err_u"] == dw["err_u"]
+
+
+def test_restore_rejects_other_cases_and_corruption(tmp_path):
+    from tpls_amd import _lib
+    from tpls_amd.case import Case
+    p = CaseParams(nx=16, ny=8, nz=16, seed=2)
+    print("restore errors, 16x8x16", flush=True)
+    ck = tmp_path / "a.ckpt"
+    with Case(p) as a:
+        a.load(*make_initial(p))
+        a.step(1)
+        a.checkpoint(ck)
+        with Case(CaseParams(nx=16, ny=8, nz=12, seed=2)) as b:
+            with pytest.raises(_lib.TplsError, match="checkpoint of a 17x9x17 FP64 case"):
+                b.restore(ck)
+        with Case(p, precision="fp32") as b:
+            with pytest.raises(_lib.TplsError, match="FP64 case"):
+                b.restore(ck)
+        raw = bytearray(ck.read_bytes())
+        raw[len(raw) // 2] ^= 0x40
+        bad = tmp_path / "bad.ckpt"
+        bad.write_bytes(bytes(raw))
+        with pytest.raises(_lib.TplsError, match="checksum mismatch"):
+            a.restore(bad)
+        bad.write_bytes(bytes(raw[: len(raw) // 3]))
+        with pytest.raises(_lib.TplsError, match="truncated"):
+            a.restore(bad)
+        with pytest.raises(_lib.TplsError, match="not a case checkpoint"):
+            a.restore(__file__)

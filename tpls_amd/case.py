@@ -126,6 +126,23 @@ class Case:
         _lib.check(io._io().tpls_case_save_field(self._h, _lib.FIELD_ID[name], io._b(filename),
                                                  description.encode()), f"save {name}")
 
+    def checkpoint(self, filename) -> None:
+        """Full-state snapshot (tpls_case_checkpoint): a case restored from
+        it steps bit-identically to this one."""
+        from . import io
+        _lib.check(io._io().tpls_case_checkpoint(self._h, io._b(filename)), "checkpoint")
+
+    def restore(self, filename) -> None:
+        from . import io
+        _lib.check(io._io().tpls_case_restore(self._h, io._b(filename)), "restore")
+
+    @property
+    def steps(self) -> int:
+        from . import io
+        n = C.c_long(0)
+        _lib.check(io._io().tpls_case_steps(self._h, C.byref(n)), "steps")
+        return n.value
+
     @staticmethod
     def device_bytes(params: CaseParams, precision: str = "fp64") -> int:
         return int(_lib.lib().tpls_case_bytes(params.maxl, params.maxm, params.maxn,

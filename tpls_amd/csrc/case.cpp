@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -205,7 +207,10 @@ int stage_init(tpls_case* c)
     return TPLS_OK;
 }
 
-int put_array(tpls_case* c, DevArray* a, const double* host)
+// Rows of a field from the host: fill(buf, first, n) writes elements
+// [first, first + n) of the packed host array, in storage type, to buf.
+template <class Fill>
+int put_rows(tpls_case* c, DevArray* a, Fill&& fill)
 {
     int rc = stage_init(c);
     if (rc) return rc;
@@ -217,13 +222,7 @@ int put_array(tpls_case* c, DevArray* a, const double* host)
         const long nr = std::min(per, rows - r0);
         char* buf = c->stage + (k & 1) * STAGE_HALF;
         if (k >= 2) HIPCHK(hipEventSynchronize(c->stage_ev[k & 1]));  // its previous DMA is done
-        const double* src = host + size_t(r0) * row;
-        if (esz == 8) {
-            std::memcpy(buf, src, size_t(nr) * row * 8);
-        } else {
-            float* f = reinterpret_cast<float*>(buf);
-            for (size_t q = 0; q < size_t(nr) * row; ++q) f[q] = float(src[q]);
-        }
+        if ((rc = fill(buf, size_t(r0) * row, size_t(nr) * row))) return rc;
         HIPCHK(hipMemcpy2DAsync(static_cast<char*>(a->ptr) + size_t(r0) * dpitch, dpitch, buf, row * esz, row * esz,
                                 size_t(nr), hipMemcpyHostToDevice, c->s));
         HIPCHK(hipEventRecord(c->stage_ev[k & 1], c->s));
@@ -232,7 +231,10 @@ int put_array(tpls_case* c, DevArray* a, const double* host)
     return TPLS_OK;
 }
 
-int get_array(tpls_case* c, const DevArray* a, double* host)
+// Rows of a field to the host: drain(buf, first, n) takes elements
+// [first, first + n) of the packed array, in storage type, from buf.
+template <class Drain>
+int get_rows(tpls_case* c, const DevArray* a, Drain&& drain)
 {
     int rc = stage_init(c);
     if (rc) return rc;
@@ -247,7 +249,7 @@ int get_array(tpls_case* c, const DevArray* a, double* host)
                              static_cast<const char*>(a->ptr) + size_t(r0) * spitch, spitch, row * esz, size_t(nr),
                              hipMemcpyDeviceToHost, c->s) != hipSuccess ||
             hipEventRecord(c->stage_ev[k & 1], c->s) != hipSuccess)
-            return fail(TPLS_ERR_HIP, "get_array: device-to-host copy");
+            return fail(TPLS_ERR_HIP, "device-to-host copy of a field");
         return int(TPLS_OK);
     };
     for (long k = 0; k < std::min(n, 2L); ++k)
@@ -255,17 +257,39 @@ int get_array(tpls_case* c, const DevArray* a, double* host)
     for (long k = 0; k < n; ++k) {
         HIPCHK(hipEventSynchronize(c->stage_ev[k & 1]));
         const long r0 = k * per, nr = std::min(per, rows - r0);
-        const char* buf = c->stage + (k & 1) * STAGE_HALF;
-        double* dst = host + size_t(r0) * row;
-        if (esz == 8) {
-            std::memcpy(dst, buf, size_t(nr) * row * 8);
-        } else {
-            const float* f = reinterpret_cast<const float*>(buf);
-            for (size_t q = 0; q < size_t(nr) * row; ++q) dst[q] = double(f[q]);
+        if ((rc = drain(c->stage + (k & 1) * STAGE_HALF, size_t(r0) * row, size_t(nr) * row))) {
+            hipStreamSynchronize(c->s);
+            return rc;
         }
         if (k + 2 < n && (rc = issue(k + 2))) return rc;
     }
     return TPLS_OK;
+}
+
+int put_array(tpls_case* c, DevArray* a, const double* host)
+{
+    return put_rows(c, a, [&](char* buf, size_t q0, size_t n) {
+        if (a->esz == 8) {
+            std::memcpy(buf, host + q0, n * 8);
+        } else {
+            float* f = reinterpret_cast<float*>(buf);
+            for (size_t q = 0; q < n; ++q) f[q] = float(host[q0 + q]);
+        }
+        return int(TPLS_OK);
+    });
+}
+
+int get_array(tpls_case* c, const DevArray* a, double* host)
+{
+    return get_rows(c, a, [&](const char* buf, size_t q0, size_t n) {
+        if (a->esz == 8) {
+            std::memcpy(host + q0, buf, n * 8);
+        } else {
+            const float* f = reinterpret_cast<const float*>(buf);
+            for (size_t q = 0; q < n; ++q) host[q0 + q] = double(f[q]);
+        }
+        return int(TPLS_OK);
+    });
 }
 
 // periodic y wrap of halo rows (what exchange2d* leaves in them)
@@ -322,6 +346,59 @@ DevArray* field_ptr(tpls_case* c, int f)
     }
     return nullptr;  // VISC2_AUG, RHS_P, DIFF_*: not kept by the case
 }
+
+// The arrays that carry state from one step to the next, by role: the
+// velocities u2, v2, w2, the two older convection and CSF histories of each
+// component, the two older phi-advection histories, phi2, the pressure (the
+// Helmholtz boundary conditions read it), the CG unknown pres_old (FP64) and
+// u_inlet.  Every other array is written within a step before it is read.
+std::vector<std::pair<std::string, DevArray*>> state_arrays(tpls_case* c)
+{
+    static const char* comp = "uvw";
+    std::vector<std::pair<std::string, DevArray*>> v;
+    for (int q = 0; q < 3; ++q) v.emplace_back(std::string(1, comp[q]) + "2", c->x[q][c->i2[q]]);
+    for (int q = 0; q < 3; ++q)
+        for (int h = 0; h < 2; ++h) {
+            v.emplace_back("conv" + std::to_string(h) + "_" + comp[q], c->conv[q][h]);
+            v.emplace_back(std::string("csf_") + comp[q] + std::to_string(h), c->csf[q][h]);
+        }
+    v.emplace_back("conv0_phi", c->cphi[0]);
+    v.emplace_back("conv1_phi", c->cphi[1]);
+    v.emplace_back("phi2", c->phi2);
+    v.emplace_back("pres", c->pres);
+    v.emplace_back("pres_old", c->pold);
+    v.emplace_back("u_inlet", c->inlet);
+    return v;
+}
+
+// Order-sensitive 64-bit checksum of a byte range, eight bytes at a step.
+uint64_t checksum(uint64_t h, const char* p, size_t n)
+{
+    size_t q = 0;
+    for (; q + 8 <= n; q += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + q, 8);
+        h = (h ^ w) * 0x100000001b3ULL;
+        h ^= h >> 29;
+    }
+    for (; q < n; ++q) h = (h ^ uint8_t(p[q])) * 0x100000001b3ULL;
+    return h;
+}
+
+constexpr char CKPT_MAGIC[8] = {'T', 'P', 'L', 'S', 'C', 'K', 'P', '1'};
+
+struct CkptHeader {
+    char magic[8];
+    int32_t maxl, maxm, maxn, precision;
+    int64_t steps_done;
+    int32_t narrays, last_its;
+};
+
+struct CkptArray {
+    int32_t esz, pad;
+    int64_t count;
+    uint64_t sum;
+};
 
 }  // namespace
 
@@ -565,6 +642,99 @@ int tpls_case_save_field(tpls_case* c, int f, const char* filename, const char* 
     int rc = tpls_case_get_field(c, f, host.data());
     if (rc) return rc;
     return tpls_save_data(filename, description, host.data(), sh.lx, sh.ux, sh.ly, sh.uy, sh.lz, sh.uz);
+}
+
+int tpls_case_checkpoint(tpls_case* c, const char* filename)
+{
+    if (!c || !filename) return fail(TPLS_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    const std::string tmp = std::string(filename) + ".tmp";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) return fail(TPLS_ERR_IO, tmp + ": " + std::strerror(errno));
+    auto arrays = state_arrays(c);
+    CkptHeader h{};
+    std::memcpy(h.magic, CKPT_MAGIC, 8);
+    h.maxl = c->maxl;
+    h.maxm = c->maxm;
+    h.maxn = c->maxn;
+    h.precision = c->prm.precision;
+    h.steps_done = c->steps_done;
+    h.narrays = int32_t(arrays.size());
+    h.last_its = c->last_its;
+    int rc = std::fwrite(&h, sizeof h, 1, fp) == 1 ? TPLS_OK : fail(TPLS_ERR_IO, tmp + ": write failed");
+    std::vector<char> buf;
+    for (auto& na : arrays) {
+        if (rc) break;
+        const DevArray* a = na.second;
+        const size_t esz = size_t(a->esz), count = size_t(a->sh.count());
+        buf.resize(count * esz);
+        rc = get_rows(c, a, [&](const char* b, size_t q0, size_t n) {
+            std::memcpy(buf.data() + q0 * esz, b, n * esz);
+            return int(TPLS_OK);
+        });
+        if (rc) break;
+        CkptArray ah{int32_t(esz), 0, int64_t(count), checksum(0xcbf29ce484222325ULL, buf.data(), buf.size())};
+        if (std::fwrite(&ah, sizeof ah, 1, fp) != 1 || std::fwrite(buf.data(), 1, buf.size(), fp) != buf.size())
+            rc = fail(TPLS_ERR_IO, tmp + ": write failed");
+    }
+    if (std::fclose(fp) != 0 && !rc) rc = fail(TPLS_ERR_IO, tmp + ": close failed");
+    if (!rc && std::rename(tmp.c_str(), filename) != 0)
+        rc = fail(TPLS_ERR_IO, std::string(filename) + ": " + std::strerror(errno));
+    if (rc) std::remove(tmp.c_str());
+    return rc;
+}
+
+int tpls_case_restore(tpls_case* c, const char* filename)
+{
+    if (!c || !filename) return fail(TPLS_ERR_ARG, "null argument");
+    FILE* fp = std::fopen(filename, "rb");
+    if (!fp) return fail(TPLS_ERR_IO, std::string(filename) + ": " + std::strerror(errno));
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{fp};
+    HIPCHK(hipStreamSynchronize(c->s));
+    auto arrays = state_arrays(c);
+    CkptHeader h{};
+    if (std::fread(&h, sizeof h, 1, fp) != 1 || std::memcmp(h.magic, CKPT_MAGIC, 8) != 0)
+        return fail(TPLS_ERR_FORMAT, std::string(filename) + ": not a case checkpoint");
+    if (h.maxl != c->maxl || h.maxm != c->maxm || h.maxn != c->maxn || h.precision != c->prm.precision)
+        return fail(TPLS_ERR_ARG, std::string(filename) + ": checkpoint of a " + std::to_string(h.maxl) + "x" +
+                                      std::to_string(h.maxm) + "x" + std::to_string(h.maxn) +
+                                      (h.precision == TPLS_FP32 ? " FP32" : " FP64") + " case");
+    if (h.narrays != int32_t(arrays.size()))
+        return fail(TPLS_ERR_FORMAT, std::string(filename) + ": unexpected number of arrays");
+    std::vector<char> buf;
+    for (auto& na : arrays) {
+        DevArray* a = na.second;
+        const size_t esz = size_t(a->esz), count = size_t(a->sh.count());
+        CkptArray ah{};
+        if (std::fread(&ah, sizeof ah, 1, fp) != 1 || size_t(ah.esz) != esz || size_t(ah.count) != count)
+            return fail(TPLS_ERR_FORMAT, std::string(filename) + ": bad record for " + na.first);
+        buf.resize(count * esz);
+        if (std::fread(buf.data(), 1, buf.size(), fp) != buf.size())
+            return fail(TPLS_ERR_FORMAT, std::string(filename) + ": truncated at " + na.first);
+        if (checksum(0xcbf29ce484222325ULL, buf.data(), buf.size()) != ah.sum)
+            return fail(TPLS_ERR_FORMAT, std::string(filename) + ": checksum mismatch in " + na.first +
+                                             " (the case state is now undefined)");
+        int rc = put_rows(c, a, [&](char* b, size_t q0, size_t n) {
+            std::memcpy(b, buf.data() + q0 * esz, n * esz);
+            return int(TPLS_OK);
+        });
+        if (rc) return rc;
+    }
+    c->steps_done = h.steps_done;
+    c->last_its = h.last_its;
+    c->diag_pending = false;
+    std::memset(&c->diag, 0, sizeof c->diag);
+    return TPLS_OK;
+}
+
+int tpls_case_steps(tpls_case* c, long* steps)
+{
+    if (!c || !steps) return fail(TPLS_ERR_ARG, "null argument");
+    *steps = c->steps_done;
+    return TPLS_OK;
 }
 
 int tpls_case_diag(tpls_case* c, tpls_diag* out)

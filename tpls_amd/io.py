@@ -17,7 +17,6 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -89,6 +88,9 @@ def _io():
         L.tpls_read_config.argtypes = [s, s, C.POINTER(_lib.Params), C.POINTER(RunConfig), ip]
         L.tpls_case_load_config.argtypes = [C.c_void_p, s, ip]
         L.tpls_case_save_field.argtypes = [C.c_void_p, i, s, s]
+        L.tpls_case_checkpoint.argtypes = [C.c_void_p, s]
+        L.tpls_case_restore.argtypes = [C.c_void_p, s]
+        L.tpls_case_steps.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
         _bound = True
     return L
 
