@@ -163,6 +163,38 @@ int tpls_case_load_config(tpls_case *c, const char *config_dir, int *ierr);
  * tpls_case_get_field). */
 int tpls_case_save_field(tpls_case *c, int field, const char *filename, const char *description);
 
+/* ---- channel outputs and the text backup (io.F90:171-791) ----
+ *
+ * The reference's outputs for visual comparison, with its argument lists
+ * (arrays with their Fortran bounds, l = maxl, m = maxm, n = maxn), its
+ * record order, coordinates (cell centres, x = dx/2 + dx*i) and face
+ * averages.  Values are written as the shortest decimals that read back to
+ * the same doubles; the header lines follow list-directed output.  The
+ * caller passes the arrays main_ns_hybrid.F90 passes (its gathered globals:
+ * the run driver of tpls_amd/run.py builds them from a case). */
+int tpls_save_backup_info(const char *filename, int lx, int ly, int mx, int my, int nx, int ny, int dx, int dy,
+                          int dz);                                   /* io.F90:180-200 (integer spacings, as there) */
+int tpls_channel_output_phi(const double *phi, int l, int m, int n, double dx, double dy, double dz,
+                            const char *filename);                   /* io.F90:217-246 */
+/* io.F90:503-534: *ierr = 1 when the file holds fewer records than the grid. */
+int tpls_channel_input_phi(const char *filename, double *phi, int l, int m, int n, int *ierr);
+int tpls_channel_output_uvw(const double *u3, const double *v3, const double *w3, const double *p, int l, int m,
+                            int n, double dx, double dy, double dz, const char *filename);   /* io.F90:551-587 */
+int tpls_channel_output2d(const double *u3, const double *v3, const double *w3, const double *p, const double *phi,
+                          const double *visc, int l, int m, int n, double dx, double dy, double dz,
+                          const char *filename);                     /* io.F90:604-643 */
+/* backup_channel / dataload_channel (io.F90:647-791): u2, v2, w2, phi with
+ * their conv0/conv1 histories and pres, pres_old, three (two) values per
+ * record. */
+int tpls_backup_channel(const double *u2, const double *v2, const double *w2, const double *conv0_u,
+                        const double *conv1_u, const double *conv0_v, const double *conv1_v, const double *conv0_w,
+                        const double *conv1_w, const double *pres_old, const double *pres, const double *phi,
+                        const double *conv0_phi, const double *conv1_phi, int l, int m, int n, const char *filename);
+int tpls_dataload_channel(double *u2, double *v2, double *w2, double *conv0_u, double *conv1_u, double *conv0_v,
+                          double *conv1_v, double *conv0_w, double *conv1_w, double *pres_old, double *pres,
+                          double *phi, double *conv0_phi, double *conv1_phi, int l, int m, int n,
+                          const char *filename);
+
 /* ---- checkpoint / resume ---- */
 
 /* A full-state snapshot of a case between steps: every device array that

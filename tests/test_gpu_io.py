@@ -141,3 +141,45 @@ def test_restore_rejects_other_cases_and_corruption(tmp_path):
             a.restore(bad)
         with pytest.raises(_lib.TplsError, match="not a case checkpoint"):
             a.restore(__file__)
+
+
+def test_run_driver_outputs_and_resume(tmp_path):
+    """python -m tpls_amd.run on a configuration directory: the reference's
+    output files at its cadence, their content taken at the reference's point
+    of the iteration, and a resumed run ending in the same state."""
+    from tpls_amd import run
+    from tpls_amd.case import Case
+    p = CaseParams(nx=24, ny=8, nz=16, seed=5)
+    cfg, out = tmp_path / "cfg", tmp_path / "out"
+    io.write_initial_conditions(cfg, p, make_initial(p))
+    opts = cfg / "tpls_config.opt"
+    io.write_options_file(opts, p, n_timesteps=4, phi_dat_frequency=2, uvw_dat_frequency=4)
+    print("run driver, 24x8x16, 4 iterations", flush=True)
+    assert run.main(["-d", str(cfg), "-f", str(opts), "--out", str(out), "--checkpoint", str(tmp_path / "a.ckpt")]) == 0
+    assert sorted(os.listdir(out)) == sorted(["AA_initial_phi.dat", "phi_channel_        2.dat",
+                                              "phi_channel_        4.dat", "uvw_channel_        4.dat",
+                                              "fieldbackup1.dat", "fieldbackup0.dat"])
+    with Case.from_config(cfg, opts) as c:
+        l, m, n = c.shape
+        prm = c._prm
+        c.step(3)
+        pre = {f: c.field(f) for f in ("phi2", "u2", "v2", "w2")}
+        inlet = c.field("u_inlet")
+        c.step(1)
+        G = lambda name, a: run.gathered(name, a, l, m, n, prm.dz, prm.height, inlet)  # noqa: E731
+        io.channel_output_phi(G("phi", pre["phi2"]), l, m, n, prm.dx, prm.dy, prm.dz, tmp_path / "phi.dat")
+        io.channel_output_uvw(G("u", pre["u2"]), G("v", pre["v2"]), G("w", pre["w2"]), G("pres", c.field("pres")),
+                              l, m, n, prm.dx, prm.dy, prm.dz, tmp_path / "uvw.dat")
+        assert (tmp_path / "phi.dat").read_bytes() == (out / "phi_channel_        4.dat").read_bytes()
+        assert (tmp_path / "uvw.dat").read_bytes() == (out / "uvw_channel_        4.dat").read_bytes()
+        c.checkpoint(tmp_path / "direct.ckpt")
+    assert (tmp_path / "direct.ckpt").read_bytes() == (tmp_path / "a.ckpt").read_bytes()
+    # the same four iterations as two runs of two
+    io.write_options_file(opts, p, n_timesteps=2, phi_dat_frequency=2, uvw_dat_frequency=4)
+    assert run.main(["-d", str(cfg), "-f", str(opts), "--out", str(tmp_path / "o1"),
+                     "--checkpoint", str(tmp_path / "b2.ckpt")]) == 0
+    io.write_options_file(opts, p, n_timesteps=4, phi_dat_frequency=2, uvw_dat_frequency=4)
+    assert run.main(["-d", str(cfg), "-f", str(opts), "--out", str(tmp_path / "o2"), "--restore",
+                     str(tmp_path / "b2.ckpt"), "--checkpoint", str(tmp_path / "b4.ckpt")]) == 0
+    assert (tmp_path / "b4.ckpt").read_bytes() == (tmp_path / "a.ckpt").read_bytes()
+    assert (tmp_path / "o2" / "phi_channel_        4.dat").read_bytes() == (out / "phi_channel_        4.dat").read_bytes()

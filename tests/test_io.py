@@ -196,7 +196,7 @@ def test_short_and_bad_files(tmp_path):
         io.load_data_into_array(f, a, LX, LY, MX, MY, NX, NY)
     g = tmp_path / "bad.dat"
     g.write_text(" DATA: bad\n DIM\n 1 3 1 1 1 1\n 1.0\n 2.0x\n 3.0\n")
-    with pytest.raises(_lib.TplsError, match="value 2 is not a real number"):
+    with pytest.raises(_lib.TplsError, match="record 2 is not a real number"):
         io.load_data(g)
     h = tmp_path / "nohdr.dat"
     h.write_text(" DATA: bad\n")
@@ -591,3 +591,102 @@ def test_written_initial_conditions_read_back(tmp_path):
         b = np.zeros_like(a)
         assert io.load_data_into_array(tmp_path / io.INITIAL_FILES[name], b, 0, p.maxl, 0, p.maxm, 0, up) == 0
         assert np.array_equal(bits(a), bits(b))
+
+
+# ---------------------------------------------------------------------------
+# channel outputs and the text backup (io.F90:171-791)
+
+def grid_fields(l, m, n, seed):
+    r = np.random.default_rng(seed)
+    planes = {"uv": n - 1, "w": n, "p": n + 1}
+    return {k: np.asfortranarray(r.standard_normal((l + 1, m + 1, planes[g]))) for k, g in io.BACKUP_ARRAYS}
+
+
+def records(path, skip):
+    return np.loadtxt(path, skiprows=skip, ndmin=2)
+
+
+def test_channel_output_phi_and_input(tmp_path):
+    l, m, n, dx, dy, dz = 7, 5, 6, 0.125, 0.1, 1 / 3
+    phi = grid_fields(l, m, n, 1)["phi"]
+    f = tmp_path / "phi_channel.dat"
+    io.channel_output_phi(phi, l, m, n, dx, dy, dz, f)
+    head = f.read_text().splitlines()[:2]
+    assert head[0] == ' VARIABLES="X ","Y ","Z ","Phi"'
+    assert head[1].split() == ['ZONE', 'T="Floor",', 'I=', str(l - 1), 'J=', str(m - 1), 'K=', str(n - 1), 'F=POINT']
+    rec = records(f, 2)
+    assert rec.shape == ((l - 1) * (m - 1) * (n - 1), 4)
+    # records: k slowest, then j, then i = 1..l-1; cell-centre coordinates
+    i, j, k = np.meshgrid(np.arange(1, l), np.arange(0, m - 1), np.arange(0, n - 1), indexing="ij")
+    i, j, k = (a.ravel(order="F") for a in (i, j, k))
+    assert np.array_equal(rec[:, 0], dx / 2 + dx * i) and np.array_equal(rec[:, 2], dz / 2 + dz * k)
+    assert np.array_equal(rec[:, 1], dy / 2 + dy * j)
+    assert np.array_equal(rec[:, 3], phi[i + 1, j, k + 1])
+    back = np.zeros_like(phi)
+    assert io.channel_input_phi(f, back, l, m, n) == 0
+    assert np.array_equal(back[2:, :m - 1, 1:n], phi[2:, :m - 1, 1:n])
+    assert (back[:2] == 0).all() and (back[:, m - 1:] == 0).all()
+
+
+def test_channel_output_uvw_and_2d(tmp_path):
+    l, m, n, dx, dy, dz = 6, 7, 5, 0.2, 0.25, 0.5
+    a = grid_fields(l, m, n, 2)
+    u3, v3, w3, p, phi, visc = a["u2"], a["v2"], a["w2"], a["pres"], a["phi"], a["pres_old"]
+    f = tmp_path / "uvw.dat"
+    io.channel_output_uvw(u3, v3, w3, p, l, m, n, dx, dy, dz, f)
+    rec = records(f, 2)
+    i, j, k = np.meshgrid(np.arange(1, l), np.arange(0, m - 1), np.arange(0, n - 1), indexing="ij")
+    i, j, k = (x.ravel(order="F") for x in (i, j, k))
+    # face averages to the cell centre, pressure one plane up (io.F90:575-581)
+    assert np.array_equal(rec[:, 3], (u3[i, j, k] + u3[i + 1, j, k]) / 2)
+    assert np.array_equal(rec[:, 4], (v3[i, j, k] + v3[i, j + 1, k]) / 2)
+    assert np.array_equal(rec[:, 5], (w3[i, j, k] + w3[i, j, k + 1]) / 2)
+    assert np.array_equal(rec[:, 6], p[i, j, k + 1])
+    g = tmp_path / "2d.dat"
+    io.channel_output2d(u3, v3, w3, p, phi, visc, l, m, n, dx, dy, dz, g)
+    assert g.read_text().splitlines()[1].split()[4:6] == ["K=", str(n - 1)]
+    rec = records(g, 2)
+    i, k = np.meshgrid(np.arange(1, l), np.arange(0, n - 1), indexing="ij")
+    i, k = i.ravel(order="F"), k.ravel(order="F")
+    jm = (m - 1) // 2
+    assert rec.shape == ((l - 1) * (n - 1), 8)
+    assert np.array_equal(rec[:, 1], dz / 2 + dz * k)
+    assert np.array_equal(rec[:, 6], phi[i + 1, jm, k + 1]) and np.array_equal(rec[:, 7], visc[i, jm, k + 1])
+
+
+def test_backup_channel_round_trip_and_layout(tmp_path):
+    l, m, n = 5, 4, 6
+    a = grid_fields(l, m, n, 3)
+    f = tmp_path / "fieldbackup1.dat"
+    io.backup_channel(a, l, m, n, f)
+    lines = f.read_text().splitlines()
+    plane = (l + 1) * (m + 1)
+    assert len(lines) == plane * (2 * (n - 1) + n + 2 * (n + 1))
+    first = [float(x) for x in lines[0].split()]
+    assert first == [a["u2"][0, 0, 0], a["conv0_u"][0, 0, 0], a["conv1_u"][0, 0, 0]]
+    last = [float(x) for x in lines[-1].split()]
+    assert last == [a["pres"][l, m, n], a["pres_old"][l, m, n]]
+    b = io.dataload_channel(f, l, m, n)
+    for k in a:
+        assert np.array_equal(b[k].view(np.int64), a[k].view(np.int64)), k
+    f.write_text("\n".join(lines[: len(lines) // 2]) + "\n")
+    with pytest.raises(_lib.TplsError, match="ends inside a backup block"):
+        io.dataload_channel(f, l, m, n)
+
+
+def test_gathered_ghost_rules():
+    from tpls_amd.run import gathered
+    maxl, maxm, maxn = 6, 5, 7
+    r = np.random.default_rng(4)
+    a = np.asfortranarray(r.standard_normal((maxl + 1, maxm + 1, maxn - 1)))
+    inlet = r.standard_normal(maxn - 1)
+    u = gathered("u", a, maxl, maxm, maxn, u_inlet=inlet)
+    assert (u[0] == inlet[None, :]).all()
+    assert np.array_equal(u[maxl], u[maxl - 2]) and np.array_equal(u[maxl - 1, 1:maxm], a[maxl - 2, 1:maxm])
+    assert np.array_equal(u[:, 0], u[:, maxm - 1]) and np.array_equal(u[1:maxl - 1, 1:maxm], a[1:maxl - 1, 1:maxm])
+    w = gathered("w", np.asfortranarray(r.standard_normal((maxl + 1, maxm + 1, maxn))), maxl, maxm, maxn)
+    assert (w[:, :, 0] == 0).all() and (w[:, :, maxn - 1] == 0).all() and (w[0] == 0).all() and (w[maxl] == 0).all()
+    phi = gathered("phi", np.asfortranarray(r.standard_normal((maxl + 1, maxm + 1, maxn + 1))), maxl, maxm, maxn,
+                   dz=0.25, height=0.5)
+    assert phi[0, 2, 3] == 3 * 0.25 - 0.5 * 0.25 - 0.5
+    assert np.array_equal(phi[:, :, maxn], phi[:, :, maxn - 1]) and np.array_equal(phi[:, :, 0], phi[:, :, 1])

@@ -10,6 +10,10 @@
 // index, and a second pass converts.  Values are written as the shortest
 // decimal that reads back to the same double (std::to_chars), and read with
 // correctly rounded conversion (std::from_chars), so files round-trip exactly.
+// Coordinates and face averages of the channel outputs round as the
+// reference's expressions do: no fused multiply-adds.
+#pragma clang fp contract(off)
+
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -47,7 +51,7 @@ inline bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\r'; }
 inline bool is_sep(char c) { return is_blank(c) || c == ',' || c == '/' || c == '\n'; }
 inline char lower(char c) { return (c >= 'A' && c <= 'Z') ? char(c - 'A' + 'a') : c; }
 
-enum Item { ITEM_VALUE, ITEM_NULL, ITEM_NONE, ITEM_BAD };
+enum Item { ITEM_VALUE, ITEM_NULL, ITEM_NONE, ITEM_BAD, ITEM_END };
 
 // A decimal beyond the double range: an underflow rounds to the nearest
 // subnormal or zero, an overflow is an input error (as the Fortran runtime
@@ -153,7 +157,8 @@ bool parse_logical(const char* b, const char* e, int* out)
 
 // The next list-directed item of the record [p, e): its constant in
 // [*tb, *te).  A repeat count r*c yields c (r*c with nothing after the star is
-// a null value).  *p advances past the item and its separator.
+// a null value); a slash ends the input items (ITEM_END).  *p advances past
+// the item and its separator.
 Item next_item(const char*& p, const char* e, const char** tb, const char** te)
 {
     while (p < e && is_blank(*p)) ++p;
@@ -164,7 +169,7 @@ Item next_item(const char*& p, const char* e, const char** tb, const char** te)
     }
     if (*p == '/') {
         p = e;
-        return ITEM_NULL;
+        return ITEM_END;
     }
     const char* b = p;
     while (p < e && !is_sep(*p)) ++p;
@@ -301,11 +306,15 @@ long bound_count(const int* b)
     return n;
 }
 
-// Reads the value records of [body, end) into out[0..count): the first item
-// of each non-blank record, in order (a null item leaves the value as it
-// was).  Returns the number of value records found (capped at count), or a
-// negative error.
-long read_values(const char* body, const char* end, const char* path, double* out, long count)
+// Reads the records of [body, end) that carry values (blank records are
+// skipped, as list-directed input skips them): record r gives the first
+// ncols items, which sink(r, v, got) stores (got[c] is false for a null item,
+// which leaves the value as it was; a slash ends the record's items early).
+// Further items of a record are ignored.  Returns the number of value
+// records (capped at count), or a negative error.  A record with fewer than
+// ncols items is an error here (Fortran would read on into the next record).
+template <class Sink>
+long read_records(const char* body, const char* end, const char* path, int ncols, long count, Sink&& sink)
 {
     const size_t bytes = size_t(end - body);
     const int nt = io_threads(bytes);
@@ -335,31 +344,54 @@ long read_values(const char* body, const char* end, const char* path, double* ou
     });
     for (int t = 0; t < nt; ++t) first[t + 1] = first[t] + nrec[t];
     std::vector<long> bad(nt, -1);
+    std::vector<int> why(nt, 0);
     parallel_for(nt, [&](int t) {
         long idx = first[t];
+        double v[8];
+        bool got[8];
         for (const char* p = cut[t]; p < cut[t + 1] && idx < count;) {
             const char* re = record_end(p, cut[t + 1]);
             const char* q = p;
-            const char *tb, *te;
-            Item it = next_item(q, re, &tb, &te);
-            if (it == ITEM_VALUE) {
-                if (!parse_real(tb, te, &out[idx])) {
+            p = re + 1;
+            bool blank = false;
+            int c = 0;
+            for (; c < ncols; ++c) {
+                const char *tb, *te;
+                Item it = next_item(q, re, &tb, &te);
+                if (it == ITEM_END) break;  // a slash: the remaining items keep their values
+                if (it == ITEM_NONE) {
+                    if (c == 0) {
+                        blank = true;  // list-directed input skips blank records
+                        break;
+                    }
+                    bad[t] = idx;
+                    why[t] = 1;
+                    return;
+                }
+                got[c] = it == ITEM_VALUE;
+                if (it == ITEM_BAD || (got[c] && !parse_real(tb, te, &v[c]))) {
                     bad[t] = idx;
                     return;
                 }
-            } else if (it == ITEM_BAD) {
-                bad[t] = idx;
-                return;
             }
-            if (it != ITEM_NONE) ++idx;
-            p = re + 1;
+            if (blank) continue;
+            for (int r = c; r < ncols; ++r) got[r] = false;
+            sink(idx, v, got);
+            ++idx;
         }
     });
     for (int t = 0; t < nt; ++t)
         if (bad[t] >= 0)
-            return set_error(TPLS_ERR_FORMAT,
-                             std::string(path) + ": value " + std::to_string(bad[t] + 1) + " is not a real number");
+            return set_error(TPLS_ERR_FORMAT, std::string(path) + ": record " + std::to_string(bad[t] + 1) +
+                                                  (why[t] ? " has too few values" : " is not a real number"));
     return std::min(first[nt], count);
+}
+
+long read_values(const char* body, const char* end, const char* path, double* out, long count)
+{
+    return read_records(body, end, path, 1, count, [&](long r, const double* v, const bool* got) {
+        if (got[0]) out[r] = v[0];
+    });
 }
 
 int write_all(FILE* fp, const char* data, size_t n, const char* path)
@@ -369,14 +401,37 @@ int write_all(FILE* fp, const char* data, size_t n, const char* path)
     return TPLS_OK;
 }
 
-// " <shortest round-trip decimal>\n" for each value: one list-directed record.
-inline char* format_value(char* p, char* e, double v)
+// count records of ncols values each, " v1 v2 ...\n" with every value the
+// shortest decimal that reads back to the same double; values(r, v) gives
+// record r.  Blocks of records are formatted in parallel and written in order.
+template <class Values>
+int write_records(FILE* fp, const char* path, long count, int ncols, Values&& values)
 {
-    *p++ = ' ';
-    auto r = std::to_chars(p, e, v);
-    p = r.ptr;
-    *p++ = '\n';
-    return p;
+    const long block = std::max(1L, (1L << 18) / ncols);
+    const int nt = io_threads(size_t(count) * size_t(ncols) * 24);
+    const size_t cap = size_t(block) * size_t(ncols) * 26 + 64;
+    std::vector<std::vector<char>> buf(nt, std::vector<char>(cap));
+    std::vector<size_t> len(nt);
+    int rc = TPLS_OK;
+    for (long base = 0; base < count && !rc; base += block * nt) {
+        parallel_for(nt, [&](int t) {
+            const long b0 = base + t * block, b1 = std::min(count, b0 + block);
+            char* p = buf[t].data();
+            char* e = p + buf[t].size();
+            double v[8];
+            for (long r = b0; r < b1; ++r) {
+                values(r, v);
+                for (int c = 0; c < ncols; ++c) {
+                    *p++ = ' ';
+                    p = std::to_chars(p, e, v[c]).ptr;
+                }
+                *p++ = '\n';
+            }
+            len[t] = b0 < b1 ? size_t(p - buf[t].data()) : 0;
+        });
+        for (int t = 0; t < nt && !rc; ++t) rc = write_all(fp, buf[t].data(), len[t], path);
+    }
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -677,21 +732,7 @@ int tpls_save_data(const char* filename, const char* description, const double* 
     std::snprintf(hdr, sizeof hdr, "%12d%12d%12d%12d%12d%12d\n", lx, ly, mx, my, nx, ny);
     head += hdr;
     int rc = write_all(fp, head.data(), head.size(), filename);
-    // format blocks of values in parallel, write them in order
-    const long block = 1L << 18;
-    const int nt = io_threads(size_t(count) * 24);
-    std::vector<std::vector<char>> buf(nt, std::vector<char>(size_t(block) * 28 + 64));
-    std::vector<size_t> len(nt);
-    for (long base = 0; base < count && !rc; base += block * nt) {
-        parallel_for(nt, [&](int t) {
-            const long b0 = base + t * block, b1 = std::min(count, b0 + block);
-            char* p = buf[t].data();
-            char* e = p + buf[t].size();
-            for (long i = b0; i < b1; ++i) p = format_value(p, e, data[i]);
-            len[t] = b0 < b1 ? size_t(p - buf[t].data()) : 0;
-        });
-        for (int t = 0; t < nt && !rc; ++t) rc = write_all(fp, buf[t].data(), len[t], filename);
-    }
+    if (!rc) rc = write_records(fp, filename, count, 1, [&](long r, double* v) { v[0] = data[r]; });
     if (std::fclose(fp) != 0 && !rc) rc = set_error(TPLS_ERR_IO, std::string(filename) + ": close failed");
     return rc;
 }
@@ -765,6 +806,240 @@ int tpls_load_data_into_array(const char* filename, double* data, int lx, int ly
 
 void tpls_free(void* p) { std::free(p); }
 
+// ---- channel outputs and text backups ----
+
+namespace {
+
+FILE* open_write(const char* filename, int* rc)
+{
+    FILE* fp = std::fopen(filename, "w");
+    *rc = fp ? TPLS_OK : set_error(TPLS_ERR_IO, std::string(filename) + ": " + std::strerror(errno));
+    return fp;
+}
+
+int close_file(FILE* fp, const char* filename, int rc)
+{
+    if (std::fclose(fp) != 0 && !rc) rc = set_error(TPLS_ERR_IO, std::string(filename) + ": close failed");
+    return rc;
+}
+
+// The Tecplot zone line the channel outputs start with.
+std::string zone_line(int l, int m, int n, bool with_j)
+{
+    std::string s = " ZONE T=\"Floor\", I= " + std::to_string(l - 1) + " ";
+    if (with_j) s += " J= " + std::to_string(m - 1) + " ";
+    return s + " K= " + std::to_string(n - 1) + "  F=POINT\n";
+}
+
+bool grid_ok(int l, int m, int n) { return l >= 2 && m >= 2 && n >= 2; }
+
+}  // namespace
+
+int tpls_save_backup_info(const char* filename, int lx, int ly, int mx, int my, int nx, int ny, int dx, int dy,
+                          int dz)
+{
+    if (!filename) return set_error(TPLS_ERR_ARG, "null argument");
+    int rc;
+    FILE* fp = open_write(filename, &rc);
+    if (!fp) return rc;
+    char b[256];
+    std::snprintf(b, sizeof b, " DIMENSIONS: Lmin Lmax Mmin Mmax Nmin Nmax\n %d %d %d %d %d %d\n DIMENSIONS: Dx,Dy,Dz\n %d %d %d\n",
+                  lx, ly, mx, my, nx, ny, dx, dy, dz);
+    rc = write_all(fp, b, std::strlen(b), filename);
+    return close_file(fp, filename, rc);
+}
+
+int tpls_channel_output_phi(const double* phi, int l, int m, int n, double dx, double dy, double dz,
+                            const char* filename)
+{
+    if (!phi || !filename || !grid_ok(l, m, n)) return set_error(TPLS_ERR_ARG, "bad argument");
+    int rc;
+    FILE* fp = open_write(filename, &rc);
+    if (!fp) return rc;
+    const std::string head = " VARIABLES=\"X \",\"Y \",\"Z \",\"Phi\"\n" + zone_line(l, m, n, true);
+    rc = write_all(fp, head.data(), head.size(), filename);
+    const long ni = l - 1, nj = m - 1, nk = n - 1, sx = l + 1, sy = long(m + 1) * sx;
+    if (!rc)
+        rc = write_records(fp, filename, ni * nj * nk, 4, [&](long r, double* v) {
+            const long i = r % ni + 1, j = (r / ni) % nj, k = r / (ni * nj);
+            v[0] = dx / 2.0 + dx * double(i);
+            v[1] = dy / 2.0 + dy * double(j);
+            v[2] = dz / 2.0 + dz * double(k);
+            v[3] = phi[(i + 1) + j * sx + (k + 1) * sy];
+        });
+    return close_file(fp, filename, rc);
+}
+
+int tpls_channel_input_phi(const char* filename, double* phi, int l, int m, int n, int* ierr)
+{
+    if (!filename || !phi || !ierr || !grid_ok(l, m, n)) return set_error(TPLS_ERR_ARG, "bad argument");
+    MappedFile f;
+    int rc = f.open(filename);
+    if (rc) return rc;
+    const char* p = f.p;
+    const char* e = f.p + f.n;
+    for (int r = 0; r < 2 && p < e; ++r) p = std::min(record_end(p, e) + 1, e);
+    const long ni = l - 1, nj = m - 1, nk = n - 1, sx = l + 1, sy = long(m + 1) * sx;
+    const long count = ni * nj * nk;
+    long got = read_records(p, e, filename, 4, count, [&](long r, const double* v, const bool* g) {
+        const long i = r % ni + 1, j = (r / ni) % nj, k = r / (ni * nj);
+        if (g[3]) phi[(i + 1) + j * sx + (k + 1) * sy] = v[3];
+    });
+    if (got < 0) return int(got);
+    *ierr = got < count ? 1 : 0;
+    return TPLS_OK;
+}
+
+int tpls_channel_output_uvw(const double* u3, const double* v3, const double* w3, const double* p, int l, int m,
+                            int n, double dx, double dy, double dz, const char* filename)
+{
+    if (!u3 || !v3 || !w3 || !p || !filename || !grid_ok(l, m, n)) return set_error(TPLS_ERR_ARG, "bad argument");
+    int rc;
+    FILE* fp = open_write(filename, &rc);
+    if (!fp) return rc;
+    const std::string head = " VARIABLES=\"X \",\"Y \",\"Z \",\"U\",\"V\",\"W\",\"P\"\n" + zone_line(l, m, n, true);
+    rc = write_all(fp, head.data(), head.size(), filename);
+    const long ni = l - 1, nj = m - 1, nk = n - 1, sx = l + 1, sy = long(m + 1) * sx;
+    if (!rc)
+        rc = write_records(fp, filename, ni * nj * nk, 7, [&](long r, double* v) {
+            const long i = r % ni + 1, j = (r / ni) % nj, k = r / (ni * nj);
+            const long c = i + j * sx + k * sy;  // u3, v3 (0:n-2) and w3 (0:n-1) share the plane stride
+            v[0] = dx / 2.0 + dx * double(i);
+            v[1] = dy / 2.0 + dy * double(j);
+            v[2] = dz / 2.0 + dz * double(k);
+            v[3] = (u3[c] + u3[c + 1]) / 2.0;
+            v[4] = (v3[c] + v3[c + sx]) / 2.0;
+            v[5] = (w3[c] + w3[c + sy]) / 2.0;
+            v[6] = p[c + sy];
+        });
+    return close_file(fp, filename, rc);
+}
+
+int tpls_channel_output2d(const double* u3, const double* v3, const double* w3, const double* p, const double* phi,
+                          const double* visc, int l, int m, int n, double dx, double dy, double dz, const char* filename)
+{
+    if (!u3 || !v3 || !w3 || !p || !phi || !visc || !filename || !grid_ok(l, m, n))
+        return set_error(TPLS_ERR_ARG, "bad argument");
+    (void)dy;
+    int rc;
+    FILE* fp = open_write(filename, &rc);
+    if (!fp) return rc;
+    const std::string head =
+        " VARIABLES=\"X \",\"Z \",\"U\",\"V\",\"W\",\"P\",\"fi\",\"Viscosity\"\n" + zone_line(l, m, n, false);
+    rc = write_all(fp, head.data(), head.size(), filename);
+    const long ni = l - 1, nk = n - 1, sx = l + 1, sy = long(m + 1) * sx;
+    const long j = (m - 1) / 2;  // the mid-channel plane
+    if (!rc)
+        rc = write_records(fp, filename, ni * nk, 8, [&](long r, double* v) {
+            const long i = r % ni + 1, k = r / ni;
+            const long c = i + j * sx + k * sy;
+            v[0] = dx / 2.0 + dx * double(i);
+            v[1] = dz / 2.0 + dz * double(k);
+            v[2] = (u3[c] + u3[c + 1]) / 2.0;
+            v[3] = (v3[c] + v3[c + sx]) / 2.0;
+            v[4] = (w3[c] + w3[c + sy]) / 2.0;
+            v[5] = p[c + sy];
+            v[6] = phi[c + 1 + sy];
+            v[7] = visc[c + sy];
+        });
+    return close_file(fp, filename, rc);
+}
+
+namespace {
+
+// The five blocks of backup_channel / dataload_channel (io.F90:647-791):
+// u, v and w with their two convection histories, phi with its two, then
+// pressure and pres_old, each block over the whole array, i fastest.
+struct BackupBlock {
+    int ncols;
+    long count;
+    double* a[3];
+};
+
+std::vector<BackupBlock> backup_blocks(double* u2, double* v2, double* w2, double* c0u, double* c1u, double* c0v,
+                                       double* c1v, double* c0w, double* c1w, double* pold, double* pres, double* phi,
+                                       double* c0p, double* c1p, int l, int m, int n)
+{
+    const long plane = long(l + 1) * (m + 1);
+    return {{3, plane * (n - 1), {u2, c0u, c1u}},
+            {3, plane * (n - 1), {v2, c0v, c1v}},
+            {3, plane * n, {w2, c0w, c1w}},
+            {3, plane * (n + 1), {phi, c0p, c1p}},
+            {2, plane * (n + 1), {pres, pold, nullptr}}};
+}
+
+}  // namespace
+
+int tpls_backup_channel(const double* u2, const double* v2, const double* w2, const double* conv0_u,
+                        const double* conv1_u, const double* conv0_v, const double* conv1_v, const double* conv0_w,
+                        const double* conv1_w, const double* pres_old, const double* pres, const double* phi,
+                        const double* conv0_phi, const double* conv1_phi, int l, int m, int n, const char* filename)
+{
+    const double* all[] = {u2, v2, w2, conv0_u, conv1_u, conv0_v, conv1_v, conv0_w, conv1_w,
+                           pres_old, pres, phi, conv0_phi, conv1_phi};
+    for (const double* a : all)
+        if (!a) return set_error(TPLS_ERR_ARG, "null array");
+    if (!filename || !grid_ok(l, m, n)) return set_error(TPLS_ERR_ARG, "bad argument");
+    auto blocks = backup_blocks(const_cast<double*>(u2), const_cast<double*>(v2), const_cast<double*>(w2),
+                                const_cast<double*>(conv0_u), const_cast<double*>(conv1_u),
+                                const_cast<double*>(conv0_v), const_cast<double*>(conv1_v),
+                                const_cast<double*>(conv0_w), const_cast<double*>(conv1_w),
+                                const_cast<double*>(pres_old), const_cast<double*>(pres), const_cast<double*>(phi),
+                                const_cast<double*>(conv0_phi), const_cast<double*>(conv1_phi), l, m, n);
+    int rc;
+    FILE* fp = open_write(filename, &rc);
+    if (!fp) return rc;
+    for (const auto& b : blocks) {
+        if (rc) break;
+        rc = write_records(fp, filename, b.count, b.ncols, [&](long r, double* v) {
+            for (int c = 0; c < b.ncols; ++c) v[c] = b.a[c][r];
+        });
+    }
+    return close_file(fp, filename, rc);
+}
+
+int tpls_dataload_channel(double* u2, double* v2, double* w2, double* conv0_u, double* conv1_u, double* conv0_v,
+                          double* conv1_v, double* conv0_w, double* conv1_w, double* pres_old, double* pres,
+                          double* phi, double* conv0_phi, double* conv1_phi, int l, int m, int n, const char* filename)
+{
+    double* all[] = {u2, v2, w2, conv0_u, conv1_u, conv0_v, conv1_v, conv0_w, conv1_w,
+                     pres_old, pres, phi, conv0_phi, conv1_phi};
+    for (double* a : all)
+        if (!a) return set_error(TPLS_ERR_ARG, "null array");
+    if (!filename || !grid_ok(l, m, n)) return set_error(TPLS_ERR_ARG, "bad argument");
+    auto blocks = backup_blocks(u2, v2, w2, conv0_u, conv1_u, conv0_v, conv1_v, conv0_w, conv1_w, pres_old, pres, phi,
+                                conv0_phi, conv1_phi, l, m, n);
+    MappedFile f;
+    int rc = f.open(filename);
+    if (rc) return rc;
+    // the blocks follow each other with no separator: find where each starts
+    // by counting value records, then parse the blocks in turn
+    const char* p = f.p;
+    const char* e = f.p + f.n;
+    for (const auto& b : blocks) {
+        const char* q = p;
+        long seen = 0;
+        while (q < e && seen < b.count) {
+            const char* re = record_end(q, e);
+            for (const char* c = q; c < re; ++c)
+                if (!is_blank(*c)) {
+                    ++seen;
+                    break;
+                }
+            q = std::min(re + 1, e);
+        }
+        if (seen < b.count)
+            return set_error(TPLS_ERR_FORMAT, std::string(filename) + ": ends inside a backup block");
+        long got = read_records(p, q, filename, b.ncols, b.count, [&](long r, const double* v, const bool* g) {
+            for (int c = 0; c < b.ncols; ++c)
+                if (g[c]) b.a[c][r] = v[c];
+        });
+        if (got < 0) return int(got);
+        p = q;
+    }
+    return TPLS_OK;
+}
+
 // ---- options ----
 
 int tpls_options_create(tpls_options** out)
@@ -819,7 +1094,7 @@ int tpls_get_integer_option(const tpls_options* o, const char* name, int* value,
     const char* p = s.data();
     const char *tb, *te;
     Item it = next_item(p, s.data() + s.size(), &tb, &te);
-    if (it == ITEM_NULL) return TPLS_OK;
+    if (it == ITEM_NULL || it == ITEM_END) return TPLS_OK;
     long v;
     if (it != ITEM_VALUE || !parse_int(tb, te, &v) || v < -2147483648L || v > 2147483647L)
         return bad_value(name, s, "an integer", ierr);
@@ -836,7 +1111,7 @@ int tpls_get_double_option(const tpls_options* o, const char* name, double* valu
     const char* p = s.data();
     const char *tb, *te;
     Item it = next_item(p, s.data() + s.size(), &tb, &te);
-    if (it == ITEM_NULL) return TPLS_OK;
+    if (it == ITEM_NULL || it == ITEM_END) return TPLS_OK;
     if (it != ITEM_VALUE || !parse_real(tb, te, value)) return bad_value(name, s, "a real number", ierr);
     return TPLS_OK;
 }
@@ -850,7 +1125,7 @@ int tpls_get_logical_option(const tpls_options* o, const char* name, int* value,
     const char* p = s.data();
     const char *tb, *te;
     Item it = next_item(p, s.data() + s.size(), &tb, &te);
-    if (it == ITEM_NULL) return TPLS_OK;
+    if (it == ITEM_NULL || it == ITEM_END) return TPLS_OK;
     if (it != ITEM_VALUE || !parse_logical(tb, te, value)) return bad_value(name, s, "a logical", ierr);
     return TPLS_OK;
 }

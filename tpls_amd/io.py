@@ -88,6 +88,13 @@ def _io():
         L.tpls_read_config.argtypes = [s, s, C.POINTER(_lib.Params), C.POINTER(RunConfig), ip]
         L.tpls_case_load_config.argtypes = [C.c_void_p, s, ip]
         L.tpls_case_save_field.argtypes = [C.c_void_p, i, s, s]
+        L.tpls_save_backup_info.argtypes = [s] + [i] * 9
+        L.tpls_channel_output_phi.argtypes = [_D, i, i, i, C.c_double, C.c_double, C.c_double, s]
+        L.tpls_channel_input_phi.argtypes = [s, _D, i, i, i, ip]
+        L.tpls_channel_output_uvw.argtypes = [_D] * 4 + [i] * 3 + [C.c_double] * 3 + [s]
+        L.tpls_channel_output2d.argtypes = [_D] * 6 + [i] * 3 + [C.c_double] * 3 + [s]
+        L.tpls_backup_channel.argtypes = [_D] * 14 + [i] * 3 + [s]
+        L.tpls_dataload_channel.argtypes = [_D] * 14 + [i] * 3 + [s]
         L.tpls_case_checkpoint.argtypes = [C.c_void_p, s]
         L.tpls_case_restore.argtypes = [C.c_void_p, s]
         L.tpls_case_steps.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
@@ -157,6 +164,81 @@ def load_data_into_array(filename, data: np.ndarray, lx: int, ly: int, mx: int, 
     check(_io().tpls_load_data_into_array(_b(filename), data.ctypes.data_as(_D), lx, ly, mx, my, nx, ny,
                                           C.byref(ierr)), "load_data_into_array")
     return ierr.value
+
+
+# ---------------------------------------------------------------------------
+# channel outputs and the text backup (io.F90:171-791)
+
+def _grid_arrays(l, m, n, **arrays):
+    """Check each array's Fortran shape (u/v: n-1 planes, w: n, p-grid: n+1)
+    and return pointers to Fortran-ordered float64 copies."""
+    planes = {"uv": n - 1, "w": n, "p": n + 1}
+    keep, ptrs = [], []
+    for name, (a, grid) in arrays.items():
+        want = (l + 1, m + 1, planes[grid])
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        if a.shape != want:
+            raise ValueError(f"{name}: shape {a.shape}, expected {want}")
+        keep.append(a)
+        ptrs.append(a.ctypes.data_as(_D))
+    return keep, ptrs
+
+
+def save_backup_info(filename, lx, ly, mx, my, nx, ny, dx, dy, dz) -> None:
+    """io.F90:180-200 (the reference declares the spacings integer)."""
+    check(_io().tpls_save_backup_info(_b(filename), lx, ly, mx, my, nx, ny, int(dx), int(dy), int(dz)),
+          "save_backup_info")
+
+
+def channel_output_phi(phi, l, m, n, dx, dy, dz, filename) -> None:
+    """io.F90:217-246."""
+    keep, ptrs = _grid_arrays(l, m, n, phi=(phi, "p"))
+    check(_io().tpls_channel_output_phi(*ptrs, l, m, n, dx, dy, dz, _b(filename)), "channel_output_phi")
+
+
+def channel_input_phi(filename, phi: np.ndarray, l, m, n) -> int:
+    """io.F90:503-534: fills the interior records into phi (a
+    Fortran-contiguous float64 array of shape (l+1, m+1, n+1)); returns ierr."""
+    if not (isinstance(phi, np.ndarray) and phi.dtype == np.float64 and phi.flags.f_contiguous
+            and phi.shape == (l + 1, m + 1, n + 1)):
+        raise TypeError("phi must be a Fortran-contiguous float64 array of shape (l+1, m+1, n+1)")
+    ierr = C.c_int(0)
+    check(_io().tpls_channel_input_phi(_b(filename), phi.ctypes.data_as(_D), l, m, n, C.byref(ierr)),
+          "channel_input_phi")
+    return ierr.value
+
+
+def channel_output_uvw(u3, v3, w3, p, l, m, n, dx, dy, dz, filename) -> None:
+    """io.F90:551-587."""
+    keep, ptrs = _grid_arrays(l, m, n, u3=(u3, "uv"), v3=(v3, "uv"), w3=(w3, "w"), p=(p, "p"))
+    check(_io().tpls_channel_output_uvw(*ptrs, l, m, n, dx, dy, dz, _b(filename)), "channel_output_uvw")
+
+
+def channel_output2d(u3, v3, w3, p, phi, visc, l, m, n, dx, dy, dz, filename) -> None:
+    """io.F90:604-643."""
+    keep, ptrs = _grid_arrays(l, m, n, u3=(u3, "uv"), v3=(v3, "uv"), w3=(w3, "w"), p=(p, "p"), phi=(phi, "p"),
+                              visc=(visc, "p"))
+    check(_io().tpls_channel_output2d(*ptrs, l, m, n, dx, dy, dz, _b(filename)), "channel_output2d")
+
+
+BACKUP_ARRAYS = (("u2", "uv"), ("v2", "uv"), ("w2", "w"), ("conv0_u", "uv"), ("conv1_u", "uv"), ("conv0_v", "uv"),
+                 ("conv1_v", "uv"), ("conv0_w", "w"), ("conv1_w", "w"), ("pres_old", "p"), ("pres", "p"),
+                 ("phi", "p"), ("conv0_phi", "p"), ("conv1_phi", "p"))
+
+
+def backup_channel(arrays: dict, l, m, n, filename) -> None:
+    """io.F90:647-716; arrays maps the names of BACKUP_ARRAYS to arrays."""
+    keep, ptrs = _grid_arrays(l, m, n, **{k: (arrays[k], g) for k, g in BACKUP_ARRAYS})
+    check(_io().tpls_backup_channel(*ptrs, l, m, n, _b(filename)), "backup_channel")
+
+
+def dataload_channel(filename, l, m, n) -> dict:
+    """io.F90:721-791: returns the arrays of BACKUP_ARRAYS read from a backup."""
+    planes = {"uv": n - 1, "w": n, "p": n + 1}
+    out = {k: np.zeros((l + 1, m + 1, planes[g]), order="F") for k, g in BACKUP_ARRAYS}
+    check(_io().tpls_dataload_channel(*[out[k].ctypes.data_as(_D) for k, _ in BACKUP_ARRAYS], l, m, n,
+                                      _b(filename)), "dataload_channel")
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -343,15 +425,16 @@ def write_initial_conditions(config_dir, p, fields, dpdl: float = -1.0, omega_ma
                    ["TPLS initial conditions", "Auto-generated by tpls_amd.io"])
 
 
-def write_options_file(filename, p, n_timesteps: int = 1, maxu: float = 10.0) -> None:
+def write_options_file(filename, p, n_timesteps: int = 1, maxu: float = 10.0, **options) -> None:
     """A tpls_config.opt for a case: the keys configure.F90 requires, one
-    process grid cell, and p's iteration counts."""
+    process grid cell, and p's iteration counts; options override or add
+    keys (phi_dat_frequency=2, backup_text_format=False, ...)."""
     o = Options()
-    for name, value in (("num_procs_x", 1), ("num_procs_y", 1), ("auto_decomp", False),
-                        ("mom_u", p.mom_u), ("mom_v", p.mom_v), ("mom_w", p.mom_w),
-                        ("levelset", p.levelset_its), ("maxu", float(maxu)),
-                        ("phi_dat_frequency", 1), ("uvw_dat_frequency", 1), ("backup_frequency", 1),
-                        ("backup_text_format", True), ("backup_hdf5_format", False),
-                        ("num_timesteps", n_timesteps)):
+    keys = {"num_procs_x": 1, "num_procs_y": 1, "auto_decomp": False, "mom_u": p.mom_u, "mom_v": p.mom_v,
+            "mom_w": p.mom_w, "levelset": p.levelset_its, "maxu": float(maxu), "phi_dat_frequency": 1,
+            "uvw_dat_frequency": 1, "backup_frequency": 1, "backup_text_format": True,
+            "backup_hdf5_format": False, "num_timesteps": n_timesteps}
+    keys.update(options)
+    for name, value in keys.items():
         o.put_typed_option(name, value)
     o.save_options(filename, ["TPLS configuration file"])
