@@ -52,7 +52,7 @@ def subroutine_args(text):
 def test_modules_built_with_reference_names():
     if not os.path.isdir(MODDIR):
         pytest.skip("Fortran modules not built (amdflang absent)")
-    for mod in list(MODULES) + ["tpls_core_c", "tpls_pressure"]:
+    for mod in list(MODULES) + ["tpls_core_c", "tpls_pressure", "tpls_io_native"]:
         assert os.path.exists(os.path.join(MODDIR, mod + ".mod")), mod
     assert os.access(DRIVER, os.X_OK)
 
@@ -154,3 +154,23 @@ def test_modules_advect_phi_and_tpls(tmp_path):
                           [(S["p"], np.float64), ((), np.float64)])
     assert relerr(p3g[:, 1:MAXM], p3o[:, 1:MAXM]) < 1e-12
     assert eg == pytest.approx(eo, rel=1e-9)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent")
+def test_io_argument_lists_match_reference():
+    """Module tpls_io_native's procedures keep io.F90's dummy argument lists."""
+    ours = subroutine_args(open(MODSRC).read())
+    ref = subroutine_args(open(os.path.join(REF_SRC, "io.F90")).read())
+    for name in ("save_data", "load_data", "load_data_into_array"):
+        assert ours[name] == ref[name], (name, ours[name], ref[name])
+
+
+def test_io_native_module_known_answers(tmp_path):
+    """io_test.f90's known answers through the Fortran module (host code, no
+    GPU): tests/fortran/drive_io_native.f90."""
+    exe = os.path.join(ROOT, "tests", "fortran", "drive_io_native")
+    if not os.access(exe, os.X_OK):
+        pytest.skip("tests/fortran/drive_io_native not built")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("ALL PASSED")

@@ -30,6 +30,7 @@ DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "drive_modules.f90")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_modules")
 DRIVE_IO_SRC = os.path.join(ROOT, "tests", "fortran", "drive_io.f90")
 DRIVE_IO = os.path.join(ROOT, "tests", "fortran", "drive_io")
+DRIVE_IO_NATIVE = os.path.join(ROOT, "tests", "fortran", "drive_io_native")
 FC = os.environ.get("TPLS_FC", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -95,8 +96,14 @@ def build_fortran(verbose: bool = True) -> None:
     _run([FC, "-O2", "-module-dir", FMOD, "-I", FMOD, "-c", DRIVER_SRC, "-o", dobj], "amdflang drive_modules.f90")
     _run([FC, "-o", DRIVER, dobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
          "link drive_modules")
-    # Fortran list-directed I/O in the data-file layout (tests/test_io.py)
+    # Fortran list-directed I/O in the data-file layout, and io_test.f90's
+    # known answers through module tpls_io_native (tests/test_io.py)
     _run([FC, "-O2", "-module-dir", FMOD, "-o", DRIVE_IO, DRIVE_IO_SRC], "amdflang drive_io.f90")
+    nobj = os.path.join(FMOD, "drive_io_native.o")
+    _run([FC, "-O2", "-module-dir", FMOD, "-I", FMOD, "-c", DRIVE_IO_NATIVE + ".f90", "-o", nobj],
+         "amdflang drive_io_native.f90")
+    _run([FC, "-o", DRIVE_IO_NATIVE, nobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
+         "link drive_io_native")
     if verbose:
         print(f"built {FLIB} and {DRIVER}")
 

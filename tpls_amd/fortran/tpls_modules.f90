@@ -522,3 +522,90 @@ contains
         abstol, dtol, maxits, its, rnorm, reason), 'tpls_ksp_solve')
   end subroutine tpls_ksp_solve
 end module tpls_pressure
+
+
+! The data-file procedures of io.F90:47-169 with their dummy argument lists,
+! on the library's parallel reader and writer (include/tpls_io.h): a drop-in
+! for the reference's tpls_io procedures of the same names (the module's
+! other procedures write NetCDF and channel outputs and stay the reference's).
+! The statuses are the reference's; a file the library cannot open or parse
+! stops the program with its message, as the Fortran runtime would.
+module tpls_io_native
+  use iso_c_binding
+  use tpls_core_c, only: tpls_check
+  implicit none
+  private
+  public :: save_data, load_data, load_data_into_array
+  interface
+    integer(c_int) function c_save_data(filename, description, data, lx, ly, mx, my, nx, ny) &
+        bind(C, name="tpls_save_data")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: filename(*), description(*)
+      real(c_double), intent(in) :: data(*)
+      integer(c_int), value :: lx, ly, mx, my, nx, ny
+    end function
+    integer(c_int) function c_data_bounds(filename, bounds) bind(C, name="tpls_data_bounds")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: filename(*)
+      integer(c_int), intent(out) :: bounds(6)
+    end function
+    integer(c_int) function c_load_data(filename, data, bounds, ierr) bind(C, name="tpls_load_data")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), intent(in) :: filename(*)
+      type(c_ptr), intent(out) :: data
+      integer(c_int), intent(out) :: bounds(6), ierr
+    end function
+    integer(c_int) function c_load_data_into_array(filename, data, lx, ly, mx, my, nx, ny, ierr) &
+        bind(C, name="tpls_load_data_into_array")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: filename(*)
+      real(c_double), intent(inout) :: data(*)
+      integer(c_int), value :: lx, ly, mx, my, nx, ny
+      integer(c_int), intent(out) :: ierr
+    end function
+    subroutine c_free(p) bind(C, name="tpls_free")
+      import :: c_ptr
+      type(c_ptr), value :: p
+    end subroutine
+  end interface
+contains
+  subroutine save_data(filename, description, data, lx, ly, mx, my, nx, ny)
+    character(len=*), intent(in) :: filename
+    character(len=*), intent(in) :: description
+    integer, intent(in) :: lx, ly, mx, my, nx, ny
+    double precision, intent(in) :: data(lx:ly, mx:my, nx:ny)
+    call tpls_check(c_save_data(trim(filename) // c_null_char, description // c_null_char, data, &
+        lx, ly, mx, my, nx, ny), 'save_data')
+  end subroutine save_data
+
+  subroutine load_data(filename, data, lx, ly, mx, my, nx, ny, ierr)
+    character(len=*), intent(in) :: filename
+    double precision, allocatable, dimension(:,:,:), intent(out) :: data
+    integer, intent(out) :: lx, ly, mx, my, nx, ny, ierr
+    type(c_ptr) :: p
+    integer(c_int) :: b(6), e
+    real(c_double), pointer :: flat(:)
+    integer :: n
+    call tpls_check(c_load_data(trim(filename) // c_null_char, p, b, e), 'load_data')
+    lx = b(1); ly = b(2); mx = b(3); my = b(4); nx = b(5); ny = b(6)
+    allocate(data(lx:ly, mx:my, nx:ny))
+    n = size(data)
+    if (n > 0) then
+      call c_f_pointer(p, flat, [n])
+      data = reshape(flat, shape(data))
+    end if
+    call c_free(p)
+    ierr = e
+  end subroutine load_data
+
+  subroutine load_data_into_array(filename, data, lx, ly, mx, my, nx, ny, ierr)
+    character(len=*), intent(in) :: filename
+    double precision, allocatable, dimension(:,:,:), intent(inout) :: data
+    integer, intent(in) :: lx, ly, mx, my, nx, ny
+    integer, intent(out) :: ierr
+    integer(c_int) :: e
+    call tpls_check(c_load_data_into_array(trim(filename) // c_null_char, data, lx, ly, mx, my, nx, ny, e), &
+        'load_data_into_array')
+    ierr = e
+  end subroutine load_data_into_array
+end module tpls_io_native
