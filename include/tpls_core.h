@@ -83,6 +83,12 @@ typedef struct {
                              tiles of 2/4/8 rows (0: default, 4); -1: the
                              unstaged sweep with tile_rows instead (same
                              results either way) */
+    int mg_fp32;          /* precision of the multigrid V-cycle (ksp_pc >= 1):
+                             0 default, FP32 in TPLS_FP32 cases and FP64 in
+                             TPLS_FP64 cases; 1 FP32; -1 FP64.  The CG's inner
+                             products, residual and stopping test stay FP64
+                             either way; an FP32 cycle changes the iterates
+                             within the solve's tolerance, not the tolerance */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */

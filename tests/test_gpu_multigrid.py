@@ -82,3 +82,37 @@ def test_mg_and_plain_cg_agree():
     assert out["mg"][2] < out["none"][2] / 5
     for k, v in errs.items():
         assert v < 1e-8, (k, v)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fp32_vcycle_agrees_with_fp64_vcycle(precision):
+    """The FP32 V-cycle (the default in FP32 cases) against the FP64 one on
+    the same case: the CG around it stays FP64 with the same stopping test,
+    so the fields after 5 steps differ by no more than the solver tolerance
+    allows (FP64 case: 1e-8, as plain CG against multigrid; FP32 case: the
+    FP32 storage's 1e-6), and the iteration counts stay close."""
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    p = CaseParams(nx=64, ny=16, nz=32, Re=100.0, mu_plus=2.0, dt=1e-4, seed=9, ksp_pc="mg")
+    init = make_initial(p)
+    out = {}
+    for m32 in (False, True):
+        print(f"[mg-fp32] 64x16x32 {precision} fp32 V-cycle {m32}", flush=True)
+        with Case(p, precision=precision, mg_fp32=m32) as c:
+            c.load(*init)
+            its = []
+            for _ in range(5):
+                c.step(1)
+                its.append(c.diag()["ksp_its"])
+            out[m32] = ({f: c.field(f) for f in ("u2", "v2", "w2", "phi2")},
+                        demean(c.field("pres"), p.maxl, p.maxm - 1, p.maxn), its)
+    P = p.maxm - 1
+    errs = {f: relerr(rows(out[True][0][f], P), rows(out[False][0][f], P)) for f in out[True][0]}
+    errs["pres"] = relerr(rows(out[True][1], P), rows(out[False][1], P))
+    print("[mg-fp32] its", out[False][2], out[True][2], " ".join(f"{k}={v:.1e}" for k, v in errs.items()),
+          flush=True)
+    tol = 1e-8 if precision == "fp64" else 1e-6
+    for k, v in errs.items():
+        assert v < tol, (k, v)
+    for a, b in zip(out[False][2], out[True][2]):
+        assert abs(a - b) <= 2

@@ -29,7 +29,8 @@ class Params(C.Structure):
                [("ksp_maxits", C.c_int), ("precision", C.c_int), ("diagnostics", C.c_int),
                 ("ksp_pc", C.c_int), ("graphs", C.c_int), ("tile_rows", C.c_int),
                 ("tile_rows_odd", C.c_int), ("prefetch_odd", C.c_int), ("sor_fused", C.c_int),
-                ("tile_rows_rb", C.c_int), ("tile_rows_rb_odd", C.c_int), ("tile_rows_jst", C.c_int)]
+                ("tile_rows_rb", C.c_int), ("tile_rows_rb_odd", C.c_int), ("tile_rows_jst", C.c_int),
+                ("mg_fp32", C.c_int)]
 
 
 class Diag(C.Structure):

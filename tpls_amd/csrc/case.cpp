@@ -450,6 +450,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
     auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
     if (prm->sor_fused < -1 || prm->sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
+    if (prm->mg_fp32 < -1 || prm->mg_fp32 > 1) return fail(TPLS_ERR_ARG, "mg_fp32 must be -1, 0 or 1");
     if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) || (prm->tile_rows_jst != -1 && !rb_ok(prm->tile_rows_jst)))
         return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
                                   "tile_rows_jst -1 (unstaged), 0 (default), 2, 4 or 8");
@@ -514,16 +515,35 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         c->use_mg = true;
         c->mg.nu = prm->ksp_pc;
         c->mg.nlev = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, c->mg.L);
-        if ((rc = alloc_field(c, &c->cg_z, sp, 8))) return bail(rc);
-        c->mg.L[0].x = c->cg_z->view<double>();
+        // the FP32 cycle needs a coarse level (a one-level hierarchy solves
+        // on the FP64 residual itself)
+        c->mg.fp32 = c->mg.nlev > 1 && (prm->mg_fp32 > 0 || (prm->mg_fp32 == 0 && prm->precision == TPLS_FP32));
         c->mg.L[0].b = c->cg_r->view<double>();
-        for (int l = 1; l < c->mg.nlev; ++l) {
-            MgLevel& o = c->mg.L[l];
-            const Shape sl{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
-            DevArray *ax, *ab;
-            if ((rc = alloc_field(c, &ax, sl, 8)) || (rc = alloc_field(c, &ab, sl, 8))) return bail(rc);
-            o.x = ax->view<double>();
-            o.b = ab->view<double>();
+        if (c->mg.fp32) {
+            for (int l = 0; l < c->mg.nlev; ++l) {
+                MgLevel32& o = c->mg.F[l];
+                const MgLevel& d = c->mg.L[l];
+                const Shape sl = l ? Shape{0, d.nx + 1, 0, d.ny + 1, 0, d.nz + 1} : sp;
+                DevArray* ax;
+                if ((rc = alloc_field(c, &ax, sl, 4))) return bail(rc);
+                o.x = ax->view<float>();
+                if (l) {
+                    DevArray* ab;
+                    if ((rc = alloc_field(c, &ab, sl, 4))) return bail(rc);
+                    o.b = ab->view<float>();
+                }
+            }
+        } else {
+            if ((rc = alloc_field(c, &c->cg_z, sp, 8))) return bail(rc);
+            c->mg.L[0].x = c->cg_z->view<double>();
+            for (int l = 1; l < c->mg.nlev; ++l) {
+                MgLevel& o = c->mg.L[l];
+                const Shape sl{0, o.nx + 1, 0, o.ny + 1, 0, o.nz + 1};
+                DevArray *ax, *ab;
+                if ((rc = alloc_field(c, &ax, sl, 8)) || (rc = alloc_field(c, &ab, sl, 8))) return bail(rc);
+                o.x = ax->view<double>();
+                o.b = ab->view<double>();
+            }
         }
     }
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);
@@ -896,13 +916,17 @@ int ps_enqueue_chunk(tpls_case* c)
     const Grid& g = c->g;
     const bool mg = c->use_mg;
     FieldView<double> x = c->pold->view<double>(), r = c->cg_r->view<double>();
-    FieldView<double> z = mg ? c->cg_z->view<double>() : r;
+    const bool z32 = mg && c->mg.fp32;
+    FieldView<double> z = mg && !z32 ? c->cg_z->view<double>() : r;
     DevArray* pb[2] = {c->cg_p0, c->cg_p1};
     long chunk = std::min<long>(c->ps_chunk, std::max<long>(1, c->prm.ksp_maxits - c->ps_launched));
     c->ps_chunk = chunk;
     auto iteration = [&](int cur) -> int {
         FieldView<double> po = pb[cur]->view<double>(), pn = pb[cur ^ 1]->view<double>();
-        LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
+        if (z32)
+            LCHK(launch_cg_p(c->s, g, c->ph, c->mg.F[0].x, po, pn, 1, red_of(c, 7), c->cg));
+        else
+            LCHK(launch_cg_p(c->s, g, c->ph, z, po, pn, mg ? 1 : 0, red_of(c, 7), c->cg));
         const bool tx = c->timed == TPLS_TIME_CG_X;
         if (tx) LCHK(hipEventRecord(next_event(c), c->s));
         LCHK(launch_cg_x(c->s, g, c->ph, pn, x, r, red_of(c, 7), c->cg));

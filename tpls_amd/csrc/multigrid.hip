@@ -23,6 +23,13 @@
 // order-independent and the kernels reproduce the oracle's values; FMA
 // contraction is off in this file for the same reason.
 //
+// Precision: the kernels are templates on the storage type of a level's x,
+// which is also the type of its arithmetic.  The FP64 hierarchy is the
+// definition above.  The FP32 hierarchy (cases stored in FP32) runs the same
+// cycle on FP32 vectors and arithmetic; its finest level reads the CG
+// residual in FP64 and the CG reads z back into FP64, so the CG's inner
+// products, residual and stopping test stay FP64.
+//
 // Kernels (one thread per cell unless noted):
 //   k_mg_pre     the first pre-smoothing sweep pair from x = 0, fused: red
 //                cells take b/d, black cells form their red neighbours'
@@ -37,6 +44,8 @@
 //                CG_BLOCKS partials);
 //   k_mg_zf      one block: mz = mean z, beta = z.r - mz sum r and the
 //                PETSc breakdown checks of a preconditioned CG step.
+#include <type_traits>
+
 #include "reduce.hpp"
 #include "tiles.hpp"
 #include "tpls_kernels.hpp"
@@ -45,28 +54,32 @@
 
 namespace tpls {
 
-__host__ __device__ __forceinline__ double mg_deff(const MgLevel& o, int i, int k)
+// Arithmetic of a level is done in C, the storage type of its x (double for
+// the FP64 hierarchy, which is the oracle's operation for operation; float
+// for the FP32 one).
+template <typename C, typename L>
+__host__ __device__ __forceinline__ C mg_deff(const L& o, int i, int k)
 {
-    double d = o.diag;
-    if (k == 1) d = d - o.cz;
-    if (k == o.nz) d = d - o.cz;
-    if (i == 1) d = d - o.cx;
-    if (i == o.nx) d = d - o.cx;
+    C d = C(o.diag);
+    if (k == 1) d = d - C(o.cz);
+    if (k == o.nz) d = d - C(o.cz);
+    if (i == 1) d = d - C(o.cx);
+    if (i == o.nx) d = d - C(o.cx);
     return d;
 }
 
 // sum of c * neighbour over the interior neighbours of (i,j,k); v(i,j,k)
 // gives the neighbour values
-template <typename V>
-__device__ __forceinline__ double mg_nsum(const MgLevel& o, const V& v, int i, int j, int k)
+template <typename C, typename L, typename V>
+__device__ __forceinline__ C mg_nsum(const L& o, const V& v, int i, int j, int k)
 {
     const int jm = j == 1 ? o.ny : j - 1, jp = j == o.ny ? 1 : j + 1;
-    const double zm = k > 1 ? v(i, j, k - 1) : 0.0;
-    const double zp = k < o.nz ? v(i, j, k + 1) : 0.0;
-    const double xm = i > 1 ? v(i - 1, j, k) : 0.0;
-    const double xp = i < o.nx ? v(i + 1, j, k) : 0.0;
-    const double ym = v(i, jm, k), yp = v(i, jp, k);
-    return o.cz * (zm + zp) + o.cx * (xm + xp) + o.cy * (ym + yp);
+    const C zm = k > 1 ? C(v(i, j, k - 1)) : C(0);
+    const C zp = k < o.nz ? C(v(i, j, k + 1)) : C(0);
+    const C xm = i > 1 ? C(v(i - 1, j, k)) : C(0);
+    const C xp = i < o.nx ? C(v(i + 1, j, k)) : C(0);
+    const C ym = C(v(i, jm, k)), yp = C(v(i, jp, k));
+    return C(o.cz) * (zm + zp) + C(o.cx) * (xm + xp) + C(o.cy) * (ym + yp);
 }
 
 // 1-D trilinear pieces: fine cell f (1-based) takes 3/4 of its parent
@@ -81,7 +94,8 @@ __device__ __forceinline__ int mg_nbr(int f, int nc, bool periodic)
     return n;
 }
 
-__device__ __forceinline__ bool mg_cell(const MgLevel& o, int& i, int& j, int& k)
+template <typename L>
+__device__ __forceinline__ bool mg_cell(const L& o, int& i, int& j, int& k)
 {
     i = blockIdx.x * BX + threadIdx.x + 1;
     j = blockIdx.y * BY + threadIdx.y + 1;
@@ -92,26 +106,30 @@ __device__ __forceinline__ bool mg_cell(const MgLevel& o, int& i, int& j, int& k
 // first pre-smoothing pair (red then black) from x = 0.  One thread per two
 // cells along x, one of each colour (nx is even on every smoothed level), so
 // no lane waits on the other colour's branch.
-__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevel o, const CgState* st)
+template <typename X, typename B>
+__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, const CgState* st)
 {
+    using C = X;
     if (st->done) return;
     const int i0 = 2 * (blockIdx.x * BX + threadIdx.x) + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i0 > o.nx || j > o.ny) return;
     const int ir = ((i0 + j + k) & 1) == 0 ? i0 : i0 + 1;
     const int ib = ir == i0 ? i0 + 1 : i0;
-    auto red = [&](int ii, int jj, int kk) { return (o.b(ii, jj, kk) + 0.0) / mg_deff(o, ii, kk); };
+    auto red = [&](int ii, int jj, int kk) { return (C(o.b(ii, jj, kk)) + C(0)) / mg_deff<C>(o, ii, kk); };
     o.x(ir, j, k) = red(ir, j, k);
-    o.x(ib, j, k) = (o.b(ib, j, k) + mg_nsum(o, red, ib, j, k)) / mg_deff(o, ib, k);
+    o.x(ib, j, k) = (C(o.b(ib, j, k)) + mg_nsum<C>(o, red, ib, j, k)) / mg_deff<C>(o, ib, k);
 }
 
-__global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevel o, int colour, const CgState* st)
+template <typename X, typename B>
+__global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevelT<X, B> o, int colour, const CgState* st)
 {
+    using C = X;
     if (st->done) return;
     int i, j, k;
     if (!mg_cell(o, i, j, k)) return;
     if (((i + j + k) & 1) != colour) return;
-    o.x(i, j, k) = (o.b(i, j, k) + mg_nsum(o, o.x, i, j, k)) / mg_deff(o, i, k);
+    o.x(i, j, k) = (C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k)) / mg_deff<C>(o, i, k);
 }
 
 // Weights of the fine cells 2I-2+q (q = 0..3) feeding coarse cell I under the
@@ -119,7 +137,8 @@ __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevel o, int colour, con
 // its neighbour toward it 1/4 (mg_nbr); w[q] = 0 for a candidate outside a
 // Neumann end.  Sums over q = 0..3 skip zero weights.  A periodic candidate
 // wraps but keeps its position q.
-__device__ __forceinline__ void mg_rweights_q(int I, int nc, int nf, bool periodic, double* w)
+template <typename C>
+__device__ __forceinline__ void mg_rweights_q(int I, int nc, int nf, bool periodic, C* w)
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -127,7 +146,7 @@ __device__ __forceinline__ void mg_rweights_q(int I, int nc, int nf, bool period
         if (f < 1) f = periodic ? f + nf : 0;
         if (f > nf) f = periodic ? f - nf : 0;
         const int par = (f + 1) / 2, nb = mg_nbr(f, nc, periodic);
-        w[q] = f == 0 ? 0.0 : (par == I ? 0.75 : 0.0) + (nb == I ? 0.25 : 0.0);
+        w[q] = f == 0 ? C(0) : (par == I ? C(0.75) : C(0)) + (nb == I ? C(0.25) : C(0));
     }
 }
 
@@ -146,15 +165,19 @@ constexpr int RR_CX = 32, RR_CY = 4, RR_T = 256;
 constexpr int RR_FX = 2 * RR_CX + 2, RR_FY = 2 * RR_CY + 2, RR_N = RR_FX * RR_FY;
 constexpr int RR_C = (RR_N + RR_T - 1) / RR_T;  // footprint cells per thread
 
+template <typename C>
 struct RrIn {
-    double b, c, xm, xp, ym, yp, zm, zp;
+    C b, c, xm, xp, ym, yp, zm, zp;
 };
 
-__global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, int nty, int rk, const CgState* st)
+template <typename XF, typename BF, typename XC>
+__global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, int ntx, int nty, int rk,
+                                                const CgState* st)
 {
+    using C = XF;
     if (st->done) return;
-    __shared__ double T[RR_FY][RR_FX];
-    __shared__ double SI[RR_FY][RR_CX];
+    __shared__ C T[RR_FY][RR_FX];
+    __shared__ C SI[RR_FY][RR_CX];
     const int tid = threadIdx.x;
     const int bk = blockIdx.x, tx = bk % ntx, r0 = bk / ntx, ty = r0 % nty, tz = r0 / nty;
     const int I0 = 1 + tx * RR_CX, J0 = 1 + ty * RR_CY, Kb = 1 + tz * rk, Ke = min(Kb + rk - 1, c.nz);
@@ -173,17 +196,17 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, i
         cjp[n] = cj[n] == f.ny ? 1 : cj[n] + 1;
         live[n] = tid + n * RR_T < RR_N && ci[n] >= 1 && ci[n] <= f.nx;
     }
-    auto load = [&](RrIn* in, int fk) {
+    auto load = [&](RrIn<C>* in, int fk) {
 #pragma unroll
         for (int n = 0; n < RR_C; ++n) {
             if (!live[n]) continue;
             const int i = ci[n], j = cj[n];
-            in[n].b = f.b(i, j, fk);
+            in[n].b = C(f.b(i, j, fk));
             in[n].c = f.x(i, j, fk);
-            in[n].zm = fk > 1 ? f.x(i, j, fk - 1) : 0.0;
-            in[n].zp = fk < f.nz ? f.x(i, j, fk + 1) : 0.0;
-            in[n].xm = i > 1 ? f.x(i - 1, j, fk) : 0.0;
-            in[n].xp = i < f.nx ? f.x(i + 1, j, fk) : 0.0;
+            in[n].zm = fk > 1 ? f.x(i, j, fk - 1) : C(0);
+            in[n].zp = fk < f.nz ? f.x(i, j, fk + 1) : C(0);
+            in[n].xm = i > 1 ? f.x(i - 1, j, fk) : C(0);
+            in[n].xp = i < f.nx ? f.x(i + 1, j, fk) : C(0);
             in[n].ym = f.x(i, cjm[n], fk);
             in[n].yp = f.x(i, cjp[n], fk);
         }
@@ -192,23 +215,23 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, i
     const int cI = tid % RR_CX, cJ = tid / RR_CX;
     const int I = I0 + cI, J = J0 + cJ;
     const bool own = cJ < RR_CY && I <= c.nx && J <= c.ny;
-    double wj[4];
+    C wj[4];
     mg_rweights_q(J, c.ny, f.ny, true, wj);
     const int f_lo = max(2 * Kb - 2, 1), f_hi = min(2 * Ke + 1, f.nz);
-    RrIn in[RR_C];
+    RrIn<C> in[RR_C];
     load(in, f_lo);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // sj of fine planes 2K-2 .. 2K+1
+    C s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // sj of fine planes 2K-2 .. 2K+1
     for (int fk = f_lo; fk <= f_hi; ++fk) {
         // the residual of plane fk over the footprint
 #pragma unroll
         for (int n = 0; n < RR_C; ++n) {
             const int e = tid + n * RR_T;
             if (e >= RR_N) continue;
-            double t = 0.0;  // outside 1..nx: weight 0
+            C t = 0;  // outside 1..nx: weight 0
             if (live[n]) {
-                const RrIn& v = in[n];
-                const double nsum = f.cz * (v.zm + v.zp) + f.cx * (v.xm + v.xp) + f.cy * (v.ym + v.yp);
-                t = v.b - (mg_deff(f, ci[n], fk) * v.c - nsum);
+                const RrIn<C>& v = in[n];
+                const C nsum = C(f.cz) * (v.zm + v.zp) + C(f.cx) * (v.xm + v.xp) + C(f.cy) * (v.ym + v.yp);
+                t = v.b - (mg_deff<C>(f, ci[n], fk) * v.c - nsum);
             }
             T[e / RR_FX][e % RR_FX] = t;
         }
@@ -216,19 +239,19 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, i
         __syncthreads();
         for (int e = tid; e < RR_FY * RR_CX; e += RR_T) {
             const int row = e / RR_CX, cx = e % RR_CX;
-            double wi[4];
+            C wi[4];
             mg_rweights_q(I0 + cx, c.nx, f.nx, false, wi);
-            double si = 0.0;
+            C si = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (wi[q] != 0.0) si += wi[q] * T[row][2 * cx + q];
+                if (wi[q] != C(0)) si += wi[q] * T[row][2 * cx + q];
             SI[row][cx] = si;
         }
         __syncthreads();
-        double sj = 0.0;
+        C sj = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (wj[q] != 0.0) sj += wj[q] * SI[2 * cJ + q < RR_FY ? 2 * cJ + q : 0][cI];
+            if (wj[q] != C(0)) sj += wj[q] * SI[2 * cJ + q < RR_FY ? 2 * cJ + q : 0][cI];
         // plane fk is candidate 2K + (fk & 1) of coarse level K = fk / 2 and
         // candidate 2K - 2 + (fk & 1) of level K + 1
         if (fk & 1) s3 = sj; else s2 = sj;
@@ -236,14 +259,14 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, i
         if ((fk & 1) || fk == f.nz) {
             // plane 2K+1 done (or the last plane): close coarse level K
             if (own && K >= Kb && K <= Ke) {
-                double wk[4];
+                C wk[4];
                 mg_rweights_q(K, c.nz, f.nz, false, wk);
-                const double sv[4] = {s0, s1, s2, s3};
-                double sk = 0.0;
+                const C sv[4] = {s0, s1, s2, s3};
+                C sk = 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (wk[q] != 0.0) sk += wk[q] * sv[q];
-                c.b(I, J, K) = 0.125 * sk;
+                    if (wk[q] != C(0)) sk += wk[q] * sv[q];
+                c.b(I, J, K) = C(0.125) * sk;
             }
             s0 = s2;
             s1 = s3;
@@ -252,20 +275,23 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevel f, MgLevel c, int ntx, i
 }
 
 // x_f += P e_c (z first, then y, then x)
-__global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevel f, MgLevel c, const CgState* st)
+template <typename XF, typename BF, typename XC>
+__global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, const CgState* st)
 {
+    using C = XF;
     if (st->done) return;
     int i, j, k;
     if (!mg_cell(f, i, j, k)) return;
     const int I = (i + 1) / 2, J = (j + 1) / 2, K = (k + 1) / 2;
     const int In = mg_nbr(i, c.nx, false), Jn = mg_nbr(j, c.ny, true), Kn = mg_nbr(k, c.nz, false);
-    const double a00 = 0.75 * c.x(I, J, K) + 0.25 * c.x(I, J, Kn);
-    const double a01 = 0.75 * c.x(I, Jn, K) + 0.25 * c.x(I, Jn, Kn);
-    const double a10 = 0.75 * c.x(In, J, K) + 0.25 * c.x(In, J, Kn);
-    const double a11 = 0.75 * c.x(In, Jn, K) + 0.25 * c.x(In, Jn, Kn);
-    const double b0 = 0.75 * a00 + 0.25 * a01;
-    const double b1 = 0.75 * a10 + 0.25 * a11;
-    f.x(i, j, k) = f.x(i, j, k) + (0.75 * b0 + 0.25 * b1);
+    const C q3 = C(0.75), q1 = C(0.25);
+    const C a00 = q3 * C(c.x(I, J, K)) + q1 * C(c.x(I, J, Kn));
+    const C a01 = q3 * C(c.x(I, Jn, K)) + q1 * C(c.x(I, Jn, Kn));
+    const C a10 = q3 * C(c.x(In, J, K)) + q1 * C(c.x(In, J, Kn));
+    const C a11 = q3 * C(c.x(In, Jn, K)) + q1 * C(c.x(In, Jn, Kn));
+    const C b0 = q3 * a00 + q1 * a01;
+    const C b1 = q3 * a10 + q1 * a11;
+    f.x(i, j, k) = f.x(i, j, k) + (q3 * b0 + q1 * b1);
 }
 
 // Coarsest level: one block, a single wavefront for up to 64 cells (its
@@ -275,15 +301,17 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevel f, MgLevel c, con
 // thread in index order.
 constexpr int MGC_T = 256, MGC_CPT = MG_COARSE_MAX / MGC_T;
 
-__global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevel o, const CgState* st)
+template <typename X>
+__global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevelT<X, X> o, const CgState* st)
 {
+    using C = X;
     if (st->done) return;
-    __shared__ double sx[MG_COARSE_MAX], sb[MG_COARSE_MAX];
-    __shared__ double mean;
+    __shared__ C sx[MG_COARSE_MAX], sb[MG_COARSE_MAX];
+    __shared__ C mean;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = o.nx * o.ny * o.nz;
     int ci[MGC_CPT], cj[MGC_CPT], ck[MGC_CPT], col[MGC_CPT];
-    double cd[MGC_CPT];
+    C cd[MGC_CPT];
 #pragma unroll
     for (int c = 0; c < MGC_CPT; ++c) {
         const int q = min(tid + c * nt, n - 1);
@@ -292,23 +320,23 @@ __global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevel o, const CgState* s
         cj[c] = r % o.ny + 1;
         ck[c] = r / o.ny + 1;
         col[c] = tid + c * nt < n ? (ci[c] + cj[c] + ck[c]) & 1 : 2;  // 2: no cell
-        cd[c] = mg_deff(o, ci[c], ck[c]);
+        cd[c] = mg_deff<C>(o, ci[c], ck[c]);
     }
 #pragma unroll
     for (int c = 0; c < MGC_CPT; ++c)
         if (col[c] != 2) {
             const int q = tid + c * nt;
             sb[q] = o.b(ci[c], cj[c], ck[c]);
-            sx[q] = 0.0;
+            sx[q] = 0;
         }
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
+        C s = 0;
         for (int q = 0; q < n; ++q) s += sb[q];
-        mean = s / (double)n;
+        mean = s / C(n);
     }
     __syncthreads();
-    const double mb = mean;
+    const C mb = mean;
     for (int q = tid; q < n; q += nt) sb[q] = sb[q] - mb;
     __syncthreads();
     const int nmax = max(o.nx, max(o.ny, o.nz));
@@ -321,28 +349,32 @@ __global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevel o, const CgState* s
             for (int c = 0; c < MGC_CPT; ++c)
                 if (col[c] == colour) {
                     const int q = tid + c * nt;
-                    sx[q] = (sb[q] + mg_nsum(o, lds, ci[c], cj[c], ck[c])) / cd[c];
+                    sx[q] = (sb[q] + mg_nsum<C>(o, lds, ci[c], cj[c], ck[c])) / cd[c];
                 }
             __syncthreads();
         }
     }
     if (tid == 0) {
-        double s = 0.0;
+        C s = 0;
         for (int q = 0; q < n; ++q) s += sx[q];
-        mean = s / (double)n;
+        mean = s / C(n);
     }
     __syncthreads();
-    const double mx = mean;
+    const C mx = mean;
 #pragma unroll
     for (int c = 0; c < MGC_CPT; ++c)
         if (col[c] != 2) o.x(ci[c], cj[c], ck[c]) = sx[tid + c * nt] - mx;
 }
 
 // Final red sweep on the finest level with the partials of z.r (r = b) and
-// sum z over all cells.  With update = 0 the sweep is skipped (a one-level
-// hierarchy, where the coarsest solve already produced z).
-__global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevel o, Tiles tl, int update, double* part, const CgState* st)
+// sum z over all cells, in FP64 from the stored z.  With update = 0 the sweep
+// is skipped (a one-level hierarchy, where the coarsest solve already
+// produced z).
+template <typename X>
+__global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevelT<X, double> o, Tiles tl, int update, double* part,
+                                                   const CgState* st)
 {
+    using C = X;
     if (st->done) return;
     double acc[2] = {0.0, 0.0};
     for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
@@ -351,15 +383,15 @@ __global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevel o, Tiles tl, int upd
         if (i > o.nx || j > o.ny) continue;
         for (int k = k0; k < k0 + CG_KC && k <= o.nz; ++k) {
             const double bv = o.b(i, j, k);
-            double z;
+            X z;
             if (update && ((i + j + k) & 1) == 0) {
-                z = (bv + mg_nsum(o, o.x, i, j, k)) / mg_deff(o, i, k);
+                z = (C(bv) + mg_nsum<C>(o, o.x, i, j, k)) / mg_deff<C>(o, i, k);
                 o.x(i, j, k) = z;
             } else {
                 z = o.x(i, j, k);
             }
-            acc[0] += z * bv;
-            acc[1] += z;
+            acc[0] += double(z) * bv;
+            acc[1] += double(z);
         }
     }
     block_partials<2>(acc, part, gridDim.x, blockIdx.x);
@@ -414,20 +446,25 @@ int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag
     return l;
 }
 
-static dim3 level_grid(const MgLevel& o) { return dim3((o.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
-static dim3 pair_grid(const MgLevel& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
+template <typename L>
+static dim3 level_grid(const L& o) { return dim3((o.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
+template <typename L>
+static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
 
-hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
+// The V-cycle on a hierarchy whose level 0 is `top` (x stored as X, b the
+// FP64 CG residual) and whose levels 1.. are `lev[1..]` (all X).
+template <typename X>
+static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>& top, const MgLevelT<X, X>* lev,
+                         int nlev, int nu, Reduce red, CgState* st)
 {
     const dim3 blk(BX, BY);
-    const int top = h.nlev - 1;
-    for (int l = 0; l < top; ++l) {
-        const MgLevel& o = h.L[l];
-        const MgLevel& oc = h.L[l + 1];
-        hipLaunchKernelGGL(k_mg_pre, pair_grid(o), blk, 0, s, o, st);
-        for (int q = 1; q < h.nu; ++q) {
-            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 0, st);
-            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 1, st);
+    const int last = nlev - 1;
+    auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
+        using B = std::decay_t<decltype(*o.b.p)>;
+        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, st);
+        for (int q = 1; q < nu; ++q) {
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 1, st);
         }
         // coarse levels per block: as many as keep >= 1024 blocks, up to 8
         // (each block recomputes the two fine planes below its first level)
@@ -435,24 +472,53 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
         int rk = 8;
         while (rk > 1 && long(ntx) * nty * ((oc.nz + rk - 1) / rk) < 1024) rk /= 2;
         const int ntz = (oc.nz + rk - 1) / rk;
-        hipLaunchKernelGGL(k_mg_rr, dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
-    }
-    const long nc = long(h.L[top].nx) * h.L[top].ny * h.L[top].nz;
-    const int ct = nc <= 64 ? 64 : MGC_T;
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(ct), 0, s, h.L[top], st);
-    for (int l = top - 1; l >= 0; --l) {
-        const MgLevel& o = h.L[l];
-        hipLaunchKernelGGL(k_mg_prolong, level_grid(o), blk, 0, s, o, h.L[l + 1], st);
-        for (int q = 0; q < h.nu; ++q) {
-            hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 1, st);
-            if (l > 0 || q < h.nu - 1) hipLaunchKernelGGL(k_mg_smooth, level_grid(o), blk, 0, s, o, 0, st);
+        hipLaunchKernelGGL((k_mg_rr<X, B, X>), dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
+    };
+    auto up = [&](const auto& o, const MgLevelT<X, X>& oc, bool finest) {
+        using B = std::decay_t<decltype(*o.b.p)>;
+        hipLaunchKernelGGL((k_mg_prolong<X, B, X>), level_grid(o), blk, 0, s, o, oc, st);
+        for (int q = 0; q < nu; ++q) {
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 1, st);
+            if (!finest || q < nu - 1) hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
+        }
+    };
+    if (last > 0) down(top, lev[1]);
+    for (int l = 1; l < last; ++l) down(lev[l], lev[l + 1]);
+    if (last > 0) {
+        const MgLevelT<X, X>& c = lev[last];
+        const long nc = long(c.nx) * c.ny * c.nz;
+        hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, c, st);
+    } else {
+        // one level: the coarsest solve runs on level 0 itself (its b is the
+        // FP64 residual only on the FP64 hierarchy, where the types agree)
+        if constexpr (std::is_same_v<X, double>) {
+            const long nc = long(top.nx) * top.ny * top.nz;
+            hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, top, st);
+        } else {
+            return hipErrorInvalidValue;
         }
     }
+    for (int l = last - 1; l >= 1; --l) up(lev[l], lev[l + 1], false);
+    if (last > 0) up(top, lev[1], true);
     const Tiles tl = cg_tiles(g);
-    hipLaunchKernelGGL(k_mg_last, dim3(tl.nblocks), blk, 0, s, h.L[0], tl, top > 0 ? 1 : 0, red.part, st);
-    const double N = double(h.L[0].nx) * h.L[0].ny * h.L[0].nz;
+    hipLaunchKernelGGL((k_mg_last<X>), dim3(tl.nblocks), blk, 0, s, top, tl, last > 0 ? 1 : 0, red.part, st);
+    const double N = double(top.nx) * top.ny * top.nz;
     hipLaunchKernelGGL(k_mg_zf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, N, st);
     return hipGetLastError();
+}
+
+hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
+{
+    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, red, st);
+    // the FP32 vectors with L's extents and coefficients
+    MgLevel32 lev[MG_MAXLEV];
+    for (int l = 0; l < h.nlev; ++l) {
+        const MgLevel& o = h.L[l];
+        lev[l] = MgLevel32{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[l].x, h.F[l].b};
+    }
+    const MgLevel& o = h.L[0];
+    const MgLevelT<float, double> top{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
+    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, red, st);
 }
 
 }  // namespace tpls

@@ -222,15 +222,15 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
 // Each plane of the new p over the block's 64 x 4 columns is formed once and
 // shared through LDS; the x/y neighbours outside the block are formed by the
 // edge lanes / rows.  FIRST is the first iteration (p = z exactly).
-template <bool FIRST>
-__device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const FieldView<double>& r,
+template <bool FIRST, typename Z>
+__device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const FieldView<Z>& r,
                                            const FieldView<double>& po, const FieldView<double>& pn, double m,
                                            double bt, double& acc)
 {
     __shared__ double sp[BY][BX];
     const int tx = threadIdx.x, ty = threadIdx.y;
     auto P = [&](int ii, int jj, int kk) -> double {
-        double z = r(ii, jj, kk) - m;
+        double z = double(r(ii, jj, kk)) - m;
         return FIRST ? z : z + bt * po(ii, jj, kk);
     };
     for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
@@ -266,7 +266,9 @@ __device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const F
     }
 }
 
-__global__ void __launch_bounds__(BX* BY) k_cg_p(Op o, Tiles tl, FieldView<double> r, FieldView<double> po,
+// r: the preconditioned residual z (FP64, or FP32 from the FP32 V-cycle), or r
+template <typename Z>
+__global__ void __launch_bounds__(BX* BY) k_cg_p(Op o, Tiles tl, FieldView<Z> r, FieldView<double> po,
                                                  FieldView<double> pn, int use_mz, double* part, const CgState* st)
 {
     if (st->done) return;
@@ -491,15 +493,20 @@ hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, Field
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> z, FieldView<double> p_old,
+template <typename Z>
+hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<Z> z, FieldView<double> p_old,
                        FieldView<double> p_new, int use_mz, Reduce red, CgState* st)
 {
     Tiles tl = cg_tiles(g);
-    hipLaunchKernelGGL(k_cg_p, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, z, p_old, p_new, use_mz,
+    hipLaunchKernelGGL(k_cg_p<Z>, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, z, p_old, p_new, use_mz,
                        red.part, st);
     hipLaunchKernelGGL(k_cg_pf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, st);
     return hipGetLastError();
 }
+template hipError_t launch_cg_p<double>(hipStream_t, const Grid&, const Phys&, FieldView<double>, FieldView<double>,
+                                        FieldView<double>, int, Reduce, CgState*);
+template hipError_t launch_cg_p<float>(hipStream_t, const Grid&, const Phys&, FieldView<float>, FieldView<double>,
+                                       FieldView<double>, int, Reduce, CgState*);
 
 hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> p, FieldView<double> x,
                        FieldView<double> r, Reduce red, CgState* st)

@@ -136,7 +136,8 @@ hipError_t launch_cg_setup_r(hipStream_t s, const Grid& g, const Phys& ph, Field
                              FieldView<double> r, Reduce red2, CgState* st);
 // p <- (z - mean) + beta/beta_old p with z = r (mean st->m) or, when
 // use_mz, z = M r (mean st->mz)
-hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> z,
+template <typename Z>
+hipError_t launch_cg_p(hipStream_t s, const Grid& g, const Phys& ph, FieldView<Z> z,
                        FieldView<double> p_old, FieldView<double> p_new, int use_mz, Reduce red,
                        CgState* st);
 hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<double> p,
@@ -149,16 +150,27 @@ hipError_t launch_pres_ghosts(hipStream_t s, const Grid& g, const Phys& ph, Fiel
 // One level of the hierarchy: the reduced operator on nx x ny x nz cells
 // (Neumann in x and z, periodic in y) and its vectors, addressed with
 // Fortran indices 1..n.  Level 0 is the pressure grid itself.
-struct MgLevel {
+// One level: x (the smoothed iterate) stored as X, b (its right-hand side) as
+// B; the level's arithmetic is done in X.
+template <typename X, typename B>
+struct MgLevelT {
     int nx, ny, nz;
     double cx, cy, cz, diag;
-    FieldView<double> x, b;
+    FieldView<X> x;
+    FieldView<B> b;
 };
+using MgLevel = MgLevelT<double, double>;
+using MgLevel32 = MgLevelT<float, float>;
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_COARSE_MAX = 2048;  // cells of the coarsest level (one block, LDS)
+// L: extents, coefficients and the FP64 vectors.  With fp32 set the cycle
+// runs on the FP32 vectors of F instead (level 0 reads its right-hand side,
+// the CG residual, from L[0].b in FP64); the coefficients are L's.
 struct MgHierarchy {
     int nlev, nu;
     MgLevel L[MG_MAXLEV];
+    int fp32;
+    MgLevel32 F[MG_MAXLEV];
 };
 // Extents and coefficients of every level (vectors left unset); returns the
 // number of levels.
