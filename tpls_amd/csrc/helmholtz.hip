@@ -64,7 +64,7 @@ __device__ __forceinline__ MuRaw<C> mu_raw(const FieldView<T>& mu, int i, int j,
 template <int COMP, typename C>
 __device__ __forceinline__ MuFaces<C> mu_cook(const MuRaw<C>& r)
 {
-    auto avg = [&](int f) { return (r.a + r.b + r.q[f][0] + r.q[f][1]) / C(4); };
+    auto avg = [&](int f) { return (r.a + r.b + r.q[f][0] + r.q[f][1]) * C(0.25); };
     MuFaces<C> m;
     if (COMP == 0) {
         m.xp = r.b, m.xm = r.a, m.yp = avg(0), m.ym = avg(1), m.zp = avg(2), m.zm = avg(3);

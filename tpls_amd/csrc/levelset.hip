@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(BX* BY) k_levelset_status(Grid g, T h, FieldVi
     ist(i, j, k) = s;
     if (s) {
         T gx = xp - xm, gy = yp - ym, gz = zp - zm;
-        T a1 = sqrt(gx * gx + gy * gy + gz * gz) / T(2);
+        T a1 = sqrt(gx * gx + gy * gy + gz * gz) * T(0.5);
         T mx = fmax(fmax(fmax(fmax(fmax(fmax(fmax(a1, fabs(xp - c)), fabs(xm - c)), fabs(yp - c)), fabs(ym - c)),
                              fabs(zp - c)),
                         fabs(zm - c)),

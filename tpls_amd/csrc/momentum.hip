@@ -26,8 +26,8 @@ __device__ __forceinline__ C conv_u(const FieldView<T>& u2, const FieldView<T>& 
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
     const int j1 = jp, j2 = wrapj(j + 2, P), j0 = j;
-    C tv = (C(v2(i, j, k)) + C(v2(i + 1, j, k)) + C(v2(i, jp, k)) + C(v2(i + 1, jp, k))) / C(4);
-    C tw = (C(w2(i, j, k)) + C(w2(i + 1, j, k)) + C(w2(i, j, k + 1)) + C(w2(i + 1, j, k + 1))) / C(4);
+    C tv = (C(v2(i, j, k)) + C(v2(i + 1, j, k)) + C(v2(i, jp, k)) + C(v2(i + 1, jp, k))) * C(0.25);
+    C tw = (C(w2(i, j, k)) + C(w2(i + 1, j, k)) + C(w2(i, j, k + 1)) + C(w2(i + 1, j, k + 1))) * C(0.25);
     C uc = u2(i, j, k), uxp = u2(i + 1, j, k), uxm = u2(i - 1, j, k);
     C dudx = C(0.5) * (uxp - uxm) / dx;
     C Xc = dudx * uc;
@@ -43,10 +43,10 @@ __device__ __forceinline__ C conv_u(const FieldView<T>& u2, const FieldView<T>& 
     C Zc = dudz * tw;
     C c1 = Xc + Yc + Zc;
     C mxp = mu(i + 1, j1, k + 1), mxm = mu(i, j1, k + 1);
-    C myp = (C(mu(i + 1, j2, k + 1)) + C(mu(i, j2, k + 1)) + mxp + mxm) / C(4);
-    C mym = (C(mu(i + 1, j0, k + 1)) + C(mu(i, j0, k + 1)) + mxp + mxm) / C(4);
-    C mzp = (C(mu(i + 1, j1, k + 2)) + C(mu(i, j1, k + 2)) + mxp + mxm) / C(4);
-    C mzm = (C(mu(i + 1, j1, k)) + C(mu(i, j1, k)) + mxp + mxm) / C(4);
+    C myp = (C(mu(i + 1, j2, k + 1)) + C(mu(i, j2, k + 1)) + mxp + mxm) * C(0.25);
+    C mym = (C(mu(i + 1, j0, k + 1)) + C(mu(i, j0, k + 1)) + mxp + mxm) * C(0.25);
+    C mzp = (C(mu(i + 1, j1, k + 2)) + C(mu(i, j1, k + 2)) + mxp + mxm) * C(0.25);
+    C mzm = (C(mu(i + 1, j1, k)) + C(mu(i, j1, k)) + mxp + mxm) * C(0.25);
     C aE = mxp * (uxp - uc) / dx;
     C aW = mxm * (uc - uxm) / dx;
     C bN = myp * (C(v2(i + 1, jp, k)) - C(v2(i, jp, k))) / dx;
@@ -62,8 +62,8 @@ __device__ __forceinline__ C conv_v(const FieldView<T>& u2, const FieldView<T>& 
                                     const FieldView<T>& mu, int i, int j, int k, int maxn, int P, C dx, C dy, C dz)
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
-    C tu = (C(u2(i - 1, jm, k)) + C(u2(i - 1, j, k)) + C(u2(i, jm, k)) + C(u2(i, j, k))) / C(4);
-    C tw = (C(w2(i, jm, k)) + C(w2(i, j, k)) + C(w2(i, jm, k + 1)) + C(w2(i, j, k + 1))) / C(4);
+    C tu = (C(u2(i - 1, jm, k)) + C(u2(i - 1, j, k)) + C(u2(i, jm, k)) + C(u2(i, j, k))) * C(0.25);
+    C tw = (C(w2(i, jm, k)) + C(w2(i, j, k)) + C(w2(i, jm, k + 1)) + C(w2(i, j, k + 1))) * C(0.25);
     C vc = v2(i, j, k), vyp = v2(i, jp, k), vym = v2(i, jm, k);
     C dvdy = C(0.5) * (vyp - vym) / dy;
     C Yc = dvdy * vc;
@@ -79,11 +79,11 @@ __device__ __forceinline__ C conv_v(const FieldView<T>& u2, const FieldView<T>& 
     C Zc = dvdz * tw;
     C c1 = Xc + Yc + Zc;
     C a = mu(i, jp, k + 1), b = mu(i, j, k + 1);
-    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, j, k + 1)) + a + b) / C(4);
-    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, j, k + 1)) + a + b) / C(4);
+    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, j, k + 1)) + a + b) * C(0.25);
+    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, j, k + 1)) + a + b) * C(0.25);
     C myp = a, mym = b;
-    C mzp = (C(mu(i, jp, k + 2)) + C(mu(i, j, k + 2)) + a + b) / C(4);
-    C mzm = (C(mu(i, jp, k)) + C(mu(i, j, k)) + a + b) / C(4);
+    C mzp = (C(mu(i, jp, k + 2)) + C(mu(i, j, k + 2)) + a + b) * C(0.25);
+    C mzm = (C(mu(i, jp, k)) + C(mu(i, j, k)) + a + b) * C(0.25);
     C aE = mxp * (C(u2(i, j, k)) - C(u2(i, jm, k))) / dy;
     C aW = mxm * (C(u2(i - 1, j, k)) - C(u2(i - 1, jm, k))) / dy;
     C bN = myp * (vyp - vc) / dy;
@@ -99,8 +99,8 @@ __device__ __forceinline__ C conv_w(const FieldView<T>& u2, const FieldView<T>& 
                                     const FieldView<T>& mu, int i, int j, int k, int P, C dx, C dy, C dz)
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P), j2 = wrapj(j + 2, P);
-    C tu = (C(u2(i - 1, j, k - 1)) + C(u2(i - 1, j, k)) + C(u2(i, j, k - 1)) + C(u2(i, j, k))) / C(4);
-    C tv = (C(v2(i, j, k - 1)) + C(v2(i, j, k)) + C(v2(i, jp, k - 1)) + C(v2(i, jp, k))) / C(4);
+    C tu = (C(u2(i - 1, j, k - 1)) + C(u2(i - 1, j, k)) + C(u2(i, j, k - 1)) + C(u2(i, j, k))) * C(0.25);
+    C tv = (C(v2(i, j, k - 1)) + C(v2(i, j, k)) + C(v2(i, jp, k - 1)) + C(v2(i, jp, k))) * C(0.25);
     C wc = w2(i, j, k), wzp = w2(i, j, k + 1), wzm = w2(i, j, k - 1);
     C dwdz = C(0.5) * (wzp - wzm) / dz;
     C Zc = dwdz * wc;
@@ -110,10 +110,10 @@ __device__ __forceinline__ C conv_w(const FieldView<T>& u2, const FieldView<T>& 
     C Yc = dwdy * tv;
     C c1 = Xc + Yc + Zc;
     C a = mu(i, jp, k + 1), b = mu(i, jp, k);
-    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, jp, k)) + a + b) / C(4);
-    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, jp, k)) + a + b) / C(4);
-    C myp = (C(mu(i, j2, k + 1)) + C(mu(i, j2, k)) + a + b) / C(4);
-    C mym = (C(mu(i, j, k + 1)) + C(mu(i, j, k)) + a + b) / C(4);
+    C mxp = (C(mu(i + 1, jp, k + 1)) + C(mu(i + 1, jp, k)) + a + b) * C(0.25);
+    C mxm = (C(mu(i - 1, jp, k + 1)) + C(mu(i - 1, jp, k)) + a + b) * C(0.25);
+    C myp = (C(mu(i, j2, k + 1)) + C(mu(i, j2, k)) + a + b) * C(0.25);
+    C mym = (C(mu(i, j, k + 1)) + C(mu(i, j, k)) + a + b) * C(0.25);
     C mzp = a, mzm = b;
     C aE = mxp * (C(u2(i, j, k)) - C(u2(i, j, k - 1))) / dz;
     C aW = mxm * (C(u2(i - 1, j, k)) - C(u2(i - 1, j, k - 1))) / dz;
@@ -195,16 +195,16 @@ __global__ void __launch_bounds__(BX* BY) k_momentum(Grid g, int comp, int mode,
         // CSF face average, main_ns_hybrid.F90:629-641
         const int jp = wrapj(j + 1, P);
         C s2;
-        if (comp == 0) s2 = (C(fc(i + 1, jp, k + 1)) + C(fc(i, jp, k + 1))) / C(2);
-        else if (comp == 1) s2 = (C(fc(i, jp, k + 1)) + C(fc(i, j, k + 1))) / C(2);
-        else s2 = (C(fc(i, jp, k + 1)) + C(fc(i, jp, k))) / C(2);
+        if (comp == 0) s2 = (C(fc(i + 1, jp, k + 1)) + C(fc(i, jp, k + 1))) * C(0.5);
+        else if (comp == 1) s2 = (C(fc(i, jp, k + 1)) + C(fc(i, j, k + 1))) * C(0.5);
+        else s2 = (C(fc(i, jp, k + 1)) + C(fc(i, jp, k))) * C(0.5);
         csf2(i, j, k) = s2;
         conv2(i, j, k) = cv;
         // AB3 + Crank-Nicolson, main_ns_hybrid.F90:743-751
         const FieldView<T>& x2 = comp == 0 ? u2 : (comp == 1 ? v2 : w2);
         C r = C(x2(i, j, k)) -
               dt * ((C(23.0 / 12.0)) * cv - (C(4.0 / 3.0)) * C(conv1(i, j, k)) + (C(5.0 / 12.0)) * C(conv0(i, j, k))) +
-              (dt / C(2)) * df +
+              (dt * C(0.5)) * df +
               dt * ((C(23.0 / 12.0)) * s2 - (C(4.0 / 3.0)) * C(csf1(i, j, k)) + (C(5.0 / 12.0)) * C(csf0(i, j, k)));
         rhs(i, j, k) = r;
     }

@@ -104,10 +104,10 @@ __device__ __forceinline__ MuFaces<C> mu_faces_u(const FieldView<T>& mu, int i, 
     C a = mu(i, j1, k + 1), b = mu(i + 1, j1, k + 1);
     m.xp = b;
     m.xm = a;
-    m.yp = (a + b + C(mu(i, j2, k + 1)) + C(mu(i + 1, j2, k + 1))) / C(4);
-    m.ym = (a + b + C(mu(i, j0, k + 1)) + C(mu(i + 1, j0, k + 1))) / C(4);
-    m.zp = (a + b + C(mu(i, j1, k + 2)) + C(mu(i + 1, j1, k + 2))) / C(4);
-    m.zm = (a + b + C(mu(i, j1, k)) + C(mu(i + 1, j1, k))) / C(4);
+    m.yp = (a + b + C(mu(i, j2, k + 1)) + C(mu(i + 1, j2, k + 1))) * C(0.25);
+    m.ym = (a + b + C(mu(i, j0, k + 1)) + C(mu(i + 1, j0, k + 1))) * C(0.25);
+    m.zp = (a + b + C(mu(i, j1, k + 2)) + C(mu(i + 1, j1, k + 2))) * C(0.25);
+    m.zm = (a + b + C(mu(i, j1, k)) + C(mu(i + 1, j1, k))) * C(0.25);
     return m;
 }
 
@@ -117,12 +117,12 @@ __device__ __forceinline__ MuFaces<C> mu_faces_v(const FieldView<T>& mu, int i, 
     const int j1 = wrapj(j + 1, P), j0 = wrapj(j, P);
     MuFaces<C> m;
     C a = mu(i, j1, k + 1), b = mu(i, j0, k + 1);
-    m.xp = (a + b + C(mu(i + 1, j1, k + 1)) + C(mu(i + 1, j0, k + 1))) / C(4);
-    m.xm = (a + b + C(mu(i - 1, j1, k + 1)) + C(mu(i - 1, j0, k + 1))) / C(4);
+    m.xp = (a + b + C(mu(i + 1, j1, k + 1)) + C(mu(i + 1, j0, k + 1))) * C(0.25);
+    m.xm = (a + b + C(mu(i - 1, j1, k + 1)) + C(mu(i - 1, j0, k + 1))) * C(0.25);
     m.yp = a;
     m.ym = b;
-    m.zp = (a + b + C(mu(i, j1, k + 2)) + C(mu(i, j0, k + 2))) / C(4);
-    m.zm = (a + b + C(mu(i, j1, k)) + C(mu(i, j0, k))) / C(4);
+    m.zp = (a + b + C(mu(i, j1, k + 2)) + C(mu(i, j0, k + 2))) * C(0.25);
+    m.zm = (a + b + C(mu(i, j1, k)) + C(mu(i, j0, k))) * C(0.25);
     return m;
 }
 
@@ -132,10 +132,10 @@ __device__ __forceinline__ MuFaces<C> mu_faces_w(const FieldView<T>& mu, int i, 
     const int j1 = wrapj(j + 1, P), j2 = wrapj(j + 2, P), j0 = wrapj(j, P);
     MuFaces<C> m;
     C a = mu(i, j1, k), b = mu(i, j1, k + 1);
-    m.xp = (a + b + C(mu(i + 1, j1, k)) + C(mu(i + 1, j1, k + 1))) / C(4);
-    m.xm = (a + b + C(mu(i - 1, j1, k)) + C(mu(i - 1, j1, k + 1))) / C(4);
-    m.yp = (a + b + C(mu(i, j2, k)) + C(mu(i, j2, k + 1))) / C(4);
-    m.ym = (a + b + C(mu(i, j0, k)) + C(mu(i, j0, k + 1))) / C(4);
+    m.xp = (a + b + C(mu(i + 1, j1, k)) + C(mu(i + 1, j1, k + 1))) * C(0.25);
+    m.xm = (a + b + C(mu(i - 1, j1, k)) + C(mu(i - 1, j1, k + 1))) * C(0.25);
+    m.yp = (a + b + C(mu(i, j2, k)) + C(mu(i, j2, k + 1))) * C(0.25);
+    m.ym = (a + b + C(mu(i, j0, k)) + C(mu(i, j0, k + 1))) * C(0.25);
     m.zp = b;
     m.zm = a;
     return m;
