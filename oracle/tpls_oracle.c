@@ -1120,10 +1120,19 @@ static void sums2(const double *a, const double *b, long plane, int nz, double *
  *   restriction: R = P^T / 8 (full weighting), the transpose of
  *   prolongation: trilinear cell-centred interpolation, weights 3/4 and 1/4
  *              per direction, reflected at Neumann boundaries, wrapped in y;
- *   coarsest: mean-free right-hand side, 20 + nmax^2/2 symmetric
- *              Gauss-Seidel sweeps (red, black, black, red) from zero, nmax
- *              being its largest extent; mean-free result.
+ *   coarsest: the exact solve on the mean-free subspace by fast
+ *              diagonalisation when every extent is <= MG_SPEC_MAX: the
+ *              operator is shift + cx Lx + cy Ly + cz Lz (shift = diag -
+ *              2 (cx + cy + cz)), Lx and Lz the cell-centred Neumann
+ *              Laplacians (cosine eigenvectors), Ly the periodic one
+ *              (constant, cosine / sine pairs, alternating); z = Q
+ *              diag(1/lambda) Q^T b with the constant mode dropped, each
+ *              1-D transform summed over its line in index order.  Larger
+ *              coarsest levels: mean-free right-hand side, 20 + nmax^2/2
+ *              symmetric Gauss-Seidel sweeps (red, black, black, red) from
+ *              zero, nmax being the largest extent; mean-free result.
  * The same definition is implemented on the GPU (tpls_amd/csrc/multigrid.hip). */
+#define MG_SPEC_MAX 128
 #define MG_MAXLEV 16
 typedef struct {
     int nx, ny, nz;
@@ -1258,8 +1267,85 @@ static void mg_restrict(const MgLev *f, const MgLev *c, const double *tf, double
             }
 }
 
+/* 1-D eigenbases (row i, column = mode) and unit eigenvalues */
+static void mg_neumann_basis(int n, double *Q, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; ++k) {
+        lam[k] = 2.0 - 2.0 * cos(pi * (double)k / (double)n);
+        const double w = k == 0 ? sqrt(1.0 / (double)n) : sqrt(2.0 / (double)n);
+        for (int i = 0; i < n; ++i) Q[i * n + k] = w * cos(pi * (double)k * (double)(2 * i + 1) / (2.0 * (double)n));
+    }
+}
+
+static void mg_periodic_basis(int n, double *Q, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    const double w0 = sqrt(1.0 / (double)n), w = sqrt(2.0 / (double)n);
+    for (int j = 0; j < n; ++j) Q[j * n] = w0;
+    lam[0] = 0.0;
+    int col = 1;
+    for (int k = 1; 2 * k < n; ++k, col += 2) {
+        lam[col] = lam[col + 1] = 2.0 - 2.0 * cos(2.0 * pi * (double)k / (double)n);
+        for (int j = 0; j < n; ++j) {
+            const double a = 2.0 * pi * (double)k * (double)j / (double)n;
+            Q[j * n + col] = w * cos(a);
+            Q[j * n + col + 1] = w * sin(a);
+        }
+    }
+    if (n % 2 == 0) {
+        for (int j = 0; j < n; ++j) Q[j * n + col] = (j % 2) ? -w0 : w0;
+        lam[col] = 4.0;
+    }
+}
+
+/* one direction of the transform: out = Q^T in (fwd) or Q in along d */
+static void mg_spec_pass(const MgLev *o, const double *Q, int d, int fwd, const double *in, double *out)
+{
+    const int nx = o->nx, ny = o->ny, nz = o->nz, n = nx * ny * nz;
+    const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+    const int stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+    for (int q = 0; q < n; ++q) {
+        const int c = (q / stride) % nd, base = q - c * stride;
+        double acc = 0.0;
+        for (int m = 0; m < nd; ++m) acc += (fwd ? Q[m * nd + c] : Q[c * nd + m]) * in[base + m * stride];
+        out[q] = acc;
+    }
+}
+
+static void mg_coarse_spec(const MgLev *o, double *x, const double *b)
+{
+    const int nx = o->nx, ny = o->ny, nz = o->nz, n = nx * ny * nz;
+    double *Qx = malloc(sizeof(double) * (nx * nx + ny * ny + nz * nz + nx + ny + nz));
+    double *Qy = Qx + nx * nx, *Qz = Qy + ny * ny, *lx = Qz + nz * nz, *ly = lx + nx, *lz = ly + ny;
+    mg_neumann_basis(nx, Qx, lx);
+    mg_periodic_basis(ny, Qy, ly);
+    mg_neumann_basis(nz, Qz, lz);
+    double *t0 = malloc(sizeof(double) * n), *t1 = malloc(sizeof(double) * n);
+    memcpy(t0, b, sizeof(double) * n);
+    mg_spec_pass(o, Qx, 0, 1, t0, t1);
+    mg_spec_pass(o, Qy, 1, 1, t1, t0);
+    mg_spec_pass(o, Qz, 2, 1, t0, t1);
+    const double shift = o->diag - 2.0 * (o->cx + o->cy + o->cz);
+    for (int q = 0; q < n; ++q) {
+        const int r = q / nx;
+        const double ev = shift + o->cx * lx[q % nx] + o->cy * ly[r % ny] + o->cz * lz[r / ny];
+        t1[q] = q == 0 ? 0.0 : t1[q] / ev;
+    }
+    mg_spec_pass(o, Qz, 2, 0, t1, t0);
+    mg_spec_pass(o, Qy, 1, 0, t0, t1);
+    mg_spec_pass(o, Qx, 0, 0, t1, x);
+    free(t0);
+    free(t1);
+    free(Qx);
+}
+
 static void mg_coarse(const MgLev *o, double *x, double *b)
 {
+    if (o->nx <= MG_SPEC_MAX && o->ny <= MG_SPEC_MAX && o->nz <= MG_SPEC_MAX) {
+        mg_coarse_spec(o, x, b);
+        return;
+    }
     long n = (long)o->nx * o->ny * o->nz;
     double sb = 0.0;
     for (long q = 0; q < n; ++q) sb += b[q];

@@ -127,6 +127,7 @@ struct tpls_case {
     // levels 1.. in their own arrays
     DevArray* cg_z = nullptr;
     MgHierarchy mg{};
+    double* mg_spec = nullptr;  // the coarsest level's eigenbasis tables (mg.spec)
     MgLevel mg_coef[MG_MAXLEV];
     Tiling tiling;              // launch shapes of the Helmholtz sweeps
     bool use_mg = false;
@@ -402,6 +403,27 @@ struct CkptArray {
 
 }  // namespace
 
+// The eigenbasis tables of a hierarchy's coarsest level in device memory
+// (mg_spectral_tables), or *out = nullptr when the level is too large for
+// them.  The caller frees *out.
+static int upload_spectral(const MgLevel& top, hipStream_t s, double** out)
+{
+    *out = nullptr;
+    const int len = mg_spectral_tables(top.nx, top.ny, top.nz, nullptr);
+    if (!len) return TPLS_OK;
+    std::vector<double> tab(static_cast<size_t>(len));
+    mg_spectral_tables(top.nx, top.ny, top.nz, tab.data());
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * size_t(len)) != hipSuccess) return fail(TPLS_ERR_NOMEM, "hipMalloc");
+    if (hipMemcpyAsync(d, tab.data(), sizeof(double) * size_t(len), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        hipFree(d);
+        return fail(TPLS_ERR_HIP, "copy of the coarsest level's tables");
+    }
+    *out = d;
+    return TPLS_OK;
+}
+
 int tpls::set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 int tpls::case_dims(const tpls_case* c, int* maxl, int* maxm, int* maxn)
@@ -519,6 +541,8 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         // on the FP64 residual itself)
         c->mg.fp32 = c->mg.nlev > 1 && (prm->mg_fp32 > 0 || (prm->mg_fp32 == 0 && prm->precision == TPLS_FP32));
         c->mg.L[0].b = c->cg_r->view<double>();
+        if ((rc = upload_spectral(c->mg.L[c->mg.nlev - 1], c->s, &c->mg_spec))) return bail(rc);
+        c->mg.spec = c->mg_spec;
         if (c->mg.fp32) {
             for (int l = 0; l < c->mg.nlev; ++l) {
                 MgLevel32& o = c->mg.F[l];
@@ -571,6 +595,7 @@ int tpls_case_destroy(tpls_case* c)
     hipFree(c->red_part);
     hipFree(c->red_out);
     hipFree(c->cg);
+    hipFree(c->mg_spec);
     if (c->h_cg) hipHostFree(c->h_cg);
     if (c->h_out) hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1828,18 +1853,30 @@ int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
             RC(c->prec == TPLS_FP32 ? step_begin<float>(c) : step_begin<double>(c));
             open[q] = 1;
         }
-        for (int left = ncases; left > 0;) {
+        // serve whichever case's CG chunk has landed, so that no finished case
+        // waits behind a slower one for its next chunk; when none has, block
+        // on the oldest open one
+        for (int left = ncases, next = 0; left > 0;) {
+            bool served = false;
             for (int q = 0; q < ncases; ++q) {
                 if (!open[q]) continue;
                 tpls_case* c = cases[q];
-                HIPCHK(hipEventSynchronize(c->ps_ev));
+                const hipError_t e = hipEventQuery(c->ps_ev);
+                if (e == hipErrorNotReady) continue;
+                HIPCHK(e);
                 bool fin = false;
                 RC(ps_poll(c, &fin));
+                served = true;
                 if (fin) {
                     RC(c->prec == TPLS_FP32 ? step_end<float>(c) : step_end<double>(c));
                     open[q] = 0;
                     --left;
                 }
+            }
+            if (!served && left > 0) {
+                while (!open[next]) next = (next + 1) % ncases;
+                HIPCHK(hipEventSynchronize(cases[next]->ps_ev));
+                next = (next + 1) % ncases;
             }
         }
     }
@@ -1863,6 +1900,10 @@ int tpls_test_mg_apply(int nx, int ny, int nz, double cx, double cy, double cz, 
     if (long(top.nx) * top.ny * top.nz > MG_COARSE_MAX) return fail(TPLS_ERR_ARG, "coarsest level too large");
     hipStream_t s = g_drop.s;
     std::vector<void*> mem;
+    double* spec = nullptr;
+    RC(upload_spectral(top, s, &spec));
+    if (spec) mem.push_back(spec);
+    h.spec = spec;
     auto release = [&]() {
         hipStreamSynchronize(s);
         for (void* p : mem) hipFree(p);

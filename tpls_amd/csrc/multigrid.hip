@@ -16,9 +16,16 @@
 //   transfer    trilinear cell-centred prolongation (3/4, 1/4 per direction,
 //               reflected at the Neumann ends, wrapped in y) and its
 //               transpose / 8 as restriction;
-//   coarsest    mean-free right-hand side, 20 + nmax^2/2 symmetric
-//               Gauss-Seidel sweeps (red, black, black, red) from zero in
-//               one block's LDS, mean-free result.
+//   coarsest    the exact solve on the mean-free subspace (the pseudo-
+//               inverse of the singular operator) by fast diagonalisation:
+//               the operator is cx Lx + cy Ly + cz Lz with 1-D Laplacians
+//               Neumann in x and z (cosine modes) and periodic in y
+//               (Fourier modes), so z = Q diag(1/lambda) Q^T b with the
+//               constant mode dropped, three transforms each way in one
+//               block's LDS; a level with an extent above MG_SPEC_MAX
+//               instead takes 20 + nmax^2/2 symmetric Gauss-Seidel sweeps
+//               (red, black, black, red) from zero on the mean-free
+//               right-hand side, mean-free result.
 // Within a colour every update reads only the other colour, so the sweeps are
 // order-independent and the kernels reproduce the oracle's values; FMA
 // contraction is off in this file for the same reason.
@@ -38,12 +45,14 @@
 //   k_mg_rr      the residual b - A x and its restriction to the next level,
 //                fused (the residual never leaves LDS);
 //   k_mg_prolong x += P e;
-//   k_mg_coarse  the coarsest solve, one block;
+//   k_mg_spec    the coarsest solve by fast diagonalisation, one block;
+//   k_mg_coarse  the coarsest solve by sweeps (large coarsest levels);
 //   k_mg_last    the final red sweep on the finest level, fused with the
 //                partials of z.r and sum z (grid-stride tiles, at most
 //                CG_BLOCKS partials);
 //   k_mg_zf      one block: mz = mean z, beta = z.r - mz sum r and the
 //                PETSc breakdown checks of a preconditioned CG step.
+#include <cmath>
 #include <type_traits>
 
 #include "reduce.hpp"
@@ -366,6 +375,60 @@ __global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevelT<X, X> o, const CgS
         if (col[c] != 2) o.x(ci[c], cj[c], ck[c]) = sx[tid + c * nt] - mx;
 }
 
+// Coarsest level by fast diagonalisation, in FP64 whatever the level's
+// storage: b into LDS, b^ = Qz^T Qy^T Qx^T b (one pass per direction, each
+// output summed over its line in index order), b^ / lambda with the constant
+// mode dropped, then z = Qx Qy Qz b^.  tab: mg_spectral_tables.
+template <typename X, typename B>
+__global__ void __launch_bounds__(MGC_T) k_mg_spec(MgLevelT<X, B> o, const double* tab, const CgState* st)
+{
+    if (st->done) return;
+    __shared__ double s0[MG_COARSE_MAX], s1[MG_COARSE_MAX];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nx = o.nx, ny = o.ny, nz = o.nz, n = nx * ny * nz;
+    const double* Qx = tab;
+    const double* Qy = Qx + nx * nx;
+    const double* Qz = Qy + ny * ny;
+    const double* lx = Qz + nz * nz;
+    const double* ly = lx + nx;
+    const double* lz = ly + ny;
+    for (int q = tid; q < n; q += nt) {
+        const int r = q / nx;
+        s0[q] = double(o.b(q % nx + 1, r % ny + 1, r / ny + 1));
+    }
+    __syncthreads();
+    // one direction: out = Q^T in (fwd) or Q in along direction d
+    auto pass = [&](const double* in, double* out, int d, bool fwd) {
+        const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+        const int stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+        const double* Q = d == 0 ? Qx : (d == 1 ? Qy : Qz);
+        for (int q = tid; q < n; q += nt) {
+            const int c = (q / stride) % nd, base = q - c * stride;
+            double acc = 0.0;
+            for (int m = 0; m < nd; ++m) acc += (fwd ? Q[m * nd + c] : Q[c * nd + m]) * in[base + m * stride];
+            out[q] = acc;
+        }
+        __syncthreads();
+    };
+    pass(s0, s1, 0, true);
+    pass(s1, s0, 1, true);
+    pass(s0, s1, 2, true);
+    const double shift = o.diag - 2.0 * (o.cx + o.cy + o.cz);
+    for (int q = tid; q < n; q += nt) {
+        const int r = q / nx;
+        const double ev = shift + o.cx * lx[q % nx] + o.cy * ly[r % ny] + o.cz * lz[r / ny];
+        s1[q] = q == 0 ? 0.0 : s1[q] / ev;
+    }
+    __syncthreads();
+    pass(s1, s0, 2, false);
+    pass(s0, s1, 1, false);
+    pass(s1, s0, 0, false);
+    for (int q = tid; q < n; q += nt) {
+        const int r = q / nx;
+        o.x(q % nx + 1, r % ny + 1, r / ny + 1) = X(s0[q]);
+    }
+}
+
 // Final red sweep on the finest level with the partials of z.r (r = b) and
 // sum z over all cells, in FP64 from the stored z.  With update = 0 the sweep
 // is skipped (a one-level hierarchy, where the coarsest solve already
@@ -455,7 +518,7 @@ static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny 
 // FP64 CG residual) and whose levels 1.. are `lev[1..]` (all X).
 template <typename X>
 static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>& top, const MgLevelT<X, X>* lev,
-                         int nlev, int nu, Reduce red, CgState* st)
+                         int nlev, int nu, const double* spec, Reduce red, CgState* st)
 {
     const dim3 blk(BX, BY);
     const int last = nlev - 1;
@@ -487,13 +550,19 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     if (last > 0) {
         const MgLevelT<X, X>& c = lev[last];
         const long nc = long(c.nx) * c.ny * c.nz;
-        hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, c, st);
+        if (spec)
+            hipLaunchKernelGGL((k_mg_spec<X, X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, c, spec, st);
+        else
+            hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, c, st);
     } else {
         // one level: the coarsest solve runs on level 0 itself (its b is the
         // FP64 residual only on the FP64 hierarchy, where the types agree)
         if constexpr (std::is_same_v<X, double>) {
             const long nc = long(top.nx) * top.ny * top.nz;
-            hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, top, st);
+            if (spec)
+                hipLaunchKernelGGL((k_mg_spec<X, double>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, top, spec, st);
+            else
+                hipLaunchKernelGGL((k_mg_coarse<X>), dim3(1), dim3(nc <= 64 ? 64 : MGC_T), 0, s, top, st);
         } else {
             return hipErrorInvalidValue;
         }
@@ -509,7 +578,7 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
 
 hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
 {
-    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, red, st);
+    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, h.spec, red, st);
     // the FP32 vectors with L's extents and coefficients
     MgLevel32 lev[MG_MAXLEV];
     for (int l = 0; l < h.nlev; ++l) {
@@ -518,7 +587,59 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     }
     const MgLevel& o = h.L[0];
     const MgLevelT<float, double> top{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
-    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, red, st);
+    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st);
+}
+
+// 1-D eigenbases.  Neumann (cell-centred, ends coupled to themselves): mode k
+// cos(pi k (2i+1) / 2n), normalised, eigenvalue 2 - 2 cos(pi k / n).
+// Periodic: the constant, then cos and sin of 2 pi k j / n for k < n/2, then
+// (-1)^j for even n; eigenvalues 0, 2 - 2 cos(2 pi k / n) (twice), 4.
+static void mg_neumann_basis(int n, double* Q, double* lam)
+{
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; ++k) {
+        lam[k] = 2.0 - 2.0 * std::cos(pi * double(k) / double(n));
+        const double w = k == 0 ? std::sqrt(1.0 / double(n)) : std::sqrt(2.0 / double(n));
+        for (int i = 0; i < n; ++i) Q[i * n + k] = w * std::cos(pi * double(k) * double(2 * i + 1) / (2.0 * double(n)));
+    }
+}
+
+static void mg_periodic_basis(int n, double* Q, double* lam)
+{
+    const double pi = 3.14159265358979323846;
+    const double w0 = std::sqrt(1.0 / double(n)), w = std::sqrt(2.0 / double(n));
+    for (int j = 0; j < n; ++j) Q[j * n] = w0;
+    lam[0] = 0.0;
+    int col = 1;
+    for (int k = 1; 2 * k < n; ++k, col += 2) {
+        lam[col] = lam[col + 1] = 2.0 - 2.0 * std::cos(2.0 * pi * double(k) / double(n));
+        for (int j = 0; j < n; ++j) {
+            const double a = 2.0 * pi * double(k) * double(j) / double(n);
+            Q[j * n + col] = w * std::cos(a);
+            Q[j * n + col + 1] = w * std::sin(a);
+        }
+    }
+    if (n % 2 == 0) {
+        for (int j = 0; j < n; ++j) Q[j * n + col] = (j % 2) ? -w0 : w0;
+        lam[col] = 4.0;
+    }
+}
+
+int mg_spectral_tables(int nx, int ny, int nz, double* tab)
+{
+    if (nx > MG_SPEC_MAX || ny > MG_SPEC_MAX || nz > MG_SPEC_MAX) return 0;
+    const int len = nx * nx + ny * ny + nz * nz + nx + ny + nz;
+    if (!tab) return len;
+    double* Qx = tab;
+    double* Qy = Qx + nx * nx;
+    double* Qz = Qy + ny * ny;
+    double* lx = Qz + nz * nz;
+    double* ly = lx + nx;
+    double* lz = ly + ny;
+    mg_neumann_basis(nx, Qx, lx);
+    mg_periodic_basis(ny, Qy, ly);
+    mg_neumann_basis(nz, Qz, lz);
+    return len;
 }
 
 }  // namespace tpls

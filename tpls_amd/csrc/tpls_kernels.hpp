@@ -171,7 +171,16 @@ struct MgHierarchy {
     MgLevel L[MG_MAXLEV];
     int fp32;
     MgLevel32 F[MG_MAXLEV];
+    // eigenbasis tables of the coarsest level (mg_spectral_tables), or null:
+    // the coarsest solve then sweeps
+    const double* spec;
 };
+// The coarsest level's separable operator cx Lx + cy Ly + cz Lz (+ the
+// shift diag - 2 (cx + cy + cz)) in its eigenbasis: Qx (nx x nx, row i,
+// column = mode), Qy, Qz, then the unit eigenvalues lx, ly, lz.  Returns the
+// table length, or 0 when an extent exceeds MG_SPEC_MAX.
+constexpr int MG_SPEC_MAX = 128;
+int mg_spectral_tables(int nx, int ny, int nz, double* tab);
 // Extents and coefficients of every level (vectors left unset); returns the
 // number of levels.
 int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag, MgLevel* L);
