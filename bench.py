@@ -30,8 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # The batched and serving runs keep one stream per case; more hardware queues
 # let more of them run side by side (HIP's default is 4).  Set before the HIP
-# runtime starts.
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# runtime starts; a value already in the environment is kept.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK_GBS = 8000.0
 
