@@ -89,6 +89,14 @@ typedef struct {
                              products, residual and stopping test stay FP64
                              either way; an FP32 cycle changes the iterates
                              within the solve's tolerance, not the tolerance */
+    int idm_dim;          /* interface method (initial_config.opt "idm",
+                             configure.F90:320-350): 0 the level-set method
+                             (do_tpls), 1 the diffuse interface method
+                             (dim, cahn_hilliard_solver.f90:41-384, called as
+                             main_ns_hybrid.F90:606-611) */
+    double epn, Pe;       /* DIM interface width and Peclet number
+                             (get_dim_configuration, configure.F90:404-431) */
+    int dim_its;          /* max_iteration_dim (configure.F90:437-465), >= 1 */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -111,6 +119,7 @@ enum {
     TPLS_F_VISC2, TPLS_F_VISC2_AUG, TPLS_F_RHS_P,
     TPLS_F_DIFF_U, TPLS_F_DIFF_V, TPLS_F_DIFF_W, TPLS_F_RHS_U, TPLS_F_RHS_V, TPLS_F_RHS_W,
     TPLS_F_U_INLET,
+    TPLS_F_CONVPREV_PHI,  /* ConvPrevPhi, the DIM explicit-term history */
     TPLS_NFIELDS
 };
 
@@ -191,6 +200,17 @@ int tpls_do_tpls(double *phi2, double *phi3, double *conv0_phi, double *conv1_ph
                  double *conv2_phi, int sx, int ex, int sy, int ey, int maxn, int ex_max,
                  int max_iteration_levelset, double height, double *err2, double dx, double dz,
                  double dt);
+/* cahn_hilliard_solver.f90:41  dim(maxl,maxn,dx,dy,dz,dt,epn,Pe,phi,phiOld,conv2_phi,
+ * ConvPrevPhi,phiNew,sx,ex,sy,ey,sz_p,ez_p,height,...,max_iteration_dim) with the
+ * MPI arguments (sz_p = 0, ez_p = maxn, strides, neighbours, communicator)
+ * dropped.  phi and phiOld are the phi2+0.5 / phi_old+0.5 arrays the caller
+ * passes (main_ns_hybrid.F90:608).  phiNew: initial iterate in, last
+ * iterate out (boundary planes included); ConvPrevPhi updated in place.  res (may be NULL): the largest
+ * |change| of the last sweep, which the reference computes and discards. */
+int tpls_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, double epn, double Pe,
+             const double *phi, const double *phiOld, const double *conv2_phi, double *ConvPrevPhi,
+             double *phiNew, int sx, int ex, int sy, int ey, double height, int max_iteration_dim,
+             double *res);
 /* levelset.f90:17  get_levelset_status(phi,D_mx,istatus,sign_mx,sx,ex,sy,ey,n,h) */
 int tpls_get_levelset_status(const double *phi, double *D_mx, int *istatus, double *sign_mx, int sx,
                              int ex, int sy, int ey, int n, double h);

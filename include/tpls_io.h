@@ -142,8 +142,9 @@ typedef struct {
  * main_ns_hybrid.F90:242-311 does (configure.F90:39-764: the same required
  * keys in the same order, the same checks, codes and messages) and sets the
  * file-backed fields of *prm (dx, dy, dz, dt, Re, mu_plus, mu_minus, height,
- * smooth_width, scap, mom_u, mom_v, mom_w, levelset_its; the others are left
- * as the caller set them) and *run.  The process grid is checked for itself
+ * smooth_width, scap, mom_u, mom_v, mom_w, levelset_its, and idm_dim, epn,
+ * Pe, dim_its from idm, epn, pe, max_iteration_dim; the others are left as
+ * the caller set them) and *run.  The process grid is checked for itself
  * (num_procs_x, num_procs_y >= 1, dividing maxl-1 and maxm-1) but not against
  * a process count: one GPU holds the whole domain.  A failed check returns
  * TPLS_ERR_CONFIG with the reference's code in *ierr and its message in

@@ -52,6 +52,7 @@ SPECS = {
 ORACLE_ONLY = {
     "do_advect_phi": "aaaaaiiiiiiiddd",
     "do_tpls": "aaaaaiiiiiiidoddd",
+    "dim": "iiddddddaaaaaiiiidio",
 }
 
 
@@ -120,6 +121,9 @@ def oracle_lib() -> _Lib:
         L.orc_pressure_solve.restype = C.c_int
         L.orc_mg_apply.restype = C.c_int
         L.orc_mg_apply.argtypes = [C.c_int] * 3 + [C.c_double] * 4 + [C.c_int, _D, _D]
+        L.orc_dim_diagonal.restype = C.c_double
+        L.orc_dim_diagonal.argtypes = [C.c_double] * 6
+        L.orc_dim_inlet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _D]
     return _oracle
 
 
@@ -190,7 +194,13 @@ class OrcParams(C.Structure):
                  "smooth_width", "scap")] + \
                [(n, C.c_int) for n in ("mom_u", "mom_v", "mom_w", "levelset_its")] + \
                [(n, C.c_double) for n in ("ksp_rtol", "ksp_abstol", "ksp_dtol")] + \
-               [("ksp_maxits", C.c_int), ("ksp_pc", C.c_int)]
+               [("ksp_maxits", C.c_int), ("ksp_pc", C.c_int), ("idm_dim", C.c_int),
+                ("epn", C.c_double), ("Pe", C.c_double), ("dim_its", C.c_int)]
+
+
+# OrcParams fields a parameter dict may leave out (the level-set method's
+# defaults).
+OPTIONAL_PARAMS = {"ksp_pc": 0, "idm_dim": 0, "epn": 0.0, "Pe": 1.0, "dim_its": 0}
 
 
 class OrcDiag(C.Structure):
@@ -207,7 +217,7 @@ FIELD_NAMES = [
     "csf_u0", "csf_u1", "csf_u2", "csf_v0", "csf_v1", "csf_v2", "csf_w0", "csf_w1", "csf_w2",
     "conv0_phi", "conv1_phi", "conv2_phi", "fx_csf", "fy_csf", "fz_csf",
     "visc2", "visc2_aug", "rhs_p", "diff_u", "diff_v", "diff_w", "rhs_u", "rhs_v", "rhs_w",
-    "u_inlet",
+    "u_inlet", "convprev_phi",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELD_NAMES)}
 
@@ -221,9 +231,19 @@ def field_shape(name, maxl, maxm, maxn):
     if name in ("w2", "w3") or name.endswith("_w") or name in ("csf_w0", "csf_w1", "csf_w2"):
         return (maxl + 1, maxm + 1, maxn)
     if name in ("phi2", "phi3", "pres", "pres_old", "conv0_phi", "conv1_phi", "conv2_phi",
-                "fx_csf", "fy_csf", "fz_csf", "visc2", "rhs_p"):
+                "fx_csf", "fy_csf", "fz_csf", "visc2", "rhs_p", "convprev_phi"):
         return (maxl + 1, maxm + 1, maxn + 1)
     return (maxl + 1, maxm + 1, maxn - 1)
+
+
+def dim_diagonal(dx, dy, dz, dt, epn, Pe) -> float:
+    return oracle_lib().lib.orc_dim_diagonal(dx, dy, dz, dt, epn, Pe)
+
+
+def dim_inlet(maxn, dz, height, epn) -> np.ndarray:
+    prof = np.zeros(maxn + 1)
+    oracle_lib().lib.orc_dim_inlet(int(maxn), float(dz), float(height), float(epn), prof.ctypes.data_as(_D))
+    return prof
 
 
 class OracleState:
@@ -234,7 +254,7 @@ class OracleState:
         self.shape = (maxl, maxm, maxn)
         p = OrcParams()
         for f in fields_of(OrcParams):
-            setattr(p, f, params.get(f, 0) if f == "ksp_pc" else params[f])
+            setattr(p, f, params.get(f, OPTIONAL_PARAMS[f]) if f in OPTIONAL_PARAMS else params[f])
         self._p = p
         self.h = self.L.orc_state_create(maxl, maxm, maxn, C.byref(p))
 

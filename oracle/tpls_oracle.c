@@ -447,6 +447,182 @@ void orc_do_tpls(double *phi2_, double *phi3_, double *c0_, double *c1_, double 
     free(rhs_); free(D_); free(sg_); free(pr_); free(ist_);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Diffuse-interface method: cahn_hilliard_solver.f90                         */
+/* ------------------------------------------------------------------------ */
+/* cahn_hilliard_solver.f90:240-245 (nmda = tao = 0.25; dx**2.d0 and dx**4.d0
+ * are real-exponent powers, evaluated with pow) */
+double orc_dim_diagonal(double dx, double dy, double dz, double dt, double epn, double Pe)
+{
+    const double nmda = 0.25, tao = 0.25;
+    double epn2 = epn * epn;
+    double a1 = 1.0 / pow(dx, 2.0) + 1.0 / pow(dy, 2.0) + 1.0 / pow(dz, 2.0);
+    double a2 = 2.0 / pow(dx, 4.0) + 2.0 / pow(dy, 4.0) + 2.0 / pow(dz, 4.0);
+    return 1.5 + nmda / 2.0 / Pe * dt * (2.0 * tao * a1 + epn2 * (4.0 * a1 * a1 + a2));
+}
+
+/* cahn_hilliard_solver.f90:346-353 */
+void orc_dim_inlet(int maxn, double dz, double height, double epn, double *prof)
+{
+    prof[0] = prof[maxn] = 0.0;
+    for (int k = 1; k <= maxn - 1; ++k) {
+        double z_val = k * dz - 0.5 * dz;
+        prof[k] = 0.5 + 0.5 * tanh((z_val - height) / (2.0 * pow(2.0, 0.5) * epn));
+    }
+}
+
+/* mobility sqrt(c^2 (1-|c|)^2) of a face value c, cahn_hilliard_solver.f90:176-191 */
+static double dim_mob(double c1)
+{
+    return sqrt(c1 * c1 * (1.0 - fabs(c1)) * (1.0 - fabs(c1)));
+}
+
+/* 7-point Laplacian of a at an interior point, cahn_hilliard_solver.f90:108-110 */
+static double dim_lap(F3 a, int i, int j, int k, double dx, double dy, double dz)
+{
+    double c = A3(a, i, j, k);
+    return (A3(a, i + 1, j, k) - 2.0 * c + A3(a, i - 1, j, k)) / dx / dx +
+           (A3(a, i, j + 1, k) - 2.0 * c + A3(a, i, j - 1, k)) / dy / dy +
+           (A3(a, i, j, k + 1) - 2.0 * c + A3(a, i, j, k - 1)) / dz / dz;
+}
+
+/* variable-coefficient divergence term of temp1 / temp3,
+ * cahn_hilliard_solver.f90:193-209 */
+static double dim_div(F3 a, int i, int j, int k, const double m[6], double dx, double dy,
+                      double dz)
+{
+    double c = A3(a, i, j, k);
+    return (m[1] * (A3(a, i + 1, j, k) - c) - m[0] * (c - A3(a, i - 1, j, k))) / dx / dx +
+           (m[3] * (A3(a, i, j + 1, k) - c) - m[2] * (c - A3(a, i, j - 1, k))) / dy / dy +
+           (m[5] * (A3(a, i, j, k + 1) - c) - m[4] * (c - A3(a, i, j, k - 1))) / dz / dz;
+}
+
+/* cahn_hilliard_solver.f90:41-384 */
+void orc_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, double epn,
+             double Pe, const double *phi_, const double *phiOld_, const double *conv2_,
+             double *cprev_, double *phiNew_, int sx, int ex, int sy, int ey, double height,
+             int max_iteration_dim, double *res_out)
+{
+    const double nmda = 0.25, tao = 0.25, alfa1 = 1.1;
+    long tot = (long)(ex - sx + 3) * (ey - sy + 3) * (maxn + 1);
+    double *bE_ = calloc(tot, sizeof(double));
+    double *d2_ = calloc(tot, sizeof(double));
+    double *src_ = calloc(tot, sizeof(double));
+    double *prof = calloc(maxn + 1, sizeof(double));
+    F3 phi = f3(phi_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 phiOld = f3(phiOld_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 conv2 = f3(conv2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 cprev = f3(cprev_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 phiNew = f3(phiNew_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 bE = f3(bE_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 d2 = f3(d2_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    F3 src = f3(src_, sx - 1, ex + 1, sy - 1, ey + 1, 0);
+    const double epn2 = epn * epn;
+
+    /* :101-115 bulk chemical potential and Laplacian of the old phi */
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double p = A3(phi, i, j, k);
+                A3(bE, i, j, k) = (1.0 - 3.0 * p + 2.0 * p * p) * p / 2.0;
+                A3(d2, i, j, k) = dim_lap(phi, i, j, k, dx, dy, dz);
+            }
+    /* :119-129 z planes, :132-135 y halos, :138-163 inlet/outlet planes */
+    for (int j = sy; j <= ey; ++j)
+        for (int i = sx; i <= ex; ++i) {
+            A3(bE, i, j, 0) = A3(bE, i, j, 1);
+            A3(bE, i, j, maxn) = A3(bE, i, j, maxn - 1);
+            A3(d2, i, j, 0) = A3(d2, i, j, 1);
+            A3(d2, i, j, maxn) = A3(d2, i, j, maxn - 1);
+        }
+    exchange2d(bE_, sx, ex, sy, ey, 0, maxn);
+    exchange2d(d2_, sx, ex, sy, ey, 0, maxn);
+    if (sx == 1)
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j) {
+                A3(d2, 0, j, k) = A3(d2, 1, j, k);
+                A3(bE, 0, j, k) = A3(bE, 1, j, k);
+            }
+    if (ex == maxl - 1)
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j) {
+                A3(d2, maxl, j, k) = A3(d2, maxl - 1, j, k);
+                A3(bE, maxl, j, k) = A3(bE, maxl - 1, j, k);
+            }
+    /* :165-237 explicit part: source0 and the stored explicit term */
+    for (int k = 1; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                double p = A3(phi, i, j, k), m[6];
+                m[0] = dim_mob((A3(phi, i - 1, j, k) + p) / 2.0);
+                m[1] = dim_mob((A3(phi, i + 1, j, k) + p) / 2.0);
+                m[2] = dim_mob((A3(phi, i, j - 1, k) + p) / 2.0);
+                m[3] = dim_mob((A3(phi, i, j + 1, k) + p) / 2.0);
+                m[4] = dim_mob(k + 1 == 2 ? p : (A3(phi, i, j, k - 1) + p) / 2.0);
+                m[5] = dim_mob((A3(phi, i, j, k + 1) + p) / 2.0);
+                double temp1 = dim_div(bE, i, j, k, m, dx, dy, dz);
+                double temp2 = dim_lap(d2, i, j, k, dx, dy, dz);
+                double temp3 = dim_div(d2, i, j, k, m, dx, dy, dz);
+                double temp4 = (temp1 - epn2 * temp3) / epn - nmda * tao / 2.0 * A3(d2, i, j, k) +
+                               epn2 * nmda / 2.0 * temp2;
+                double temp5 = temp4 / Pe - A3(conv2, i, j, k);
+                A3(src, i, j, k) = temp5 * 2.0 - A3(cprev, i, j, k);
+                A3(cprev, i, j, k) = temp5;
+            }
+    const double diagonal = orc_dim_diagonal(dx, dy, dz, dt, epn, Pe);
+    orc_dim_inlet(maxn, dz, height, epn, prof);
+    double res = 0.0;
+    /* :248-370 relaxation sweeps */
+    for (int iteration = 1; iteration <= max_iteration_dim; ++iteration) {
+        res = 0.0;
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i) A3(d2, i, j, k) = dim_lap(phiNew, i, j, k, dx, dy, dz);
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                A3(d2, i, j, 0) = A3(d2, i, j, 1);
+                A3(d2, i, j, maxn) = A3(d2, i, j, maxn - 1);
+            }
+        exchange2d(d2_, sx, ex, sy, ey, 0, maxn);
+        if (sx == 1)
+            for (int k = 1; k <= maxn - 1; ++k)
+                for (int j = sy; j <= ey; ++j) A3(d2, 0, j, k) = A3(d2, 1, j, k);
+        if (ex == maxl - 1)
+            for (int k = 1; k <= maxn - 1; ++k)
+                for (int j = sy; j <= ey; ++j) A3(d2, maxl, j, k) = A3(d2, maxl - 1, j, k);
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i) A3(bE, i, j, k) = dim_lap(d2, i, j, k, dx, dy, dz);
+        for (int k = 1; k <= maxn - 1; ++k)
+            for (int j = sy; j <= ey; ++j)
+                for (int i = sx; i <= ex; ++i) {
+                    double some = 2.0 * A3(phi, i, j, k) - 0.5 * A3(phiOld, i, j, k) -
+                                  1.5 * A3(phiNew, i, j, k) + dt * A3(src, i, j, k) +
+                                  dt * nmda / 2.0 / Pe * (tao * A3(d2, i, j, k) - epn2 * A3(bE, i, j, k));
+                    double old = A3(phiNew, i, j, k);
+                    double nw = old + alfa1 * some / diagonal;
+                    if (nw < 0.0) nw = 0.0;
+                    if (nw > 1.0) nw = 1.0;
+                    A3(phiNew, i, j, k) = nw;
+                    res = dmax(res, fabs(nw - old));
+                }
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                A3(phiNew, i, j, 0) = A3(phiNew, i, j, 1);
+                A3(phiNew, i, j, maxn) = A3(phiNew, i, j, maxn - 1);
+            }
+        exchange2d(phiNew_, sx, ex, sy, ey, 0, maxn);
+        if (sx == 1)
+            for (int k = 1; k <= maxn - 1; ++k)
+                for (int j = sy; j <= ey; ++j) A3(phiNew, 0, j, k) = prof[k];
+        if (ex == maxl - 1)
+            for (int k = 1; k <= maxn - 1; ++k)
+                for (int j = sy; j <= ey; ++j) A3(phiNew, maxl, j, k) = A3(phiNew, maxl - 1, j, k);
+    }
+    if (res_out) *res_out = res;
+    free(bE_); free(d2_); free(src_); free(prof);
+}
+
 /* levelset.f90:241-414 */
 void orc_get_csf(double *fx_, double *fy_, double *fz_, const double *fi_, int sx, int ex,
                  int sy, int ey, int n, int ex_max, double dx, double dy, double dz,
@@ -1611,7 +1787,7 @@ static int field_grid(int id)
         return 1;
     case ORC_PHI2: case ORC_PHI3: case ORC_PRES: case ORC_PRES_OLD: case ORC_CONV0_PHI:
     case ORC_CONV1_PHI: case ORC_CONV2_PHI: case ORC_FX_CSF: case ORC_FY_CSF: case ORC_FZ_CSF:
-    case ORC_VISC2: case ORC_RHS_P:
+    case ORC_VISC2: case ORC_RHS_P: case ORC_CONVPREV_PHI:
         return 2;
     case ORC_VISC2_AUG: return 3;
     case ORC_U_INLET: return 4;
@@ -1726,11 +1902,28 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
     /* main:585 */
     orc_do_advect_phi(f[ORC_PHI2], f[ORC_U2], f[ORC_V2], f[ORC_W2], f[ORC_CONV2_PHI], sx, ex, sy,
                       ey, maxl, maxn, ex_max, dx, dy, dz);
-    /* main:596-599 */
-    double err2 = 0.0;
-    orc_do_tpls(f[ORC_PHI2], f[ORC_PHI3], f[ORC_CONV0_PHI], f[ORC_CONV1_PHI], f[ORC_CONV2_PHI], sx,
-                ex, sy, ey, maxn, ex_max, P->levelset_its, P->height, &err2, dx, dz, dt);
-    dg->err_phi = err2;
+    const long n_p = s->cnt[ORC_PHI2];
+    if (!P->idm_dim) {
+        /* main:596-599 */
+        double err2 = 0.0;
+        orc_do_tpls(f[ORC_PHI2], f[ORC_PHI3], f[ORC_CONV0_PHI], f[ORC_CONV1_PHI], f[ORC_CONV2_PHI],
+                    sx, ex, sy, ey, maxn, ex_max, P->levelset_its, P->height, &err2, dx, dz, dt);
+        dg->err_phi = err2;
+    } else {
+        /* main:606-611: dim(..., phi2+0.5, phi_old+0.5, ...) then phi3 = phi3-0.5.
+         * phi_old is set equal to phi2 at start-up and at the end of every
+         * step (main:444, 2507-2508), so phi_old+0.5 is the same temporary.
+         * phi3 enters as the last step's phi3 (zero before the first step,
+         * main:443). */
+        double *pp = malloc(sizeof(double) * n_p);
+        for (long q = 0; q < n_p; ++q) pp[q] = f[ORC_PHI2][q] + 0.5;
+        double res = 0.0;
+        orc_dim(maxl, maxn, dx, dy, dz, dt, P->epn, P->Pe, pp, pp, f[ORC_CONV2_PHI],
+                f[ORC_CONVPREV_PHI], f[ORC_PHI3], sx, ex, sy, ey, P->height, P->dim_its, &res);
+        for (long q = 0; q < n_p; ++q) f[ORC_PHI3][q] = f[ORC_PHI3][q] - 0.5;
+        free(pp);
+        dg->err_phi = res;
+    }
     /* main:618-627 */
     exchange2d(f[ORC_PHI3], sx, ex, sy, ey, 0, maxn);
     orc_get_csf(f[ORC_FX_CSF], f[ORC_FY_CSF], f[ORC_FZ_CSF], f[ORC_PHI3], sx, ex, sy, ey, maxn,
@@ -1756,18 +1949,34 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
                     A3(cw, i, j, k) = (A3(fz, i, j + 1, k + 1) + A3(fz, i, j + 1, k)) / 2.0;
     }
     /* main:646-710 */
-    orc_get_viscosity(f[ORC_VISC2], f[ORC_PHI2], ex, ey, sx, sy, maxn, P->smooth_width, P->Re,
-                      P->mu_minus, P->mu_plus);
+    /* DIM: get_viscosity(viscosity2, phi2-0.5, ...) and phi_val = phi2-0.5 at
+     * the inlet/outlet planes (main:650-651, 664-668, 689-693) */
+    const double vshift = P->idm_dim ? 0.5 : 0.0;
+    if (!P->idm_dim) {
+        orc_get_viscosity(f[ORC_VISC2], f[ORC_PHI2], ex, ey, sx, sy, maxn, P->smooth_width, P->Re,
+                          P->mu_minus, P->mu_plus);
+    } else {
+        double *pm = malloc(sizeof(double) * n_p);
+        for (long q = 0; q < n_p; ++q) pm[q] = f[ORC_PHI2][q] - 0.5;
+        orc_get_viscosity(f[ORC_VISC2], pm, ex, ey, sx, sy, maxn, P->smooth_width, P->Re,
+                          P->mu_minus, P->mu_plus);
+        free(pm);
+    }
     {
         F3 v2 = f3(f[ORC_VISC2], 0, maxl, 0, maxm, 0);
         F3 va = f3(f[ORC_VISC2_AUG], -1, maxl + 1, -1, maxm + 1, 0);
         for (int k = 0; k <= maxn; ++k)
             for (int j = sy; j <= ey; ++j) {
                 for (int i = sx; i <= ex; ++i) A3(va, i, j, k) = A3(v2, i, j, k);
-                double m0 = visc_of_phi(A3(phi2, 0, j, k), P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
+                double p0 = A3(phi2, 0, j, k), p1 = A3(phi2, ex_max + 1, j, k);
+                if (P->idm_dim) {
+                    p0 = p0 - vshift;
+                    p1 = p1 - vshift;
+                }
+                double m0 = visc_of_phi(p0, P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
                 A3(va, 0, j, k) = m0;
                 A3(v2, 0, j, k) = m0;
-                double m1 = visc_of_phi(A3(phi2, ex_max + 1, j, k), P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
+                double m1 = visc_of_phi(p1, P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
                 A3(va, ex_max + 1, j, k) = m1;
                 A3(v2, ex_max + 1, j, k) = m1;
             }

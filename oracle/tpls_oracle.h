@@ -57,6 +57,24 @@ void orc_do_tpls(double *phi2, double *phi3, double *conv0_phi, double *conv1_ph
                  double *conv2_phi, int sx, int ex, int sy, int ey, int maxn, int ex_max,
                  int max_iteration_levelset, double height, double *err2, double dx,
                  double dz, double dt);
+/* cahn_hilliard_solver.f90:41-384, dim (MPI arguments dropped; single rank).
+ * phi and phiOld are the phi2+0.5 and phi_old+0.5 temporaries that
+ * main_ns_hybrid.F90:608 passes; phiNew is the initial iterate on entry and
+ * the last one, with its boundary planes, on return.  res: the largest
+ * |change| of the last sweep (the reference computes it and never uses it).
+ * The reference module uses the MPI module for its halo exchanges, so it
+ * cannot be compiled here; this restatement is pinned by closed forms, by
+ * the fixed point of its sweep against a direct solve of the linear system
+ * the sweep relaxes, and by that system's diagonal (tests/test_oracle_dim.py). */
+void orc_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, double epn,
+             double Pe, const double *phi, const double *phiOld, const double *conv2_phi,
+             double *ConvPrevPhi, double *phiNew, int sx, int ex, int sy, int ey, double height,
+             int max_iteration_dim, double *res);
+/* The scalar factors of dim: diagonal (cahn_hilliard_solver.f90:243-245) and
+ * the inlet profile phiNew(0,.,k), k = 0..maxn (k = 0 and maxn unused)
+ * (cahn_hilliard_solver.f90:350-352). */
+double orc_dim_diagonal(double dx, double dy, double dz, double dt, double epn, double Pe);
+void orc_dim_inlet(int maxn, double dz, double height, double epn, double *prof);
 /* levelset.f90:17-94 */
 void orc_get_levelset_status(const double *phi, double *D_mx, int *istatus, double *sign_mx,
                              int sx, int ex, int sy, int ey, int n, double h);
@@ -139,6 +157,9 @@ typedef struct {
     double ksp_rtol, ksp_abstol, ksp_dtol;
     int ksp_maxits;
     int ksp_pc;  /* 0 none, n >= 1 multigrid V(n,n) */
+    int idm_dim; /* 0: level-set method (do_tpls), 1: diffuse interface (dim) */
+    double epn, Pe;
+    int dim_its; /* max_iteration_dim */
 } orc_params;
 
 typedef struct {
@@ -161,7 +182,7 @@ enum {
     ORC_FX_CSF, ORC_FY_CSF, ORC_FZ_CSF,
     ORC_VISC2, ORC_VISC2_AUG, ORC_RHS_P,
     ORC_DIFF_U, ORC_DIFF_V, ORC_DIFF_W, ORC_RHS_U, ORC_RHS_V, ORC_RHS_W,
-    ORC_U_INLET,
+    ORC_U_INLET, ORC_CONVPREV_PHI,
     ORC_NFIELDS
 };
 

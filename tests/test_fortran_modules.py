@@ -32,7 +32,8 @@ REF_SRC = "/root/reference/src"
 MODULES = {"weno": "weno.f90", "advect_phi": "advect_phi.f90", "levelset": "levelset.f90",
            "two_phase_levelset": "two_phase_levelset.f90", "momentum_allflux": "momentum_allflux.f90",
            "jacobi_iteration_allflux": "jacobi_iteration_allflux.f90",
-           "sor_iteration_allflux": "sor_iteration_allflux.f90", "pressure": "pressure.f90"}
+           "sor_iteration_allflux": "sor_iteration_allflux.f90", "pressure": "pressure.f90",
+           "cahn_hilliard_solver": "cahn_hilliard_solver.f90"}
 RID = {"weno_appox_convection": 1, "get_levelset_status": 2, "do_levelset_iteration": 3, "get_csf": 4,
        "get_viscosity": 5, "get_conv_all": 6, "get_diffusion_all": 7, "do_jacobi_u": 8, "do_jacobi_v": 9,
        "do_jacobi_w": 10, "do_sor_u": 11, "do_sor_v": 12, "do_sor_w": 13, "get_source_pres": 14,
@@ -67,7 +68,7 @@ def test_argument_lists_match_reference():
             assert name in ours, f"{fn}: {name} not wrapped"
             assert ours[name] == args, (name, ours[name], args)
             checked += 1
-    assert checked == 18
+    assert checked == 19
 
 
 def _write_stream(path, rid, maxl, maxm, maxn, args):

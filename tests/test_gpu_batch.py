@@ -19,7 +19,10 @@ def _mix():
           CaseParams(nx=64, ny=16, nz=32, Re=300.0, mu_plus=5.0, seed=2, ksp_pc="mg"),
           CaseParams(nx=48, ny=8, nz=16, Re=80.0, mu_plus=0.5, noise=0.05, seed=3)]
     cs += batch_configs(3, seed=7, grids=((64, 16, 32), (128, 32, 64)))
-    precs = ["fp64", "fp32", "fp64", "fp32", "fp64", "fp32"]
+    # diffuse-interface cases beside the level-set ones
+    cs += [CaseParams(nx=32, ny=8, nz=16, Re=90.0, seed=4, idm="dim"),
+           CaseParams(nx=48, ny=16, nz=24, Re=120.0, mu_plus=3.0, seed=5, idm="dim", ksp_pc="mg")]
+    precs = ["fp64", "fp32", "fp64", "fp32", "fp64", "fp32", "fp64", "fp32"]
     return cs, precs
 
 

@@ -30,7 +30,8 @@ class Params(C.Structure):
                 ("ksp_pc", C.c_int), ("graphs", C.c_int), ("tile_rows", C.c_int),
                 ("tile_rows_odd", C.c_int), ("prefetch_odd", C.c_int), ("sor_fused", C.c_int),
                 ("tile_rows_rb", C.c_int), ("tile_rows_rb_odd", C.c_int), ("tile_rows_jst", C.c_int),
-                ("mg_fp32", C.c_int)]
+                ("mg_fp32", C.c_int), ("idm_dim", C.c_int), ("epn", C.c_double), ("Pe", C.c_double),
+                ("dim_its", C.c_int)]
 
 
 class Diag(C.Structure):
@@ -48,7 +49,7 @@ FIELD_NAMES = [
     "csf_u0", "csf_u1", "csf_u2", "csf_v0", "csf_v1", "csf_v2", "csf_w0", "csf_w1", "csf_w2",
     "conv0_phi", "conv1_phi", "conv2_phi", "fx_csf", "fy_csf", "fz_csf",
     "visc2", "visc2_aug", "rhs_p", "diff_u", "diff_v", "diff_w", "rhs_u", "rhs_v", "rhs_w",
-    "u_inlet",
+    "u_inlet", "convprev_phi",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELD_NAMES)}
 
@@ -62,6 +63,7 @@ ROUTINES = {
     "wenoappoxconvection": "aaaaiiiiiiaaa",
     "do_advect_phi": "aaaaaiiiiiiiddd",
     "do_tpls": "aaaaaiiiiiiidoddd",
+    "dim": "iiddddddaaaaaiiiidio",
     "get_levelset_status": "aaIaiiiiid",
     "do_levelset_iteration": "aaaIaiiiiiiddo",
     "get_csf": "aaaaiiiiiiddddd",

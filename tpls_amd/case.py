@@ -19,7 +19,7 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
     if name in ("w2", "w3") or name.endswith("_w") or name.startswith("csf_w"):
         return (maxl + 1, maxm + 1, maxn)
     if name in ("phi2", "phi3", "pres", "pres_old", "conv0_phi", "conv1_phi", "conv2_phi",
-                "fx_csf", "fy_csf", "fz_csf", "visc2", "rhs_p"):
+                "fx_csf", "fy_csf", "fz_csf", "visc2", "rhs_p", "convprev_phi"):
         return (maxl + 1, maxm + 1, maxn + 1)
     return (maxl + 1, maxm + 1, maxn - 1)
 
@@ -57,7 +57,7 @@ def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True
 
 # The tpls_params fields a TPLS configuration sets (include/tpls_io.h).
 FILE_FIELDS = ("dx", "dy", "dz", "dt", "Re", "mu_plus", "mu_minus", "height", "smooth_width", "scap",
-               "mom_u", "mom_v", "mom_w", "levelset_its")
+               "mom_u", "mom_v", "mom_w", "levelset_its", "idm_dim", "epn", "Pe", "dim_its")
 
 
 class Case:
@@ -104,12 +104,11 @@ class Case:
         ``case.run``."""
         from . import io
         prm, run = io.read_configuration(config_dir, options_file)
-        if run.idm_dim:
-            raise _lib.TplsError("idm dim (the diffuse interface method) is not built; only idm lsm runs")
+        dim = dict(idm="dim", epn=run.epn, Pe=run.Pe, dim_its=run.max_iteration_dim) if run.idm_dim else {}
         p = CaseParams(nx=run.maxl - 1, ny=run.maxm - 1, nz=run.maxn - 1, Re=prm.Re, mu_minus=prm.mu_minus,
                        mu_plus=prm.mu_plus, height=prm.height, scap=prm.scap, dt=prm.dt,
                        smooth_width_scale=prm.smooth_width * (run.maxn - 1), mom_u=prm.mom_u,
-                       mom_v=prm.mom_v, mom_w=prm.mom_w, levelset_its=prm.levelset_its, **solver)
+                       mom_v=prm.mom_v, mom_w=prm.mom_w, levelset_its=prm.levelset_its, **dim, **solver)
         case = cls(p, precision, diagnostics, graphs, tiles, sor_fused, file_params=prm)
         case.run = run
         case.load_config(config_dir)

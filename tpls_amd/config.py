@@ -37,6 +37,12 @@ class CaseParams:
     ksp_mg_nu: int = 2             # pre- and post-smoothing sweeps of the V-cycle
     noise: float = 1e-3            # amplitude of the seeded uniform perturbation
     seed: int = 0
+    # interface method (initial_config.opt "idm"): "lsm" level set or "dim"
+    # diffuse interface (Cahn-Hilliard, cahn_hilliard_solver.f90)
+    idm: str = "lsm"
+    epn: float | None = None       # DIM interface width; None: 0.5 dz (twophase_initialisation_wave.F90:71)
+    Pe: float | None = None        # DIM Peclet number; None: 1/epn^2 (twophase_initialisation_wave.F90:72)
+    dim_its: int = 18              # max_iteration_dim (tpls_config.opt:55)
 
     @property
     def maxl(self) -> int:
@@ -59,6 +65,15 @@ class CaseParams:
         return self.smooth_width_scale * self.dx
 
     @property
+    def dim_epn(self) -> float:
+        return 0.5 * self.dx if self.epn is None else self.epn
+
+    @property
+    def dim_Pe(self) -> float:
+        e = self.dim_epn
+        return 1.0 / (e * e) if self.Pe is None else self.Pe
+
+    @property
     def cells(self) -> int:
         return self.nx * self.ny * self.nz
 
@@ -72,6 +87,10 @@ class CaseParams:
         for n in ("mom_u", "mom_v", "mom_w"):
             if getattr(self, n) < 2:
                 raise ValueError(f"{n} must be >= 2")
+        if self.idm not in ("lsm", "dim"):
+            raise ValueError("idm must be lsm or dim")
+        if self.idm == "dim" and self.dim_its < 1:
+            raise ValueError("dim_its (max_iteration_dim) must be >= 1")
 
     def solver_dict(self) -> dict:
         d = self.dx
@@ -81,7 +100,9 @@ class CaseParams:
                     mom_u=self.mom_u, mom_v=self.mom_v, mom_w=self.mom_w,
                     levelset_its=self.levelset_its, ksp_rtol=self.ksp_rtol,
                     ksp_abstol=self.ksp_abstol, ksp_dtol=self.ksp_dtol,
-                    ksp_maxits=self.ksp_maxits, ksp_pc=self.pc_code())
+                    ksp_maxits=self.ksp_maxits, ksp_pc=self.pc_code(),
+                    idm_dim=1 if self.idm == "dim" else 0, epn=self.dim_epn, Pe=self.dim_Pe,
+                    dim_its=self.dim_its)
 
     def pc_code(self) -> int:
         """Preconditioner code of the solver interface: 0 none, n >= 1 the

@@ -54,7 +54,7 @@ int grid_of(int f)
         return G_W;
     case TPLS_F_PHI2: case TPLS_F_PHI3: case TPLS_F_PRES: case TPLS_F_PRES_OLD: case TPLS_F_CONV0_PHI:
     case TPLS_F_CONV1_PHI: case TPLS_F_CONV2_PHI: case TPLS_F_FX_CSF: case TPLS_F_FY_CSF: case TPLS_F_FZ_CSF:
-    case TPLS_F_VISC2: case TPLS_F_RHS_P:
+    case TPLS_F_VISC2: case TPLS_F_RHS_P: case TPLS_F_CONVPREV_PHI:
         return G_P;
     case TPLS_F_VISC2_AUG: return G_AUG;
     case TPLS_F_U_INLET: return G_INLET;
@@ -132,6 +132,12 @@ struct tpls_case {
     Tiling tiling;              // launch shapes of the Helmholtz sweeps
     bool use_mg = false;
     DevArray* inlet;
+    // diffuse interface method (prm.idm_dim): ConvPrevPhi, the sweeps' inlet
+    // profile (maxn+1 values of the storage type, outside `owned` so that
+    // loads keep it) and the scalar factors of dim
+    DevArray* cprev = nullptr;
+    void* dim_prof = nullptr;
+    DimCoef dimc{};
     double* red_part = nullptr;
     double* red_out = nullptr;
     CgState* cg = nullptr;
@@ -335,6 +341,7 @@ DevArray* field_ptr(tpls_case* c, int f)
     case TPLS_F_RHS_V: return c->rhs[1];
     case TPLS_F_RHS_W: return c->rhs[2];
     case TPLS_F_U_INLET: return c->inlet;
+    case TPLS_F_CONVPREV_PHI: return c->cprev;
     default: break;
     }
     if (f >= TPLS_F_CONV0_U && f <= TPLS_F_CONV2_W) {
@@ -365,6 +372,7 @@ std::vector<std::pair<std::string, DevArray*>> state_arrays(tpls_case* c)
         }
     v.emplace_back("conv0_phi", c->cphi[0]);
     v.emplace_back("conv1_phi", c->cphi[1]);
+    v.emplace_back("convprev_phi", c->cprev);
     v.emplace_back("phi2", c->phi2);
     v.emplace_back("pres", c->pres);
     v.emplace_back("pres_old", c->pold);
@@ -450,7 +458,7 @@ size_t tpls_case_bytes(int maxl, int maxm, int maxn, int precision)
     n += (6 * uv + 3 * w) * esz;             // u,v,w x 3 buffers
     n += (6 * uv + 3 * w) * esz * 2;         // conv + csf histories
     n += (2 * uv + w) * esz;                 // rhs
-    n += p * esz * (3 + 4 + 3 + 4 + 1);      // cphi, phi2/3/ra/rb, D/sg/ist, fx/fy/fz, mu
+    n += p * esz * (3 + 4 + 3 + 4 + 1 + 1);  // cphi, phi2/3/ra/rb, D/sg/ist, fx/fy/fz, mu, ConvPrevPhi
     n += p * esz;                            // pres
     n += pd * rows * (maxn + 1) * 8 * 4;     // pold + CG vectors
     n += pd * rows * (maxn + 1) * 8 * 2;     // multigrid z and coarse levels (< 1/7 x 2), if enabled
@@ -463,6 +471,8 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (maxl < 6 || maxm < 5 || maxn < 6) return fail(TPLS_ERR_ARG, "grid too small");
     if ((maxm - 1) % 2) return fail(TPLS_ERR_ARG, "maxm-1 must be even (red-black colouring across the periodic seam)");
     if (prm->mom_u < 2 || prm->mom_v < 2 || prm->mom_w < 2) return fail(TPLS_ERR_ARG, "mom_* must be >= 2");
+    if (prm->idm_dim && !(prm->epn > 0.0 && prm->Pe > 0.0 && prm->dim_its >= 1))
+        return fail(TPLS_ERR_ARG, "idm dim needs epn > 0, Pe > 0 and dim_its (max_iteration_dim) >= 1");
     if (prm->precision != TPLS_FP64 && prm->precision != TPLS_FP32) return fail(TPLS_ERR_ARG, "bad precision");
     if (prm->ksp_pc < 0 || prm->ksp_pc > 8) return fail(TPLS_ERR_ARG, "ksp_pc must be 0 (none) or 1..8 (multigrid sweeps)");
     auto rows_ok = [](int r) { return r == 0 || r == 1 || r == 2 || r == 4 || r == 8; };
@@ -533,6 +543,17 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     for (DevArray** a : pd)
         if ((rc = alloc_field(c, a, sp, 8))) return bail(rc);
     if ((rc = alloc_field(c, &c->inlet, si, e))) return bail(rc);
+    if ((rc = alloc_field(c, &c->cprev, sp, e))) return bail(rc);
+    if (prm->idm_dim) {
+        dim_coefficients(prm->dx, prm->dy, prm->dz, prm->dt, prm->epn, prm->Pe, &c->dimc);
+        std::vector<double> prof(maxn + 1);
+        dim_inlet(maxn, prm->dz, prm->height, prm->epn, prof.data());
+        std::vector<float> pf(prof.begin(), prof.end());
+        const void* src = e == 4 ? static_cast<const void*>(pf.data()) : static_cast<const void*>(prof.data());
+        if (hipMalloc(&c->dim_prof, size_t(maxn + 1) * e) != hipSuccess) return bail(fail(TPLS_ERR_NOMEM, "profile"));
+        if (hipMemcpy(c->dim_prof, src, size_t(maxn + 1) * e, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(TPLS_ERR_HIP, "profile upload"));
+    }
     if (prm->ksp_pc > 0) {
         c->use_mg = true;
         c->mg.nu = prm->ksp_pc;
@@ -596,6 +617,7 @@ int tpls_case_destroy(tpls_case* c)
     hipFree(c->red_out);
     hipFree(c->cg);
     hipFree(c->mg_spec);
+    hipFree(c->dim_prof);
     if (c->h_cg) hipHostFree(c->h_cg);
     if (c->h_out) hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -1080,7 +1102,7 @@ struct StepPlan {
 StepPlan plan_step(const tpls_case* c)
 {
     StepPlan p{c->phi3, c->pra, c->prb, {0, 0, 0}};
-    const int nit = c->prm.levelset_its;
+    const int nit = c->prm.idm_dim ? 0 : c->prm.levelset_its;  // dim writes phi3 in place
     if (nit > 0) {
         DevArray* last = ((nit - 1) & 1) ? c->prb : c->pra;
         p.phi3 = last;
@@ -1113,28 +1135,37 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
     // ---- do_advect_phi (main:585) ----
     LCHK(launch_advect_phi<T>(s, g, ph, V(c->phi2), V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
                               V(c->cphi[2])));
-    // ---- do_tpls (main:596) ----
-    LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
-    LCHK(launch_phi_bc<T>(s, g, ph.dz, ph.height, V(c->phi3), 1));
-    LCHK(launch_levelset_status<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg)));
-    {
-        DevArray* src = c->phi3;
-        DevArray* bufs[2] = {c->pra, c->prb};
-        const int nit = c->prm.levelset_its;
-        for (int it = 1; it <= nit; ++it) {
-            DevArray* dst = bufs[(it - 1) & 1];
-            LCHK(launch_levelset_iter<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg), V(src), V(dst)));
-            // levelset.f90:168-216 (h = dx is passed as dz there)
-            LCHK(launch_phi_bc<T>(s, g, ph.dx, ph.height, V(dst), 1));
-            if (dg && it == nit) LCHK(launch_levelset_err<T>(s, g, V(src), V(dst), red_of(c, 0)));
-            src = dst;
+    if (c->prm.idm_dim) {
+        // ---- dim (main:606-611): phiNew iterates in pra, d2phi in D,
+        // source0 in sg; phiNew starts from phi3, i.e. phi2 after a step
+        // and zero before the first ----
+        LCHK(launch_dim<T>(s, g, ph, c->dimc, c->prm.dim_its, c->steps_done == 0 ? 1 : 0, V(c->phi2), V(c->cphi[2]),
+                           V(c->cprev), V(c->phi3), V(c->pra), V(c->D), V(c->sg), static_cast<const T*>(c->dim_prof),
+                           dg ? c->red_out : nullptr));
+    } else {
+        // ---- do_tpls (main:596) ----
+        LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
+        LCHK(launch_phi_bc<T>(s, g, ph.dz, ph.height, V(c->phi3), 1));
+        LCHK(launch_levelset_status<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg)));
+        {
+            DevArray* src = c->phi3;
+            DevArray* bufs[2] = {c->pra, c->prb};
+            const int nit = c->prm.levelset_its;
+            for (int it = 1; it <= nit; ++it) {
+                DevArray* dst = bufs[(it - 1) & 1];
+                LCHK(launch_levelset_iter<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg), V(src), V(dst)));
+                // levelset.f90:168-216 (h = dx is passed as dz there)
+                LCHK(launch_phi_bc<T>(s, g, ph.dx, ph.height, V(dst), 1));
+                if (dg && it == nit) LCHK(launch_levelset_err<T>(s, g, V(src), V(dst), red_of(c, 0)));
+                src = dst;
+            }
         }
     }
     // ---- CSF from the reinitialised phi3 (main:618-627) ----
     LCHK(launch_csf<T>(s, g, ph, V(pl.phi3), V(c->fx), V(c->fy), V(c->fz)));
     LCHK(launch_csf_bc<T>(s, g, V(c->fx), V(c->fy), V(c->fz)));
     // ---- viscosity incl. inlet/outlet planes (main:646-710) ----
-    LCHK(launch_viscosity<T>(s, g, ph, V(c->phi2), V(c->mu), 0, g.maxl));
+    LCHK(launch_viscosity<T>(s, g, ph, V(c->phi2), V(c->mu), 0, g.maxl, c->prm.idm_dim ? 1 : 0));
     // ---- conv, diffusion, CSF faces, AB3/CN right-hand sides (main:629-641, 732-757) ----
     DevArray* fcs[3] = {c->fx, c->fy, c->fz};
     for (int q = 0; q < 3; ++q)
@@ -1199,6 +1230,8 @@ int step_begin(tpls_case* c)
     if (c->use_mg) mg_coefficients(c);
     if (c->prm.graphs && !c->timed) {
         std::vector<const void*> key = {c->phi2, c->phi3, c->pra, c->prb, c->cphi[0], c->cphi[1], c->cphi[2]};
+        // dim's first step starts its iterate from zero, later ones from phi2
+        key.push_back(reinterpret_cast<const void*>(uintptr_t(c->prm.idm_dim && c->steps_done == 0)));
         for (int q = 0; q < 3; ++q) {
             key.push_back(c->x[q][c->i2[q]]);
             for (int b = 0; b < 3; ++b) {
@@ -1513,6 +1546,48 @@ int tpls_do_tpls(double* phi2, double* phi3, double* c0, double* c1, double* c2,
         RC(unstage({&a3}));
     }
     wrap_rows(phi3, sp, maxm);
+    return TPLS_OK;
+}
+
+int tpls_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, double epn, double Pe, const double* phi,
+             const double* phiOld, const double* conv2_phi, double* ConvPrevPhi, double* phiNew, int sx, int ex, int sy,
+             int ey, double height, int max_iteration_dim, double* res)
+{
+    RC(drop_init());
+    RC(check_decomp(sx, ex, sy, ey, maxl - 1));
+    if (!phi || !phiOld || !conv2_phi || !ConvPrevPhi || !phiNew) return fail(TPLS_ERR_ARG, "null array");
+    if (max_iteration_dim < 0 || !(epn > 0.0) || !(Pe > 0.0))
+        return fail(TPLS_ERR_ARG, "dim needs epn > 0, Pe > 0 and max_iteration_dim >= 0");
+    const int maxm = ey + 1;
+    Grid g{maxl, maxm, maxn, maxm - 1};
+    const Shape sp = shape_of(G_P, maxl, maxm, maxn);
+    std::vector<double> prof(maxn + 1);
+    dim_inlet(maxn, dz, height, epn, prof.data());
+    DimCoef dc;
+    dim_coefficients(dx, dy, dz, dt, epn, Pe, &dc);
+    Stg a_phi, a_po, a_c2, a_cp, a_pn, a_d2, a_src, a_prof;
+    RC(stage(0, phi, sp, 8, 0, false, a_phi));
+    RC(stage(1, phiOld, sp, 8, 0, false, a_po));
+    RC(stage(2, conv2_phi, sp, 8, 0, false, a_c2));
+    RC(stage(3, ConvPrevPhi, sp, 8, 0, true, a_cp));
+    RC(stage(4, phiNew, sp, 8, 0, true, a_pn));
+    RC(stage(5, nullptr, sp, 8, 1, false, a_d2));
+    RC(stage(6, nullptr, sp, 8, 1, false, a_src));
+    RC(stage(7, prof.data(), Shape{0, maxn, 0, 0, 0, 0}, 8, 0, false, a_prof));
+    hipStream_t s = g_drop.s;
+    HIPCHK(launch_dim_core<double>(s, g, phys_d(dx, dy, dz, dt), dc, max_iteration_dim, 0.0, a_phi.a.view<double>(),
+                                   a_po.a.view<double>(), a_c2.a.view<double>(), a_cp.a.view<double>(),
+                                   a_pn.a.view<double>(), a_d2.a.view<double>(), a_src.a.view<double>(),
+                                   static_cast<const double*>(a_prof.a.ptr), res ? g_drop.red_out : nullptr));
+    RC(unstage({&a_cp, &a_pn}));
+    if (res) {
+        HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *res = g_drop.h_out[0];
+    }
+    // cahn_hilliard_solver.f90:340-341 refresh phiNew's y halos (the inlet and
+    // outlet columns of those rows are set from the interior rows here)
+    wrap_rows(phiNew, sp, maxm);
     return TPLS_OK;
 }
 

@@ -1229,6 +1229,11 @@ int tpls_read_config(const char* config_dir, const char* options_file, tpls_para
     if ((rc = initial_conditions(c, prm, run))) return rc < 0 ? rc : set_error(TPLS_ERR_CONFIG, c.msg);
     c.o = opts.o;
     if ((rc = run_options(c, prm, run))) return rc < 0 ? rc : set_error(TPLS_ERR_CONFIG, c.msg);
+    // the interface method and its parameters, as the case reads them
+    prm->idm_dim = run->idm_dim;
+    prm->epn = run->idm_dim ? run->epn : 0.0;
+    prm->Pe = run->idm_dim ? run->Pe : 0.0;
+    prm->dim_its = run->idm_dim ? run->max_iteration_dim : 0;
     *ierr = 0;
     return TPLS_OK;
 }

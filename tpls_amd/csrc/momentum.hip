@@ -10,13 +10,15 @@ namespace tpls {
 
 template <typename T>
 __global__ void __launch_bounds__(BX* BY) k_viscosity(Grid g, T sw, T Re, T mum, T mup, FieldView<T> phi,
-                                                      FieldView<T> mu, int ilo, int ihi)
+                                                      FieldView<T> mu, int ilo, int ihi, int shift)
 {
     const int i = blockIdx.x * BX + threadIdx.x + ilo;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     const int k = blockIdx.z;
     if (i > ihi || j > g.P) return;
-    mu(i, j, k) = visc_of_phi<T>(phi(i, j, k), sw, Re, mum, mup);
+    T pv = phi(i, j, k);
+    if (shift) pv = pv - T(0.5);
+    mu(i, j, k) = visc_of_phi<T>(pv, sw, Re, mum, mup);
 }
 
 // ---- convective terms, momentum_allflux.f90:281-337 / 350-402 / 415-463 ----
@@ -212,11 +214,11 @@ __global__ void __launch_bounds__(BX* BY) k_momentum(Grid g, int comp, int mode,
 
 template <typename T>
 hipError_t launch_viscosity(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, FieldView<T> mu, int ilo,
-                            int ihi)
+                            int ihi, int shift)
 {
     dim3 grd((ihi - ilo + 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn + 1);
     hipLaunchKernelGGL(k_viscosity<T>, grd, dim3(BX, BY), 0, s, g, T(ph.sw), T(ph.Re), T(ph.mu_minus), T(ph.mu_plus),
-                       phi, mu, ilo, ihi);
+                       phi, mu, ilo, ihi, shift);
     return hipGetLastError();
 }
 
@@ -235,7 +237,7 @@ hipError_t launch_momentum(hipStream_t s, const Grid& g, const Phys& ph, int com
 
 #define TPLS_INST_MOM(T)                                                                                          \
     template hipError_t launch_viscosity<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, FieldView<T>,   \
-                                            int, int);                                                          \
+                                            int, int, int);                                                     \
     template hipError_t launch_momentum<T>(hipStream_t, const Grid&, const Phys&, int, int, FieldView<T>,        \
                                            FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, \
                                            FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, \

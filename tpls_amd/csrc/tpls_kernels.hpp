@@ -53,10 +53,38 @@ template <typename T>
 hipError_t launch_csf_bc(hipStream_t s, const Grid& g, FieldView<T> fx, FieldView<T> fy,
                          FieldView<T> fz);
 
+// ---- diffuse interface (cahn_hilliard_solver.f90) ----
+// Scalar factors of dim, in FP64: the explicit part's coefficients
+// nmda*tao/2 and epn^2*nmda/2, the sweep's dt*nmda/2/Pe, tao, alfa1 and
+// diagonal.
+struct DimCoef {
+    double epn, epn2, Pe, dt, c_tao, c_epn, c_rel, tao, alfa1, diagonal;
+};
+void dim_coefficients(double dx, double dy, double dz, double dt, double epn, double Pe, DimCoef* c);
+// inlet profile phiNew(0,.,k), k = 0..maxn (maxn+1 values)
+void dim_inlet(int maxn, double dz, double height, double epn, double* prof);
+// One dim call: phi3 = (last iterate of max_iteration_dim sweeps) - 0.5, with
+// ConvPrevPhi updated.  zero: phiNew starts from zero (the first step) instead
+// of phi2.  pn, d2, src: work arrays of the p shape; inlet: maxn+1 values on
+// the device; res: if non-null, the last sweep's largest |change|.
+template <typename T>
+hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, int zero,
+                      FieldView<T> phi2, FieldView<T> conv2, FieldView<T> cprev, FieldView<T> phi3,
+                      FieldView<T> pn, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res);
+// The body of dim on given arrays: phi = phi_in + shift, phiOld = phio +
+// shift; pn holds the initial iterate (boundary planes included) and receives
+// the last.
+template <typename T>
+hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, double shift,
+                           FieldView<T> phi, FieldView<T> phio, FieldView<T> conv2, FieldView<T> cprev,
+                           FieldView<T> pn, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res);
+
 // ---- momentum (momentum_allflux.f90, main_ns_hybrid.F90:629-757) ----
+// shift: 0, or 1 for the diffuse interface method, whose viscosity is that
+// of phi2 - 0.5 (main_ns_hybrid.F90:650-651, 664-668, 689-693)
 template <typename T>
 hipError_t launch_viscosity(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi,
-                            FieldView<T> mu, int ilo, int ihi);
+                            FieldView<T> mu, int ilo, int ihi, int shift = 0);
 // mode bit 1: write conv, bit 2: write diff, bit 4: fused AB3/CN right-hand
 // side (needs conv0/1, csf faces and histories)
 template <typename T>

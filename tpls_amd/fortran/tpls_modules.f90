@@ -2,13 +2,13 @@
 !
 ! These modules carry the reference's module and procedure names and dummy
 ! argument lists (weno.f90:18, advect_phi.f90:19, two_phase_levelset.f90:20,
-! levelset.f90:17/97/241, momentum_allflux.f90:22/224/474,
+! levelset.f90:17/97/241, cahn_hilliard_solver.f90:41, momentum_allflux.f90:22/224/474,
 ! jacobi_iteration_allflux.f90:21/102/195/285, sor_iteration_allflux.f90:19/
 ! 152/280, pressure.f90:16/59), so main_ns_hybrid.F90 compiles against them
 ! unchanged and its calls run the gfx950 kernels of libtpls_hip.so through
 ! the C ABI of include/tpls_core.h (module tpls_core_c below).
 !
-! The MPI arguments of do_advect_phi and do_tpls (neighbours, communicator,
+! The MPI arguments of do_advect_phi, do_tpls and dim (neighbours, communicator,
 ! stride datatypes) are accepted and ignored: the whole domain lives on one
 ! GPU and the halo exchanges are in-kernel periodic wraps.  A non-zero status
 ! from the C ABI stops the program with the library's message, as the
@@ -196,6 +196,16 @@ module tpls_core_c
       integer(c_int), intent(out) :: its, reason
       real(c_double), intent(out) :: rnorm
     end function
+
+    integer(c_int) function c_dim(maxl, maxn, dx, dy, dz, dt, epn, pe, phi, phi_old, conv2_phi, &
+        conv_prev, phi_new, sx, ex, sy, ey, height, its, res) bind(C, name="tpls_dim")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: maxl, maxn, sx, ex, sy, ey, its
+      real(c_double), value :: dx, dy, dz, dt, epn, pe, height
+      real(c_double), intent(in) :: phi(*), phi_old(*), conv2_phi(*)
+      real(c_double), intent(inout) :: conv_prev(*), phi_new(*)
+      type(c_ptr), value :: res
+    end function
   end interface
 
 contains
@@ -341,6 +351,35 @@ contains
         max_iteration_levelset, height, err2, dx, dz, dt), 'do_tpls')
   end subroutine do_tpls
 end module two_phase_levelset
+
+
+module cahn_hilliard_solver
+  use tpls_core_c
+  implicit none
+contains
+  subroutine dim(maxl, maxn, dx, dy, dz, dt, epn, Pe, phi, phiOld, conv2_phi, &
+      ConvPrevPhi, phiNew, sx, ex, sy, ey, sz_p, ez_p, height, &
+      stride_p_xz, stride_p_yz, neighbours, comm2d_quasiperiodic, &
+      max_iteration_dim)
+    integer, intent(in) :: maxl, maxn
+    integer, intent(in) :: sx, ex, sy, ey, sz_p, ez_p
+    integer, intent(in) :: stride_p_xz, stride_p_yz
+    integer, intent(in) :: neighbours(6)
+    integer, intent(in) :: comm2d_quasiperiodic
+    integer, intent(in) :: max_iteration_dim
+    double precision, intent(in) :: dx, dy, dz, dt
+    double precision, intent(in) :: Pe, epn
+    double precision, intent(in) :: height
+    double precision, intent(in) :: phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(in) :: phiOld(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(in) :: conv2_phi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: ConvPrevPhi(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    double precision, intent(inout) :: phiNew(sx-1:ex+1, sy-1:ey+1, sz_p:ez_p)
+    if (sz_p /= 0 .or. ez_p /= maxn) call tpls_check(-2_c_int, 'dim: z bounds')
+    call tpls_check(c_dim(maxl, maxn, dx, dy, dz, dt, epn, Pe, phi, phiOld, conv2_phi, ConvPrevPhi, &
+        phiNew, sx, ex, sy, ey, height, max_iteration_dim, c_null_ptr), 'dim')
+  end subroutine dim
+end module cahn_hilliard_solver
 
 
 module momentum_allflux

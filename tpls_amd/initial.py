@@ -16,6 +16,9 @@ periodic wrap.
 * Level set: the first-order signed distance (z - eta)/sqrt(1+|grad eta|^2)
   to eta = height + 0.05 * sum of 4 seeded sinusoids
   sin(2 pi (k x/Lx + l y/Ly) + theta).
+  With the diffuse interface method (idm "dim") phi is the phase fraction
+  0.5 + 0.5 tanh(d / (2 sqrt(2) epn)) of that distance d, the profile of
+  twophase_initialisation_wave.F90:192-195.
 * Pressure: zero (the solver's first initial guess is zero as well,
   main_ns_hybrid.F90:423).
 
@@ -85,6 +88,8 @@ def make_initial(p: CaseParams):
         gy += 0.05 * np.cos(arg) * 2 * np.pi * ly[m] / Ly
     norm = np.sqrt(1.0 + gx * gx + gy * gy)
     phi = np.asfortranarray((z[None, None, :] - eta[:, :, None]) / norm[:, :, None])
+    if p.idm == "dim":
+        phi = np.asfortranarray(0.5 + 0.5 * np.tanh(phi / (2.0 * 2.0 ** 0.5 * p.dim_epn)))
     _wrap_y(phi)
     pres = np.zeros((maxl + 1, maxm + 1, maxn + 1), order="F")
     return u, v, w, pres, phi

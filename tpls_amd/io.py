@@ -419,8 +419,11 @@ def write_initial_conditions(config_dir, p, fields, dpdl: float = -1.0, omega_ma
                         ("scap", float(p.scap)), ("omega_max", float(omega_max)),
                         ("smooth_width", float(p.smooth_width)), ("dx", float(p.dx)),
                         ("dy", float(p.dx)), ("dz", float(p.dx)), ("dt", float(p.dt)),
-                        ("idm", "lsm")):
+                        ("idm", p.idm)):
         o.put_typed_option(name, value)
+    if p.idm == "dim":
+        o.put_typed_option("pe", float(p.dim_Pe))
+        o.put_typed_option("epn", float(p.dim_epn))
     o.save_options(path(INITIAL_CONFIG_FILE),
                    ["TPLS initial conditions", "Auto-generated by tpls_amd.io"])
 
@@ -434,6 +437,8 @@ def write_options_file(filename, p, n_timesteps: int = 1, maxu: float = 10.0, **
             "mom_w": p.mom_w, "levelset": p.levelset_its, "maxu": float(maxu), "phi_dat_frequency": 1,
             "uvw_dat_frequency": 1, "backup_frequency": 1, "backup_text_format": True,
             "backup_hdf5_format": False, "num_timesteps": n_timesteps}
+    if p.idm == "dim":
+        keys["max_iteration_dim"] = p.dim_its
     keys.update(options)
     for name, value in keys.items():
         o.put_typed_option(name, value)
