@@ -75,6 +75,9 @@ def parse():
                          "capture every graph)")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="timed steps of the plain-CG comparison run (0 to skip)")
+    ap.add_argument("--dim-steps", type=int, default=3,
+                    help="timed steps of the same grid with the diffuse interface method (0 to skip)")
+    ap.add_argument("--dim-only", action="store_true", help="run only the diffuse-interface workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 64 Mi doubles three times (the reference "
@@ -181,6 +184,35 @@ def plain_cg_run(p, a):
     return {"ms_per_step": round(el / a.plain_steps * 1e3, 3),
             "value": round(q.cells * a.plain_steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
             "steps": a.plain_steps, "warmup": 1, "cg_its_mean": round(float(np.mean(its)), 1)}
+
+
+def dim_run(p, a):
+    """The same grid and solver with the diffuse interface method (idm dim,
+    max_iteration_dim relaxation sweeps per step in place of the level-set
+    update and reinitialisation); the phase field starts from the
+    generator's equilibrium profile."""
+    import numpy as np
+    from dataclasses import replace
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    q = replace(p, idm="dim")
+    case = Case(q, precision=a.precision, **case_opts(a))
+    case.load(*make_initial(q))
+    for _ in range(max(a.warmup, 1)):
+        case.step(1)
+    case.sync()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(a.dim_steps):
+        case.step(1)
+        case.sync()
+        its.append(case.diag()["ksp_its"])
+    el = time.perf_counter() - t0
+    case.close()
+    return {"ms_per_step": round(el / a.dim_steps * 1e3, 3),
+            "value": round(q.cells * a.dim_steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
+            "steps": a.dim_steps, "warmup": max(a.warmup, 1), "max_iteration_dim": q.dim_its,
+            "cg_its_mean": round(float(np.mean(its)), 1)}
 
 
 def copy_probe():
@@ -311,8 +343,8 @@ def main():
         copy_probe()
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs one case per GPU on a single GPU; "
-                                   "--gpus must be 1 (see DESIGN.md for how the path shards)"}))
+        print(json.dumps({"error": "--gpus must be 1: the benchmark measures one GPU (independent cases "
+                                   "shard across GPUs by HBM budget, one large case by slabs along x)"}))
         sys.exit(2)
     import numpy as np
     from tpls_amd.case import Case
@@ -324,6 +356,9 @@ def main():
 
     nx, ny, nz = (int(v) for v in a.grid.split("x"))
     p = scaled(CHANNEL_512x128x256, nx=nx, ny=ny, nz=nz, ksp_pc=a.pc, ksp_mg_nu=a.mg_nu)
+    if a.dim_only:
+        print(json.dumps({f"dim_{a.precision}": dim_run(p, a)}), flush=True)
+        return
     init = make_initial(p)
     case = Case(p, precision=a.precision, **case_opts(a))
     # the Jacobi and even-parity sweeps run staged unless the shape says -1
@@ -399,6 +434,8 @@ def main():
     }
     if plain:
         out["plain_cg"] = plain
+    if a.dim_steps > 0:
+        out[f"dim_{a.precision}"] = dim_run(p, a)
     if ref is not None:
         out["fp32"] = fp32_run(p, a, ref, ref_volume)
     if a.batch_cases > 0:

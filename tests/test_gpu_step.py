@@ -1,7 +1,7 @@
 """Whole timesteps on the GPU (device-resident case, tpls_step) against the
 CPU oracle's step driver on identical seeded inputs.
 
-Tolerances (north-star): plain FP64 agrees within 1e-12 (normwise relative)
+Tolerances: plain FP64 agrees within 1e-12 (normwise relative)
 after one step and within 1e-9 on phi, u, v, w and p after many steps; the
 pressure is compared modulo its constant null-space component.  Both sides
 run the same CG with the same stopping rule; the iteration counts must agree
