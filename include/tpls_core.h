@@ -97,6 +97,9 @@ typedef struct {
     double epn, Pe;       /* DIM interface width and Peclet number
                              (get_dim_configuration, configure.F90:404-431) */
     int dim_its;          /* max_iteration_dim (configure.F90:437-465), >= 1 */
+    int dim_split;        /* 1: each DIM relaxation sweep as two launches (the
+                             Laplacian, then the in-place update) instead of
+                             one fused out-of-place pass; identical results */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */

@@ -35,9 +35,11 @@ def fused_code(sor_fused) -> int:
 
 
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
-                graphs: bool = False, tiles=None, sor_fused=None, mg_fp32=None) -> _lib.Params:
+                graphs: bool = False, tiles=None, sor_fused=None, mg_fp32=None,
+                dim_split: bool = False) -> _lib.Params:
     prm = _lib.Params()
     prm.mg_fp32 = {None: 0, True: 1, False: -1}[mg_fp32]
+    prm.dim_split = 1 if dim_split else 0
     for k, v in p.solver_dict().items():
         setattr(prm, k, v)
     prm.precision = {"fp64": _lib.FP64, "fp32": _lib.FP32}[precision]
@@ -63,7 +65,7 @@ FILE_FIELDS = ("dx", "dy", "dz", "dt", "Re", "mu_plus", "mu_minus", "height", "s
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                  graphs: bool = False, tiles=None, sor_fused=None, file_params: _lib.Params | None = None,
-                 mg_fp32=None):
+                 mg_fp32=None, dim_split: bool = False):
         """graphs: replay each step's launches from HIP graphs (same results).
         tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst[, sor_fused]]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
@@ -73,7 +75,8 @@ class Case:
         all / none; or the library's code).  Every choice computes the same
         values.  mg_fp32: precision of the multigrid V-cycle (None: the
         case's precision; True: FP32; False: FP64); the CG around it stays
-        FP64.  file_params: the tpls_params of a configuration
+        FP64.  dim_split: with idm dim, run each relaxation sweep as two
+        launches instead of one fused pass (same values).  file_params: the tpls_params of a configuration
         (tpls_amd.io.read_configuration), whose file-backed fields replace
         those params implies."""
         if tiles == "auto":
@@ -84,7 +87,7 @@ class Case:
         self.precision = precision
         self.shape = (params.maxl, params.maxm, params.maxn)
         self.tiles = tiles
-        self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused, mg_fp32)
+        self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused, mg_fp32, dim_split)
         if file_params is not None:
             for k in FILE_FIELDS:
                 setattr(self._prm, k, getattr(file_params, k))

@@ -1136,12 +1136,13 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
     LCHK(launch_advect_phi<T>(s, g, ph, V(c->phi2), V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
                               V(c->cphi[2])));
     if (c->prm.idm_dim) {
-        // ---- dim (main:606-611): phiNew iterates in pra, d2phi in D,
-        // source0 in sg; phiNew starts from phi3, i.e. phi2 after a step
-        // and zero before the first ----
+        // ---- dim (main:606-611): phiNew iterates in pra (and prb, the
+        // fused sweeps' ping-pong partner), d2phi in D, source0 in sg;
+        // phiNew starts from phi3, i.e. phi2 after a step and zero before
+        // the first ----
         LCHK(launch_dim<T>(s, g, ph, c->dimc, c->prm.dim_its, c->steps_done == 0 ? 1 : 0, V(c->phi2), V(c->cphi[2]),
-                           V(c->cprev), V(c->phi3), V(c->pra), V(c->D), V(c->sg), static_cast<const T*>(c->dim_prof),
-                           dg ? c->red_out : nullptr));
+                           V(c->cprev), V(c->phi3), V(c->pra), V(c->prb), V(c->D), V(c->sg),
+                           static_cast<const T*>(c->dim_prof), dg ? c->red_out : nullptr, c->prm.dim_split ? 0 : 1));
     } else {
         // ---- do_tpls (main:596) ----
         LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
@@ -1565,21 +1566,27 @@ int tpls_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, dou
     dim_inlet(maxn, dz, height, epn, prof.data());
     DimCoef dc;
     dim_coefficients(dx, dy, dz, dt, epn, Pe, &dc);
-    Stg a_phi, a_po, a_c2, a_cp, a_pn, a_d2, a_src, a_prof;
+    Stg a_phi, a_po, a_c2, a_cp, a_pn, a_pn2, a_d2, a_src, a_prof;
     RC(stage(0, phi, sp, 8, 0, false, a_phi));
     RC(stage(1, phiOld, sp, 8, 0, false, a_po));
     RC(stage(2, conv2_phi, sp, 8, 0, false, a_c2));
     RC(stage(3, ConvPrevPhi, sp, 8, 0, true, a_cp));
-    RC(stage(4, phiNew, sp, 8, 0, true, a_pn));
+    // the fused sweeps alternate between two copies of the initial iterate
+    RC(stage(4, phiNew, sp, 8, 0, false, a_pn));
+    RC(stage(8, phiNew, sp, 8, 0, false, a_pn2));
     RC(stage(5, nullptr, sp, 8, 1, false, a_d2));
     RC(stage(6, nullptr, sp, 8, 1, false, a_src));
     RC(stage(7, prof.data(), Shape{0, maxn, 0, 0, 0, 0}, 8, 0, false, a_prof));
     hipStream_t s = g_drop.s;
     HIPCHK(launch_dim_core<double>(s, g, phys_d(dx, dy, dz, dt), dc, max_iteration_dim, 0.0, a_phi.a.view<double>(),
                                    a_po.a.view<double>(), a_c2.a.view<double>(), a_cp.a.view<double>(),
-                                   a_pn.a.view<double>(), a_d2.a.view<double>(), a_src.a.view<double>(),
-                                   static_cast<const double*>(a_prof.a.ptr), res ? g_drop.red_out : nullptr));
-    RC(unstage({&a_cp, &a_pn}));
+                                   a_pn.a.view<double>(), a_pn2.a.view<double>(), a_d2.a.view<double>(),
+                                   a_src.a.view<double>(), static_cast<const double*>(a_prof.a.ptr),
+                                   res ? g_drop.red_out : nullptr, 1));
+    Stg* last = (max_iteration_dim & 1) ? &a_pn2 : &a_pn;
+    last->host = phiNew;
+    last->out = true;
+    RC(unstage({&a_cp, last}));
     if (res) {
         HIPCHK(hipMemcpyAsync(g_drop.h_out, g_drop.red_out, sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

@@ -199,6 +199,121 @@ __global__ void __launch_bounds__(BX* BY)
     }
 }
 
+// One relaxation sweep in one pass (the two launches above fused), out of
+// place: pi is the previous iterate, po receives the new one.  A block owns a
+// tile of DIM_TI x TJ columns and marches up a chunk of ks levels.  Per level
+// c it stages plane c+1 of the iterate with a two-cell ring (rows wrapped,
+// columns clamped to the stored boundary planes 0..maxl; loaded one level
+// ahead into registers), forms d2phi of
+// plane c on the tile with a one-cell ring (ring columns outside the
+// interior take the clamped interior value, as the copies of
+// cahn_hilliard_solver.f90:264-292 make them), then updates plane c-1.  Every
+// output reads the same values k_dim_lap + k_dim_update read, so the results
+// are identical.  The boundary planes of po that no sweep writes (the inlet
+// and outlet columns at k = 0 and maxn) must already hold the first iterate:
+// k_dim_init fills both buffers.
+constexpr int DIM_TI = 64;
+
+template <typename T, int TJ>
+__global__ void __launch_bounds__(DIM_TI* TJ)
+    k_dim_sweep(Grid g, int nbx, int nby, int ks, T dx, T dy, T dz, T dt, T c_rel, T tao, T epn2, T alfa1,
+                T diagonal, T h, FieldView<T> phi2, FieldView<T> phio, FieldView<T> src,
+                const T* __restrict__ inlet, FieldView<T> pi, FieldView<T> po, unsigned long long* resmax)
+{
+    constexpr int PW = DIM_TI + 4, PH = TJ + 4;  // iterate window: two-cell ring
+    constexpr int DW = DIM_TI + 2, DH = TJ + 2;  // d2phi window: one-cell ring
+    constexpr int NT = DIM_TI * TJ;
+    __shared__ T ps[4][PH][PW];
+    __shared__ T ds[4][DH][DW];
+    const int t = blockIdx.x;
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int L = g.maxl, n = g.maxn, P = g.P;
+    const int i0 = 1 + bx * DIM_TI, j0 = 1 + by * TJ;
+    const int kb = 1 + bz * ks, ke = min(kb + ks - 1, n - 1);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * DIM_TI + tx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool mine = i <= L - 1 && j <= P;
+    auto wrapP = [&](int jj) { return (jj - 1 + 2 * P) % P + 1; };
+    // plane c of the iterate: loaded into registers (this thread's share of
+    // the window), then stored into its ring slot
+    constexpr int NQ = (PW * PH + NT - 1) / NT;
+    auto fetch = [&](int c, T (&r)[NQ]) {
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            const int q = min(tid + m * NT, PW * PH - 1);
+            const int li = q % PW, lj = q / PW;
+            r[m] = pi(min(max(i0 - 2 + li, 0), L), wrapP(j0 - 2 + lj), c);
+        }
+    };
+    auto put = [&](int c, const T (&r)[NQ]) {
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            const int q = tid + m * NT;
+            if (q < PW * PH) ps[c & 3][q / PW][q % PW] = r[m];
+        }
+    };
+    auto stage = [&](int c) {
+        T r[NQ];
+        fetch(c, r);
+        put(c, r);
+    };
+    // d2phi of plane c (1 <= c <= n-1) over the one-cell ring window
+    auto d2plane = [&](int c) {
+        for (int q = tid; q < DW * DH; q += NT) {
+            const int a = q % DW, b = q / DW;
+            const int ca = min(max(i0 - 1 + a, 1), L - 1);
+            const int li = ca - (i0 - 2), lj = b + 1;
+            const T cc = ps[c & 3][lj][li];
+            ds[c & 3][b][a] = (ps[c & 3][lj][li + 1] - T(2) * cc + ps[c & 3][lj][li - 1]) / dx / dx +
+                              (ps[c & 3][lj + 1][li] - T(2) * cc + ps[c & 3][lj - 1][li]) / dy / dy +
+                              (ps[(c + 1) & 3][lj][li] - T(2) * cc + ps[(c - 1) & 3][lj][li]) / dz / dz;
+        }
+    };
+    double change = 0.0;
+    // update of plane k from d2phi planes clampz(k-1), k, clampz(k+1)
+    auto update = [&](int k) {
+        if (!mine) return;
+        const int a = tx + 1, b = ty + 1;
+        const int km = k > 1 ? k - 1 : 1, kp = k < n - 1 ? k + 1 : n - 1;
+        const T d = ds[k & 3][b][a];
+        const T B = (ds[k & 3][b][a + 1] - T(2) * d + ds[k & 3][b][a - 1]) / dx / dx +
+                    (ds[k & 3][b + 1][a] - T(2) * d + ds[k & 3][b - 1][a]) / dy / dy +
+                    (ds[kp & 3][b][a] - T(2) * d + ds[km & 3][b][a]) / dz / dz;
+        const T p = sh(phi2(i, j, k), h), pold = sh(phio(i, j, k), h);
+        const T old = ps[k & 3][ty + 2][tx + 2];
+        const T some = T(2) * p - T(0.5) * pold - T(1.5) * old + dt * src(i, j, k) + c_rel * (tao * d - epn2 * B);
+        T nw = old + alfa1 * some / diagonal;
+        if (nw < T(0)) nw = T(0);
+        if (nw > T(1)) nw = T(1);
+        po(i, j, k) = nw;
+        if (k == 1) po(i, j, 0) = nw;
+        if (k == n - 1) po(i, j, n) = nw;
+        if (i == L - 1) po(L, j, k) = nw;
+        if (i == 1) po(0, j, k) = inlet[k];
+        change = fmax(change, double(fabs(nw - old)));
+    };
+    const int clo = max(kb - 1, 1), chi = min(ke + 1, n - 1);
+    stage(clo - 1);
+    stage(clo);
+    // plane c+2 is in flight while plane c's d2phi and plane c-1's update run
+    T nxt[NQ];
+    fetch(clo + 1, nxt);
+    for (int c = clo; c <= chi; ++c) {
+        put(c + 1, nxt);
+        if (c + 2 <= n) fetch(c + 2, nxt);
+        __syncthreads();
+        d2plane(c);
+        __syncthreads();
+        if (c - 1 >= kb) update(c - 1);
+    }
+    if (ke == n - 1) update(ke);
+    if (resmax) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) change = fmax(change, __shfl_xor(change, off, 64));
+        if (tx == 0) atomicMax(resmax, (unsigned long long)__double_as_longlong(change));
+    }
+}
+
 // phi3 = phiNew - 0.5 over i = 0..maxl, k = 0..maxn (main_ns_hybrid.F90:611)
 template <typename T>
 __global__ void __launch_bounds__(BX* BY) k_dim_final(Grid g, FieldView<T> pn, FieldView<T> phi3)
@@ -243,10 +358,15 @@ void dim_inlet(int maxn, double dz, double height, double epn, double* prof)
 static inline dim3 grid_int(const Grid& g) { return dim3((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn - 1); }
 static inline dim3 grid_all(const Grid& g) { return dim3((g.maxl + 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn + 1); }
 
+// levels per block of the fused sweep
+constexpr int DIM_KS = 32;
+constexpr int DIM_TJ = 8;
+
 template <typename T>
 hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, double shift,
                            FieldView<T> phi, FieldView<T> phio, FieldView<T> conv2, FieldView<T> cprev,
-                           FieldView<T> pn, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res)
+                           FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
+                           double* res, int fused)
 {
     if (res) {
         hipError_t e = launch_fill<double>(s, res, 1, 0.0);
@@ -257,11 +377,22 @@ hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const D
     hipLaunchKernelGGL(k_dim_lap<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, h, phi, d2);
     hipLaunchKernelGGL(k_dim_source<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(c.epn), T(c.epn2), T(c.Pe),
                        T(c.c_tao), T(c.c_epn), h, phi, d2, conv2, cprev, src);
+    const int nbx = (g.maxl - 1 + DIM_TI - 1) / DIM_TI, nby = (g.P + DIM_TJ - 1) / DIM_TJ;
+    const int nbz = (g.maxn - 1 + DIM_KS - 1) / DIM_KS;
     for (int it = 1; it <= its; ++it) {
-        hipLaunchKernelGGL(k_dim_lap<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(0), pn, d2);
-        hipLaunchKernelGGL(k_dim_update<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(c.dt), T(c.c_rel), T(c.tao),
-                           T(c.epn2), T(c.alfa1), T(c.diagonal), h, phi, phio, src, d2, inlet, pn,
-                           res && it == its ? reinterpret_cast<unsigned long long*>(res) : nullptr);
+        unsigned long long* rm = res && it == its ? reinterpret_cast<unsigned long long*>(res) : nullptr;
+        if (fused) {
+            // odd sweeps read pn and write pn2, even sweeps the reverse
+            FieldView<T> a = (it & 1) ? pn : pn2, b = (it & 1) ? pn2 : pn;
+            hipLaunchKernelGGL((k_dim_sweep<T, DIM_TJ>), dim3(nbx * nby * nbz), dim3(DIM_TI, DIM_TJ), 0, s, g, nbx,
+                               nby, DIM_KS, dx, dy, dz, T(c.dt), T(c.c_rel), T(c.tao), T(c.epn2), T(c.alfa1),
+                               T(c.diagonal), h, phi, phio, src, inlet, a, b, rm);
+        } else {
+            hipLaunchKernelGGL(k_dim_lap<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(0), pn, d2);
+            hipLaunchKernelGGL(k_dim_update<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(c.dt), T(c.c_rel),
+                               T(c.tao), T(c.epn2), T(c.alfa1), T(c.diagonal), h, phi, phio, src, d2, inlet, pn,
+                               rm);
+        }
     }
     return hipGetLastError();
 }
@@ -269,23 +400,27 @@ hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const D
 template <typename T>
 hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, int zero,
                       FieldView<T> phi2, FieldView<T> conv2, FieldView<T> cprev, FieldView<T> phi3, FieldView<T> pn,
-                      FieldView<T> d2, FieldView<T> src, const T* inlet, double* res)
+                      FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res, int fused)
 {
     const dim3 blk(BX, BY);
     hipLaunchKernelGGL(k_dim_init<T>, grid_all(g), blk, 0, s, g, zero, phi2, pn);
-    hipError_t e = launch_dim_core<T>(s, g, ph, c, its, 0.5, phi2, phi2, conv2, cprev, pn, d2, src, inlet, res);
+    if (fused) hipLaunchKernelGGL(k_dim_init<T>, grid_all(g), blk, 0, s, g, zero, phi2, pn2);
+    hipError_t e =
+        launch_dim_core<T>(s, g, ph, c, its, 0.5, phi2, phi2, conv2, cprev, pn, pn2, d2, src, inlet, res, fused);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dim_final<T>, grid_all(g), blk, 0, s, g, pn, phi3);
+    // the last iterate: in pn2 after an odd number of fused sweeps
+    hipLaunchKernelGGL(k_dim_final<T>, grid_all(g), blk, 0, s, g, fused && (its & 1) ? pn2 : pn, phi3);
     return hipGetLastError();
 }
 
 #define TPLS_INST_DIM(T)                                                                                         \
     template hipError_t launch_dim<T>(hipStream_t, const Grid&, const Phys&, const DimCoef&, int, int,          \
                                       FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>,     \
-                                      FieldView<T>, FieldView<T>, const T*, double*);                           \
+                                      FieldView<T>, FieldView<T>, FieldView<T>, const T*, double*, int);        \
     template hipError_t launch_dim_core<T>(hipStream_t, const Grid&, const Phys&, const DimCoef&, int, double,  \
                                            FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>,              \
-                                           FieldView<T>, FieldView<T>, FieldView<T>, const T*, double*);
+                                           FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, const T*,    \
+                                           double*, int);
 TPLS_INST_DIM(double)
 TPLS_INST_DIM(float)
 

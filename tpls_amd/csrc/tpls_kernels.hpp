@@ -65,19 +65,24 @@ void dim_coefficients(double dx, double dy, double dz, double dt, double epn, do
 void dim_inlet(int maxn, double dz, double height, double epn, double* prof);
 // One dim call: phi3 = (last iterate of max_iteration_dim sweeps) - 0.5, with
 // ConvPrevPhi updated.  zero: phiNew starts from zero (the first step) instead
-// of phi2.  pn, d2, src: work arrays of the p shape; inlet: maxn+1 values on
-// the device; res: if non-null, the last sweep's largest |change|.
+// of phi2.  pn, pn2, d2, src: work arrays of the p shape (pn2 only for the
+// fused sweep); inlet: maxn+1 values on the device; res: if non-null, the
+// last sweep's largest |change|.  fused: each sweep as one out-of-place pass
+// (identical results) instead of the Laplacian and the in-place update.
 template <typename T>
 hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, int zero,
                       FieldView<T> phi2, FieldView<T> conv2, FieldView<T> cprev, FieldView<T> phi3,
-                      FieldView<T> pn, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res);
+                      FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
+                      double* res, int fused);
 // The body of dim on given arrays: phi = phi_in + shift, phiOld = phio +
-// shift; pn holds the initial iterate (boundary planes included) and receives
-// the last.
+// shift; pn (and pn2 when fused) hold the initial iterate, boundary planes
+// included; the last iterate ends in pn, or in pn2 after an odd number of
+// fused sweeps.
 template <typename T>
 hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, double shift,
                            FieldView<T> phi, FieldView<T> phio, FieldView<T> conv2, FieldView<T> cprev,
-                           FieldView<T> pn, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res);
+                           FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
+                           double* res, int fused);
 
 // ---- momentum (momentum_allflux.f90, main_ns_hybrid.F90:629-757) ----
 // shift: 0, or 1 for the diffuse interface method, whose viscosity is that
