@@ -444,6 +444,19 @@ def main():
         out["batched_fp32"] = batched_run(a)
     if a.serve_seconds > 0:
         out["serving_fp32"] = serving_run(a)
+    # throughput ratios of the batched FP32 run against the FP64 single case
+    # (multigrid CG) and against the plain FP64 path (plain CG, the
+    # reference's -pc_type none configuration)
+    ratios = {}
+    if "batched_fp32" in out:
+        bv = out["batched_fp32"]["value"]
+        ratios["batched_fp32_vs_fp64_mg"] = round(bv / out["value"], 3)
+        if plain:
+            ratios["batched_fp32_vs_plain_fp64_cg"] = round(bv / plain["value"], 2)
+    if "fp32" in out:
+        ratios["fp32_vs_fp64_mg"] = round(out["fp32"]["value"] / out["value"], 3)
+    if ratios:
+        out["ratios"] = ratios
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)
