@@ -1,10 +1,11 @@
 // WENO5 level-set advection: weno.f90:18-215 and advect_phi.f90:19-119.
 //
 // The fused kernel computes conv2_phi = div(u * phi_face) on every p cell in
-// one pass: the six face values a cell needs are reconstructed in registers
-// from phi2 (read with periodic y wrap, which is what the reference's
-// phi_augaug plus exchange2d_augaug1/2/3 provide) instead of being written to
-// and re-read from dfx/dfy/dfz arrays.  HBM traffic per cell: phi2, u2, v2,
+// one pass: the face values a cell needs are reconstructed in registers from
+// phi2 (read with periodic y wrap, which is what the reference's phi_augaug
+// plus exchange2d_augaug1/2/3 provide) and shared between the neighbouring
+// cells of a block, instead of being written to and re-read from dfx/dfy/dfz
+// arrays.  HBM traffic per cell: phi2, u2, v2,
 // w2 read once, conv2_phi written once (40 B in FP64).
 #include "tpls_kernels.hpp"
 
@@ -43,28 +44,27 @@ __device__ __forceinline__ C weno5f(C v1, C v2, C v3, C v4, C v5)
 // Face value on the u face I (between p cells I and I+1) at p row pj, level
 // pk; uu = u at that face.  weno.f90:45-76 (WENO for 3 <= I <= maxl-3,
 // first-order upwind otherwise, which also covers the inlet face of :86-100).
+// The upwind direction selects the five stencil points and one WENO5
+// evaluation follows, so lanes of mixed sign do not run both.
 template <typename C, typename T>
 __device__ __forceinline__ C face_x(const FieldView<T>& phi, int I, int pj, int pk, C uu, int maxl)
 {
+    const bool up = uu >= C(0);
     if (I >= 3 && I <= maxl - 3) {
-        if (uu >= C(0))
-            return weno5f<C>(phi(I - 2, pj, pk), phi(I - 1, pj, pk), phi(I, pj, pk), phi(I + 1, pj, pk),
-                             phi(I + 2, pj, pk));
-        return weno5f<C>(phi(I + 3, pj, pk), phi(I + 2, pj, pk), phi(I + 1, pj, pk), phi(I, pj, pk),
-                         phi(I - 1, pj, pk));
+        const int s = up ? 1 : -1, b = up ? I : I + 1;
+        return weno5f<C>(phi(b - 2 * s, pj, pk), phi(b - s, pj, pk), phi(b, pj, pk), phi(b + s, pj, pk),
+                         phi(b + 2 * s, pj, pk));
     }
-    return uu >= C(0) ? C(phi(I, pj, pk)) : C(phi(I + 1, pj, pk));
+    return C(phi(up ? I : I + 1, pj, pk));
 }
 
 // v face J (between p rows J and J+1), weno.f90:104-133: always WENO.
 template <typename C, typename T>
 __device__ __forceinline__ C face_y(const FieldView<T>& phi, int I, int J, int pk, C vv, int P)
 {
-    if (vv >= C(0))
-        return weno5f<C>(phi(I, wrapj(J - 2, P), pk), phi(I, wrapj(J - 1, P), pk), phi(I, wrapj(J, P), pk),
-                         phi(I, wrapj(J + 1, P), pk), phi(I, wrapj(J + 2, P), pk));
-    return weno5f<C>(phi(I, wrapj(J + 3, P), pk), phi(I, wrapj(J + 2, P), pk), phi(I, wrapj(J + 1, P), pk),
-                     phi(I, wrapj(J, P), pk), phi(I, wrapj(J - 1, P), pk));
+    const int s = vv >= C(0) ? 1 : -1, b = vv >= C(0) ? J : J + 1;
+    return weno5f<C>(phi(I, wrapj(b - 2 * s, P), pk), phi(I, wrapj(b - s, P), pk), phi(I, wrapj(b, P), pk),
+                     phi(I, wrapj(b + s, P), pk), phi(I, wrapj(b + 2 * s, P), pk));
 }
 
 // w face K (between p levels K and K+1), weno.f90:135-167: WENO for
@@ -73,37 +73,62 @@ template <typename C, typename T>
 __device__ __forceinline__ C face_z(const FieldView<T>& phi, int I, int pj, int K, C ww, int maxn)
 {
     if (K >= 3 && K <= maxn - 3) {
-        if (ww >= C(0))
-            return weno5f<C>(phi(I, pj, K - 2), phi(I, pj, K - 1), phi(I, pj, K), phi(I, pj, K + 1),
-                             phi(I, pj, K + 2));
-        return weno5f<C>(phi(I, pj, K + 3), phi(I, pj, K + 2), phi(I, pj, K + 1), phi(I, pj, K),
-                         phi(I, pj, K - 1));
+        const int s = ww >= C(0) ? 1 : -1, b = ww >= C(0) ? K : K + 1;
+        return weno5f<C>(phi(I, pj, b - 2 * s), phi(I, pj, b - s), phi(I, pj, b), phi(I, pj, b + s),
+                         phi(I, pj, b + 2 * s));
     }
     return (C(phi(I, pj, K)) + C(phi(I, pj, K + 1))) / C(2);
 }
 
+// Levels per thread of the advection kernel: the top face of one level is the
+// bottom face of the next.
+constexpr int ADV_KZ = 4;
+
+// conv2_phi of ADV_KZ cells stacked in k.  Each face is reconstructed once per
+// thread column: the west face is the east face of the lane to the left
+// (shuffle), the south face the north face of the row below (LDS), the bottom
+// face the previous level's top face; the first lane, the first row and the
+// first level compute their own.  Lanes past the grid's edge work on clamped
+// indices and store nothing, so every lane takes part in the exchanges.
 template <typename T>
 __global__ void __launch_bounds__(BX* BY) k_advect_phi(Grid g, T dx, T dy, T dz, FieldView<T> phi,
                                                        FieldView<T> u, FieldView<T> v, FieldView<T> w,
                                                        FieldView<T> conv)
 {
     using C = T;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.maxl - 1 || j > g.P) return;
-    const int jm = wrapj(j - 1, g.P);
-    // advect_phi.f90:330-334
-    C ue = u(i, jm, k - 1), uw = u(i - 1, jm, k - 1);
-    C vn = v(i, j, k - 1), vs = v(i, jm, k - 1);
-    C wt = w(i, jm, k), wb = w(i, jm, k - 1);
-    C fe = face_x<C>(phi, i, j, k, ue, g.maxl);
-    C fw = face_x<C>(phi, i - 1, j, k, uw, g.maxl);
-    C fn = face_y<C>(phi, i, j, k, vn, g.P);
-    C fs = face_y<C>(phi, i, j - 1, k, vs, g.P);
-    C ft = face_z<C>(phi, i, j, k, wt, g.maxn);
-    C fb = face_z<C>(phi, i, j, k - 1, wb, g.maxn);
-    conv(i, j, k) = (ue * fe - uw * fw) / dx + (vn * fn - vs * fs) / dy + (wt * ft - wb * fb) / dz;
+    __shared__ C fn_s[BY][BX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * BX + tx + 1;
+    const int j = blockIdx.y * BY + ty + 1;
+    const int k0 = blockIdx.z * ADV_KZ + 1;
+    const int ic = min(i, g.maxl - 1), jc = min(j, g.P);
+    const bool col = i <= g.maxl - 1 && j <= g.P;
+    const int jm = wrapj(jc - 1, g.P);
+    C ft = C(0), wt = C(0);
+#pragma unroll
+    for (int kk = 0; kk < ADV_KZ; ++kk) {
+        const int k = min(k0 + kk, g.maxn - 1);
+        // advect_phi.f90:330-334
+        const C ue = u(ic, jm, k - 1);
+        const C vn = v(ic, jc, k - 1), vs = v(ic, jm, k - 1);
+        const C wb = kk ? wt : C(w(ic, jm, k - 1));
+        wt = w(ic, jm, k);
+        const C fe = face_x<C>(phi, ic, jc, k, ue, g.maxl);
+        C uw = __shfl_up(ue, 1, 64), fw = __shfl_up(fe, 1, 64);
+        if (tx == 0) {
+            uw = u(ic - 1, jm, k - 1);
+            fw = face_x<C>(phi, ic - 1, jc, k, uw, g.maxl);
+        }
+        const C fn = face_y<C>(phi, ic, jc, k, vn, g.P);
+        fn_s[ty][tx] = fn;
+        __syncthreads();
+        const C fs = ty ? fn_s[ty - 1][tx] : face_y<C>(phi, ic, jc - 1, k, vs, g.P);
+        __syncthreads();
+        const C fb = kk ? ft : face_z<C>(phi, ic, jc, k - 1, wb, g.maxn);
+        ft = face_z<C>(phi, ic, jc, k, wt, g.maxn);
+        if (col && k0 + kk <= g.maxn - 1)
+            conv(i, j, k) = (ue * fe - uw * fw) / dx + (vn * fn - vs * fs) / dy + (wt * ft - wb * fb) / dz;
+    }
 }
 
 // WENOAppoxConvection as a standalone routine: dfx on I in 0..maxl-1, dfy on
@@ -131,7 +156,7 @@ hipError_t launch_advect_phi(hipStream_t s, const Grid& g, const Phys& ph, Field
                              FieldView<T> u2, FieldView<T> v2, FieldView<T> w2, FieldView<T> conv2_phi)
 {
     dim3 blk(BX, BY, 1);
-    dim3 grd((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, g.maxn - 1);
+    dim3 grd((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, (g.maxn - 1 + ADV_KZ - 1) / ADV_KZ);
     hipLaunchKernelGGL(k_advect_phi<T>, grd, blk, 0, s, g, T(ph.dx), T(ph.dy), T(ph.dz), phi2, u2, v2, w2,
                        conv2_phi);
     return hipGetLastError();
