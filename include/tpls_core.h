@@ -167,7 +167,8 @@ int tpls_case_set_tiling(tpls_case *c, int rows, int rows_odd, int prefetch_odd,
                          int rows_jst, int sor_fused);
 /* Liquid volume V = sum over the interior cells of H_eps(phi2) dx dy dz,
  * accumulated in FP64, with the smoothed Heaviside of get_viscosity
- * (momentum_allflux.f90:499-512).  The reference has no mass diagnostic; this
+ * (momentum_allflux.f90:499-512); of phi2 - 0.5 with idm_dim, as the
+ * viscosity is (main_ns_hybrid.F90:650-651).  The reference has no mass diagnostic; this
  * one measures the volume drift of a run against another (e.g. FP32 storage
  * against FP64). */
 int tpls_case_volume(tpls_case *c, double *volume);

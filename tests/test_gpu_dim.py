@@ -158,6 +158,20 @@ def test_dim_fused_sweep_equals_two_launch_sweep(precision, shape):
     print(f"[dim fused] {precision} {nx}x{ny}x{nz}: 3 steps identical", flush=True)
 
 
+def test_dim_volume_uses_the_shifted_phase():
+    from tpls_amd.case import Case
+    p = CaseParams(nx=32, ny=16, nz=24, seed=4, idm="dim")
+    with Case(p) as c:
+        c.load(*make_initial(p))
+        c.step(2)
+        phi = c.field("phi2")[1:p.maxl, 1:p.maxm, 1:p.maxn] - 0.5
+        v_gpu = c.volume()
+    sw = p.smooth_width
+    pi = float(np.float32(3.1415926535))
+    H = np.where(phi < -sw, 0.0, np.where(phi > sw, 1.0, 0.5 + phi / (2 * sw) + np.sin(pi * phi / sw) / (2 * pi)))
+    assert v_gpu == pytest.approx(H.sum() * p.dx ** 3, rel=1e-12)
+
+
 def test_dim_restored_case_steps_bit_identically(tmp_path):
     from tpls_amd.case import Case
     p = CaseParams(nx=40, ny=12, nz=20, Re=60.0, mu_plus=3.0, dt=1e-4, seed=9, idm="dim", dim_its=10)

@@ -872,9 +872,9 @@ int tpls_case_volume(tpls_case* c, double* volume)
     if (!c || !volume) return fail(TPLS_ERR_ARG, "null argument");
     const Reduce red = Reduce{c->red_part, c->red_out + 6};
     if (c->prec == TPLS_FP32)
-        HIPCHK(launch_volume<float>(c->s, c->g, c->ph, c->phi2->view<float>(), red));
+        HIPCHK(launch_volume<float>(c->s, c->g, c->ph, c->phi2->view<float>(), red, c->prm.idm_dim ? 0.5 : 0.0));
     else
-        HIPCHK(launch_volume<double>(c->s, c->g, c->ph, c->phi2->view<double>(), red));
+        HIPCHK(launch_volume<double>(c->s, c->g, c->ph, c->phi2->view<double>(), red, c->prm.idm_dim ? 0.5 : 0.0));
     HIPCHK(hipMemcpyAsync(c->h_out + 6, c->red_out + 6, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     *volume = c->h_out[6];

@@ -420,7 +420,8 @@ __global__ void __launch_bounds__(256) k_final1(const double* part, int nb, doub
 // with the smoothed Heaviside of the viscosity (momentum_allflux.f90:499-512).
 // The reference has no mass diagnostic; this one measures volume drift.
 template <typename T>
-__global__ void __launch_bounds__(256) k_volume(Grid g, double sw, double dv, FieldView<T> phi, double* part)
+__global__ void __launch_bounds__(256) k_volume(Grid g, double sw, double dv, double shift, FieldView<T> phi,
+                                                double* part)
 {
     const int nx = g.maxl - 1, nz = g.maxn - 1;
     const long n = (long)nx * g.P * nz;
@@ -430,7 +431,7 @@ __global__ void __launch_bounds__(256) k_volume(Grid g, double sw, double dv, Fi
         const long rr = q / nx;
         const int j = int(rr % g.P) + 1;
         const int k = int(rr / g.P) + 1;
-        const double pv = double(phi(i, j, k));
+        const double pv = double(phi(i, j, k)) - shift;
         double H;
         if (pv < -sw)
             H = 0.0;
@@ -543,9 +544,9 @@ hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> 
 }
 
 template <typename T>
-hipError_t launch_volume(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, Reduce red)
+hipError_t launch_volume(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, Reduce red, double shift)
 {
-    hipLaunchKernelGGL(k_volume<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, ph.sw, ph.dx * ph.dy * ph.dz, phi,
+    hipLaunchKernelGGL(k_volume<T>, dim3(REDUCE_BLOCKS), dim3(256), 0, s, g, ph.sw, ph.dx * ph.dy * ph.dz, shift, phi,
                        red.part);
     hipLaunchKernelGGL(k_final1, dim3(1), dim3(256), 0, s, red.part, REDUCE_BLOCKS, red.out);
     return hipGetLastError();
@@ -596,7 +597,7 @@ hipError_t launch_copy(hipStream_t s, T* dst, const T* src, long n)
                                               FieldView<T>, const T*, FieldView<T>);                                \
     template hipError_t launch_levelset_err<T>(hipStream_t, const Grid&, FieldView<T>, FieldView<T>, Reduce);       \
     template hipError_t launch_difference<T>(hipStream_t, const Grid&, int, FieldView<T>, FieldView<T>, Reduce);    \
-    template hipError_t launch_volume<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, Reduce);              \
+    template hipError_t launch_volume<T>(hipStream_t, const Grid&, const Phys&, FieldView<T>, Reduce, double);      \
     template hipError_t launch_fill<T>(hipStream_t, T*, long, T);                                                   \
     template hipError_t launch_copy<T>(hipStream_t, T*, const T*, long);
 TPLS_INST_PR(double)

@@ -136,9 +136,11 @@ hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int
 template <typename T>
 hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b,
                              Reduce red);
-// V = sum of H_eps(phi) dx dy dz over the interior, FP64, into red.out[0]
+// V = sum of H_eps(phi - shift) dx dy dz over the interior, FP64, into
+// red.out[0]; shift 0.5 for the diffuse interface method
 template <typename T>
-hipError_t launch_volume(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, Reduce red);
+hipError_t launch_volume(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi, Reduce red,
+                         double shift);
 
 // ---- pressure (pressure.f90, pressure_solver.F90) ----
 template <typename T>
