@@ -47,6 +47,20 @@ def test_case_from_config_steps_like_case_from_arrays(tmp_path, precision):
             assert np.array_equal(a.field(f), b.field(f)), f
 
 
+def test_dim_case_from_config_steps_like_case_from_arrays(tmp_path):
+    from tpls_amd.case import Case
+    p = CaseParams(nx=32, ny=8, nz=16, Re=80.0, mu_plus=2.0, dt=1e-4, seed=7, idm="dim", dim_its=9)
+    init = write_case(str(tmp_path), p)
+    print("dim case from config, 32x8x16", flush=True)
+    with Case.from_config(tmp_path, tmp_path / "tpls_config.opt") as a, Case(p) as b:
+        assert a.params.idm == "dim" and a.params.dim_its == 9
+        b.load(*init)
+        a.step(3)
+        b.step(3)
+        for f in FIELDS + ("convprev_phi",):
+            assert np.array_equal(a.field(f), b.field(f)), f
+
+
 def test_save_field_reads_back(tmp_path):
     from tpls_amd.case import Case
     p = CaseParams(nx=32, ny=8, nz=16, seed=3)

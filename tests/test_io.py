@@ -593,6 +593,31 @@ def test_written_initial_conditions_read_back(tmp_path):
         assert np.array_equal(bits(a), bits(b))
 
 
+def test_written_dim_configuration_reads_back_into_the_case_params(tmp_path):
+    """An idm dim case written in the generator's layout (pe and epn after
+    idm, create_initial_conditions_program.F90:152-156; max_iteration_dim in
+    the options) reads back into tpls_params' idm_dim, epn, Pe, dim_its, and
+    the phase field is the generator's tanh profile in [0, 1]."""
+    from tpls_amd.config import CaseParams
+    from tpls_amd.initial import make_initial
+    p = CaseParams(nx=16, ny=8, nz=16, seed=2, idm="dim", dim_its=11)
+    fields = make_initial(p)
+    assert 0.0 < fields[4].min() and fields[4].max() < 1.0
+    io.write_initial_conditions(tmp_path, p, fields)
+    io.write_options_file(tmp_path / "tpls_config.opt", p, n_timesteps=2)
+    keys = [l.split()[0] for l in open(tmp_path / io.INITIAL_CONFIG_FILE) if l.strip() and not l.startswith("#")]
+    assert keys[-3:] == ["idm", "pe", "epn"]
+    prm, run = io.read_configuration(tmp_path, tmp_path / "tpls_config.opt")
+    assert run.idm_dim == 1 and prm.idm_dim == 1 and prm.dim_its == run.max_iteration_dim == 11
+    # F20.10: epn = dz/2 = 1/32 and Pe = 1024 are exact
+    assert prm.epn == p.dim_epn == 1 / 32 and prm.Pe == p.dim_Pe == 1024.0
+    lsm = CaseParams(nx=16, ny=8, nz=16, seed=2)
+    io.write_initial_conditions(tmp_path, lsm, make_initial(lsm))
+    io.write_options_file(tmp_path / "tpls_config.opt", lsm, n_timesteps=2)
+    prm, run = io.read_configuration(tmp_path, tmp_path / "tpls_config.opt")
+    assert prm.idm_dim == 0 and prm.dim_its == 0
+
+
 # ---------------------------------------------------------------------------
 # channel outputs and the text backup (io.F90:171-791)
 

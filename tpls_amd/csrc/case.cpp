@@ -1147,14 +1147,14 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
         // ---- do_tpls (main:596) ----
         LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
         LCHK(launch_phi_bc<T>(s, g, ph.dz, ph.height, V(c->phi3), 1));
-        LCHK(launch_levelset_status<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg)));
+        LCHK(launch_levelset_status_code<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>()));
         {
             DevArray* src = c->phi3;
             DevArray* bufs[2] = {c->pra, c->prb};
             const int nit = c->prm.levelset_its;
             for (int it = 1; it <= nit; ++it) {
                 DevArray* dst = bufs[(it - 1) & 1];
-                LCHK(launch_levelset_iter<T>(s, g, ph.dx, V(c->phi3), V(c->D), c->ist->view<int>(), V(c->sg), V(src), V(dst)));
+                LCHK(launch_levelset_iter_code<T>(s, g, ph.dx, V(c->D), c->ist->view<int>(), V(src), V(dst)));
                 // levelset.f90:168-216 (h = dx is passed as dz there)
                 LCHK(launch_phi_bc<T>(s, g, ph.dx, ph.height, V(dst), 1));
                 if (dg && it == nit) LCHK(launch_levelset_err<T>(s, g, V(src), V(dst), red_of(c, 0)));

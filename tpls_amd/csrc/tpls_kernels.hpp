@@ -46,6 +46,14 @@ hipError_t launch_levelset_iter(hipStream_t s, const Grid& g, double h, FieldVie
 template <typename T>
 hipError_t launch_levelset_err(hipStream_t s, const Grid& g, FieldView<T> a, FieldView<T> b,
                                Reduce red);
+// The status and iteration with the cell state packed into one int per cell
+// (ist, the sign of phi and the iteration's branch); same values, less traffic.
+template <typename T>
+hipError_t launch_levelset_status_code(hipStream_t s, const Grid& g, double h, FieldView<T> phi,
+                                       FieldView<T> D, FieldView<int> code);
+template <typename T>
+hipError_t launch_levelset_iter_code(hipStream_t s, const Grid& g, double h, FieldView<T> D,
+                                     FieldView<int> code, FieldView<T> pr, FieldView<T> pr_new);
 template <typename T>
 hipError_t launch_csf(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi,
                       FieldView<T> fx, FieldView<T> fy, FieldView<T> fz);
