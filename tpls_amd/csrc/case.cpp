@@ -572,11 +572,10 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
                 DevArray* ax;
                 if ((rc = alloc_field(c, &ax, sl, 4))) return bail(rc);
                 o.x = ax->view<float>();
-                if (l) {
-                    DevArray* ab;
-                    if ((rc = alloc_field(c, &ab, sl, 4))) return bail(rc);
-                    o.b = ab->view<float>();
-                }
+                // level 0: the CG residual converted once per cycle
+                DevArray* ab;
+                if ((rc = alloc_field(c, &ab, sl, 4))) return bail(rc);
+                o.b = ab->view<float>();
             }
         } else {
             if ((rc = alloc_field(c, &c->cg_z, sp, 8))) return bail(rc);

@@ -114,9 +114,11 @@ __device__ __forceinline__ bool mg_cell(const L& o, int& i, int& j, int& k)
 
 // first pre-smoothing pair (red then black) from x = 0.  One thread per two
 // cells along x, one of each colour (nx is even on every smoothed level), so
-// no lane waits on the other colour's branch.
+// no lane waits on the other colour's branch.  bc (when bc.p is set) receives
+// b converted to the cycle's type, which is all the later kernels of the
+// level use of it.
 template <typename X, typename B>
-__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, const CgState* st)
+__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X> bc, const CgState* st)
 {
     using C = X;
     if (st->done) return;
@@ -128,6 +130,10 @@ __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, const CgSta
     auto red = [&](int ii, int jj, int kk) { return (C(o.b(ii, jj, kk)) + C(0)) / mg_deff<C>(o, ii, kk); };
     o.x(ir, j, k) = red(ir, j, k);
     o.x(ib, j, k) = (C(o.b(ib, j, k)) + mg_nsum<C>(o, red, ib, j, k)) / mg_deff<C>(o, ib, k);
+    if (bc.p) {
+        bc(ir, j, k) = C(o.b(ir, j, k));
+        bc(ib, j, k) = C(o.b(ib, j, k));
+    }
 }
 
 template <typename X, typename B>
@@ -518,13 +524,15 @@ static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny 
 // FP64 CG residual) and whose levels 1.. are `lev[1..]` (all X).
 template <typename X>
 static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>& top, const MgLevelT<X, X>* lev,
-                         int nlev, int nu, const double* spec, Reduce red, CgState* st)
+                         int nlev, int nu, const double* spec, Reduce red, CgState* st,
+                         const MgLevelT<X, X>* topx = nullptr)
 {
     const dim3 blk(BX, BY);
     const int last = nlev - 1;
+    const FieldView<X> nobc{nullptr, 0, 0};
     auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
         using B = std::decay_t<decltype(*o.b.p)>;
-        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, st);
+        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
         for (int q = 1; q < nu; ++q) {
             hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
             hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 1, st);
@@ -545,7 +553,23 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
             if (!finest || q < nu - 1) hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
         }
     };
-    if (last > 0) down(top, lev[1]);
+    if (last > 0 && topx) {
+        // the finest level with b converted once by its first pass
+        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top), blk, 0, s, top, topx->b, st);
+        const MgLevelT<X, X>& o = *topx;
+        const MgLevelT<X, X>& oc = lev[1];
+        for (int q = 1; q < nu; ++q) {
+            hipLaunchKernelGGL((k_mg_smooth<X, X>), level_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, X>), level_grid(o), blk, 0, s, o, 1, st);
+        }
+        const int ntx = (oc.nx + RR_CX - 1) / RR_CX, nty = (oc.ny + RR_CY - 1) / RR_CY;
+        int rk = 8;
+        while (rk > 1 && long(ntx) * nty * ((oc.nz + rk - 1) / rk) < 1024) rk /= 2;
+        const int ntz = (oc.nz + rk - 1) / rk;
+        hipLaunchKernelGGL((k_mg_rr<X, X, X>), dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
+    } else if (last > 0) {
+        down(top, lev[1]);
+    }
     for (int l = 1; l < last; ++l) down(lev[l], lev[l + 1]);
     if (last > 0) {
         const MgLevelT<X, X>& c = lev[last];
@@ -568,7 +592,10 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         }
     }
     for (int l = last - 1; l >= 1; --l) up(lev[l], lev[l + 1], false);
-    if (last > 0) up(top, lev[1], true);
+    if (last > 0 && topx)
+        up(*topx, lev[1], true);
+    else if (last > 0)
+        up(top, lev[1], true);
     const Tiles tl = cg_tiles(g);
     hipLaunchKernelGGL((k_mg_last<X>), dim3(tl.nblocks), blk, 0, s, top, tl, last > 0 ? 1 : 0, red.part, st);
     const double N = double(top.nx) * top.ny * top.nz;
@@ -587,7 +614,9 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     }
     const MgLevel& o = h.L[0];
     const MgLevelT<float, double> top{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
-    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st);
+    // F[0].b: the FP64 residual in FP32, written by the first pass
+    const MgLevel32 topx{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
+    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr);
 }
 
 // 1-D eigenbases.  Neumann (cell-centred, ends coupled to themselves): mode k
