@@ -14,6 +14,7 @@ program drive_modules
   use advect_phi
   use levelset
   use two_phase_levelset
+  use cahn_hilliard_solver
   use momentum_allflux
   use jacobi_iteration_allflux
   use sor_iteration_allflux
@@ -127,6 +128,12 @@ program drive_modules
     call do_tpls(p1, p2, p3, p4, p5, 1, maxl - 1, 1, maxm - 1, 0, maxn, maxn, maxl - 1, -1, -1, &
         i1, nb, -1, d1, s1, d2, d3, d4)
     write(fout) p2, s1
+  case (18)   ! dim(maxl,maxn,dx,dy,dz,dt,epn,Pe,phi,phiOld,conv2_phi,ConvPrevPhi,phiNew, bounds, height, strides, MPI args, its)
+    call get_p(p1); call get_p(p2); call get_p(p3); call get_p(p4); call get_p(p5)
+    read(fin) i1, d1, d2, d3, d4, d5
+    call dim(maxl, maxn, d1, d1, d1, d2, d3, d4, p1, p2, p3, p4, p5, 1, maxl - 1, 1, maxm - 1, 0, maxn, &
+        d5, -1, -1, nb, -1, i1)
+    write(fout) p4, p5
   case default
     write(0, *) 'drive_modules: unknown routine id', rid
     error stop 2

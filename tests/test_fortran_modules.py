@@ -37,7 +37,7 @@ MODULES = {"weno": "weno.f90", "advect_phi": "advect_phi.f90", "levelset": "leve
 RID = {"weno_appox_convection": 1, "get_levelset_status": 2, "do_levelset_iteration": 3, "get_csf": 4,
        "get_viscosity": 5, "get_conv_all": 6, "get_diffusion_all": 7, "do_jacobi_u": 8, "do_jacobi_v": 9,
        "do_jacobi_w": 10, "do_sor_u": 11, "do_sor_v": 12, "do_sor_w": 13, "get_source_pres": 14,
-       "get_uvw_pres": 15, "do_advect_phi": 16, "do_tpls": 17}
+       "get_uvw_pres": 15, "do_advect_phi": 16, "do_tpls": 17, "dim": 18}
 
 
 def subroutine_args(text):
@@ -155,6 +155,31 @@ def test_modules_advect_phi_and_tpls(tmp_path):
                           [(S["p"], np.float64), ((), np.float64)])
     assert relerr(p3g[:, 1:MAXM], p3o[:, 1:MAXM]) < 1e-12
     assert eg == pytest.approx(eo, rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_module_dim(tmp_path):
+    """cahn_hilliard_solver::dim through the Fortran module (reference
+    argument list, MPI arguments ignored) against the oracle, bit for bit."""
+    if not os.access(DRIVER, os.X_OK):
+        pytest.fail("tests/fortran/drive_modules is not built")
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(37)
+    k = np.arange(MAXN + 1)[None, None, :]
+    base = 0.5 + 0.5 * np.tanh(((k - 0.5) * DX - 0.5) / (2 * np.sqrt(2) * 0.5 * DX))
+    phi = np.asfortranarray(base + 0.02 * r.standard_normal(S["p"]) + 0.5)
+    phi[:, 0], phi[:, MAXM] = phi[:, MAXM - 1], phi[:, 1]
+    conv2, cprev = rand(r, S["p"], scale=0.1), rand(r, S["p"], scale=0.1)
+    pnew = np.asfortranarray(np.clip(phi - 0.5, 0, 1))
+    dt, epn, Pe, height, its = 2e-4, 0.5 * DX, 1.0 / (0.5 * DX) ** 2, 0.45, 6
+    co, no = cprev.copy(order="F"), pnew.copy(order="F")
+    B.oracle("dim", MAXL, MAXN, DX, DX, DX, dt, epn, Pe, phi, phi, conv2, co, no, 1, MAXL - 1, 1, MAXM - 1,
+             height, its)
+    print(f"[fortran] dim via module cahn_hilliard_solver, {its} sweeps", flush=True)
+    cg, ng = _run_driver(tmp_path, 18, MAXL, MAXM, MAXN, [phi, phi, conv2, cprev, pnew, its, DX, dt, epn, Pe, height],
+                         [(S["p"], np.float64), (S["p"], np.float64)])
+    assert np.array_equal(cg, co)
+    assert np.array_equal(ng[:, 1:MAXM], no[:, 1:MAXM])
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent")
