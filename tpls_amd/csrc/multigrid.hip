@@ -103,15 +103,6 @@ __device__ __forceinline__ int mg_nbr(int f, int nc, bool periodic)
     return n;
 }
 
-template <typename L>
-__device__ __forceinline__ bool mg_cell(const L& o, int& i, int& j, int& k)
-{
-    i = blockIdx.x * BX + threadIdx.x + 1;
-    j = blockIdx.y * BY + threadIdx.y + 1;
-    k = blockIdx.z + 1;
-    return i <= o.nx && j <= o.ny;
-}
-
 // first pre-smoothing pair (red then black) from x = 0.  One thread per two
 // cells along x, one of each colour (nx is even on every smoothed level), so
 // no lane waits on the other colour's branch.  bc (when bc.p is set) receives
@@ -136,14 +127,18 @@ __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X
     }
 }
 
+// One colour sweep.  A thread owns the pair (i0, i0+1) along x and updates
+// the one of the sweep's colour (nx is even on every smoothed level), so
+// every lane of a wavefront has a cell to update.
 template <typename X, typename B>
 __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevelT<X, B> o, int colour, const CgState* st)
 {
     using C = X;
     if (st->done) return;
-    int i, j, k;
-    if (!mg_cell(o, i, j, k)) return;
-    if (((i + j + k) & 1) != colour) return;
+    const int i0 = 2 * (blockIdx.x * BX + threadIdx.x) + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    if (i0 > o.nx || j > o.ny) return;
+    const int i = ((i0 + j + k) & 1) == colour ? i0 : i0 + 1;
     o.x(i, j, k) = (C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k)) / mg_deff<C>(o, i, k);
 }
 
@@ -529,8 +524,6 @@ int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag
 }
 
 template <typename L>
-static dim3 level_grid(const L& o) { return dim3((o.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
-template <typename L>
 static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
 
 // The V-cycle on a hierarchy whose level 0 is `top` (x stored as X, b the
@@ -547,8 +540,8 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         using B = std::decay_t<decltype(*o.b.p)>;
         hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
         for (int q = 1; q < nu; ++q) {
-            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
-            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 1, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1, st);
         }
         // coarse levels per block: as many as keep >= 1024 blocks, up to 8
         // (each block recomputes the two fine planes below its first level)
@@ -563,8 +556,8 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         hipLaunchKernelGGL((k_mg_prolong<X, B, X>), dim3((oc.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz), blk, 0,
                            s, o, oc, st);
         for (int q = 0; q < nu; ++q) {
-            hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 1, st);
-            if (!finest || q < nu - 1) hipLaunchKernelGGL((k_mg_smooth<X, B>), level_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1, st);
+            if (!finest || q < nu - 1) hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
         }
     };
     if (last > 0 && topx) {
@@ -573,8 +566,8 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         const MgLevelT<X, X>& o = *topx;
         const MgLevelT<X, X>& oc = lev[1];
         for (int q = 1; q < nu; ++q) {
-            hipLaunchKernelGGL((k_mg_smooth<X, X>), level_grid(o), blk, 0, s, o, 0, st);
-            hipLaunchKernelGGL((k_mg_smooth<X, X>), level_grid(o), blk, 0, s, o, 1, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, X>), pair_grid(o), blk, 0, s, o, 0, st);
+            hipLaunchKernelGGL((k_mg_smooth<X, X>), pair_grid(o), blk, 0, s, o, 1, st);
         }
         const int ntx = (oc.nx + RR_CX - 1) / RR_CX, nty = (oc.ny + RR_CY - 1) / RR_CY;
         int rk = 8;
