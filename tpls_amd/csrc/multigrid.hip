@@ -65,7 +65,8 @@ namespace tpls {
 
 // Arithmetic of a level is done in C, the storage type of its x (double for
 // the FP64 hierarchy, which is the oracle's operation for operation; float
-// for the FP32 one).
+// for the FP32 one, whose divisions by the diagonal use the hardware
+// reciprocal: sdiv).
 template <typename C, typename L>
 __host__ __device__ __forceinline__ C mg_deff(const L& o, int i, int k)
 {
@@ -118,9 +119,9 @@ __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X
     if (i0 > o.nx || j > o.ny) return;
     const int ir = ((i0 + j + k) & 1) == 0 ? i0 : i0 + 1;
     const int ib = ir == i0 ? i0 + 1 : i0;
-    auto red = [&](int ii, int jj, int kk) { return (C(o.b(ii, jj, kk)) + C(0)) / mg_deff<C>(o, ii, kk); };
+    auto red = [&](int ii, int jj, int kk) { return sdiv(C(o.b(ii, jj, kk)) + C(0), mg_deff<C>(o, ii, kk)); };
     o.x(ir, j, k) = red(ir, j, k);
-    o.x(ib, j, k) = (C(o.b(ib, j, k)) + mg_nsum<C>(o, red, ib, j, k)) / mg_deff<C>(o, ib, k);
+    o.x(ib, j, k) = sdiv(C(o.b(ib, j, k)) + mg_nsum<C>(o, red, ib, j, k), mg_deff<C>(o, ib, k));
     if (bc.p) {
         bc(ir, j, k) = C(o.b(ir, j, k));
         bc(ib, j, k) = C(o.b(ib, j, k));
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevelT<X, B> o, int colo
     const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
     if (i0 > o.nx || j > o.ny) return;
     const int i = ((i0 + j + k) & 1) == colour ? i0 : i0 + 1;
-    o.x(i, j, k) = (C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k)) / mg_deff<C>(o, i, k);
+    o.x(i, j, k) = sdiv(C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
 }
 
 // Weights of the fine cells 2I-2+q (q = 0..3) feeding coarse cell I under the
@@ -462,7 +463,7 @@ __global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevelT<X, double> o, Tiles
             const double bv = o.b(i, j, k);
             X z;
             if (update && ((i + j + k) & 1) == 0) {
-                z = (C(bv) + mg_nsum<C>(o, o.x, i, j, k)) / mg_deff<C>(o, i, k);
+                z = sdiv(C(bv) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
                 o.x(i, j, k) = z;
             } else {
                 z = o.x(i, j, k);
