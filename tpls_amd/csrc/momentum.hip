@@ -21,10 +21,36 @@ __global__ void __launch_bounds__(BX* BY) k_viscosity(Grid g, T sw, T Re, T mum,
     mu(i, j, k) = visc_of_phi<T>(pv, sw, Re, mum, mup);
 }
 
+// A grid spacing as the stencils divide by it: a / h is the correctly rounded
+// division in FP64 (the reference's operation) and a times the hardware
+// reciprocal of h (within an ulp) in FP32; 2 * h is a spacing too.
+template <typename C>
+struct Sp {
+    C h, r;
+};
+template <typename C>
+__device__ __forceinline__ Sp<C> spacing(C h)
+{
+    return Sp<C>{h, recip(h)};
+}
+template <typename C>
+__device__ __forceinline__ C operator/(C a, Sp<C> s)
+{
+    if constexpr (sizeof(C) == 4)
+        return a * s.r;
+    else
+        return a / s.h;
+}
+template <typename C>
+__device__ __forceinline__ Sp<C> operator*(C f, Sp<C> s)
+{
+    return spacing(f * s.h);
+}
+
 // ---- convective terms, momentum_allflux.f90:281-337 / 350-402 / 415-463 ----
 template <typename C, typename T>
 __device__ __forceinline__ C conv_u(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
-                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, C dx, C dy, C dz)
+                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, Sp<C> dx, Sp<C> dy, Sp<C> dz)
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
     const int j1 = jp, j2 = wrapj(j + 2, P), j0 = j;
@@ -61,7 +87,7 @@ __device__ __forceinline__ C conv_u(const FieldView<T>& u2, const FieldView<T>& 
 
 template <typename C, typename T>
 __device__ __forceinline__ C conv_v(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
-                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, C dx, C dy, C dz)
+                                    const FieldView<T>& mu, int i, int j, int k, int maxn, int P, Sp<C> dx, Sp<C> dy, Sp<C> dz)
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P);
     C tu = (C(u2(i - 1, jm, k)) + C(u2(i - 1, j, k)) + C(u2(i, jm, k)) + C(u2(i, j, k))) * C(0.25);
@@ -98,7 +124,7 @@ __device__ __forceinline__ C conv_v(const FieldView<T>& u2, const FieldView<T>& 
 
 template <typename C, typename T>
 __device__ __forceinline__ C conv_w(const FieldView<T>& u2, const FieldView<T>& v2, const FieldView<T>& w2,
-                                    const FieldView<T>& mu, int i, int j, int k, int P, C dx, C dy, C dz)
+                                    const FieldView<T>& mu, int i, int j, int k, int P, Sp<C> dx, Sp<C> dy, Sp<C> dz)
 {
     const int jp = wrapj(j + 1, P), jm = wrapj(j - 1, P), j2 = wrapj(j + 2, P);
     C tu = (C(u2(i - 1, j, k - 1)) + C(u2(i - 1, j, k)) + C(u2(i, j, k - 1)) + C(u2(i, j, k))) * C(0.25);
@@ -130,7 +156,7 @@ __device__ __forceinline__ C conv_w(const FieldView<T>& u2, const FieldView<T>& 
 // ---- diffusive terms, momentum_allflux.f90:70-110 / 121-164 / 175-206 ----
 template <typename C, typename T>
 __device__ __forceinline__ C diff_uv(const FieldView<T>& x2, const MuFaces<C>& m, int i, int j, int k, int maxn,
-                                     int P, C dx, C dy, C dz)
+                                     int P, Sp<C> dx, Sp<C> dy, Sp<C> dz)
 {
     C c = x2(i, j, k);
     C xm = (k == 0) ? C(-2) * c + C(x2(i, j, 1)) / C(3) : C(x2(i, j, k - 1));
@@ -145,8 +171,8 @@ __device__ __forceinline__ C diff_uv(const FieldView<T>& x2, const MuFaces<C>& m
 }
 
 template <typename C, typename T>
-__device__ __forceinline__ C diff_w(const FieldView<T>& w2, const MuFaces<C>& m, int i, int j, int k, int P, C dx,
-                                    C dy, C dz)
+__device__ __forceinline__ C diff_w(const FieldView<T>& w2, const MuFaces<C>& m, int i, int j, int k, int P, Sp<C> dx,
+                                    Sp<C> dy, Sp<C> dz)
 {
     C c = w2(i, j, k);
     C xE = m.xp * ((C(w2(i + 1, j, k)) - c) / dx);
@@ -169,6 +195,7 @@ __global__ void __launch_bounds__(BX* BY) k_momentum(Grid g, int comp, int mode,
                                                      FieldView<T> rhs)
 {
     using C = T;
+    const Sp<C> sx = spacing(dx), sy = spacing(dy), sz = spacing(dz);
     const int i = blockIdx.x * BX + threadIdx.x + 1;
     const int j = blockIdx.y * BY + threadIdx.y + 1;
     const int k = blockIdx.z;
@@ -181,14 +208,14 @@ __global__ void __launch_bounds__(BX* BY) k_momentum(Grid g, int comp, int mode,
     C cv = C(0), df = C(0);
     if (in_range) {
         if (comp == 0) {
-            if (mode & 1 || mode & 4) cv = conv_u<C>(u2, v2, w2, mu, i, j, k, maxn, P, dx, dy, dz);
-            if (mode & 2 || mode & 4) df = diff_uv<C>(u2, mu_faces_u<C>(mu, i, j, k, P), i, j, k, maxn, P, dx, dy, dz);
+            if (mode & 1 || mode & 4) cv = conv_u<C>(u2, v2, w2, mu, i, j, k, maxn, P, sx, sy, sz);
+            if (mode & 2 || mode & 4) df = diff_uv<C>(u2, mu_faces_u<C>(mu, i, j, k, P), i, j, k, maxn, P, sx, sy, sz);
         } else if (comp == 1) {
-            if (mode & 1 || mode & 4) cv = conv_v<C>(u2, v2, w2, mu, i, j, k, maxn, P, dx, dy, dz);
-            if (mode & 2 || mode & 4) df = diff_uv<C>(v2, mu_faces_v<C>(mu, i, j, k, P), i, j, k, maxn, P, dx, dy, dz);
+            if (mode & 1 || mode & 4) cv = conv_v<C>(u2, v2, w2, mu, i, j, k, maxn, P, sx, sy, sz);
+            if (mode & 2 || mode & 4) df = diff_uv<C>(v2, mu_faces_v<C>(mu, i, j, k, P), i, j, k, maxn, P, sx, sy, sz);
         } else {
-            if (mode & 1 || mode & 4) cv = conv_w<C>(u2, v2, w2, mu, i, j, k, P, dx, dy, dz);
-            if (mode & 2 || mode & 4) df = diff_w<C>(w2, mu_faces_w<C>(mu, i, j, k, P), i, j, k, P, dx, dy, dz);
+            if (mode & 1 || mode & 4) cv = conv_w<C>(u2, v2, w2, mu, i, j, k, P, sx, sy, sz);
+            if (mode & 2 || mode & 4) df = diff_w<C>(w2, mu_faces_w<C>(mu, i, j, k, P), i, j, k, P, sx, sy, sz);
         }
     }
     if (mode & 1) conv2(i, j, k) = cv;
