@@ -26,6 +26,7 @@
 // evaluation of the reference loops.
 #include <cmath>
 
+#include "tiles.hpp"
 #include "tpls_kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -378,14 +379,15 @@ hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const D
     hipLaunchKernelGGL(k_dim_source<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(c.epn), T(c.epn2), T(c.Pe),
                        T(c.c_tao), T(c.c_epn), h, phi, d2, conv2, cprev, src);
     const int nbx = (g.maxl - 1 + DIM_TI - 1) / DIM_TI, nby = (g.P + DIM_TJ - 1) / DIM_TJ;
-    const int nbz = (g.maxn - 1 + DIM_KS - 1) / DIM_KS;
+    static const int slots = resident_blocks(k_dim_sweep<T, DIM_TJ>, DIM_TI * DIM_TJ);
+    const int ks = march_chunk(nbx * nby, g.maxn - 1, slots, DIM_KS), nbz = (g.maxn - 1 + ks - 1) / ks;
     for (int it = 1; it <= its; ++it) {
         unsigned long long* rm = res && it == its ? reinterpret_cast<unsigned long long*>(res) : nullptr;
         if (fused) {
             // odd sweeps read pn and write pn2, even sweeps the reverse
             FieldView<T> a = (it & 1) ? pn : pn2, b = (it & 1) ? pn2 : pn;
             hipLaunchKernelGGL((k_dim_sweep<T, DIM_TJ>), dim3(nbx * nby * nbz), dim3(DIM_TI, DIM_TJ), 0, s, g, nbx,
-                               nby, DIM_KS, dx, dy, dz, T(c.dt), T(c.c_rel), T(c.tao), T(c.epn2), T(c.alfa1),
+                               nby, ks, dx, dy, dz, T(c.dt), T(c.c_rel), T(c.tao), T(c.epn2), T(c.alfa1),
                                T(c.diagonal), h, phi, phio, src, inlet, a, b, rm);
         } else {
             hipLaunchKernelGGL(k_dim_lap<T>, grid_int(g), blk, 0, s, g, dx, dy, dz, T(0), pn, d2);

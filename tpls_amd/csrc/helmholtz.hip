@@ -942,13 +942,16 @@ hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp,
     const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0);
     const int z = xold_is_zero ? 1 : 0;
     if (tl.jst) {
-        // staged sweep: tile rows that divide the period, chunks of 32 levels
+        // staged sweep: tile rows that divide the period, chunks of levels that fill
+        // the last round of resident blocks
         const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
         auto go_st = [&](auto comp_c, auto ty_c, auto zero_c) {
             constexpr int CP = decltype(comp_c)::value, TY = decltype(ty_c)::value;
             constexpr bool ZERO = decltype(zero_c)::value;
-            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, ks = 32;
-            const int nbx = (imax + BX - 1) / BX, nby = g.P / TY, nbz = (nz_of(CP, g) + ks - 1) / ks;
+            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
+            const int nbx = (imax + BX - 1) / BX, nby = g.P / TY;
+            static const int slots = resident_blocks(k_jacobi_st<CP, TY, 8, ZERO, T>, BX * TY);
+            const int ks = march_chunk(nbx * nby, nz, slots, 32), nbz = (nz + ks - 1) / ks;
             hipLaunchKernelGGL((k_jacobi_st<CP, TY, 8, ZERO, T>), dim3(nbx * nby * nbz), dim3(BX, TY), 0, s, g, nbx,
                                nby, ks, a2, b2, c2, xold, rhs, mu, x);
             return hipGetLastError();
@@ -992,13 +995,16 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
     auto go = [&](auto comp_c) {
         constexpr int CP = decltype(comp_c)::value;
         if (iteration_time % 2 == 0 && tl.jst) {
-            // staged planes, as the Jacobi sweep: rows that divide the period, chunks of 32 levels
+            // staged planes, as the Jacobi sweep: rows that divide the period, chunks of
+            // levels sized to the resident blocks
             const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
-            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, ks = 32;
-            const int nbx3 = (imax + BX - 1) / BX, nbz = (nz_of(CP, g) + ks - 1) / ks;
+            const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
+            const int nbx3 = (imax + BX - 1) / BX;
             auto go_st = [&](auto ty_c) {
                 constexpr int TY = decltype(ty_c)::value;
                 const int nby = g.P / TY;
+                static const int slots = resident_blocks(k_sor3d_st<CP, TY, 8, T>, 32 * TY);
+                const int ks = march_chunk(nbx3 * nby, nz, slots, 32), nbz = (nz + ks - 1) / ks;
                 hipLaunchKernelGGL((k_sor3d_st<CP, TY, 8, T>), dim3(nbx3 * nby * nbz), dim3(32 * TY), 0, s, g, nbx3,
                                    nby, ks, flg, a2, b2, c2, x, rhs, mu);
                 return hipGetLastError();

@@ -37,6 +37,44 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk)
     return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
+// Blocks of `kernel` (static LDS only) that the device can hold at once: its
+// CU count times the blocks of `threads` lanes one CU holds.  Read once per
+// kernel instantiation (the caller keeps it in a static), on the first device
+// it is launched on.
+template <typename K>
+inline int resident_blocks(K kernel, int threads)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess)
+        return 0;
+    return cus * per;
+}
+
+// Levels per block of a sweep that marches up k: nxy column tiles times nbz
+// chunks of ks levels run in rounds of `slots` resident blocks, a round
+// lasting about ks levels plus the pipeline's fill (three levels).  The chunk
+// count that minimises rounds x (ks + 3) keeps the last round from running
+// nearly empty.  slots <= 0 (unknown) falls back to `dflt` levels.
+inline int march_chunk(int nxy, int nz, int slots, int dflt)
+{
+    if (slots <= 0 || nz <= 0) return dflt;
+    int best_ks = dflt;
+    long best = -1;
+    for (int nbz = 1; nbz <= nz; ++nbz) {
+        const int ks = (nz + nbz - 1) / nbz;
+        if ((nz + ks - 1) / ks != nbz) continue;  // this ks is a smaller chunk count's
+        const long rounds = (long(nxy) * nbz + slots - 1) / slots;
+        const long cost = rounds * (ks + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_ks = ks;
+        }
+    }
+    return best_ks;
+}
+
 // Decode tile t into the first interior cell (i, j, k0) of this thread.
 __device__ __forceinline__ void tile_origin(const Tiles& tl, int t, int& i, int& j, int& k0)
 {
