@@ -23,14 +23,16 @@ from dataclasses import dataclass, fields
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# TPLS_ORACLE_SO names another build of tpls_oracle.c (tests/test_oracle_glue.py
+# loads copies with planted mistakes through it)
+ORACLE_SO = os.environ.get("TPLS_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libtpls_ref.so")
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 
-# Argument kinds: a = double array, I = int32 array, i = int, d = double,
-# o = double output scalar.
+# Argument kinds: a = double array, I = int32 array, i = int, l = long,
+# d = double, o = double output scalar.
 SPECS = {
     "weno_appox_convection": ("aaaaiiiiiiaaa", "weno", "wenoappoxconvection"),
     "get_levelset_status": ("aaIaiiiiid", "levelset", "get_levelset_status"),
@@ -53,6 +55,13 @@ ORACLE_ONLY = {
     "do_advect_phi": "aaaaaiiiiiiiddd",
     "do_tpls": "aaaaaiiiiiiidoddd",
     "dim": "iiddddddaaaaaiiiidio",
+    # main_ns_hybrid.F90's own statements of the step
+    "csf_faces": "aaaaaaiii",
+    "visc_aug": "aaaiiiddddd",
+    "momentum_rhs": "aaaaaaaaald",
+    "u_inout_bc": "aaiiidd",
+    "v_bc": "aiii",
+    "w_bc": "aiii",
 }
 
 
@@ -85,6 +94,8 @@ class _Lib:
                 cargs.append(_ptr(a, kind))
             elif kind == "i":
                 cargs.append(C.byref(C.c_int(int(a))) if self.by_ref else C.c_int(int(a)))
+            elif kind == "l":
+                cargs.append(C.c_long(int(a)))
             elif kind == "d":
                 cargs.append(C.byref(C.c_double(float(a))) if self.by_ref else C.c_double(float(a)))
             elif kind == "o":

@@ -1871,10 +1871,13 @@ static void rot3(double **a0, double **a1, double **a2)
 }
 
 /* u3 inlet copy and outlet pressure extrapolation, main_ns_hybrid.F90:764-776
- * (repeated at :787-799); the outlet reads pres with an unshifted j */
-static void u_inout_bc(double *u3_, F3 pres, int maxl, int maxm, int maxn, double dt, double dx)
+ * (repeated at :787-799); the outlet reads pres with an unshifted j.  u3 has
+ * the u shape (0:maxl, 0:maxm, 0:maxn-2), pres the p shape. */
+void orc_u_inout_bc(double *u3_, const double *pres_, int maxl, int maxm, int maxn, double dt,
+                    double dx)
 {
     F3 U = f3(u3_, 0, maxl, 0, maxm, 0);
+    F3 pres = f3((double *)pres_, 0, maxl, 0, maxm, 0);
     int ex_max = maxl - 1;
     for (int k = 0; k <= maxn - 2; ++k)
         for (int j = 1; j <= maxm - 1; ++j) {
@@ -1887,6 +1890,105 @@ static void u_inout_bc(double *u3_, F3 pres, int maxl, int maxm, int maxn, doubl
         }
 }
 
+/* v3 and w3 boundary conditions inside the Helmholtz loops: v zero-gradient
+ * in x (main_ns_hybrid.F90:826-836); w zero at both walls, then zero-gradient
+ * in x with the wall corners of the inlet column zero
+ * (main_ns_hybrid.F90:864-879). */
+void orc_v_bc(double *v3_, int maxl, int maxm, int maxn)
+{
+    F3 V = f3(v3_, 0, maxl, 0, maxm, 0);
+    const int ex_max = maxl - 1;
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = 1; j <= maxm - 1; ++j) {
+            A3(V, 0, j, k) = A3(V, 1, j, k);
+            A3(V, ex_max + 1, j, k) = A3(V, ex_max, j, k);
+        }
+}
+
+void orc_w_bc(double *w3_, int maxl, int maxm, int maxn)
+{
+    F3 W = f3(w3_, 0, maxl, 0, maxm, 0);
+    const int ex_max = maxl - 1;
+    for (int j = 1; j <= maxm - 1; ++j)
+        for (int i = 1; i <= ex_max; ++i) {
+            A3(W, i, j, 0) = 0.0;
+            A3(W, i, j, maxn - 1) = 0.0;
+        }
+    for (int j = 1; j <= maxm - 1; ++j) {
+        for (int k = 1; k <= maxn - 2; ++k) A3(W, 0, j, k) = A3(W, 1, j, k);
+        A3(W, 0, j, 0) = 0.0;
+        A3(W, 0, j, maxn - 1) = 0.0;
+        for (int k = 0; k <= maxn - 1; ++k) A3(W, ex_max + 1, j, k) = A3(W, ex_max, j, k);
+    }
+}
+
+/* CSF forces averaged onto the velocity faces, main_ns_hybrid.F90:629-641:
+ * csf_u(i,j,k) = (fx(i+1,j+1,k+1) + fx(i,j+1,k+1))/2, csf_v(i,j,k) =
+ * (fy(i,j+1,k+1) + fy(i,j,k+1))/2 over k = 0..maxn-2, csf_w(i,j,k) =
+ * (fz(i,j+1,k+1) + fz(i,j+1,k))/2 over k = 0..maxn-1; i = 1..maxl-1,
+ * j = 1..maxm-1.  fx, fy, fz: p shape; csf_u, csf_v: u shape; csf_w: w shape. */
+void orc_csf_faces(const double *fx_, const double *fy_, const double *fz_, double *cu_,
+                   double *cv_, double *cw_, int maxl, int maxm, int maxn)
+{
+    F3 fx = f3((double *)fx_, 0, maxl, 0, maxm, 0), fy = f3((double *)fy_, 0, maxl, 0, maxm, 0);
+    F3 fz = f3((double *)fz_, 0, maxl, 0, maxm, 0);
+    F3 cu = f3(cu_, 0, maxl, 0, maxm, 0), cv = f3(cv_, 0, maxl, 0, maxm, 0);
+    F3 cw = f3(cw_, 0, maxl, 0, maxm, 0);
+    const int sx = 1, ex = maxl - 1, sy = 1, ey = maxm - 1;
+    for (int k = 0; k <= maxn - 2; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i) {
+                A3(cu, i, j, k) = (A3(fx, i + 1, j + 1, k + 1) + A3(fx, i, j + 1, k + 1)) / 2.0;
+                A3(cv, i, j, k) = (A3(fy, i, j + 1, k + 1) + A3(fy, i, j, k + 1)) / 2.0;
+            }
+    for (int k = 0; k <= maxn - 1; ++k)
+        for (int j = sy; j <= ey; ++j)
+            for (int i = sx; i <= ex; ++i)
+                A3(cw, i, j, k) = (A3(fz, i, j + 1, k + 1) + A3(fz, i, j + 1, k)) / 2.0;
+}
+
+/* The augmented viscosity, main_ns_hybrid.F90:654-710: the interior of
+ * visc2 copied, and the inlet and outlet planes (i = 0 and maxl) of both
+ * visc2 and visc2_aug recomputed from phi2's planes there (phi2 - shift:
+ * shift 0.5 with the diffuse interface method, main:664-668, 689-693), for
+ * j = 1..maxm-1 and every k.  visc2, phi2: p shape; visc2_aug:
+ * (-1:maxl+1, -1:maxm+1, 0:maxn). */
+void orc_visc_aug(double *visc2_, double *visc2_aug_, const double *phi2_, int maxl, int maxm,
+                  int maxn, double sw, double Re, double mu_minus, double mu_plus, double shift)
+{
+    F3 v2 = f3(visc2_, 0, maxl, 0, maxm, 0);
+    F3 va = f3(visc2_aug_, -1, maxl + 1, -1, maxm + 1, 0);
+    F3 phi2 = f3((double *)phi2_, 0, maxl, 0, maxm, 0);
+    const int sx = 1, ex = maxl - 1, sy = 1, ey = maxm - 1, ex_max = maxl - 1;
+    for (int k = 0; k <= maxn; ++k)
+        for (int j = sy; j <= ey; ++j) {
+            for (int i = sx; i <= ex; ++i) A3(va, i, j, k) = A3(v2, i, j, k);
+            double p0 = A3(phi2, 0, j, k), p1 = A3(phi2, ex_max + 1, j, k);
+            if (shift != 0.0) {
+                p0 = p0 - shift;
+                p1 = p1 - shift;
+            }
+            double m0 = visc_of_phi(p0, sw, Re, mu_minus, mu_plus);
+            A3(va, 0, j, k) = m0;
+            A3(v2, 0, j, k) = m0;
+            double m1 = visc_of_phi(p1, sw, Re, mu_minus, mu_plus);
+            A3(va, ex_max + 1, j, k) = m1;
+            A3(v2, ex_max + 1, j, k) = m1;
+        }
+}
+
+/* The momentum right-hand side of one component, main_ns_hybrid.F90:743-757:
+ * RHS = x2 - dt*AB3(conv) + dt/2*diff + dt*AB3(csf), AB3(a) = 23/12 a2 -
+ * 4/3 a1 + 5/12 a0, over n elements. */
+void orc_momentum_rhs(double *R, const double *x2, const double *c2, const double *c1,
+                      const double *c0, const double *df, const double *s2, const double *s1,
+                      const double *s0, long n, double dt)
+{
+    for (long q = 0; q < n; ++q)
+        R[q] = x2[q] - dt * ((23.0 / 12.0) * c2[q] - (4.0 / 3.0) * c1[q] + (5.0 / 12.0) * c0[q]) +
+               (dt / 2.0) * df[q] + dt * ((23.0 / 12.0) * s2[q] - (4.0 / 3.0) * s1[q] + (5.0 / 12.0) * s0[q]);
+}
+
 static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
 {
     const int maxl = s->maxl, maxm = s->maxm, maxn = s->maxn;
@@ -1895,8 +1997,6 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
     const double dx = P->dx, dy = P->dy, dz = P->dz, dt = P->dt;
     double **f = s->f;
     double t0 = now_s();
-    F3 phi2 = f3(f[ORC_PHI2], 0, maxl, 0, maxm, 0);
-    F3 pres = f3(f[ORC_PRES], 0, maxl, 0, maxm, 0);
     long n_uv = s->cnt[ORC_U2], n_w = s->cnt[ORC_W2];
 
     /* main:585 */
@@ -1932,22 +2032,8 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
     exchange2d(f[ORC_FY_CSF], sx, ex, sy, ey, 0, maxn);
     exchange2d(f[ORC_FZ_CSF], sx, ex, sy, ey, 0, maxn);
     /* main:629-641 */
-    {
-        F3 fx = f3(f[ORC_FX_CSF], 0, maxl, 0, maxm, 0), fy = f3(f[ORC_FY_CSF], 0, maxl, 0, maxm, 0);
-        F3 fz = f3(f[ORC_FZ_CSF], 0, maxl, 0, maxm, 0);
-        F3 cu = f3(f[ORC_CSF_U2], 0, maxl, 0, maxm, 0), cv = f3(f[ORC_CSF_V2], 0, maxl, 0, maxm, 0);
-        F3 cw = f3(f[ORC_CSF_W2], 0, maxl, 0, maxm, 0);
-        for (int k = 0; k <= maxn - 2; ++k)
-            for (int j = sy; j <= ey; ++j)
-                for (int i = sx; i <= ex; ++i) {
-                    A3(cu, i, j, k) = (A3(fx, i + 1, j + 1, k + 1) + A3(fx, i, j + 1, k + 1)) / 2.0;
-                    A3(cv, i, j, k) = (A3(fy, i, j + 1, k + 1) + A3(fy, i, j, k + 1)) / 2.0;
-                }
-        for (int k = 0; k <= maxn - 1; ++k)
-            for (int j = sy; j <= ey; ++j)
-                for (int i = sx; i <= ex; ++i)
-                    A3(cw, i, j, k) = (A3(fz, i, j + 1, k + 1) + A3(fz, i, j + 1, k)) / 2.0;
-    }
+    orc_csf_faces(f[ORC_FX_CSF], f[ORC_FY_CSF], f[ORC_FZ_CSF], f[ORC_CSF_U2], f[ORC_CSF_V2],
+                  f[ORC_CSF_W2], maxl, maxm, maxn);
     /* main:646-710 */
     /* DIM: get_viscosity(viscosity2, phi2-0.5, ...) and phi_val = phi2-0.5 at
      * the inlet/outlet planes (main:650-651, 664-668, 689-693) */
@@ -1962,27 +2048,10 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
                           P->mu_minus, P->mu_plus);
         free(pm);
     }
-    {
-        F3 v2 = f3(f[ORC_VISC2], 0, maxl, 0, maxm, 0);
-        F3 va = f3(f[ORC_VISC2_AUG], -1, maxl + 1, -1, maxm + 1, 0);
-        for (int k = 0; k <= maxn; ++k)
-            for (int j = sy; j <= ey; ++j) {
-                for (int i = sx; i <= ex; ++i) A3(va, i, j, k) = A3(v2, i, j, k);
-                double p0 = A3(phi2, 0, j, k), p1 = A3(phi2, ex_max + 1, j, k);
-                if (P->idm_dim) {
-                    p0 = p0 - vshift;
-                    p1 = p1 - vshift;
-                }
-                double m0 = visc_of_phi(p0, P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
-                A3(va, 0, j, k) = m0;
-                A3(v2, 0, j, k) = m0;
-                double m1 = visc_of_phi(p1, P->smooth_width, P->Re, P->mu_minus, P->mu_plus);
-                A3(va, ex_max + 1, j, k) = m1;
-                A3(v2, ex_max + 1, j, k) = m1;
-            }
-        exchange2d_aug12(f[ORC_VISC2_AUG], sx, ex, sy, ey, 0, maxn);
-        exchange2d(f[ORC_VISC2], sx, ex, sy, ey, 0, maxn);
-    }
+    orc_visc_aug(f[ORC_VISC2], f[ORC_VISC2_AUG], f[ORC_PHI2], maxl, maxm, maxn, P->smooth_width,
+                 P->Re, P->mu_minus, P->mu_plus, vshift);
+    exchange2d_aug12(f[ORC_VISC2_AUG], sx, ex, sy, ey, 0, maxn);
+    exchange2d(f[ORC_VISC2], sx, ex, sy, ey, 0, maxn);
     /* main:724-735 */
     exchange2d(f[ORC_U2], sx, ex, sy, ey, 0, maxn - 2);
     exchange2d(f[ORC_V2], sx, ex, sy, ey, 0, maxn - 2);
@@ -1996,16 +2065,10 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
         int cid[3][7] = {{ORC_RHS_U, ORC_U2, ORC_CONV2_U, ORC_CONV1_U, ORC_CONV0_U, ORC_DIFF_U, ORC_CSF_U2},
                          {ORC_RHS_V, ORC_V2, ORC_CONV2_V, ORC_CONV1_V, ORC_CONV0_V, ORC_DIFF_V, ORC_CSF_V2},
                          {ORC_RHS_W, ORC_W2, ORC_CONV2_W, ORC_CONV1_W, ORC_CONV0_W, ORC_DIFF_W, ORC_CSF_W2}};
-        for (int c = 0; c < 3; ++c) {
-            double *R = f[cid[c][0]], *x2 = f[cid[c][1]], *c2 = f[cid[c][2]], *c1 = f[cid[c][3]];
-            double *c0 = f[cid[c][4]], *df = f[cid[c][5]], *s2 = f[cid[c][6]];
-            double *s1 = f[cid[c][6] - 1], *s0 = f[cid[c][6] - 2];
-            long cnt = c == 2 ? n_w : n_uv;
-            for (long q = 0; q < cnt; ++q)
-                R[q] = x2[q] - dt * ((23.0 / 12.0) * c2[q] - (4.0 / 3.0) * c1[q] + (5.0 / 12.0) * c0[q]) +
-                       (dt / 2.0) * df[q] +
-                       dt * ((23.0 / 12.0) * s2[q] - (4.0 / 3.0) * s1[q] + (5.0 / 12.0) * s0[q]);
-        }
+        for (int c = 0; c < 3; ++c)
+            orc_momentum_rhs(f[cid[c][0]], f[cid[c][1]], f[cid[c][2]], f[cid[c][3]], f[cid[c][4]],
+                             f[cid[c][5]], f[cid[c][6]], f[cid[c][6] - 1], f[cid[c][6] - 2],
+                             c == 2 ? n_w : n_uv, dt);
     }
     /* main:758-808: u */
     {
@@ -2016,14 +2079,14 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
             exchange2d(s->u3_old, sx, ex, sy, ey, 0, maxn - 2);
             orc_do_jacobi_u(u3, s->u3_old, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx,
                             sy, maxn, ex_max);
-            u_inout_bc(u3, pres, maxl, maxm, maxn, dt, dx);
+            orc_u_inout_bc(u3, f[ORC_PRES], maxl, maxm, maxn, dt, dx);
             exchange2d(u3, sx, ex, sy, ey, 0, maxn - 2);
             orc_do_sor_u(u3, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn,
                          ex_max, 0, it);
             exchange2d(u3, sx, ex, sy, ey, 0, maxn - 2);
             orc_do_sor_u(u3, f[ORC_RHS_U], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn,
                          ex_max, 1, it);
-            u_inout_bc(u3, pres, maxl, maxm, maxn, dt, dx);
+            orc_u_inout_bc(u3, f[ORC_PRES], maxl, maxm, maxn, dt, dx);
         }
         dg->err_u = orc_get_difference(u3, s->u3_old, ex, ey, maxn - 2, sx, sy, 0);
     }
@@ -2032,7 +2095,6 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
     /* main:813-845: v */
     {
         double *v3 = f[ORC_V3];
-        F3 V = f3(v3, 0, maxl, 0, maxm, 0);
         memset(v3, 0, sizeof(double) * n_uv);
         for (int it = 1; it <= P->mom_v / 2; ++it) {
             memcpy(s->v3_old, v3, sizeof(double) * n_uv);
@@ -2043,18 +2105,13 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
             orc_do_sor_v(v3, f[ORC_RHS_V], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 0, it);
             exchange2d(v3, sx, ex, sy, ey, 0, maxn - 2);
             orc_do_sor_v(v3, f[ORC_RHS_V], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 1, it);
-            for (int k = 0; k <= maxn - 2; ++k)
-                for (int j = sy; j <= ey; ++j) {
-                    A3(V, 0, j, k) = A3(V, 1, j, k);
-                    A3(V, ex_max + 1, j, k) = A3(V, ex_max, j, k);
-                }
+            orc_v_bc(v3, maxl, maxm, maxn);
         }
         dg->err_v = orc_get_difference(v3, s->v3_old, ex, ey, maxn - 2, sx, sy, 0);
     }
     /* main:847-887: w */
     {
         double *w3 = f[ORC_W3];
-        F3 W = f3(w3, 0, maxl, 0, maxm, 0);
         memset(w3, 0, sizeof(double) * n_w);
         for (int it = 1; it <= P->mom_w / 2; ++it) {
             memcpy(s->w3_old, w3, sizeof(double) * n_w);
@@ -2065,17 +2122,7 @@ static void step_impl(orc_state *s, orc_diag *dg, double *t_nonp, double *t_p)
             orc_do_sor_w(w3, f[ORC_RHS_W], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 0, it);
             exchange2d(w3, sx, ex, sy, ey, 0, maxn - 1);
             orc_do_sor_w(w3, f[ORC_RHS_W], f[ORC_VISC2_AUG], dx, dy, dz, dt, ex, ey, sx, sy, maxn, 1, it);
-            for (int j = sy; j <= ey; ++j)
-                for (int i = sx; i <= ex; ++i) {
-                    A3(W, i, j, 0) = 0.0;
-                    A3(W, i, j, maxn - 1) = 0.0;
-                }
-            for (int j = sy; j <= ey; ++j) {
-                for (int k = 1; k <= maxn - 2; ++k) A3(W, 0, j, k) = A3(W, 1, j, k);
-                A3(W, 0, j, 0) = 0.0;
-                A3(W, 0, j, maxn - 1) = 0.0;
-                for (int k = 0; k <= maxn - 1; ++k) A3(W, ex_max + 1, j, k) = A3(W, ex_max, j, k);
-            }
+            orc_w_bc(w3, maxl, maxm, maxn);
         }
         dg->err_w = orc_get_difference(w3, s->w3_old, ex, ey, maxn - 1, sx, sy, 0);
     }

@@ -24,8 +24,13 @@
  * dependency-free reference module is checked bit-for-bit against the
  * reference Fortran compiled by oracle/Makefile into oracle/_ref/.  The step
  * driver's main-program pieces (CSF face averages, inlet/outlet viscosity,
- * AB3 right-hand sides, Helmholtz outer loops, history rotation) are pinned
- * by invariants and by composing the reference modules in the tests.  The
+ * AB3 right-hand sides, velocity boundary conditions, history rotation) and
+ * the two modules that need MPI (advect_phi.f90, two_phase_levelset.f90)
+ * are pinned by what the mathematics fixes (tests/test_oracle_glue.py:
+ * exactness on uniform, linear and quadratic-in-time data, the reference's
+ * own routines applied to the planes main builds, zero divergence after the
+ * step), and that file shows that copies of this oracle with plausible
+ * mistakes planted in those statements fail it.  The
  * pressure solve restates PETSc 3.6 KSPCG (pc none, constant null space) on
  * the symmetric reduced system: PETSc is not available here, so its parity
  * with PETSc itself (iterates, iteration counts) is UNPINNED.  Its solution is
@@ -75,6 +80,32 @@ void orc_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, dou
  * (cahn_hilliard_solver.f90:350-352). */
 double orc_dim_diagonal(double dx, double dy, double dz, double dt, double epn, double Pe);
 void orc_dim_inlet(int maxn, double dz, double height, double epn, double *prof);
+/* ---- main-program statements of the step (main_ns_hybrid.F90) ----
+ * Single rank, arrays in the shapes of main_ns_hybrid.F90:416-549 (p shape
+ * (0:maxl,0:maxm,0:maxn), u/v (0:maxl,0:maxm,0:maxn-2), w (0:maxl,0:maxm,
+ * 0:maxn-1), visc2_aug (-1:maxl+1,-1:maxm+1,0:maxn)).  The reference has them
+ * inline in the main program, which needs MPI, PETSc and netCDF and cannot be
+ * compiled here; each is pinned by what the mathematics fixes
+ * (tests/test_oracle_glue.py). */
+/* main_ns_hybrid.F90:629-641: CSF forces averaged onto the velocity faces */
+void orc_csf_faces(const double *fx, const double *fy, const double *fz, double *csf_u,
+                   double *csf_v, double *csf_w, int maxl, int maxm, int maxn);
+/* main_ns_hybrid.F90:654-710: visc2_aug's interior and the inlet/outlet
+ * planes of visc2 and visc2_aug from phi2 - shift */
+void orc_visc_aug(double *visc2, double *visc2_aug, const double *phi2, int maxl, int maxm,
+                  int maxn, double smooth_width, double Re, double mu_minus, double mu_plus,
+                  double shift);
+/* main_ns_hybrid.F90:743-757: one component's AB3 + Crank-Nicolson RHS */
+void orc_momentum_rhs(double *R, const double *x2, const double *conv2, const double *conv1,
+                      const double *conv0, const double *diff, const double *csf2,
+                      const double *csf1, const double *csf0, long n, double dt);
+/* main_ns_hybrid.F90:764-776 / 787-799: u3 inlet copy and outlet condition */
+void orc_u_inout_bc(double *u3, const double *pres, int maxl, int maxm, int maxn, double dt,
+                    double dx);
+/* main_ns_hybrid.F90:826-836 and 864-879: v3 and w3 conditions */
+void orc_v_bc(double *v3, int maxl, int maxm, int maxn);
+void orc_w_bc(double *w3, int maxl, int maxm, int maxn);
+
 /* levelset.f90:17-94 */
 void orc_get_levelset_status(const double *phi, double *D_mx, int *istatus, double *sign_mx,
                              int sx, int ex, int sy, int ey, int n, double h);
