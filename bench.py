@@ -48,7 +48,9 @@ def parse():
     ap.add_argument("--pc", default="mg", choices=["none", "mg"],
                     help="pressure CG preconditioner: none (plain CG) or mg (multigrid V-cycle)")
     ap.add_argument("--mg-nu", type=int, default=2, help="V-cycle pre/post smoothing sweeps")
-    ap.add_argument("--roofline-kernel", default="jacobi", choices=["jacobi", "sor_odd", "sor_even", "cg_x"])
+    ap.add_argument("--roofline-kernel", default="jacobi", choices=["jacobi", "sor_odd", "sor_even", "cg_x", "advect"])
+    ap.add_argument("--second-roofline", default="advect",
+                    help="a further kernel class timed the same way and reported under rooflines ('' to skip)")
     ap.add_argument("--roofline-steps", type=int, default=2, help="steps over which the kernel is timed")
     ap.add_argument("--fp32-compare", type=int, default=1,
                     help="also run the case with FP32 storage over the same steps and report its "
@@ -115,7 +117,29 @@ def cpu_baseline(p, its_gpu: float, threads: int):
     }
 
 
-KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x"}
+def cpu_one_core():
+    """The oracle on one host core, on the CPU-runnable workload (128x32x64,
+    Re 100, viscosity ratio 2, dt 1e-4, the same multigrid CG): two steps
+    after one untimed step."""
+    from dataclasses import replace
+    from oracle import binding as B
+    from tpls_amd.config import CHANNEL_128x32x64
+    from tpls_amd.initial import make_initial
+    q = replace(CHANNEL_128x32x64, ksp_pc="mg")
+    B.set_threads(1)
+    st = B.OracleState(q.maxl, q.maxm, q.maxn, q.solver_dict())
+    st.load(*make_initial(q))
+    st.step(1)
+    t0 = time.perf_counter()
+    st.step(2)
+    el = time.perf_counter() - t0
+    st.close()
+    return {"value": round(q.cells * 2 / el * 1e-6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle C restatement, 1 thread: 2 steps of 128x32x64 (multigrid CG), {el:.2f} s"}
+
+
+KERNEL_NAMES = {"sor_odd": "k_sor2d", "jacobi": "k_jacobi", "sor_even": "k_sor3d", "cg_x": "k_cg_x",
+                "advect": "k_advect_phi"}
 FUSED_NAMES = {"sor_odd": "k_rb2d", "sor_even": "k_rb3d"}
 STAGED_NAMES = {"jacobi": "k_jacobi_st", "sor_even": "k_sor3d_st"}
 
@@ -143,6 +167,9 @@ def algorithmic_bytes(p, kernel: str, precision: str = "fp64", split: bool = Fal
     L, P, N = p.maxl, p.maxm - 1, p.maxn
     if kernel == "cg_x":
         return 40.0 * (L - 1) * P * (N - 1)          # FP64 in both precisions
+    if kernel == "advect":
+        # phi2, u2, v2, w2 read, conv2_phi written, per p cell
+        return 40.0 * (L - 1) * P * (N - 1) * (0.5 if precision == "fp32" else 1.0)
     cells = ((L - 2) * P * (N - 1) + (L - 1) * P * (N - 1) + (L - 1) * P * (N - 2)) / 3.0
     per = 32.0 if kernel == "jacobi" or not split else 24.0
     return per * cells * (0.5 if precision == "fp32" else 1.0)
@@ -385,12 +412,15 @@ def main():
     keep = a.fp32_compare and a.precision == "fp64"
     ref = {f: case.field(f) for f in COMPARED} if keep else None
     ref_volume = case.volume() if keep else None
-    # the dominant kernel, timed with HIP events on the case's stream over
-    # further steps (launched eagerly: events bracket each launch)
-    case.time_kernels(a.roofline_kernel)
-    case.step(a.roofline_steps)
-    ms, n = case.kernel_time()
-    case.time_kernels(None)
+    # the dominant kernel (and a second class), timed with HIP events on the
+    # case's stream over further steps (launched eagerly: events bracket each
+    # launch)
+    timed = {}
+    for kern in [a.roofline_kernel] + ([a.second_roofline] if a.second_roofline else []):
+        case.time_kernels(kern)
+        case.step(a.roofline_steps)
+        timed[kern] = case.kernel_time()
+        case.time_kernels(None)
     case.close()
 
     plain = None
@@ -400,12 +430,18 @@ def main():
     cells = p.cells
     value = cells * a.steps / elapsed * 1e-6
     lat_ms = np.array(lat) * 1e3
-    split = sor_split(a, a.roofline_kernel)
-    name = kernel_name(a.roofline_kernel, split, staged)
-    kbytes = algorithmic_bytes(p, a.roofline_kernel, a.precision, split)
-    avg_ms = ms / max(n, 1)
-    achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
-    traffic = measured_traffic(name, nx, ny, nz, a.precision)
+    def roofline_of(kern):
+        ms, n = timed[kern]
+        split = sor_split(a, kern)
+        name = kernel_name(kern, split, staged)
+        kbytes = algorithmic_bytes(p, kern, a.precision, split)
+        avg_ms = ms / max(n, 1)
+        achieved = kbytes / (avg_ms * 1e-3) / 1e9 if n else 0.0
+        return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(name, nx, ny, nz, a.precision), "avg_launch_us": round(avg_ms * 1e3, 2),
+                "launches": n, "timed_over_steps": a.roofline_steps, "algorithmic_bytes_per_launch": int(kbytes)}
+
     out = {
         "metric": "Mcell-steps/s batched-FP32 vs plain-FP64 at stated error; p50/p99 step latency",
         "value": round(value, 4),
@@ -422,18 +458,16 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32 (CG in f64)",
         "data": "synthetic (seeded two-layer channel flow)",
         "config": {"workload": f"plain {a.precision} single channel case {nx}x{ny}x{nz} cells, "
-                               f"Re {p.Re:g}, viscosity ratio {p.mu_plus / p.mu_minus:g}, dt {p.dt:g}, "
-                               f"CG rtol {p.ksp_rtol:g}",
+                               f"Re {p.Re:g}, viscosity ratio {p.mu_plus / p.mu_minus:g}, density ratio 1 "
+                               f"(the reference has no density), dt {p.dt:g}, CG rtol {p.ksp_rtol:g}",
                    "solver": "cg" if a.pc == "none" else f"cg + multigrid V({a.mg_nu},{a.mg_nu})",
                    "launch": "hip graphs" if a.graphs else "eager",
                    "tiles": "autotuned" if a.tiles == "auto" else "default",
                    "cg_its_mean": round(float(np.mean(its)), 1), "ksp_reason": d["ksp_reason"]},
-        "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "avg_launch_us": round(avg_ms * 1e3, 2), "launches": n,
-                     "timed_over_steps": a.roofline_steps,
-                     "algorithmic_bytes_per_launch": int(kbytes)},
+        "roofline": roofline_of(a.roofline_kernel),
     }
+    if a.second_roofline:
+        out["rooflines"] = {k: roofline_of(k) for k in timed}
     if plain:
         out["plain_cg"] = plain
     if a.dim_steps > 0:
@@ -447,6 +481,7 @@ def main():
     # throughput ratios of the batched FP32 run against the FP64 single case
     # (multigrid CG) and against the plain FP64 path (plain CG, the
     # reference's -pc_type none configuration)
+    # the same-solver ratio (multigrid CG on both sides) is the headline one
     ratios = {}
     if "batched_fp32" in out:
         bv = out["batched_fp32"]["value"]
@@ -460,6 +495,7 @@ def main():
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(p, float(np.mean(its)), threads)
+        out["cpu_baseline"]["one_core"] = cpu_one_core()
     print(json.dumps(out), flush=True)
 
 

@@ -100,6 +100,12 @@ typedef struct {
     int dim_split;        /* 1: each DIM relaxation sweep as two launches (the
                              Laplacian, then the in-place update) instead of
                              one fused out-of-place pass; identical results */
+    int march_levels;     /* levels each block of the k-marching sweeps (the
+                             staged Jacobi and even-parity SOR sweeps, the
+                             fused DIM sweep) covers: 0 sizes the chunks to
+                             the device's resident blocks; n >= 1 fixes n
+                             (clamped to the column height).  Identical
+                             results either way */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */
@@ -157,6 +163,14 @@ int tpls_step(tpls_case *c, int nsteps);
  * alone.  A batched counterpart of main_ns_hybrid.F90:583-918 for
  * independent cases; the reference runs one case per job. */
 int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
+/* tpls_batch_step with each case's outcome in status[q] (TPLS_OK or the
+ * error of that case's step).  A case whose step fails stops there and is
+ * marked failed (later steps of it are refused); the other cases complete
+ * their steps unperturbed.  Returns TPLS_OK, or the first failure's code
+ * with tpls_last_error() naming its case.  A diverged pressure solve is not
+ * a failure here: it is reported in the case's tpls_diag.ksp_reason (< 0),
+ * as KSPGetConvergedReason reports it (main_ns_hybrid.F90:903-911). */
+int tpls_batch_step_status(tpls_case **cases, int ncases, int nsteps, int *status);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Change the launch shapes of the Helmholtz sweeps (the tile_rows,
@@ -184,7 +198,8 @@ enum {
     TPLS_TIME_CG_X = 1,      /* CG update x += alpha p, r -= alpha A p */
     TPLS_TIME_JACOBI = 2,    /* Jacobi sweep (do_jacobi_u/v/w) */
     TPLS_TIME_SOR_ODD = 3,   /* SOR half sweep, odd iteration_time (k-ascending columns) */
-    TPLS_TIME_SOR_EVEN = 4   /* SOR half sweep, even iteration_time (3-D colouring) */
+    TPLS_TIME_SOR_EVEN = 4,  /* SOR half sweep, even iteration_time (3-D colouring) */
+    TPLS_TIME_ADVECT = 5     /* WENO5 level-set advection (do_advect_phi) */
 };
 int tpls_case_time_kernels(tpls_case *c, int kernel_class);
 int tpls_case_kernel_time(tpls_case *c, double *ms, long *launches);

@@ -19,6 +19,9 @@ program drive_modules
   use jacobi_iteration_allflux
   use sor_iteration_allflux
   use pressure
+  use tpls_state, only: st_maxl => maxl, st_maxm => maxm, st_maxn => maxn, pres, RHS_p, u3, u_inlet, &
+      st_dx => dx, st_dy => dy, st_dz => dz, st_dt => dt
+  use pressure_solver
   implicit none
 
   integer :: rid, maxl, maxm, maxn, fin, fout
@@ -30,6 +33,9 @@ program drive_modules
   double precision, allocatable :: mu(:,:,:), aa(:,:,:), inl(:)
   integer, allocatable :: ist(:,:,:)
   integer :: nb(6)
+  type(tpls_ksp) :: ksp
+  integer :: ierr, its
+  double precision :: rnorm
 
   call get_command_argument(1, fn_in)
   call get_command_argument(2, fn_out)
@@ -134,6 +140,28 @@ program drive_modules
     call dim(maxl, maxn, d1, d1, d1, d2, d3, d4, p1, p2, p3, p4, p5, 1, maxl - 1, 1, maxm - 1, 0, maxn, &
         d5, -1, -1, nb, -1, i1)
     write(fout) p4, p5
+  case (19)   ! main_ns_hybrid.F90:896-913: pres = pres_old, KSPSolve, the prints, copy_pressure_out_of_petsc
+    st_maxl = maxl; st_maxm = maxm; st_maxn = maxn
+    allocate(pres(0:maxl, 0:maxm, 0:maxn), RHS_p(0:maxl, 0:maxm, 0:maxn), u3(0:maxl, 0:maxm, 0:maxn-2))
+    allocate(u_inlet(0:maxn-2))
+    call get_p(p1)
+    read(fin) RHS_p, u3, u_inlet
+    read(fin) st_dx, st_dy, st_dz, st_dt, d1
+    call tpls_ksp_create(ksp, d1, 1.d-50, 1.d5, 10000)
+    pres = p1
+    call KSPSolve(ksp, PETSC_NULL_OBJECT, PETSC_NULL_OBJECT, ierr)
+    if (ierr /= 0) error stop 3
+    call output_converged_reason(ksp)
+    call KSPGetResidualNorm(ksp, rnorm, ierr)
+    if (ierr /= 0) error stop 4
+    write(*, *) 'approximate, preconditioned, residual norm', rnorm
+    call KSPGetIterationNumber(ksp, its, ierr)
+    if (ierr /= 0) error stop 5
+    write(*, *) 'iterations =', its
+    call copy_pressure_out_of_petsc(ksp, ierr)
+    if (ierr /= 0) error stop 6
+    call KSPGetConvergedReason(ksp, i1, ierr)
+    write(fout) pres, its, i1, rnorm
   case default
     write(0, *) 'drive_modules: unknown routine id', rid
     error stop 2

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from oracle import binding as B
-from tests.compare import relerr
+from tests.compare import demean, relerr
 from tests.fields import level_set, rand, shapes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -69,6 +69,16 @@ def test_argument_lists_match_reference():
             assert ours[name] == args, (name, ours[name], args)
             checked += 1
     assert checked == 19
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent")
+def test_pressure_solver_argument_lists_match_reference():
+    """The two procedures of pressure_solver.F90 that main calls keep their
+    argument lists (pressure_solver.F90:88, 444)."""
+    ours = subroutine_args(open(MODSRC).read())
+    ref = subroutine_args(open(os.path.join(REF_SRC, "pressure_solver.F90")).read())
+    for name in ("output_converged_reason", "copy_pressure_out_of_petsc"):
+        assert ours[name] == ref[name], (name, ours[name], ref[name])
 
 
 def _write_stream(path, rid, maxl, maxm, maxn, args):
@@ -180,6 +190,39 @@ def test_module_dim(tmp_path):
                          [(S["p"], np.float64), (S["p"], np.float64)])
     assert np.array_equal(cg, co)
     assert np.array_equal(ng[:, 1:MAXM], no[:, 1:MAXM])
+
+
+@pytest.mark.gpu
+def test_main_pressure_block_through_pressure_solver(tmp_path):
+    """main_ns_hybrid.F90:896-913 as main writes it (pres = pres_old;
+    KSPSolve(ksp, PETSC_NULL_OBJECT, PETSC_NULL_OBJECT, ierr);
+    output_converged_reason; KSPGetResidualNorm; KSPGetIterationNumber;
+    copy_pressure_out_of_petsc) through modules tpls_state and
+    pressure_solver, against the oracle's solve of the same system."""
+    if not os.access(DRIVER, os.X_OK):
+        pytest.fail("tests/fortran/drive_modules is not built")
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(41)
+    rhs, u3 = rand(r, S["p"]), rand(r, S["uv"])
+    inlet = np.ascontiguousarray(r.standard_normal(MAXN - 1))
+    p0 = rand(r, S["p"], scale=0.01)
+    dt, rtol = 2e-4, 1e-10
+    po = p0.copy(order="F")
+    reason_o, its_o, _ = B.pressure_solve(po, rhs, u3, inlet, MAXL, MAXM, MAXN, DX, DX, DX, dt, rtol=rtol)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_stream(fin, 19, MAXL, MAXM, MAXN, [p0, rhs, u3, inlet, DX, DX, DX, dt, rtol])
+    res = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    print("[fortran] main's KSP block:", res.stdout.strip().replace("\n", " | "), flush=True)
+    assert "KSP_CONVERGED_RTOL" in res.stdout
+    raw = open(fout, "rb").read()
+    n = int(np.prod(S["p"]))
+    pg = np.frombuffer(raw, np.float64, n).reshape(S["p"], order="F")
+    its_g, reason_g = np.frombuffer(raw, np.int32, 2, offset=8 * n)
+    assert reason_g == reason_o == 2
+    assert abs(int(its_g) - its_o) <= 1
+    P = MAXM - 1
+    assert relerr(demean(pg, MAXL, P, MAXN)[:, 1:P + 1], demean(po, MAXL, P, MAXN)[:, 1:P + 1]) < 1e-8
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent")

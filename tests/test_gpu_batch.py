@@ -66,3 +66,41 @@ def test_batch_rejects_a_case_twice():
     with Case(p) as c:
         with pytest.raises(_lib.TplsError, match="twice"):
             batch_step([c, c], 1)
+
+
+def test_a_failing_case_leaves_the_others_unperturbed():
+    """One case's step fails (forced on the host before it launches anything):
+    tpls_batch_step_status reports it, the case refuses further steps, and
+    every other case completes the same steps bit for bit as alone."""
+    from tpls_amd import _lib
+    from tpls_amd.case import Case, batch_step_status
+    from tpls_amd.initial import make_initial
+    cs = [CaseParams(nx=32, ny=16, nz=24, seed=1), CaseParams(nx=48, ny=8, nz=16, Re=80.0, seed=2),
+          CaseParams(nx=64, ny=16, nz=32, Re=300.0, mu_plus=5.0, seed=3, ksp_pc="mg")]
+    want = []
+    for p in cs:
+        with Case(p, graphs=True) as c:
+            c.load(*make_initial(p))
+            c.step(3)
+            want.append({f: c.field(f) for f in FIELDS})
+    hs = []
+    for p in cs:
+        c = Case(p, graphs=True)
+        c.load(*make_initial(p))
+        hs.append(c)
+    try:
+        assert [rc for rc, _ in batch_step_status(hs, 1)] == [0, 0, 0]
+        lib = _lib.lib()
+        assert lib.tpls_test_fail_next_step(hs[1]._h) == 0
+        st = batch_step_status(hs, 2)
+        print(f"[batch failure] statuses {st}", flush=True)
+        assert st[0][0] == 0 and st[2][0] == 0 and st[1][0] != 0
+        assert "injected" in st[1][1]
+        with pytest.raises(_lib.TplsError, match="earlier step"):
+            hs[1].step(1)
+        for q in (0, 2):
+            for f in FIELDS:
+                assert np.array_equal(hs[q].field(f), want[q][f]), (q, f)
+    finally:
+        for c in hs:
+            c.close()

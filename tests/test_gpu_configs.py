@@ -1,0 +1,194 @@
+"""The benchmark workloads at their own sizes, through the launch options the
+benchmark uses (autotuned launch shapes, HIP-graph replay, multigrid CG).
+
+* 512x128x256 plain FP64 (Re 500, viscosity ratio 10, dt 5e-5) against the
+  CPU oracle's step driver for two steps: 1e-12 normwise relative after the
+  first step, 1e-11 after the second, CG iterations within one.
+* The same grid in FP32 storage against the FP64 GPU run over 500 steps:
+  phi and velocity within 1e-5 relative (the precision variant's stated
+  bound), the pressure within 1e-3 (modulo its mean), the liquid volume
+  within the measured drift.
+* The 64-case batch of the batched workload (the benchmark's seed-0 draw over
+  the 64x16x32 .. 256x64x128 grids, FP32): every case batched equals the same
+  case alone, bit for bit; one case of each grid in FP64 against the oracle
+  after one step (1e-12).
+* The serving mix (Poisson arrivals of the batched distribution plus 5% at
+  512x128x256, here with a 512x128x256 request among them): every request's
+  result in the mix equals its result alone, bit for bit.
+"""
+import dataclasses
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import binding as B
+from tests.compare import demean, relerr, rows
+from tpls_amd.batch import GRIDS, batch_configs
+from tpls_amd.config import CHANNEL_512x128x256
+from tpls_amd.initial import make_initial
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u2", "v2", "w2", "phi2")
+BENCH_OPTS = dict(graphs=True, tiles="auto")
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def compare(gpu, orc, p):
+    P = p.maxm - 1
+    errs = {f: relerr(rows(gpu.field(f), P), rows(orc.field(f), P)) for f in FIELDS}
+    errs["pres"] = relerr(rows(demean(gpu.field("pres"), p.maxl, P, p.maxn), P),
+                          rows(demean(orc.field("pres"), p.maxl, P, p.maxn), P))
+    return errs
+
+
+def test_512x128x256_fp64_two_steps_against_the_oracle():
+    from tpls_amd.case import Case
+    p = dataclasses.replace(CHANNEL_512x128x256, ksp_pc="mg")
+    B.set_threads(threads())
+    init = make_initial(p)
+    gpu = Case(p, precision="fp64", **BENCH_OPTS)
+    gpu.load(*init)
+    orc = B.OracleState(p.maxl, p.maxm, p.maxn, p.solver_dict())
+    orc.load(*init)
+    del init
+    try:
+        for s, tol in ((1, 1e-12), (2, 1e-11)):
+            t0 = time.time()
+            d_o = orc.step(1)
+            t1 = time.time()
+            gpu.step(1)
+            d_g = gpu.diag()
+            errs = compare(gpu, orc, p)
+            print(f"[512x128x256 fp64] step {s} its {d_o['ksp_its']}/{d_g['ksp_its']} oracle {t1 - t0:.1f}s "
+                  + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+            assert abs(d_o["ksp_its"] - d_g["ksp_its"]) <= 1
+            assert d_g["ksp_reason"] in (2, 3), d_g
+            for k, v in errs.items():
+                assert v < tol, (s, k, v)
+            # the residual prints of the Helmholtz loop (get_difference)
+            for k in ("err_u", "err_v", "err_w", "err_phi"):
+                assert d_g[k] == pytest.approx(d_o[k], rel=1e-6), (s, k, d_g[k], d_o[k])
+    finally:
+        gpu.close()
+        orc.close()
+
+
+def test_512x128x256_fp32_against_fp64_over_500_steps():
+    from tpls_amd.case import Case
+    p = dataclasses.replace(CHANNEL_512x128x256, ksp_pc="mg")
+    init = make_initial(p)
+    c64 = Case(p, precision="fp64", diagnostics=False, **BENCH_OPTS)
+    c32 = Case(p, precision="fp32", diagnostics=False, **BENCH_OPTS)
+    c64.load(*init)
+    c32.load(*init)
+    del init
+    P = p.maxm - 1
+    worst = {}
+    done = 0
+    try:
+        for s in (1, 100, 250, 500):
+            c64.step(s - done)
+            c32.step(s - done)
+            done = s
+            line = []
+            for f in FIELDS:
+                e = relerr(rows(c32.field(f), P), rows(c64.field(f), P))
+                worst[f] = max(worst.get(f, 0.0), e)
+                line.append(f"{f}={e:.1e}")
+            e = relerr(rows(demean(c32.field("pres"), p.maxl, P, p.maxn), P),
+                       rows(demean(c64.field("pres"), p.maxl, P, p.maxn), P))
+            worst["pres"] = max(worst.get("pres", 0.0), e)
+            v64, v32 = c64.volume(), c32.volume()
+            dv = abs(v32 - v64) / v64
+            worst["volume"] = max(worst.get("volume", 0.0), dv)
+            print(f"[512x128x256 fp32] step {s} " + " ".join(line) + f" pres={e:.1e} volume={dv:.2e}",
+                  flush=True)
+    finally:
+        c64.close()
+        c32.close()
+    for f in FIELDS:
+        assert worst[f] < 1e-5, (f, worst[f])
+    assert worst["pres"] < 1e-3, worst["pres"]
+    assert worst["volume"] < 1e-6, worst["volume"]
+
+
+def _run_cases(cfgs, precision, steps, together):
+    from tpls_amd.case import Case, batch_step
+    cases = []
+    try:
+        for p in cfgs:
+            c = Case(p, precision=precision, diagnostics=False, **BENCH_OPTS)
+            cases.append(c)
+            c.load(*make_initial(p))
+        if together:
+            batch_step(cases, steps)
+        else:
+            for c in cases:
+                c.step(steps)
+        return [({f: c.field(f) for f in FIELDS + ("pres",)}, c.diag()["ksp_its"]) for c in cases]
+    finally:
+        for c in cases:
+            c.close()
+
+
+def test_64_case_batch_equals_each_case_alone():
+    cfgs = batch_configs(64, seed=0)
+    assert {(p.nx, p.ny, p.nz) for p in cfgs} == set(GRIDS)
+    alone = _run_cases(cfgs, "fp32", 3, together=False)
+    print(f"[batch 64] alone done: {sum(p.cells for p in cfgs)} cells", flush=True)
+    mixed = _run_cases(cfgs, "fp32", 3, together=True)
+    for q, ((fa, ia), (fb, ib)) in enumerate(zip(alone, mixed)):
+        assert ia == ib, (q, ia, ib)
+        for f in fa:
+            assert np.array_equal(fa[f], fb[f]), (q, cfgs[q].nx, f)
+    print("[batch 64] batched == alone for all 64 cases", flush=True)
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_batch_grid_fp64_one_step_against_the_oracle(grid):
+    """The first case of the seed-0 draw on each grid, in FP64."""
+    from tpls_amd.case import Case
+    p = next(q for q in batch_configs(64, seed=0) if (q.nx, q.ny, q.nz) == grid)
+    B.set_threads(threads())
+    init = make_initial(p)
+    orc = B.OracleState(p.maxl, p.maxm, p.maxn, p.solver_dict())
+    orc.load(*init)
+    with Case(p, precision="fp64", **BENCH_OPTS) as gpu:
+        gpu.load(*init)
+        d_o = orc.step(1)
+        gpu.step(1)
+        d_g = gpu.diag()
+        errs = compare(gpu, orc, p)
+    orc.close()
+    print(f"[batch grid {grid}] Re {p.Re:.0f} ratio {p.mu_plus:.2f} dt {p.dt:.2e} its {d_o['ksp_its']}/"
+          f"{d_g['ksp_its']} " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    assert abs(d_o["ksp_its"] - d_g["ksp_its"]) <= 1
+    for k, v in errs.items():
+        assert v < 1e-12, (k, v)
+
+
+def test_serving_mix_with_a_512x128x256_request():
+    from tpls_amd.case import Case
+    from tpls_amd.serve import Server, poisson_requests
+    reqs = poisson_requests(rate=20.0, duration=0.5, seed=12, big_fraction=0.05, steps=(3, 8))
+    assert any(r.params.nx == 512 for r in reqs)
+    assert len({r.params.nx for r in reqs}) >= 4          # 64, 192, 256 and 512 cells in x
+    for r in reqs:
+        r.keep = ("phi2", "u2", "w2", "pres")
+    srv = Server(200e9, graphs=True)
+    rep = srv.run(reqs)
+    print("[serve mix]", rep, flush=True)
+    assert rep["requests_finished"] == len(reqs)
+    assert rep["batch_size_max"] >= 2
+    for r in reqs:
+        with Case(r.params, precision=r.precision, diagnostics=False, graphs=True, tiles="auto") as c:
+            c.load(*make_initial(r.params))
+            c.step(r.steps)
+            for f in r.keep:
+                assert np.array_equal(c.field(f), r.result[f]), (r.rid, r.params.nx, f)

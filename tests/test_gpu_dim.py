@@ -158,6 +158,29 @@ def test_dim_fused_sweep_equals_two_launch_sweep(precision, shape):
     print(f"[dim fused] {precision} {nx}x{ny}x{nz}: 3 steps identical", flush=True)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dim_sweep_chunk_length_changes_no_value(precision):
+    """The fused sweep marches each block up a chunk of levels with a
+    one-level register prefetch and an LDS ring; chunks of 1, 2, 8, 17 levels
+    and the whole column give the two-launch sweeps' fields bit for bit."""
+    from tpls_amd.case import Case
+    p = CaseParams(nx=64, ny=16, nz=40, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=14, idm="dim", dim_its=5)
+    init = make_initial(p)
+    with Case(p, precision=precision, dim_split=True) as sp:
+        sp.load(*init)
+        sp.step(2)
+        want = {f: sp.field(f) for f in ("phi2", "convprev_phi", "u2", "pres")}
+        werr = sp.diag()["err_phi"]
+    for ks in (1, 2, 8, 17, p.nz):
+        with Case(p, precision=precision, march_levels=ks) as f:
+            f.load(*init)
+            f.step(2)
+            assert f.diag()["err_phi"] == werr, ks
+            for fld, w in want.items():
+                assert np.array_equal(f.field(fld), w), (ks, fld)
+    print(f"[dim chunk] {precision}: chunks of 1/2/8/17/{p.nz} levels identical", flush=True)
+
+
 def test_dim_volume_uses_the_shifted_phase():
     from tpls_amd.case import Case
     p = CaseParams(nx=32, ny=16, nz=24, seed=4, idm="dim")

@@ -165,6 +165,30 @@ def test_jacobi_and_sor(L, comp):
     assert all(exact), exact
 
 
+@pytest.mark.parametrize("comp", ["uv", "w"])
+def test_get_difference(L, comp):
+    """get_difference (jacobi_iteration_allflux.f90:285-311) as main calls it
+    after each component's Helmholtz loop (main_ns_hybrid.F90:803, 840, 882):
+    the sum of squared differences over i in sx..ex, j in sy..ey and every k
+    of the array.  The GPU sums in a fixed tree order, the oracle
+    sequentially, so the two agree to rounding, and both agree with numpy's
+    sum of the same squares."""
+    S = shapes(MAXL, MAXM, MAXN)
+    shp = S[comp]
+    ez = shp[2] - 1
+    r = np.random.default_rng(28)
+    a, b = rand(r, shp), rand(r, shp)
+    b[3, 4, 5] = a[3, 4, 5] + 1e3          # one dominant term
+    do = B.oracle("get_difference", a, b, EX, EY, ez, SX, SY, 0)
+    dg = L.call("get_difference", a, b, EX, EY, ez, SX, SY, 0)[0]
+    dn = float(((a[SX:EX + 1, SY:EY + 1, :] - b[SX:EX + 1, SY:EY + 1, :]) ** 2).sum())
+    report("get_difference_" + comp, rel=abs(dg - do) / do)
+    assert dg == pytest.approx(do, rel=1e-13)
+    assert dg == pytest.approx(dn, rel=1e-13)
+    # identical arrays: exactly zero
+    assert L.call("get_difference", a, a.copy(order="F"), EX, EY, ez, SX, SY, 0)[0] == 0.0
+
+
 def test_pressure_pieces(L):
     S = shapes(MAXL, MAXM, MAXN)
     r = np.random.default_rng(27)

@@ -41,6 +41,11 @@ def run_pair(p: CaseParams, nsteps: int, check_every: int, tol, label):
         t2 = time.time()
         assert abs(d_o["ksp_its"] - d_g["ksp_its"]) <= 1, (s, d_o["ksp_its"], d_g["ksp_its"])
         assert d_g["ksp_reason"] in (2, 3), d_g
+        # the residual prints: get_difference of each velocity's last two
+        # Helmholtz iterates (main_ns_hybrid.F90:803-808, 840-845, 882-887)
+        # and the reinitialisation's err2 (main_ns_hybrid.F90:603-605)
+        for k in ("err_u", "err_v", "err_w", "err_phi"):
+            assert d_g[k] == pytest.approx(d_o[k], rel=1e-6), (s, k, d_g[k], d_o[k])
         if s == 1 or s % check_every == 0 or s == nsteps:
             errs = {}
             for f in FIELDS:

@@ -65,3 +65,30 @@ def test_fused_sor_pass_matches_the_two_sweeps(grid, precision):
         for f in FIELDS:
             assert np.array_equal(got[f], ref[f]), (grid, "jst", rb, f)
     print(f"[sor pass] {grid} {precision}: fused == split for tile rows 2/4/8", flush=True)
+
+
+# a column of 40 levels (u, v: 40, w: 39 updated levels per column)
+TALL = CaseParams(nx=64, ny=16, nz=41, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=24, mom_u=8, mom_v=8,
+                  mom_w=8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_march_chunk_length_changes_no_value(precision):
+    """The staged Jacobi and even-parity SOR sweeps march each block up a
+    chunk of levels, prefetching 8 levels ahead into registers and one plane
+    ahead into LDS.  Chunks shorter than, equal to and longer than the
+    prefetch depth (1, 8, 9, 17 levels), and one chunk of the whole column,
+    give the fields of the unstaged sweeps bit for bit."""
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    p = TALL
+    init = make_initial(p)
+    ref = run(p, init, precision, (4, 4, 4, 8, 8, -1), True)
+    for ks in (1, 8, 9, 17, p.nz):
+        for jst in (4, 8):
+            with Case(p, precision=precision, tiles=(4, 4, 4, 8, 8, jst), sor_fused=False, march_levels=ks) as c:
+                c.load(*init)
+                c.step(3)
+                for f in FIELDS:
+                    assert np.array_equal(c.field(f), ref[f]), (ks, jst, f)
+    print(f"[march chunk] {precision}: chunks of 1/8/9/17/{p.nz} levels identical", flush=True)

@@ -36,8 +36,9 @@ def fused_code(sor_fused) -> int:
 
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                 graphs: bool = False, tiles=None, sor_fused=None, mg_fp32=None,
-                dim_split: bool = False) -> _lib.Params:
+                dim_split: bool = False, march_levels: int = 0) -> _lib.Params:
     prm = _lib.Params()
+    prm.march_levels = int(march_levels)
     prm.mg_fp32 = {None: 0, True: 1, False: -1}[mg_fp32]
     prm.dim_split = 1 if dim_split else 0
     for k, v in p.solver_dict().items():
@@ -65,7 +66,7 @@ FILE_FIELDS = ("dx", "dy", "dz", "dt", "Re", "mu_plus", "mu_minus", "height", "s
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                  graphs: bool = False, tiles=None, sor_fused=None, file_params: _lib.Params | None = None,
-                 mg_fp32=None, dim_split: bool = False):
+                 mg_fp32=None, dim_split: bool = False, march_levels: int = 0):
         """graphs: replay each step's launches from HIP graphs (same results).
         tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst[, sor_fused]]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
@@ -76,7 +77,9 @@ class Case:
         values.  mg_fp32: precision of the multigrid V-cycle (None: the
         case's precision; True: FP32; False: FP64); the CG around it stays
         FP64.  dim_split: with idm dim, run each relaxation sweep as two
-        launches instead of one fused pass (same values).  file_params: the tpls_params of a configuration
+        launches instead of one fused pass (same values).  march_levels: levels
+        per block of the k-marching sweeps (0: sized to the device; same
+        values).  file_params: the tpls_params of a configuration
         (tpls_amd.io.read_configuration), whose file-backed fields replace
         those params implies."""
         if tiles == "auto":
@@ -87,7 +90,8 @@ class Case:
         self.precision = precision
         self.shape = (params.maxl, params.maxm, params.maxn)
         self.tiles = tiles
-        self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused, mg_fp32, dim_split)
+        self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused, mg_fp32, dim_split,
+                                march_levels)
         if file_params is not None:
             for k in FILE_FIELDS:
                 setattr(self._prm, k, getattr(file_params, k))
@@ -183,7 +187,7 @@ class Case:
     def sync(self) -> None:
         _lib.check(_lib.lib().tpls_case_sync(self._h), "tpls_case_sync")
 
-    TIMED = {None: 0, False: 0, "cg_x": 1, True: 1, "jacobi": 2, "sor_odd": 3, "sor_even": 4}
+    TIMED = {None: 0, False: 0, "cg_x": 1, True: 1, "jacobi": 2, "sor_odd": 3, "sor_even": 4, "advect": 5}
 
     def time_kernels(self, kernel=True) -> None:
         """Bracket every launch of one kernel class with HIP events on the
@@ -232,3 +236,16 @@ def batch_step(cases, nsteps: int = 1) -> None:
     Case.step would compute for it alone."""
     arr = (C.c_void_p * len(cases))(*[c._h for c in cases])
     _lib.check(_lib.lib().tpls_batch_step(arr, len(cases), int(nsteps)), "tpls_batch_step")
+
+
+def batch_step_status(cases, nsteps: int = 1) -> list:
+    """batch_step that isolates failures (tpls_batch_step_status): returns one
+    status per case, 0 or the error code of that case's step (its message
+    then in the second element of the pair); a failed case stops, the others
+    complete their steps."""
+    n = len(cases)
+    arr = (C.c_void_p * n)(*[c._h for c in cases])
+    st = (C.c_int * n)()
+    rc = _lib.lib().tpls_batch_step_status(arr, n, int(nsteps), st)
+    msg = _lib.lib().tpls_last_error().decode(errors="replace") if rc else ""
+    return [(int(v), msg if v else "") for v in st]

@@ -7,6 +7,7 @@
 // cells of a block, instead of being written to and re-read from dfx/dfy/dfz
 // arrays.  HBM traffic per cell: phi2, u2, v2,
 // w2 read once, conv2_phi written once (40 B in FP64).
+#include "tiles.hpp"
 #include "tpls_kernels.hpp"
 
 namespace tpls {
@@ -80,54 +81,177 @@ __device__ __forceinline__ C face_z(const FieldView<T>& phi, int I, int pj, int 
     return (C(phi(I, pj, K)) + C(phi(I, pj, K + 1))) / C(2);
 }
 
-// Levels per thread of the advection kernel: the top face of one level is the
-// bottom face of the next.
-constexpr int ADV_KZ = 4;
+// The advection kernel: a block of 64 x ADV_TY threads owns a tile of 64 x
+// ADV_TY p columns and marches it up a chunk of levels.  phi2 is staged
+// plane by plane into an LDS ring of four planes (levels k .. k+3 while level
+// k is computed), each with the three-cell halo the WENO5 stencils need in x
+// and y (ADV_W x ADV_H values); the next plane is loaded into registers one
+// level ahead.  Each thread keeps the six values of its own column that the
+// z stencils need (levels k-2 .. k+3) in registers, taken from the ring as
+// planes arrive.  Per level the block reconstructs each x face of its rows
+// (65 per row: the east face of every column and the west face of the
+// first), each y face (ADV_TY + 1 per column) and each column's top face
+// once, multiplies them by their face velocity, and forms conv2_phi from the
+// flux differences (the bottom flux is the level below's top flux).  The
+// velocities of the next level are loaded while this one is computed.  HBM
+// traffic per cell: phi2, u2, v2, w2 read once (the halo re-reads of
+// neighbouring tiles hit L2), conv2_phi written once.
+constexpr int ADV_TY = 8;
+constexpr int ADV_NT = BX * ADV_TY;
+constexpr int ADV_W = BX + 6, ADV_H = ADV_TY + 6;
+constexpr int ADV_NXF = (BX + 1) * ADV_TY;                     // x faces per level
+constexpr int ADV_NYF = BX * (ADV_TY + 1);                     // y faces per level
+constexpr int ADV_NS = (ADV_W * ADV_H + ADV_NT - 1) / ADV_NT;  // staged values per thread
+constexpr int ADV_KS = 32;                                     // levels per chunk when unsized
 
-// conv2_phi of ADV_KZ cells stacked in k.  Each face is reconstructed once per
-// thread column: the west face is the east face of the lane to the left
-// (shuffle), the south face the north face of the row below (LDS), the bottom
-// face the previous level's top face; the first lane, the first row and the
-// first level compute their own.  Lanes past the grid's edge work on clamped
-// indices and store nothing, so every lane takes part in the exchanges.
+__device__ __forceinline__ int wrap_any(int j, int P) { return ((j - 1) % P + P) % P + 1; }
+
+// Face value between LDS positions c and c + st (a row or a column of a
+// staged plane): WENO5 from the upwind side (weno.f90:45-76, 104-133).
+template <typename C>
+__device__ __forceinline__ C weno_lds(const C* a, int c, int st, bool up)
+{
+    const int b = up ? c : c + st, s = up ? st : -st;
+    return weno5f<C>(a[b - 2 * s], a[b - s], a[b], a[b + s], a[b + 2 * s]);
+}
+
+// The z face between levels K and K+1 from a column's values z[m] = phi(K-2+m),
+// m = 0..5 (weno.f90:135-167).
+template <typename C>
+__device__ __forceinline__ C zface(const C (&z)[6], int K, int n, C ww)
+{
+    if (K >= 3 && K <= n - 3) {
+        const bool up = ww >= C(0);
+        return weno5f<C>(up ? z[0] : z[5], up ? z[1] : z[4], up ? z[2] : z[3], up ? z[3] : z[2], up ? z[4] : z[1]);
+    }
+    return (z[2] + z[3]) / C(2);
+}
+
 template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_advect_phi(Grid g, T dx, T dy, T dz, FieldView<T> phi,
-                                                       FieldView<T> u, FieldView<T> v, FieldView<T> w,
-                                                       FieldView<T> conv)
+__global__ void __launch_bounds__(ADV_NT) k_advect_phi(Grid g, int nbx, int nby, int ks, T dx, T dy, T dz,
+                                                       FieldView<T> phi, FieldView<T> u, FieldView<T> v,
+                                                       FieldView<T> w, FieldView<T> conv)
 {
     using C = T;
-    __shared__ C fn_s[BY][BX];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = blockIdx.x * BX + tx + 1;
-    const int j = blockIdx.y * BY + ty + 1;
-    const int k0 = blockIdx.z * ADV_KZ + 1;
-    const int ic = min(i, g.maxl - 1), jc = min(j, g.P);
-    const bool col = i <= g.maxl - 1 && j <= g.P;
-    const int jm = wrapj(jc - 1, g.P);
-    C ft = C(0), wt = C(0);
+    __shared__ C ring[4][ADV_H * ADV_W];
+    __shared__ C fx_s[ADV_TY][BX + 1];  // u * dfx on the x faces of the tile's rows
+    __shared__ C fy_s[ADV_TY + 1][BX];  // v * dfy on the y faces of its columns
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int tid = threadIdx.x, tx = tid % BX, ty = tid / BX;
+    const int L = g.maxl, n = g.maxn, P = g.P;
+    const int i0 = 1 + bx * BX, j0 = 1 + by * ADV_TY;
+    const int kb = 1 + bz * ks, ke = min(kb + ks - 1, n - 1);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool mine = i <= L - 1 && j <= P;
+    const int ic = min(i, L - 1), jm = wrap_any(j - 1, P);
+    const long sz = phi.sz;
+    // element offsets (level 0) of this thread's share of the staged window:
+    // x from i0-3 clamped to the array, y rows j0-3 .. periodic
+    long soff[ADV_NS];
 #pragma unroll
-    for (int kk = 0; kk < ADV_KZ; ++kk) {
-        const int k = min(k0 + kk, g.maxn - 1);
-        // advect_phi.f90:330-334
-        const C ue = u(ic, jm, k - 1);
-        const C vn = v(ic, jc, k - 1), vs = v(ic, jm, k - 1);
-        const C wb = kk ? wt : C(w(ic, jm, k - 1));
-        wt = w(ic, jm, k);
-        const C fe = face_x<C>(phi, ic, jc, k, ue, g.maxl);
-        C uw = __shfl_up(ue, 1, 64), fw = __shfl_up(fe, 1, 64);
-        if (tx == 0) {
-            uw = u(ic - 1, jm, k - 1);
-            fw = face_x<C>(phi, ic - 1, jc, k, uw, g.maxl);
+    for (int m = 0; m < ADV_NS; ++m) {
+        const int q = min(tid + m * ADV_NT, ADV_W * ADV_H - 1);
+        const int li = q % ADV_W, lj = q / ADV_W;
+        soff[m] = min(max(i0 - 3 + li, 0), L) + wrap_any(j0 - 3 + lj, P) * phi.sy;
+    }
+    auto fetch = [&](int k, C (&r)[ADV_NS]) {
+        const long kz = long(min(max(k, 0), n)) * sz;
+#pragma unroll
+        for (int m = 0; m < ADV_NS; ++m) r[m] = phi.p[soff[m] + kz];
+    };
+    auto put = [&](int k, const C (&r)[ADV_NS]) {
+        C* a = ring[k & 3];
+#pragma unroll
+        for (int m = 0; m < ADV_NS; ++m) {
+            const int q = tid + m * ADV_NT;
+            if (q < ADV_W * ADV_H) a[q] = r[m];
         }
-        const C fn = face_y<C>(phi, ic, jc, k, vn, g.P);
-        fn_s[ty][tx] = fn;
+    };
+    // this thread's faces: x faces tid and tid + ADV_NT (when < ADV_NXF), y
+    // faces likewise; LDS positions and velocity offsets (advect_phi.f90:330-334:
+    // the x face I of p row jr carries u(I, jr-1, k-1), the y face between p
+    // rows jr and jr+1 carries v(I, jr, k-1), the top face w(i, j-1, k))
+    int xc[2], yc[2];
+    bool xw[2];
+    long uoff[2], voff[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int fx = min(tid + h * ADV_NT, ADV_NXF - 1), fy = min(tid + h * ADV_NT, ADV_NYF - 1);
+        const int Ix = i0 - 1 + fx % (BX + 1), rx = fx / (BX + 1);
+        const int Iy = i0 + fy % BX, ry = fy / BX;
+        xc[h] = (rx + 3) * ADV_W + (Ix - i0 + 3);
+        xw[h] = Ix >= 3 && Ix <= L - 3;
+        uoff[h] = min(Ix, L) + wrap_any(j0 + rx - 1, P) * u.sy;
+        yc[h] = (ry + 2) * ADV_W + (Iy - i0 + 3);
+        voff[h] = min(Iy, L) + wrap_any(j0 - 1 + ry, P) * v.sy;
+    }
+    const bool x2 = tid + ADV_NT < ADV_NXF, y2 = tid + ADV_NT < ADV_NYF;
+    const long woff = ic + jm * w.sy;
+    C ux[2], vy[2], wt, ux_n[2] = {C(0), C(0)}, vy_n[2] = {C(0), C(0)}, wt_n = C(0);
+    auto load_vel = [&](int k, C (&uu)[2], C (&vv)[2], C& ww) {
+        const long kz = long(k - 1) * u.sz;
+        uu[0] = u.p[uoff[0] + kz];
+        uu[1] = u.p[uoff[1] + kz];
+        vv[0] = v.p[voff[0] + long(k - 1) * v.sz];
+        vv[1] = v.p[voff[1] + long(k - 1) * v.sz];
+        ww = w.p[woff + long(k) * w.sz];
+    };
+    // prologue: planes kb .. kb+2 into the ring, the column window phi(kb-3 ..
+    // kb+2) and the bottom flux of level kb (the face K = kb-1)
+    C z[6];
+    {
+        C r[ADV_NS];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            fetch(kb + q, r);
+            put(kb + q, r);
+        }
+        const long col = ic + wrap_any(j, P) * phi.sy;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) z[m] = phi.p[col + long(min(max(kb - 3 + m, 0), n)) * sz];
+    }
+    const C wb = w.p[woff + long(kb - 1) * w.sz];
+    C hb = wb * zface<C>(z, kb - 1, n, wb);
+    C nxt[ADV_NS];
+    fetch(kb + 3, nxt);
+    load_vel(kb, ux, vy, wt);
+    for (int k = kb; k <= ke; ++k) {
+        // the ring slot of level k+3 held level k-1, last read before the
+        // previous level's second barrier
+        put(k + 3, nxt);
+        if (k < ke) {
+            fetch(k + 4, nxt);
+            load_vel(k + 1, ux_n, vy_n, wt_n);
+        }
         __syncthreads();
-        const C fs = ty ? fn_s[ty - 1][tx] : face_y<C>(phi, ic, jc - 1, k, vs, g.P);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) z[m] = z[m + 1];
+        z[5] = ring[(k + 3) & 3][(ty + 3) * ADV_W + tx + 3];
+        const C* pl = ring[k & 3];
+        {
+            const bool up = ux[0] >= C(0);
+            const C f = xw[0] ? weno_lds<C>(pl, xc[0], 1, up) : pl[up ? xc[0] : xc[0] + 1];
+            (&fx_s[0][0])[tid] = ux[0] * f;
+        }
+        if (x2) {
+            const bool up = ux[1] >= C(0);
+            const C f = xw[1] ? weno_lds<C>(pl, xc[1], 1, up) : pl[up ? xc[1] : xc[1] + 1];
+            (&fx_s[0][0])[tid + ADV_NT] = ux[1] * f;
+        }
+        (&fy_s[0][0])[tid] = vy[0] * weno_lds<C>(pl, yc[0], ADV_W, vy[0] >= C(0));
+        if (y2) (&fy_s[0][0])[tid + ADV_NT] = vy[1] * weno_lds<C>(pl, yc[1], ADV_W, vy[1] >= C(0));
+        const C ht = wt * zface<C>(z, k, n, wt);
         __syncthreads();
-        const C fb = kk ? ft : face_z<C>(phi, ic, jc, k - 1, wb, g.maxn);
-        ft = face_z<C>(phi, ic, jc, k, wt, g.maxn);
-        if (col && k0 + kk <= g.maxn - 1)
-            conv(i, j, k) = (ue * fe - uw * fw) / dx + (vn * fn - vs * fs) / dy + (wt * ft - wb * fb) / dz;
+        if (mine)
+            conv(i, j, k) = (fx_s[ty][tx + 1] - fx_s[ty][tx]) / dx + (fy_s[ty + 1][tx] - fy_s[ty][tx]) / dy +
+                            (ht - hb) / dz;
+        hb = ht;
+        ux[0] = ux_n[0];
+        ux[1] = ux_n[1];
+        vy[0] = vy_n[0];
+        vy[1] = vy_n[1];
+        wt = wt_n;
     }
 }
 
@@ -155,10 +279,11 @@ template <typename T>
 hipError_t launch_advect_phi(hipStream_t s, const Grid& g, const Phys& ph, FieldView<T> phi2,
                              FieldView<T> u2, FieldView<T> v2, FieldView<T> w2, FieldView<T> conv2_phi)
 {
-    dim3 blk(BX, BY, 1);
-    dim3 grd((g.maxl - 1 + BX - 1) / BX, (g.P + BY - 1) / BY, (g.maxn - 1 + ADV_KZ - 1) / ADV_KZ);
-    hipLaunchKernelGGL(k_advect_phi<T>, grd, blk, 0, s, g, T(ph.dx), T(ph.dy), T(ph.dz), phi2, u2, v2, w2,
-                       conv2_phi);
+    const int nbx = (g.maxl - 1 + BX - 1) / BX, nby = (g.P + ADV_TY - 1) / ADV_TY, nz = g.maxn - 1;
+    static const int slots = resident_blocks(k_advect_phi<T>, ADV_NT);
+    const int ks = march_chunk(nbx * nby, nz, slots, ADV_KS), nbz = (nz + ks - 1) / ks;
+    hipLaunchKernelGGL(k_advect_phi<T>, dim3(nbx * nby * nbz), dim3(ADV_NT), 0, s, g, nbx, nby, ks, T(ph.dx),
+                       T(ph.dy), T(ph.dz), phi2, u2, v2, w2, conv2_phi);
     return hipGetLastError();
 }
 

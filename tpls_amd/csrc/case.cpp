@@ -102,6 +102,15 @@ long pitch_for(long nx, int esz) { return (nx * esz + 127) / 128 * 128 / esz; }
 // ---------------------------------------------------------------------------
 struct tpls_case {
     int maxl, maxm, maxn, prec;
+    // a step of this case failed part-way (its buffer roles may be half
+    // rotated): the case refuses further steps
+    int broken = 0;
+    std::string broken_msg;
+    int fail_next = 0;  // test hook: the next step_begin fails before any launch
+    // the first step's launches and the first CG chunk run eagerly, so that
+    // every launcher's one-time device queries (resident_blocks) happen
+    // before any stream capture, never inside one
+    int pre_warm = 0, cg_warm = 0;
     tpls_params prm;
     Grid g;
     Phys ph;
@@ -486,6 +495,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) || (prm->tile_rows_jst != -1 && !rb_ok(prm->tile_rows_jst)))
         return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
                                   "tile_rows_jst -1 (unstaged), 0 (default), 2, 4 or 8");
+    if (prm->march_levels < 0) return fail(TPLS_ERR_ARG, "march_levels must be 0 (sized to the device) or >= 1");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
         const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
@@ -513,6 +523,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->tile_rows_rb) c->tiling.rb_rows = prm->tile_rows_rb;
     if (prm->tile_rows_rb_odd) c->tiling.rb_rows_odd = prm->tile_rows_rb_odd;
     c->tiling.jst = prm->tile_rows_jst < 0 ? 0 : (prm->tile_rows_jst ? prm->tile_rows_jst : 4);
+    c->tiling.ks = prm->march_levels;
     int rc = TPLS_OK;
     auto bail = [&](int r) {
         tpls_case_destroy(c);
@@ -822,7 +833,7 @@ int tpls_case_sync(tpls_case* c)
 int tpls_case_time_kernels(tpls_case* c, int kernel_class)
 {
     if (!c) return fail(TPLS_ERR_ARG, "null case");
-    if (kernel_class < TPLS_TIME_OFF || kernel_class > TPLS_TIME_SOR_EVEN) return fail(TPLS_ERR_ARG, "bad kernel class");
+    if (kernel_class < TPLS_TIME_OFF || kernel_class > TPLS_TIME_ADVECT) return fail(TPLS_ERR_ARG, "bad kernel class");
     HIPCHK(hipStreamSynchronize(c->s));
     c->timed = kernel_class;
     c->t_ms = 0.0;
@@ -883,9 +894,14 @@ int tpls_case_volume(tpls_case* c, double* volume)
 int tpls_step(tpls_case* c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(TPLS_ERR_ARG, "bad argument");
+    if (c->broken) return fail(TPLS_ERR_ARG, "the case failed in an earlier step: " + c->broken_msg);
     for (int n = 0; n < nsteps; ++n) {
         int rc = c->prec == TPLS_FP32 ? step_impl<float>(c) : step_impl<double>(c);
-        if (rc) return rc;
+        if (rc) {
+            c->broken = 1;
+            c->broken_msg = g_err;
+            return rc;
+        }
     }
     return TPLS_OK;
 }
@@ -980,7 +996,8 @@ int ps_enqueue_chunk(tpls_case* c)
         if (mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
         return TPLS_OK;
     };
-    const bool graphs = c->prm.graphs && !c->timed;
+    const bool graphs = c->prm.graphs && !c->timed && c->cg_warm;
+    c->cg_warm = 1;
     for (long q = 0; q < chunk; ++q) {
         if (graphs) {
             hipGraphExec_t& ex = c->it_graph[c->ps_cur];
@@ -1132,8 +1149,11 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
     auto V = [](DevArray* a) { return a->view<T>(); };
 
     // ---- do_advect_phi (main:585) ----
+    const bool ta = c->timed == TPLS_TIME_ADVECT;
+    if (ta) LCHK(hipEventRecord(next_event(c), s));
     LCHK(launch_advect_phi<T>(s, g, ph, V(c->phi2), V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
                               V(c->cphi[2])));
+    if (ta) LCHK(hipEventRecord(next_event(c), s));
     if (c->prm.idm_dim) {
         // ---- dim (main:606-611): phiNew iterates in pra (and prb, the
         // fused sweeps' ping-pong partner), d2phi in D, source0 in sg;
@@ -1141,7 +1161,8 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
         // the first ----
         LCHK(launch_dim<T>(s, g, ph, c->dimc, c->prm.dim_its, c->steps_done == 0 ? 1 : 0, V(c->phi2), V(c->cphi[2]),
                            V(c->cprev), V(c->phi3), V(c->pra), V(c->prb), V(c->D), V(c->sg),
-                           static_cast<const T*>(c->dim_prof), dg ? c->red_out : nullptr, c->prm.dim_split ? 0 : 1));
+                           static_cast<const T*>(c->dim_prof), dg ? c->red_out : nullptr, c->prm.dim_split ? 0 : 1,
+                           c->tiling.ks));
     } else {
         // ---- do_tpls (main:596) ----
         LCHK(launch_tpls_update<T>(s, g, ph, V(c->phi2), V(c->cphi[0]), V(c->cphi[1]), V(c->cphi[2]), V(c->phi3)));
@@ -1226,9 +1247,15 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
 template <typename T>
 int step_begin(tpls_case* c)
 {
+    if (c->fail_next) {
+        c->fail_next = 0;
+        return fail(TPLS_ERR_ARG, "failure injected by tpls_test_fail_next_step");
+    }
     const StepPlan pl = plan_step(c);
     if (c->use_mg) mg_coefficients(c);
-    if (c->prm.graphs && !c->timed) {
+    const bool warm = c->pre_warm;
+    c->pre_warm = 1;
+    if (c->prm.graphs && !c->timed && warm) {
         std::vector<const void*> key = {c->phi2, c->phi3, c->pra, c->prb, c->cphi[0], c->cphi[1], c->cphi[2]};
         // dim's first step starts its iterate from zero, later ones from phi2
         key.push_back(reinterpret_cast<const void*>(uintptr_t(c->prm.idm_dim && c->steps_done == 0)));
@@ -1919,7 +1946,7 @@ int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, con
 // every phase of the step is enqueued for all cases before the host waits on
 // any of them, so the GPU overlaps the cases' kernels.  Each case computes
 // exactly what tpls_step computes for it alone.
-int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
+int tpls_batch_step_status(tpls_case** cases, int ncases, int nsteps, int* status)
 {
     if (!cases || ncases < 0 || nsteps < 0) return fail(TPLS_ERR_ARG, "bad argument");
     for (int q = 0; q < ncases; ++q) {
@@ -1927,41 +1954,100 @@ int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
         for (int r = 0; r < q; ++r)
             if (cases[r] == cases[q]) return fail(TPLS_ERR_ARG, "a case appears twice in the batch");
     }
+    std::vector<int> st(ncases, TPLS_OK);
     std::vector<char> open(ncases);
+    int first_bad = -1;
+    std::string first_msg;
+    // a case whose step fails stops here: its stream is drained, it is marked
+    // broken, and the other cases go on with their steps
+    auto drop = [&](int q, int rc) {
+        tpls_case* c = cases[q];
+        st[q] = rc;
+        open[q] = 0;
+        c->broken = 1;
+        c->broken_msg = g_err;
+        if (first_bad < 0) {
+            first_bad = q;
+            first_msg = g_err;
+        }
+        (void)hipStreamSynchronize(c->s);
+        (void)hipGetLastError();
+    };
+    for (int q = 0; q < ncases; ++q)
+        if (cases[q]->broken) {
+            fail(TPLS_ERR_ARG, "the case failed in an earlier step: " + cases[q]->broken_msg);
+            drop(q, TPLS_ERR_ARG);
+        }
     for (int n = 0; n < nsteps; ++n) {
+        int left = 0;
         for (int q = 0; q < ncases; ++q) {
+            if (st[q]) continue;
             tpls_case* c = cases[q];
-            RC(c->prec == TPLS_FP32 ? step_begin<float>(c) : step_begin<double>(c));
+            const int rc = c->prec == TPLS_FP32 ? step_begin<float>(c) : step_begin<double>(c);
+            if (rc) {
+                drop(q, rc);
+                continue;
+            }
             open[q] = 1;
+            ++left;
         }
         // serve whichever case's CG chunk has landed, so that no finished case
         // waits behind a slower one for its next chunk; when none has, block
         // on the oldest open one
-        for (int left = ncases, next = 0; left > 0;) {
+        for (int next = 0; left > 0;) {
             bool served = false;
             for (int q = 0; q < ncases; ++q) {
                 if (!open[q]) continue;
                 tpls_case* c = cases[q];
                 const hipError_t e = hipEventQuery(c->ps_ev);
                 if (e == hipErrorNotReady) continue;
-                HIPCHK(e);
-                bool fin = false;
-                RC(ps_poll(c, &fin));
                 served = true;
-                if (fin) {
-                    RC(c->prec == TPLS_FP32 ? step_end<float>(c) : step_end<double>(c));
+                int rc = e == hipSuccess ? TPLS_OK : fail(TPLS_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+                bool fin = false;
+                if (!rc) rc = ps_poll(c, &fin);
+                if (!rc && fin) rc = c->prec == TPLS_FP32 ? step_end<float>(c) : step_end<double>(c);
+                if (rc) {
+                    drop(q, rc);
+                    --left;
+                } else if (fin) {
                     open[q] = 0;
                     --left;
                 }
             }
             if (!served && left > 0) {
                 while (!open[next]) next = (next + 1) % ncases;
-                HIPCHK(hipEventSynchronize(cases[next]->ps_ev));
+                const hipError_t e = hipEventSynchronize(cases[next]->ps_ev);
+                if (e != hipSuccess) {
+                    drop(next, fail(TPLS_ERR_HIP, std::string("hipEventSynchronize: ") + hipGetErrorString(e)));
+                    --left;
+                }
                 next = (next + 1) % ncases;
             }
         }
     }
-    for (int q = 0; q < ncases; ++q) RC(finish_diag(cases[q]));
+    for (int q = 0; q < ncases; ++q)
+        if (!st[q]) {
+            const int rc = finish_diag(cases[q]);
+            if (rc) drop(q, rc);
+        }
+    if (status)
+        for (int q = 0; q < ncases; ++q) status[q] = st[q];
+    if (first_bad >= 0) return fail(st[first_bad], "case " + std::to_string(first_bad) + ": " + first_msg);
+    return TPLS_OK;
+}
+
+int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
+{
+    return tpls_batch_step_status(cases, ncases, nsteps, nullptr);
+}
+
+// Test hook, not part of the public interface: the case's next step fails
+// before it launches anything (host-side), so that failure isolation can be
+// tested without a GPU fault.
+int tpls_test_fail_next_step(tpls_case* c)
+{
+    if (!c) return fail(TPLS_ERR_ARG, "null case");
+    c->fail_next = 1;
     return TPLS_OK;
 }
 

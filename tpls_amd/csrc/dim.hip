@@ -24,6 +24,7 @@
 // reference's operation order.  Every expression rounds as written (no FMA
 // contraction), which makes the FP64 path bit-identical to a sequential
 // evaluation of the reference loops.
+#include <algorithm>
 #include <cmath>
 
 #include "tiles.hpp"
@@ -367,7 +368,7 @@ template <typename T>
 hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, double shift,
                            FieldView<T> phi, FieldView<T> phio, FieldView<T> conv2, FieldView<T> cprev,
                            FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
-                           double* res, int fused)
+                           double* res, int fused, int ks_fixed)
 {
     if (res) {
         hipError_t e = launch_fill<double>(s, res, 1, 0.0);
@@ -380,7 +381,8 @@ hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const D
                        T(c.c_tao), T(c.c_epn), h, phi, d2, conv2, cprev, src);
     const int nbx = (g.maxl - 1 + DIM_TI - 1) / DIM_TI, nby = (g.P + DIM_TJ - 1) / DIM_TJ;
     static const int slots = resident_blocks(k_dim_sweep<T, DIM_TJ>, DIM_TI * DIM_TJ);
-    const int ks = march_chunk(nbx * nby, g.maxn - 1, slots, DIM_KS), nbz = (g.maxn - 1 + ks - 1) / ks;
+    const int ks = ks_fixed > 0 ? std::min(ks_fixed, g.maxn - 1) : march_chunk(nbx * nby, g.maxn - 1, slots, DIM_KS);
+    const int nbz = (g.maxn - 1 + ks - 1) / ks;
     for (int it = 1; it <= its; ++it) {
         unsigned long long* rm = res && it == its ? reinterpret_cast<unsigned long long*>(res) : nullptr;
         if (fused) {
@@ -402,13 +404,14 @@ hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const D
 template <typename T>
 hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, int zero,
                       FieldView<T> phi2, FieldView<T> conv2, FieldView<T> cprev, FieldView<T> phi3, FieldView<T> pn,
-                      FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res, int fused)
+                      FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet, double* res, int fused,
+                      int ks)
 {
     const dim3 blk(BX, BY);
     hipLaunchKernelGGL(k_dim_init<T>, grid_all(g), blk, 0, s, g, zero, phi2, pn);
     if (fused) hipLaunchKernelGGL(k_dim_init<T>, grid_all(g), blk, 0, s, g, zero, phi2, pn2);
     hipError_t e =
-        launch_dim_core<T>(s, g, ph, c, its, 0.5, phi2, phi2, conv2, cprev, pn, pn2, d2, src, inlet, res, fused);
+        launch_dim_core<T>(s, g, ph, c, its, 0.5, phi2, phi2, conv2, cprev, pn, pn2, d2, src, inlet, res, fused, ks);
     if (e != hipSuccess) return e;
     // the last iterate: in pn2 after an odd number of fused sweeps
     hipLaunchKernelGGL(k_dim_final<T>, grid_all(g), blk, 0, s, g, fused && (its & 1) ? pn2 : pn, phi3);
@@ -418,11 +421,11 @@ hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoe
 #define TPLS_INST_DIM(T)                                                                                         \
     template hipError_t launch_dim<T>(hipStream_t, const Grid&, const Phys&, const DimCoef&, int, int,          \
                                       FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>,     \
-                                      FieldView<T>, FieldView<T>, FieldView<T>, const T*, double*, int);        \
+                                      FieldView<T>, FieldView<T>, FieldView<T>, const T*, double*, int, int);   \
     template hipError_t launch_dim_core<T>(hipStream_t, const Grid&, const Phys&, const DimCoef&, int, double,  \
                                            FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>,              \
                                            FieldView<T>, FieldView<T>, FieldView<T>, FieldView<T>, const T*,    \
-                                           double*, int);
+                                           double*, int, int);
 TPLS_INST_DIM(double)
 TPLS_INST_DIM(float)
 

@@ -13,6 +13,7 @@
 // sweeps, the fused pass and the CPU restatement agree bit for bit.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <type_traits>
 
 #include "tiles.hpp"
@@ -951,7 +952,8 @@ hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp,
             const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
             const int nbx = (imax + BX - 1) / BX, nby = g.P / TY;
             static const int slots = resident_blocks(k_jacobi_st<CP, TY, 8, ZERO, T>, BX * TY);
-            const int ks = march_chunk(nbx * nby, nz, slots, 32), nbz = (nz + ks - 1) / ks;
+            const int ks = tl.ks > 0 ? std::min(tl.ks, nz) : march_chunk(nbx * nby, nz, slots, 32);
+            const int nbz = (nz + ks - 1) / ks;
             hipLaunchKernelGGL((k_jacobi_st<CP, TY, 8, ZERO, T>), dim3(nbx * nby * nbz), dim3(BX, TY), 0, s, g, nbx,
                                nby, ks, a2, b2, c2, xold, rhs, mu, x);
             return hipGetLastError();
@@ -1004,7 +1006,8 @@ hipError_t launch_sor(hipStream_t s, const Grid& g, const Phys& ph, int comp, Fi
                 constexpr int TY = decltype(ty_c)::value;
                 const int nby = g.P / TY;
                 static const int slots = resident_blocks(k_sor3d_st<CP, TY, 8, T>, 32 * TY);
-                const int ks = march_chunk(nbx3 * nby, nz, slots, 32), nbz = (nz + ks - 1) / ks;
+                const int ks = tl.ks > 0 ? std::min(tl.ks, nz) : march_chunk(nbx3 * nby, nz, slots, 32);
+                const int nbz = (nz + ks - 1) / ks;
                 hipLaunchKernelGGL((k_sor3d_st<CP, TY, 8, T>), dim3(nbx3 * nby * nbz), dim3(32 * TY), 0, s, g, nbx3,
                                    nby, ks, flg, a2, b2, c2, x, rhs, mu);
                 return hipGetLastError();

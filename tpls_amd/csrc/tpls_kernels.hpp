@@ -76,12 +76,13 @@ void dim_inlet(int maxn, double dz, double height, double epn, double* prof);
 // of phi2.  pn, pn2, d2, src: work arrays of the p shape (pn2 only for the
 // fused sweep); inlet: maxn+1 values on the device; res: if non-null, the
 // last sweep's largest |change|.  fused: each sweep as one out-of-place pass
-// (identical results) instead of the Laplacian and the in-place update.
+// (identical results) instead of the Laplacian and the in-place update; ks:
+// levels per block of that pass (0: sized to the resident blocks).
 template <typename T>
 hipError_t launch_dim(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, int zero,
                       FieldView<T> phi2, FieldView<T> conv2, FieldView<T> cprev, FieldView<T> phi3,
                       FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
-                      double* res, int fused);
+                      double* res, int fused, int ks = 0);
 // The body of dim on given arrays: phi = phi_in + shift, phiOld = phio +
 // shift; pn (and pn2 when fused) hold the initial iterate, boundary planes
 // included; the last iterate ends in pn, or in pn2 after an odd number of
@@ -90,7 +91,7 @@ template <typename T>
 hipError_t launch_dim_core(hipStream_t s, const Grid& g, const Phys& ph, const DimCoef& c, int its, double shift,
                            FieldView<T> phi, FieldView<T> phio, FieldView<T> conv2, FieldView<T> cprev,
                            FieldView<T> pn, FieldView<T> pn2, FieldView<T> d2, FieldView<T> src, const T* inlet,
-                           double* res, int fused);
+                           double* res, int fused, int ks = 0);
 
 // ---- momentum (momentum_allflux.f90, main_ns_hybrid.F90:629-757) ----
 // shift: 0, or 1 for the diffuse interface method, whose viscosity is that
@@ -120,6 +121,9 @@ struct Tiling {
     // Jacobi sweep with staged planes and tiles of jst rows (0: the unstaged
     // sweep with `rows`)
     int jst = 4;
+    // levels per block of the k-marching staged sweeps (0: sized to the
+    // resident blocks by march_chunk)
+    int ks = 0;
 };
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,

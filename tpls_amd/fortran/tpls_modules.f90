@@ -15,8 +15,13 @@
 ! reference's abort() does for a failing MPI or PETSc call.
 !
 ! The pressure solve main_ns_hybrid.F90:896-913 goes through PETSc objects
-! (KSPSolve and the callbacks of pressure_solver.F90); module tpls_pressure
-! offers the whole sequence as one call.
+! (KSPSolve and the callbacks of pressure_solver.F90).  Module tpls_state
+! holds the solver-global data of state.F90 that those callbacks read, and
+! module pressure_solver offers the calls main makes on its KSP object
+! (KSPSolve, KSPGetResidualNorm, KSPGetIterationNumber,
+! KSPGetConvergedReason, output_converged_reason, copy_pressure_out_of_petsc)
+! with main's argument lists, so that block of main compiles as it stands;
+! module tpls_pressure offers the whole sequence as one call.
 
 module tpls_core_c
   use iso_c_binding
@@ -543,6 +548,139 @@ end module pressure
 ! pres, compute_rhs, compute_matrix, KSPSolve, iteration count, residual norm,
 ! converged reason, copy_pressure_out_of_petsc) as one call.  pres holds the
 ! initial guess on entry and the solution with its ghost planes on return.
+! The solver-global data of state.F90:12-53, which the reference's PETSc
+! callbacks (copy_pressure_in_to_petsc, compute_rhs, compute_matrix) read, in
+! plain Fortran types: main_ns_hybrid.F90 allocates and fills these arrays
+! as it does with the reference's module of the same name.
+module tpls_state
+  implicit none
+  integer :: maxl, maxm, maxn
+  integer :: global_dim_x, global_dim_y, global_dim_z
+  double precision, allocatable, dimension(:, :, :) :: pres
+  double precision :: dx, dy, dz, dt
+  double precision, allocatable, dimension(:, :, :) :: u3
+  double precision, allocatable, dimension(:, :, :) :: RHS_p
+  double precision, allocatable, dimension(:) :: u_inlet
+end module tpls_state
+
+
+! The KSP calls of main_ns_hybrid.F90:896-913 and pressure_solver.F90's
+! output_converged_reason (:88-145) and copy_pressure_out_of_petsc
+! (:444-486), with their argument lists.  KSPSolve solves the system that
+! compute_matrix / compute_rhs assemble (pressure_solver.F90:234-441) from
+! tpls_state's RHS_p, u3, u_inlet and spacings, starting from pres as
+! copy_pressure_in_to_petsc does, with conjugate gradients on the GPU (the
+! CG, no-preconditioner configuration of the PETSc options database,
+! constant null space, KSPConvergedReason codes); the solution stays in the
+! KSP object until copy_pressure_out_of_petsc writes it, ghost planes
+! included, into pres.  tpls_ksp_create replaces the KSP set-up of
+! configure_mpi_petsc.F90:143-183, taking the tolerances that set-up reads
+! from the options database.
+module pressure_solver
+  use tpls_core_c
+  use tpls_state
+  implicit none
+  private
+  public :: tpls_ksp, tpls_ksp_create, PETSC_NULL_OBJECT, KSPSolve, KSPGetResidualNorm, &
+      KSPGetIterationNumber, KSPGetConvergedReason, output_converged_reason, copy_pressure_out_of_petsc
+
+  integer(8), parameter :: PETSC_NULL_OBJECT = 0
+
+  type tpls_ksp
+     double precision :: rtol = 1.d-5, abstol = 1.d-50, dtol = 1.d5
+     integer :: maxits = 10000
+     integer :: its = 0, reason = 0
+     double precision :: rnorm = 0.d0
+     double precision, allocatable :: x(:, :, :)
+  end type tpls_ksp
+
+contains
+
+  subroutine tpls_ksp_create(ksp, rtol, abstol, dtol, maxits)
+    type(tpls_ksp), intent(out) :: ksp
+    double precision, intent(in) :: rtol, abstol, dtol
+    integer, intent(in) :: maxits
+    ksp%rtol = rtol
+    ksp%abstol = abstol
+    ksp%dtol = dtol
+    ksp%maxits = maxits
+  end subroutine tpls_ksp_create
+
+  subroutine KSPSolve(ksp, b, x, ierr)
+    type(tpls_ksp), intent(inout) :: ksp
+    integer(8), intent(in) :: b, x
+    integer, intent(out) :: ierr
+    if (.not. allocated(ksp%x)) allocate(ksp%x(0:maxl, 0:maxm, 0:maxn))
+    ksp%x = pres
+    ierr = c_pressure_solve(ksp%x, RHS_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt, ksp%rtol, &
+        ksp%abstol, ksp%dtol, ksp%maxits, ksp%its, ksp%rnorm, ksp%reason)
+  end subroutine KSPSolve
+
+  subroutine KSPGetResidualNorm(ksp, rnorm, ierr)
+    type(tpls_ksp), intent(in) :: ksp
+    double precision, intent(out) :: rnorm
+    integer, intent(out) :: ierr
+    rnorm = ksp%rnorm
+    ierr = 0
+  end subroutine KSPGetResidualNorm
+
+  subroutine KSPGetIterationNumber(ksp, its, ierr)
+    type(tpls_ksp), intent(in) :: ksp
+    integer, intent(out) :: its, ierr
+    its = ksp%its
+    ierr = 0
+  end subroutine KSPGetIterationNumber
+
+  subroutine KSPGetConvergedReason(ksp, reason, ierr)
+    type(tpls_ksp), intent(in) :: ksp
+    integer, intent(out) :: reason, ierr
+    reason = ksp%reason
+    ierr = 0
+  end subroutine KSPGetConvergedReason
+
+  ! The name of PETSc's KSPConvergedReason code, printed as the reference does.
+  subroutine output_converged_reason(ksp)
+    type(tpls_ksp), intent(inout) :: ksp
+    character(29) :: converged_reason
+    select case (ksp%reason)
+    case (-10); converged_reason = 'KSP_DIVERGED_INDEFINITE_MAT'
+    case (-9); converged_reason = 'KSP_DIVERGED_NAN'
+    case (-8); converged_reason = 'KSP_DIVERGED_INDEFINITE_PC'
+    case (-7); converged_reason = 'KSP_DIVERGED_NONSYMMETRIC'
+    case (-6); converged_reason = 'KSP_DIVERGED_BREAKDOWN_BICG'
+    case (-5); converged_reason = 'KSP_DIVERGED_BREAKDOWN'
+    case (-4); converged_reason = 'KSP_DIVERGED_DTOL'
+    case (-3); converged_reason = 'KSP_DIVERGED_ITS'
+    case (-2); converged_reason = 'KSP_DIVERGED_NULL'
+    case (-1); converged_reason = 'UNUSED'
+    case (0); converged_reason = 'KSP_CONVERGED_ITERATING'
+    case (1); converged_reason = 'KSP_CONVERGED_RTOL_NORMAL'
+    case (2); converged_reason = 'KSP_CONVERGED_RTOL'
+    case (3); converged_reason = 'KSP_CONVERGED_ATOL'
+    case (4); converged_reason = 'KSP_CONVERGED_ITS'
+    case (5); converged_reason = 'KSP_CONVERGED_CG_NEG_CURVE'
+    case (6); converged_reason = 'KSP_CONVERGED_CG_CONSTRAINED'
+    case (7); converged_reason = 'KSP_CONVERGED_STEP_LENGTH'
+    case (8); converged_reason = 'KSP_CONVERGED_HAPPY_BREAKDOWN'
+    case (9); converged_reason = 'KSP_CONVERGED_ATOL_NORMAL'
+    case default; converged_reason = ' '
+    end select
+    write(*, *) 'reason =', converged_reason
+  end subroutine output_converged_reason
+
+  subroutine copy_pressure_out_of_petsc(ksp, ierr)
+    type(tpls_ksp), intent(in) :: ksp
+    integer, intent(out) :: ierr
+    if (.not. allocated(ksp%x)) then
+       ierr = -1
+       return
+    end if
+    pres = ksp%x
+    ierr = 0
+  end subroutine copy_pressure_out_of_petsc
+end module pressure_solver
+
+
 module tpls_pressure
   use tpls_core_c
   implicit none

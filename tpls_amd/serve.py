@@ -10,7 +10,13 @@ of the loop:
 3. advances every admitted case by one step with one tpls_batch_step, so
    requests that arrived while others were running join the next batch;
 4. retires the cases that have done their steps: the result is read back and
-   the case's memory goes back to the budget.
+   the case's memory goes back to the budget;
+5. evicts as failed a case whose step failed (tpls_batch_step_status gives
+   each case its own status; a failed case stops, the others complete their
+   step unperturbed) or whose pressure solve diverged (a negative
+   KSPConvergedReason: -3 iteration limit, -4 divergence, -9 NaN/Inf), which
+   the reference only prints (main_ns_hybrid.F90:903-911): its memory goes
+   back to the budget and the batch goes on without it.
 
 Every case computes exactly what it computes alone (the batch only overlaps
 independent cases), which tests/test_gpu_serve.py checks.  Reported: per-step
@@ -48,6 +54,8 @@ class Request:
     step_ms: list = field(default_factory=list)
     result: dict = field(default_factory=dict)
     its: list = field(default_factory=list)
+    status: str = "queued"         # queued, running, done or failed
+    error: str = ""
 
 
 class Server:
@@ -63,6 +71,7 @@ class Server:
         self.batch_sizes = []
         self.step_samples = []     # (batch-step ms, cases that advanced in it)
         self.cell_steps = 0
+        self.failed = []
 
     def _bytes(self, r: Request) -> int:
         from .case import Case
@@ -78,21 +87,32 @@ class Server:
         self.used += nb
         self.peak_used = max(self.peak_used, self.used)
         r.admitted = now
+        r.status = "running"
 
     def _retire(self, entry, now: float) -> None:
         r, c, _, nb = entry
         r.result = {f: c.field(f) for f in r.keep}
         r.finished = now
+        r.status = "done"
         c.close()
         self.used -= nb
         self.done.append(r)
+
+    def _evict(self, entry, now: float, why: str) -> None:
+        r, c, _, nb = entry
+        r.finished = now
+        r.status = "failed"
+        r.error = why
+        c.close()
+        self.used -= nb
+        self.failed.append(r)
 
     def run(self, requests: list[Request], horizon: float = float("inf"), drain: bool = True,
             max_wall: float = float("inf")) -> dict:
         """Serve the requests (sorted by arrival).  Arrivals stop at horizon;
         with drain the loop runs until every admitted request has finished,
         otherwise it stops at horizon; it never runs past max_wall seconds."""
-        from .case import batch_step
+        from .case import batch_step_status
         pending = sorted(requests, key=lambda r: r.arrival)
         queue = []
         t0 = time.perf_counter()
@@ -118,16 +138,24 @@ class Server:
                 break
             cases = [e[1] for e in self.active]
             s0 = time.perf_counter()
-            batch_step(cases, 1)
+            status = batch_step_status(cases, 1)
             ms = (time.perf_counter() - s0) * 1e3
             self.batch_sizes.append(len(cases))
             self.step_samples.append((ms, len(cases)))
             now = time.perf_counter() - t0
             keep = []
-            for e in self.active:
+            for e, (rc, msg) in zip(self.active, status):
                 r = e[0]
+                if rc:
+                    self._evict(e, now, f"step failed with status {rc}: {msg}")
+                    continue
+                d = e[1].diag()
+                if d["ksp_reason"] < 0:
+                    self._evict(e, now, f"pressure solve diverged (KSPConvergedReason {d['ksp_reason']} "
+                                        f"after {d['ksp_its']} iterations)")
+                    continue
                 r.step_ms.append(ms)
-                r.its.append(e[1].diag()["ksp_its"])
+                r.its.append(d["ksp_its"])
                 self.cell_steps += r.params.cells
                 e[2] -= 1
                 if e[2] == 0:
@@ -150,7 +178,8 @@ class Server:
         pct = lambda a, q: round(float(np.percentile(a, q)), 3)
         return {"value": round(self.cell_steps / wall * 1e-6, 4), "unit": "Mcell-steps/s",
                 "wall_s": round(wall, 3), "requests_submitted": submitted, "requests_arrived": arrived,
-                "requests_finished": len(self.done), "requests_unfinished": len(self.active) + queued,
+                "requests_finished": len(self.done), "requests_failed": len(self.failed),
+                "requests_unfinished": len(self.active) + queued,
                 "step_ms_p50": pct(step, 50), "step_ms_p99": pct(step, 99),
                 "request_ms_p50": pct(req, 50), "request_ms_p99": pct(req, 99),
                 "queue_ms_p50": pct(wait, 50), "queue_ms_p99": pct(wait, 99),
