@@ -1685,12 +1685,14 @@ int orc_pressure_solve(double *pres_, const double *RHS_p_, const double *u3_,
     double rr, sr, szr, sz, mz = 0.0;
     sums2(r, r, plane, o.nz, &rr, &sr);
     double m = sr / (double)N;
-    double dp = sqrt(rr - sr * m > 0 ? rr - sr * m : 0);
+    /* a NaN in b or in the initial guess propagates into the norm, as it does
+     * in PETSc's VecNorm */
+    double dp = sqrt(rr - sr * m > 0 || isnan(rr - sr * m) ? rr - sr * m : 0);
     double beta = rr - sr * m;
     double rnorm0 = bnorm > 0 ? bnorm : dp;
     double ttol = rtol * rnorm0 > abstol ? rtol * rnorm0 : abstol;
     int reason = 0, its = 0;
-    if (isnan(dp)) reason = R_DIV_NAN;
+    if (isnan(dp) || isnan(bnorm)) reason = R_DIV_NAN;
     else if (dp <= ttol) reason = dp < abstol ? R_CONV_ATOL : R_CONV_RTOL;
     /* the preconditioner is applied only to a residual that failed the test */
     if (pc >= 1 && !reason) {

@@ -189,6 +189,20 @@ def test_get_difference(L, comp):
     assert L.call("get_difference", a, a.copy(order="F"), EX, EY, ez, SX, SY, 0)[0] == 0.0
 
 
+@pytest.mark.parametrize("where", ["rhs", "guess"])
+def test_pressure_solve_reports_nan_input(L, where):
+    """A NaN in RHS_p or in the initial pressure: KSP_DIVERGED_NAN (-9) at
+    once, as the oracle reports it."""
+    S = shapes(MAXL, MAXM, MAXN)
+    r = np.random.default_rng(29)
+    rhs, u3 = rand(r, S["p"]), rand(r, S["uv"])
+    inl = np.ascontiguousarray(r.standard_normal(MAXN - 1))
+    p0 = np.zeros(S["p"], order="F")
+    (rhs if where == "rhs" else p0)[4, 3, 5] = np.nan
+    its, _, reason = L.pressure_solve(p0, rhs, u3, inl, MAXL, MAXM, MAXN, DX, DX, DX, DT, 1e-10, 1e-50, 1e5, 1000)
+    assert reason == -9 and its == 0
+
+
 def test_pressure_pieces(L):
     S = shapes(MAXL, MAXM, MAXN)
     r = np.random.default_rng(27)

@@ -126,6 +126,25 @@ def test_solve_matches_full_petsc_system(pc):
     assert err < 1e-10, (pc, its, err)
 
 
+@pytest.mark.parametrize("pc", [0, 2])
+@pytest.mark.parametrize("where", ["rhs", "guess"])
+def test_nan_input_is_reported_as_diverged_nan(pc, where):
+    """A NaN in the right-hand side or in the initial guess reaches the norm
+    of the initial residual, so the solve stops at once with PETSc's
+    KSP_DIVERGED_NAN (-9), the code output_converged_reason prints as such
+    (pressure_solver.F90:103-104), instead of reporting convergence."""
+    maxl, maxm, maxn = 13, 9, 11
+    rng = np.random.default_rng(6)
+    rhs_p = np.zeros((maxl + 1, maxm + 1, maxn + 1), order="F")
+    rhs_p[1:maxl, 1:maxm, 1:maxn] = rng.standard_normal((maxl - 1, maxm - 1, maxn - 1))
+    u3 = np.zeros((maxl + 1, maxm + 1, maxn - 1), order="F")
+    inl = np.zeros(maxn - 1)
+    pres = np.zeros((maxl + 1, maxm + 1, maxn + 1), order="F")
+    (rhs_p if where == "rhs" else pres)[4, 3, 5] = np.nan
+    reason, its, _ = B.pressure_solve(pres, rhs_p, u3, inl, maxl, maxm, maxn, 0.1, 0.1, 0.1, 1e-3, pc=pc)
+    assert reason == -9 and its == 0
+
+
 def test_solve_detects_sign_and_axis_mistakes():
     """The direct solution is sensitive to the mistakes the elimination could
     make: a swapped x/y coefficient or a dropped inlet term moves it far

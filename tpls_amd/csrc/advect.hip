@@ -39,7 +39,11 @@ __device__ __forceinline__ C weno5f(C v1, C v2, C v3, C v4, C v5)
     C e1 = v1 * C(1.0 / 3.0) - v2 * C(7.0 / 6.0) + v3 * C(11.0 / 6.0);
     C e2 = -(v2 * C(1.0 / 6.0)) + v3 * C(5.0 / 6.0) + v4 * C(1.0 / 3.0);
     C e3 = v3 * C(1.0 / 3.0) + v4 * C(5.0 / 6.0) - v5 * C(1.0 / 6.0);
-    return (n1 * e1 + n2 * e2 + n3 * e3) / (n1 + n2 + n3);
+    const C num = n1 * e1 + n2 * e2 + n3 * e3, den = n1 + n2 + n3;
+    if constexpr (sizeof(C) == 4)
+        return __fdividef(num, den);  // FP32 state: the hardware reciprocal (within 2 ulp)
+    else
+        return num / den;
 }
 
 // Face value on the u face I (between p cells I and I+1) at p row pj, level
@@ -146,6 +150,7 @@ __global__ void __launch_bounds__(ADV_NT) k_advect_phi(Grid g, int nbx, int nby,
     const bool mine = i <= L - 1 && j <= P;
     const int ic = min(i, L - 1), jm = wrap_any(j - 1, P);
     const long sz = phi.sz;
+    const C rdx = C(1) / dx, rdy = C(1) / dy, rdz = C(1) / dz;
     // element offsets (level 0) of this thread's share of the staged window:
     // x from i0-3 clamped to the array, y rows j0-3 .. periodic
     long soff[ADV_NS];
@@ -168,16 +173,25 @@ __global__ void __launch_bounds__(ADV_NT) k_advect_phi(Grid g, int nbx, int nby,
             if (q < ADV_W * ADV_H) a[q] = r[m];
         }
     };
-    // this thread's faces: x faces tid and tid + ADV_NT (when < ADV_NXF), y
-    // faces likewise; LDS positions and velocity offsets (advect_phi.f90:330-334:
-    // the x face I of p row jr carries u(I, jr-1, k-1), the y face between p
-    // rows jr and jr+1 carries v(I, jr, k-1), the top face w(i, j-1, k))
+    // this thread's faces: x face tid and y face tid, and the faces beyond
+    // ADV_NT: the 8 extra x faces go to the first lanes of the seventh
+    // wavefront and the 64 extra y faces to the eighth, so that the two SIMDs
+    // holding them evaluate 7 face reconstructions per level for the block
+    // and the other two 6, instead of one wavefront doing 5 of its own.  LDS
+    // positions and velocity offsets (advect_phi.f90:330-334: the x face I of
+    // p row jr carries u(I, jr-1, k-1), the y face between p rows jr and jr+1
+    // carries v(I, jr, k-1), the top face w(i, j-1, k))
+    constexpr int XW = 6 * BX, YW = 7 * BX;  // first thread of the extra x / y faces
+    static_assert(ADV_NXF - ADV_NT <= BX && ADV_NYF - ADV_NT <= BX && YW + BX == ADV_NT, "extra faces");
+    const bool x2 = tid >= XW && tid < XW + (ADV_NXF - ADV_NT), y2 = tid >= YW && tid < YW + (ADV_NYF - ADV_NT);
+    const int xf[2] = {tid, x2 ? ADV_NT + tid - XW : ADV_NXF - 1};
+    const int yf[2] = {tid, y2 ? ADV_NT + tid - YW : ADV_NYF - 1};
     int xc[2], yc[2];
     bool xw[2];
     long uoff[2], voff[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        const int fx = min(tid + h * ADV_NT, ADV_NXF - 1), fy = min(tid + h * ADV_NT, ADV_NYF - 1);
+        const int fx = xf[h], fy = yf[h];
         const int Ix = i0 - 1 + fx % (BX + 1), rx = fx / (BX + 1);
         const int Iy = i0 + fy % BX, ry = fy / BX;
         xc[h] = (rx + 3) * ADV_W + (Ix - i0 + 3);
@@ -186,7 +200,6 @@ __global__ void __launch_bounds__(ADV_NT) k_advect_phi(Grid g, int nbx, int nby,
         yc[h] = (ry + 2) * ADV_W + (Iy - i0 + 3);
         voff[h] = min(Iy, L) + wrap_any(j0 - 1 + ry, P) * v.sy;
     }
-    const bool x2 = tid + ADV_NT < ADV_NXF, y2 = tid + ADV_NT < ADV_NYF;
     const long woff = ic + jm * w.sy;
     C ux[2], vy[2], wt, ux_n[2] = {C(0), C(0)}, vy_n[2] = {C(0), C(0)}, wt_n = C(0);
     auto load_vel = [&](int k, C (&uu)[2], C (&vv)[2], C& ww) {
@@ -237,15 +250,19 @@ __global__ void __launch_bounds__(ADV_NT) k_advect_phi(Grid g, int nbx, int nby,
         if (x2) {
             const bool up = ux[1] >= C(0);
             const C f = xw[1] ? weno_lds<C>(pl, xc[1], 1, up) : pl[up ? xc[1] : xc[1] + 1];
-            (&fx_s[0][0])[tid + ADV_NT] = ux[1] * f;
+            (&fx_s[0][0])[xf[1]] = ux[1] * f;
         }
         (&fy_s[0][0])[tid] = vy[0] * weno_lds<C>(pl, yc[0], ADV_W, vy[0] >= C(0));
-        if (y2) (&fy_s[0][0])[tid + ADV_NT] = vy[1] * weno_lds<C>(pl, yc[1], ADV_W, vy[1] >= C(0));
+        if (y2) (&fy_s[0][0])[yf[1]] = vy[1] * weno_lds<C>(pl, yc[1], ADV_W, vy[1] >= C(0));
         const C ht = wt * zface<C>(z, k, n, wt);
         __syncthreads();
-        if (mine)
-            conv(i, j, k) = (fx_s[ty][tx + 1] - fx_s[ty][tx]) / dx + (fy_s[ty + 1][tx] - fy_s[ty][tx]) / dy +
-                            (ht - hb) / dz;
+        if (mine) {
+            const C ax = fx_s[ty][tx + 1] - fx_s[ty][tx], ay = fy_s[ty + 1][tx] - fy_s[ty][tx], az = ht - hb;
+            if constexpr (sizeof(C) == 4)
+                conv(i, j, k) = ax * rdx + ay * rdy + az * rdz;  // FP32: reciprocal multiplies
+            else
+                conv(i, j, k) = ax / dx + ay / dy + az / dz;     // FP64: the reference's divisions
+        }
         hb = ht;
         ux[0] = ux_n[0];
         ux[1] = ux_n[1];

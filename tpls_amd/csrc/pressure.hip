@@ -194,7 +194,8 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
     double bnorm = sqrt(tot[0]);
     double m = tot[1] / N;
     double beta = tot[2] - tot[1] * m;
-    double dp = sqrt(beta > 0.0 ? beta : 0.0);
+    // a NaN in b or in the initial guess reaches beta and must reach dp
+    double dp = sqrt(beta > 0.0 || isnan(beta) ? beta : 0.0);
     double rnorm0 = bnorm > 0.0 ? bnorm : dp;
     double ttol = st->rtol * rnorm0 > st->abstol ? st->rtol * rnorm0 : st->abstol;
     st->bnorm = bnorm;
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
     st->i = 0;
     st->reason = 0;
     st->done = 0;
-    if (isnan(dp)) {
+    if (isnan(dp) || isnan(bnorm)) {
         st->reason = -9;
         st->done = 1;
     } else if (dp <= ttol) {
