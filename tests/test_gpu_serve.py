@@ -86,7 +86,7 @@ def test_diverged_and_nan_cases_are_evicted_without_perturbing_the_batch():
     assert reqs[1].status == "failed" and "-9" in reqs[1].error
     assert reqs[2].status == "failed" and "-3" in reqs[2].error
     assert srv.used == 0
-    with Case(good, diagnostics=False, graphs=True, tiles="auto") as c:
+    with Case(good, precision=reqs[0].precision, diagnostics=False, graphs=True, tiles="auto") as c:
         c.load(*make_initial(good))
         c.step(4)
         for f in reqs[0].keep:

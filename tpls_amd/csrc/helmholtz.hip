@@ -802,7 +802,14 @@ __global__ void __launch_bounds__(BX* TY) k_jacobi_st(Grid g, int nbx, int nby, 
                               C(xs[(k + 1) & 3][ty + 1][tx + 1]), zp, zm);
             }
             const C v = helmholtz<C>(mu_cook<COMP, C>(mr), ax2, ay2, az2, xp, xm, yp, ym, zp, zm, rv);
-            if (mine && k <= ke) x(i, j, k) = v;
+            if (mine && k <= ke) {
+                // FP64: a streaming store (no L2 allocation for a line no later
+                // level of this launch reads) measured 3 % faster; FP32 slower
+                if constexpr (sizeof(C) == 8)
+                    __builtin_nontemporal_store(v, &x(i, j, k));
+                else
+                    x(i, j, k) = v;
+            }
             __syncthreads();
         }
     }
