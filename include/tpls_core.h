@@ -266,6 +266,20 @@ int tpls_do_jacobi_w(double *w3, const double *w3_old, const double *RHS, const 
 /* jacobi_iteration_allflux.f90:285  get_difference(ua,ub,ex,ey,ez,sx,sy,sz,diff) */
 int tpls_get_difference(const double *ua, const double *ub, int ex, int ey, int ez, int sx, int sy,
                         int sz, double *diff);
+/* get_difference on arrays the caller already holds in device memory, for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that keep every buffer resident, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference: it allocates nothing and copies nothing.
+ * ua, ub: device arrays of the bounds (sx-1:ex+1, sy-1:ey+1, sz:ez),
+ * unpadded, x fastest; diff: one double of device memory, which receives
+ * the sum when `stream` (a hipStream_t; NULL for the null stream) reaches
+ * it.  Its only other device memory is a file-scope array of 1024 partial
+ * sums (8 KiB) per GPU, held until the process ends; the reduction always
+ * runs that many blocks, so no array size needs more.  Calls that overlap on
+ * one GPU race on that array.  TPLS_ERR_ARG for host memory or empty bounds,
+ * TPLS_ERR_DECOMP unless sx = sy = 1 and sz = 0. */
+int tpls_get_difference_noalloc(const double *ua, const double *ub, int ex, int ey, int ez, int sx,
+                                int sy, int sz, double *diff, void *stream);
 /* sor_iteration_allflux.f90:19 / 152 / 280  do_sor_u / _v / _w (only do_sor_u
  * takes ex_max, as in the reference) */
 int tpls_do_sor_u(double *u3, const double *RHS, const double *viscosity, double dx, double dy,
