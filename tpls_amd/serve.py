@@ -72,17 +72,27 @@ class Server:
         self.step_samples = []     # (batch-step ms, cases that advanced in it)
         self.cell_steps = 0
         self.failed = []
+        self._init = {}            # rid -> future of its initial state
+        self._pool = None
 
     def _bytes(self, r: Request) -> int:
         from .case import Case
         return Case.device_bytes(r.params, r.precision)
+
+    def _prepare(self, r: Request) -> None:
+        """Start building the request's initial state on a host thread as soon
+        as it arrives, so that admission does not stall the batch loop."""
+        from .initial import make_initial
+        if self._pool is not None and r.rid not in self._init:
+            self._init[r.rid] = self._pool.submit(make_initial, r.params)
 
     def _admit(self, r: Request, now: float) -> None:
         from .case import Case
         from .initial import make_initial
         nb = self._bytes(r)
         c = Case(r.params, precision=r.precision, diagnostics=False, graphs=self.graphs, tiles=self.tiles)
-        c.load(*make_initial(r.params))
+        fut = self._init.pop(r.rid, None)
+        c.load(*(fut.result() if fut is not None else make_initial(r.params)))
         self.active.append([r, c, r.steps, nb])
         self.used += nb
         self.peak_used = max(self.peak_used, self.used)
@@ -112,9 +122,19 @@ class Server:
         """Serve the requests (sorted by arrival).  Arrivals stop at horizon;
         with drain the loop runs until every admitted request has finished,
         otherwise it stops at horizon; it never runs past max_wall seconds."""
+        from concurrent.futures import ThreadPoolExecutor
         from .case import batch_step_status
         pending = sorted(requests, key=lambda r: r.arrival)
         queue = []
+        self._pool = ThreadPoolExecutor(max_workers=2)
+        try:
+            return self._run(pending, queue, len(requests), horizon, drain, max_wall, batch_step_status)
+        finally:
+            self._pool.shutdown(wait=False, cancel_futures=True)
+            self._pool = None
+            self._init.clear()
+
+    def _run(self, pending, queue, submitted, horizon, drain, max_wall, batch_step_status) -> dict:
         t0 = time.perf_counter()
         nxt = 0
         while True:
@@ -122,6 +142,8 @@ class Server:
             while nxt < len(pending) and pending[nxt].arrival <= now and pending[nxt].arrival <= horizon:
                 queue.append(pending[nxt])
                 nxt += 1
+            for r in queue[:8]:                    # the next admissions' initial states
+                self._prepare(r)
             while queue and (not self.max_batch or len(self.active) < self.max_batch):
                 if self.used + self._bytes(queue[0]) > self.budget:
                     if not self.active:
@@ -166,7 +188,7 @@ class Server:
         wall = time.perf_counter() - t0
         for e in self.active:          # stopped at the horizon: release the rest
             e[1].close()
-        return self.report(wall, len(requests), nxt, len(queue))
+        return self.report(wall, submitted, nxt, len(queue))
 
     def report(self, wall: float, submitted: int, arrived: int, queued: int) -> dict:
         # per-step latency over every case-step (a batch step counts once per
@@ -178,6 +200,7 @@ class Server:
         pct = lambda a, q: round(float(np.percentile(a, q)), 3)
         return {"value": round(self.cell_steps / wall * 1e-6, 4), "unit": "Mcell-steps/s",
                 "wall_s": round(wall, 3), "requests_submitted": submitted, "requests_arrived": arrived,
+                "requests_admitted": len(self.done) + len(self.failed) + len(self.active),
                 "requests_finished": len(self.done), "requests_failed": len(self.failed),
                 "requests_unfinished": len(self.active) + queued,
                 "step_ms_p50": pct(step, 50), "step_ms_p99": pct(step, 99),
@@ -218,9 +241,16 @@ def main():
     ap.add_argument("--max-steps", type=int, default=2000)
     ap.add_argument("--no-drain", action="store_true", help="stop at the end of the arrival window")
     a = ap.parse_args()
-    reqs = poisson_requests(a.rate, a.minutes * 60, a.seed, steps=(a.min_steps, a.max_steps))
+    window = a.minutes * 60
+    reqs = poisson_requests(a.rate, window, a.seed, steps=(a.min_steps, a.max_steps))
+    offered = sum(r.params.cells * r.steps for r in reqs) / window * 1e-6
     srv = Server(a.budget_gb * 1e9)
-    out = srv.run(reqs, horizon=a.minutes * 60, drain=not a.no_drain)
+    out = srv.run(reqs, horizon=window, drain=not a.no_drain)
+    # the load the arrivals offer against what the GPU served: a ratio below
+    # one means the queue grows for as long as arrivals continue
+    out.update({"arrival_rate_per_s": a.rate, "window_s": window, "steps_per_request": [a.min_steps, a.max_steps],
+                "offered_mcell_steps_per_s": round(offered, 1),
+                "served_over_offered": round(out["value"] / offered, 4) if offered else None})
     print(json.dumps({"serving": out}), flush=True)
 
 
