@@ -312,6 +312,43 @@ int tpls_pressure_solve(double *pres, const double *RHS_p, const double *u3,
                         double dz, double dt, double rtol, double abstol, double dtol, int maxits,
                         int *its, double *rnorm, int *reason);
 
+/* ---- the halo exchanges of one rank (mpi.f90) ----
+ * mpi.f90:119-170 exchange2d, :215-270 exchange2d_aug1, :294-369
+ * exchange2d_aug2 with a single rank whose y neighbours are itself and whose
+ * x neighbours are MPI_PROC_NULL: the y halo rows take the opposite interior
+ * rows over x = sx-1..ex+1 of every z-plane.  kind 0: exchange2d on
+ * (sx-1:ex+1, sy-1:ey+1, sz:ez), rows sy-1 <- ey and ey+1 <- sy; kind 1:
+ * exchange2d_aug1 on (sx-2:ex+2, sy-2:ey+2, sz:ez), the same rows; kind 2:
+ * exchange2d_aug2 on that shape, rows sy-2 <- ey-1 and ey+2 <- sy+1.  An
+ * array attached to the mirror whose device copy holds data the host has not
+ * seen is exchanged on the device.  TPLS_ERR_DECOMP unless sx = sy = 1. */
+int tpls_exchange2d(double *uu, int ex, int ey, int ez, int sx, int sy, int sz, int kind);
+
+/* ---- device mirrors of host arrays for the entry points above ----
+ * The reference's procedures run on host arrays that main_ns_hybrid.F90
+ * allocates once (main_ns_hybrid.F90:416-549) and reuses every step.  An
+ * attached host array keeps a device copy that every per-routine entry point
+ * (and tpls_exchange2d) uses in place of a copy over the host link.  Host
+ * pages are protected so that data moves only where needed: a host write
+ * marks its pages for upload before the next device use; a device write
+ * makes the host pages fetch on their first host access.  So the caller's
+ * code reads and writes the arrays as before.
+ * Rules: an attached array keeps its address and stays allocated until it is
+ * detached (detach before deallocate); system calls that write into it
+ * (read(2), Fortran unformatted READ into the array) need it detached; the
+ * partial pages at either end of an array are copied on every device use.
+ * tpls_mirror_flush makes every host copy current (before output, for
+ * instance).  tpls_mirror_counters: out[0] bytes uploaded, out[1] bytes
+ * downloaded, out[2] protection faults resolved, out[3] uploads, out[4]
+ * downloads, out[5] arrays attached.  Attaching and detaching are not
+ * meant to race with entry points on other threads. */
+int tpls_mirror_attach(void *host, long long bytes);
+int tpls_mirror_detach(void *host);
+int tpls_mirror_detach_all(void);
+int tpls_mirror_flush(void);
+int tpls_mirror_set_granule(long long bytes);
+int tpls_mirror_counters(long long *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -161,6 +161,35 @@ def pressure_solve(pres, rhs_p, u3, u_inlet, maxl, maxm, maxn, dx, dy, dz, dt,
     return its.value, rnorm.value, reason.value
 
 
+def exchange2d(uu, ex, ey, ez, sx, sy, sz, kind=0):
+    """tpls_exchange2d: mpi.f90's exchange2d (kind 0), exchange2d_aug1 (1) or
+    exchange2d_aug2 (2) on one rank, in place."""
+    f = lib().tpls_exchange2d
+    f.argtypes = [_D] + [C.c_int] * 7
+    check(f(_ptr(uu, "a"), ex, ey, ez, sx, sy, sz, kind), "tpls_exchange2d")
+
+
+def mirror_attach(a):
+    """Attach a Fortran-contiguous float64 array to the device mirror."""
+    f = lib().tpls_mirror_attach
+    f.argtypes = [C.c_void_p, C.c_longlong]
+    _ptr(a, "a")
+    check(f(a.ctypes.data, a.nbytes), "tpls_mirror_attach")
+
+
+def mirror_detach(a):
+    f = lib().tpls_mirror_detach
+    f.argtypes = [C.c_void_p]
+    check(f(a.ctypes.data), "tpls_mirror_detach")
+
+
+def mirror_counters() -> dict:
+    """Bytes up / down over the host link, faults resolved, copies, arrays attached."""
+    c = (C.c_longlong * 6)()
+    check(lib().tpls_mirror_counters(c), "tpls_mirror_counters")
+    return dict(zip(("up", "down", "faults", "uploads", "downloads", "attached"), list(c)))
+
+
 def test_mg_apply(r, nx, ny, nz, cx, cy, cz, diag, nu):
     """Test hook: z = M r, one V-cycle of the pressure solve's multigrid
     preconditioner on the GPU (r flat, x fastest); returns (z, levels)."""

@@ -31,6 +31,7 @@ DRIVER = os.path.join(ROOT, "tests", "fortran", "drive_modules")
 DRIVE_IO_SRC = os.path.join(ROOT, "tests", "fortran", "drive_io.f90")
 DRIVE_IO = os.path.join(ROOT, "tests", "fortran", "drive_io")
 DRIVE_IO_NATIVE = os.path.join(ROOT, "tests", "fortran", "drive_io_native")
+DRIVE_MAIN = os.path.join(ROOT, "tests", "fortran", "drive_main_step")
 FC = os.environ.get("TPLS_FC", "/opt/rocm/bin/amdflang")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -106,6 +107,12 @@ def build_fortran(verbose: bool = True) -> None:
          "amdflang drive_io_native.f90")
     _run([FC, "-o", DRIVE_IO_NATIVE, nobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
          "link drive_io_native")
+    # main_ns_hybrid.F90's timestep walked call for call through the modules
+    mobj = os.path.join(FMOD, "drive_main_step.o")
+    _run([FC, "-O2", "-module-dir", FMOD, "-I", FMOD, "-c", DRIVE_MAIN + ".f90", "-o", mobj],
+         "amdflang drive_main_step.f90")
+    _run([FC, "-o", DRIVE_MAIN, mobj, FLIB, f"-L{HERE}", "-ltpls_hip", "-Wl,-rpath,$ORIGIN/../../tpls_amd"],
+         "link drive_main_step")
     if verbose:
         print(f"built {FLIB} and {DRIVER}")
 

@@ -15,6 +15,7 @@
 #include "../../include/tpls_core.h"
 #include "../../include/tpls_io.h"
 #include "host.hpp"
+#include "mirror.hpp"
 #include "tpls_kernels.hpp"
 
 using namespace tpls;
@@ -1376,6 +1377,8 @@ struct DropCtx {
     double* red_part = nullptr;
     double* red_out = nullptr;
     double* h_out = nullptr;
+    // device copies of attached host arrays already bound in the current call
+    std::vector<void*> mused;
     // cached device case for tpls_pressure_solve
     tpls_case* pcase = nullptr;
     int pl = 0, pm = 0, pn = 0;
@@ -1395,6 +1398,7 @@ thread_local DropCtx g_drop;
 int drop_init()
 {
     DropCtx& d = g_drop;
+    d.mused.clear();
     if (d.s) return TPLS_OK;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TPLS_ERR_NOGPU, "no HIP device");
@@ -1405,18 +1409,42 @@ int drop_init()
     return TPLS_OK;
 }
 
-// A staged array: contiguous device copy of a host Fortran array.
+// A staged array: contiguous device copy of a host Fortran array.  For a host
+// array attached to the mirror the copy is the mirror's own (mirrored), which
+// holds the array between calls; otherwise a per-thread staging buffer.
 struct Stg {
     DevArray a;
     void* host = nullptr;
     bool out = false;
     int esz = 8;
+    bool mirrored = false;
 };
 
 // mode: 0 = upload, 1 = zero-fill (whole array written by the routine)
 int stage(int slot, const void* host, const Shape& sh, int esz, int mode, bool out, Stg& st)
 {
     DropCtx& d = g_drop;
+    const size_t nbytes = size_t(sh.count()) * esz;
+    st.mirrored = false;
+    void* dv = host && mirror::any() ? mirror::device_of(host, nbytes) : nullptr;
+    const bool again = dv && std::find(d.mused.begin(), d.mused.end(), dv) != d.mused.end();
+    if (dv && !again) {
+        // an attached array, bound once per call: the kernels use its device
+        // copy, made current (or zeroed, for an array the routine overwrites)
+        if (mode == 0) RC(mirror::to_device(host, nbytes));
+        else HIPCHK(hipMemsetAsync(dv, 0, nbytes, d.s));
+        d.mused.push_back(dv);
+        st.a.ptr = dv;
+        st.a.sh = sh;
+        st.a.pitch = sh.nx();
+        st.a.esz = esz;
+        st.host = const_cast<void*>(host);
+        st.out = out;
+        st.esz = esz;
+        st.mirrored = true;
+        return TPLS_OK;
+    }
+    if (again && mode != 0) return fail(TPLS_ERR_ARG, "an attached array bound twice in one call must be read twice");
     if ((int)d.buf.size() <= slot) {
         d.buf.resize(slot + 1, nullptr);
         d.cap.resize(slot + 1, 0);
@@ -1436,20 +1464,99 @@ int stage(int slot, const void* host, const Shape& sh, int esz, int mode, bool o
     st.host = const_cast<void*>(host);
     st.out = out;
     st.esz = esz;
-    if (mode == 0)
+    if (again) {
+        // the second binding of an attached array in one call (a routine that
+        // reads and rewrites it through two buffers): a private device copy
+        HIPCHK(hipMemcpyAsync(st.a.ptr, dv, bytes, hipMemcpyDeviceToDevice, d.s));
+    } else if (mode == 0) {
+        // host memory that may share pages with attached arrays is made
+        // current and readable before the copy engine reads it
+        if (host && mirror::any()) RC(mirror::host_access(host, bytes, false));
         HIPCHK(hipMemcpyAsync(st.a.ptr, host, bytes, hipMemcpyHostToDevice, d.s));
-    else
+    } else {
         HIPCHK(hipMemsetAsync(st.a.ptr, 0, bytes, d.s));
+    }
     return TPLS_OK;
 }
 
+// Results back to the host arrays: copied down, or, for attached arrays,
+// left in (or copied into) the mirror's device copy, whose host pages then
+// fetch on first access.
 int unstage(std::initializer_list<Stg*> outs)
 {
     DropCtx& d = g_drop;
-    for (Stg* st : outs)
-        HIPCHK(hipMemcpyAsync(st->host, st->a.ptr, size_t(st->a.sh.count()) * st->esz, hipMemcpyDeviceToHost, d.s));
+    Stg* kept[16];
+    int nkept = 0;
+    for (Stg* st : outs) {
+        const size_t bytes = size_t(st->a.sh.count()) * st->esz;
+        void* dv = st->mirrored ? st->a.ptr : mirror::any() ? mirror::device_of(st->host, bytes) : nullptr;
+        if (dv) {
+            if (nkept == 16) return fail(TPLS_ERR_ARG, "too many outputs");
+            if (dv != st->a.ptr) HIPCHK(hipMemcpyAsync(dv, st->a.ptr, bytes, hipMemcpyDeviceToDevice, d.s));
+            kept[nkept++] = st;
+            continue;
+        }
+        if (mirror::any()) RC(mirror::host_access(st->host, bytes, true));
+        HIPCHK(hipMemcpyAsync(st->host, st->a.ptr, bytes, hipMemcpyDeviceToHost, d.s));
+    }
     HIPCHK(hipStreamSynchronize(d.s));
+    for (int q = 0; q < nkept; ++q) RC(mirror::written(kept[q]->host, size_t(kept[q]->a.sh.count()) * kept[q]->esz));
     return TPLS_OK;
+}
+
+// The y-halo refresh a routine ends with (wrap_rows), on the mirror's device
+// copy when the device holds data the host has not seen.
+int wrap_dropin(double* h, const Shape& sh, int maxm)
+{
+    const size_t bytes = size_t(sh.count()) * sizeof(double);
+    void* dv = mirror::any() ? mirror::device_of(h, bytes) : nullptr;
+    RowPairs rp{};
+    const int P = maxm - 1;
+    for (int j = sh.ly; j <= sh.uy; ++j) {
+        if (j >= 1 && j <= P) continue;
+        if (rp.n == MAX_ROW_PAIRS) {
+            dv = nullptr;
+            break;
+        }
+        rp.dst[rp.n] = j - sh.ly;
+        rp.src[rp.n] = (j < 1 ? j + P : j - P) - sh.ly;
+        ++rp.n;
+    }
+    if (!dv || rp.n == 0 || mirror::host_current(h, bytes)) {
+        wrap_rows(h, sh, maxm);
+        return TPLS_OK;
+    }
+    RC(mirror::to_device(h, bytes));
+    HIPCHK(launch_copy_rows(g_drop.s, static_cast<double*>(dv), sh.nx(), sh.ny(), sh.nz(), 0, int(sh.nx()), rp));
+    HIPCHK(hipStreamSynchronize(g_drop.s));
+    return mirror::written(h, bytes);
+}
+
+// A case array from, or to, a host array that may be attached: the mirror's
+// device copy is copied on the device instead of over the host link.
+int put_any(tpls_case* c, DevArray* a, const double* host)
+{
+    const size_t bytes = size_t(a->sh.count()) * sizeof(double);
+    void* dv = a->esz == 8 && mirror::any() ? mirror::device_of(host, bytes) : nullptr;
+    if (!dv) return put_array(c, a, host);
+    RC(mirror::to_device(host, bytes));
+    const size_t row = size_t(a->sh.nx()) * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(a->ptr, size_t(a->pitch) * sizeof(double), dv, row, row,
+                            size_t(a->sh.ny() * a->sh.nz()), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return TPLS_OK;
+}
+
+int get_any(tpls_case* c, const DevArray* a, double* host)
+{
+    const size_t bytes = size_t(a->sh.count()) * sizeof(double);
+    void* dv = a->esz == 8 && mirror::any() ? mirror::device_of(host, bytes) : nullptr;
+    if (!dv) return get_array(c, a, host);
+    const size_t row = size_t(a->sh.nx()) * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(dv, row, a->ptr, size_t(a->pitch) * sizeof(double), row,
+                            size_t(a->sh.ny() * a->sh.nz()), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return mirror::written(host, bytes);
 }
 
 int check_decomp(int sx, int ex, int sy, int ey, int ex_max)
@@ -1517,9 +1624,9 @@ int tpls_do_advect_phi(double* phi2, double* u2, double* v2, double* w2, double*
                                      a_v.a.view<double>(), a_w.a.view<double>(), a_c.a.view<double>()));
     RC(unstage({&a_c}));
     // advect_phi.f90:317-322 refresh the y halos of u2, v2, w2 in place
-    wrap_rows(u2, shape_of(G_UV, maxl, maxm, maxn), maxm);
-    wrap_rows(v2, shape_of(G_UV, maxl, maxm, maxn), maxm);
-    wrap_rows(w2, shape_of(G_W, maxl, maxm, maxn), maxm);
+    RC(wrap_dropin(u2, shape_of(G_UV, maxl, maxm, maxn), maxm));
+    RC(wrap_dropin(v2, shape_of(G_UV, maxl, maxm, maxn), maxm));
+    RC(wrap_dropin(w2, shape_of(G_W, maxl, maxm, maxn), maxm));
     return TPLS_OK;
 }
 
@@ -1572,7 +1679,7 @@ int tpls_do_tpls(double* phi2, double* phi3, double* c0, double* c1, double* c2,
     } else {
         RC(unstage({&a3}));
     }
-    wrap_rows(phi3, sp, maxm);
+    RC(wrap_dropin(phi3, sp, maxm));
     return TPLS_OK;
 }
 
@@ -1620,7 +1727,7 @@ int tpls_dim(int maxl, int maxn, double dx, double dy, double dz, double dt, dou
     }
     // cahn_hilliard_solver.f90:340-341 refresh phiNew's y halos (the inlet and
     // outlet columns of those rows are set from the interior rows here)
-    wrap_rows(phiNew, sp, maxm);
+    RC(wrap_dropin(phiNew, sp, maxm));
     return TPLS_OK;
 }
 
@@ -1925,17 +2032,17 @@ int tpls_pressure_solve(double* pres, const double* RHS_p, const double* u3, con
     c->prm.ksp_dtol = dtol;
     c->prm.ksp_maxits = maxits;
     // the case's rhs[0..2] hold u3 (uv grid) and RHS_p (p grid, double)
-    RC(put_array(c, c->x[0][0], u3));
-    RC(put_array(c, c->pold, pres));
-    RC(put_array(c, c->cg_p0, RHS_p));
+    RC(put_any(c, c->x[0][0], u3));
+    RC(put_any(c, c->pold, pres));
+    RC(put_any(c, c->cg_p0, RHS_p));
     std::vector<double> inl(u_inlet, u_inlet + (maxn - 1));
     RC(put_array(c, c->inlet, inl.data()));
     FieldView<double> u = c->x[0][0]->view<double>();
     RC(pressure_solve<double>(c, u, u, u, c->cg_p0->view<double>(), 1, static_cast<const double*>(c->inlet->ptr)));
     HIPCHK(launch_pres_ghosts<double>(c->s, c->g, c->ph, c->pold->view<double>(), u,
                                       static_cast<const double*>(c->inlet->ptr), c->pres->view<double>()));
-    RC(get_array(c, c->pres, pres));
-    wrap_rows(pres, shape_of(G_P, maxl, maxm, maxn), maxm);
+    RC(get_any(c, c->pres, pres));
+    RC(wrap_dropin(pres, shape_of(G_P, maxl, maxm, maxn), maxm));
     *its = c->diag.ksp_its;
     *rnorm = c->diag.ksp_rnorm;
     *reason = c->diag.ksp_reason;

@@ -242,6 +242,17 @@ hipError_t launch_fill(hipStream_t s, T* p, long n, T v);
 template <typename T>
 hipError_t launch_copy(hipStream_t s, T* dst, const T* src, long n);
 
+// ---- y-halo rows of one rank with y-periodic neighbours (mpi.f90) ----
+// On a contiguous array of nx * ny * nz doubles (x fastest): for every pair q
+// < npairs, row dst[q] of every z-plane takes row src[q], over x in
+// [i0, i0 + ni).  Rows are 0-based; no destination row is a source row.
+constexpr int MAX_ROW_PAIRS = 6;
+struct RowPairs {
+    int n;
+    int dst[MAX_ROW_PAIRS], src[MAX_ROW_PAIRS];
+};
+hipError_t launch_copy_rows(hipStream_t s, double* a, long nx, long ny, long nz, int i0, int ni, const RowPairs& rp);
+
 // Launch-shape helpers shared by the .hip files.
 constexpr int BX = 64;  // one wavefront along x
 constexpr int BY = 4;

@@ -22,6 +22,11 @@
 ! KSPGetConvergedReason, output_converged_reason, copy_pressure_out_of_petsc)
 ! with main's argument lists, so that block of main compiles as it stands;
 ! module tpls_pressure offers the whole sequence as one call.
+!
+! Module tpls_mpi holds the halo exchanges main calls between procedures
+! (exchange2d, exchange2d_aug1, exchange2d_aug2 of mpi.f90) for the single
+! rank, and module tpls_mirror attaches main's arrays to device copies that
+! stay resident between calls (include/tpls_core.h, tpls_mirror_attach).
 
 module tpls_core_c
   use iso_c_binding
@@ -200,6 +205,12 @@ module tpls_core_c
       real(c_double), value :: dx, dy, dz, dt, rtol, abstol, dtol
       integer(c_int), intent(out) :: its, reason
       real(c_double), intent(out) :: rnorm
+    end function
+
+    integer(c_int) function c_exchange2d(uu, ex, ey, ez, sx, sy, sz, kind) bind(C, name="tpls_exchange2d")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: uu(*)
+      integer(c_int), value :: ex, ey, ez, sx, sy, sz, kind
     end function
 
     integer(c_int) function c_dim(maxl, maxn, dx, dy, dz, dt, epn, pe, phi, phi_old, conv2_phi, &
@@ -786,3 +797,128 @@ contains
     ierr = e
   end subroutine load_data_into_array
 end module tpls_io_native
+
+
+! The halo exchanges of mpi.f90 that main_ns_hybrid.F90 calls between the
+! procedures above (exchange2d :119, exchange2d_aug1 :215, exchange2d_aug2
+! :294), with their argument lists, for the single rank whose y neighbours
+! are itself: tpls_exchange2d copies the y halo rows, on the device when the
+! array is attached to the mirror and its device copy is the newer.  The
+! stride datatypes, neighbours and communicator are accepted and ignored.
+module tpls_mpi
+  use tpls_core_c
+  implicit none
+  integer :: master_id
+  parameter(master_id=0)
+contains
+  logical function is_master(my_id)
+    integer, intent(in) :: my_id
+    is_master = (master_id == my_id)
+  end function is_master
+
+  subroutine exchange2d(uu, stride_p_xz, stride_p_yz, neighbours, ex, ey, ez, sx, sy, sz, comm_topology)
+    integer, intent(in) :: ex, ey, ez, sx, sy, sz
+    integer, intent(in) :: stride_p_xz, stride_p_yz
+    integer, intent(in) :: neighbours(6), comm_topology
+    double precision, dimension(sx-1:ex+1, sy-1:ey+1, sz:ez), intent(inout) :: uu
+    call tpls_check(c_exchange2d(uu, ex, ey, ez, sx, sy, sz, 0), 'exchange2d')
+  end subroutine exchange2d
+
+  subroutine exchange2d_aug1(uu, stride_p_aug1_xz, stride_p_aug1_yz, neighbours, ex, ey, ez, sx, sy, sz, &
+      comm_topology)
+    integer, intent(in) :: ex, ey, ez, sx, sy, sz
+    integer, intent(in) :: stride_p_aug1_xz, stride_p_aug1_yz
+    integer, intent(in) :: neighbours(6), comm_topology
+    double precision, dimension(sx-2:ex+2, sy-2:ey+2, sz:ez), intent(inout) :: uu
+    call tpls_check(c_exchange2d(uu, ex, ey, ez, sx, sy, sz, 1), 'exchange2d_aug1')
+  end subroutine exchange2d_aug1
+
+  subroutine exchange2d_aug2(uu, stride_p_aug2_xz, stride_p_aug2_yz, neighbours, ex, ey, ez, sx, sy, sz, &
+      comm_topology)
+    integer, intent(in) :: ex, ey, ez, sx, sy, sz
+    integer, intent(in) :: stride_p_aug2_xz, stride_p_aug2_yz
+    integer, intent(in) :: neighbours(6), comm_topology
+    double precision, dimension(sx-2:ex+2, sy-2:ey+2, sz:ez), intent(inout) :: uu
+    call tpls_check(c_exchange2d(uu, ex, ey, ez, sx, sy, sz, 2), 'exchange2d_aug2')
+  end subroutine exchange2d_aug2
+end module tpls_mpi
+
+
+! Device copies of main's arrays that stay resident between calls
+! (tpls_mirror_attach in include/tpls_core.h).  main_ns_hybrid.F90 attaches
+! its arrays once after allocating them (main_ns_hybrid.F90:416-549) and
+! detaches them before deallocating them or reading files into them; its
+! statements between calls read and write the arrays as before.
+module tpls_mirror
+  use iso_c_binding
+  use tpls_core_c, only: tpls_check
+  implicit none
+  private
+  public :: mirror_attach, mirror_detach, mirror_detach_all, mirror_flush, mirror_counters
+
+  interface mirror_attach
+    module procedure attach3, attach1
+  end interface
+  interface mirror_detach
+    module procedure detach3, detach1
+  end interface
+
+  interface
+    integer(c_int) function c_attach(p, n) bind(C, name="tpls_mirror_attach")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: p
+      integer(c_long_long), value :: n
+    end function
+    integer(c_int) function c_detach(p) bind(C, name="tpls_mirror_detach")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+    end function
+    integer(c_int) function c_detach_all() bind(C, name="tpls_mirror_detach_all")
+      import :: c_int
+    end function
+    integer(c_int) function c_flush() bind(C, name="tpls_mirror_flush")
+      import :: c_int
+    end function
+    integer(c_int) function c_counters(out) bind(C, name="tpls_mirror_counters")
+      import :: c_int, c_long_long
+      integer(c_long_long), intent(out) :: out(6)
+    end function
+  end interface
+
+contains
+  subroutine attach3(a)
+    double precision, intent(inout), target, contiguous :: a(:, :, :)
+    call tpls_check(c_attach(c_loc(a), int(size(a), c_long_long) * 8_c_long_long), 'mirror_attach')
+  end subroutine attach3
+
+  subroutine attach1(a)
+    double precision, intent(inout), target, contiguous :: a(:)
+    call tpls_check(c_attach(c_loc(a), int(size(a), c_long_long) * 8_c_long_long), 'mirror_attach')
+  end subroutine attach1
+
+  subroutine detach3(a)
+    double precision, intent(inout), target, contiguous :: a(:, :, :)
+    call tpls_check(c_detach(c_loc(a)), 'mirror_detach')
+  end subroutine detach3
+
+  subroutine detach1(a)
+    double precision, intent(inout), target, contiguous :: a(:)
+    call tpls_check(c_detach(c_loc(a)), 'mirror_detach')
+  end subroutine detach1
+
+  subroutine mirror_detach_all()
+    call tpls_check(c_detach_all(), 'mirror_detach_all')
+  end subroutine mirror_detach_all
+
+  subroutine mirror_flush()
+    call tpls_check(c_flush(), 'mirror_flush')
+  end subroutine mirror_flush
+
+  ! bytes uploaded, bytes downloaded, faults, uploads, downloads, arrays attached
+  subroutine mirror_counters(out)
+    integer(8), intent(out) :: out(6)
+    integer(c_long_long) :: c(6)
+    call tpls_check(c_counters(c), 'mirror_counters')
+    out = c
+  end subroutine mirror_counters
+end module tpls_mirror

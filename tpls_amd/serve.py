@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 from dataclasses import dataclass, field, replace
 
@@ -118,27 +119,35 @@ class Server:
         self.failed.append(r)
 
     def run(self, requests: list[Request], horizon: float = float("inf"), drain: bool = True,
-            max_wall: float = float("inf")) -> dict:
+            max_wall: float = float("inf"), progress: float = 0.0) -> dict:
         """Serve the requests (sorted by arrival).  Arrivals stop at horizon;
         with drain the loop runs until every admitted request has finished,
-        otherwise it stops at horizon; it never runs past max_wall seconds."""
+        otherwise it stops at horizon; it never runs past max_wall seconds.
+        With progress > 0 a line of counts goes to stderr every that many
+        seconds."""
         from concurrent.futures import ThreadPoolExecutor
         from .case import batch_step_status
         pending = sorted(requests, key=lambda r: r.arrival)
         queue = []
         self._pool = ThreadPoolExecutor(max_workers=2)
         try:
-            return self._run(pending, queue, len(requests), horizon, drain, max_wall, batch_step_status)
+            return self._run(pending, queue, len(requests), horizon, drain, max_wall, batch_step_status, progress)
         finally:
             self._pool.shutdown(wait=False, cancel_futures=True)
             self._pool = None
             self._init.clear()
 
-    def _run(self, pending, queue, submitted, horizon, drain, max_wall, batch_step_status) -> dict:
+    def _run(self, pending, queue, submitted, horizon, drain, max_wall, batch_step_status, progress) -> dict:
         t0 = time.perf_counter()
         nxt = 0
+        told = 0.0
         while True:
             now = time.perf_counter() - t0
+            if progress > 0 and now - told >= progress:
+                told = now
+                print(f"[serve] t={now:.0f}s arrived={nxt} queued={len(queue)} active={len(self.active)} "
+                      f"finished={len(self.done)} failed={len(self.failed)} "
+                      f"Mcell-steps/s={self.cell_steps / max(now, 1e-9) * 1e-6:.1f}", file=sys.stderr, flush=True)
             while nxt < len(pending) and pending[nxt].arrival <= now and pending[nxt].arrival <= horizon:
                 queue.append(pending[nxt])
                 nxt += 1
@@ -208,7 +217,11 @@ class Server:
                 "queue_ms_p50": pct(wait, 50), "queue_ms_p99": pct(wait, 99),
                 "batch_size_mean": round(float(np.mean(self.batch_sizes)), 2) if self.batch_sizes else 0,
                 "batch_size_max": int(max(self.batch_sizes)) if self.batch_sizes else 0,
-                "hbm_budget_gb": round(self.budget / 1e9, 3), "hbm_peak_gb": round(self.peak_used / 1e9, 3)}
+                "hbm_budget_gb": round(self.budget / 1e9, 3), "hbm_peak_gb": round(self.peak_used / 1e9, 3),
+                "failures": [{"rid": r.rid, "grid": [r.params.nx, r.params.ny, r.params.nz],
+                              "Re": round(r.params.Re, 2), "viscosity_ratio": round(r.params.mu_plus / r.params.mu_minus, 4),
+                              "dt": r.params.dt, "precision": r.precision, "steps_done": len(r.step_ms),
+                              "steps": r.steps, "error": r.error} for r in self.failed[:50]]}
 
 
 def poisson_requests(rate: float, duration: float, seed: int = 0, big_fraction: float = 0.05,
@@ -240,12 +253,13 @@ def main():
     ap.add_argument("--min-steps", type=int, default=50)
     ap.add_argument("--max-steps", type=int, default=2000)
     ap.add_argument("--no-drain", action="store_true", help="stop at the end of the arrival window")
+    ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines on stderr (0: none)")
     a = ap.parse_args()
     window = a.minutes * 60
     reqs = poisson_requests(a.rate, window, a.seed, steps=(a.min_steps, a.max_steps))
     offered = sum(r.params.cells * r.steps for r in reqs) / window * 1e-6
     srv = Server(a.budget_gb * 1e9)
-    out = srv.run(reqs, horizon=window, drain=not a.no_drain)
+    out = srv.run(reqs, horizon=window, drain=not a.no_drain, progress=a.progress)
     # the load the arrivals offer against what the GPU served: a ratio below
     # one means the queue grows for as long as arrivals continue
     out.update({"arrival_rate_per_s": a.rate, "window_s": window, "steps_per_request": [a.min_steps, a.max_steps],
