@@ -53,7 +53,7 @@ def subroutine_args(text):
 def test_modules_built_with_reference_names():
     if not os.path.isdir(MODDIR):
         pytest.skip("Fortran modules not built (amdflang absent)")
-    for mod in list(MODULES) + ["tpls_core_c", "tpls_pressure", "tpls_io_native"]:
+    for mod in list(MODULES) + ["tpls_core_c", "tpls_pressure", "tpls_io_native", "tpls_mpi", "tpls_mirror"]:
         assert os.path.exists(os.path.join(MODDIR, mod + ".mod")), mod
     assert os.access(DRIVER, os.X_OK)
 
@@ -78,6 +78,21 @@ def test_pressure_solver_argument_lists_match_reference():
     ours = subroutine_args(open(MODSRC).read())
     ref = subroutine_args(open(os.path.join(REF_SRC, "pressure_solver.F90")).read())
     for name in ("output_converged_reason", "copy_pressure_out_of_petsc"):
+        assert ours[name] == ref[name], (name, ours[name], ref[name])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent")
+def test_mpi_argument_lists_match_reference():
+    """Module tpls_mpi keeps mpi.f90's argument lists for the procedures
+    main_ns_hybrid.F90 and its set-up call (the exchanges, the decomposition,
+    the neighbours, the stride datatypes); exchange2d_augaug* are called only
+    inside advect_phi.f90, which the GPU modules replace."""
+    ours = subroutine_args(open(MODSRC).read())
+    ref = subroutine_args(open(os.path.join(REF_SRC, "mpi.f90")).read())
+    names = [n for n in ref if not n.startswith("exchange2d_augaug")]
+    assert len(names) == 11
+    for name in names:
+        assert name in ours, name
         assert ours[name] == ref[name], (name, ours[name], ref[name])
 
 

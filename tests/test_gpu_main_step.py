@@ -112,5 +112,6 @@ def test_main_step_through_the_modules(tmp_path):
     gpu.close()
     mb = (c1[0] + c1[1]) / NSTEPS / 1e6
     print(f"[main-step] s/step: modules+mirror {s1['sec'].tolist()} ({mb:.0f} MB over the host link per step, "
-          f"{c1[2] / NSTEPS:.0f} faults per step), modules copying {s0['sec'].tolist()}, "
+          f"{c1[2] / NSTEPS:.0f} faults, {(c1[3] + c1[4]) / NSTEPS:.0f} copies per step), "
+          f"modules copying {s0['sec'].tolist()}, "
           f"tpls_step {t_case:.4f}", flush=True)

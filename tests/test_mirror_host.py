@@ -102,8 +102,16 @@ def test_attach_uploads_once_and_reads_are_free(M, arr):
     assert M.mh_to_device(arr.ctypes.data, arr.nbytes) == 0
     d = delta(c0, counters(M))
     assert d["faults"] == 0
-    # only the two partial end pages cross: they are never protected
-    assert d["up"] < 2 * PAGE and d["down"] == 0
+    # nothing changed: nothing crosses, not even the unprotected end pages
+    assert d["up"] == 0 and d["down"] == 0
+    # the end pages are never protected: a write there is found by comparison
+    arr[0] = 5.0
+    arr[-1] = -5.0
+    assert counters(M)["faults"] == c0["faults"]
+    assert M.mh_to_device(arr.ctypes.data, arr.nbytes) == 0
+    d = delta(c0, counters(M))
+    assert 0 < d["up"] < 2 * PAGE
+    assert dev(M, arr)[0] == 5.0 and dev(M, arr)[-1] == -5.0
 
 
 def test_host_write_uploads_only_the_touched_granule(M, arr):

@@ -33,6 +33,9 @@ struct Entry {
     char* d = nullptr;         // device copy of the n bytes
     size_t gsz = 0, ng = 0;    // granule bytes, granule count over [p0, p1)
     uint8_t* st = nullptr;     // per-granule state
+    // the end bytes [h, p0) and [p1, h + n) as the device copy holds them, so
+    // that an upload of unchanged end bytes is skipped
+    char* shadow = nullptr;
 };
 
 Backend g_be{};
@@ -104,6 +107,31 @@ int down(Entry* e, uintptr_t a, uintptr_t b)
     if (g_be.download(reinterpret_cast<void*>(a), e->d + (a - e->h), b - a)) return -1;
     g_ctr[1] += static_cast<long long>(b - a);
     g_ctr[4] += 1;
+    return 0;
+}
+
+// The unprotected end bytes: uploaded when they differ from what the device
+// copy holds, downloaded after every device write.
+int ends_up(Entry* e)
+{
+    const size_t nh = e->p0 - e->h, nt = e->h + e->n - e->p1;
+    if (nh && std::memcmp(e->shadow, reinterpret_cast<const void*>(e->h), nh) != 0) {
+        if (up(e, e->h, e->p0)) return -1;
+        std::memcpy(e->shadow, reinterpret_cast<const void*>(e->h), nh);
+    }
+    if (nt && std::memcmp(e->shadow + nh, reinterpret_cast<const void*>(e->p1), nt) != 0) {
+        if (up(e, e->p1, e->h + e->n)) return -1;
+        std::memcpy(e->shadow + nh, reinterpret_cast<const void*>(e->p1), nt);
+    }
+    return 0;
+}
+
+int ends_down(Entry* e)
+{
+    const size_t nh = e->p0 - e->h, nt = e->h + e->n - e->p1;
+    if (down(e, e->h, e->p0) || down(e, e->p1, e->h + e->n)) return -1;
+    std::memcpy(e->shadow, reinterpret_cast<const void*>(e->h), nh);
+    std::memcpy(e->shadow + nh, reinterpret_cast<const void*>(e->p1), nt);
     return 0;
 }
 
@@ -255,6 +283,7 @@ void drop(Entry* e)
 {
     if (e->d && g_have_be) g_be.release(e->d);
     delete[] e->st;
+    delete[] e->shadow;
     delete e;
 }
 
@@ -285,8 +314,7 @@ int to_device(const void* host, size_t bytes)
     Entry* e = exact(host, bytes);
     if (!e) return fail(TPLS_ERR_ARG, "mirror: not an attached array");
     install();
-    if (up(e, e->h, e->p0) || up(e, e->p1, e->h + e->n))
-        return fail(TPLS_ERR_HIP, "mirror: host-to-device copy failed");
+    if (ends_up(e)) return fail(TPLS_ERR_HIP, "mirror: host-to-device copy failed");
     return runs(e, 0, e->ng, DIRTY, [&](size_t a, size_t b) {
         const uintptr_t x = gbeg(e, a), y = gend(e, b - 1);
         // read-only first, so that a host write during the copy faults
@@ -306,8 +334,7 @@ int written(const void* host, size_t bytes)
         if (protect(e->p0, e->p1, PROT_NONE)) return fail(TPLS_ERR_ARG, "mirror: mprotect failed");
         std::fill(e->st, e->st + e->ng, uint8_t(STALE));
     }
-    if (down(e, e->h, e->p0) || down(e, e->p1, e->h + e->n))
-        return fail(TPLS_ERR_HIP, "mirror: device-to-host copy failed");
+    if (ends_down(e)) return fail(TPLS_ERR_HIP, "mirror: device-to-host copy failed");
     return 0;
 }
 
@@ -369,6 +396,10 @@ int tpls_mirror_attach(void* host, long long bytes)
     e->ng = (e->p1 - e->p0 + e->gsz - 1) / e->gsz;
     e->st = new uint8_t[std::max<size_t>(1, e->ng)];
     std::fill(e->st, e->st + std::max<size_t>(1, e->ng), uint8_t(CLEAN));
+    const size_t nh = e->p0 - h, nt = h + n - e->p1;
+    e->shadow = new char[std::max<size_t>(1, nh + nt)];
+    std::memcpy(e->shadow, host, nh);
+    std::memcpy(e->shadow + nh, reinterpret_cast<const void*>(e->p1), nt);
     e->d = static_cast<char*>(g_be.alloc(n));
     if (!e->d) {
         drop(e);

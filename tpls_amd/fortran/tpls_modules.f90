@@ -816,6 +816,63 @@ contains
     is_master = (master_id == my_id)
   end function is_master
 
+  ! mpi.f90:26-49 with one process in x and y: sx = 1, ex = maxl-1, sy = 1,
+  ! ey = maxm-1 (the same arithmetic with dims = 1, coords = 0)
+  subroutine mpi_decomp_2d(sx, ex, sy, ey, n_local_x, n_local_y, maxl, maxm, coords, dims, Ndim)
+    integer, intent(inout) :: sx, ex, sy, ey, n_local_x, n_local_y
+    integer, intent(in) :: maxl, maxm, Ndim
+    integer, intent(in) :: coords(Ndim), dims(Ndim)
+    n_local_x = (maxl - 1) / dims(1)
+    sx = coords(1) * n_local_x + 1
+    ex = sx + n_local_x - 1
+    n_local_y = (maxm - 1) / dims(2)
+    sy = coords(2) * n_local_y + 1
+    ey = sy + n_local_y - 1
+  end subroutine mpi_decomp_2d
+
+  ! mpi.f90:52-66 on one rank: the y neighbours are the rank itself, the x
+  ! and z neighbours none (-1); the exchanges above read none of them
+  subroutine get_mpi_neighbours(neighbours, comm3d)
+    integer :: neighbours(6), comm3d
+    neighbours = -1
+    neighbours(1) = 0
+    neighbours(3) = 0
+  end subroutine get_mpi_neighbours
+
+  ! mpi.f90's stride datatypes (get_stride_p*, :69-604) describe the halo
+  ! planes for MPI; the single-rank exchanges take the bounds instead, so
+  ! the handles are set to 0
+  subroutine get_stride_p(stride_p_yz, stride_p_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_xz, stride_p_yz, ex, ey, ez, sx, sy, sz
+    stride_p_xz = 0
+    stride_p_yz = 0
+  end subroutine get_stride_p
+  subroutine get_stride_p_aug1(stride_p_aug1_yz, stride_p_aug1_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_aug1_xz, stride_p_aug1_yz, ex, ey, ez, sx, sy, sz
+    stride_p_aug1_xz = 0
+    stride_p_aug1_yz = 0
+  end subroutine get_stride_p_aug1
+  subroutine get_stride_p_aug2(stride_p_aug2_yz, stride_p_aug2_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_aug2_xz, stride_p_aug2_yz, ex, ey, ez, sx, sy, sz
+    stride_p_aug2_xz = 0
+    stride_p_aug2_yz = 0
+  end subroutine get_stride_p_aug2
+  subroutine get_stride_p_augaug1(stride_p_augaug1_yz, stride_p_augaug1_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_augaug1_xz, stride_p_augaug1_yz, ex, ey, ez, sx, sy, sz
+    stride_p_augaug1_xz = 0
+    stride_p_augaug1_yz = 0
+  end subroutine get_stride_p_augaug1
+  subroutine get_stride_p_augaug2(stride_p_augaug2_yz, stride_p_augaug2_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_augaug2_xz, stride_p_augaug2_yz, ex, ey, ez, sx, sy, sz
+    stride_p_augaug2_xz = 0
+    stride_p_augaug2_yz = 0
+  end subroutine get_stride_p_augaug2
+  subroutine get_stride_p_augaug3(stride_p_augaug2_yz, stride_p_augaug2_xz, sx, ex, sy, ey, sz, ez)
+    integer :: stride_p_augaug2_xz, stride_p_augaug2_yz, ex, ey, ez, sx, sy, sz
+    stride_p_augaug2_xz = 0
+    stride_p_augaug2_yz = 0
+  end subroutine get_stride_p_augaug3
+
   subroutine exchange2d(uu, stride_p_xz, stride_p_yz, neighbours, ex, ey, ez, sx, sy, sz, comm_topology)
     integer, intent(in) :: ex, ey, ez, sx, sy, sz
     integer, intent(in) :: stride_p_xz, stride_p_yz
