@@ -67,6 +67,9 @@ def parse():
                     help="cases of the batched FP32 run reported beside the single case (0 to skip)")
     ap.add_argument("--batch-steps", type=int, default=4, help="timed batched steps")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
+    ap.add_argument("--batch-grouped", type=int, default=1,
+                    help="1: the batch's pressure solves run grouped (one launch per kernel across the cases); "
+                         "2: the Helmholtz loops as well; 0: each case on its own stream")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving run reported beside the single case (0 to skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0,
@@ -318,8 +321,9 @@ def batched_run(a):
     latency (every case advances one step per batch step)."""
     import numpy as np
     from tpls_amd.batch import batch_configs
-    from tpls_amd.case import Case, batch_step
+    from tpls_amd.case import Case, batch_step, set_batch_grouped
     from tpls_amd.initial import make_initial
+    set_batch_grouped(a.batch_grouped)
     cfgs = batch_configs(a.batch_cases, seed=0)
     cases = []
     for p in cfgs:
@@ -345,6 +349,8 @@ def batched_run(a):
             "p50_step_ms": round(float(np.percentile(lat_ms, 50)), 3),
             "p99_step_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "cg_its_mean": round(float(np.mean(its)), 1),
+            "grouping": {0: "each case on its own stream", 1: "pressure solves grouped across the cases",
+                         2: "Helmholtz loops and pressure solves grouped across the cases"}[a.batch_grouped],
             "grids": "seeded draw from 64x16x32, 128x32x64, 192x48x96, 256x64x128"}
 
 

@@ -171,6 +171,13 @@ int tpls_batch_step(tpls_case **cases, int ncases, int nsteps);
  * a failure here: it is reported in the case's tpls_diag.ksp_reason (< 0),
  * as KSPGetConvergedReason reports it (main_ns_hybrid.F90:903-911). */
 int tpls_batch_step_status(tpls_case **cases, int ncases, int nsteps, int *status);
+/* How tpls_batch_step runs a batch: mode 0 runs each case's step on its own
+ * stream; mode 1 (the default) runs the pressure solves grouped: every CG
+ * iteration and every multigrid kernel once across the cases, on one stream,
+ * in lock-step; mode 2 groups the Helmholtz loops and the solve's set-up as
+ * well.  Each case computes exactly what its own step computes in every
+ * mode.  Batches with a case under kernel timing run per case. */
+int tpls_batch_set_grouped(int mode);
 /* Diagnostics of the last step. */
 int tpls_case_diag(tpls_case *c, tpls_diag *out);
 /* Change the launch shapes of the Helmholtz sweeps (the tile_rows,

@@ -45,11 +45,19 @@ def _run(cases, precs, steps, together, graphs=False):
     return out
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_batch_equals_alone(graphs):
+@pytest.mark.parametrize("graphs,grouped", [(False, 1), (True, 1), (False, 2), (True, 2), (False, 0), (True, 0)])
+def test_batch_equals_alone(graphs, grouped):
+    """Grouping mode 1: the cases' pressure solves run in lock-step, one
+    launch per kernel across them; mode 2: the Helmholtz loops and the
+    solve's set-up as well; mode 0: each case on its own stream."""
+    from tpls_amd.case import set_batch_grouped
     cases, precs = _mix()
     alone = _run(cases, precs, 4, together=False)
-    mixed = _run(cases, precs, 4, together=True, graphs=graphs)
+    set_batch_grouped(grouped)
+    try:
+        mixed = _run(cases, precs, 4, together=True, graphs=graphs)
+    finally:
+        set_batch_grouped(1)
     for q, ((fa, da), (fb, db)) in enumerate(zip(alone, mixed)):
         print(f"[batch] case {q} {cases[q].nx}x{cases[q].ny}x{cases[q].nz} {precs[q]} "
               f"its {da['ksp_its']}/{db['ksp_its']}", flush=True)

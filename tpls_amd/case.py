@@ -238,6 +238,14 @@ def batch_step(cases, nsteps: int = 1) -> None:
     _lib.check(_lib.lib().tpls_batch_step(arr, len(cases), int(nsteps)), "tpls_batch_step")
 
 
+def set_batch_grouped(mode) -> None:
+    """How batch_step runs a batch (tpls_batch_set_grouped): 0 (or False)
+    each case on its own stream; 1 (or True, the default) the pressure
+    solves grouped, one launch per kernel across the cases; 2 the Helmholtz
+    loops and the solve's set-up grouped as well."""
+    _lib.check(_lib.lib().tpls_batch_set_grouped(int(mode)), "tpls_batch_set_grouped")
+
+
 def batch_step_status(cases, nsteps: int = 1) -> list:
     """batch_step that isolates failures (tpls_batch_step_status): returns one
     status per case, 0 or the error code of that case's step (its message

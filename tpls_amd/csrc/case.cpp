@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
@@ -14,6 +16,7 @@
 
 #include "../../include/tpls_core.h"
 #include "../../include/tpls_io.h"
+#include "group.hpp"
 #include "host.hpp"
 #include "mirror.hpp"
 #include "tpls_kernels.hpp"
@@ -108,6 +111,12 @@ struct tpls_case {
     int broken = 0;
     std::string broken_msg;
     int fail_next = 0;  // test hook: the next step_begin fails before any launch
+    uint64_t uid = 0;   // unique for the life of the process (cache keys)
+    // the step runs grouped with other cases' (tpls_batch_step): 1, the
+    // pressure solve's iterations (the case's own launches leave out the
+    // set-up's V-cycle); 2, also the Helmholtz loops and the solve's set-up
+    // (the case's own launches stop after the momentum right-hand sides)
+    int group_solve = 0;
     // the first step's launches and the first CG chunk run eagerly, so that
     // every launcher's one-time device queries (resident_blocks) happen
     // before any stream capture, never inside one
@@ -509,6 +518,8 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TPLS_ERR_NOGPU, "no HIP device");
     tpls_case* c = new tpls_case;
+    static std::atomic<uint64_t> next_uid{1};
+    c->uid = next_uid++;
     c->maxl = maxl;
     c->maxm = maxm;
     c->maxn = maxn;
@@ -1045,7 +1056,7 @@ int ps_setup(tpls_case* c, FieldView<T> u3, FieldView<T> v3, FieldView<T> w3, Fi
     LCHK(launch_cg_init(c->s, c->cg, c->prm.ksp_rtol, c->prm.ksp_abstol, c->prm.ksp_dtol, c->prm.ksp_maxits));
     LCHK(launch_cg_setup_b<T>(c->s, g, c->ph, u3, v3, w3, inlet, rhs_p, use_rhs, r, red_of(c, 7), c->cg));
     LCHK(launch_cg_setup_r<T>(c->s, g, c->ph, c->pold->view<double>(), r, red_of(c, 7), c->cg));
-    if (c->use_mg) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
+    if (c->use_mg && !c->group_solve) LCHK(launch_mg_vcycle(c->s, g, c->mg, red_of(c, 7), c->cg));
     return TPLS_OK;
 }
 
@@ -1194,6 +1205,8 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
         LCHK(launch_momentum<T>(s, g, ph, q, 4, V(c->x[0][c->i2[0]]), V(c->x[1][c->i2[1]]), V(c->x[2][c->i2[2]]),
                                 V(c->mu), V(c->conv[q][2]), V(c->rhs[q]), V(fcs[q]), V(c->conv[q][0]),
                                 V(c->conv[q][1]), V(c->csf[q][0]), V(c->csf[q][1]), V(c->csf[q][2]), V(c->rhs[q])));
+    // a step grouped with the Helmholtz loops runs the rest with other cases'
+    if (c->group_solve == 2) return TPLS_OK;
     // ---- Helmholtz outer loops (main:758-887) ----
     const int mom[3] = {c->prm.mom_u, c->prm.mom_v, c->prm.mom_w};
     for (int q = 0; q < 3; ++q) {
@@ -1243,10 +1256,10 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
                        static_cast<const T*>(c->inlet->ptr));
 }
 
-// The step up to the first CG chunk: eager launches, or the replay of a
-// graph captured for the same buffer roles.
+// The step up to the pressure solve's iterations: eager launches, or the
+// replay of a graph captured for the same buffer roles.
 template <typename T>
-int step_begin(tpls_case* c)
+int step_pre(tpls_case* c)
 {
     if (c->fail_next) {
         c->fail_next = 0;
@@ -1260,6 +1273,7 @@ int step_begin(tpls_case* c)
         std::vector<const void*> key = {c->phi2, c->phi3, c->pra, c->prb, c->cphi[0], c->cphi[1], c->cphi[2]};
         // dim's first step starts its iterate from zero, later ones from phi2
         key.push_back(reinterpret_cast<const void*>(uintptr_t(c->prm.idm_dim && c->steps_done == 0)));
+        key.push_back(reinterpret_cast<const void*>(uintptr_t(c->group_solve)));
         for (int q = 0; q < 3; ++q) {
             key.push_back(c->x[q][c->i2[q]]);
             for (int b = 0; b < 3; ++b) {
@@ -1287,6 +1301,15 @@ int step_begin(tpls_case* c)
     c->pra = pl.pra;
     c->prb = pl.prb;
     for (int q = 0; q < 3; ++q) c->fin[q] = pl.fin[q];
+    return TPLS_OK;
+}
+
+// The step up to the first CG chunk.
+template <typename T>
+int step_begin(tpls_case* c)
+{
+    c->group_solve = 0;
+    RC(step_pre<T>(c));
     ps_reset(c);
     return ps_enqueue_chunk(c);
 }
@@ -1358,6 +1381,219 @@ int step_impl(tpls_case* c)
     if (!rc) rc = step_end<T>(c);
     if (!rc) rc = finish_diag(c);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// The pressure solves of a batch in lock-step: every CG iteration and every
+// V-cycle kernel runs as one launch across the cases (group.hpp), on the
+// group's own stream, instead of one launch per case on each case's stream.
+// The small grids of a batch are launch-bound one by one; grouped, a level's
+// sweep of 64 cases is one launch with the blocks of all of them.  Each case
+// computes exactly what its own launches compute: the kernels of a case
+// whose solve has converged return at once, as they do in a chunk launched
+// past convergence.
+// ---------------------------------------------------------------------------
+int g_batch_grouped = 1;
+
+struct SolveGroup {
+    std::vector<uint64_t> key;        // the cases' uids, in batch order
+    std::vector<double> coef;         // their level-0 coefficients
+    GroupPlan vc, it[2], gather;      // first V-cycle; iteration by p parity; state gather
+    GroupPlan hh;                     // this step's Helmholtz loops and solve set-up
+    CgState* d_st = nullptr;          // gathered states, read back in one copy
+    CgState* h_st = nullptr;
+    int n = 0;
+    bool any_mg = false;
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr, done = nullptr;
+    ~SolveGroup()
+    {
+        if (s) (void)hipStreamSynchronize(s);
+        if (d_st) (void)hipFree(d_st);
+        if (h_st) (void)hipHostFree(h_st);
+        if (ev) (void)hipEventDestroy(ev);
+        if (done) (void)hipEventDestroy(done);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+thread_local std::unique_ptr<SolveGroup> t_group;
+
+std::vector<double> group_coef(const std::vector<tpls_case*>& cs)
+{
+    std::vector<double> v;
+    for (tpls_case* c : cs) {
+        v.push_back(c->ph.dx);
+        v.push_back(c->ph.dy);
+        v.push_back(c->ph.dz);
+        if (c->use_mg) {
+            const MgLevel& o = c->mg.L[0];
+            v.insert(v.end(), {o.cx, o.cy, o.cz, o.diag});
+        }
+    }
+    return v;
+}
+
+// The group for these cases, built (tables, buffers, stream) on first use
+// and kept while the batch keeps its cases.
+int group_for(const std::vector<tpls_case*>& cs, SolveGroup** out)
+{
+    std::vector<uint64_t> key;
+    for (tpls_case* c : cs) key.push_back(c->uid);
+    const std::vector<double> coef = group_coef(cs);
+    if (t_group && t_group->key == key && t_group->coef == coef) {
+        *out = t_group.get();
+        return TPLS_OK;
+    }
+    t_group.reset();
+    auto G = std::make_unique<SolveGroup>();
+    G->key = key;
+    G->coef = coef;
+    G->n = int(cs.size());
+    HIPCHK(hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&G->ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&G->done, hipEventDisableTiming));
+    if (hipMalloc(&G->d_st, sizeof(CgState) * cs.size()) != hipSuccess) return fail(TPLS_ERR_NOMEM, "group states");
+    if (hipHostMalloc(&G->h_st, sizeof(CgState) * cs.size()) != hipSuccess) return fail(TPLS_ERR_NOMEM, "pinned");
+    std::vector<MgGroupCase> mg;
+    std::vector<CgGroupCase> cg;
+    std::vector<const CgState*> src;
+    for (tpls_case* c : cs) {
+        const bool z32 = c->use_mg && c->mg.fp32;
+        CgGroupCase k{};
+        k.g = c->g;
+        k.ph = c->ph;
+        k.x = c->pold->view<double>();
+        k.r = c->cg_r->view<double>();
+        k.pb[0] = c->cg_p0->view<double>();
+        k.pb[1] = c->cg_p1->view<double>();
+        k.z32 = z32 ? 1 : 0;
+        k.z = c->use_mg && !z32 ? c->cg_z->view<double>() : k.r;
+        if (z32) k.zf = c->mg.F[0].x;
+        k.use_mz = c->use_mg ? 1 : 0;
+        k.part = c->red_part;
+        k.st = c->cg;
+        cg.push_back(k);
+        if (c->use_mg) mg.push_back(MgGroupCase{&c->mg, c->g, c->red_part, c->cg});
+        src.push_back(c->cg);
+    }
+    G->any_mg = !mg.empty();
+    if (mg_group_vcycle(G->vc, mg)) return fail(TPLS_ERR_ARG, "a multigrid hierarchy the grouped solve cannot run");
+    for (int cur = 0; cur < 2; ++cur) {
+        cg_group_iteration(G->it[cur], cg, cur);
+        mg_group_vcycle(G->it[cur], mg);
+    }
+    cg_group_gather(G->gather, src, G->d_st);
+    for (GroupPlan* P : {&G->vc, &G->it[0], &G->it[1], &G->gather}) HIPCHK(P->upload());
+    t_group = std::move(G);
+    *out = t_group.get();
+    return TPLS_OK;
+}
+
+// Solve the pressure systems of the cases whose step_pre has been enqueued
+// (each with group_solve set), then enqueue their step_end on their own
+// streams.  A failure fails every case of the group.
+int group_solve(const std::vector<tpls_case*>& cs, int mode)
+{
+    SolveGroup* G = nullptr;
+    RC(group_for(cs, &G));
+    // mode 2: the Helmholtz loops and the set-up, in this step's buffer roles
+    G->hh.clear();
+    std::vector<HhGroupCase<double>> h64;
+    std::vector<HhGroupCase<float>> h32;
+    std::vector<SetupGroupCase<double>> s64;
+    std::vector<SetupGroupCase<float>> s32;
+    bool diag = false;
+    auto fill = [&](tpls_case* c, auto& hv, auto& sv, auto zero) {
+        using T = decltype(zero);
+        HhGroupCase<T> h{};
+        h.g = c->g;
+        h.ph = c->ph;
+        for (int q = 0; q < 3; ++q) {
+            for (int b = 0; b < 3; ++b) h.x[q][b] = c->x[q][b]->view<T>();
+            h.i2[q] = c->i2[q];
+            h.rhs[q] = c->rhs[q]->view<T>();
+        }
+        h.mu = c->mu->view<T>();
+        h.pres = c->pres->view<T>();
+        h.mom[0] = c->prm.mom_u;
+        h.mom[1] = c->prm.mom_v;
+        h.mom[2] = c->prm.mom_w;
+        h.tl = c->tiling;
+        const bool dg = c->prm.diagnostics != 0;
+        diag = diag || dg;
+        h.part = dg ? c->red_part : nullptr;
+        h.out = c->red_out + 1;
+        hv.push_back(h);
+        SetupGroupCase<T> su{};
+        su.g = c->g;
+        su.ph = c->ph;
+        su.u3 = c->x[0][c->fin[0]]->view<T>();
+        su.v3 = c->x[1][c->fin[1]]->view<T>();
+        su.w3 = c->x[2][c->fin[2]]->view<T>();
+        su.inlet = static_cast<const T*>(c->inlet->ptr);
+        su.rhs_p = c->cg_r->view<double>();
+        su.r = c->cg_r->view<double>();
+        su.x = c->pold->view<double>();
+        su.part = c->red_part;
+        su.st = c->cg;
+        su.rtol = c->prm.ksp_rtol;
+        su.abstol = c->prm.ksp_abstol;
+        su.dtol = c->prm.ksp_dtol;
+        su.maxits = c->prm.ksp_maxits;
+        sv.push_back(su);
+    };
+    if (mode == 2) {
+        for (tpls_case* c : cs) {
+            if (c->prec == TPLS_FP32) fill(c, h32, s32, 0.0f);
+            else fill(c, h64, s64, 0.0);
+        }
+        RC(hh_group_loops<double>(G->hh, h64, diag) ? fail(TPLS_ERR_HIP, "grouped Helmholtz loops") : TPLS_OK);
+        RC(hh_group_loops<float>(G->hh, h32, diag) ? fail(TPLS_ERR_HIP, "grouped Helmholtz loops") : TPLS_OK);
+        cg_group_setup<double>(G->hh, s64);
+        cg_group_setup<float>(G->hh, s32);
+        LCHK(G->hh.upload());
+    }
+    for (tpls_case* c : cs) {
+        LCHK(hipEventRecord(c->ps_ev, c->s));
+        LCHK(hipStreamWaitEvent(G->s, c->ps_ev, 0));
+    }
+    LCHK(G->hh.run(G->s));
+    LCHK(G->vc.run(G->s));
+    long first = 1, launched = 0, maxits = 1;
+    for (tpls_case* c : cs) {
+        first = std::max<long>(first, c->use_mg ? std::max(4, c->last_its) : std::max(8, c->last_its - 2));
+        maxits = std::max<long>(maxits, c->prm.ksp_maxits);
+    }
+    int cur = 0;
+    for (long chunk = first;;) {
+        chunk = std::min<long>(chunk, std::max<long>(1, maxits - launched));
+        for (long q = 0; q < chunk; ++q) {
+            LCHK(G->it[cur].run(G->s));
+            cur ^= 1;
+        }
+        launched += chunk;
+        LCHK(G->gather.run(G->s));
+        LCHK(hipMemcpyAsync(G->h_st, G->d_st, sizeof(CgState) * G->n, hipMemcpyDeviceToHost, G->s));
+        LCHK(hipEventRecord(G->ev, G->s));
+        LCHK(hipEventSynchronize(G->ev));
+        bool all = true;
+        for (int q = 0; q < G->n; ++q)
+            if (!G->h_st[q].done && launched < cs[q]->prm.ksp_maxits) all = false;
+        if (all) break;
+        chunk = G->any_mg ? 2 : 16;
+    }
+    LCHK(hipEventRecord(G->done, G->s));
+    for (int q = 0; q < G->n; ++q) {
+        tpls_case* c = cs[q];
+        *c->h_cg = G->h_st[q];
+        c->diag.ksp_its = G->h_st[q].its;
+        c->diag.ksp_reason = G->h_st[q].reason;
+        c->diag.ksp_rnorm = G->h_st[q].dp;
+        c->last_its = G->h_st[q].its;
+        LCHK(hipStreamWaitEvent(c->s, G->done, 0));
+    }
+    return TPLS_OK;
 }
 
 }  // namespace
@@ -2085,7 +2321,42 @@ int tpls_batch_step_status(tpls_case** cases, int ncases, int nsteps, int* statu
             fail(TPLS_ERR_ARG, "the case failed in an earlier step: " + cases[q]->broken_msg);
             drop(q, TPLS_ERR_ARG);
         }
+    int grouped = g_batch_grouped;
+    for (int q = 0; q < ncases; ++q)
+        if (cases[q]->timed) grouped = 0;  // kernel timing brackets per-case launches
     for (int n = 0; n < nsteps; ++n) {
+        if (grouped) {
+            std::vector<tpls_case*> cs;
+            std::vector<int> qs;
+            for (int q = 0; q < ncases; ++q) {
+                if (st[q]) continue;
+                tpls_case* c = cases[q];
+                c->group_solve = grouped;
+                const int rc = c->prec == TPLS_FP32 ? step_pre<float>(c) : step_pre<double>(c);
+                if (rc) {
+                    drop(q, rc);
+                    continue;
+                }
+                cs.push_back(c);
+                qs.push_back(q);
+            }
+            if (cs.empty()) continue;
+            int rc = group_solve(cs, grouped);
+            if (rc) {
+                const std::string msg = g_err;
+                for (int q : qs) {
+                    g_err = msg;
+                    drop(q, rc);
+                }
+                continue;
+            }
+            for (size_t k = 0; k < cs.size(); ++k) {
+                tpls_case* c = cs[k];
+                rc = c->prec == TPLS_FP32 ? step_end<float>(c) : step_end<double>(c);
+                if (rc) drop(qs[k], rc);
+            }
+            continue;
+        }
         int left = 0;
         for (int q = 0; q < ncases; ++q) {
             if (st[q]) continue;
@@ -2146,6 +2417,13 @@ int tpls_batch_step_status(tpls_case** cases, int ncases, int nsteps, int* statu
 int tpls_batch_step(tpls_case** cases, int ncases, int nsteps)
 {
     return tpls_batch_step_status(cases, ncases, nsteps, nullptr);
+}
+
+int tpls_batch_set_grouped(int mode)
+{
+    if (mode < 0 || mode > 2) return fail(TPLS_ERR_ARG, "grouping mode must be 0, 1 or 2");
+    g_batch_grouped = mode;
+    return TPLS_OK;
 }
 
 // Test hook, not part of the public interface: the case's next step fails

@@ -55,6 +55,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "group.hpp"
 #include "reduce.hpp"
 #include "tiles.hpp"
 #include "tpls_kernels.hpp"
@@ -110,12 +111,26 @@ __device__ __forceinline__ int mg_nbr(int f, int nc, bool periodic)
 // b converted to the cycle's type, which is all the later kernels of the
 // level use of it.
 template <typename X, typename B>
-__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X> bc, const CgState* st)
+struct MgPre {
+    struct Args {
+        MgLevelT<X, B> o;
+        FieldView<X> bc;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename X, typename B>
+__device__ __forceinline__ void MgPre<X, B>::run(const Args& a, const Blk& bk)
 {
     using C = X;
-    if (st->done) return;
-    const int i0 = 2 * (blockIdx.x * BX + threadIdx.x) + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const MgLevelT<X, B>& o = a.o;
+    const FieldView<X>& bc = a.bc;
+    if (a.st->done) return;
+    const int i0 = 2 * (bk.x * BX + int(threadIdx.x)) + 1;
+    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
     if (i0 > o.nx || j > o.ny) return;
     const int ir = ((i0 + j + k) & 1) == 0 ? i0 : i0 + 1;
     const int ib = ir == i0 ? i0 + 1 : i0;
@@ -128,19 +143,41 @@ __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X
     }
 }
 
+template <typename X, typename B>
+__global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X> bc, const CgState* st)
+{
+    MgPre<X, B>::run({o, bc, st}, blk_here());
+}
+
 // One colour sweep.  A thread owns the pair (i0, i0+1) along x and updates
 // the one of the sweep's colour (nx is even on every smoothed level), so
 // every lane of a wavefront has a cell to update.
 template <typename X, typename B>
+struct MgSmooth {
+    struct Args {
+        MgLevelT<X, B> o;
+        int colour;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk)
+    {
+        using C = X;
+        const MgLevelT<X, B>& o = a.o;
+        if (a.st->done) return;
+        const int i0 = 2 * (bk.x * BX + int(threadIdx.x)) + 1;
+        const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
+        if (i0 > o.nx || j > o.ny) return;
+        const int i = ((i0 + j + k) & 1) == a.colour ? i0 : i0 + 1;
+        o.x(i, j, k) = sdiv(C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
+    }
+};
+
+template <typename X, typename B>
 __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevelT<X, B> o, int colour, const CgState* st)
 {
-    using C = X;
-    if (st->done) return;
-    const int i0 = 2 * (blockIdx.x * BX + threadIdx.x) + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-    if (i0 > o.nx || j > o.ny) return;
-    const int i = ((i0 + j + k) & 1) == colour ? i0 : i0 + 1;
-    o.x(i, j, k) = sdiv(C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
+    MgSmooth<X, B>::run({o, colour, st}, blk_here());
 }
 
 // Weights of the fine cells 2I-2+q (q = 0..3) feeding coarse cell I under the
@@ -182,15 +219,30 @@ struct RrIn {
 };
 
 template <typename XF, typename BF, typename XC>
-__global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, int ntx, int nty, int rk,
-                                                const CgState* st)
+struct MgRr {
+    struct Args {
+        MgLevelT<XF, BF> f;
+        MgLevelT<XC, XC> c;
+        int ntx, nty, rk;
+        const CgState* st;
+    };
+    static constexpr int THREADS = RR_T;
+    static dim3 block() { return dim3(RR_T); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& blk);
+};
+
+template <typename XF, typename BF, typename XC>
+__device__ __forceinline__ void MgRr<XF, BF, XC>::run(const Args& a, const Blk& blk)
 {
     using C = XF;
-    if (st->done) return;
+    const MgLevelT<XF, BF>& f = a.f;
+    const MgLevelT<XC, XC>& c = a.c;
+    const int ntx = a.ntx, nty = a.nty, rk = a.rk;
+    if (a.st->done) return;
     __shared__ C T[RR_FY][RR_FX];
     __shared__ C SI[RR_FY][RR_CX];
     const int tid = threadIdx.x;
-    const int bk = blockIdx.x, tx = bk % ntx, r0 = bk / ntx, ty = r0 % nty, tz = r0 / nty;
+    const int bk = blk.x, tx = bk % ntx, r0 = bk / ntx, ty = r0 % nty, tz = r0 / nty;
     const int I0 = 1 + tx * RR_CX, J0 = 1 + ty * RR_CY, Kb = 1 + tz * rk, Ke = min(Kb + rk - 1, c.nz);
     const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;  // footprint origin (fine, unwrapped)
     // this thread's footprint cells: (fine i, wrapped fine j, its y neighbours)
@@ -285,6 +337,13 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevelT<XF, BF> f, MgLevelT<XC,
     }
 }
 
+template <typename XF, typename BF, typename XC>
+__global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, int ntx, int nty, int rk,
+                                                const CgState* st)
+{
+    MgRr<XF, BF, XC>::run({f, c, ntx, nty, rk, st}, blk_here());
+}
+
 // x_f += P e_c (z first, then y, then x)
 // x += P e for the two fine cells 2I-1, 2I of coarse column I (fine nx is
 // twice the coarse one): they share their parent's terms (the four values at
@@ -293,12 +352,26 @@ __global__ void __launch_bounds__(RR_T) k_mg_rr(MgLevelT<XF, BF> f, MgLevelT<XC,
 // values for two cells instead of 16.  Each cell's sum is formed in the same
 // order as for one cell alone.
 template <typename XF, typename BF, typename XC>
-__global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, const CgState* st)
+struct MgProlong {
+    struct Args {
+        MgLevelT<XF, BF> f;
+        MgLevelT<XC, XC> c;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename XF, typename BF, typename XC>
+__device__ __forceinline__ void MgProlong<XF, BF, XC>::run(const Args& a, const Blk& bk)
 {
     using C = XF;
-    if (st->done) return;
-    const int I = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+    const MgLevelT<XF, BF>& f = a.f;
+    const MgLevelT<XC, XC>& c = a.c;
+    if (a.st->done) return;
+    const int I = bk.x * BX + int(threadIdx.x) + 1;
+    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
     if (I > c.nx || j > f.ny) return;
     const int J = (j + 1) / 2, K = (k + 1) / 2;
     const int Jn = mg_nbr(j, c.ny, true), Kn = mg_nbr(k, c.nz, false);
@@ -318,6 +391,12 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLev
     }
 }
 
+template <typename XF, typename BF, typename XC>
+__global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, const CgState* st)
+{
+    MgProlong<XF, BF, XC>::run({f, c, st}, blk_here());
+}
+
 // Coarsest level: one block, a single wavefront for up to 64 cells (its
 // barriers cost next to nothing), else 256 threads; x and b live in LDS
 // (n <= MG_COARSE_MAX).  Each thread's cells, their colours and diagonals are
@@ -326,10 +405,22 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLev
 constexpr int MGC_T = 256, MGC_CPT = MG_COARSE_MAX / MGC_T;
 
 template <typename X>
-__global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevelT<X, X> o, const CgState* st)
+struct MgCoarse {
+    struct Args {
+        MgLevelT<X, X> o;
+        const CgState* st;
+    };
+    static constexpr int THREADS = MGC_T;
+    static dim3 block() { return dim3(MGC_T); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
+
+template <typename X>
+__device__ __forceinline__ void MgCoarse<X>::run(const Args& a, const Blk&)
 {
     using C = X;
-    if (st->done) return;
+    const MgLevelT<X, X>& o = a.o;
+    if (a.st->done) return;
     __shared__ C sx[MG_COARSE_MAX], sb[MG_COARSE_MAX];
     __shared__ C mean;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -390,14 +481,34 @@ __global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevelT<X, X> o, const CgS
         if (col[c] != 2) o.x(ci[c], cj[c], ck[c]) = sx[tid + c * nt] - mx;
 }
 
+template <typename X>
+__global__ void __launch_bounds__(MGC_T) k_mg_coarse(MgLevelT<X, X> o, const CgState* st)
+{
+    MgCoarse<X>::run({o, st}, blk_here());
+}
+
 // Coarsest level by fast diagonalisation, in FP64 whatever the level's
 // storage: b into LDS, b^ = Qz^T Qy^T Qx^T b (one pass per direction, each
 // output summed over its line in index order), b^ / lambda with the constant
 // mode dropped, then z = Qx Qy Qz b^.  tab: mg_spectral_tables.
 template <typename X, typename B>
-__global__ void __launch_bounds__(MGC_T) k_mg_spec(MgLevelT<X, B> o, const double* tab, const CgState* st)
+struct MgSpec {
+    struct Args {
+        MgLevelT<X, B> o;
+        const double* tab;
+        const CgState* st;
+    };
+    static constexpr int THREADS = MGC_T;
+    static dim3 block() { return dim3(MGC_T); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
+
+template <typename X, typename B>
+__device__ __forceinline__ void MgSpec<X, B>::run(const Args& a, const Blk&)
 {
-    if (st->done) return;
+    const MgLevelT<X, B>& o = a.o;
+    const double* tab = a.tab;
+    if (a.st->done) return;
     __shared__ double s0[MG_COARSE_MAX], s1[MG_COARSE_MAX];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nx = o.nx, ny = o.ny, nz = o.nz, n = nx * ny * nz;
@@ -444,18 +555,40 @@ __global__ void __launch_bounds__(MGC_T) k_mg_spec(MgLevelT<X, B> o, const doubl
     }
 }
 
+template <typename X, typename B>
+__global__ void __launch_bounds__(MGC_T) k_mg_spec(MgLevelT<X, B> o, const double* tab, const CgState* st)
+{
+    MgSpec<X, B>::run({o, tab, st}, blk_here());
+}
+
 // Final red sweep on the finest level with the partials of z.r (r = b) and
 // sum z over all cells, in FP64 from the stored z.  With update = 0 the sweep
 // is skipped (a one-level hierarchy, where the coarsest solve already
 // produced z).
 template <typename X>
-__global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevelT<X, double> o, Tiles tl, int update, double* part,
-                                                   const CgState* st)
+struct MgLast {
+    struct Args {
+        MgLevelT<X, double> o;
+        Tiles tl;
+        int update;
+        double* part;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename X>
+__device__ __forceinline__ void MgLast<X>::run(const Args& a, const Blk& bk)
 {
     using C = X;
-    if (st->done) return;
+    const MgLevelT<X, double>& o = a.o;
+    const Tiles& tl = a.tl;
+    const int update = a.update;
+    if (a.st->done) return;
     double acc[2] = {0.0, 0.0};
-    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    for (int t = bk.x; t < tl.ntiles; t += bk.n) {
         int i, j, k0;
         tile_origin(tl, t, i, j, k0);
         if (i > o.nx || j > o.ny) continue;
@@ -472,11 +605,34 @@ __global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevelT<X, double> o, Tiles
             acc[1] += double(z);
         }
     }
-    block_partials<2>(acc, part, gridDim.x, blockIdx.x);
+    block_partials<2>(acc, a.part, bk.n, bk.x);
 }
 
-__global__ void __launch_bounds__(256) k_mg_zf(const double* part, int nb, double N, CgState* st)
+template <typename X>
+__global__ void __launch_bounds__(BX* BY) k_mg_last(MgLevelT<X, double> o, Tiles tl, int update, double* part,
+                                                   const CgState* st)
 {
+    MgLast<X>::run({o, tl, update, part, st}, blk_here());
+}
+
+struct MgZf {
+    struct Args {
+        const double* part;
+        int nb;
+        double N;
+        CgState* st;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
+
+__device__ __forceinline__ void MgZf::run(const Args& a, const Blk&)
+{
+    const double* part = a.part;
+    const int nb = a.nb;
+    const double N = a.N;
+    CgState* st = a.st;
     if (st->done) return;
     double tot[2];
     finish_sums<2>(part, nb, tot);
@@ -495,6 +651,11 @@ __global__ void __launch_bounds__(256) k_mg_zf(const double* part, int nb, doubl
         st->its = st->i + 1;
         st->done = 1;
     }
+}
+
+__global__ void __launch_bounds__(256) k_mg_zf(const double* part, int nb, double N, CgState* st)
+{
+    MgZf::run({part, nb, N, st}, blk_here());
 }
 
 // ---- host side ----
@@ -625,6 +786,223 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     // F[0].b: the FP64 residual in FP32, written by the first pass
     const MgLevel32 topx{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
     return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr);
+}
+
+// ---- the same V-cycle for many cases in lock-step, one launch per kernel
+// and level across the cases (group.hpp) ----
+namespace {
+
+// rr's grid and coarse levels per block, as vcycle() chooses them
+struct RrShape {
+    int ntx, nty, rk, nblk;
+};
+template <typename L>
+RrShape rr_shape(const L& oc)
+{
+    RrShape r;
+    r.ntx = (oc.nx + RR_CX - 1) / RR_CX;
+    r.nty = (oc.ny + RR_CY - 1) / RR_CY;
+    r.rk = 8;
+    while (r.rk > 1 && long(r.ntx) * r.nty * ((oc.nz + r.rk - 1) / r.rk) < 1024) r.rk /= 2;
+    r.nblk = r.ntx * r.nty * ((oc.nz + r.rk - 1) / r.rk);
+    return r;
+}
+
+// One case's levels in the cycle's arithmetic X, as launch_mg_vcycle forms them.
+template <typename X>
+struct CycleView {
+    MgLevelT<X, double> top;
+    bool hx;               // FP32 finest level with its b converted once (topx)
+    MgLevelT<X, X> topx;
+    MgLevelT<X, X> lev[MG_MAXLEV];
+    int last, nu;
+    const double* spec;
+    const MgGroupCase* c;
+};
+
+template <typename X>
+CycleView<X> cycle_view(const MgGroupCase& gc)
+{
+    const MgHierarchy& h = *gc.h;
+    CycleView<X> v{};
+    v.last = h.nlev - 1;
+    v.nu = h.nu;
+    v.spec = h.spec;
+    v.c = &gc;
+    if constexpr (std::is_same_v<X, double>) {
+        v.top = h.L[0];
+        v.hx = false;
+        for (int l = 0; l < h.nlev; ++l) v.lev[l] = h.L[l];
+    } else {
+        for (int l = 0; l < h.nlev; ++l) {
+            const MgLevel& o = h.L[l];
+            v.lev[l] = MgLevel32{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[l].x, h.F[l].b};
+        }
+        const MgLevel& o = h.L[0];
+        v.top = MgLevelT<float, double>{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
+        v.hx = h.F[0].b.p != nullptr;
+        v.topx = MgLevel32{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
+    }
+    return v;
+}
+
+// Smoothing sweeps of one level for a set of cases: colours in the order
+// vcycle() runs them (down: nu-1 pairs red, black after k_mg_pre; up: nu
+// pairs black, red, the last red dropped on the finest level).
+template <typename X, typename B, typename Get>
+void group_sweeps(GroupPlan& P, const std::vector<const CycleView<X>*>& vs, int pairs, bool up, bool finest,
+                  Get&& level_of)
+{
+    for (int q = 0; q < pairs; ++q)
+        for (int h = 0; h < 2; ++h) {
+            const int colour = up ? 1 - h : h;
+            if (up && finest && h == 1 && q == pairs - 1) continue;
+            GroupItems<typename MgSmooth<X, B>::Args> it;
+            for (const CycleView<X>* v : vs) {
+                const MgLevelT<X, B>& o = level_of(*v);
+                it.add({o, colour, v->c->st}, pair_grid(o));
+            }
+            P.add<MgSmooth<X, B>>(it.v);
+        }
+}
+
+template <typename X>
+void group_cycle(GroupPlan& P, const std::vector<CycleView<X>>& all)
+{
+    const FieldView<X> nobc{nullptr, 0, 0};
+    int lmax = 0;
+    for (const auto& v : all) lmax = std::max(lmax, v.last);
+    const int nu = all.front().nu;
+    // descent: level l of every case that has a coarser level below it, then
+    // the coarsest solve of every case whose coarsest level is l
+    for (int l = 0; l <= lmax; ++l) {
+        std::vector<const CycleView<X>*> dn, dnx, co;
+        for (const auto& v : all) {
+            if (l < v.last) (l == 0 && v.hx ? dnx : dn).push_back(&v);
+            if (l == v.last) co.push_back(&v);
+        }
+        if (l == 0) {
+            GroupItems<typename MgPre<X, double>::Args> pre;
+            for (const auto* v : dn) pre.add({v->top, nobc, v->c->st}, pair_grid(v->top));
+            for (const auto* v : dnx) pre.add({v->top, v->topx.b, v->c->st}, pair_grid(v->top));
+            P.add<MgPre<X, double>>(pre.v);
+            group_sweeps<X, double>(P, dn, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
+            group_sweeps<X, X>(P, dnx, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+            GroupItems<typename MgRr<X, double, X>::Args> rr;
+            for (const auto* v : dn) {
+                const RrShape r = rr_shape(v->lev[1]);
+                rr.add({v->top, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
+            }
+            P.add<MgRr<X, double, X>>(rr.v);
+            GroupItems<typename MgRr<X, X, X>::Args> rrx;
+            for (const auto* v : dnx) {
+                const RrShape r = rr_shape(v->lev[1]);
+                rrx.add({v->topx, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
+            }
+            P.add<MgRr<X, X, X>>(rrx.v);
+        } else if (!dn.empty()) {
+            GroupItems<typename MgPre<X, X>::Args> pre;
+            for (const auto* v : dn) pre.add({v->lev[l], nobc, v->c->st}, pair_grid(v->lev[l]));
+            P.add<MgPre<X, X>>(pre.v);
+            group_sweeps<X, X>(P, dn, nu - 1, false, false, [l](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.lev[l]; });
+            GroupItems<typename MgRr<X, X, X>::Args> rr;
+            for (const auto* v : dn) {
+                const RrShape r = rr_shape(v->lev[l + 1]);
+                rr.add({v->lev[l], v->lev[l + 1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
+            }
+            P.add<MgRr<X, X, X>>(rr.v);
+        }
+        if (!co.empty()) {
+            if (l == 0) {
+                // a one-level hierarchy: the coarsest solve on level 0 (FP64 only)
+                if constexpr (std::is_same_v<X, double>) {
+                    GroupItems<typename MgSpec<X, double>::Args> sp;
+                    GroupItems<typename MgCoarse<X>::Args> cs;
+                    for (const auto* v : co) {
+                        if (v->spec) sp.add({v->top, v->spec, v->c->st}, dim3(1));
+                        else cs.add({v->top, v->c->st}, dim3(1));
+                    }
+                    P.add<MgSpec<X, double>>(sp.v);
+                    P.add<MgCoarse<X>>(cs.v);
+                }
+            } else {
+                GroupItems<typename MgSpec<X, X>::Args> sp;
+                GroupItems<typename MgCoarse<X>::Args> cs;
+                for (const auto* v : co) {
+                    if (v->spec) sp.add({v->lev[l], v->spec, v->c->st}, dim3(1));
+                    else cs.add({v->lev[l], v->c->st}, dim3(1));
+                }
+                P.add<MgSpec<X, X>>(sp.v);
+                P.add<MgCoarse<X>>(cs.v);
+            }
+        }
+    }
+    // ascent
+    for (int l = lmax - 1; l >= 0; --l) {
+        std::vector<const CycleView<X>*> up, upx;
+        for (const auto& v : all)
+            if (l < v.last) (l == 0 && v.hx ? upx : up).push_back(&v);
+        if (l == 0) {
+            GroupItems<typename MgProlong<X, double, X>::Args> pr;
+            for (const auto* v : up)
+                pr.add({v->top, v->lev[1], v->c->st},
+                       dim3((v->lev[1].nx + BX - 1) / BX, (v->top.ny + BY - 1) / BY, v->top.nz));
+            P.add<MgProlong<X, double, X>>(pr.v);
+            GroupItems<typename MgProlong<X, X, X>::Args> prx;
+            for (const auto* v : upx)
+                prx.add({v->topx, v->lev[1], v->c->st},
+                        dim3((v->lev[1].nx + BX - 1) / BX, (v->topx.ny + BY - 1) / BY, v->topx.nz));
+            P.add<MgProlong<X, X, X>>(prx.v);
+            group_sweeps<X, double>(P, up, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
+            group_sweeps<X, X>(P, upx, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+        } else if (!up.empty()) {
+            GroupItems<typename MgProlong<X, X, X>::Args> pr;
+            for (const auto* v : up)
+                pr.add({v->lev[l], v->lev[l + 1], v->c->st},
+                       dim3((v->lev[l + 1].nx + BX - 1) / BX, (v->lev[l].ny + BY - 1) / BY, v->lev[l].nz));
+            P.add<MgProlong<X, X, X>>(pr.v);
+            group_sweeps<X, X>(P, up, nu, true, false, [l](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.lev[l]; });
+        }
+    }
+    // the final red sweep with z.r and sum z, and the scalar update
+    GroupItems<typename MgLast<X>::Args> la;
+    GroupItems<MgZf::Args> zf;
+    for (const auto& v : all) {
+        const Tiles tl = cg_tiles(v.c->g);
+        la.add({v.top, tl, v.last > 0 ? 1 : 0, v.c->part, v.c->st}, dim3(tl.nblocks));
+        zf.add({v.c->part, tl.nblocks, double(v.top.nx) * v.top.ny * v.top.nz, v.c->st}, dim3(1));
+    }
+    P.add<MgLast<X>>(la.v);
+    P.add<MgZf>(zf.v);
+}
+
+}  // namespace
+
+int mg_group_vcycle(GroupPlan& P, const std::vector<MgGroupCase>& cs)
+{
+    // cases of one precision and sweep count run in lock-step; classes in turn
+    for (int fp32 = 0; fp32 < 2; ++fp32)
+        for (int nu = 1; nu <= 16; ++nu) {
+            std::vector<const MgGroupCase*> sel;
+            for (const auto& c : cs)
+                if ((c.h->fp32 != 0) == (fp32 != 0) && c.h->nu == nu) sel.push_back(&c);
+            if (sel.empty()) continue;
+            if (fp32) {
+                std::vector<CycleView<float>> v;
+                for (const auto* c : sel) {
+                    if (c->h->nlev < 2) return -1;  // as launch_mg_vcycle: no one-level FP32 cycle
+                    v.push_back(cycle_view<float>(*c));
+                }
+                group_cycle<float>(P, v);
+            } else {
+                std::vector<CycleView<double>> v;
+                for (const auto* c : sel) v.push_back(cycle_view<double>(*c));
+                group_cycle<double>(P, v);
+            }
+        }
+    for (const auto& c : cs)
+        if (c.h->nu < 1 || c.h->nu > 16) return -1;
+    return 0;
 }
 
 // 1-D eigenbases.  Neumann (cell-centred, ends coupled to themselves): mode k

@@ -31,6 +31,7 @@
 // The streaming kernels walk tiles of 64 x 4 columns x 4 levels (one
 // wavefront spans a 512-byte row segment; the 4-level march reuses the z
 // neighbours in registers), grid-stride over at most CG_BLOCKS blocks.
+#include "group.hpp"
 #include "reduce.hpp"
 #include "tiles.hpp"
 #include "tpls_kernels.hpp"
@@ -115,14 +116,48 @@ long cg_num_blocks(const Grid& g) { return cg_tiles(g).nblocks; }
 // written into the r buffer; partials of sum b.  RHS_p is either read
 // (use_rhs) or formed on the fly as div(u3)/dt.
 template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, Tiles tl, double dx, double dy, double dz, double dt,
-                                                       FieldView<T> u3, FieldView<T> v3, FieldView<T> w3,
-                                                       const T* u_inlet, FieldView<double> rhs_p, int use_rhs,
-                                                       FieldView<double> b, double* part)
+struct OpCgSetupB {
+    struct Args {
+        Grid g;
+        Tiles tl;
+        double dx;
+        double dy;
+        double dz;
+        double dt;
+        FieldView<T> u3;
+        FieldView<T> v3;
+        FieldView<T> w3;
+        const T* u_inlet;
+        FieldView<double> rhs_p;
+        int use_rhs;
+        FieldView<double> b;
+        double* part;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+template <typename T>
+__device__ __forceinline__ void OpCgSetupB<T>::run(const Args& args_, const Blk& bk_)
 {
+    Grid g = args_.g;
+    Tiles tl = args_.tl;
+    double dx = args_.dx;
+    double dy = args_.dy;
+    double dz = args_.dz;
+    double dt = args_.dt;
+    FieldView<T> u3 = args_.u3;
+    FieldView<T> v3 = args_.v3;
+    FieldView<T> w3 = args_.w3;
+    const T* u_inlet = args_.u_inlet;
+    FieldView<double> rhs_p = args_.rhs_p;
+    int use_rhs = args_.use_rhs;
+    FieldView<double> b = args_.b;
+    double* part = args_.part;
     const int nx = g.maxl - 1, nz = g.maxn - 1;
     double acc[1] = {0.0};
-    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    for (int t = bk_.x; t < tl.ntiles; t += bk_.n) {
         int i, j, k0;
         tile_origin(tl, t, i, j, k0);
         if (i > nx || j > g.P) continue;
@@ -137,13 +172,40 @@ __global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, Tiles tl, double 
             acc[0] += val;
         }
     }
-    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+    block_partials<1>(acc, part, bk_.n, bk_.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_cg_setup_b(Grid g, Tiles tl, double dx, double dy, double dz, double dt,
+                                                       FieldView<T> u3, FieldView<T> v3, FieldView<T> w3,
+                                                       const T* u_inlet, FieldView<double> rhs_p, int use_rhs,
+                                                       FieldView<double> b, double* part)
+{
+    OpCgSetupB<T>::run({g, tl, dx, dy, dz, dt, u3, v3, w3, u_inlet, rhs_p, use_rhs, b, part}, blk_here());
 }
 
 // Reset the device state of a solve (a kernel rather than a host copy so that
 // the set-up can be captured in a graph).
-__global__ void k_cg_init(CgState* st, double rtol, double abstol, double dtol, int maxits)
+struct OpCgInit {
+    struct Args {
+        CgState* st;
+        double rtol;
+        double abstol;
+        double dtol;
+        int maxits;
+    };
+    static constexpr int THREADS = 64;
+    static dim3 block() { return dim3(64); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+__device__ __forceinline__ void OpCgInit::run(const Args& args_, const Blk& bk_)
 {
+    CgState* st = args_.st;
+    double rtol = args_.rtol;
+    double abstol = args_.abstol;
+    double dtol = args_.dtol;
+    int maxits = args_.maxits;
     if (threadIdx.x != 0) return;
     CgState s{};
     s.rtol = rtol;
@@ -153,20 +215,65 @@ __global__ void k_cg_init(CgState* st, double rtol, double abstol, double dtol, 
     *st = s;
 }
 
-__global__ void __launch_bounds__(256) k_cg_setup_bf(const double* part, int nb, double N, CgState* st)
+__global__ void k_cg_init(CgState* st, double rtol, double abstol, double dtol, int maxits)
 {
+    OpCgInit::run({st, rtol, abstol, dtol, maxits}, blk_here());
+}
+
+struct OpCgSetupBf {
+    struct Args {
+        const double* part;
+        int nb;
+        double N;
+        CgState* st;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+__device__ __forceinline__ void OpCgSetupBf::run(const Args& args_, const Blk& bk_)
+{
+    const double* part = args_.part;
+    int nb = args_.nb;
+    double N = args_.N;
+    CgState* st = args_.st;
     double tot[1];
     finish_sums<1>(part, nb, tot);
     if (tid256() == 0) st->mb = tot[0] / N;
 }
 
-// b <- b - mean(b); r = b - A x; partials of |b|^2, sum r, sum r^2.
-__global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Op o, Tiles tl, FieldView<double> x, FieldView<double> r,
-                                                       double* part, const CgState* st)
+__global__ void __launch_bounds__(256) k_cg_setup_bf(const double* part, int nb, double N, CgState* st)
 {
+    OpCgSetupBf::run({part, nb, N, st}, blk_here());
+}
+
+// b <- b - mean(b); r = b - A x; partials of |b|^2, sum r, sum r^2.
+struct OpCgSetupR {
+    struct Args {
+        Op o;
+        Tiles tl;
+        FieldView<double> x;
+        FieldView<double> r;
+        double* part;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+__device__ __forceinline__ void OpCgSetupR::run(const Args& args_, const Blk& bk_)
+{
+    Op o = args_.o;
+    Tiles tl = args_.tl;
+    FieldView<double> x = args_.x;
+    FieldView<double> r = args_.r;
+    double* part = args_.part;
+    const CgState* st = args_.st;
     const double mb = st->mb;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    for (int t = bk_.x; t < tl.ntiles; t += bk_.n) {
         int i, j, k0;
         tile_origin(tl, t, i, j, k0);
         if (i > o.nx || j > o.ny) continue;
@@ -183,11 +290,33 @@ __global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Op o, Tiles tl, FieldView
             acc[2] += rr * rr;
         }
     }
-    block_partials<3>(acc, part, gridDim.x, blockIdx.x);
+    block_partials<3>(acc, part, bk_.n, bk_.x);
 }
 
-__global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb, double N, CgState* st)
+__global__ void __launch_bounds__(BX* BY) k_cg_setup_r(Op o, Tiles tl, FieldView<double> x, FieldView<double> r,
+                                                       double* part, const CgState* st)
 {
+    OpCgSetupR::run({o, tl, x, r, part, st}, blk_here());
+}
+
+struct OpCgSetupRf {
+    struct Args {
+        const double* part;
+        int nb;
+        double N;
+        CgState* st;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+__device__ __forceinline__ void OpCgSetupRf::run(const Args& args_, const Blk& bk_)
+{
+    const double* part = args_.part;
+    int nb = args_.nb;
+    double N = args_.N;
+    CgState* st = args_.st;
     double tot[3];
     finish_sums<3>(part, nb, tot);
     if (tid256() != 0) return;
@@ -219,6 +348,11 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
     }
 }
 
+__global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb, double N, CgState* st)
+{
+    OpCgSetupRf::run({part, nb, N, st}, blk_here());
+}
+
 // p <- z + (beta/beta_old) p  (p <- z on the first iteration), w = A p, p.w.
 // Each plane of the new p over the block's 64 x 4 columns is formed once and
 // shared through LDS; the x/y neighbours outside the block are formed by the
@@ -226,7 +360,7 @@ __global__ void __launch_bounds__(256) k_cg_setup_rf(const double* part, int nb,
 template <bool FIRST, typename Z>
 __device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const FieldView<Z>& r,
                                            const FieldView<double>& po, const FieldView<double>& pn, double m,
-                                           double bt, double& acc)
+                                           double bt, double& acc, const Blk& bk)
 {
     __shared__ double sp[BY][BX];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -234,7 +368,7 @@ __device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const F
         double z = double(r(ii, jj, kk)) - m;
         return FIRST ? z : z + bt * po(ii, jj, kk);
     };
-    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    for (int t = bk.x; t < tl.ntiles; t += bk.n) {
         int i, j, k0;
         tile_origin(tl, t, i, j, k0);
         const bool act = i <= o.nx && j <= o.ny;
@@ -269,21 +403,60 @@ __device__ __forceinline__ void cg_p_tiles(const Op& o, const Tiles& tl, const F
 
 // r: the preconditioned residual z (FP64, or FP32 from the FP32 V-cycle), or r
 template <typename Z>
+struct CgP {
+    struct Args {
+        Op o;
+        Tiles tl;
+        FieldView<Z> r;
+        FieldView<double> po, pn;
+        int use_mz;
+        double* part;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk)
+    {
+        const CgState* st = a.st;
+        if (st->done) return;
+        const double m = a.use_mz ? st->mz : st->m;
+        double acc[1] = {0.0};
+        if (st->i == 0)
+            cg_p_tiles<true>(a.o, a.tl, a.r, a.po, a.pn, m, 0.0, acc[0], bk);
+        else
+            cg_p_tiles<false>(a.o, a.tl, a.r, a.po, a.pn, m, st->beta / st->betaold, acc[0], bk);
+        block_partials<1>(acc, a.part, bk.n, bk.x);
+    }
+};
+
+template <typename Z>
 __global__ void __launch_bounds__(BX* BY) k_cg_p(Op o, Tiles tl, FieldView<Z> r, FieldView<double> po,
                                                  FieldView<double> pn, int use_mz, double* part, const CgState* st)
 {
-    if (st->done) return;
-    const double m = use_mz ? st->mz : st->m;
-    double acc[1] = {0.0};
-    if (st->i == 0)
-        cg_p_tiles<true>(o, tl, r, po, pn, m, 0.0, acc[0]);
-    else
-        cg_p_tiles<false>(o, tl, r, po, pn, m, st->beta / st->betaold, acc[0]);
-    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+    CgP<Z>::run({o, tl, r, po, pn, use_mz, part, st}, blk_here());
 }
+
+struct CgPf {
+    struct Args {
+        const double* part;
+        int nb;
+        CgState* st;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
 
 __global__ void __launch_bounds__(256) k_cg_pf(const double* part, int nb, CgState* st)
 {
+    CgPf::run({part, nb, st}, blk_here());
+}
+
+__device__ __forceinline__ void CgPf::run(const Args& a, const Blk&)
+{
+    const double* part = a.part;
+    const int nb = a.nb;
+    CgState* st = a.st;
     if (st->done) return;
     double tot[1];
     finish_sums<1>(part, nb, tot);
@@ -300,13 +473,37 @@ __global__ void __launch_bounds__(256) k_cg_pf(const double* part, int nb, CgSta
 }
 
 // x <- x + alpha p, r <- r - alpha A p, partials of sum r, sum r^2.
+struct CgX {
+    struct Args {
+        Op o;
+        Tiles tl;
+        FieldView<double> p, x, r;
+        double* part;
+        const CgState* st;
+    };
+    static constexpr int THREADS = BX * BY;
+    static dim3 block() { return dim3(BX, BY); }
+    static __device__ __forceinline__ void run(const Args& args, const Blk& bk);
+};
+
 __global__ void __launch_bounds__(BX* BY) k_cg_x(Op o, Tiles tl, FieldView<double> p, FieldView<double> x,
                                                  FieldView<double> r, double* part, const CgState* st)
 {
+    CgX::run({o, tl, p, x, r, part, st}, blk_here());
+}
+
+__device__ __forceinline__ void CgX::run(const Args& args, const Blk& bk)
+{
+    const Op& o = args.o;
+    const Tiles& tl = args.tl;
+    const FieldView<double>& p = args.p;
+    const FieldView<double>& x = args.x;
+    const FieldView<double>& r = args.r;
+    const CgState* st = args.st;
     if (st->done) return;
     const double a = st->alpha;
     double acc[2] = {0.0, 0.0};
-    for (int t = blockIdx.x; t < tl.ntiles; t += gridDim.x) {
+    for (int t = bk.x; t < tl.ntiles; t += bk.n) {
         int i, j, k0;
         tile_origin(tl, t, i, j, k0);
         if (i > o.nx || j > o.ny) continue;
@@ -329,11 +526,32 @@ __global__ void __launch_bounds__(BX* BY) k_cg_x(Op o, Tiles tl, FieldView<doubl
             pc = pp;
         }
     }
-    block_partials<2>(acc, part, gridDim.x, blockIdx.x);
+    block_partials<2>(acc, args.part, bk.n, bk.x);
 }
+
+struct CgXf {
+    struct Args {
+        const double* part;
+        int nb;
+        double N;
+        CgState* st;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&);
+};
 
 __global__ void __launch_bounds__(256) k_cg_xf(const double* part, int nb, double N, CgState* st)
 {
+    CgXf::run({part, nb, N, st}, blk_here());
+}
+
+__device__ __forceinline__ void CgXf::run(const Args& a, const Blk&)
+{
+    const double* part = a.part;
+    const int nb = a.nb;
+    const double N = a.N;
+    CgState* st = a.st;
     if (st->done) return;
     double tot[2];
     finish_sums<2>(part, nb, tot);
@@ -394,12 +612,31 @@ __global__ void __launch_bounds__(BX* BY) k_pres_out(Grid g, double dx, double d
 // Sum of squared differences over i in 1..maxl-1, j in 1..P, k in 0..nz-1
 // (levelset.f90:218-228 and get_difference, jacobi_iteration_allflux.f90:297-308).
 template <typename T>
-__global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, FieldView<T> b, double* part)
+struct OpSqdiff {
+    struct Args {
+        Grid g;
+        int nz;
+        FieldView<T> a;
+        FieldView<T> b;
+        double* part;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+template <typename T>
+__device__ __forceinline__ void OpSqdiff<T>::run(const Args& args_, const Blk& bk_)
 {
+    Grid g = args_.g;
+    int nz = args_.nz;
+    FieldView<T> a = args_.a;
+    FieldView<T> b = args_.b;
+    double* part = args_.part;
     const int nx = g.maxl - 1;
     const long n = (long)nx * g.P * nz;
     double acc[1] = {0.0};
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+    for (long q = bk_.x * 256L + threadIdx.x; q < n; q += (long)bk_.n * 256) {
         int i = int(q % nx) + 1;
         long rr = q / nx;
         int j = int(rr % g.P) + 1;
@@ -407,14 +644,39 @@ __global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, 
         double d = double(a(i, j, k)) - double(b(i, j, k));
         acc[0] += d * d;
     }
-    block_partials<1>(acc, part, gridDim.x, blockIdx.x);
+    block_partials<1>(acc, part, bk_.n, bk_.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sqdiff(Grid g, int nz, FieldView<T> a, FieldView<T> b, double* part)
+{
+    OpSqdiff<T>::run({g, nz, a, b, part}, blk_here());
+}
+
+struct OpFinal1 {
+    struct Args {
+        const double* part;
+        int nb;
+        double* out;
+    };
+    static constexpr int THREADS = 256;
+    static dim3 block() { return dim3(256); }
+    static __device__ __forceinline__ void run(const Args& args_, const Blk& bk_);
+};
+
+__device__ __forceinline__ void OpFinal1::run(const Args& args_, const Blk& bk_)
+{
+    const double* part = args_.part;
+    int nb = args_.nb;
+    double* out = args_.out;
+    double tot[1];
+    finish_sums<1>(part, nb, tot);
+    if (tid256() == 0) out[0] = tot[0];
 }
 
 __global__ void __launch_bounds__(256) k_final1(const double* part, int nb, double* out)
 {
-    double tot[1];
-    finish_sums<1>(part, nb, tot);
-    if (tid256() == 0) out[0] = tot[0];
+    OpFinal1::run({part, nb, out}, blk_here());
 }
 
 // Liquid volume V = sum over interior cells of H_eps(phi) dx dy dz in FP64,
@@ -517,6 +779,98 @@ hipError_t launch_cg_x(hipStream_t s, const Grid& g, const Phys& ph, FieldView<d
     hipLaunchKernelGGL(k_cg_x, dim3(tl.nblocks), dim3(BX, BY), 0, s, make_op(g, ph), tl, p, x, r, red.part, st);
     hipLaunchKernelGGL(k_cg_xf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, n_unknowns(g), st);
     return hipGetLastError();
+}
+
+// ---- the CG iteration for many cases in lock-step (group.hpp) ----
+struct CgGather {
+    struct Args {
+        const CgState* src;
+        CgState* dst;
+    };
+    static constexpr int THREADS = 64;
+    static dim3 block() { return dim3(64); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk&)
+    {
+        if (threadIdx.x == 0) *a.dst = *a.src;
+    }
+};
+
+int cg_group_iteration(GroupPlan& P, const std::vector<CgGroupCase>& cs, int cur)
+{
+    GroupItems<CgP<double>::Args> pd;
+    GroupItems<CgP<float>::Args> pf32;
+    GroupItems<CgPf::Args> pfin;
+    GroupItems<CgX::Args> xs;
+    GroupItems<CgXf::Args> xfin;
+    for (const CgGroupCase& c : cs) {
+        const Tiles tl = cg_tiles(c.g);
+        const Op o = make_op(c.g, c.ph);
+        const FieldView<double> po = c.pb[cur], pn = c.pb[cur ^ 1];
+        if (c.z32) pf32.add({o, tl, c.zf, po, pn, c.use_mz, c.part, c.st}, dim3(tl.nblocks));
+        else pd.add({o, tl, c.z, po, pn, c.use_mz, c.part, c.st}, dim3(tl.nblocks));
+        pfin.add({c.part, tl.nblocks, c.st}, dim3(1));
+        xs.add({o, tl, pn, c.x, c.r, c.part, c.st}, dim3(tl.nblocks));
+        xfin.add({c.part, tl.nblocks, n_unknowns(c.g), c.st}, dim3(1));
+    }
+    P.add<CgP<double>>(pd.v);
+    P.add<CgP<float>>(pf32.v);
+    P.add<CgPf>(pfin.v);
+    P.add<CgX>(xs.v);
+    P.add<CgXf>(xfin.v);
+    return 0;
+}
+
+template <typename T>
+void group_difference(GroupPlan& P, const std::vector<const HhGroupCase<T>*>& on, const std::vector<FieldView<T>>& a,
+                      const std::vector<FieldView<T>>& b, int q)
+{
+    GroupItems<typename OpSqdiff<T>::Args> sq;
+    GroupItems<OpFinal1::Args> fin;
+    for (size_t k = 0; k < on.size(); ++k) {
+        const Grid& g = on[k]->g;
+        sq.add({g, q == 2 ? g.maxn : g.maxn - 1, a[k], b[k], on[k]->part}, dim3(REDUCE_BLOCKS));
+        fin.add({on[k]->part, REDUCE_BLOCKS, on[k]->out + q}, dim3(1));
+    }
+    P.add<OpSqdiff<T>>(sq.v);
+    P.add<OpFinal1>(fin.v);
+}
+template void group_difference<double>(GroupPlan&, const std::vector<const HhGroupCase<double>*>&,
+                                       const std::vector<FieldView<double>>&, const std::vector<FieldView<double>>&,
+                                       int);
+template void group_difference<float>(GroupPlan&, const std::vector<const HhGroupCase<float>*>&,
+                                      const std::vector<FieldView<float>>&, const std::vector<FieldView<float>>&, int);
+
+template <typename T>
+void cg_group_setup(GroupPlan& P, const std::vector<SetupGroupCase<T>>& cs)
+{
+    GroupItems<OpCgInit::Args> init;
+    GroupItems<typename OpCgSetupB<T>::Args> sb;
+    GroupItems<OpCgSetupBf::Args> sbf;
+    GroupItems<OpCgSetupR::Args> sr;
+    GroupItems<OpCgSetupRf::Args> srf;
+    for (const SetupGroupCase<T>& c : cs) {
+        const Tiles tl = cg_tiles(c.g);
+        init.add({c.st, c.rtol, c.abstol, c.dtol, c.maxits}, dim3(1));
+        sb.add({c.g, tl, c.ph.dx, c.ph.dy, c.ph.dz, c.ph.dt, c.u3, c.v3, c.w3, c.inlet, c.rhs_p, 0, c.r, c.part},
+               dim3(tl.nblocks));
+        sbf.add({c.part, tl.nblocks, n_unknowns(c.g), c.st}, dim3(1));
+        sr.add({make_op(c.g, c.ph), tl, c.x, c.r, c.part, c.st}, dim3(tl.nblocks));
+        srf.add({c.part, tl.nblocks, n_unknowns(c.g), c.st}, dim3(1));
+    }
+    P.add<OpCgInit>(init.v);
+    P.add<OpCgSetupB<T>>(sb.v);
+    P.add<OpCgSetupBf>(sbf.v);
+    P.add<OpCgSetupR>(sr.v);
+    P.add<OpCgSetupRf>(srf.v);
+}
+template void cg_group_setup<double>(GroupPlan&, const std::vector<SetupGroupCase<double>>&);
+template void cg_group_setup<float>(GroupPlan&, const std::vector<SetupGroupCase<float>>&);
+
+void cg_group_gather(GroupPlan& P, const std::vector<const CgState*>& src, CgState* dst)
+{
+    GroupItems<CgGather::Args> g;
+    for (size_t q = 0; q < src.size(); ++q) g.add({src[q], dst + q}, dim3(1));
+    P.add<CgGather>(g.v);
 }
 
 template <typename T>
