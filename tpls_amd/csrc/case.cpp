@@ -1402,7 +1402,7 @@ struct SolveGroup {
     GroupPlan hh;                     // this step's Helmholtz loops and solve set-up
     CgState* d_st = nullptr;          // gathered states, read back in one copy
     CgState* h_st = nullptr;
-    int n = 0;
+    int n = 0, cap = 0;
     bool any_mg = false;
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr, done = nullptr;
@@ -1445,16 +1445,33 @@ int group_for(const std::vector<tpls_case*>& cs, SolveGroup** out)
         *out = t_group.get();
         return TPLS_OK;
     }
-    t_group.reset();
-    auto G = std::make_unique<SolveGroup>();
-    G->key = key;
-    G->coef = coef;
+    // a new set of cases (serving re-forms its batch every step): the
+    // stream, events and buffers stay, the tables are rebuilt
+    if (!t_group) {
+        auto G = std::make_unique<SolveGroup>();
+        HIPCHK(hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&G->ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&G->done, hipEventDisableTiming));
+        t_group = std::move(G);
+    }
+    SolveGroup* G = t_group.get();
+    // the tables are rewritten in place: nothing of a previous call (one that
+    // failed part-way included) may still be reading them
+    HIPCHK(hipStreamSynchronize(G->s));
+    G->key.clear();  // invalid until the rebuild completes
+    if (G->cap < int(cs.size())) {
+        const int cap = std::max<int>(int(cs.size()), 2 * G->cap);
+        if (G->d_st) (void)hipFree(G->d_st);
+        if (G->h_st) (void)hipHostFree(G->h_st);
+        G->d_st = nullptr;
+        G->h_st = nullptr;
+        G->cap = 0;
+        if (hipMalloc(&G->d_st, sizeof(CgState) * cap) != hipSuccess) return fail(TPLS_ERR_NOMEM, "group states");
+        if (hipHostMalloc(&G->h_st, sizeof(CgState) * cap) != hipSuccess) return fail(TPLS_ERR_NOMEM, "pinned");
+        G->cap = cap;
+    }
     G->n = int(cs.size());
-    HIPCHK(hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&G->ev, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&G->done, hipEventDisableTiming));
-    if (hipMalloc(&G->d_st, sizeof(CgState) * cs.size()) != hipSuccess) return fail(TPLS_ERR_NOMEM, "group states");
-    if (hipHostMalloc(&G->h_st, sizeof(CgState) * cs.size()) != hipSuccess) return fail(TPLS_ERR_NOMEM, "pinned");
+    for (GroupPlan* P : {&G->vc, &G->it[0], &G->it[1], &G->gather}) P->clear();
     std::vector<MgGroupCase> mg;
     std::vector<CgGroupCase> cg;
     std::vector<const CgState*> src;
@@ -1485,8 +1502,9 @@ int group_for(const std::vector<tpls_case*>& cs, SolveGroup** out)
     }
     cg_group_gather(G->gather, src, G->d_st);
     for (GroupPlan* P : {&G->vc, &G->it[0], &G->it[1], &G->gather}) HIPCHK(P->upload());
-    t_group = std::move(G);
-    *out = t_group.get();
+    G->key = key;
+    G->coef = coef;
+    *out = G;
     return TPLS_OK;
 }
 
@@ -1552,6 +1570,7 @@ int group_solve(const std::vector<tpls_case*>& cs, int mode)
         RC(hh_group_loops<float>(G->hh, h32, diag) ? fail(TPLS_ERR_HIP, "grouped Helmholtz loops") : TPLS_OK);
         cg_group_setup<double>(G->hh, s64);
         cg_group_setup<float>(G->hh, s32);
+        LCHK(hipStreamSynchronize(G->s));  // the previous step's tables are rewritten
         LCHK(G->hh.upload());
     }
     for (tpls_case* c : cs) {
@@ -2321,7 +2340,7 @@ int tpls_batch_step_status(tpls_case** cases, int ncases, int nsteps, int* statu
             fail(TPLS_ERR_ARG, "the case failed in an earlier step: " + cases[q]->broken_msg);
             drop(q, TPLS_ERR_ARG);
         }
-    int grouped = g_batch_grouped;
+    int grouped = ncases > 1 ? g_batch_grouped : 0;  // one case gains nothing from grouping
     for (int q = 0; q < ncases; ++q)
         if (cases[q]->timed) grouped = 0;  // kernel timing brackets per-case launches
     for (int n = 0; n < nsteps; ++n) {
