@@ -80,7 +80,7 @@ typedef struct {
     int tile_rows_rb;     /* rows per tile of that pass, even parity: 2/4/8 */
     int tile_rows_rb_odd; /* the same, odd parity (0 = default for both) */
     int tile_rows_jst;    /* Jacobi sweep with its planes staged through LDS,
-                             tiles of 2/4/8 rows (0: default, 4); -1: the
+                             tiles of 2/4/8/16 rows (0: default, 4); -1: the
                              unstaged sweep with tile_rows instead (same
                              results either way) */
     int mg_fp32;          /* precision of the multigrid V-cycle (ksp_pc >= 1):

@@ -42,7 +42,7 @@ def test_every_launch_shape_gives_the_same_fields(precision):
     ref = run(p, init, precision, None, True)
     shapes = [((r, 4, 4, 4, 8, -1), True) for r in ROWS] + [((4, ro, d), True) for ro, d in ODD]
     shapes += [((r, 4, 4, rb, rbo), False) for r in (1, 4) for rb in (2, 4, 8) for rbo in (2, 4, 8)]
-    shapes += [((4, 2, 8, 4, 8, jst), split) for jst in (2, 4, 8) for split in (True, False, None)]
+    shapes += [((4, 2, 8, 4, 8, jst), split) for jst in (2, 4, 8, 16) for split in (True, False, None)]
     for t, split in shapes:
         got = run(p, init, precision, t, split)
         for f in FIELDS:
@@ -61,9 +61,10 @@ def test_fused_sor_pass_matches_the_two_sweeps(grid, precision):
         got = run(p, init, precision, (4, 2, 8, rb, rb), False)
         for f in FIELDS:
             assert np.array_equal(got[f], ref[f]), (grid, rb, f)
-        got = run(p, init, precision, (4, 2, 8, 4, 8, rb), True)
+    for jst in (2, 4, 8, 16):
+        got = run(p, init, precision, (4, 2, 8, 4, 8, jst), True)
         for f in FIELDS:
-            assert np.array_equal(got[f], ref[f]), (grid, "jst", rb, f)
+            assert np.array_equal(got[f], ref[f]), (grid, "jst", jst, f)
     print(f"[sor pass] {grid} {precision}: fused == split for tile rows 2/4/8", flush=True)
 
 
@@ -77,18 +78,19 @@ def test_march_chunk_length_changes_no_value(precision):
     """The staged Jacobi and even-parity SOR sweeps march each block up a
     chunk of levels, prefetching 8 levels ahead into registers and one plane
     ahead into LDS.  Chunks shorter than, equal to and longer than the
-    prefetch depth (1, 8, 9, 17 levels), and one chunk of the whole column,
-    give the fields of the unstaged sweeps bit for bit."""
+    prefetch depth (1, 8, 9, 17 levels; the 16-row Jacobi tiles prefetch 4
+    levels: 4 and 5), and one chunk of the whole column, give the fields of
+    the unstaged sweeps bit for bit."""
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
     p = TALL
     init = make_initial(p)
     ref = run(p, init, precision, (4, 4, 4, 8, 8, -1), True)
-    for ks in (1, 8, 9, 17, p.nz):
-        for jst in (4, 8):
+    for ks in (1, 4, 5, 8, 9, 17, p.nz):
+        for jst in (4, 8, 16):
             with Case(p, precision=precision, tiles=(4, 4, 4, 8, 8, jst), sor_fused=False, march_levels=ks) as c:
                 c.load(*init)
                 c.step(3)
                 for f in FIELDS:
                     assert np.array_equal(c.field(f), ref[f]), (ks, jst, f)
-    print(f"[march chunk] {precision}: chunks of 1/8/9/17/{p.nz} levels identical", flush=True)
+    print(f"[march chunk] {precision}: chunks of 1/4/5/8/9/17/{p.nz} levels identical", flush=True)

@@ -27,7 +27,7 @@ from dataclasses import replace
 ROWS = (1, 2, 4, 8)
 ODD = ((1, 4), (2, 4), (4, 4), (1, 8), (2, 8), (4, 8))
 RB = (2, 4, 8)
-JST = (2, 4, 8)
+JST = (2, 4, 8, 16)
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 _cache: dict = {}
 

@@ -502,9 +502,10 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
     if (prm->sor_fused < -1 || prm->sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
     if (prm->mg_fp32 < -1 || prm->mg_fp32 > 1) return fail(TPLS_ERR_ARG, "mg_fp32 must be -1, 0 or 1");
-    if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) || (prm->tile_rows_jst != -1 && !rb_ok(prm->tile_rows_jst)))
+    if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) ||
+        (prm->tile_rows_jst != -1 && prm->tile_rows_jst != 16 && !rb_ok(prm->tile_rows_jst)))
         return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
-                                  "tile_rows_jst -1 (unstaged), 0 (default), 2, 4 or 8");
+                                  "tile_rows_jst -1 (unstaged), 0 (default), 2, 4, 8 or 16");
     if (prm->march_levels < 0) return fail(TPLS_ERR_ARG, "march_levels must be 0 (sized to the device) or >= 1");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
@@ -873,9 +874,9 @@ int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd,
     auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
     auto rb_ok = [](int r) { return r == 2 || r == 4 || r == 8; };
     if (!rows_ok(rows) || !rows_ok(rows_odd) || (prefetch_odd != 4 && prefetch_odd != 8) || !rb_ok(rows_rb) ||
-        !rb_ok(rows_rb_odd) || (rows_jst != -1 && !rb_ok(rows_jst)))
+        !rb_ok(rows_rb_odd) || (rows_jst != -1 && rows_jst != 16 && !rb_ok(rows_jst)))
         return fail(TPLS_ERR_ARG, "tile rows must be 1, 2, 4 or 8, prefetch_odd 4 or 8, the fused pass's rows 2, 4 "
-                                  "or 8 and the staged Jacobi's 2, 4 or 8 (-1: unstaged)");
+                                  "or 8 and the staged Jacobi's 2, 4, 8 or 16 (-1: unstaged)");
     HIPCHK(hipStreamSynchronize(c->s));
     c->tiling.rows = rows;
     c->tiling.rows_odd = rows_odd;

@@ -1324,19 +1324,22 @@ hipError_t jacobi_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<
     if (tl.jst) {
         // staged sweep: tile rows that divide the period, chunks of levels that fill
         // the last round of resident blocks
-        const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
+        const int ty = (tl.jst == 4 || tl.jst == 8 || tl.jst == 16) && g.P % tl.jst == 0 ? tl.jst : 2;
         auto go_st = [&](auto comp_c, auto ty_c, auto zero_c) {
             constexpr int CP = decltype(comp_c)::value, TY = decltype(ty_c)::value;
             constexpr bool ZERO = decltype(zero_c)::value;
+            // 16-row tiles (16 waves) load 4 levels ahead, within 128 registers
+            constexpr int D = TY == 16 ? 4 : 8;
             const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
             const int nbx = (imax + BX - 1) / BX, nby = g.P / TY;
-            static const int slots = resident_blocks(k_jacobi_st<CP, TY, 8, ZERO, T>, BX * TY);
+            static const int slots = resident_blocks(k_jacobi_st<CP, TY, D, ZERO, T>, BX * TY);
             const int ks = tl.ks > 0 ? std::min(tl.ks, nz) : march_chunk(nbx * nby, nz, e.slots(slots), 32);
             const int nbz = (nz + ks - 1) / ks;
-            return e.template go<OpJacobiSt<CP, TY, 8, ZERO, T>>({g, nbx, nby, ks, a2, b2, c2, xold, rhs, mu, x},
+            return e.template go<OpJacobiSt<CP, TY, D, ZERO, T>>({g, nbx, nby, ks, a2, b2, c2, xold, rhs, mu, x},
                                                                dim3(nbx * nby * nbz));
         };
         auto by_ty = [&](auto comp_c, auto zero_c) {
+            if (ty == 16) return go_st(comp_c, std::integral_constant<int, 16>{}, zero_c);
             if (ty == 8) return go_st(comp_c, std::integral_constant<int, 8>{}, zero_c);
             if (ty == 4) return go_st(comp_c, std::integral_constant<int, 4>{}, zero_c);
             return go_st(comp_c, std::integral_constant<int, 2>{}, zero_c);
@@ -1383,7 +1386,7 @@ hipError_t sor_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T> 
         if (iteration_time % 2 == 0 && tl.jst) {
             // staged planes, as the Jacobi sweep: rows that divide the period, chunks of
             // levels sized to the resident blocks
-            const int ty = (tl.jst == 4 || tl.jst == 8) && g.P % tl.jst == 0 ? tl.jst : 2;
+            const int ty = (tl.jst == 4 || tl.jst == 8 || tl.jst == 16) && g.P % tl.jst == 0 ? tl.jst : 2;
             const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
             const int nbx3 = (imax + BX - 1) / BX;
             auto go_st = [&](auto ty_c) {
@@ -1395,6 +1398,7 @@ hipError_t sor_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T> 
                 return e.template go<OpSor3dSt<CP, TY, 8, T>>({g, nbx3, nby, ks, flg, a2, b2, c2, x, rhs, mu},
                                                              dim3(nbx3 * nby * nbz));
             };
+            if (ty == 16) return go_st(std::integral_constant<int, 16>{});
             if (ty == 8) return go_st(std::integral_constant<int, 8>{});
             if (ty == 4) return go_st(std::integral_constant<int, 4>{});
             return go_st(std::integral_constant<int, 2>{});
