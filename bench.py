@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--second-roofline", default="advect",
                     help="a further kernel class timed the same way and reported under rooflines ('' to skip)")
     ap.add_argument("--roofline-steps", type=int, default=2, help="steps over which the kernel is timed")
+    ap.add_argument("--mg32-compare", type=int, default=1,
+                    help="also time the FP64 case with its V-cycle in FP32 (CG, residual and stopping test "
+                         "stay FP64) and report its difference from the all-FP64 run")
     ap.add_argument("--fp32-compare", type=int, default=1,
                     help="also run the case with FP32 storage over the same steps and report its "
                          "throughput and error against this run (0 to skip)")
@@ -278,23 +281,30 @@ def sor_split(a, kernel: str) -> bool:
     return code < 0 or not code & bit
 
 
-def fp32_run(p, a, ref, ref_volume):
-    """The same case with FP32 storage over the same warm-up and timed steps:
-    throughput and the error of the final state against the FP64 run (max and
-    L2 relative; pressure modulo its mean; relative liquid-volume difference)."""
-    import numpy as np
+def timed_variant(p, a, precision: str, **opts):
+    """The case with other options over the headline run's warm-up and timed
+    steps; returns the open case, the seconds of the timed steps and the CG
+    iterations of each."""
     from tpls_amd.case import Case
     from tpls_amd.initial import make_initial
-    case = Case(p, precision="fp32", **case_opts(a))
+    case = Case(p, precision=precision, **case_opts(a), **opts)
     case.load(*make_initial(p))
     for _ in range(a.warmup):
         case.step(1)
     case.sync()
+    its = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         case.step(1)
         case.sync()
-    el = time.perf_counter() - t0
+        its.append(case.diag()["ksp_its"])
+    return case, time.perf_counter() - t0, its
+
+
+def field_error(p, case, ref) -> dict:
+    """Max and L2 relative difference of the case's fields from ref over the
+    interior rows (pressure modulo its mean)."""
+    import numpy as np
     P = p.maxm - 1
     err = {}
     for f in COMPARED:
@@ -305,12 +315,37 @@ def fp32_run(p, a, ref, ref_volume):
         d = g - r
         err[f] = {"max_rel": float(np.abs(d).max() / np.abs(r).max()),
                   "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(r))}
+    return err
+
+
+def fp32_run(p, a, ref, ref_volume):
+    """The same case with FP32 storage over the same warm-up and timed steps:
+    throughput and the error of the final state against the FP64 run (max and
+    L2 relative; pressure modulo its mean; relative liquid-volume difference)."""
+    case, el, _ = timed_variant(p, a, "fp32")
+    err = field_error(p, case, ref)
     vol = case.volume()
     case.close()
     return {"value": round(p.cells * a.steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
             "ms_per_step": round(el / a.steps * 1e3, 3), "steps": a.steps, "warmup": a.warmup,
             "error_vs_fp64_after_steps": a.warmup + a.steps, "error": err,
             "volume_rel_diff": abs(vol - ref_volume) / ref_volume}
+
+
+def mg32_run(p, a, ref):
+    """The FP64 case with its multigrid V-cycle in FP32 (tpls_params.mg_fp32):
+    storage, the step's arithmetic, the CG vectors, inner products and stopping
+    test stay FP64, so the solve meets the same tolerance.  Throughput and the
+    difference of the final state from the all-FP64 run."""
+    import numpy as np
+    case, el, its = timed_variant(p, a, "fp64", mg_fp32=True)
+    err = field_error(p, case, ref)
+    case.close()
+    return {"value": round(p.cells * a.steps / el * 1e-6, 4), "unit": "Mcell-steps/s",
+            "ms_per_step": round(el / a.steps * 1e3, 3), "steps": a.steps, "warmup": a.warmup,
+            "cg_its_mean": round(float(np.mean(its)), 1),
+            "max_rel_diff_vs_fp64_mg": max(e["max_rel"] for e in err.values()),
+            "diff_after_steps": a.warmup + a.steps}
 
 
 def batched_run(a):
@@ -415,7 +450,8 @@ def main():
     case.sync()
     elapsed = time.perf_counter() - t0
     d = case.diag()
-    keep = a.fp32_compare and a.precision == "fp64"
+    mg32 = bool(a.mg32_compare) and a.precision == "fp64" and a.pc == "mg"
+    keep = (a.fp32_compare or mg32) and a.precision == "fp64"
     ref = {f: case.field(f) for f in COMPARED} if keep else None
     ref_volume = case.volume() if keep else None
     # the dominant kernel (and a second class), timed with HIP events on the
@@ -478,8 +514,10 @@ def main():
         out["plain_cg"] = plain
     if a.dim_steps > 0:
         out[f"dim_{a.precision}"] = dim_run(p, a)
-    if ref is not None:
+    if ref is not None and a.fp32_compare:
         out["fp32"] = fp32_run(p, a, ref, ref_volume)
+    if ref is not None and mg32:
+        out["fp64_mg_fp32"] = mg32_run(p, a, ref)
     if a.batch_cases > 0:
         out["batched_fp32"] = batched_run(a)
     if a.serve_seconds > 0:
@@ -496,6 +534,8 @@ def main():
             ratios["batched_fp32_vs_plain_fp64_cg"] = round(bv / plain["value"], 2)
     if "fp32" in out:
         ratios["fp32_vs_fp64_mg"] = round(out["fp32"]["value"] / out["value"], 3)
+    if "fp64_mg_fp32" in out:
+        ratios["fp64_mg_fp32_vs_fp64_mg"] = round(out["fp64_mg_fp32"]["value"] / out["value"], 3)
     if ratios:
         out["ratios"] = ratios
     if not a.no_cpu_baseline:

@@ -76,7 +76,12 @@ typedef struct {
                            reference's two in-place sweeps, with identical
                            results (every cell sees the same inputs):
                            -1 none, 1 odd iteration_time (the column
-                           sweeps), 2 even, 3 all; 0: default (1) */
+                           sweeps), 2 even, 3 all; 0: default (1).  Bit 2
+                           (4, 5): where the even iterations run split
+                           with staged tiles and the y period is a
+                           multiple of 8, their Jacobi step, its boundary
+                           conditions and the first colour sweep run as
+                           one pass (again identical results) */
     int tile_rows_rb;     /* rows per tile of that pass, even parity: 2/4/8 */
     int tile_rows_rb_odd; /* the same, odd parity (0 = default for both) */
     int tile_rows_jst;    /* Jacobi sweep with its planes staged through LDS,

@@ -25,10 +25,11 @@ GRIDS = {
 
 
 def run(p, init, precision, tiles, split):
-    """split: True (two sweeps everywhere), False (fused everywhere) or None
-    (the default schedule)."""
+    """split: True (two sweeps everywhere), False (fused everywhere), None
+    (the default schedule) or a tpls_params.sor_fused code."""
     from tpls_amd.case import Case
-    with Case(p, precision=precision, tiles=tiles, sor_fused=None if split is None else not split) as c:
+    fused = split if isinstance(split, int) and not isinstance(split, bool) else (None if split is None else not split)
+    with Case(p, precision=precision, tiles=tiles, sor_fused=fused) as c:
         c.load(*init)
         c.step(3)
         return {f: c.field(f) for f in FIELDS}
@@ -94,3 +95,43 @@ def test_march_chunk_length_changes_no_value(precision):
                 for f in FIELDS:
                     assert np.array_equal(c.field(f), ref[f]), (ks, jst, f)
     print(f"[march chunk] {precision}: chunks of 1/4/5/8/9/17/{p.nz} levels identical", flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_jacobi_fused_with_the_first_colour_sweep(precision):
+    """sor_fused bit 2: an even iteration's Jacobi step, its boundary
+    conditions and the first colour sweep in one out-of-place pass
+    (launch_jsor0).  Every grid, alone and with the odd iterations fused as
+    well, gives the fields of the reference's schedule bit for bit: u's outlet
+    extrapolation, the inlet copies and w's walls are the boundary kernel's.
+    A period that 8 does not divide (72x12x20) takes the unfused path."""
+    from tpls_amd.initial import make_initial
+    for grid in ("64x16x24", "130x8x38", "72x12x20"):
+        p = GRIDS[grid]
+        init = make_initial(p)
+        ref = run(p, init, precision, None, True)
+        for code in (4, 5):
+            for jst in (4, 8):
+                got = run(p, init, precision, (4, 2, 8, 4, 8, jst), code)
+                for f in FIELDS:
+                    assert np.array_equal(got[f], ref[f]), (grid, code, jst, f)
+    print(f"[jsor] {precision}: fused Jacobi + first colour sweep identical", flush=True)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_jacobi_colour_pass_chunks(precision):
+    """The fused pass forms J one level below and above its chunk of levels:
+    chunks of 1, 2, 3, 4, 5, 8, 9 and 17 levels and the whole column give the
+    fields of the unfused sweeps bit for bit (u, v: 40 levels, w: 39)."""
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    p = TALL
+    init = make_initial(p)
+    ref = run(p, init, precision, (4, 4, 4, 8, 8, -1), True)
+    for ks in (1, 2, 3, 4, 5, 8, 9, 17, p.nz):
+        with Case(p, precision=precision, tiles=(4, 4, 4, 8, 8, 8), sor_fused=5, march_levels=ks) as c:
+            c.load(*init)
+            c.step(3)
+            for f in FIELDS:
+                assert np.array_equal(c.field(f), ref[f]), (ks, f)
+    print(f"[jsor chunk] {precision}: chunks identical", flush=True)

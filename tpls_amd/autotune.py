@@ -15,7 +15,9 @@ and precision) and tunes on a miss.
     python -m tpls_amd.autotune 512x128x256 64x16x32 ... [--precision fp32]
 
 prints the tuned shapes as JSON lines for tuning.json.  A shape is
-(rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd, rows_jst, sor_fused).
+(rows, rows_odd, prefetch_odd, rows_rb, rows_rb_odd, rows_jst, sor_fused),
+sor_fused with bit 2 when the even iterations' Jacobi step and first colour
+sweep run as one pass.
 """
 from __future__ import annotations
 
@@ -47,6 +49,17 @@ def _per_launch(c, cls: str, steps: int) -> float:
     return ms / max(n, 1)
 
 
+def _per_step(c, steps: int) -> float:
+    """Milliseconds per step over `steps` steps, after one untimed step."""
+    import time
+    c.step(1)
+    c.sync()
+    t0 = time.perf_counter()
+    c.step(steps)
+    c.sync()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
 def tune(p, precision: str = "fp64", steps: int = 1, report: dict | None = None, mode: int = 0):
     """The launch shape with the least kernel time on this grid.  mode: the
     SOR schedule asked for (a tpls_params.sor_fused code); with 0, the
@@ -76,7 +89,11 @@ def tune(p, precision: str = "fp64", steps: int = 1, report: dict | None = None,
             if not even_fused:
                 times[("jst", r)] += _per_launch(c, "sor_even", 2 * steps)
         times[("jst", -1)] = times[("rows", best_rows)]
-        best_jst = min((-1,) + JST, key=lambda r: times[("jst", r)])
+        # a larger tile only when it is clearly faster (1 %): the sweeps' sum
+        # ties within noise between 8 and 16 rows in FP64, where the 8-row
+        # Jacobi alone is the faster
+        fastest = min(times[("jst", r)] for r in (-1,) + JST)
+        best_jst = next(r for r in (-1,) + JST if times[("jst", r)] <= 1.01 * fastest)
         best_rb = 8
         if even_fused:
             for rb in RB:
@@ -97,12 +114,29 @@ def tune(p, precision: str = "fp64", steps: int = 1, report: dict | None = None,
         ro, d, rbo = min(odd, key=odd.get)
         code = (even + (1 if rbo else 0)) or -1
         best = (best_rows, ro, d, best_rb, rbo or 8, best_jst, code)
+        # the even iterations' Jacobi step fused with their first colour
+        # sweep (sor_fused bit 2) replaces three launches by one, so only
+        # whole steps show its worth; kept when clearly faster (1 %)
+        if best_jst > 0 and not even_fused and p.ny % 8 == 0 and mode in (0, 1):
+            c.set_tiling(*best)
+            times[("jsor", 0)] = _per_step(c, 2 * steps)
+            with_jsor = (code if code > 0 else 0) | 4
+            c.set_tiling(*best[:6], with_jsor)
+            times[("jsor", 1)] = _per_step(c, 2 * steps)
+            if times[("jsor", 1)] < 0.99 * times[("jsor", 0)]:
+                best = best[:6] + (with_jsor,)
     if report is not None:
         report.update({"/".join(map(str, k)): round(v * 1e3, 2) for k, v in times.items()})
     return best
 
 
 def tuned(p, precision: str = "fp64", mode: int = 0):
+    if mode > 0 and mode & 4:
+        # the fused Jacobi + colour pass asked for: the shapes of the schedule
+        # without it, with bit 2 set
+        b = (mode & 3) or -1
+        base = tuned(p, precision, 0 if b == 1 else b)  # 1 is the default schedule
+        return base[:6] + (((base[6] if len(base) > 6 and base[6] > 0 else 0) | 4),)
     k = key(p, precision, mode)
     if k in _cache:
         return _cache[k]

@@ -26,7 +26,8 @@ def field_shape(name: str, maxl: int, maxm: int, maxn: int):
 
 def fused_code(sor_fused) -> int:
     """tpls_params.sor_fused from None (the default), a bool (all or no
-    SOR iterations fused) or the code itself (-1, 1, 2, 3)."""
+    SOR iterations fused) or the code itself (-1 .. 7; bit 2 fuses an even
+    iteration's Jacobi step with its first colour sweep)."""
     if sor_fused is None:
         return 0
     if isinstance(sor_fused, bool):

@@ -500,7 +500,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
     if (prm->prefetch_odd != 0 && prm->prefetch_odd != 4 && prm->prefetch_odd != 8)
         return fail(TPLS_ERR_ARG, "prefetch_odd must be 0 (default), 4 or 8");
     auto rb_ok = [](int r) { return r == 0 || r == 2 || r == 4 || r == 8; };
-    if (prm->sor_fused < -1 || prm->sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
+    if (prm->sor_fused < -1 || prm->sor_fused > 7) return fail(TPLS_ERR_ARG, "sor_fused must be -1 .. 7");
     if (prm->mg_fp32 < -1 || prm->mg_fp32 > 1) return fail(TPLS_ERR_ARG, "mg_fp32 must be -1, 0 or 1");
     if (!rb_ok(prm->tile_rows_rb) || !rb_ok(prm->tile_rows_rb_odd) ||
         (prm->tile_rows_jst != -1 && prm->tile_rows_jst != 16 && !rb_ok(prm->tile_rows_jst)))
@@ -869,7 +869,7 @@ int tpls_case_kernel_time(tpls_case* c, double* ms, long* launches)
 int tpls_case_set_tiling(tpls_case* c, int rows, int rows_odd, int prefetch_odd, int rows_rb, int rows_rb_odd,
                          int rows_jst, int sor_fused)
 {
-    if (sor_fused < -1 || sor_fused > 3) return fail(TPLS_ERR_ARG, "sor_fused must be -1, 0, 1, 2 or 3");
+    if (sor_fused < -1 || sor_fused > 7) return fail(TPLS_ERR_ARG, "sor_fused must be -1 .. 7");
     if (!c) return fail(TPLS_ERR_ARG, "null case");
     auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
     auto rb_ok = [](int r) { return r == 2 || r == 4 || r == 8; };
@@ -1226,6 +1226,16 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
             FieldView<T> xj = fused ? V(c->x[q][c->i2[q]]) : xd;
             const bool tj = c->timed == TPLS_TIME_JACOBI;
             const bool ts = c->timed == (it % 2 ? TPLS_TIME_SOR_ODD : TPLS_TIME_SOR_EVEN);
+            if (!fused && it % 2 == 0 && (c->tiling.fused & FUSED_JSOR) && c->tiling.jst && jsor0_fits(g) && !tj &&
+                !ts) {
+                // the Jacobi step, its boundary conditions and the first
+                // colour sweep in one pass, then the second sweep as below
+                LCHK(launch_jsor0<T>(s, g, ph, q, xs, xd, V(c->rhs[q]), V(c->mu), V(c->pres), c->tiling));
+                LCHK(launch_sor<T>(s, g, ph, q, xd, V(c->rhs[q]), V(c->mu), 1, it, c->tiling));
+                LCHK(launch_helmholtz_bc<T>(s, g, ph, q, xd, V(c->pres)));
+                std::swap(srcb, dstb);
+                continue;
+            }
             if (tj) LCHK(hipEventRecord(next_event(c), s));
             LCHK(launch_jacobi<T>(s, g, ph, q, xs, V(c->rhs[q]), V(c->mu), xj, it == 1, c->tiling));
             if (tj) LCHK(hipEventRecord(next_event(c), s));

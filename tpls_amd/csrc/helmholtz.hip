@@ -421,10 +421,15 @@ __device__ __forceinline__ void rb_z(int k, int maxn, C cur, C lo, C hi, C& zp, 
         zm = lo;
         return;
     }
-    const bool bot = k == 0;
-    const C ghost = C(-2) * cur + (bot ? hi : lo) / C(3);
-    zm = bot ? ghost : lo;
-    zp = (k == maxn - 2) ? ghost : hi;
+    // the ghost (and its division) only on the wall levels: k is uniform
+    // across a wavefront in every caller, so this is a scalar branch
+    zm = lo;
+    zp = hi;
+    if (k == 0 || k == maxn - 2) {
+        const C ghost = C(-2) * cur + (k == 0 ? hi : lo) / C(3);
+        if (k == 0) zm = ghost;
+        if (k == maxn - 2) zp = ghost;
+    }
 }
 
 template <int COMP, typename C>
@@ -1183,6 +1188,233 @@ void OpSor3dSt<COMP, TY, D, T>::launch(dim3 grid, hipStream_t s, const Args& a)
     hipLaunchKernelGGL((k_sor3d_st<COMP, TY, D, T>), grid, block(), 0, s, a.g, a.nbx, a.nby, a.ks, a.flg, a.ax2, a.ay2, a.az2, a.x, a.rhs, a.mu);
 }
 
+// An even iteration_time's Jacobi sweep, the boundary conditions after it
+// and the first colour sweep (flg 0), in one k-march from xs into xd: the
+// values main:758-887 leave in xd between its first SOR colour sweep and the
+// second, bit for bit.  A block owns a tile of 64 x 8 columns and a chunk of
+// levels.  At step m it forms the Jacobi value J(m) of every cell of the tile
+// and its one-cell ring (the ring rows wrapped in y), from x and mu planes
+// staged as in k_rb2d, and publishes the plane of J values in LDS; then each
+// cell of the tile at level m-1 takes its J value or, for the first colour,
+// the SOR update from the J values around it: the in-plane ones from LDS, its
+// own column's J(m-2) and J(m) from registers, with the viscosity faces,
+// 1/diag and RHS it used for J(m-1).  The boundary cells read are those the
+// boundary kernel would write: x(0) = J(1); x(imax+1) = J(imax) (v, w); the
+// walls of w are 0.  The block also writes those boundary cells.  u's outlet
+// column i = imax keeps J here: its boundary cells need the pressure (the
+// outlet extrapolation), and k_jsor0_outlet then writes them and updates the
+// column's first-colour cells, which read only second-colour neighbours.  xd
+// then holds exactly what the Jacobi launch, the boundary kernel and the
+// colour sweep leave.  J is formed one level below and above the chunk, so
+// chunks overlap only in reads.
+template <int COMP, int D, typename T>
+struct OpJSor0 {
+    static constexpr int TX = 64, TY = 8, RW = TX + 2, RR = TY + 2, NC = RW * RR;
+    static constexpr int NT = (NC + 63) / 64 * 64;
+    struct Args {
+        Grid g;
+        int nbx;
+        int nby;
+        int ks;
+        T ax2;
+        T ay2;
+        T az2;
+        T dtdx;
+        FieldView<T> xs;
+        FieldView<T> rhs;
+        FieldView<T> mu;
+        FieldView<T> pres;
+        FieldView<T> xd;
+    };
+    static constexpr int THREADS = NT;
+    static dim3 block() { return dim3(NT); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+    static void launch(dim3 grid, hipStream_t s, const Args& a);
+};
+
+template <int COMP, int D, typename T>
+__device__ __forceinline__ void OpJSor0<COMP, D, T>::run(const Args& a, const Blk& bk)
+{
+    using C = T;
+    const Grid& g = a.g;
+    const FieldView<T>& xs = a.xs;
+    const FieldView<T>& mu = a.mu;
+    const FieldView<T>& rhs = a.rhs;
+    const FieldView<T>& xd = a.xd;
+    const C ax2 = a.ax2, ay2 = a.ay2, az2 = a.az2;
+    constexpr int SR = TY + 4, SW = TX + 4, SN = SR * SW, NS = (SN + NT - 1) / NT;
+    __shared__ C xsh[4][SR][SW];
+    __shared__ C msh[4][SR][SW];
+    __shared__ C jr[2][RR][RW];
+    const int t = xcd_tile(bk.x, bk.n);
+    const int bx = t % a.nbx, rest = t / a.nbx, by = rest % a.nby, bz = rest / a.nby;
+    const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
+    const int P = g.P, maxn = g.maxn, maxl = g.maxl, imax = imax_of<COMP>(maxl);
+    const int k0 = kmin_of<COMP>(), k1 = maxn - 2, mtop = COMP == 2 ? maxn - 1 : maxn - 2;
+    const int kb = k0 + bz * a.ks, ke = min(kb + a.ks - 1, k1);
+    // this lane's cell of the tile and ring: (i0-1+c, j0-1+r)
+    const int e = int(threadIdx.x);
+    const bool lane = e < NC;
+    const int r = lane ? e / RW : 1, c = lane ? e % RW : 1;
+    const int i = i0 - 1 + c, jj = j0 - 1 + r, j = wrapj(jj, P);
+    const bool tile = lane && r >= 1 && r <= TY && c >= 1 && c <= TX && i <= imax;
+    const int ic = min(max(i, 1), imax);
+    // staged planes, as k_rb2d: x on rows j0-2.., mu on rows j0-1.., both on
+    // columns i0-2.., clamped into the arrays
+    long ox[NS], om[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int q = min(e + n * NT, SN - 1);
+        const int er = q / SW, ec = q % SW;
+        sl[n] = q;
+        ox[n] = min(max(i0 - 2 + ec, 0), maxl) + long(wrapj(j0 - 2 + er, P)) * xs.sy;
+        om[n] = min(i0 - 2 + ec, maxl + 1) + long(wrapj(j0 - 1 + er, P)) * mu.sy;
+    }
+    auto stage_load = [&](C* vx, C* vm, int st) {
+        const long kx = min(max(st, 0), mtop) * xs.sz, km = min(max(st + 1, 0), maxn) * mu.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            vx[n] = xs.p[ox[n] + kx];
+            vm[n] = mu.p[om[n] + km];
+        }
+    };
+    // x plane p in slot p & 3, mu plane p in slot p & 3 (stage st: x plane
+    // st, mu plane st + 1; planes here are >= -2, so & 3 is their residue)
+    auto stage_store = [&](const C* vx, const C* vm, int st) {
+        C* px = &xsh[st & 3][0][0];
+        C* pm = &msh[(st + 1) & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            px[sl[n]] = vx[n];
+            pm[sl[n]] = vm[n];
+        }
+    };
+    auto lev = [&](int m) { return min(max(m, k0), k1); };
+    const int m0 = kb - 1;
+    {
+        C vx[NS], vm[NS];
+#pragma unroll
+        for (int st = m0 - 1; st <= m0 + 1; ++st) {
+            stage_load(vx, vm, st);
+            stage_store(vx, vm, st);
+        }
+    }
+    C rx[D][NS], rm_[D][NS], rh[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        stage_load(rx[d], rm_[d], m0 + 2 + d);
+        rh[d] = rhs(ic, j, lev(m0 + d));
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // as in k_rb2d
+    __syncthreads();
+    // this column's J values at m-2 and m-1, and the faces and RHS of m-1
+    C jm2 = C(0), jm1 = C(0), rprev = C(0), rdprev = C(0);
+    MuFaces<C> fprev{};
+    const int send = m0 + (ke + 2 - m0 + D - 1) / D * D;
+    for (int sb = m0; sb < send; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int m = sb + d, b = m & 3;
+            stage_store(rx[d], rm_[d], m + 2);
+            stage_load(rx[d], rm_[d], m + 2 + D);
+            const C rv = rh[d];
+            rh[d] = rhs(ic, j, lev(m + D));
+            // J(m) of this cell (the walls of w are 0)
+            const MuFaces<C> f = mu_cook<COMP, C>(
+                mu_raw_at<COMP, C>([&](int di, int dj, int dk) { return msh[(m + dk) & 3][r + dj][c + 1 + di]; }));
+            C zp, zm;
+            rb_z<COMP, C>(m, maxn, C(xsh[b][r + 1][c + 1]), C(xsh[(m - 1) & 3][r + 1][c + 1]),
+                          C(xsh[(m + 1) & 3][r + 1][c + 1]), zp, zm);
+            const C rd = helmholtz_rdiag<C>(f, ax2, ay2, az2);
+            C jv = helmholtz_rd<C>(rd, f, ax2, ay2, az2, C(xsh[b][r + 1][c + 2]), C(xsh[b][r + 1][c]),
+                                   C(xsh[b][r + 2][c + 1]), C(xsh[b][r][c + 1]), zp, zm, rv);
+            if (COMP == 2 && (m <= 0 || m >= maxn - 1)) jv = C(0);
+            if (lane) jr[m & 1][r][c] = jv;
+            // level m-1: J, or the first colour's update from the J values
+            const int k = m - 1;
+            if (tile && k >= kb && k <= ke) {
+                const C cur = jm1;
+                // u's outlet column waits for k_jsor0_outlet, which needs the
+                // pressure; v's and w's x(imax+1) is J(imax)
+                const bool defer = COMP == 0 && i == imax;
+                C v = cur;
+                if (!defer && ((i + jj + k) & 1) != 0) {
+                    const int q = k & 1;
+                    const C xm = i == 1 ? cur : C(jr[q][r][c - 1]);
+                    const C xp = i == imax ? cur : C(jr[q][r][c + 1]);
+                    C zp2, zm2;
+                    rb_z<COMP, C>(k, maxn, cur, jm2, jv, zp2, zm2);
+                    const C res = helmholtz_rd<C>(rdprev, fprev, ax2, ay2, az2, xp, xm, C(jr[q][r + 1][c]),
+                                                  C(jr[q][r - 1][c]), zp2, zm2, rprev);
+                    v = cur + C(1.2) * (res - cur);
+                }
+                xd(i, jj, k) = v;
+                if (i == 1) xd(0, jj, k) = cur;
+                if (COMP != 0 && i == imax) xd(imax + 1, jj, k) = cur;
+            }
+            jm2 = jm1;
+            jm1 = jv;
+            fprev = f;
+            rprev = rv;
+            rdprev = rd;
+            __syncthreads();
+        }
+    }
+    // the walls of w, for the chunks that hold the first and last levels
+    if (COMP == 2 && tile) {
+        if (kb == k0) {
+            xd(i, jj, 0) = C(0);
+            if (i == 1) xd(0, jj, 0) = C(0);
+            if (i == imax) xd(imax + 1, jj, 0) = C(0);
+        }
+        if (ke == k1) {
+            xd(i, jj, maxn - 1) = C(0);
+            if (i == 1) xd(0, jj, maxn - 1) = C(0);
+            if (i == imax) xd(imax + 1, jj, maxn - 1) = C(0);
+        }
+    }
+}
+
+template <int COMP, int D, typename T>
+__global__ void __launch_bounds__((OpJSor0<COMP, D, T>::NT)) k_jsor0(typename OpJSor0<COMP, D, T>::Args a)
+{
+    OpJSor0<COMP, D, T>::run(a, blk_here());
+}
+
+template <int COMP, int D, typename T>
+void OpJSor0<COMP, D, T>::launch(dim3 grid, hipStream_t s, const Args& a)
+{
+    hipLaunchKernelGGL((k_jsor0<COMP, D, T>), grid, block(), 0, s, a);
+}
+
+// u's outlet column after k_jsor0 (which left J there): the boundary cells
+// x(L-1) (the pressure extrapolation, as k_bc_uv) and x(L) = 0, then the
+// first-colour update of the column's cells with k_sor3d's arithmetic.  One
+// thread per (j, k); each writes only its own cells and reads only
+// second-colour cells (J values) and its own x(L-1).
+template <typename T>
+__global__ void k_jsor0_outlet(Grid g, T ax2, T ay2, T az2, T dtdx, FieldView<T> x, FieldView<T> rhs,
+                               FieldView<T> mu, FieldView<T> pres)
+{
+    using C = T;
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = int(blockIdx.y) + 1;
+    if (k > g.maxn - 2) return;
+    const int L = g.maxl, i = L - 2;
+    const C cur = x(i, j, k);
+    const C bcr = cur + dtdx * (pres(L, j, k + 1) + pres(L - 2, j, k + 1) - T(2) * pres(L - 1, j, k + 1));
+    x(L - 1, j, k) = bcr;
+    x(L, j, k) = C(0);
+    if (((i + j + k) & 1) == 0) return;
+    const MuFaces<C> m = mu_cook<0, C>(mu_raw<0, C>(mu, i, j, k, g.P));
+    C zp, zm;
+    z_nbrs<0, C>(x, i, j, k, g.maxn, zp, zm);
+    const C res = helmholtz<C>(m, ax2, ay2, az2, bcr, C(x(i - 1, j, k)), C(x(i, wrapj(j + 1, g.P), k)),
+                               C(x(i, wrapj(j - 1, g.P), k)), zp, zm, C(rhs(i, j, k)));
+    x(i, j, k) = cur + C(1.2) * (res - cur);
+}
+
 // Outer-loop boundary conditions.
 // u: main_ns_hybrid.F90:764-776 (inlet copy, outlet pressure extrapolation
 //    with the unshifted j of pres, fictitious face maxl = 0);
@@ -1483,6 +1715,46 @@ hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int com
 }
 
 template <typename T, class E>
+hipError_t jsor0_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T> xs, FieldView<T> xd,
+                      FieldView<T> rhs, FieldView<T> mu, FieldView<T> pres, const Tiling& tl)
+{
+    if (!jsor0_fits(g)) return hipErrorInvalidValue;
+    const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
+    const T a2 = T(ax / 2.0), b2 = T(ay / 2.0), c2 = T(az / 2.0), dtdx = T(ph.dt / ph.dx);
+    auto go = [&](auto comp_c) {
+        constexpr int CP = decltype(comp_c)::value;
+        // FP64 loads 4 levels ahead: 8 would not fit the 168 registers of
+        // an 11-wave block
+        constexpr int D = sizeof(T) == 8 ? 4 : 8;
+        using Op = OpJSor0<CP, D, T>;
+        const int imax = CP == 0 ? g.maxl - 2 : g.maxl - 1, nz = nz_of(CP, g);
+        const int nbx = (imax + Op::TX - 1) / Op::TX, nby = g.P / Op::TY;
+        static const int slots = resident_blocks(k_jsor0<CP, D, T>, Op::NT);
+        const int ks = tl.ks > 0 ? std::min(tl.ks, nz) : march_chunk(nbx * nby, nz, e.slots(slots), 32);
+        const int nbz = (nz + ks - 1) / ks;
+        hipError_t rc = e.template go<Op>({g, nbx, nby, ks, a2, b2, c2, dtdx, xs, rhs, mu, pres, xd},
+                                          dim3(nbx * nby * nbz));
+        if (rc == hipSuccess && CP == 0) {
+            hipLaunchKernelGGL(k_jsor0_outlet<T>, dim3((g.maxn - 1 + 63) / 64, g.P), dim3(64), 0, e.s, g, a2, b2, c2,
+                               dtdx, xd, rhs, mu, pres);
+            rc = hipGetLastError();
+        }
+        return rc;
+    };
+    if (comp == 0) return go(std::integral_constant<int, 0>{});
+    if (comp == 1) return go(std::integral_constant<int, 1>{});
+    return go(std::integral_constant<int, 2>{});
+}
+
+template <typename T>
+hipError_t launch_jsor0(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xs, FieldView<T> xd,
+                        FieldView<T> rhs, FieldView<T> mu, FieldView<T> pres, const Tiling& tl)
+{
+    DirectEmit e{s};
+    return jsor0_emit<T>(e, g, ph, comp, xs, xd, rhs, mu, pres, tl);
+}
+
+template <typename T, class E>
 hipError_t helmholtz_bc_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T> x, FieldView<T> pres)
 {
     if (comp == 2) {
@@ -1635,7 +1907,9 @@ template int hh_group_loops<float>(GroupPlan&, const std::vector<HhGroupCase<flo
                                       FieldView<T>, int, int, const Tiling&);                                        \
     template hipError_t launch_sor_pair<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,   \
                                            FieldView<T>, FieldView<T>, int, const Tiling&);                          \
-    template hipError_t launch_helmholtz_bc<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>);
+    template hipError_t launch_helmholtz_bc<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>); \
+    template hipError_t launch_jsor0<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,      \
+                                        FieldView<T>, FieldView<T>, FieldView<T>, const Tiling&);
 TPLS_INST_HH(double)
 TPLS_INST_HH(float)
 

@@ -168,13 +168,26 @@ __device__ __forceinline__ float sdiv<float>(float a, float d)
 
 // (1/diag) * (...) of the Crank-Nicolson Helmholtz update,
 // jacobi_iteration_allflux.f90:84-90 (neighbour order x+, x-, y+, y-, z+, z-).
+// In two parts, so that a kernel that updates one cell twice with the same
+// viscosity faces can form 1/diag once: helmholtz_rdiag, then helmholtz_rd
+// with it; helmholtz is the two together.
+template <typename C>
+__device__ __forceinline__ C helmholtz_rdiag(const MuFaces<C>& m, C ax2, C ay2, C az2)
+{
+    C diag = C(1) + ax2 * (m.xp + m.xm) + ay2 * (m.yp + m.ym) + az2 * (m.zp + m.zm);
+    return recip(diag);
+}
+template <typename C>
+__device__ __forceinline__ C helmholtz_rd(C rd, const MuFaces<C>& m, C ax2, C ay2, C az2, C xp, C xm, C yp, C ym,
+                                          C zp, C zm, C rhs)
+{
+    return rd * (ax2 * (m.xp * xp + m.xm * xm) + ay2 * (m.yp * yp + m.ym * ym) + az2 * (m.zp * zp + m.zm * zm) + rhs);
+}
 template <typename C>
 __device__ __forceinline__ C helmholtz(const MuFaces<C>& m, C ax2, C ay2, C az2, C xp, C xm, C yp,
                                        C ym, C zp, C zm, C rhs)
 {
-    C diag = C(1) + ax2 * (m.xp + m.xm) + ay2 * (m.yp + m.ym) + az2 * (m.zp + m.zm);
-    return recip(diag) *
-           (ax2 * (m.xp * xp + m.xm * xm) + ay2 * (m.yp * yp + m.ym * ym) + az2 * (m.zp * zp + m.zm * zm) + rhs);
+    return helmholtz_rd(helmholtz_rdiag(m, ax2, ay2, az2), m, ax2, ay2, az2, xp, xm, yp, ym, zp, zm, rhs);
 }
 
 }  // namespace tpls

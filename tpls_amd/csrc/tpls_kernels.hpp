@@ -125,6 +125,9 @@ struct Tiling {
     // resident blocks by march_chunk)
     int ks = 0;
 };
+// Tiling::fused bit 2: an even iteration's Jacobi step and first colour sweep
+// in one pass (launch_jsor0) where the even sweeps are split and staged
+constexpr int FUSED_JSOR = 4;
 template <typename T>
 hipError_t launch_jacobi(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xold,
                          FieldView<T> rhs, FieldView<T> mu, FieldView<T> x, bool xold_is_zero,
@@ -145,6 +148,14 @@ hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int com
 template <typename T>
 hipError_t launch_helmholtz_bc(hipStream_t s, const Grid& g, const Phys& ph, int comp,
                                FieldView<T> x, FieldView<T> pres);
+// an even iteration_time's Jacobi step from xs, its boundary conditions and
+// the first SOR colour sweep (flg 0), into xd in one pass: xd then holds what
+// launch_jacobi, launch_helmholtz_bc and launch_sor(flg 0) leave in it.
+// Needs a period divisible by 8 (jsor0_fits).
+inline bool jsor0_fits(const Grid& g) { return g.P % 8 == 0; }
+template <typename T>
+hipError_t launch_jsor0(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xs, FieldView<T> xd,
+                        FieldView<T> rhs, FieldView<T> mu, FieldView<T> pres, const Tiling& tl = Tiling{});
 template <typename T>
 hipError_t launch_difference(hipStream_t s, const Grid& g, int nz, FieldView<T> a, FieldView<T> b,
                              Reduce red);
