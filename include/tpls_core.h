@@ -111,6 +111,11 @@ typedef struct {
                              the device's resident blocks; n >= 1 fixes n
                              (clamped to the column height).  Identical
                              results either way */
+    int mg_pair;          /* the multigrid's finest level runs each
+                             smoothing pair (both colour sweeps) as one
+                             out-of-place pass: 0 where the level has
+                             2 M cells or more, 1 wherever its ny is a
+                             multiple of 8, -1 never.  Identical results */
 } tpls_params;
 
 /* Per-step diagnostics (the values main_ns_hybrid.F90 prints). */

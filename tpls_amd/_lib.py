@@ -32,7 +32,7 @@ class Params(C.Structure):
                 ("tile_rows_rb", C.c_int), ("tile_rows_rb_odd", C.c_int), ("tile_rows_jst", C.c_int),
                 ("mg_fp32", C.c_int), ("idm_dim", C.c_int), ("epn", C.c_double), ("Pe", C.c_double),
                 ("dim_its", C.c_int), ("dim_split", C.c_int),
-                ("march_levels", C.c_int)]
+                ("march_levels", C.c_int), ("mg_pair", C.c_int)]
 
 
 class Diag(C.Structure):

@@ -37,9 +37,10 @@ def fused_code(sor_fused) -> int:
 
 def make_params(p: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                 graphs: bool = False, tiles=None, sor_fused=None, mg_fp32=None,
-                dim_split: bool = False, march_levels: int = 0) -> _lib.Params:
+                dim_split: bool = False, march_levels: int = 0, mg_pair: int = 0) -> _lib.Params:
     prm = _lib.Params()
     prm.march_levels = int(march_levels)
+    prm.mg_pair = int(mg_pair)
     prm.mg_fp32 = {None: 0, True: 1, False: -1}[mg_fp32]
     prm.dim_split = 1 if dim_split else 0
     for k, v in p.solver_dict().items():
@@ -67,7 +68,7 @@ FILE_FIELDS = ("dx", "dy", "dz", "dt", "Re", "mu_plus", "mu_minus", "height", "s
 class Case:
     def __init__(self, params: CaseParams, precision: str = "fp64", diagnostics: bool = True,
                  graphs: bool = False, tiles=None, sor_fused=None, file_params: _lib.Params | None = None,
-                 mg_fp32=None, dim_split: bool = False, march_levels: int = 0):
+                 mg_fp32=None, dim_split: bool = False, march_levels: int = 0, mg_pair: int = 0):
         """graphs: replay each step's launches from HIP graphs (same results).
         tiles: (rows, rows_odd, prefetch_odd[, rows_rb, rows_rb_odd[, rows_jst[, sor_fused]]]) launch
         shapes of the Helmholtz sweeps, "auto" for the tuned shapes of this
@@ -80,6 +81,8 @@ class Case:
         FP64.  dim_split: with idm dim, run each relaxation sweep as two
         launches instead of one fused pass (same values).  march_levels: levels
         per block of the k-marching sweeps (0: sized to the device; same
+        values).  mg_pair: the multigrid's finest-level smoothing pairs as one
+        pass each (0: on 2 M cells or more, 1: always, -1: never; same
         values).  file_params: the tpls_params of a configuration
         (tpls_amd.io.read_configuration), whose file-backed fields replace
         those params implies."""
@@ -92,7 +95,7 @@ class Case:
         self.shape = (params.maxl, params.maxm, params.maxn)
         self.tiles = tiles
         self._prm = make_params(params, precision, diagnostics, graphs, tiles, sor_fused, mg_fp32, dim_split,
-                                march_levels)
+                                march_levels, mg_pair)
         if file_params is not None:
             for k in FILE_FIELDS:
                 setattr(self._prm, k, getattr(file_params, k))

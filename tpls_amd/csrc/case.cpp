@@ -507,6 +507,7 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
         return fail(TPLS_ERR_ARG, "tile_rows_rb and tile_rows_rb_odd must be 0 (default), 2, 4 or 8 and "
                                   "tile_rows_jst -1 (unstaged), 0 (default), 2, 4, 8 or 16");
     if (prm->march_levels < 0) return fail(TPLS_ERR_ARG, "march_levels must be 0 (sized to the device) or >= 1");
+    if (prm->mg_pair < -1 || prm->mg_pair > 1) return fail(TPLS_ERR_ARG, "mg_pair must be -1, 0 or 1");
     if (prm->ksp_pc > 0) {
         MgLevel L[MG_MAXLEV];
         const int nl = mg_plan(maxl - 1, maxm - 1, maxn - 1, 1, 1, 1, 6, L);
@@ -612,6 +613,15 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
                 o.x = ax->view<double>();
                 o.b = ab->view<double>();
             }
+        }
+        // level 0's second vector: its smoothing pairs as one pass each
+        const MgLevel& l0 = c->mg.L[0];
+        const bool pair = prm->mg_pair > 0 ? l0.ny % 8 == 0 : (prm->mg_pair == 0 && mg_pair_fits(l0.nx, l0.ny, l0.nz));
+        if (c->mg.nlev > 1 && pair) {
+            DevArray* a2;
+            if ((rc = alloc_field(c, &a2, sp, c->mg.fp32 ? 4 : 8))) return bail(rc);
+            if (c->mg.fp32) c->mg.x2f = a2->view<float>();
+            else c->mg.x2 = a2->view<double>();
         }
     }
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);

@@ -180,6 +180,182 @@ __global__ void __launch_bounds__(BX* BY) k_mg_smooth(MgLevelT<X, B> o, int colo
     MgSmooth<X, B>::run({o, colour, st}, blk_here());
 }
 
+template <typename L>
+static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
+
+// Both sweeps of a smoothing pair, colour f then 1 - f, in one k-march and
+// out of place: xout receives what k_mg_smooth(f) then k_mg_smooth(1 - f)
+// leave in x, bit for bit (each cell reads the values it reads there; within
+// a colour the updates are order-free).  A block owns MP_TX x MP_TY columns
+// and a chunk of levels; a lane owns a pair of columns (a, a+1) of the tile
+// and its one-cell ring, one row.  At step m the lane updates its pair's
+// colour-f cell at level m from the old x (planes staged in LDS as in the
+// Helmholtz marches) and publishes it in LDS; the pair's colour-(1-f) cell
+// at level m-1 is in the same column, so the lane updates it next from the
+// colour-f values of level m-1 in LDS (in-plane) and its own colour-f values
+// of levels m-2 and m (z).  Absent neighbours are 0 as in mg_nsum.  The
+// first colour is formed one level below and above the chunk, so chunks
+// overlap only in reads.
+constexpr int MP_TX = 64, MP_TY = 8;
+template <typename X, typename B>
+struct MgPair {
+    static constexpr int PW = (MP_TX + 2) / 2, NP = PW * (MP_TY + 2), NT = (NP + 63) / 64 * 64;
+    static constexpr int D = 4;  // levels of old x and b loaded ahead
+    struct Args {
+        MgLevelT<X, B> o;
+        FieldView<X> xout;
+        int f;
+        int nbx, nby, ks;
+        const CgState* st;
+    };
+    static constexpr int THREADS = NT;
+    static dim3 block() { return dim3(NT); }
+    static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
+};
+
+template <typename X, typename B>
+__device__ __forceinline__ void MgPair<X, B>::run(const Args& a, const Blk& bk)
+{
+    using C = X;
+    const MgLevelT<X, B>& o = a.o;
+    if (a.st->done) return;
+    constexpr int SR = MP_TY + 4, SW = MP_TX + 4, SN = SR * SW, NS = (SN + NT - 1) / NT;
+    __shared__ C xs[4][SR][SW];
+    __shared__ C fp[2][MP_TY + 2][MP_TX + 2];
+    const int t = xcd_tile(bk.x, bk.n);
+    const int bx = t % a.nbx, rest = t / a.nbx, by = rest % a.nby, bz = rest / a.nby;
+    const int i0 = 1 + bx * MP_TX, j0 = 1 + by * MP_TY;
+    const int nx = o.nx, ny = o.ny, nz = o.nz, f = a.f;
+    const int kb = 1 + bz * a.ks, ke = min(kb + a.ks - 1, nz);
+    auto wrap = [&](int jj) { return jj < 1 ? jj + ny : (jj > ny ? jj - ny : jj); };
+    const int e = int(threadIdx.x);
+    const bool lane = e < NP;
+    const int r = lane ? e / PW : 1, h = lane ? e % PW : 0;
+    const int j = wrap(j0 - 1 + r);
+    const int ia = i0 - 1 + 2 * h;  // the pair's first column; region column 2h
+    const bool row_in = lane && r >= 1 && r <= MP_TY;
+    // staged old x: rows j0-2.. (wrapped), columns i0-2.. and planes clamped
+    // into the arrays (a clamped value is read by no updated cell)
+    long ox[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int q = min(e + n * NT, SN - 1);
+        const int er = q / SW, ec = q % SW;
+        sl[n] = q;
+        ox[n] = min(max(i0 - 2 + ec, 0), nx + 1) + long(wrap(j0 - 2 + er)) * o.x.sy;
+    }
+    auto stage_load = [&](C* v, int st) {
+        const long kx = long(min(max(st, 1), nz)) * o.x.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) v[n] = o.x.p[ox[n] + kx];
+    };
+    auto stage_store = [&](const C* v, int st) {
+        C* px = &xs[st & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) px[sl[n]] = v[n];
+    };
+    // b of the pair's two cells at a level (clamped into the interior)
+    const int ib0 = min(max(ia, 1), nx), ib1 = min(max(ia + 1, 1), nx);
+    auto b_at = [&](C* v, int m) {
+        const int mm = min(max(m, 1), nz);
+        v[0] = C(o.b(ib0, j, mm));
+        v[1] = C(o.b(ib1, j, mm));
+    };
+    const int m0 = kb - 1;
+    {
+        C v[NS];
+#pragma unroll
+        for (int st = m0 - 1; st <= m0 + 1; ++st) {
+            stage_load(v, st);
+            stage_store(v, st);
+        }
+    }
+    C rx[D][NS], rb[D][2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        stage_load(rx[d], m0 + 2 + d);
+        b_at(rb[d], m0 + d);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const C cx = C(o.cx), cy = C(o.cy), cz = C(o.cz);
+    C fm1 = C(0), fm2 = C(0);  // this lane's colour-f values of the last two steps
+    C bprev[2] = {C(0), C(0)};
+    const int send = m0 + (ke + 2 - m0 + D - 1) / D * D;
+    for (int sb = m0; sb < send; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int m = sb + d, sx = m & 3;
+            stage_store(rx[d], m + 2);
+            stage_load(rx[d], m + 2 + D);
+            const C bc[2] = {rb[d][0], rb[d][1]};
+            b_at(rb[d], m + D);
+            // colour f at level m, column cf (region column c)
+            const int off = (ia + j + m + f) & 1;
+            const int cf = ia + off, c = 2 * h + off;
+            const C xm = cf > 1 ? C(xs[sx][r + 1][c]) : C(0);
+            const C xp = cf < nx ? C(xs[sx][r + 1][c + 2]) : C(0);
+            const C ym = C(xs[sx][r][c + 1]), yp = C(xs[sx][r + 2][c + 1]);
+            const C zm = m > 1 ? C(xs[(m - 1) & 3][r + 1][c + 1]) : C(0);
+            const C zp = m < nz ? C(xs[(m + 1) & 3][r + 1][c + 1]) : C(0);
+            const C fv = sdiv(bc[off] + (cz * (zm + zp) + cx * (xm + xp) + cy * (ym + yp)), mg_deff<C>(o, cf, m));
+            if (lane) fp[m & 1][r][c] = fv;
+            // colour 1 - f at level k = m - 1, the same column
+            const int k = m - 1;
+            if (row_in && k >= kb && k <= ke) {
+                const int q = k & 1;
+                // (a ring column's own update is discarded: clamp its slots)
+                const C sxm = cf > 1 ? C(fp[q][r][max(c - 1, 0)]) : C(0);
+                const C sxp = cf < nx ? C(fp[q][r][min(c + 1, MP_TX + 1)]) : C(0);
+                const C szm = k > 1 ? fm2 : C(0);
+                const C szp = k < nz ? fv : C(0);
+                const C sv = sdiv(bprev[off] + (cz * (szm + szp) + cx * (sxm + sxp) +
+                                                cy * (C(fp[q][r - 1][c]) + C(fp[q][r + 1][c]))),
+                                  mg_deff<C>(o, cf, k));
+                // the pair's other column held colour f at level k: fm1
+                const int co = ia + (off ^ 1);
+                if (cf >= i0 && cf < i0 + MP_TX && cf <= nx) a.xout(cf, j, k) = sv;
+                if (co >= i0 && co < i0 + MP_TX && co <= nx) a.xout(co, j, k) = fm1;
+            }
+            fm2 = fm1;
+            fm1 = fv;
+            bprev[0] = bc[0];
+            bprev[1] = bc[1];
+            __syncthreads();
+        }
+    }
+}
+
+template <typename X, typename B>
+__global__ void __launch_bounds__((MgPair<X, B>::NT)) k_mg_pair(typename MgPair<X, B>::Args a)
+{
+    MgPair<X, B>::run(a, blk_here());
+}
+
+// The pair sweeps of a level as one k_mg_pair launch into xout when there is
+// one (the case allocates it where mg_pair_fits, or as tpls_params.mg_pair
+// asks) and the level has whole tiles in y, else two k_mg_smooth launches in
+// place.  Returns the vector holding the result.
+template <typename X, typename B>
+static FieldView<X> mg_pair(hipStream_t s, const MgLevelT<X, B>& o, FieldView<X> xout, int f, const CgState* st)
+{
+    if (xout.p && o.ny % MP_TY == 0) {
+        using Op = MgPair<X, B>;
+        static const int slots = resident_blocks(k_mg_pair<X, B>, Op::NT);
+        const int nbx = (o.nx + MP_TX - 1) / MP_TX, nby = o.ny / MP_TY;
+        const int ks = march_chunk(nbx * nby, o.nz, slots, 32);
+        const int nbz = (o.nz + ks - 1) / ks;
+        hipLaunchKernelGGL((k_mg_pair<X, B>), dim3(nbx * nby * nbz), dim3(Op::NT), 0, s,
+                           typename Op::Args{o, xout, f, nbx, nby, ks, st});
+        return xout;
+    }
+    const dim3 blk(BX, BY);
+    hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, f, st);
+    hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1 - f, st);
+    return o.x;
+}
+
 // Weights of the fine cells 2I-2+q (q = 0..3) feeding coarse cell I under the
 // transpose of the interpolation: the fine cell's parent contributes 3/4 and
 // its neighbour toward it 1/4 (mg_nbr); w[q] = 0 for a candidate outside a
@@ -685,26 +861,19 @@ int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag
     return l;
 }
 
-template <typename L>
-static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz); }
 
 // The V-cycle on a hierarchy whose level 0 is `top` (x stored as X, b the
 // FP64 CG residual) and whose levels 1.. are `lev[1..]` (all X).
 template <typename X>
 static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>& top, const MgLevelT<X, X>* lev,
                          int nlev, int nu, const double* spec, Reduce red, CgState* st,
-                         const MgLevelT<X, X>* topx = nullptr)
+                         const MgLevelT<X, X>* topx = nullptr, FieldView<X> x2 = FieldView<X>{nullptr, 0, 0})
 {
     const dim3 blk(BX, BY);
     const int last = nlev - 1;
     const FieldView<X> nobc{nullptr, 0, 0};
-    auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
+    auto rr = [&](const auto& o, const MgLevelT<X, X>& oc) {
         using B = std::decay_t<decltype(*o.b.p)>;
-        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
-        for (int q = 1; q < nu; ++q) {
-            hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
-            hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1, st);
-        }
         // coarse levels per block: as many as keep >= 1024 blocks, up to 8
         // (each block recomputes the two fine planes below its first level)
         const int ntx = (oc.nx + RR_CX - 1) / RR_CX, nty = (oc.ny + RR_CY - 1) / RR_CY;
@@ -713,31 +882,55 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         const int ntz = (oc.nz + rk - 1) / rk;
         hipLaunchKernelGGL((k_mg_rr<X, B, X>), dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
     };
-    auto up = [&](const auto& o, const MgLevelT<X, X>& oc, bool finest) {
+    // The smoothing pairs of the way down (red, black) on o, then the
+    // residual-restriction to oc.  With xb (the finest level's second
+    // vector) each pair is one out-of-place pass (mg_pair), alternating
+    // between o.x and xb.  Returns the vector the level's values end in.
+    auto pairs_rr = [&](auto o, const MgLevelT<X, X>& oc, FieldView<X> xb) {
+        using B = std::decay_t<decltype(*o.b.p)>;
+        const FieldView<X> xa = o.x;
+        for (int q = 1; q < nu; ++q) {
+            if (xb.p) {
+                o.x = mg_pair(s, o, o.x.p == xa.p ? xb : xa, 0, st);
+            } else {
+                hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
+                hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1, st);
+            }
+        }
+        rr(o, oc);
+        return o.x;
+    };
+    auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
+        using B = std::decay_t<decltype(*o.b.p)>;
+        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
+        pairs_rr(o, oc, nobc);
+    };
+    // The way up: prolongation, then the pairs (black, red), the red of the
+    // last dropped on the finest level (k_mg_last does it).  o.x holds the
+    // level's values; with xb the pairs alternate between xa and xb as above.
+    auto up = [&](auto o, const MgLevelT<X, X>& oc, bool finest, FieldView<X> xa, FieldView<X> xb) {
         using B = std::decay_t<decltype(*o.b.p)>;
         hipLaunchKernelGGL((k_mg_prolong<X, B, X>), dim3((oc.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz), blk, 0,
                            s, o, oc, st);
         for (int q = 0; q < nu; ++q) {
+            const bool pair = !finest || q < nu - 1;
+            if (xb.p && pair) {
+                o.x = mg_pair(s, o, o.x.p == xa.p ? xb : xa, 1, st);
+                continue;
+            }
             hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 1, st);
-            if (!finest || q < nu - 1) hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
+            if (pair) hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, 0, st);
         }
+        return o.x;
     };
+    FieldView<X> cur = top.x;  // where the finest level's values are
     if (last > 0 && topx) {
         // the finest level with b converted once by its first pass
         hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top), blk, 0, s, top, topx->b, st);
-        const MgLevelT<X, X>& o = *topx;
-        const MgLevelT<X, X>& oc = lev[1];
-        for (int q = 1; q < nu; ++q) {
-            hipLaunchKernelGGL((k_mg_smooth<X, X>), pair_grid(o), blk, 0, s, o, 0, st);
-            hipLaunchKernelGGL((k_mg_smooth<X, X>), pair_grid(o), blk, 0, s, o, 1, st);
-        }
-        const int ntx = (oc.nx + RR_CX - 1) / RR_CX, nty = (oc.ny + RR_CY - 1) / RR_CY;
-        int rk = 8;
-        while (rk > 1 && long(ntx) * nty * ((oc.nz + rk - 1) / rk) < 1024) rk /= 2;
-        const int ntz = (oc.nz + rk - 1) / rk;
-        hipLaunchKernelGGL((k_mg_rr<X, X, X>), dim3(ntx * nty * ntz), dim3(RR_T), 0, s, o, oc, ntx, nty, rk, st);
+        cur = pairs_rr(*topx, lev[1], x2);
     } else if (last > 0) {
-        down(top, lev[1]);
+        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top), blk, 0, s, top, nobc, st);
+        cur = pairs_rr(top, lev[1], x2);
     }
     for (int l = 1; l < last; ++l) down(lev[l], lev[l + 1]);
     if (last > 0) {
@@ -760,11 +953,19 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
             return hipErrorInvalidValue;
         }
     }
-    for (int l = last - 1; l >= 1; --l) up(lev[l], lev[l + 1], false);
-    if (last > 0 && topx)
-        up(*topx, lev[1], true);
-    else if (last > 0)
-        up(top, lev[1], true);
+    for (int l = last - 1; l >= 1; --l) up(lev[l], lev[l + 1], false, nobc, nobc);
+    if (last > 0 && topx) {
+        MgLevelT<X, X> o = *topx;
+        o.x = cur;
+        cur = up(o, lev[1], true, topx->x, x2);
+    } else if (last > 0) {
+        MgLevelT<X, double> o = top;
+        o.x = cur;
+        cur = up(o, lev[1], true, top.x, x2);
+    }
+    // the pairs come in twos (nu - 1 down, nu - 1 up), so the values are
+    // back in the level's own vector
+    if (cur.p != top.x.p) return hipErrorInvalidValue;
     const Tiles tl = cg_tiles(g);
     hipLaunchKernelGGL((k_mg_last<X>), dim3(tl.nblocks), blk, 0, s, top, tl, last > 0 ? 1 : 0, red.part, st);
     const double N = double(top.nx) * top.ny * top.nz;
@@ -774,7 +975,7 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
 
 hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
 {
-    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, h.spec, red, st);
+    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, h.spec, red, st, nullptr, h.x2);
     // the FP32 vectors with L's extents and coefficients
     MgLevel32 lev[MG_MAXLEV];
     for (int l = 0; l < h.nlev; ++l) {
@@ -785,7 +986,7 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     const MgLevelT<float, double> top{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
     // F[0].b: the FP64 residual in FP32, written by the first pass
     const MgLevel32 topx{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
-    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr);
+    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr, h.x2f);
 }
 
 // ---- the same V-cycle for many cases in lock-step, one launch per kernel

@@ -232,7 +232,15 @@ struct MgHierarchy {
     // eigenbasis tables of the coarsest level (mg_spectral_tables), or null:
     // the coarsest solve then sweeps
     const double* spec;
+    // a second vector of level 0's shape for the cycle's arithmetic (FP64:
+    // x2, FP32: x2f), or null: with it each smoothing pair of level 0 runs as
+    // one out-of-place pass (where mg_pair_fits)
+    FieldView<double> x2{nullptr, 0, 0};
+    FieldView<float> x2f{nullptr, 0, 0};
 };
+// Levels whose smoothing pairs run as one pass: whole 8-row tiles in y and
+// enough cells (2 M) to fill the device with 64 x 8 tiles.
+inline bool mg_pair_fits(int nx, int ny, int nz) { return ny % 8 == 0 && long(nx) * ny * nz >= (1L << 21); }
 // The coarsest level's separable operator cx Lx + cy Ly + cz Lz (+ the
 // shift diag - 2 (cx + cy + cz)) in its eigenbasis: Qx (nx x nx, row i,
 // column = mode), Qy, Qz, then the unit eigenvalues lx, ly, lz.  Returns the
