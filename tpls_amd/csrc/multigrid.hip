@@ -391,7 +391,7 @@ constexpr int RR_C = (RR_N + RR_T - 1) / RR_T;  // footprint cells per thread
 
 template <typename C>
 struct RrIn {
-    C b, c, xm, xp, ym, yp, zm, zp;
+    C b;  // the cell's right-hand side (its x neighbours come from the staged planes)
 };
 
 template <typename XF, typename BF, typename XC>
@@ -415,40 +415,57 @@ __device__ __forceinline__ void MgRr<XF, BF, XC>::run(const Args& a, const Blk& 
     const MgLevelT<XC, XC>& c = a.c;
     const int ntx = a.ntx, nty = a.nty, rk = a.rk;
     if (a.st->done) return;
+    // x planes over the footprint and a one-cell ring, a ring of four in LDS
+    // (plane fk in slot fk & 3): each value is loaded once per block and plane
+    constexpr int SX = RR_FX + 2, SY = RR_FY + 2, SXN = SX * SY, NSX = (SXN + RR_T - 1) / RR_T;
+    __shared__ C XS[4][SY][SX];
     __shared__ C T[RR_FY][RR_FX];
     __shared__ C SI[RR_FY][RR_CX];
     const int tid = threadIdx.x;
     const int bk = blk.x, tx = bk % ntx, r0 = bk / ntx, ty = r0 % nty, tz = r0 / nty;
     const int I0 = 1 + tx * RR_CX, J0 = 1 + ty * RR_CY, Kb = 1 + tz * rk, Ke = min(Kb + rk - 1, c.nz);
     const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;  // footprint origin (fine, unwrapped)
-    // this thread's footprint cells: (fine i, wrapped fine j, its y neighbours)
-    int ci[RR_C], cj[RR_C], cjm[RR_C], cjp[RR_C];
+    // fine row J (unwrapped) in 1..ny: the footprint can span more than one
+    // period on small levels
+    auto wrapf = [&](int J) { return ((J - 1) % f.ny + f.ny) % f.ny + 1; };
+    // this thread's footprint cells: fine i and wrapped j (footprint row r is
+    // fine row fj0 + r, staged row r + 1; column q is fine i fi0 + q, staged
+    // column q + 1)
+    int ci[RR_C], cj[RR_C];
     bool live[RR_C];
 #pragma unroll
     for (int n = 0; n < RR_C; ++n) {
         const int e = min(tid + n * RR_T, RR_N - 1);
         const int row = e / RR_FX, col = e % RR_FX;
         ci[n] = fi0 + col;
-        // the footprint can span more than one period on small levels
-        cj[n] = ((fj0 + row - 1) % f.ny + f.ny) % f.ny + 1;
-        cjm[n] = cj[n] == 1 ? f.ny : cj[n] - 1;
-        cjp[n] = cj[n] == f.ny ? 1 : cj[n] + 1;
+        cj[n] = wrapf(fj0 + row);
         live[n] = tid + n * RR_T < RR_N && ci[n] >= 1 && ci[n] <= f.nx;
     }
+    // staged x: columns fi0-1.. clamped into the array (a clamped value is
+    // read by no live cell), rows wrapped, planes clamped to 1..nz
+    long xo[NSX];
+    int xsl[NSX];
+#pragma unroll
+    for (int n = 0; n < NSX; ++n) {
+        const int q = min(tid + n * RR_T, SXN - 1);
+        const int er = q / SX, ec = q % SX;
+        xsl[n] = q;
+        xo[n] = min(max(fi0 - 1 + ec, 0), f.nx + 1) + long(wrapf(fj0 - 1 + er)) * f.x.sy;
+    }
+    auto xload = [&](C* v, int fk) {
+        const long kz = long(min(max(fk, 1), f.nz)) * f.x.sz;
+#pragma unroll
+        for (int n = 0; n < NSX; ++n) v[n] = f.x.p[xo[n] + kz];
+    };
+    auto xstore = [&](const C* v, int fk) {
+        C* px = &XS[fk & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NSX; ++n) px[xsl[n]] = v[n];
+    };
     auto load = [&](RrIn<C>* in, int fk) {
 #pragma unroll
-        for (int n = 0; n < RR_C; ++n) {
-            if (!live[n]) continue;
-            const int i = ci[n], j = cj[n];
-            in[n].b = C(f.b(i, j, fk));
-            in[n].c = f.x(i, j, fk);
-            in[n].zm = fk > 1 ? f.x(i, j, fk - 1) : C(0);
-            in[n].zp = fk < f.nz ? f.x(i, j, fk + 1) : C(0);
-            in[n].xm = i > 1 ? f.x(i - 1, j, fk) : C(0);
-            in[n].xp = i < f.nx ? f.x(i + 1, j, fk) : C(0);
-            in[n].ym = f.x(i, cjm[n], fk);
-            in[n].yp = f.x(i, cjp[n], fk);
-        }
+        for (int n = 0; n < RR_C; ++n)
+            if (live[n]) in[n].b = C(f.b(ci[n], cj[n], fk));
     };
     // this thread's coarse cell in the y and z sums
     const int cI = tid % RR_CX, cJ = tid / RR_CX;
@@ -459,21 +476,39 @@ __device__ __forceinline__ void MgRr<XF, BF, XC>::run(const Args& a, const Blk& 
     const int f_lo = max(2 * Kb - 2, 1), f_hi = min(2 * Ke + 1, f.nz);
     RrIn<C> in[RR_C];
     load(in, f_lo);
+    C xr[NSX];
+    for (int fk = f_lo - 1; fk <= f_lo + 1; ++fk) {
+        xload(xr, fk);
+        xstore(xr, fk);
+    }
+    xload(xr, f_lo + 2);
+    __syncthreads();
     C s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // sj of fine planes 2K-2 .. 2K+1
     for (int fk = f_lo; fk <= f_hi; ++fk) {
         // the residual of plane fk over the footprint
+        const int sc = fk & 3, sm = (fk - 1) & 3, sp = (fk + 1) & 3;
 #pragma unroll
         for (int n = 0; n < RR_C; ++n) {
             const int e = tid + n * RR_T;
             if (e >= RR_N) continue;
+            const int row = e / RR_FX, col = e % RR_FX;
             C t = 0;  // outside 1..nx: weight 0
             if (live[n]) {
-                const RrIn<C>& v = in[n];
-                const C nsum = C(f.cz) * (v.zm + v.zp) + C(f.cx) * (v.xm + v.xp) + C(f.cy) * (v.ym + v.yp);
-                t = v.b - (mg_deff<C>(f, ci[n], fk) * v.c - nsum);
+                const int i = ci[n];
+                const C xc = XS[sc][row + 1][col + 1];
+                const C zm = fk > 1 ? C(XS[sm][row + 1][col + 1]) : C(0);
+                const C zp = fk < f.nz ? C(XS[sp][row + 1][col + 1]) : C(0);
+                const C xm = i > 1 ? C(XS[sc][row + 1][col]) : C(0);
+                const C xp = i < f.nx ? C(XS[sc][row + 1][col + 2]) : C(0);
+                const C ym = XS[sc][row][col + 1], yp = XS[sc][row + 2][col + 1];
+                const C nsum = C(f.cz) * (zm + zp) + C(f.cx) * (xm + xp) + C(f.cy) * (ym + yp);
+                t = in[n].b - (mg_deff<C>(f, i, fk) * xc - nsum);
             }
-            T[e / RR_FX][e % RR_FX] = t;
+            T[row][col] = t;
         }
+        // plane fk+2 to its slot (the one plane fk-2 had, read last at fk-1)
+        xstore(xr, fk + 2);
+        xload(xr, fk + 3);
         if (fk < f_hi) load(in, fk + 1);
         __syncthreads();
         for (int e = tid; e < RR_FY * RR_CX; e += RR_T) {
