@@ -271,7 +271,8 @@ def tiles_of(a):
 
 def case_opts(a) -> dict:
     """Launch options of every case the bench creates (none changes a value)."""
-    return {"graphs": bool(a.graphs), "tiles": tiles_of(a), "sor_fused": a.sor_fused}
+    # sor_fused 0: the tuning table's schedule for the grid
+    return {"graphs": bool(a.graphs), "tiles": tiles_of(a), "sor_fused": a.sor_fused or None}
 
 
 def sor_split(a, kernel: str) -> bool:

@@ -1,5 +1,6 @@
 // Level-set update, Russo-Smereka reinitialisation and CSF force:
 // two_phase_levelset.f90:20-118 and levelset.f90:17-414.
+#include "tiles.hpp"
 #include "tpls_kernels.hpp"
 
 namespace tpls {
@@ -155,28 +156,23 @@ __global__ void __launch_bounds__(BX* BY) k_levelset_status_code(Grid g, T h, Fi
     }
 }
 
+// One iteration's new value of a cell from its code, iterate c, D and the
+// six neighbours (read only for the upwind branches).
 template <typename T>
-__global__ void __launch_bounds__(BX* BY) k_levelset_iter_code(Grid g, T h, FieldView<T> D, FieldView<int> code,
-                                                               FieldView<T> pr, FieldView<T> pn)
+__device__ __forceinline__ T iter_code_value(int cd, T c, T dv, T xm, T xp, T ym, T yp, T zm, T zp, T h)
 {
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.maxl - 1 || j > g.P) return;
     const T dtau = T(0.3) * h;
-    const int cd = code(i, j, k), sv = cd >> 1;
-    T c = pr(i, j, k);
+    const int sv = cd >> 1;
     T dc;
     if (cd & 1) {
-        dc = T(sv - 1) * fabs(c) - D(i, j, k);
+        dc = T(sv - 1) * fabs(c) - dv;
     } else {
         dc = T(0);
         if (sv != 0) {
             const T sg = sv == 1 ? T(1) : (sv == 2 ? T(-1) : T(0));
-            const int jp = wrapj(j + 1, g.P), jm = wrapj(j - 1, g.P);
-            T dxm = c - pr(i - 1, j, k), dxp = pr(i + 1, j, k) - c;
-            T dym = c - pr(i, jm, k), dyp = pr(i, jp, k) - c;
-            T dzm = c - pr(i, j, k - 1), dzp = pr(i, j, k + 1) - c;
+            T dxm = c - xm, dxp = xp - c;
+            T dym = c - ym, dyp = yp - c;
+            T dzm = c - zm, dzp = zp - c;
             T gx, gy, gz;
             if (sv == 1) {
                 T aP = fmax(dxm, T(0)), bM = fmin(dxp, T(0));
@@ -197,7 +193,103 @@ __global__ void __launch_bounds__(BX* BY) k_levelset_iter_code(Grid g, T h, Fiel
             }
         }
     }
-    pn(i, j, k) = c - dtau * dc / h;
+    return c - dtau * dc / h;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BX* BY) k_levelset_iter_code(Grid g, T h, FieldView<T> D, FieldView<int> code,
+                                                               FieldView<T> pr, FieldView<T> pn)
+{
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.maxl - 1 || j > g.P) return;
+    const int cd = code(i, j, k);
+    const T c = pr(i, j, k);
+    const int jp = wrapj(j + 1, g.P), jm = wrapj(j - 1, g.P);
+    const T dv = (cd & 1) ? T(D(i, j, k)) : T(0);
+    pn(i, j, k) = iter_code_value<T>(cd, c, dv, pr(i - 1, j, k), pr(i + 1, j, k), pr(i, jm, k), pr(i, jp, k),
+                                     pr(i, j, k - 1), pr(i, j, k + 1), h);
+}
+
+// The same iteration marching a 64 x 8 tile up a chunk of levels: the
+// iterate's planes (tile and a one-cell ring, rows wrapped) are loaded once
+// per block into an LDS ring of four, D prefetched levels ahead of use like
+// the code (on every cell: a load under the cell's condition would drain the
+// memory counters at every level).  Same values as k_levelset_iter_code.
+constexpr int LS_TY = 8;
+template <typename T, int D>
+__global__ void __launch_bounds__(BX* LS_TY) k_levelset_iter_st(Grid g, int nbx, int nby, int ks, T h, FieldView<T> Dd,
+                                                                FieldView<int> code, FieldView<T> pr,
+                                                                FieldView<T> pn)
+{
+    constexpr int R = LS_TY + 2, W = BX + 2, N = R * W, NT = BX * LS_TY, NS = (N + NT - 1) / NT;
+    __shared__ T ps[4][R][W];
+    const int t = xcd_tile(int(blockIdx.x), int(gridDim.x));
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int i0 = 1 + bx * BX, j0 = 1 + by * LS_TY;
+    const int kb = 1 + bz * ks, ke = min(kb + ks - 1, g.maxn - 1);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int i = i0 + tx, j = j0 + ty;  // j <= P: the tile rows divide the period
+    const bool mine = i <= g.maxl - 1;
+    const int ic = min(i, g.maxl - 1);
+    long off[NS];
+    int sl[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int e = min(tid + n * NT, N - 1);
+        const int r = e / W, c = e % W;
+        sl[n] = e;
+        off[n] = min(max(i0 - 1 + c, 0), g.maxl) + long(wrapj(j0 - 1 + r, g.P)) * pr.sy;
+    }
+    auto load = [&](T* v, int k) {
+        const long kz = long(min(max(k, 0), g.maxn)) * pr.sz;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) v[n] = pr.p[off[n] + kz];
+    };
+    auto store = [&](const T* v, int k) {
+        T* q = &ps[k & 3][0][0];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) q[sl[n]] = v[n];
+    };
+    {
+        T v[NS];
+#pragma unroll
+        for (int k = kb - 1; k <= kb + 1; ++k) {
+            load(v, k);
+            store(v, k);
+        }
+    }
+    T rx[D][NS], dv[D];
+    int cd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        load(rx[d], kb + 2 + d);
+        const int kk = min(kb + d, ke);
+        cd[d] = code(ic, j, kk);
+        dv[d] = Dd(ic, j, kk);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int kend = kb + (ke - kb + D) / D * D;
+    for (int sb = kb; sb < kend; sb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int k = sb + d, b = k & 3;
+            store(rx[d], k + 2);
+            load(rx[d], k + 2 + D);
+            const int cv = cd[d];
+            const T dd = dv[d];
+            const int kn = min(k + D, ke);
+            cd[d] = code(ic, j, kn);
+            dv[d] = Dd(ic, j, kn);
+            const T v = iter_code_value<T>(cv, T(ps[b][ty + 1][tx + 1]), (cv & 1) ? dd : T(0), T(ps[b][ty + 1][tx]),
+                                           T(ps[b][ty + 1][tx + 2]), T(ps[b][ty][tx + 1]), T(ps[b][ty + 2][tx + 1]),
+                                           T(ps[(k - 1) & 3][ty + 1][tx + 1]), T(ps[(k + 1) & 3][ty + 1][tx + 1]), h);
+            if (mine && k <= ke) pn(i, j, k) = v;
+            __syncthreads();
+        }
+    }
 }
 
 // levelset.f90:270-363: curvature (clipped above at 1/min(dx,dy,dz)) and the
@@ -346,6 +438,18 @@ template <typename T>
 hipError_t launch_levelset_iter_code(hipStream_t s, const Grid& g, double h, FieldView<T> D, FieldView<int> code,
                                      FieldView<T> pr, FieldView<T> pr_new)
 {
+    if (g.P % LS_TY == 0) {
+        // the staged march: tiles of 64 x 8 columns, chunks of levels that
+        // fill the last round of resident blocks
+        constexpr int DP = 4;
+        static const int slots = resident_blocks(k_levelset_iter_st<T, DP>, BX * LS_TY);
+        const int nbx = (g.maxl - 1 + BX - 1) / BX, nby = g.P / LS_TY, nz = g.maxn - 1;
+        const int ks = march_chunk(nbx * nby, nz, slots, 32);
+        const int nbz = (nz + ks - 1) / ks;
+        hipLaunchKernelGGL((k_levelset_iter_st<T, DP>), dim3(nbx * nby * nbz), dim3(BX, LS_TY), 0, s, g, nbx, nby, ks,
+                           T(h), D, code, pr, pr_new);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_levelset_iter_code<T>, grid3(g, g.maxn - 1), dim3(BX, BY), 0, s, g, T(h), D, code, pr,
                        pr_new);
     return hipGetLastError();
