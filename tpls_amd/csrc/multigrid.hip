@@ -948,7 +948,9 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         hipLaunchKernelGGL((k_mg_prolong<X, B, X>), dim3((oc.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz), blk, 0,
                            s, o, oc, st);
         for (int q = 0; q < nu; ++q) {
-            const bool pair = !finest || q < nu - 1;
+            // the finest level's last red runs in k_mg_last, or, with xb, in
+            // the last pair pass
+            const bool pair = !finest || q < nu - 1 || xb.p;
             if (xb.p && pair) {
                 o.x = mg_pair(s, o, o.x.p == xa.p ? xb : xa, 1, st);
                 continue;
@@ -958,14 +960,23 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         }
         return o.x;
     };
-    FieldView<X> cur = top.x;  // where the finest level's values are
+    // Where the finest level's values are.  With x2 its sweeps after the
+    // first pass are 2 nu - 1 pair passes (nu - 1 down, nu up, the last red
+    // among them), each moving the values to the other vector: the first
+    // pass writes x2, so that they end in the level's own vector.
+    const bool pairs = last > 0 && x2.p;
+    FieldView<X> cur = pairs ? x2 : top.x;
+    MgLevelT<X, double> top0 = top;
+    top0.x = cur;
     if (last > 0 && topx) {
         // the finest level with b converted once by its first pass
-        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top), blk, 0, s, top, topx->b, st);
-        cur = pairs_rr(*topx, lev[1], x2);
+        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, topx->b, st);
+        MgLevelT<X, X> o = *topx;
+        o.x = cur;
+        cur = pairs_rr(o, lev[1], pairs ? top.x : nobc);
     } else if (last > 0) {
-        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top), blk, 0, s, top, nobc, st);
-        cur = pairs_rr(top, lev[1], x2);
+        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, nobc, st);
+        cur = pairs_rr(top0, lev[1], pairs ? top.x : nobc);
     }
     for (int l = 1; l < last; ++l) down(lev[l], lev[l + 1]);
     if (last > 0) {
@@ -992,17 +1003,17 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     if (last > 0 && topx) {
         MgLevelT<X, X> o = *topx;
         o.x = cur;
-        cur = up(o, lev[1], true, topx->x, x2);
+        cur = up(o, lev[1], true, x2, pairs ? top.x : nobc);
     } else if (last > 0) {
         MgLevelT<X, double> o = top;
         o.x = cur;
-        cur = up(o, lev[1], true, top.x, x2);
+        cur = up(o, lev[1], true, x2, pairs ? top.x : nobc);
     }
-    // the pairs come in twos (nu - 1 down, nu - 1 up), so the values are
-    // back in the level's own vector
     if (cur.p != top.x.p) return hipErrorInvalidValue;
+    // the last red sweep (unless a pair pass did it) and the partials of z.r
     const Tiles tl = cg_tiles(g);
-    hipLaunchKernelGGL((k_mg_last<X>), dim3(tl.nblocks), blk, 0, s, top, tl, last > 0 ? 1 : 0, red.part, st);
+    hipLaunchKernelGGL((k_mg_last<X>), dim3(tl.nblocks), blk, 0, s, top, tl, last > 0 && !pairs ? 1 : 0, red.part,
+                       st);
     const double N = double(top.nx) * top.ny * top.nz;
     hipLaunchKernelGGL(k_mg_zf, dim3(1), dim3(256), 0, s, red.part, tl.nblocks, N, st);
     return hipGetLastError();
