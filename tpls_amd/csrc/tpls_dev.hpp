@@ -97,7 +97,7 @@ struct MuFaces {
 };
 
 template <typename C, typename T>
-__device__ __forceinline__ MuFaces<C> mu_faces_u(const FieldView<T>& mu, int i, int j, int k, int P)
+__device__ __forceinline__ MuFaces<C> mu_faces_u(const T& mu, int i, int j, int k, int P)
 {
     const int j1 = wrapj(j + 1, P), j2 = wrapj(j + 2, P), j0 = wrapj(j, P);
     MuFaces<C> m;
@@ -112,7 +112,7 @@ __device__ __forceinline__ MuFaces<C> mu_faces_u(const FieldView<T>& mu, int i, 
 }
 
 template <typename C, typename T>
-__device__ __forceinline__ MuFaces<C> mu_faces_v(const FieldView<T>& mu, int i, int j, int k, int P)
+__device__ __forceinline__ MuFaces<C> mu_faces_v(const T& mu, int i, int j, int k, int P)
 {
     const int j1 = wrapj(j + 1, P), j0 = wrapj(j, P);
     MuFaces<C> m;
@@ -127,7 +127,7 @@ __device__ __forceinline__ MuFaces<C> mu_faces_v(const FieldView<T>& mu, int i, 
 }
 
 template <typename C, typename T>
-__device__ __forceinline__ MuFaces<C> mu_faces_w(const FieldView<T>& mu, int i, int j, int k, int P)
+__device__ __forceinline__ MuFaces<C> mu_faces_w(const T& mu, int i, int j, int k, int P)
 {
     const int j1 = wrapj(j + 1, P), j2 = wrapj(j + 2, P), j0 = wrapj(j, P);
     MuFaces<C> m;
