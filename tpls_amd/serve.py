@@ -254,7 +254,15 @@ def main():
     ap.add_argument("--max-steps", type=int, default=2000)
     ap.add_argument("--no-drain", action="store_true", help="stop at the end of the arrival window")
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines on stderr (0: none)")
+    # at serving batch sizes (300-480 cases) each case on its own stream
+    # measured 4.5 % faster than the grouped pressure solves (700 against 670
+    # Mcell-steps/s over 3 minutes): enough streams to fill the device, and no
+    # lock-step wait for the slowest case's CG
+    ap.add_argument("--grouped", type=int, default=0, choices=[0, 1, 2],
+                    help="batch launch grouping (tpls_batch_set_grouped); 0: each case on its own stream")
     a = ap.parse_args()
+    from .case import set_batch_grouped
+    set_batch_grouped(a.grouped)
     window = a.minutes * 60
     reqs = poisson_requests(a.rate, window, a.seed, steps=(a.min_steps, a.max_steps))
     offered = sum(r.params.cells * r.steps for r in reqs) / window * 1e-6
