@@ -363,7 +363,9 @@ def batched_run(a):
     cfgs = batch_configs(a.batch_cases, seed=0)
     cases = []
     for p in cfgs:
-        c = Case(p, precision="fp32", diagnostics=False, **case_opts(a))
+        # the finest multigrid level's smoothing pairs as one pass each on
+        # every grid of the draw (ny a multiple of 8)
+        c = Case(p, precision="fp32", diagnostics=False, mg_pair=1, **case_opts(a))
         c.load(*make_initial(p))
         cases.append(c)
     batch_step(cases, a.batch_warmup)

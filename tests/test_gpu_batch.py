@@ -26,12 +26,12 @@ def _mix():
     return cs, precs
 
 
-def _run(cases, precs, steps, together, graphs=False):
+def _run(cases, precs, steps, together, graphs=False, mg_pair=0):
     from tpls_amd.case import Case, batch_step
     from tpls_amd.initial import make_initial
     hs = []
     for p, pr in zip(cases, precs):
-        c = Case(p, precision=pr, graphs=graphs)
+        c = Case(p, precision=pr, graphs=graphs, mg_pair=mg_pair)
         c.load(*make_initial(p))
         hs.append(c)
     if together:
@@ -112,3 +112,23 @@ def test_a_failing_case_leaves_the_others_unperturbed():
     finally:
         for c in hs:
             c.close()
+
+
+@pytest.mark.parametrize("grouped", [1, 0])
+def test_batch_with_pair_passes_equals_alone(grouped):
+    """With tpls_params.mg_pair = 1 every multigrid case whose ny is a
+    multiple of 8 runs its finest level's smoothing pairs as one pass each;
+    the grouped V-cycle runs them across the cases (k_group of k_mg_pair).
+    Each case batched equals itself alone, bit for bit."""
+    from tpls_amd.case import set_batch_grouped
+    cases, precs = _mix()
+    alone = _run(cases, precs, 4, together=False, mg_pair=1)
+    set_batch_grouped(grouped)
+    try:
+        mixed = _run(cases, precs, 4, together=True, mg_pair=1)
+    finally:
+        set_batch_grouped(1)
+    for q, ((fa, da), (fm, dm)) in enumerate(zip(alone, mixed)):
+        assert da["ksp_its"] == dm["ksp_its"], q
+        for f in FIELDS:
+            assert np.array_equal(fa[f], fm[f]), (q, f)

@@ -1065,6 +1065,7 @@ struct CycleView {
     int last, nu;
     const double* spec;
     const MgGroupCase* c;
+    FieldView<X> x2;  // level 0's second vector, or null: its pairs as one pass each
 };
 
 template <typename X>
@@ -1090,7 +1091,39 @@ CycleView<X> cycle_view(const MgGroupCase& gc)
         v.hx = h.F[0].b.p != nullptr;
         v.topx = MgLevel32{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
     }
+    if constexpr (std::is_same_v<X, double>)
+        v.x2 = h.x2;
+    else
+        v.x2 = h.x2f;
+    if (v.last == 0 || v.top.ny % MP_TY != 0) v.x2 = FieldView<X>{nullptr, 0, 0};
     return v;
+}
+
+// Level 0's vector after m pair passes of a case whose level 0 runs them:
+// its first pass writes the second vector, each pair pass moves to the other.
+template <typename X>
+FieldView<X> pair_buf(const CycleView<X>& v, int m)
+{
+    return m % 2 == 0 ? v.x2 : v.top.x;
+}
+
+// Pair passes m0 .. m0+count-1 of level 0 (colour f first) for a set of cases.
+template <typename X, typename B, typename Get>
+void group_pairs(GroupPlan& P, const std::vector<const CycleView<X>*>& vs, int f, int m0, int count, Get&& level_of)
+{
+    if (vs.empty()) return;
+    static const int slots = resident_blocks(k_mg_pair<X, B>, MgPair<X, B>::NT);
+    for (int q = 0; q < count; ++q) {
+        GroupItems<typename MgPair<X, B>::Args> it;
+        for (const CycleView<X>* v : vs) {
+            MgLevelT<X, B> o = level_of(*v);
+            o.x = pair_buf(*v, m0 + q);
+            const int nbx = (o.nx + MP_TX - 1) / MP_TX, nby = o.ny / MP_TY;
+            const int ks = march_chunk(nbx * nby, o.nz, std::max(1, slots / int(vs.size())), 32);
+            it.add({o, pair_buf(*v, m0 + q + 1), f, nbx, nby, ks, v->c->st}, dim3(nbx * nby * ((o.nz + ks - 1) / ks)));
+        }
+        P.add<MgPair<X, B>>(it.v);
+    }
 }
 
 // Smoothing sweeps of one level for a set of cases: colours in the order
@@ -1129,22 +1162,40 @@ void group_cycle(GroupPlan& P, const std::vector<CycleView<X>>& all)
             if (l == v.last) co.push_back(&v);
         }
         if (l == 0) {
+            // cases whose level 0 runs pair passes (p) and the others
+            std::vector<const CycleView<X>*> dnp, dnxp, dns, dnxs;
+            for (const auto* v : dn) (v->x2.p ? dnp : dns).push_back(v);
+            for (const auto* v : dnx) (v->x2.p ? dnxp : dnxs).push_back(v);
             GroupItems<typename MgPre<X, double>::Args> pre;
-            for (const auto* v : dn) pre.add({v->top, nobc, v->c->st}, pair_grid(v->top));
-            for (const auto* v : dnx) pre.add({v->top, v->topx.b, v->c->st}, pair_grid(v->top));
+            for (const auto* v : dn) {
+                MgLevelT<X, double> t = v->top;
+                if (v->x2.p) t.x = pair_buf(*v, 0);
+                pre.add({t, nobc, v->c->st}, pair_grid(t));
+            }
+            for (const auto* v : dnx) {
+                MgLevelT<X, double> t = v->top;
+                if (v->x2.p) t.x = pair_buf(*v, 0);
+                pre.add({t, v->topx.b, v->c->st}, pair_grid(t));
+            }
             P.add<MgPre<X, double>>(pre.v);
-            group_sweeps<X, double>(P, dn, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
-            group_sweeps<X, X>(P, dnx, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+            group_sweeps<X, double>(P, dns, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
+            group_sweeps<X, X>(P, dnxs, nu - 1, false, false, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+            group_pairs<X, double>(P, dnp, 0, 0, nu - 1, [](const CycleView<X>& v) { return v.top; });
+            group_pairs<X, X>(P, dnxp, 0, 0, nu - 1, [](const CycleView<X>& v) { return v.topx; });
             GroupItems<typename MgRr<X, double, X>::Args> rr;
             for (const auto* v : dn) {
                 const RrShape r = rr_shape(v->lev[1]);
-                rr.add({v->top, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
+                MgLevelT<X, double> t = v->top;
+                if (v->x2.p) t.x = pair_buf(*v, nu - 1);
+                rr.add({t, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
             }
             P.add<MgRr<X, double, X>>(rr.v);
             GroupItems<typename MgRr<X, X, X>::Args> rrx;
             for (const auto* v : dnx) {
                 const RrShape r = rr_shape(v->lev[1]);
-                rrx.add({v->topx, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
+                MgLevelT<X, X> t = v->topx;
+                if (v->x2.p) t.x = pair_buf(*v, nu - 1);
+                rrx.add({t, v->lev[1], r.ntx, r.nty, r.rk, v->c->st}, dim3(r.nblk));
             }
             P.add<MgRr<X, X, X>>(rrx.v);
         } else if (!dn.empty()) {
@@ -1191,17 +1242,27 @@ void group_cycle(GroupPlan& P, const std::vector<CycleView<X>>& all)
             if (l < v.last) (l == 0 && v.hx ? upx : up).push_back(&v);
         if (l == 0) {
             GroupItems<typename MgProlong<X, double, X>::Args> pr;
-            for (const auto* v : up)
-                pr.add({v->top, v->lev[1], v->c->st},
-                       dim3((v->lev[1].nx + BX - 1) / BX, (v->top.ny + BY - 1) / BY, v->top.nz));
+            for (const auto* v : up) {
+                MgLevelT<X, double> t = v->top;
+                if (v->x2.p) t.x = pair_buf(*v, nu - 1);
+                pr.add({t, v->lev[1], v->c->st}, dim3((v->lev[1].nx + BX - 1) / BX, (t.ny + BY - 1) / BY, t.nz));
+            }
             P.add<MgProlong<X, double, X>>(pr.v);
             GroupItems<typename MgProlong<X, X, X>::Args> prx;
-            for (const auto* v : upx)
-                prx.add({v->topx, v->lev[1], v->c->st},
-                        dim3((v->lev[1].nx + BX - 1) / BX, (v->topx.ny + BY - 1) / BY, v->topx.nz));
+            for (const auto* v : upx) {
+                MgLevelT<X, X> t = v->topx;
+                if (v->x2.p) t.x = pair_buf(*v, nu - 1);
+                prx.add({t, v->lev[1], v->c->st}, dim3((v->lev[1].nx + BX - 1) / BX, (t.ny + BY - 1) / BY, t.nz));
+            }
             P.add<MgProlong<X, X, X>>(prx.v);
-            group_sweeps<X, double>(P, up, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
-            group_sweeps<X, X>(P, upx, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+            std::vector<const CycleView<X>*> upp, upxp, ups, upxs;
+            for (const auto* v : up) (v->x2.p ? upp : ups).push_back(v);
+            for (const auto* v : upx) (v->x2.p ? upxp : upxs).push_back(v);
+            group_sweeps<X, double>(P, ups, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, double>& { return v.top; });
+            group_sweeps<X, X>(P, upxs, nu, true, true, [](const CycleView<X>& v) -> const MgLevelT<X, X>& { return v.topx; });
+            // all nu pairs, the last red among them: 2 nu - 1 passes end in top.x
+            group_pairs<X, double>(P, upp, 1, nu - 1, nu, [](const CycleView<X>& v) { return v.top; });
+            group_pairs<X, X>(P, upxp, 1, nu - 1, nu, [](const CycleView<X>& v) { return v.topx; });
         } else if (!up.empty()) {
             GroupItems<typename MgProlong<X, X, X>::Args> pr;
             for (const auto* v : up)
@@ -1216,7 +1277,7 @@ void group_cycle(GroupPlan& P, const std::vector<CycleView<X>>& all)
     GroupItems<MgZf::Args> zf;
     for (const auto& v : all) {
         const Tiles tl = cg_tiles(v.c->g);
-        la.add({v.top, tl, v.last > 0 ? 1 : 0, v.c->part, v.c->st}, dim3(tl.nblocks));
+        la.add({v.top, tl, v.last > 0 && !v.x2.p ? 1 : 0, v.c->part, v.c->st}, dim3(tl.nblocks));
         zf.add({v.c->part, tl.nblocks, double(v.top.nx) * v.top.ny * v.top.nz, v.c->st}, dim3(1));
     }
     P.add<MgLast<X>>(la.v);
