@@ -70,9 +70,10 @@ def parse():
                     help="cases of the batched FP32 run reported beside the single case (0 to skip)")
     ap.add_argument("--batch-steps", type=int, default=4, help="timed batched steps")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched workload")
-    ap.add_argument("--batch-grouped", type=int, default=1,
-                    help="1: the batch's pressure solves run grouped (one launch per kernel across the cases); "
-                         "2: the Helmholtz loops as well; 0: each case on its own stream")
+    ap.add_argument("--batch-grouped", type=int, default=2,
+                    help="2: the batch's Helmholtz loops and pressure solves run grouped (one launch per kernel "
+                         "across the cases; the fastest mode for the 64-case batch); 1: the pressure solves only; "
+                         "0: each case on its own stream")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the serving run reported beside the single case (0 to skip)")
     ap.add_argument("--serve-rate", type=float, default=1.0,

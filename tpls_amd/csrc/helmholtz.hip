@@ -1233,11 +1233,9 @@ struct OpJSor0 {
         T ax2;
         T ay2;
         T az2;
-        T dtdx;
         FieldView<T> xs;
         FieldView<T> rhs;
         FieldView<T> mu;
-        FieldView<T> pres;
         FieldView<T> xd;
     };
     static constexpr int THREADS = NT;
@@ -1746,7 +1744,7 @@ hipError_t jsor0_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T
         static const int slots = resident_blocks(k_jsor0<CP, D, T>, Op::NT);
         const int ks = tl.ks > 0 ? std::min(tl.ks, nz) : march_chunk(nbx * nby, nz, e.slots(slots), 32);
         const int nbz = (nz + ks - 1) / ks;
-        hipError_t rc = e.template go<Op>({g, nbx, nby, ks, a2, b2, c2, dtdx, xs, rhs, mu, pres, xd},
+        hipError_t rc = e.template go<Op>({g, nbx, nby, ks, a2, b2, c2, xs, rhs, mu, xd},
                                           dim3(nbx * nby * nbz));
         if (rc == hipSuccess && CP == 0) {
             hipLaunchKernelGGL(k_jsor0_outlet<T>, dim3((g.maxn - 1 + 63) / 64, g.P), dim3(64), 0, e.s, g, a2, b2, c2,

@@ -200,7 +200,9 @@ constexpr int MP_TX = 64, MP_TY = 8;
 template <typename X, typename B>
 struct MgPair {
     static constexpr int PW = (MP_TX + 2) / 2, NP = PW * (MP_TY + 2), NT = (NP + 63) / 64 * 64;
-    static constexpr int D = 4;  // levels of old x and b loaded ahead
+    // levels of old x and b loaded ahead: 8 measured 1.5 % faster per FP32
+    // step, 4 faster in FP64
+    static constexpr int D = sizeof(X) == 4 ? 8 : 4;
     struct Args {
         MgLevelT<X, B> o;
         FieldView<X> xout;
