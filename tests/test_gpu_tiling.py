@@ -69,6 +69,24 @@ def test_fused_sor_pass_matches_the_two_sweeps(grid, precision):
     print(f"[sor pass] {grid} {precision}: fused == split for tile rows 2/4/8", flush=True)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_components_with_different_iteration_counts(precision):
+    """In FP32 the three components' Helmholtz loops run in lock-step; an
+    odd iteration that all three reach runs their fused passes as one launch
+    (launch_sor_pair3), the iterations past the shortest loop one component
+    at a time (FP64 runs each loop whole).  Outer-loop counts 8, 6 and 4
+    (u, v, w) give the reference's schedule bit for bit."""
+    import dataclasses
+    from tpls_amd.initial import make_initial
+    p = dataclasses.replace(GRIDS["64x16x24"], mom_u=8, mom_v=6, mom_w=4)
+    init = make_initial(p)
+    ref = run(p, init, precision, None, True)
+    for t in ((4, 2, 8, 8, 8), (4, 2, 8, 8, 8, 8), (4, 2, 8, 4, 4)):
+        got = run(p, init, precision, t, False if len(t) == 5 else 5)
+        for f in FIELDS:
+            assert np.array_equal(got[f], ref[f]), (t, f)
+
+
 # a column of 40 levels (u, v: 40, w: 39 updated levels per column)
 TALL = CaseParams(nx=64, ny=16, nz=41, Re=150.0, mu_plus=3.0, dt=2e-4, noise=1e-2, seed=24, mom_u=8, mom_v=8,
                   mom_w=8)

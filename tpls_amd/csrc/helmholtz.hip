@@ -575,6 +575,22 @@ struct MuColumn {
 // writes it to LDS; at step s the planes x s-1..s+1 and mu s-1..s+2 are read
 // (rings of four and five).  The steps are padded to whole rounds of the
 // register ring, with nothing stored outside k0..k1.
+// LDS of the fused odd pass: staged x and mu planes and the red planes.  One
+// variable per tile shape and type, whatever the component, so that a launch
+// running the three components' tiles (k_rb2d3) holds one set per block.
+template <int TX, int TY, typename C>
+struct Rb2dShared {
+    C xs[4][TY + 4][TX + 4];
+    C ms[5][TY + 4][TX + 4];
+    C plane[3][TY + 2][TX + 2];  // red values of two levels and a spare for lanes with nothing to publish
+};
+template <int TX, int TY, typename C>
+__device__ __forceinline__ Rb2dShared<TX, TY, C>& rb2d_shared()
+{
+    __shared__ Rb2dShared<TX, TY, C> sh;
+    return sh;
+}
+
 template <int COMP, int TX, int TY, int D, typename T>
 struct OpRb2d {
     struct Args {
@@ -610,9 +626,10 @@ __device__ __forceinline__ void OpRb2d<COMP, TX, TY, D, T>::run(const Args& args
     using LN = RbLanes<TX, TY>;
     constexpr int SR = TY + 4, SW = TX + 4, SN = SR * SW, WR = TX + 2;  // staged planes; red plane width
     constexpr int NT = LN::NT, NS = (SN + NT - 1) / NT;
-    __shared__ C xs[4][SR][SW];
-    __shared__ C ms[5][SR][SW];
-    __shared__ C plane[3][TY + 2][WR];  // red values of two levels and a spare for lanes with nothing to publish
+    Rb2dShared<TX, TY, C>& sh = rb2d_shared<TX, TY, C>();
+    C(&xs)[4][SR][SW] = sh.xs;
+    C(&ms)[5][SR][SW] = sh.ms;
+    C(&plane)[3][TY + 2][WR] = sh.plane;
     const int t = xcd_tile(bk_.x, bk_.n);
     const int i0 = 1 + (t % nbx) * TX, j0 = 1 + (t / nbx) * TY;
     const int P = g.P, maxn = g.maxn, imax = imax_of<COMP>(g.maxl);
@@ -742,6 +759,28 @@ template <int COMP, int TX, int TY, int D, typename T>
 void OpRb2d<COMP, TX, TY, D, T>::launch(dim3 grid, hipStream_t s, const Args& a)
 {
     hipLaunchKernelGGL((k_rb2d<COMP, TX, TY, D, T>), grid, block(), 0, s, a.g, a.nbx, a.ax2, a.ay2, a.az2, a.xin, a.xout, a.rhs, a.mu);
+}
+
+// The odd pass of u, v and w in one launch: blocks [0, b1) run u's tiles,
+// [b1, b2) v's and [b2, ..) w's, with b1 and b2 multiples of 8 so that each
+// component's tile b runs on XCD b % 8 as in its own launch.  One pass of
+// one component is a single round of blocks (its k march is sequential, so
+// it cannot be split into chunks of levels); the three together, at W waves
+// per SIMD, keep three blocks on each CU and leave no CU idle at the end of
+// one component's pass.
+template <int TX, int TY, int D, int W, typename T>
+__global__ void __launch_bounds__((RbLanes<TX, TY>::NT)) __attribute__((amdgpu_waves_per_eu(W)))
+k_rb2d3(typename OpRb2d<0, TX, TY, D, T>::Args a0, typename OpRb2d<1, TX, TY, D, T>::Args a1,
+        typename OpRb2d<2, TX, TY, D, T>::Args a2, int b1, int b2, int n0, int n1, int n2)
+{
+    const int b = int(blockIdx.x);
+    if (b < b1) {
+        if (b < n0) OpRb2d<0, TX, TY, D, T>::run(a0, Blk{b, 0, 0, n0});
+    } else if (b < b2) {
+        if (b - b1 < n1) OpRb2d<1, TX, TY, D, T>::run(a1, Blk{b - b1, 0, 0, n1});
+    } else if (b - b2 < n2) {
+        OpRb2d<2, TX, TY, D, T>::run(a2, Blk{b - b2, 0, 0, n2});
+    }
 }
 
 // Even iteration_time, levels kb..ke of one chunk.  Step s: red lanes update
@@ -1726,6 +1765,41 @@ hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int com
     return sor_pair_emit<T>(e, g, ph, comp, xin, xout, rhs, mu, iteration_time, tl);
 }
 
+// sor_pair_emit of u, v and w at one odd iteration_time as one launch
+// (k_rb2d3).  8-row tiles only: hipErrorInvalidValue where sor_pair_emit
+// would not take them (sor_pair3_fits).
+template <typename T>
+hipError_t launch_sor_pair3(hipStream_t s, const Grid& g, const Phys& ph, const FieldView<T>* xin,
+                            const FieldView<T>* xout, const FieldView<T>* rhs, FieldView<T> mu, const Tiling& tl)
+{
+    if (!sor_pair3_fits(g, tl)) return hipErrorInvalidValue;
+    const double ax = ph.dt / (ph.dx * ph.dx), ay = ph.dt / (ph.dy * ph.dy), az = ph.dt / (ph.dz * ph.dz);
+    const T a2 = T(ax / 2.0), bb = T(ay / 2.0), c2 = T(az / 2.0);
+    constexpr int TX = 32, TY = 8;
+    const int nby = g.P / TY;
+    const int n0 = (g.maxl - 2 + TX - 1) / TX * nby, n12 = (g.maxl - 1 + TX - 1) / TX * nby;
+    const int b1 = (n0 + GROUP_ALIGN - 1) / GROUP_ALIGN * GROUP_ALIGN;
+    const int b2 = b1 + (n12 + GROUP_ALIGN - 1) / GROUP_ALIGN * GROUP_ALIGN;
+    const int nb0 = n0 / nby, nb12 = n12 / nby;
+    auto go = [&](auto d_c, auto w_c) {
+        constexpr int D = decltype(d_c)::value, W = decltype(w_c)::value;
+        hipLaunchKernelGGL((k_rb2d3<TX, TY, D, W, T>), dim3(b2 + n12), dim3(RbLanes<TX, TY>::NT), 0, s,
+                           typename OpRb2d<0, TX, TY, D, T>::Args{g, nb0, a2, bb, c2, xin[0], xout[0], rhs[0], mu},
+                           typename OpRb2d<1, TX, TY, D, T>::Args{g, nb12, a2, bb, c2, xin[1], xout[1], rhs[1], mu},
+                           typename OpRb2d<2, TX, TY, D, T>::Args{g, nb12, a2, bb, c2, xin[2], xout[2], rhs[2], mu},
+                           b1, b2, n0, n12, n12);
+        return hipGetLastError();
+    };
+    // 4 waves per SIMD (three 5-wave blocks per CU) with the deepest
+    // prefetch that fits 128 registers unspilled: 2 levels in FP64, 4 in
+    // FP32.  Measured at 512 x 128 x 256 (one launch against the three
+    // one-component passes): FP64 500 against 520 us, FP32 step 17.3 against
+    // 17.9 ms; 4 levels at 3 waves per SIMD, or 8 (FP32) / 4 (FP64) at 4
+    // waves with spills, were slower.
+    if (sizeof(T) == 8) return go(std::integral_constant<int, 2>{}, std::integral_constant<int, 4>{});
+    return go(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
+}
+
 template <typename T, class E>
 hipError_t jsor0_emit(E& e, const Grid& g, const Phys& ph, int comp, FieldView<T> xs, FieldView<T> xd,
                       FieldView<T> rhs, FieldView<T> mu, FieldView<T> pres, const Tiling& tl)
@@ -1920,9 +1994,12 @@ template int hh_group_loops<float>(GroupPlan&, const std::vector<HhGroupCase<flo
     template hipError_t launch_sor_pair<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,   \
                                            FieldView<T>, FieldView<T>, int, const Tiling&);                          \
     template hipError_t launch_helmholtz_bc<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>); \
+    template hipError_t launch_sor_pair3<T>(hipStream_t, const Grid&, const Phys&, const FieldView<T>*,              \
+                                            const FieldView<T>*, const FieldView<T>*, FieldView<T>, const Tiling&); \
     template hipError_t launch_jsor0<T>(hipStream_t, const Grid&, const Phys&, int, FieldView<T>, FieldView<T>,      \
                                         FieldView<T>, FieldView<T>, FieldView<T>, const Tiling&);
 TPLS_INST_HH(double)
 TPLS_INST_HH(float)
 
 }  // namespace tpls
+

@@ -143,6 +143,14 @@ template <typename T>
 hipError_t launch_sor_pair(hipStream_t s, const Grid& g, const Phys& ph, int comp, FieldView<T> xin,
                            FieldView<T> xout, FieldView<T> rhs, FieldView<T> mu, int iteration_time,
                            const Tiling& tl = Tiling{});
+// launch_sor_pair of u, v and w (xin[q], xout[q], rhs[q]) at an odd
+// iteration_time as one launch, which keeps more blocks on each CU than the
+// one-component passes (same values).  Where the one-component pass takes
+// 8-row tiles: sor_pair3_fits.
+inline bool sor_pair3_fits(const Grid& g, const Tiling& tl) { return tl.rb_rows_odd == 8 && g.P % 8 == 0; }
+template <typename T>
+hipError_t launch_sor_pair3(hipStream_t s, const Grid& g, const Phys& ph, const FieldView<T>* xin,
+                            const FieldView<T>* xout, const FieldView<T>* rhs, FieldView<T> mu, const Tiling& tl);
 // outer-loop boundary conditions (main_ns_hybrid.F90:764-776, 826-836,
 // 864-879; the same statements close do_jacobi_v / do_jacobi_w)
 template <typename T>
