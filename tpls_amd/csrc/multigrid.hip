@@ -122,16 +122,11 @@ struct MgPre {
     static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
 
+// The pre-smoothing pair of the cells (i0, j, k) and (i0+1, j, k), i0 odd.
 template <typename X, typename B>
-__device__ __forceinline__ void MgPre<X, B>::run(const Args& a, const Blk& bk)
+__device__ __forceinline__ void mg_pre_cells(const MgLevelT<X, B>& o, const FieldView<X>& bc, int i0, int j, int k)
 {
     using C = X;
-    const MgLevelT<X, B>& o = a.o;
-    const FieldView<X>& bc = a.bc;
-    if (a.st->done) return;
-    const int i0 = 2 * (bk.x * BX + int(threadIdx.x)) + 1;
-    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
-    if (i0 > o.nx || j > o.ny) return;
     const int ir = ((i0 + j + k) & 1) == 0 ? i0 : i0 + 1;
     const int ib = ir == i0 ? i0 + 1 : i0;
     auto red = [&](int ii, int jj, int kk) { return sdiv(C(o.b(ii, jj, kk)) + C(0), mg_deff<C>(o, ii, kk)); };
@@ -144,9 +139,29 @@ __device__ __forceinline__ void MgPre<X, B>::run(const Args& a, const Blk& bk)
 }
 
 template <typename X, typename B>
+__device__ __forceinline__ void MgPre<X, B>::run(const Args& a, const Blk& bk)
+{
+    const MgLevelT<X, B>& o = a.o;
+    if (a.st->done) return;
+    const int i0 = 2 * (bk.x * BX + int(threadIdx.x)) + 1;
+    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
+    if (i0 > o.nx || j > o.ny) return;
+    mg_pre_cells(o, a.bc, i0, j, k);
+}
+
+template <typename X, typename B>
 __global__ void __launch_bounds__(BX* BY) k_mg_pre(MgLevelT<X, B> o, FieldView<X> bc, const CgState* st)
 {
     MgPre<X, B>::run({o, bc, st}, blk_here());
+}
+
+// The cell of colour `colour` among (i0, j, k) and (i0+1, j, k), i0 odd.
+template <typename X, typename B>
+__device__ __forceinline__ void mg_smooth_cell(const MgLevelT<X, B>& o, int colour, int i0, int j, int k)
+{
+    using C = X;
+    const int i = ((i0 + j + k) & 1) == colour ? i0 : i0 + 1;
+    o.x(i, j, k) = sdiv(C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
 }
 
 // One colour sweep.  A thread owns the pair (i0, i0+1) along x and updates
@@ -163,14 +178,12 @@ struct MgSmooth {
     static dim3 block() { return dim3(BX, BY); }
     static __device__ __forceinline__ void run(const Args& a, const Blk& bk)
     {
-        using C = X;
         const MgLevelT<X, B>& o = a.o;
         if (a.st->done) return;
         const int i0 = 2 * (bk.x * BX + int(threadIdx.x)) + 1;
         const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
         if (i0 > o.nx || j > o.ny) return;
-        const int i = ((i0 + j + k) & 1) == a.colour ? i0 : i0 + 1;
-        o.x(i, j, k) = sdiv(C(o.b(i, j, k)) + mg_nsum<C>(o, o.x, i, j, k), mg_deff<C>(o, i, k));
+        mg_smooth_cell(o, a.colour, i0, j, k);
     }
 };
 
@@ -576,16 +589,12 @@ struct MgProlong {
     static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
 
+// x += P e for the fine cells (2I-1, j, k) and (2I, j, k).
 template <typename XF, typename BF, typename XC>
-__device__ __forceinline__ void MgProlong<XF, BF, XC>::run(const Args& a, const Blk& bk)
+__device__ __forceinline__ void mg_prolong_cells(const MgLevelT<XF, BF>& f, const MgLevelT<XC, XC>& c, int I, int j,
+                                                 int k)
 {
     using C = XF;
-    const MgLevelT<XF, BF>& f = a.f;
-    const MgLevelT<XC, XC>& c = a.c;
-    if (a.st->done) return;
-    const int I = bk.x * BX + int(threadIdx.x) + 1;
-    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
-    if (I > c.nx || j > f.ny) return;
     const int J = (j + 1) / 2, K = (k + 1) / 2;
     const int Jn = mg_nbr(j, c.ny, true), Kn = mg_nbr(k, c.nz, false);
     const C q3 = C(0.75), q1 = C(0.25);
@@ -605,9 +614,74 @@ __device__ __forceinline__ void MgProlong<XF, BF, XC>::run(const Args& a, const 
 }
 
 template <typename XF, typename BF, typename XC>
+__device__ __forceinline__ void MgProlong<XF, BF, XC>::run(const Args& a, const Blk& bk)
+{
+    const MgLevelT<XF, BF>& f = a.f;
+    const MgLevelT<XC, XC>& c = a.c;
+    if (a.st->done) return;
+    const int I = bk.x * BX + int(threadIdx.x) + 1;
+    const int j = bk.y * BY + int(threadIdx.y) + 1, k = bk.z + 1;
+    if (I > c.nx || j > f.ny) return;
+    mg_prolong_cells(f, c, I, j, k);
+}
+
+template <typename XF, typename BF, typename XC>
 __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLevelT<XC, XC> c, const CgState* st)
 {
     MgProlong<XF, BF, XC>::run({f, c, st}, blk_here());
+}
+
+// A level of at most MG1_CELLS cells visited in one block: on the way down
+// the first pre-smoothing pair and the nu - 1 pairs (red, black) after it,
+// on the way up the prolongation and the nu pairs (black, red).  Each sweep
+// covers the level's column pairs block-stride with mg_pre_cells /
+// mg_smooth_cell / mg_prolong_cells, a barrier between sweeps: every cell
+// computes what k_mg_pre, k_mg_smooth and k_mg_prolong compute (within a
+// colour the updates are order-free), in one launch instead of 2 nu - 1 on
+// the way down and 2 nu + 1 on the way up.
+constexpr int MG1_T = 1024;
+constexpr long MG1_CELLS = 8192;
+template <typename L>
+static bool mg_one_block(const L& o) { return long(o.nx) * o.ny * o.nz <= MG1_CELLS; }
+
+template <typename X, typename B, typename F>
+__device__ __forceinline__ void mg1_pairs(const MgLevelT<X, B>& o, F&& cell)
+{
+    const int hx = o.nx / 2, n = hx * o.ny * o.nz;
+    for (int q = int(threadIdx.x); q < n; q += MG1_T) {
+        const int r = q / hx;
+        cell(2 * (q % hx) + 1, r % o.ny + 1, r / o.ny + 1);
+    }
+    __syncthreads();
+}
+
+template <typename X, typename B>
+__global__ void __launch_bounds__(MG1_T) k_mg_down1(MgLevelT<X, B> o, int nu, const CgState* st)
+{
+    if (st->done) return;
+    const FieldView<X> nobc{nullptr, 0, 0};
+    mg1_pairs(o, [&](int i0, int j, int k) { mg_pre_cells(o, nobc, i0, j, k); });
+    for (int q = 1; q < nu; ++q) {
+        mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 0, i0, j, k); });
+        mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 1, i0, j, k); });
+    }
+}
+
+template <typename X, typename B, typename XC>
+__global__ void __launch_bounds__(MG1_T) k_mg_up1(MgLevelT<X, B> o, MgLevelT<XC, XC> c, int nu, const CgState* st)
+{
+    if (st->done) return;
+    // the prolongation: one thread per coarse column I and fine (j, k)
+    const int n = c.nx * o.ny * o.nz;
+    for (int q = int(threadIdx.x); q < n; q += MG1_T) {
+        const int r = q / c.nx;
+        mg_prolong_cells(o, c, q % c.nx + 1, r % o.ny + 1, r / o.ny + 1);
+    }
+    __syncthreads();
+    for (int q = 0; q < nu; ++q) {
+        mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 1, i0, j, k); });
+        mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 0, i0, j, k); });
+    }
 }
 
 // Coarsest level: one block, a single wavefront for up to 64 cells (its
@@ -939,6 +1013,11 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     };
     auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
         using B = std::decay_t<decltype(*o.b.p)>;
+        if (mg_one_block(o)) {
+            hipLaunchKernelGGL((k_mg_down1<X, B>), dim3(1), dim3(MG1_T), 0, s, o, nu, st);
+            rr(o, oc);
+            return;
+        }
         hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
         pairs_rr(o, oc, nobc);
     };
@@ -947,6 +1026,10 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     // level's values; with xb the pairs alternate between xa and xb as above.
     auto up = [&](auto o, const MgLevelT<X, X>& oc, bool finest, FieldView<X> xa, FieldView<X> xb) {
         using B = std::decay_t<decltype(*o.b.p)>;
+        if (!finest && !xb.p && mg_one_block(o)) {
+            hipLaunchKernelGGL((k_mg_up1<X, B, X>), dim3(1), dim3(MG1_T), 0, s, o, oc, nu, st);
+            return o.x;
+        }
         hipLaunchKernelGGL((k_mg_prolong<X, B, X>), dim3((oc.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz), blk, 0,
                            s, o, oc, st);
         for (int q = 0; q < nu; ++q) {
