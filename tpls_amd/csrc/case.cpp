@@ -1282,8 +1282,15 @@ int enqueue_pre(tpls_case* c, const StepPlan& pl)
         const int nits = std::max({mom[0], mom[1], mom[2]}) / 2;
         for (int it = 1; it <= nits; ++it) {
             const bool three = it % 2 && it <= nmin;
-            for (int q = 0; q < 3; ++q)
-                if (it <= mom[q] / 2) RC(iteration(q, it, three));
+            // an odd iteration's Jacobi steps go out right after the same
+            // component's even iteration, whose output they read while the
+            // cache still holds it (the first odd iteration's on their own)
+            const bool next3 = it % 2 == 0 && it + 1 <= nmin;
+            for (int q = 0; q < 3; ++q) {
+                if (it > mom[q] / 2 || (three && it > 1)) continue;
+                RC(iteration(q, it, three));
+                if (next3) RC(iteration(q, it + 1, true));
+            }
             if (!three) continue;
             FieldView<T> xin[3], xout[3], rhs[3];
             for (int q = 0; q < 3; ++q) {
