@@ -4,7 +4,8 @@ colour sweeps per pair (mg_pair = -1) bit for bit: a first-colour cell reads
 only old second-colour values and a second-colour cell only new first-colour
 ones, in both forms.  FP64 and FP32 V-cycles, V(1,1) to V(3,3), and grids
 whose march chunks are 1, 2 and 4 levels long (the pass forms the first
-colour one level beyond each end of its chunk), three steps each."""
+colour one level beyond each end of its chunk), three steps each; and level
+1's pairs, which run the same way where that level has 2 M unknowns."""
 from dataclasses import replace
 
 import numpy as np
@@ -41,3 +42,26 @@ def test_pair_pass_equals_the_two_sweeps(grid, precision, mg_fp32):
         assert its == its_ref
         for f in FIELDS:
             assert np.array_equal(got[f], ref[f]), (grid, precision, mg_fp32, nu, f)
+
+
+@pytest.mark.parametrize("precision,mg_fp32", [("fp64", False), ("fp32", None)])
+def test_level_one_pairs_at_full_size(precision, mg_fp32):
+    """At 512x128x256 level 1 (256x64x128, 2 M unknowns) has a second vector
+    as well, and its smoothing pairs run as one pass each too: one step,
+    V(1,1) and V(2,2), bit for bit the in-place sweeps of every level."""
+    from tpls_amd.case import Case
+    from tpls_amd.initial import make_initial
+    for nu in (1, 2):
+        p = replace(BASE, nx=512, ny=128, nz=256, ksp_mg_nu=nu)
+        init = make_initial(p)
+        print(f"[mg-pair] 512x128x256 {precision} V({nu},{nu})", flush=True)
+        out = []
+        for pair in (-1, 1):
+            with Case(p, precision=precision, mg_fp32=mg_fp32, mg_pair=pair) as c:
+                c.load(*init)
+                c.step(1)
+                out.append(({f: c.field(f) for f in FIELDS}, c.diag()["ksp_its"]))
+        (ref, its_ref), (got, its) = out
+        assert its == its_ref
+        for f in FIELDS:
+            assert np.array_equal(got[f], ref[f]), (precision, nu, f)

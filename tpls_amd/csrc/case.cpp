@@ -622,6 +622,15 @@ int tpls_case_create(const tpls_params* prm, int maxl, int maxm, int maxn, tpls_
             if ((rc = alloc_field(c, &a2, sp, c->mg.fp32 ? 4 : 8))) return bail(rc);
             if (c->mg.fp32) c->mg.x2f = a2->view<float>();
             else c->mg.x2 = a2->view<double>();
+            // and level 1's, where it has a coarser level and fits the pass
+            const MgLevel& l1 = c->mg.L[1];
+            if (c->mg.nlev > 2 && mg_pair_fits(l1.nx, l1.ny, l1.nz)) {
+                DevArray* a21;
+                const Shape s1{0, l1.nx + 1, 0, l1.ny + 1, 0, l1.nz + 1};
+                if ((rc = alloc_field(c, &a21, s1, c->mg.fp32 ? 4 : 8))) return bail(rc);
+                if (c->mg.fp32) c->mg.x21f = a21->view<float>();
+                else c->mg.x21 = a21->view<double>();
+            }
         }
     }
     const long nb = std::max<long>(cg_num_blocks(c->g), (long)REDUCE_BLOCKS);

@@ -978,7 +978,8 @@ int mg_plan(int nx, int ny, int nz, double cx, double cy, double cz, double diag
 template <typename X>
 static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>& top, const MgLevelT<X, X>* lev,
                          int nlev, int nu, const double* spec, Reduce red, CgState* st,
-                         const MgLevelT<X, X>* topx = nullptr, FieldView<X> x2 = FieldView<X>{nullptr, 0, 0})
+                         const MgLevelT<X, X>* topx = nullptr, FieldView<X> x2 = FieldView<X>{nullptr, 0, 0},
+                         FieldView<X> x21 = FieldView<X>{nullptr, 0, 0})
 {
     const dim3 blk(BX, BY);
     const int last = nlev - 1;
@@ -1063,7 +1064,21 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, nobc, st);
         cur = pairs_rr(top0, lev[1], pairs ? top.x : nobc);
     }
-    for (int l = 1; l < last; ++l) down(lev[l], lev[l + 1]);
+    // Level 1 with its second vector (x21) runs its pairs as level 0 does:
+    // the first pass writes x21, and the 2 nu - 1 pair passes after it end
+    // in the level's own vector, which level 0's prolongation reads.
+    const bool pairs1 = last > 1 && x21.p;
+    FieldView<X> cur1 = lev[1].x;
+    for (int l = 1; l < last; ++l) {
+        if (l == 1 && pairs1) {
+            MgLevelT<X, X> o = lev[1];
+            o.x = x21;
+            hipLaunchKernelGGL((k_mg_pre<X, X>), pair_grid(o), blk, 0, s, o, nobc, st);
+            cur1 = pairs_rr(o, lev[2], lev[1].x);
+        } else {
+            down(lev[l], lev[l + 1]);
+        }
+    }
     if (last > 0) {
         const MgLevelT<X, X>& c = lev[last];
         const long nc = long(c.nx) * c.ny * c.nz;
@@ -1084,7 +1099,15 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
             return hipErrorInvalidValue;
         }
     }
-    for (int l = last - 1; l >= 1; --l) up(lev[l], lev[l + 1], false, nobc, nobc);
+    for (int l = last - 1; l >= 1; --l) {
+        if (l == 1 && pairs1) {
+            MgLevelT<X, X> o = lev[1];
+            o.x = cur1;
+            if (up(o, lev[2], false, x21, lev[1].x).p != lev[1].x.p) return hipErrorInvalidValue;
+        } else {
+            up(lev[l], lev[l + 1], false, nobc, nobc);
+        }
+    }
     if (last > 0 && topx) {
         MgLevelT<X, X> o = *topx;
         o.x = cur;
@@ -1106,7 +1129,7 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
 
 hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, Reduce red, CgState* st)
 {
-    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, h.spec, red, st, nullptr, h.x2);
+    if (!h.fp32) return vcycle<double>(s, g, h.L[0], h.L, h.nlev, h.nu, h.spec, red, st, nullptr, h.x2, h.x21);
     // the FP32 vectors with L's extents and coefficients
     MgLevel32 lev[MG_MAXLEV];
     for (int l = 0; l < h.nlev; ++l) {
@@ -1117,7 +1140,8 @@ hipError_t launch_mg_vcycle(hipStream_t s, const Grid& g, const MgHierarchy& h, 
     const MgLevelT<float, double> top{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, o.b};
     // F[0].b: the FP64 residual in FP32, written by the first pass
     const MgLevel32 topx{o.nx, o.ny, o.nz, o.cx, o.cy, o.cz, o.diag, h.F[0].x, h.F[0].b};
-    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr, h.x2f);
+    return vcycle<float>(s, g, top, lev, h.nlev, h.nu, h.spec, red, st, h.F[0].b.p ? &topx : nullptr, h.x2f,
+                         h.x21f);
 }
 
 // ---- the same V-cycle for many cases in lock-step, one launch per kernel

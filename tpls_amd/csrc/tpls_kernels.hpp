@@ -245,6 +245,9 @@ struct MgHierarchy {
     // one out-of-place pass (where mg_pair_fits)
     FieldView<double> x2{nullptr, 0, 0};
     FieldView<float> x2f{nullptr, 0, 0};
+    // the same for level 1 (x21, x21f: level 1's shape), or null
+    FieldView<double> x21{nullptr, 0, 0};
+    FieldView<float> x21f{nullptr, 0, 0};
 };
 // Levels whose smoothing pairs run as one pass: whole 8-row tiles in y and
 // enough cells (2 M) to fill the device with 64 x 8 tiles.
