@@ -631,14 +631,13 @@ __global__ void __launch_bounds__(BX* BY) k_mg_prolong(MgLevelT<XF, BF> f, MgLev
     MgProlong<XF, BF, XC>::run({f, c, st}, blk_here());
 }
 
-// A level of at most MG1_CELLS cells visited in one block: on the way down
-// the first pre-smoothing pair and the nu - 1 pairs (red, black) after it,
-// on the way up the prolongation and the nu pairs (black, red).  Each sweep
-// covers the level's column pairs block-stride with mg_pre_cells /
-// mg_smooth_cell / mg_prolong_cells, a barrier between sweeps: every cell
-// computes what k_mg_pre, k_mg_smooth and k_mg_prolong compute (within a
-// colour the updates are order-free), in one launch instead of 2 nu - 1 on
-// the way down and 2 nu + 1 on the way up.
+// Levels of at most MG1_CELLS cells are visited in one block (k_mg_bottom):
+// on the way down the first pre-smoothing pair and the nu - 1 pairs (red,
+// black) after it, on the way up the prolongation and the nu pairs (black,
+// red).  Each sweep covers the level's column pairs block-stride with
+// mg_pre_cells / mg_smooth_cell / mg_prolong_cells, a barrier between
+// sweeps: every cell computes what k_mg_pre, k_mg_smooth and k_mg_prolong
+// compute (within a colour the updates are order-free).
 constexpr int MG1_T = 1024;
 constexpr long MG1_CELLS = 8192;
 template <typename L>
@@ -655,10 +654,10 @@ __device__ __forceinline__ void mg1_pairs(const MgLevelT<X, B>& o, F&& cell)
     __syncthreads();
 }
 
+// The visits of one such level, in the block.
 template <typename X, typename B>
-__global__ void __launch_bounds__(MG1_T) k_mg_down1(MgLevelT<X, B> o, int nu, const CgState* st)
+__device__ __forceinline__ void mg1_down(const MgLevelT<X, B>& o, int nu)
 {
-    if (st->done) return;
     const FieldView<X> nobc{nullptr, 0, 0};
     mg1_pairs(o, [&](int i0, int j, int k) { mg_pre_cells(o, nobc, i0, j, k); });
     for (int q = 1; q < nu; ++q) {
@@ -668,9 +667,8 @@ __global__ void __launch_bounds__(MG1_T) k_mg_down1(MgLevelT<X, B> o, int nu, co
 }
 
 template <typename X, typename B, typename XC>
-__global__ void __launch_bounds__(MG1_T) k_mg_up1(MgLevelT<X, B> o, MgLevelT<XC, XC> c, int nu, const CgState* st)
+__device__ __forceinline__ void mg1_up(const MgLevelT<X, B>& o, const MgLevelT<XC, XC>& c, int nu)
 {
-    if (st->done) return;
     // the prolongation: one thread per coarse column I and fine (j, k)
     const int n = c.nx * o.ny * o.nz;
     for (int q = int(threadIdx.x); q < n; q += MG1_T) {
@@ -682,6 +680,60 @@ __global__ void __launch_bounds__(MG1_T) k_mg_up1(MgLevelT<X, B> o, MgLevelT<XC,
         mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 1, i0, j, k); });
         mg1_pairs(o, [&](int i0, int j, int k) { mg_smooth_cell(o, 0, i0, j, k); });
     }
+}
+
+// The residual-restriction f -> c in the block: every fine cell's residual
+// into LDS (r), then each coarse cell's b_c = P^T r / 8 with k_mg_rr's sums
+// (along x, then y, then z, each over the candidates q = 0..3 with nonzero
+// weight, in order), so the coarse values are k_mg_rr's bit for bit.
+template <typename X>
+__device__ __forceinline__ void mg1_rr(const MgLevelT<X, X>& f, const MgLevelT<X, X>& c, X* r)
+{
+    using C = X;
+    const int nf = f.nx * f.ny * f.nz;
+    for (int q = int(threadIdx.x); q < nf; q += MG1_T) {
+        const int i = q % f.nx + 1, rr = q / f.nx, j = rr % f.ny + 1, k = rr / f.ny + 1;
+        const int jm = j == 1 ? f.ny : j - 1, jp = j == f.ny ? 1 : j + 1;
+        const C xc = f.x(i, j, k);
+        const C zm = k > 1 ? C(f.x(i, j, k - 1)) : C(0);
+        const C zp = k < f.nz ? C(f.x(i, j, k + 1)) : C(0);
+        const C xm = i > 1 ? C(f.x(i - 1, j, k)) : C(0);
+        const C xp = i < f.nx ? C(f.x(i + 1, j, k)) : C(0);
+        const C ym = f.x(i, jm, k), yp = f.x(i, jp, k);
+        const C nsum = C(f.cz) * (zm + zp) + C(f.cx) * (xm + xp) + C(f.cy) * (ym + yp);
+        r[q] = C(f.b(i, j, k)) - (mg_deff<C>(f, i, k) * xc - nsum);
+    }
+    __syncthreads();
+    const int nc = c.nx * c.ny * c.nz;
+    for (int q = int(threadIdx.x); q < nc; q += MG1_T) {
+        const int I = q % c.nx + 1, rr = q / c.nx, J = rr % c.ny + 1, K = rr / c.ny + 1;
+        C wi[4], wj[4], wk[4];
+        mg_rweights_q(I, c.nx, f.nx, false, wi);
+        mg_rweights_q(J, c.ny, f.ny, true, wj);
+        mg_rweights_q(K, c.nz, f.nz, false, wk);
+        C sk = 0;
+#pragma unroll
+        for (int qz = 0; qz < 4; ++qz) {
+            if (wk[qz] == C(0)) continue;
+            const int fk = 2 * K - 2 + qz;
+            C sj = 0;
+#pragma unroll
+            for (int qy = 0; qy < 4; ++qy) {
+                if (wj[qy] == C(0)) continue;
+                int fj = 2 * J - 2 + qy;
+                if (fj < 1) fj += f.ny;
+                if (fj > f.ny) fj -= f.ny;
+                C si = 0;
+#pragma unroll
+                for (int qx = 0; qx < 4; ++qx)
+                    if (wi[qx] != C(0)) si += wi[qx] * r[(2 * I - 2 + qx - 1) + f.nx * ((fj - 1) + f.ny * (fk - 1))];
+                sj += wj[qy] * si;
+            }
+            sk += wk[qz] * sj;
+        }
+        c.b(I, J, K) = C(0.125) * sk;
+    }
+    __syncthreads();
 }
 
 // Coarsest level: one block, a single wavefront for up to 64 cells (its
@@ -846,6 +898,34 @@ template <typename X, typename B>
 __global__ void __launch_bounds__(MGC_T) k_mg_spec(MgLevelT<X, B> o, const double* tab, const CgState* st)
 {
     MgSpec<X, B>::run({o, tab, st}, blk_here());
+}
+
+// The bottom of the V-cycle in one block: levels L[0] .. L[n-2] (each of at
+// most MG1_CELLS cells) down (mg1_down, mg1_rr) and up (mg1_up) as the
+// multi-block kernels would run them, and the coarsest level L[n-1] as
+// k_mg_spec (tab set) or k_mg_coarse: one launch instead of about
+// (2 nu + 1 + 2 nu + 1) (n - 1) + 1.
+constexpr int MG1_MAXLEV = 6;
+template <typename X>
+struct MgBottom {
+    MgLevelT<X, X> L[MG1_MAXLEV];
+    int n, nu;
+};
+
+template <typename X>
+__global__ void __launch_bounds__(MG1_T) k_mg_bottom(MgBottom<X> a, const double* tab, const CgState* st)
+{
+    if (st->done) return;
+    __shared__ X r[MG1_CELLS];
+    const int n = a.n;
+    for (int l = 0; l + 1 < n; ++l) {
+        mg1_down(a.L[l], a.nu);
+        mg1_rr(a.L[l], a.L[l + 1], r);
+    }
+    if (tab) MgSpec<X, X>::run({a.L[n - 1], tab, st}, Blk{0, 0, 0, 1});
+    else MgCoarse<X>::run({a.L[n - 1], st}, Blk{0, 0, 0, 1});
+    __syncthreads();
+    for (int l = n - 2; l >= 0; --l) mg1_up(a.L[l], a.L[l + 1], a.nu);
 }
 
 // Final red sweep on the finest level with the partials of z.r (r = b) and
@@ -1014,11 +1094,6 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     };
     auto down = [&](const auto& o, const MgLevelT<X, X>& oc) {
         using B = std::decay_t<decltype(*o.b.p)>;
-        if (mg_one_block(o)) {
-            hipLaunchKernelGGL((k_mg_down1<X, B>), dim3(1), dim3(MG1_T), 0, s, o, nu, st);
-            rr(o, oc);
-            return;
-        }
         hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o), blk, 0, s, o, nobc, st);
         pairs_rr(o, oc, nobc);
     };
@@ -1027,10 +1102,6 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     // level's values; with xb the pairs alternate between xa and xb as above.
     auto up = [&](auto o, const MgLevelT<X, X>& oc, bool finest, FieldView<X> xa, FieldView<X> xb) {
         using B = std::decay_t<decltype(*o.b.p)>;
-        if (!finest && !xb.p && mg_one_block(o)) {
-            hipLaunchKernelGGL((k_mg_up1<X, B, X>), dim3(1), dim3(MG1_T), 0, s, o, oc, nu, st);
-            return o.x;
-        }
         hipLaunchKernelGGL((k_mg_prolong<X, B, X>), dim3((oc.nx + BX - 1) / BX, (o.ny + BY - 1) / BY, o.nz), blk, 0,
                            s, o, oc, st);
         for (int q = 0; q < nu; ++q) {
@@ -1069,7 +1140,13 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     // in the level's own vector, which level 0's prolongation reads.
     const bool pairs1 = last > 1 && x21.p;
     FieldView<X> cur1 = lev[1].x;
-    for (int l = 1; l < last; ++l) {
+    // levels lb .. last, each of at most MG1_CELLS cells (at least one of
+    // them smoothed), run as one single-block launch (k_mg_bottom)
+    int lb = last;
+    while (lb > 1 && mg_one_block(lev[lb - 1])) --lb;
+    const bool bottom = last > 1 && lb < last && last - lb + 1 <= MG1_MAXLEV;
+    const int lmid = bottom ? lb : last;
+    for (int l = 1; l < lmid; ++l) {
         if (l == 1 && pairs1) {
             MgLevelT<X, X> o = lev[1];
             o.x = x21;
@@ -1079,7 +1156,13 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
             down(lev[l], lev[l + 1]);
         }
     }
-    if (last > 0) {
+    if (bottom) {
+        MgBottom<X> mb{};
+        for (int l = lb; l <= last; ++l) mb.L[l - lb] = lev[l];
+        mb.n = last - lb + 1;
+        mb.nu = nu;
+        hipLaunchKernelGGL((k_mg_bottom<X>), dim3(1), dim3(MG1_T), 0, s, mb, spec, st);
+    } else if (last > 0) {
         const MgLevelT<X, X>& c = lev[last];
         const long nc = long(c.nx) * c.ny * c.nz;
         if (spec)
@@ -1099,7 +1182,7 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
             return hipErrorInvalidValue;
         }
     }
-    for (int l = last - 1; l >= 1; --l) {
+    for (int l = lmid - 1; l >= 1; --l) {
         if (l == 1 && pairs1) {
             MgLevelT<X, X> o = lev[1];
             o.x = cur1;
