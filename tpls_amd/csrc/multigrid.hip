@@ -208,9 +208,12 @@ static dim3 pair_grid(const L& o) { return dim3((o.nx / 2 + BX - 1) / BX, (o.ny 
 // colour-f values of level m-1 in LDS (in-plane) and its own colour-f values
 // of levels m-2 and m (z).  Absent neighbours are 0 as in mg_nsum.  The
 // first colour is formed one level below and above the chunk, so chunks
-// overlap only in reads.
+// overlap only in reads.  ZERO: the old x is zero (nothing is loaded): the
+// pass is then the first pre-smoothing pair, k_mg_pre's values bit for bit
+// (b + 0 where k_mg_pre adds 0, and the same first-colour values around
+// each second-colour cell).
 constexpr int MP_TX = 64, MP_TY = 8;
-template <typename X, typename B>
+template <typename X, typename B, bool ZERO = false>
 struct MgPair {
     static constexpr int PW = (MP_TX + 2) / 2, NP = PW * (MP_TY + 2), NT = (NP + 63) / 64 * 64;
     // levels of old x and b loaded ahead: 8 measured 1.5 % faster per FP32
@@ -228,8 +231,8 @@ struct MgPair {
     static __device__ __forceinline__ void run(const Args& a, const Blk& bk);
 };
 
-template <typename X, typename B>
-__device__ __forceinline__ void MgPair<X, B>::run(const Args& a, const Blk& bk)
+template <typename X, typename B, bool ZERO>
+__device__ __forceinline__ void MgPair<X, B, ZERO>::run(const Args& a, const Blk& bk)
 {
     using C = X;
     const MgLevelT<X, B>& o = a.o;
@@ -263,7 +266,7 @@ __device__ __forceinline__ void MgPair<X, B>::run(const Args& a, const Blk& bk)
     auto stage_load = [&](C* v, int st) {
         const long kx = long(min(max(st, 1), nz)) * o.x.sz;
 #pragma unroll
-        for (int n = 0; n < NS; ++n) v[n] = o.x.p[ox[n] + kx];
+        for (int n = 0; n < NS; ++n) v[n] = ZERO ? C(0) : o.x.p[ox[n] + kx];
     };
     auto stage_store = [&](const C* v, int st) {
         C* px = &xs[st & 3][0][0];
@@ -342,28 +345,35 @@ __device__ __forceinline__ void MgPair<X, B>::run(const Args& a, const Blk& bk)
     }
 }
 
-template <typename X, typename B>
-__global__ void __launch_bounds__((MgPair<X, B>::NT)) k_mg_pair(typename MgPair<X, B>::Args a)
+template <typename X, typename B, bool ZERO = false>
+__global__ void __launch_bounds__((MgPair<X, B, ZERO>::NT)) k_mg_pair(typename MgPair<X, B, ZERO>::Args a)
 {
-    MgPair<X, B>::run(a, blk_here());
+    MgPair<X, B, ZERO>::run(a, blk_here());
 }
 
 // The pair sweeps of a level as one k_mg_pair launch into xout when there is
 // one (the case allocates it where mg_pair_fits, or as tpls_params.mg_pair
 // asks) and the level has whole tiles in y, else two k_mg_smooth launches in
 // place.  Returns the vector holding the result.
-template <typename X, typename B>
+template <typename X, typename B, bool ZERO = false>
 static FieldView<X> mg_pair(hipStream_t s, const MgLevelT<X, B>& o, FieldView<X> xout, int f, const CgState* st)
 {
     if (xout.p && o.ny % MP_TY == 0) {
-        using Op = MgPair<X, B>;
-        static const int slots = resident_blocks(k_mg_pair<X, B>, Op::NT);
+        using Op = MgPair<X, B, ZERO>;
+        static const int slots = resident_blocks(k_mg_pair<X, B, ZERO>, Op::NT);
         const int nbx = (o.nx + MP_TX - 1) / MP_TX, nby = o.ny / MP_TY;
         const int ks = march_chunk(nbx * nby, o.nz, slots, 32);
         const int nbz = (o.nz + ks - 1) / ks;
-        hipLaunchKernelGGL((k_mg_pair<X, B>), dim3(nbx * nby * nbz), dim3(Op::NT), 0, s,
+        hipLaunchKernelGGL((k_mg_pair<X, B, ZERO>), dim3(nbx * nby * nbz), dim3(Op::NT), 0, s,
                            typename Op::Args{o, xout, f, nbx, nby, ks, st});
         return xout;
+    }
+    if (ZERO) {
+        // the first pre-smoothing pair into xout as k_mg_pre forms it
+        MgLevelT<X, B> o2 = o;
+        o2.x = xout.p ? xout : o.x;
+        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o2), dim3(BX, BY), 0, s, o2, FieldView<X>{nullptr, 0, 0}, st);
+        return o2.x;
     }
     const dim3 blk(BX, BY);
     hipLaunchKernelGGL((k_mg_smooth<X, B>), pair_grid(o), blk, 0, s, o, f, st);
@@ -1132,7 +1142,10 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         o.x = cur;
         cur = pairs_rr(o, lev[1], pairs ? top.x : nobc);
     } else if (last > 0) {
-        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, nobc, st);
+        // with the second vector the first pre-smoothing pair is a pair pass
+        // from x = 0 into it (k_mg_pre's values; nothing of x is read)
+        if (pairs) mg_pair<X, double, true>(s, top0, x2, 0, st);
+        else hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, nobc, st);
         cur = pairs_rr(top0, lev[1], pairs ? top.x : nobc);
     }
     // Level 1 with its second vector (x21) runs its pairs as level 0 does:
@@ -1150,7 +1163,7 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
         if (l == 1 && pairs1) {
             MgLevelT<X, X> o = lev[1];
             o.x = x21;
-            hipLaunchKernelGGL((k_mg_pre<X, X>), pair_grid(o), blk, 0, s, o, nobc, st);
+            mg_pair<X, X, true>(s, o, x21, 0, st);
             cur1 = pairs_rr(o, lev[2], lev[1].x);
         } else {
             down(lev[l], lev[l + 1]);
