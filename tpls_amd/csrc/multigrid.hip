@@ -225,6 +225,9 @@ struct MgPair {
         int f;
         int nbx, nby, ks;
         const CgState* st;
+        // ZERO only: when set, receives b converted to X at every cell the
+        // pass writes (as k_mg_pre's bc)
+        FieldView<X> bc;
     };
     static constexpr int THREADS = NT;
     static dim3 block() { return dim3(NT); }
@@ -335,6 +338,10 @@ __device__ __forceinline__ void MgPair<X, B, ZERO>::run(const Args& a, const Blk
                 const int co = ia + (off ^ 1);
                 if (cf >= i0 && cf < i0 + MP_TX && cf <= nx) a.xout(cf, j, k) = sv;
                 if (co >= i0 && co < i0 + MP_TX && co <= nx) a.xout(co, j, k) = fm1;
+                if (ZERO && a.bc.p) {
+                    if (cf >= i0 && cf < i0 + MP_TX && cf <= nx) a.bc(cf, j, k) = bprev[off];
+                    if (co >= i0 && co < i0 + MP_TX && co <= nx) a.bc(co, j, k) = bprev[off ^ 1];
+                }
             }
             fm2 = fm1;
             fm1 = fv;
@@ -356,7 +363,8 @@ __global__ void __launch_bounds__((MgPair<X, B, ZERO>::NT)) k_mg_pair(typename M
 // asks) and the level has whole tiles in y, else two k_mg_smooth launches in
 // place.  Returns the vector holding the result.
 template <typename X, typename B, bool ZERO = false>
-static FieldView<X> mg_pair(hipStream_t s, const MgLevelT<X, B>& o, FieldView<X> xout, int f, const CgState* st)
+static FieldView<X> mg_pair(hipStream_t s, const MgLevelT<X, B>& o, FieldView<X> xout, int f, const CgState* st,
+                            FieldView<X> bc = FieldView<X>{nullptr, 0, 0})
 {
     if (xout.p && o.ny % MP_TY == 0) {
         using Op = MgPair<X, B, ZERO>;
@@ -365,14 +373,14 @@ static FieldView<X> mg_pair(hipStream_t s, const MgLevelT<X, B>& o, FieldView<X>
         const int ks = march_chunk(nbx * nby, o.nz, slots, 32);
         const int nbz = (o.nz + ks - 1) / ks;
         hipLaunchKernelGGL((k_mg_pair<X, B, ZERO>), dim3(nbx * nby * nbz), dim3(Op::NT), 0, s,
-                           typename Op::Args{o, xout, f, nbx, nby, ks, st});
+                           typename Op::Args{o, xout, f, nbx, nby, ks, st, bc});
         return xout;
     }
     if (ZERO) {
         // the first pre-smoothing pair into xout as k_mg_pre forms it
         MgLevelT<X, B> o2 = o;
         o2.x = xout.p ? xout : o.x;
-        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o2), dim3(BX, BY), 0, s, o2, FieldView<X>{nullptr, 0, 0}, st);
+        hipLaunchKernelGGL((k_mg_pre<X, B>), pair_grid(o2), dim3(BX, BY), 0, s, o2, bc, st);
         return o2.x;
     }
     const dim3 blk(BX, BY);
@@ -1136,8 +1144,10 @@ static hipError_t vcycle(hipStream_t s, const Grid& g, const MgLevelT<X, double>
     MgLevelT<X, double> top0 = top;
     top0.x = cur;
     if (last > 0 && topx) {
-        // the finest level with b converted once by its first pass
-        hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, topx->b, st);
+        // the finest level with b converted once by its first pass (a pair
+        // pass from x = 0 where there is a second vector)
+        if (pairs) mg_pair<X, double, true>(s, top0, x2, 0, st, topx->b);
+        else hipLaunchKernelGGL((k_mg_pre<X, double>), pair_grid(top0), blk, 0, s, top0, topx->b, st);
         MgLevelT<X, X> o = *topx;
         o.x = cur;
         cur = pairs_rr(o, lev[1], pairs ? top.x : nobc);
